@@ -1,0 +1,279 @@
+"""godunov_five_eq_amd -- MI355X-native five-equation Godunov block step.
+
+Python view of the C ABI declared in include/godunov_five_eq.h (the product is libg5.so: HIP
+kernels for gfx950 plus the host runtime).  This module only marshals arguments; every
+computation runs in the library.  Loading fails loudly if libg5.so has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libg5.so")
+
+# return codes and enums of include/godunov_five_eq.h
+G5_OK, G5_ERR_INVALID, G5_ERR_HIP, G5_ERR_UNSUPPORTED, G5_ERR_NOMEM, G5_ERR_STATE = range(6)
+BC_NONE, BC_PERIODIC, BC_ZERO_GRADIENT, BC_WALL = range(4)
+TI_HANCOCK, TI_RK1, TI_RK2_SSP = range(3)
+PREC_FP64, PREC_MIXED = range(2)
+
+BC_NAMES = {"NONE": BC_NONE, "PERIODIC": BC_PERIODIC, "ZERO_GRADIENT": BC_ZERO_GRADIENT, "WALL": BC_WALL}
+TI_NAMES = {"HANCOCK": TI_HANCOCK, "RK1": TI_RK1, "RK2_SSP": TI_RK2_SSP}
+
+
+class G5Config(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("bsize", ctypes.c_int32),
+        ("nb", ctypes.c_int32 * 3),
+        ("periodic", ctypes.c_int32 * 3),
+        ("bc", ctypes.c_int32 * 6),
+        ("integrator", ctypes.c_int32),
+        ("thinc_enabled", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("dx", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("slope_type", ctypes.c_double),
+        ("smallr", ctypes.c_double),
+        ("smallc", ctypes.c_double),
+        ("gamma", ctypes.c_double * 2),
+        ("pinf", ctypes.c_double * 2),
+        ("thinc_beta", ctypes.c_double),
+        ("thinc_epsilon", ctypes.c_double),
+        ("xmin", ctypes.c_double * 3),
+    ]
+
+
+class G5Region(ctypes.Structure):
+    _fields_ = [
+        ("alpha0", ctypes.c_double),
+        ("alpha_rho", ctypes.c_double * 2),
+        ("p", ctypes.c_double),
+        ("u", ctypes.c_double),
+        ("v", ctypes.c_double),
+        ("w", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libg5.so (raises if it is missing: there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        d, i32, sz = ctypes.c_double, ctypes.c_int32, ctypes.c_size_t
+        vp = ctypes.c_void_p
+        sig = {
+            "g5_version": (ctypes.c_char_p, []),
+            "g5_last_error": (ctypes.c_char_p, [vp]),
+            "g5_create": (ctypes.c_int, [P(G5Config), P(vp)]),
+            "g5_destroy": (ctypes.c_int, [vp]),
+            "g5_shape": (ctypes.c_int, [vp, P(i32), P(i32), P(i32)]),
+            "g5_mem_bytes": (ctypes.c_int, [vp, P(ctypes.c_uint64)]),
+            "g5_upload_state": (ctypes.c_int, [vp, P(d), sz]),
+            "g5_download_state": (ctypes.c_int, [vp, P(d), sz]),
+            "g5_state_device_ptr": (ctypes.c_int, [vp, P(vp)]),
+            "g5_compute_dt": (ctypes.c_int, [vp, P(d)]),
+            "g5_fill_outside": (ctypes.c_int, [vp, P(d), sz]),
+            "g5_outside_coords": (ctypes.c_int, [vp, P(i32), P(i32), sz]),
+            "g5_do_time_step": (ctypes.c_int, [vp, d]),
+            "g5_run": (ctypes.c_int, [vp, i32, P(d), d, P(i32)]),
+            "g5_synchronize": (ctypes.c_int, [vp]),
+            "g5_last_run_ms": (ctypes.c_int, [vp, P(d)]),
+            "g5_time_step_kernel": (ctypes.c_int, [vp, i32, d, P(d)]),
+            "g5_init_shock_bubble": (ctypes.c_int, [P(G5Config), P(G5Region), d, P(d), d, P(d), sz]),
+            "g5_rankine_hugoniot": (ctypes.c_int, [d, d, d, d, d, d, P(d), P(d), P(d), P(d)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names of the C ABI entry points (for the export check)."""
+    return [
+        "g5_version", "g5_last_error", "g5_create", "g5_destroy", "g5_shape", "g5_mem_bytes",
+        "g5_upload_state", "g5_download_state", "g5_state_device_ptr", "g5_compute_dt",
+        "g5_fill_outside", "g5_outside_coords", "g5_do_time_step", "g5_run", "g5_synchronize", "g5_last_run_ms",
+        "g5_time_step_kernel", "g5_init_shock_bubble", "g5_rankine_hugoniot",
+    ]
+
+
+class G5Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"g5 error {code}: {msg}")
+        self.code = code
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def nvar_of(dim):
+    return 8 if dim == 2 else 9
+
+
+def make_config(dim, nb, dx, *, bsize=16, periodic=(0, 0, 0), bc=("ZERO_GRADIENT",) * 6,
+                integrator="HANCOCK", thinc=False, precision=PREC_FP64, cfl=0.5, slope_type=1.0,
+                smallr=1e-10, smallc=1e-10, gamma=(1.4, 1.4), pinf=(0.0, 0.0), thinc_beta=1.5,
+                thinc_epsilon=1e-3, xmin=(0.0, 0.0, 0.0)):
+    """Build a G5Config; defaults follow models/HydroSettings.cpp:18-29."""
+    c = G5Config()
+    c.dim = dim
+    c.bsize = bsize
+    nb = list(nb) + [1] * (3 - len(nb))
+    for k in range(3):
+        c.nb[k] = nb[k]
+        c.periodic[k] = int(periodic[k]) if k < len(periodic) else 0
+        c.xmin[k] = xmin[k] if k < len(xmin) else 0.0
+    for f in range(6):
+        b = bc[f]
+        c.bc[f] = BC_NAMES[b] if isinstance(b, str) else int(b)
+    c.integrator = TI_NAMES[integrator] if isinstance(integrator, str) else int(integrator)
+    c.thinc_enabled = 1 if thinc else 0
+    c.precision = precision
+    c.dx = dx
+    c.cfl = cfl
+    c.slope_type = slope_type
+    c.smallr = smallr
+    c.smallc = smallc
+    c.gamma[0], c.gamma[1] = gamma
+    c.pinf[0], c.pinf[1] = pinf
+    c.thinc_beta = thinc_beta
+    c.thinc_epsilon = thinc_epsilon
+    return c
+
+
+def state_size(cfg):
+    nblocks = cfg.nb[0] * cfg.nb[1] * (cfg.nb[2] if cfg.dim == 3 else 1)
+    return nblocks * nvar_of(cfg.dim) * cfg.bsize ** cfg.dim
+
+
+def region(alpha0, alpha_rho, p, u=0.0, v=0.0, w=0.0):
+    r = G5Region()
+    r.alpha0 = alpha0
+    r.alpha_rho[0], r.alpha_rho[1] = alpha_rho
+    r.p, r.u, r.v, r.w = p, u, v, w
+    return r
+
+
+def init_shock_bubble(cfg, regions, x_front, center, radius):
+    """Host initial condition of InitShockBubbleDataFunctor -> state array [block][var][cell]."""
+    n = state_size(cfg)
+    U = np.empty(n, dtype=np.float64)
+    regs = (G5Region * 3)(*regions)
+    c3 = (ctypes.c_double * 3)(*(list(center) + [0.0] * (3 - len(center))))
+    rc = lib().g5_init_shock_bubble(ctypes.byref(cfg), regs, x_front, c3, radius, _dptr(U), n)
+    if rc != G5_OK:
+        raise G5Error(rc, "g5_init_shock_bubble")
+    return U
+
+
+def rankine_hugoniot(gamma, pinf, rho1, p1, mach=0.0, p2=0.0):
+    """(rho2, u2, p2, shock_speed) behind a shock moving into gas at rest."""
+    out = [ctypes.c_double() for _ in range(4)]
+    rc = lib().g5_rankine_hugoniot(gamma, pinf, rho1, p1, mach, p2, *[ctypes.byref(o) for o in out])
+    if rc != G5_OK:
+        raise G5Error(rc, "g5_rankine_hugoniot")
+    return tuple(o.value for o in out)
+
+
+class Context:
+    """One solve on the GPU (the GodunovImplemBase surface, scheme/GodunovImplemBase.h:59-198)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self._h = ctypes.c_void_p()
+        rc = lib().g5_create(ctypes.byref(cfg), ctypes.byref(self._h))
+        if rc != G5_OK:
+            raise G5Error(rc, lib().g5_last_error(None).decode())
+        nb, nv, nc = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        lib().g5_shape(self._h, ctypes.byref(nb), ctypes.byref(nv), ctypes.byref(nc))
+        self.nblocks, self.nvar, self.ncell = nb.value, nv.value, nc.value
+        self.nstate = self.nblocks * self.nvar * self.ncell
+
+    def _check(self, rc):
+        if rc != G5_OK:
+            raise G5Error(rc, lib().g5_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().g5_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).ravel()
+        self._check(lib().g5_upload_state(self._h, _dptr(U), U.size))
+
+    def download(self):
+        U = np.empty(self.nstate, dtype=np.float64)
+        self._check(lib().g5_download_state(self._h, _dptr(U), U.size))
+        return U
+
+    def compute_dt(self):
+        dt = ctypes.c_double()
+        self._check(lib().g5_compute_dt(self._h, ctypes.byref(dt)))
+        return dt.value
+
+    def do_time_step(self, dt):
+        self._check(lib().g5_do_time_step(self._h, dt))
+
+    def run(self, nsteps, t=0.0, t_end=1e300):
+        tt = ctypes.c_double(t)
+        done = ctypes.c_int32()
+        self._check(lib().g5_run(self._h, nsteps, ctypes.byref(tt), t_end, ctypes.byref(done)))
+        return tt.value, done.value
+
+    def last_run_ms(self):
+        ms = ctypes.c_double()
+        self._check(lib().g5_last_run_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        self._check(lib().g5_synchronize(self._h))
+
+    def mem_bytes(self):
+        b = ctypes.c_uint64()
+        self._check(lib().g5_mem_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def outside_coords(self):
+        n = ctypes.c_int32()
+        self._check(lib().g5_outside_coords(self._h, ctypes.byref(n), None, 0))
+        coords = np.empty(3 * n.value, dtype=np.int32)
+        self._check(lib().g5_outside_coords(self._h, ctypes.byref(n),
+                                            coords.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                            coords.size))
+        return coords.reshape(-1, 3)
+
+    def fill_outside(self):
+        n_out = self.outside_coords().shape[0]
+        ring = np.empty(n_out * self.nvar * self.ncell, dtype=np.float64)
+        self._check(lib().g5_fill_outside(self._h, _dptr(ring), ring.size))
+        return ring
+
+    def time_step_kernel(self, nsteps, dt):
+        ms = ctypes.c_double()
+        self._check(lib().g5_time_step_kernel(self._h, nsteps, dt, ctypes.byref(ms)))
+        return ms.value

@@ -1,0 +1,801 @@
+// g5_device.h -- device-side physics of the five-equation step (FP64 path).
+//
+// Each function restates one reference function; the FP64 operation order is the reference's
+// (the file is compiled with -ffp-contract=off, and FP64 division and sqrt are IEEE correctly
+// rounded on gfx950), so results are reproducible to the bit on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "g5_types.h"
+
+#define G5D __device__ __forceinline__
+
+namespace g5
+{
+
+// ---------------------------------------------------------------------------------------------
+// state containers (named fields instead of indexed arrays so everything stays in registers)
+// ---------------------------------------------------------------------------------------------
+struct Cons // conservative state, models/FiveEq.h:35-50 order
+{
+  double d, e, mu, mv, mw, a0, ar0, a1, ar1;
+};
+
+struct Prim // the primitive values the flux path reads (get_prim_variables .h:196-241)
+{
+  double r0, r1, a, p, u, v, w;
+};
+
+struct Slope // limited slopes of the reconstructed variables
+{
+  double r0, r1, p, u, v, w;
+};
+
+struct Face // reconstructed face state (IA0 in a)
+{
+  double r0, r1, a, p, u, v, w;
+};
+
+struct Flux // face flux scaled by dt/dx (set_flux, ComputeFluxesAndStoreFunctor.h:253-307)
+{
+  double ar0, ar1, a0, a1, e, mu, mv, mw, us;
+};
+
+// ---------------------------------------------------------------------------------------------
+// mixture stiffened-gas EOS (Allaire et al. 2002 isobaric closure):
+//   xi = sum alpha_k/(gamma_k-1), Pi = sum alpha_k gamma_k pinf_k/(gamma_k-1)
+//   p = (rho e - Pi)/xi ; e = (p xi + Pi)/rho ; rho c^2 = ((xi+1) p + Pi)/xi
+// ---------------------------------------------------------------------------------------------
+G5D double
+eos_xi(SolveDesc const & S, double a0, double a1)
+{
+  return a0 * S.g1[0] + a1 * S.g1[1];
+}
+G5D double
+eos_pi(SolveDesc const & S, double a0, double a1)
+{
+  return a0 * S.gp[0] + a1 * S.gp[1];
+}
+G5D double
+eos_pressure(SolveDesc const & S, double rho, double eint, double a0, double a1)
+{
+  return (rho * eint - eos_pi(S, a0, a1)) / eos_xi(S, a0, a1);
+}
+G5D double
+eos_eint(SolveDesc const & S, double rho, double p, double a0, double a1)
+{
+  return (p * eos_xi(S, a0, a1) + eos_pi(S, a0, a1)) / rho;
+}
+G5D double
+eos_sound_speed(SolveDesc const & S, double rho, double p, double a0, double a1)
+{
+  const double x = eos_xi(S, a0, a1);
+  const double rc2 = ((x + 1.0) * p + eos_pi(S, a0, a1)) / x;
+  return sqrt(rc2 / rho);
+}
+
+// ---------------------------------------------------------------------------------------------
+// models/utils.h:200-250 computePrimitives (+ hydro_sanitizer :36-73, sanitize_alpha :82-87).
+// Returns the mixture sound speed; prim.a is the sanitised IA0.
+// ---------------------------------------------------------------------------------------------
+template <int DIM>
+G5D double
+cons_to_prim(SolveDesc const & S, Cons const & U, Prim & q)
+{
+  const double d = fmax(U.d, S.smallr);
+  const double ux = U.mu / d;
+  const double uy = U.mv / d;
+  double       uz = 0.0;
+  double       ekin;
+  if (DIM == 2)
+  {
+    ekin = 0.5 * (ux * ux + uy * uy);
+  }
+  else
+  {
+    uz = U.mw / d;
+    ekin = 0.5 * (ux * ux + uy * uy + uz * uz);
+  }
+  const double eint = U.e / d - ekin;
+  const double p = eos_pressure(S, d, eint, U.a0, 1.0 - U.a0);
+  double       a = fmax(0.0, fmin(1.0, U.a0));
+  double       r0 = U.ar0, r1 = U.ar1;
+  const double eps = 1e-12;
+  if (r0 < eps)
+    r0 = 0.0;
+  if (r1 < eps)
+    r1 = 0.0;
+  if (a < eps)
+    a = 0.0;
+  if (a > 1.0 - eps)
+    a = 1.0;
+  q.r0 = r0;
+  q.r1 = r1;
+  q.a = a;
+  q.p = p;
+  q.u = ux;
+  q.v = uy;
+  q.w = uz;
+  return eos_sound_speed(S, d, p, a, 1.0 - a);
+}
+
+// scheme/ComputeLimitedSlopesFunctor.cpp:114-137 slope_unsplit_scalar
+G5D double
+slope1(double q, double qp, double qm, double slope_type)
+{
+  double dq = 0.0;
+  if (slope_type == 1.0 or slope_type == 2.0)
+  {
+    const double dlft = slope_type * (q - qm);
+    const double drgt = slope_type * (qp - q);
+    const double dcen = 0.5 * (qp - qm);
+    const double dsgn = (dcen >= 0.0) ? 1.0 : -1.0;
+    const double slop = fmin(fabs(dlft), fabs(drgt));
+    const double dlim = (dlft * drgt) <= 0.0 ? 0.0 : slop;
+    dq = dsgn * fmin(dlim, fabs(dcen));
+  }
+  return dq;
+}
+
+G5D Slope
+slopes_of(Prim const & c, Prim const & p, Prim const & m, double st)
+{
+  Slope s;
+  s.r0 = slope1(c.r0, p.r0, m.r0, st);
+  s.r1 = slope1(c.r1, p.r1, m.r1, st);
+  s.p = slope1(c.p, p.p, m.p, st);
+  s.u = slope1(c.u, p.u, m.u, st);
+  s.v = slope1(c.v, p.v, m.v, st);
+  s.w = slope1(c.w, p.w, m.w, st);
+  return s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// MUSCL-Hancock half step: the cell-centred part of reconstruct_state_{2d,3d}
+// (scheme/ComputeFluxesAndStoreFunctor.cpp:134-202, :230-316).  The sound speed and the Hancock
+// source terms depend on the cell only, so they are computed once per cell and reused by every
+// face of the cell (the reference recomputes the same values per face).
+// ---------------------------------------------------------------------------------------------
+struct Hancock // 0.5 * (sr0_0, sr0_1, sp0, su0, sv0, sw0)
+{
+  double r0, r1, p, u, v, w;
+};
+
+template <int DIM>
+G5D Hancock
+hancock_terms(SolveDesc const & S, Prim const & q, Slope const & sx, Slope const & sy, Slope const & sz,
+              double dtdx)
+{
+  Hancock      h;
+  const double r = q.r0 + q.r1;
+  const double c = eos_sound_speed(S, r, q.p, q.a, 1.0 - q.a);
+  const double dtdy = dtdx, dtdz = dtdx;
+  const double uu = q.u, vv = q.v, ww = q.w;
+  if (DIM == 2)
+  {
+    const double sr0 =
+      (-uu * (sx.r0 + sx.r1) - sx.u * r) * dtdx + (-vv * (sy.r0 + sy.r1) - sy.v * r) * dtdy;
+    const double sr0_0 = sr0 * q.a;
+    const double sr0_1 = sr0 * (1.0 - q.a);
+    const double su0 = (-uu * sx.u - sx.p / r) * dtdx + (-vv * sy.u) * dtdy;
+    const double sv0 = (-uu * sx.v) * dtdx + (-vv * sy.v - sy.p / r) * dtdy;
+    const double sp0 = (-uu * sx.p - sx.u * r * c * c) * dtdx + (-vv * sy.p - sy.v * r * c * c) * dtdy;
+    h.r0 = 0.5 * sr0_0;
+    h.r1 = 0.5 * sr0_1;
+    h.p = 0.5 * sp0;
+    h.u = 0.5 * su0;
+    h.v = 0.5 * sv0;
+    h.w = 0.0;
+  }
+  else
+  {
+    const double sr0 = (-uu * (sx.r0 + sx.r1) - sx.u * r) * dtdx +
+                       (-vv * (sy.r0 + sy.r1) - sy.v * r) * dtdy +
+                       (-ww * (sz.r0 + sz.r1) - sz.w * r) * dtdz;
+    const double sr0_0 = sr0 * q.a;
+    const double sr0_1 = sr0 * (1.0 - q.a);
+    const double su0 = (-uu * sx.u - sx.p / r) * dtdx + (-vv * sy.u) * dtdy + (-ww * sz.u) * dtdz;
+    const double sv0 = (-uu * sx.v) * dtdx + (-vv * sy.v - sy.p / r) * dtdy + (-ww * sz.v) * dtdz;
+    const double sw0 = (-uu * sx.w) * dtdx + (-vv * sy.w) * dtdy + (-ww * sz.w - sz.p / r) * dtdz;
+    const double sp0 = (-uu * sx.p - sx.u * r * c * c) * dtdx + (-vv * sy.p - sy.v * r * c * c) * dtdy +
+                       (-ww * sz.p - sz.w * r * c * c) * dtdz;
+    h.r0 = 0.5 * sr0_0;
+    h.r1 = 0.5 * sr0_1;
+    h.p = 0.5 * sp0;
+    h.u = 0.5 * su0;
+    h.v = 0.5 * sv0;
+    h.w = 0.5 * sw0;
+  }
+  return h;
+}
+
+// q + off_x*sx + off_y*sy (+ off_z*sz), one offset +-0.5 on axis AX, the others 0.0 (kept so that
+// the operation sequence is the reference's).
+template <int DIM, int AX>
+G5D double
+muscl_extrap(double qv, double sxv, double syv, double szv, double o)
+{
+  const double ox = AX == 0 ? o : 0.0;
+  const double oy = AX == 1 ? o : 0.0;
+  const double oz = AX == 2 ? o : 0.0;
+  if (DIM == 2)
+    return qv + ox * sxv + oy * syv;
+  return qv + ox * sxv + oy * syv + oz * szv;
+}
+
+// reconstruct_state_{2d,3d} face part (scheme/ComputeFluxesAndStoreFunctor.cpp:164-211)
+template <int DIM, int AX, bool HANCOCK>
+G5D Face
+face_muscl(SolveDesc const & S, Prim const & q, Slope const & sx, Slope const & sy, Slope const & sz,
+           Hancock const & h, double o)
+{
+  Face f;
+  f.r0 = muscl_extrap<DIM, AX>(q.r0, sx.r0, sy.r0, sz.r0, o);
+  f.r1 = muscl_extrap<DIM, AX>(q.r1, sx.r1, sy.r1, sz.r1, o);
+  f.p = muscl_extrap<DIM, AX>(q.p, sx.p, sy.p, sz.p, o);
+  f.u = muscl_extrap<DIM, AX>(q.u, sx.u, sy.u, sz.u, o);
+  f.v = muscl_extrap<DIM, AX>(q.v, sx.v, sy.v, sz.v, o);
+  f.w = DIM == 3 ? muscl_extrap<DIM, AX>(q.w, sx.w, sy.w, sz.w, o) : 0.0;
+  if (HANCOCK)
+  {
+    f.r0 += h.r0;
+    f.r1 += h.r1;
+    f.p += h.p;
+    f.u += h.u;
+    f.v += h.v;
+    if (DIM == 3)
+      f.w += h.w;
+  }
+  f.r0 = fmax(S.smallr, f.r0);
+  f.r1 = fmax(S.smallr, f.r1);
+  f.a = q.a;
+  return f;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Deterministic tanh (same algorithm as the host oracle): tanh(a) = -m/(2+m), m = expm1(-2a),
+// expm1 by Cody-Waite reduction and the degree-13 Taylor polynomial.
+// ---------------------------------------------------------------------------------------------
+G5D double
+det_expm1_neg(double y)
+{
+  const double ln2_hi = 6.93147180369123816490e-01;
+  const double ln2_lo = 1.90821492927058770002e-10;
+  const double inv_ln2 = 1.44269504088896338700e+00;
+  double       k = rint(y * inv_ln2);
+  if (k < -60.0)
+    k = -60.0;
+  const double r = (y - k * ln2_hi) - k * ln2_lo;
+  double       p = 1.6059043836821613e-10;
+  p = p * r + 2.08767569878681e-09;
+  p = p * r + 2.505210838544172e-08;
+  p = p * r + 2.755731922398589e-07;
+  p = p * r + 2.7557319223985893e-06;
+  p = p * r + 2.48015873015873e-05;
+  p = p * r + 1.98412698412698413e-04;
+  p = p * r + 1.38888888888888894e-03;
+  p = p * r + 8.33333333333333322e-03;
+  p = p * r + 4.16666666666666644e-02;
+  p = p * r + 1.66666666666666657e-01;
+  p = p * r + 5.00000000000000000e-01;
+  p = p * r + 1.0;
+  p = p * r;
+  if (k == 0.0)
+    return p;
+  const double two_k = ldexp(1.0, static_cast<int>(k));
+  return two_k * p + (two_k - 1.0);
+}
+
+G5D double
+det_tanh(double x)
+{
+  const double a = fabs(x);
+  if (a > 22.0)
+    return x > 0 ? 1.0 : -1.0;
+  const double m = det_expm1_neg(-2.0 * a);
+  const double t = -m / (2.0 + m);
+  return x < 0 ? -t : t;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Youngs (1982) normal from the 3^d alpha stencil (same weights and order as the host oracle).
+// ---------------------------------------------------------------------------------------------
+G5D void
+youngs2(double const (&a)[9], double & nx, double & ny)
+{
+  // a[(di+1) + 3*(dj+1)]
+  const double gx = ((a[2] + 2.0 * a[5]) + a[8]) - ((a[0] + 2.0 * a[3]) + a[6]);
+  const double gy = ((a[6] + 2.0 * a[7]) + a[8]) - ((a[0] + 2.0 * a[1]) + a[2]);
+  const double nrm = sqrt(gx * gx + gy * gy);
+  if (nrm > 0.0)
+  {
+    nx = gx / nrm;
+    ny = gy / nrm;
+  }
+  else
+  {
+    nx = 0.0;
+    ny = 0.0;
+  }
+}
+
+// 3-D: a[(di+1) + 3*(dj+1) + 9*(dk+1)]; plane sums in the order t2 outer, t1 inner with
+// o[d]=s, o[(d+1)%3]=t1, o[(d+2)%3]=t2 and weights (1,2,1)x(1,2,1)
+template <int D>
+G5D double
+youngs_plane(double const (&a)[27], int s)
+{
+  double acc = 0.0;
+#pragma unroll
+  for (int t2 = -1; t2 <= 1; ++t2)
+#pragma unroll
+    for (int t1 = -1; t1 <= 1; ++t1)
+    {
+      const double w = (t1 == 0 ? 2.0 : 1.0) * (t2 == 0 ? 2.0 : 1.0);
+      int          o[3];
+      o[D] = s;
+      o[(D + 1) % 3] = t1;
+      o[(D + 2) % 3] = t2;
+      acc = acc + w * a[(o[0] + 1) + 3 * (o[1] + 1) + 9 * (o[2] + 1)];
+    }
+  return acc;
+}
+
+G5D void
+youngs3(double const (&a)[27], double & nx, double & ny, double & nz)
+{
+  const double gx = youngs_plane<0>(a, 1) - youngs_plane<0>(a, -1);
+  const double gy = youngs_plane<1>(a, 1) - youngs_plane<1>(a, -1);
+  const double gz = youngs_plane<2>(a, 1) - youngs_plane<2>(a, -1);
+  const double nrm = sqrt(gx * gx + gy * gy + gz * gz);
+  if (nrm > 0.0)
+  {
+    nx = gx / nrm;
+    ny = gy / nrm;
+    nz = gz / nrm;
+  }
+  else
+  {
+    nx = ny = nz = 0.0;
+  }
+}
+
+// THINC alpha and partial densities of one face side (ComputeFluxesAndStoreTHINCFunctor.cpp
+// :140-222 / :310-395).  am, ac, ap: alpha at -1, 0, +1 along the face axis; nd: |normal[axis]|
+// (only used when THINC is requested); ib: +1 for the right face of the cell, -1 for the left;
+// o: the MUSCL offset (0.5*ib) on the face axis (MUSCL fallback).
+struct ThincAxis // per-cell, per-axis THINC quantities shared by the two faces
+{
+  bool   req;
+  double A, B, C, D;
+};
+
+G5D ThincAxis
+thinc_axis(SolveDesc const & S, double qa, double am, double ac, double ap, double nd)
+{
+  ThincAxis    t;
+  const double monotonicity = (ap - ac) * (ac - am);
+  t.req = (qa > S.thinc_eps) and (qa < 1.0 - S.thinc_eps) and monotonicity > 0;
+  t.A = t.B = t.C = t.D = 0.0;
+  if (t.req)
+  {
+    const double beta = S.thinc_beta * nd + 0.01;
+    t.A = (ap + am) / 2;
+    t.B = (ap - am) / 2;
+    t.C = det_tanh(0.5 * beta);
+    const double phi = ac;
+    t.D = det_tanh(0.5 * beta * (phi - t.A) / t.B);
+  }
+  return t;
+}
+
+// reconstruct_state_{2d,3d} of the THINC functor: alpha-related part and MUSCL part for p, u, v, w
+template <int DIM, int AX, bool HANCOCK>
+G5D Face
+face_thinc(Prim const & q, Slope const & sx, Slope const & sy, Slope const & sz, Hancock const & h,
+           ThincAxis const & th, int ib)
+{
+  Face         f;
+  const double o = 0.5 * static_cast<double>(ib); // HALF_F * offsets[AX]
+  if (th.req)
+  {
+    const double bb = static_cast<double>(ib);
+    f.a = th.A + bb * th.B * (th.C + bb * th.D / th.C) / (1.0 + bb * th.D);
+    f.r0 = q.r0 + q.r0 / q.a * (f.a - q.a);
+    f.r1 = q.r1 + q.r1 / (1.0 - q.a) * (q.a - f.a);
+  }
+  else
+  {
+    f.a = q.a;
+    f.r0 = muscl_extrap<DIM, AX>(q.r0, sx.r0, sy.r0, sz.r0, o);
+    f.r1 = muscl_extrap<DIM, AX>(q.r1, sx.r1, sy.r1, sz.r1, o);
+  }
+  f.p = muscl_extrap<DIM, AX>(q.p, sx.p, sy.p, sz.p, o);
+  f.u = muscl_extrap<DIM, AX>(q.u, sx.u, sy.u, sz.u, o);
+  f.v = muscl_extrap<DIM, AX>(q.v, sx.v, sy.v, sz.v, o);
+  f.w = DIM == 3 ? muscl_extrap<DIM, AX>(q.w, sx.w, sy.w, sz.w, o) : 0.0;
+  if (HANCOCK)
+  {
+    f.r0 += h.r0;
+    f.r1 += h.r1;
+    f.p += h.p;
+    f.u += h.u;
+    f.v += h.v;
+    if (DIM == 3)
+      f.w += h.w;
+  }
+  return f;
+}
+
+// ---------------------------------------------------------------------------------------------
+// models/RiemannSolvers.h:57-262 riemann_hllc, for a face normal to axis AX, followed by the flux
+// scaling and the u<->v (u<->w) rotation of compute_fluxes_and_store_{2d,3d}
+// (ComputeFluxesAndStoreFunctor.cpp:374-451, :502-640).  un/t1/t2 are the normal and the two
+// transverse velocities in the reference's swapped order.
+// ---------------------------------------------------------------------------------------------
+template <int DIM>
+G5D Flux
+hllc(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
+{
+  // swapped velocity slots (IU, IV, IW) after my_swap
+  const double ulN = AX == 0 ? L.u : (AX == 1 ? L.v : L.w);
+  const double ulT1 = AX == 1 ? L.u : L.v;
+  const double ulT2 = AX == 2 ? L.u : L.w;
+  const double urN = AX == 0 ? R.u : (AX == 1 ? R.v : R.w);
+  const double urT1 = AX == 1 ? R.u : R.v;
+  const double urT2 = AX == 2 ? R.u : R.w;
+
+  const double smallr = S.smallr, smallp = S.smallp, smallc = S.smallc;
+
+  const double rl = fmax(L.r0 + L.r1, smallr);
+  double       pl = fmax(L.p, rl * smallp);
+  const double ul = ulN;
+  const double eintl = eos_eint(S, rl, pl, L.a, 1.0 - L.a);
+  pl = eos_pressure(S, rl, eintl, L.a, 1.0 - L.a);
+  double ecinl = 0.5 * rl * ul * ul;
+  ecinl += 0.5 * rl * ulT1 * ulT1;
+  if (DIM == 3)
+    ecinl += 0.5 * rl * ulT2 * ulT2;
+  const double cl = eos_sound_speed(S, rl, pl, L.a, 1.0 - L.a);
+  const double etotl = rl * eintl + ecinl;
+  const double ptotl = pl;
+
+  const double rr = fmax(R.r0 + R.r1, smallr);
+  double       pr = fmax(R.p, rr * smallp);
+  const double ur = urN;
+  const double eintr = eos_eint(S, rr, pr, R.a, 1.0 - R.a);
+  pr = eos_pressure(S, rr, eintr, R.a, 1.0 - R.a);
+  double ecinr = 0.5 * rr * ur * ur;
+  ecinr += 0.5 * rr * urT1 * urT1;
+  if (DIM == 3)
+    ecinr += 0.5 * rr * urT2 * urT2;
+  const double cr = eos_sound_speed(S, rr, pr, R.a, 1.0 - R.a);
+  const double etotr = rr * eintr + ecinr;
+  const double ptotr = pr;
+
+  const double cfastl = sqrt(fmax(cl * cl, smallc * smallc));
+  const double cfastr = sqrt(fmax(cr * cr, smallc * smallc));
+  const double SL = fmin(ul, ur) - fmax(cfastl, cfastr);
+  const double SR = fmax(ul, ur) + fmax(cfastl, cfastr);
+
+  const double rcl = rl * (ul - SL);
+  const double rcr = rr * (SR - ur);
+
+  const double ustar0 = (rcr * ur + rcl * ul + (ptotl - ptotr)) / (rcr + rcl);
+  const double ptotstar = (rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur)) / (rcr + rcl);
+
+  const double factorL = (SL - ul) / (SL - ustar0);
+  const double etotstarl = ((SL - ul) * etotl - ptotl * ul + ptotstar * ustar0) / (SL - ustar0);
+  const double factorR = (SR - ur) / (SR - ustar0);
+  const double etotstarr = ((SR - ur) * etotr - ptotr * ur + ptotstar * ustar0) / (SR - ustar0);
+
+  double ro, ro_0, ro_1, uo, ptoto, etoto;
+  if (SL > 0.0)
+  {
+    ro_0 = L.r0;
+    ro_1 = L.r1;
+    ro = rl;
+    uo = ul;
+    ptoto = ptotl;
+    etoto = etotl;
+  }
+  else if (ustar0 > 0.0)
+  {
+    ro_0 = L.r0 * factorL;
+    ro_1 = L.r1 * factorL;
+    ro = rl * factorL;
+    uo = ustar0;
+    ptoto = ptotstar;
+    etoto = etotstarl;
+  }
+  else if (SR > 0.0)
+  {
+    ro_0 = R.r0 * factorR;
+    ro_1 = R.r1 * factorR;
+    ro = rr * factorR;
+    uo = ustar0;
+    ptoto = ptotstar;
+    etoto = etotstarr;
+  }
+  else
+  {
+    ro_0 = R.r0;
+    ro_1 = R.r1;
+    ro = rr;
+    uo = ur;
+    ptoto = ptotr;
+    etoto = etotr;
+  }
+
+  const double fr0 = ro_0 * uo;
+  const double fr1 = ro_1 * uo;
+  const double fd = (ro_0 + ro_1) * uo;
+  const double fn = ro * uo * uo + ptoto;
+  const double fe = (etoto + ptoto) * uo;
+  const bool   up = fd > 0.0;
+  const double ft1 = fd * (up ? ulT1 : urT1);
+  const double ft2 = DIM == 3 ? fd * (up ? ulT2 : urT2) : 0.0;
+  const double phi0 = up ? L.a : R.a;
+  const double phi1 = up ? 1.0 - L.a : 1.0 - R.a;
+  const double ustar = uo;
+
+  Flux F;
+  F.ar0 = fr0 * dtdx;
+  F.ar1 = fr1 * dtdx;
+  F.a0 = (ustar * phi0) * dtdx;
+  F.a1 = (ustar * phi1) * dtdx;
+  F.e = fe * dtdx;
+  // un-swap: x: (IU,IV,IW) = (n,t1,t2); y: (t1,n,t2); z: (t2,t1,n)
+  const double fu = AX == 0 ? fn : (AX == 1 ? ft1 : ft2);
+  const double fv = AX == 1 ? fn : ft1;
+  const double fw = AX == 2 ? fn : ft2;
+  F.mu = fu * dtdx;
+  F.mv = fv * dtdx;
+  F.mw = DIM == 3 ? fw * dtdx : 0.0;
+  F.us = ustar * dtdx;
+  return F;
+}
+
+// ---------------------------------------------------------------------------------------------
+// scheme/ReadFluxesAndConservativeUpdateFunctor.cpp:146-212 (same-level branch) + update_U
+// (.h:319-393): accumulate left minus right flux with the Allaire-Kokh alpha source term (time-n
+// alpha of the cell), then add to the running U2 with the 1e-12 clamps.
+// ---------------------------------------------------------------------------------------------
+template <int DIM>
+G5D void
+apply_flux_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double ua1)
+{
+  const double eps = 1e-12;
+  double       acc_r0 = 0.0 + FL.ar0;
+  double       acc_r1 = 0.0 + FL.ar1;
+  double       acc_a0 = 0.0 + FL.a0;
+  double       acc_a1 = 0.0 + FL.a1;
+  double       acc_e = 0.0 + FL.e;
+  double       acc_mu = 0.0 + FL.mu;
+  double       acc_mv = 0.0 + FL.mv;
+  double       acc_mw = 0.0 + FL.mw;
+  acc_a0 -= 1.0 * FL.us * ua0;
+  acc_a1 -= 1.0 * FL.us * ua1;
+  acc_r0 = acc_r0 - FR.ar0;
+  acc_r1 = acc_r1 - FR.ar1;
+  acc_a0 = acc_a0 - FR.a0;
+  acc_a1 = acc_a1 - FR.a1;
+  acc_e = acc_e - FR.e;
+  acc_mu = acc_mu - FR.mu;
+  acc_mv = acc_mv - FR.mv;
+  acc_mw = acc_mw - FR.mw;
+  acc_a0 += 1.0 * FR.us * ua0;
+  acc_a1 += 1.0 * FR.us * ua1;
+
+  u2.ar0 += acc_r0;
+  if (u2.ar0 < eps)
+    u2.ar0 = 0.0;
+  u2.ar1 += acc_r1;
+  if (u2.ar1 < eps)
+    u2.ar1 = 0.0;
+  u2.a0 += acc_a0;
+  if (u2.a0 < eps)
+    u2.a0 = 0.0;
+  if (u2.a0 > 1.0 - eps)
+    u2.a0 = 1.0;
+  u2.a1 += acc_a1;
+  if (u2.a1 < eps)
+    u2.a1 = 0.0;
+  if (u2.a1 > 1.0 - eps)
+    u2.a1 = 1.0;
+  u2.e += acc_e;
+  u2.mu += acc_mu;
+  u2.mv += acc_mv;
+  if (DIM == 3)
+    u2.mw += acc_mw;
+}
+
+// scheme/FixVolumeFractionsNormalization.cpp:58-70
+G5D void
+fix_volume_fractions(Cons & u)
+{
+  const double sum_alpha = u.a0 + u.a1;
+  const double sum_alpha_rho = u.ar0 + u.ar1;
+  const double inv = 1.0 / sum_alpha;
+  u.a0 *= inv;
+  u.a1 *= inv;
+  u.d = sum_alpha_rho;
+}
+
+// scheme/ComputeDtFiveEqFunctor.cpp:85-134 compute_cfl (one cell's contribution)
+template <int DIM>
+G5D double
+cell_inv_dt(SolveDesc const & S, Cons const & u)
+{
+  Prim         q;
+  const double c = cons_to_prim<DIM>(S, u, q);
+  const double vx = c + fabs(q.u);
+  const double vy = c + fabs(q.v);
+  if (DIM == 2)
+    return vx / S.dx + vy / S.dx;
+  const double vz = c + fabs(q.w);
+  return vx / S.dx + vy / S.dx + vz / S.dx;
+}
+
+// ---------------------------------------------------------------------------------------------
+// ghost/border resolution.  Block N (brick coordinates, already wrapped in periodic directions,
+// possibly one step outside the brick) and in-block cell c map to (source block, in-block cell,
+// negated momentum axis): same-level neighbour lookup of StencilHelper plus the outside-block fill
+// of border_conditions/FillOutside.cpp:69-170 -- the inside block is the outside block moved back
+// along its outside normal, the faces are visited in order xmin..zmax and the last one that
+// applies defines the copy (ZERO_GRADIENT: edge cell; WALL: mirrored cell, normal momentum
+// negated).
+// ---------------------------------------------------------------------------------------------
+template <int DIM, int B>
+G5D int64_t
+resolve_block_cell(SolveDesc const & S, int const (&N)[3], int const (&c)[3], int & neg)
+{
+  int  I[3], n[3];
+  bool outside = false;
+  neg = -1;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    n[d] = d < DIM ? (N[d] < 0 ? -1 : (N[d] >= S.nb[d] ? 1 : 0)) : 0;
+    outside = outside or (n[d] != 0);
+    I[d] = N[d] - n[d];
+  }
+  int cc[3] = { c[0], c[1], c[2] };
+  if (outside)
+  {
+#pragma unroll
+    for (int face = 0; face < 2 * DIM; ++face)
+    {
+      const int  d = face / 2;
+      const bool left = (face % 2) == 0;
+      const bool at_border = left ? (I[d] == 0) : (I[d] == S.nb[d] - 1);
+      if (at_border and n[d] != 0)
+      {
+        if (S.bc[face] == G5_BC_ZERO_GRADIENT)
+        {
+          cc[0] = c[0];
+          cc[1] = c[1];
+          cc[2] = c[2];
+          cc[d] = left ? 0 : B - 1;
+          neg = -1;
+        }
+        else if (S.bc[face] == G5_BC_WALL)
+        {
+          cc[0] = c[0];
+          cc[1] = c[1];
+          cc[2] = c[2];
+          cc[d] = B - 1 - c[d];
+          neg = d;
+        }
+      }
+    }
+  }
+  const int             blk = S.base + I[0] + S.nb[0] * (I[1] + S.nb[1] * I[2]);
+  const int             cell = cc[0] + B * (cc[1] + (DIM == 3 ? B * cc[2] : 0));
+  constexpr int64_t     NV = DIM == 2 ? 8 : 9;
+  constexpr int64_t     NC = DIM == 2 ? B * B : B * B * B;
+  return static_cast<int64_t>(blk) * NV * NC + cell;
+}
+
+// the cell at tile coordinates g (in [-2, B+1]^d) of owned block (bx,by,bz)
+template <int DIM, int B>
+G5D int64_t
+resolve_cell(SolveDesc const & S, int bx, int by, int bz, int gi, int gj, int gk, int & neg)
+{
+  const int g[3] = { gi, gj, gk };
+  const int b3[3] = { bx, by, bz };
+  int       c[3], N[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    if (d >= DIM)
+    {
+      c[d] = 0;
+      N[d] = 0;
+      continue;
+    }
+    const int dir = g[d] < 0 ? -1 : (g[d] >= B ? 1 : 0);
+    c[d] = g[d] - dir * B;
+    int Nd = b3[d] + dir;
+    if (S.periodic[d])
+      Nd = Nd < 0 ? Nd + S.nb[d] : (Nd >= S.nb[d] ? Nd - S.nb[d] : Nd);
+    N[d] = Nd;
+  }
+  return resolve_block_cell<DIM, B>(S, N, c, neg);
+}
+
+// load the conservative state of one cell: base = offset of (block, var 0, cell)
+template <int DIM, int B>
+G5D Cons
+load_cons(double const * __restrict__ U, int64_t base, int neg)
+{
+  constexpr int NC = DIM == 2 ? B * B : B * B * B;
+  Cons          u;
+  u.d = U[base + 0 * NC];
+  u.e = U[base + 1 * NC];
+  u.mu = U[base + 2 * NC];
+  u.mv = U[base + 3 * NC];
+  if (DIM == 3)
+  {
+    u.mw = U[base + 4 * NC];
+    u.a0 = U[base + 5 * NC];
+    u.ar0 = U[base + 6 * NC];
+    u.a1 = U[base + 7 * NC];
+    u.ar1 = U[base + 8 * NC];
+  }
+  else
+  {
+    u.mw = 0.0;
+    u.a0 = U[base + 4 * NC];
+    u.ar0 = U[base + 5 * NC];
+    u.a1 = U[base + 6 * NC];
+    u.ar1 = U[base + 7 * NC];
+  }
+  if (neg == 0)
+    u.mu *= -1;
+  else if (neg == 1)
+    u.mv *= -1;
+  else if (neg == 2)
+    u.mw *= -1;
+  return u;
+}
+
+template <int DIM, int B>
+G5D void
+store_cons(double * __restrict__ U, int64_t base, Cons const & u)
+{
+  constexpr int NC = DIM == 2 ? B * B : B * B * B;
+  U[base + 0 * NC] = u.d;
+  U[base + 1 * NC] = u.e;
+  U[base + 2 * NC] = u.mu;
+  U[base + 3 * NC] = u.mv;
+  if (DIM == 3)
+  {
+    U[base + 4 * NC] = u.mw;
+    U[base + 5 * NC] = u.a0;
+    U[base + 6 * NC] = u.ar0;
+    U[base + 7 * NC] = u.a1;
+    U[base + 8 * NC] = u.ar1;
+  }
+  else
+  {
+    U[base + 4 * NC] = u.a0;
+    U[base + 5 * NC] = u.ar0;
+    U[base + 6 * NC] = u.a1;
+    U[base + 7 * NC] = u.ar1;
+  }
+}
+
+// wavefront max (butterfly over the 64 lanes; fmax is exact so the order does not matter)
+G5D double
+wave_max(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+} // namespace g5

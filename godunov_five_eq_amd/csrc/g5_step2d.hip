@@ -1,0 +1,373 @@
+// g5_step2d.hip -- fused 2-D five-equation step, one 16x16 block per workgroup (FP64).
+//
+// One launch does, for every owned block, what the reference does in seven launches
+// (scheme/GodunovImplemV0.cpp:83-151 plus the border fill and the next dt reduction):
+//   1. gather the 20x20 tile (ghost width 2) from the neighbour blocks, resolving the outside
+//      border conditions on the fly, and convert it to primitives into LDS
+//      (ConvertToPrimitivesVariablesFunctor + FillOutsideCellFunctor);
+//   2. limited slopes, sound speed and Hancock half-step terms per cell, then the reconstructed
+//      face states (ComputeLimitedSlopesFunctor + reconstruct_state_2d, MUSCL or THINC);
+//   3. HLLC on the 17x16 x-faces and 16x17 y-faces (riemann_hllc);
+//   4. flux exchange through LDS and the x then y conservative update with clamps,
+//      volume-fraction normalisation (ReadFluxesAndConservativeUpdateFunctor,
+//      FixVolumeFractionsNormalization);
+//   5. the CFL contribution of the new state, reduced over the wavefront with shuffles and over
+//      the workgroup in LDS, then one atomicMax per block (ComputeDtFiveEqFunctor).
+// Thread t owns interior cell (t%16, t/16); threads 0..63 (wave 0) also own the 64 ring cells
+// whose single face state the block needs, threads 0..143 convert the 144 halo cells.
+#include "g5_device.h"
+#include "g5_kernels.h"
+
+namespace g5
+{
+
+namespace
+{
+
+constexpr int B2 = 16;
+constexpr int T2 = B2 + 4;   // tile edge
+constexpr int NT2 = T2 * T2; // 400 tile cells
+constexpr int NF2 = 17 * 16; // faces per direction
+
+G5D int
+tidx2(int gi, int gj)
+{
+  return (gi + 2) + T2 * (gj + 2);
+}
+// x-face fi in [0,16] of row j ; y-face fj in [0,16] of column i
+G5D int
+fx2(int fi, int j)
+{
+  return fi + 17 * j;
+}
+G5D int
+fy2(int i, int fj)
+{
+  return i + 16 * fj;
+}
+
+// halo cell h in [0,144): 64 predictor-ring cells then 80 outer cells
+G5D void
+halo_cell2(int h, int & gi, int & gj)
+{
+  if (h < 64)
+  {
+    const int r = h & 15, s = h >> 4;
+    gi = s == 0 ? -1 : (s == 1 ? 16 : r);
+    gj = s == 0 ? r : (s == 1 ? r : (s == 2 ? -1 : 16));
+    return;
+  }
+  int c = h - 64;
+  if (c < 4)
+  {
+    gi = (c & 1) ? 16 : -1;
+    gj = (c & 2) ? 16 : -1;
+    return;
+  }
+  c -= 4;
+  if (c < 40)
+  {
+    gi = (c % 20) - 2;
+    gj = c < 20 ? -2 : 17;
+    return;
+  }
+  c -= 40; // 36 cells: columns -2 and 17, rows -1..16
+  gi = c < 18 ? -2 : 17;
+  gj = (c % 18) - 1;
+}
+
+G5D Prim
+sq_load2(double const * sQ, int idx)
+{
+  Prim p;
+  p.r0 = sQ[0 * NT2 + idx];
+  p.r1 = sQ[1 * NT2 + idx];
+  p.a = sQ[2 * NT2 + idx];
+  p.p = sQ[3 * NT2 + idx];
+  p.u = sQ[4 * NT2 + idx];
+  p.v = sQ[5 * NT2 + idx];
+  p.w = 0.0;
+  return p;
+}
+
+G5D void
+sq_store2(double * sQ, int idx, Prim const & p)
+{
+  sQ[0 * NT2 + idx] = p.r0;
+  sQ[1 * NT2 + idx] = p.r1;
+  sQ[2 * NT2 + idx] = p.a;
+  sQ[3 * NT2 + idx] = p.p;
+  sQ[4 * NT2 + idx] = p.u;
+  sQ[5 * NT2 + idx] = p.v;
+}
+
+// face states: 6 doubles (r0, r1, a, p, u, v) at slot k of a [6][NF2] array
+G5D void
+st_store2(double * A, int k, Face const & f)
+{
+  A[0 * NF2 + k] = f.r0;
+  A[1 * NF2 + k] = f.r1;
+  A[2 * NF2 + k] = f.a;
+  A[3 * NF2 + k] = f.p;
+  A[4 * NF2 + k] = f.u;
+  A[5 * NF2 + k] = f.v;
+}
+G5D Face
+st_load2(double const * A, int k)
+{
+  Face f;
+  f.r0 = A[0 * NF2 + k];
+  f.r1 = A[1 * NF2 + k];
+  f.a = A[2 * NF2 + k];
+  f.p = A[3 * NF2 + k];
+  f.u = A[4 * NF2 + k];
+  f.v = A[5 * NF2 + k];
+  f.w = 0.0;
+  return f;
+}
+// fluxes: 8 doubles (ar0, ar1, a0, a1, e, mu, mv, us) at slot k of a [8][NF2] array
+G5D void
+fl_store2(double * A, int k, Flux const & F)
+{
+  A[0 * NF2 + k] = F.ar0;
+  A[1 * NF2 + k] = F.ar1;
+  A[2 * NF2 + k] = F.a0;
+  A[3 * NF2 + k] = F.a1;
+  A[4 * NF2 + k] = F.e;
+  A[5 * NF2 + k] = F.mu;
+  A[6 * NF2 + k] = F.mv;
+  A[7 * NF2 + k] = F.us;
+}
+G5D Flux
+fl_load2(double const * A, int k)
+{
+  Flux F;
+  F.ar0 = A[0 * NF2 + k];
+  F.ar1 = A[1 * NF2 + k];
+  F.a0 = A[2 * NF2 + k];
+  F.a1 = A[3 * NF2 + k];
+  F.e = A[4 * NF2 + k];
+  F.mu = A[5 * NF2 + k];
+  F.mv = A[6 * NF2 + k];
+  F.mw = 0.0;
+  F.us = A[7 * NF2 + k];
+  return F;
+}
+
+// the reconstructed face state of tile cell (gi,gj) on axis ax with side ib (+1 right, -1 left)
+template <bool THINC, bool HANCOCK>
+G5D void
+predict_faces2(SolveDesc const & S, double const * sQ, int gi, int gj, double dtdx, bool all,
+               int ax1, int ib1, Face & fxm, Face & fxp, Face & fym, Face & fyp, Face & f1)
+{
+  const Prim  qc = sq_load2(sQ, tidx2(gi, gj));
+  const Prim  qxm = sq_load2(sQ, tidx2(gi - 1, gj));
+  const Prim  qxp = sq_load2(sQ, tidx2(gi + 1, gj));
+  const Prim  qym = sq_load2(sQ, tidx2(gi, gj - 1));
+  const Prim  qyp = sq_load2(sQ, tidx2(gi, gj + 1));
+  const Slope sx = slopes_of(qc, qxp, qxm, S.slope_type);
+  const Slope sy = slopes_of(qc, qyp, qym, S.slope_type);
+  const Slope sz = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+  Hancock     h = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+  if (HANCOCK)
+    h = hancock_terms<2>(S, qc, sx, sy, sz, dtdx);
+  if (!THINC)
+  {
+    if (all)
+    {
+      fxm = face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, -0.5);
+      fxp = face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, 0.5);
+      fym = face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, -0.5);
+      fyp = face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, 0.5);
+    }
+    else
+    {
+      const double o = 0.5 * static_cast<double>(ib1);
+      f1 = ax1 == 0 ? face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, o)
+                    : face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, o);
+    }
+    return;
+  }
+  // THINC: Youngs normal from the 3x3 alpha stencil, needed only in mixed cells
+  double nx = 0.0, ny = 0.0;
+  if (qc.a > S.thinc_eps and qc.a < 1.0 - S.thinc_eps)
+  {
+    double a9[9];
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int di = -1; di <= 1; ++di)
+        a9[(di + 1) + 3 * (dj + 1)] = sQ[2 * NT2 + tidx2(gi + di, gj + dj)];
+    youngs2(a9, nx, ny);
+  }
+  if (all)
+  {
+    const ThincAxis tx = thinc_axis(S, qc.a, qxm.a, qc.a, qxp.a, fabs(nx));
+    const ThincAxis ty = thinc_axis(S, qc.a, qym.a, qc.a, qyp.a, fabs(ny));
+    fxm = face_thinc<2, 0, HANCOCK>(qc, sx, sy, sz, h, tx, -1);
+    fxp = face_thinc<2, 0, HANCOCK>(qc, sx, sy, sz, h, tx, 1);
+    fym = face_thinc<2, 1, HANCOCK>(qc, sx, sy, sz, h, ty, -1);
+    fyp = face_thinc<2, 1, HANCOCK>(qc, sx, sy, sz, h, ty, 1);
+  }
+  else
+  {
+    if (ax1 == 0)
+    {
+      const ThincAxis tx = thinc_axis(S, qc.a, qxm.a, qc.a, qxp.a, fabs(nx));
+      f1 = face_thinc<2, 0, HANCOCK>(qc, sx, sy, sz, h, tx, ib1);
+    }
+    else
+    {
+      const ThincAxis ty = thinc_axis(S, qc.a, qym.a, qc.a, qyp.a, fabs(ny));
+      f1 = face_thinc<2, 1, HANCOCK>(qc, sx, sy, sz, h, ty, ib1);
+    }
+  }
+}
+
+} // namespace
+
+G5D int
+logical_block(StepArgs const & A)
+{
+  // workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous run of blocks so
+  // that neighbouring blocks (which re-read each other's halo) share an L2
+  const int w = blockIdx.x;
+  return (w & 7) * A.per_xcd + (w >> 3);
+}
+
+template <bool THINC, bool HANCOCK>
+__global__ void __launch_bounds__(256)
+k_step2d(StepArgs A)
+{
+  __shared__ double sQ[6 * NT2];
+  __shared__ double sE[16 * NF2];
+  __shared__ double sRed[4];
+
+  const int t = threadIdx.x;
+  const int blk = logical_block(A);
+  if (blk >= A.nblocks)
+    return;
+  const BlockDesc bd = A.blocks[blk];
+  const SolveDesc S = A.solves[bd.solve];
+  SolveCtl *      C = A.ctl + bd.solve;
+  const int64_t   own = static_cast<int64_t>(blk) * 8 * 256 + t;
+  const bool      host_dt = A.dt_fixed > 0.0;
+  if (!host_dt and C->active == 0)
+  {
+    // finished solve: carry the state over so both buffers hold it
+    for (int v = 0; v < 8; ++v)
+      A.U2[own + v * 256] = A.U[own + v * 256];
+    return;
+  }
+  const double dt = host_dt ? A.dt_fixed : C->dt;
+  const double dtdx = dt / S.dx;
+  const int    i = t & 15, j = t >> 4;
+
+  // ---- 1. gather + primitives ----
+  Cons uown = load_cons<2, B2>(A.U, own, -1);
+  {
+    Prim q;
+    cons_to_prim<2>(S, uown, q);
+    sq_store2(sQ, tidx2(i, j), q);
+  }
+  if (t < 144)
+  {
+    int gi, gj, neg;
+    halo_cell2(t, gi, gj);
+    const int64_t off = resolve_cell<2, B2>(S, bd.bx, bd.by, 0, gi, gj, 0, neg);
+    const Cons    u = load_cons<2, B2>(A.U, off, neg);
+    Prim          q;
+    cons_to_prim<2>(S, u, q);
+    sq_store2(sQ, tidx2(gi, gj), q);
+  }
+  __syncthreads();
+
+  // ---- 2. slopes, Hancock terms, face states ----
+  double * XL = sE;           // [6][NF2] left states of the x faces
+  double * YL = sE + 6 * NF2; // [6][NF2] left states of the y faces
+  Face     fxm, fxp, fym, fyp, fdummy, fring;
+  predict_faces2<THINC, HANCOCK>(S, sQ, i, j, dtdx, true, 0, 0, fxm, fxp, fym, fyp, fdummy);
+  st_store2(XL, fx2(i + 1, j), fxp);
+  st_store2(YL, fy2(i, j + 1), fyp);
+  // ring cells: s=0 (-1,r) +x ; s=1 (16,r) -x ; s=2 (r,-1) +y ; s=3 (r,16) -y
+  const int rs = t >> 4, rr = t & 15;
+  if (t < 64)
+  {
+    int gi, gj;
+    halo_cell2(t, gi, gj);
+    const int ax = rs < 2 ? 0 : 1;
+    const int ib = (rs & 1) ? -1 : 1;
+    predict_faces2<THINC, HANCOCK>(S, sQ, gi, gj, dtdx, false, ax, ib, fdummy, fdummy, fdummy,
+                                   fdummy, fring);
+    if (rs == 0)
+      st_store2(XL, fx2(0, rr), fring);
+    else if (rs == 2)
+      st_store2(YL, fy2(rr, 0), fring);
+  }
+  __syncthreads();
+
+  // ---- 3. HLLC ----
+  const Flux Fx = hllc<2>(S, st_load2(XL, fx2(i, j)), fxm, dtdx, 0);
+  const Flux Fy = hllc<2>(S, st_load2(YL, fy2(i, j)), fym, dtdx, 1);
+  Flux       Fr;
+  const bool ring_face = t < 64 and (rs & 1);
+  if (ring_face)
+  {
+    const int  ax = rs == 1 ? 0 : 1;
+    const Face L = ax == 0 ? st_load2(XL, fx2(16, rr)) : st_load2(YL, fy2(rr, 16));
+    Fr = hllc<2>(S, L, fring, dtdx, ax);
+  }
+  __syncthreads();
+
+  // ---- 4. flux exchange and update ----
+  double * FX = sE;           // [8][NF2]
+  double * FY = sE + 8 * NF2; // [8][NF2]
+  fl_store2(FX, fx2(i, j), Fx);
+  fl_store2(FY, fy2(i, j), Fy);
+  if (ring_face)
+  {
+    if (rs == 1)
+      fl_store2(FX, fx2(16, rr), Fr);
+    else
+      fl_store2(FY, fy2(rr, 16), Fr);
+  }
+  __syncthreads();
+  Cons u2 = uown;
+  apply_flux_pair<2>(u2, Fx, fl_load2(FX, fx2(i + 1, j)), uown.a0, uown.a1);
+  apply_flux_pair<2>(u2, Fy, fl_load2(FY, fy2(i, j + 1)), uown.a0, uown.a1);
+  fix_volume_fractions(u2);
+  store_cons<2, B2>(A.U2, own, u2);
+
+  // ---- 5. CFL of the new state ----
+  double inv = wave_max(cell_inv_dt<2>(S, u2));
+  if ((t & 63) == 0)
+    sRed[t >> 6] = inv;
+  __syncthreads();
+  if (t == 0)
+  {
+    inv = fmax(fmax(sRed[0], sRed[1]), fmax(sRed[2], sRed[3]));
+    atomicMax(&C->invdt_bits, static_cast<unsigned long long>(__double_as_longlong(inv)));
+  }
+}
+
+void
+launch_step2d(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
+{
+  const dim3 grid(8 * A.per_xcd), block(256);
+  if (thinc)
+  {
+    if (hancock)
+      hipLaunchKernelGGL((k_step2d<true, true>), grid, block, 0, stream, A);
+    else
+      hipLaunchKernelGGL((k_step2d<true, false>), grid, block, 0, stream, A);
+  }
+  else
+  {
+    if (hancock)
+      hipLaunchKernelGGL((k_step2d<false, true>), grid, block, 0, stream, A);
+    else
+      hipLaunchKernelGGL((k_step2d<false, false>), grid, block, 0, stream, A);
+  }
+}
+
+} // namespace g5

@@ -1,0 +1,196 @@
+/*
+ * godunov_five_eq.h -- C ABI of the MI355X-native five-equation Godunov block step.
+ *
+ * Drop-in boundary for the reference's GodunovImplemBase plugin surface
+ * (scheme/GodunovImplemBase.h:59-198) and the solver hooks next to it
+ * (SolverGodunovFiveEq.cpp:368-390 compute_dt_local, :481-526 godunov_unsplit,
+ * :978-1019 fill_outside_quadrants).  Plain C types only: no torch, no C++ types.
+ *
+ * Data layout (both host and device buffers): U[block][var][cell], doubles.
+ *   - block: the owned blocks of a brick of nb[0] x nb[1] (x nb[2]) blocks, lexicographic with x
+ *     fastest;
+ *   - var: the FiveEq ids of models/FiveEq.h:35-50 (2-D: ID=0, IE=1, IU=2, IV=3, IA0=4, IAD0=5,
+ *     IA1=6, IAD1=7; 3-D: IW=4 shifts IA0..IAD1 to 5..8);
+ *   - cell: bsize^dim cells of the block, x fastest.
+ *
+ * Every entry point returns G5_OK (0) or an error code; g5_last_error() gives the message.
+ * No exception crosses the ABI.  A context is not thread-safe; it owns its device buffers and
+ * one HIP stream.
+ */
+#ifndef GODUNOV_FIVE_EQ_H_
+#define GODUNOV_FIVE_EQ_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+enum
+{
+  G5_OK = 0,
+  G5_ERR_INVALID = 1,     /* bad argument or configuration */
+  G5_ERR_HIP = 2,         /* a HIP runtime call failed */
+  G5_ERR_UNSUPPORTED = 3, /* valid configuration this build does not implement */
+  G5_ERR_NOMEM = 4,       /* allocation failed or HBM budget exceeded */
+  G5_ERR_STATE = 5        /* call out of order (e.g. step before upload) */
+};
+
+/* border conditions, border_conditions/FillOutside.h:51-55 (BC_HYDRO) */
+enum
+{
+  G5_BC_NONE = 0,
+  G5_BC_PERIODIC = 1,
+  G5_BC_ZERO_GRADIENT = 2,
+  G5_BC_WALL = 3
+};
+
+/* time integrators, SolverGodunovFiveEq.cpp:484-524 */
+enum
+{
+  G5_TI_HANCOCK = 0,
+  G5_TI_RK1 = 1,
+  G5_TI_RK2_SSP = 2
+};
+
+/* arithmetic of the step */
+enum
+{
+  G5_PREC_FP64 = 0, /* everything in FP64 */
+  G5_PREC_MIXED = 1 /* FP32 slopes, wave speeds and THINC; FP64 state, EOS pressure recovery, dt */
+};
+
+/*
+ * Solve configuration.  Field meaning follows the reference .ini keys:
+ * [hydro] cfl slope_type smallr smallc (models/HydroSettings.cpp:18-29),
+ * [material{0,1}_mie_gruneisen_{ig,sg}] gamma pinf, [THINC] enabled beta epsilon,
+ * [mesh] boundary_type_{x,y,z}{min,max}, [p4est_connectivity] periodic_{x,y,z},
+ * [amr] bx/by/bz (= bsize) and time_integrator.
+ * dx is the cell length (compute_cell_length(level, bx) * scaling_factor).
+ */
+typedef struct g5_config
+{
+  int32_t dim;         /* 2 or 3 */
+  int32_t bsize;       /* cells per block edge; the HIP step kernels take 16 */
+  int32_t nb[3];       /* blocks per direction (nb[2] = 1 in 2-D) */
+  int32_t periodic[3]; /* brick periodicity per direction */
+  int32_t bc[6];       /* G5_BC_* for faces xmin, xmax, ymin, ymax, zmin, zmax */
+  int32_t integrator;  /* G5_TI_* */
+  int32_t thinc_enabled;
+  int32_t precision; /* G5_PREC_* */
+  double  dx;
+  double  cfl;
+  double  slope_type; /* 1 minmod, 2 MC */
+  double  smallr;
+  double  smallc;
+  double  gamma[2]; /* per material */
+  double  pinf[2];  /* per material (0 for an ideal gas) */
+  double  thinc_beta;
+  double  thinc_epsilon;
+  double  xmin[3]; /* domain origin, used by the initial-condition generators */
+} g5_config;
+
+typedef struct g5_ctx g5_ctx;
+
+/* ---- library ---- */
+
+/* version string of this build */
+const char * g5_version(void);
+
+/* message of the last failed call on ctx (or of the last failed g5_create when ctx is NULL) */
+const char * g5_last_error(const g5_ctx * ctx);
+
+/* ---- context life cycle ---- */
+
+/* Replaces SolverGodunovFiveEq's allocation of U/U2 (SolverGodunovFiveEq.cpp:147-157) and
+ * GodunovImplemV0::create + resize_auxiliary_data (scheme/GodunovImplemV0.h:103-114,
+ * scheme/GodunovImplemV0.cpp:35-61): validates cfg, allocates the device state and scratch. */
+int g5_create(const g5_config * cfg, g5_ctx ** out);
+
+int g5_destroy(g5_ctx * ctx);
+
+/* number of owned blocks, variables per cell and cells per block */
+int g5_shape(const g5_ctx * ctx, int32_t * n_blocks, int32_t * nvar, int32_t * cells_per_block);
+
+/* Replaces GodunovImplemBase::total_mem_size_in_bytes (scheme/GodunovImplemBase.h:97-99):
+ * device bytes held by the context (state and scratch). */
+int g5_mem_bytes(const g5_ctx * ctx, uint64_t * bytes);
+
+/* ---- state transfer (the solver's U, SolverGodunovFiveEq.h:349-358) ---- */
+
+/* n = number of doubles, must equal n_blocks * nvar * cells_per_block */
+int g5_upload_state(g5_ctx * ctx, const double * U_host, size_t n);
+int g5_download_state(g5_ctx * ctx, double * U_host, size_t n);
+
+/* device pointer of the current state (valid until the next step) */
+int g5_state_device_ptr(g5_ctx * ctx, double ** U_dev);
+
+/* ---- the step ---- */
+
+/* Replaces SolverGodunovFiveEq::compute_dt_local (SolverGodunovFiveEq.cpp:368-390) and
+ * ComputeDtFiveEqFunctor::apply (scheme/ComputeDtFiveEqFunctor.cpp:45-81):
+ * *dt = cfl / max over owned cells of sum_d (c + |u_d|)/dx.  Synchronises the stream. */
+int g5_compute_dt(g5_ctx * ctx, double * dt);
+
+/* Replaces SolverGodunovFiveEq::fill_outside_quadrants (SolverGodunovFiveEq.cpp:978-1019,
+ * FillOutsideCellFunctor border_conditions/FillOutside.cpp:42-170): gathers the outside-block ring
+ * into out (n = n_outside * nvar * cells_per_block doubles, blocks in the order of
+ * g5_outside_coords) so callers can inspect it.  The step kernels resolve the same border
+ * conditions on the fly and do not need this call. */
+int g5_fill_outside(g5_ctx * ctx, double * out_host, size_t n);
+
+/* number of outside blocks and their brick coordinates (3 int32 per block) */
+int g5_outside_coords(const g5_ctx * ctx, int32_t * n_outside, int32_t * coords, size_t ncoords);
+
+/* Replaces GodunovImplemV0::do_time_step(U, U2, dt) (scheme/GodunovImplemV0.cpp:83-151) together
+ * with the HANCOCK/RK1 branch of godunov_unsplit (SolverGodunovFiveEq.cpp:484-495): border fill,
+ * primitives, slopes, per-direction fluxes and updates, volume-fraction normalisation, U <- U2.
+ * The RK2_SSP integrator runs both stages (SolverGodunovFiveEq.cpp:496-524).  Asynchronous. */
+int g5_do_time_step(g5_ctx * ctx, double dt);
+
+/* Runs up to nsteps iterations of next_iteration_impl (SolverGodunovFiveEq.cpp:395-476) with the
+ * dt computed and clamped to tEnd on the device (no host round trip per step; the step sequence
+ * is captured into a HIP graph).  *t is advanced; *steps_done receives the count.  Synchronises. */
+int g5_run(g5_ctx * ctx, int32_t nsteps, double * t, double tEnd, int32_t * steps_done);
+
+/* HIP-event duration (stream time, ms) of the step loop of the last g5_run call: nsteps fused
+ * step kernels plus their one-thread dt finalisation kernels */
+int g5_last_run_ms(const g5_ctx * ctx, double * ms);
+
+/* wait for all work of the context */
+int g5_synchronize(g5_ctx * ctx);
+
+/* Time the fused step kernel on the context's stream with HIP events: runs nsteps steps with a
+ * fixed dt and reports the mean kernel duration in milliseconds (used by the benchmark). */
+int g5_time_step_kernel(g5_ctx * ctx, int32_t nsteps, double dt, double * mean_ms);
+
+/* ---- initial conditions (init/InitShockBubble.cpp:50-225, common.h:68-144) ---- */
+
+/* One region's primitive state, the [regionN] keys of common.h:80-109. */
+typedef struct g5_region
+{
+  double alpha0;
+  double alpha_rho[2];
+  double p;
+  double u, v, w;
+} g5_region;
+
+/* Shock-bubble initial condition of InitShockBubbleDataFunctor: region 0 post-shock
+ * (x <= x_front), region 1 pre-shock, region 2 bubble of radius r at center c, with the bubble
+ * volume fraction of each cell computed from the sphere.  Fills U_host (owned blocks). */
+int g5_init_shock_bubble(const g5_config * cfg, const g5_region * regions3, double x_front,
+                         const double * center3, double radius, double * U_host, size_t n);
+
+/* Post-shock state of a planar shock moving into a resting stiffened gas (Rankine-Hugoniot);
+ * for p_inf = 0 this is app/tools/moving_shock.py:79-163.  Either mach > 0 (shock Mach number) or
+ * p2 > 0 (post-shock pressure) is given; outputs rho2, u2, p2 and the shock speed. */
+int g5_rankine_hugoniot(double gamma, double pinf, double rho1, double p1, double mach,
+                        double p2_in, double * rho2, double * u2, double * p2, double * shock_speed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GODUNOV_FIVE_EQ_H_ */

@@ -1,0 +1,55 @@
+"""Workload definitions shared by the tests, smoke() and bench.py.
+
+shock_bubble_2d_air_helium: 2-D air-helium shock-bubble of app/test_shock_bubble_air_helium_2d.ini
+  at 512x128 cells (domain 4x1, 16x16 blocks), air gamma=1.4, helium gamma=1.648, bubble radius
+  0.25 at (1.0, 0.5), density ratio 0.138, Mach 1.22 shock, minmod, HLLC, CFL 0.8, HANCOCK,
+  ZERO_GRADIENT in x, WALL in y.
+shock_bubble_3d_air_water: 3-D air bubble (radius 0.2) in water (gamma 4.4, pinf 6e8 Pa,
+  rho 1000) hit by a 1e9 Pa planar shock, THINC beta 1.6, HLLC, CFL 0.5, HANCOCK.  Unit cube,
+  bubble centre (0.5, 0.5, 0.5), shock front at x = 0.25, pre-shock pressure 1e5 Pa, air rho 1.
+"""
+import godunov_five_eq_amd as g5
+
+
+def air_helium_2d(nbx=32, nby=8, **kw):
+    dx = 1.0 / (16 * nby)
+    rho2, u2, p2, _ = g5.rankine_hugoniot(1.4, 0.0, 1.0, 1.0, mach=1.22)
+    opts = dict(bc=("ZERO_GRADIENT", "ZERO_GRADIENT", "WALL", "WALL", "NONE", "NONE"), cfl=0.8,
+                slope_type=1.0, gamma=(1.4, 1.648), pinf=(0.0, 0.0))
+    opts.update(kw)
+    cfg = g5.make_config(2, (nbx, nby), dx, **opts)
+    regs = [g5.region(1.0, (rho2, 0.0), p2, u2), g5.region(1.0, (1.0, 0.0), 1.0),
+            g5.region(0.0, (0.0, 0.138), 1.0)]
+    # bubble of radius 0.25 at x=1, mid height; shock front at x=0.6
+    yc = 0.5 * nby * 16 * dx
+    U0 = g5.init_shock_bubble(cfg, regs, 0.6, (1.0, yc), 0.25)
+    return cfg, U0
+
+
+def air_water_3d(nb=16, thinc=True, **kw):
+    dx = 1.0 / (16 * nb)
+    pinf_w = 6e8
+    rho2, u2, p2, _ = g5.rankine_hugoniot(4.4, pinf_w, 1000.0, 1e5, p2=1e9)
+    opts = dict(bc=("ZERO_GRADIENT",) * 6, cfl=0.5, slope_type=1.0, gamma=(1.4, 4.4),
+                pinf=(0.0, pinf_w), thinc=thinc, thinc_beta=1.6, thinc_epsilon=1e-3)
+    opts.update(kw)
+    cfg = g5.make_config(3, (nb, nb, nb), dx, **opts)
+    # material 0 = air, material 1 = water
+    regs = [g5.region(0.0, (0.0, rho2), p2, u2), g5.region(0.0, (0.0, 1000.0), 1e5),
+            g5.region(1.0, (1.0, 0.0), 1e5)]
+    U0 = g5.init_shock_bubble(cfg, regs, 0.25, (0.5, 0.5, 0.5), 0.2)
+    return cfg, U0
+
+
+def air_helium_3d(nb=4, **kw):
+    """Small 3-D air-helium shock-bubble (MUSCL) for parity tests."""
+    dx = 1.0 / (16 * nb)
+    rho2, u2, p2, _ = g5.rankine_hugoniot(1.4, 0.0, 1.0, 1.0, mach=1.22)
+    opts = dict(bc=("ZERO_GRADIENT", "ZERO_GRADIENT", "WALL", "WALL", "WALL", "WALL"), cfl=0.8,
+                slope_type=1.0, gamma=(1.4, 1.648), pinf=(0.0, 0.0))
+    opts.update(kw)
+    cfg = g5.make_config(3, (nb, nb, nb), dx, **opts)
+    regs = [g5.region(1.0, (rho2, 0.0), p2, u2), g5.region(1.0, (1.0, 0.0), 1.0),
+            g5.region(0.0, (0.0, 0.138), 1.0)]
+    U0 = g5.init_shock_bubble(cfg, regs, 0.2, (0.5, 0.5, 0.5), 0.25)
+    return cfg, U0

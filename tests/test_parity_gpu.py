@@ -1,0 +1,180 @@
+"""Parity of the HIP path (libg5.so through the C ABI) with the CPU FP64 oracle.
+
+Bar: the FP64 path is expected bit-identical to the oracle (same operation order, contraction off,
+IEEE division and sqrt on gfx950); the test asserts relative L1 <= 1e-12 per conserved variable
+(the tolerance of the FP64 parity requirement) and reports whether the match is exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+RTOL_L1 = 1e-12
+
+
+def _announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel_l1(a, b, dim):
+    nv = 8 if dim == 2 else 9
+    A = a.reshape(-1, nv, 16 ** dim)
+    B = b.reshape(-1, nv, 16 ** dim)
+    out = []
+    for v in range(nv):
+        den = np.abs(B[:, v]).sum()
+        num = np.abs(A[:, v] - B[:, v]).sum()
+        out.append(num / den if den > 0 else num)
+    return np.array(out)
+
+
+def check_parity(U_gpu, U_orc, dim, label):
+    assert np.all(np.isfinite(U_gpu)), label
+    err = rel_l1(U_gpu, U_orc, dim)
+    exact = np.array_equal(U_gpu, U_orc)
+    _announce(f"{label}: max rel L1 = {err.max():.3e} bit-exact={exact}")
+    assert err.max() <= RTOL_L1, (label, err)
+    return exact
+
+
+def run_both(cfg, U0, nsteps, g5, orc, label):
+    _announce(f"{label}: dim={cfg.dim} nb={list(cfg.nb)} thinc={cfg.thinc_enabled} steps={nsteps}")
+    o = orc.Oracle(cfg)
+    o.set_state(U0)
+    t_o, n_o = o.run(nsteps)
+    with g5.Context(cfg) as ctx:
+        ctx.upload(U0)
+        t_g, n_g = ctx.run(nsteps)
+        U_g = ctx.download()
+    assert n_o == n_g == nsteps
+    assert t_g == t_o, (t_g, t_o)
+    return U_g, o.get_state()
+
+
+def test_parity_2d_air_helium_200_steps(g5, orc):
+    # the plain FP64 2-D air-helium shock-bubble at 512x128, 200 steps
+    cfg, U0 = cases.air_helium_2d()
+    U_g, U_o = run_both(cfg, U0, 200, g5, orc, "2d air-helium 512x128")
+    assert check_parity(U_g, U_o, 2, "2d air-helium 512x128 200 steps")
+
+
+def test_parity_2d_thinc(g5, orc):
+    cfg, U0 = cases.air_helium_2d(nbx=16, nby=8, thinc=True, thinc_beta=1.6)
+    U_g, U_o = run_both(cfg, U0, 60, g5, orc, "2d air-helium THINC")
+    assert check_parity(U_g, U_o, 2, "2d air-helium THINC 60 steps")
+
+
+def test_parity_2d_mc_periodic(g5, orc):
+    cfg, U0 = cases.air_helium_2d(nbx=6, nby=5, slope_type=2.0, periodic=(1, 1, 0), bc=("PERIODIC",) * 6)
+    U_g, U_o = run_both(cfg, U0, 40, g5, orc, "2d MC periodic")
+    assert check_parity(U_g, U_o, 2, "2d MC periodic 40 steps")
+
+
+def test_parity_2d_air_water_walls(g5, orc):
+    # stiffened gas (pinf 6e8) in 2-D with walls on all four sides (corner rule exercised)
+    rho2, u2, p2, _ = g5.rankine_hugoniot(4.4, 6e8, 1000.0, 1e5, p2=1e9)
+    cfg = g5.make_config(2, (8, 8), 1.0 / 128, bc=("WALL",) * 4 + ("NONE", "NONE"), cfl=0.5,
+                         gamma=(1.4, 4.4), pinf=(0.0, 6e8), thinc=True, thinc_beta=1.6)
+    regs = [g5.region(0.0, (0.0, rho2), p2, u2), g5.region(0.0, (0.0, 1000.0), 1e5),
+            g5.region(1.0, (1.0, 0.0), 1e5)]
+    U0 = g5.init_shock_bubble(cfg, regs, 0.25, (0.5, 0.5), 0.2)
+    U_g, U_o = run_both(cfg, U0, 40, g5, orc, "2d air-water walls THINC")
+    assert check_parity(U_g, U_o, 2, "2d air-water walls THINC 40 steps")
+
+
+def test_parity_3d_muscl(g5, orc):
+    cfg, U0 = cases.air_helium_3d(nb=4)
+    U_g, U_o = run_both(cfg, U0, 12, g5, orc, "3d air-helium MUSCL 64^3")
+    assert check_parity(U_g, U_o, 3, "3d air-helium MUSCL 64^3 12 steps")
+
+
+def test_parity_3d_air_water_thinc(g5, orc):
+    cfg, U0 = cases.air_water_3d(nb=4)
+    U_g, U_o = run_both(cfg, U0, 10, g5, orc, "3d air-water THINC 64^3")
+    assert check_parity(U_g, U_o, 3, "3d air-water THINC 64^3 10 steps")
+
+
+def test_parity_3d_periodic_mixed_bc(g5, orc):
+    cfg, U0 = cases.air_helium_3d(nb=3, periodic=(0, 1, 0),
+                                  bc=("WALL", "ZERO_GRADIENT", "PERIODIC", "PERIODIC", "ZERO_GRADIENT", "WALL"))
+    U_g, U_o = run_both(cfg, U0, 8, g5, orc, "3d mixed borders")
+    assert check_parity(U_g, U_o, 3, "3d mixed borders 8 steps")
+
+
+def test_compute_dt_and_host_steps(g5, orc):
+    # compute_dt + do_time_step (the drop-in GodunovImplemV0 surface) against the oracle
+    cfg, U0 = cases.air_helium_2d(nbx=8, nby=4)
+    _announce("host-driven steps 2d")
+    o = orc.Oracle(cfg)
+    o.set_state(U0)
+    with g5.Context(cfg) as ctx:
+        ctx.upload(U0)
+        for _ in range(5):
+            dt_g = ctx.compute_dt()
+            dt_o = o.compute_dt()
+            assert dt_g == dt_o
+            ctx.do_time_step(dt_g)
+            o.godunov_unsplit(dt_o)
+        U_g = ctx.download()
+    check_parity(U_g, o.get_state(), 2, "host-driven 2d 5 steps")
+
+
+def test_fill_outside_matches_oracle(g5, orc):
+    for dim in (2, 3):
+        if dim == 2:
+            cfg, U0 = cases.air_helium_2d(nbx=3, nby=2, bc=("WALL", "ZERO_GRADIENT", "WALL", "ZERO_GRADIENT", "NONE", "NONE"))
+        else:
+            cfg, U0 = cases.air_helium_3d(nb=2, bc=("WALL", "ZERO_GRADIENT", "ZERO_GRADIENT", "WALL", "WALL", "ZERO_GRADIENT"))
+        _announce(f"fill_outside dim={dim}")
+        o = orc.Oracle(cfg)
+        o.set_state(U0)
+        o.fill_outside()
+        nv = 8 if dim == 2 else 9
+        Ua = o.get_state_all().reshape(-1, nv * 16 ** dim)
+        coords_o = o.block_coords()
+        lookup = {tuple(c): k for k, c in enumerate(coords_o)}
+        with g5.Context(cfg) as ctx:
+            ctx.upload(U0)
+            ring = ctx.fill_outside().reshape(-1, nv * 16 ** dim)
+            coords = ctx.outside_coords()
+        assert len(coords) == o.noutside
+        for k, c in enumerate(coords):
+            assert np.array_equal(ring[k], Ua[lookup[tuple(c)]]), (dim, c)
+
+
+def test_rk2_ssp_parity(g5, orc):
+    cfg, U0 = cases.air_helium_2d(nbx=8, nby=4, integrator="RK2_SSP")
+    U_g, U_o = run_both(cfg, U0, 10, g5, orc, "2d RK2_SSP")
+    check_parity(U_g, U_o, 2, "2d RK2_SSP 10 steps")
+
+
+def test_gpu_deterministic(g5):
+    cfg, U0 = cases.air_water_3d(nb=2)
+    _announce("determinism 3d")
+    out = []
+    for _ in range(2):
+        with g5.Context(cfg) as ctx:
+            ctx.upload(U0)
+            ctx.run(5)
+            out.append(ctx.download())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_run_stops_at_tend(g5, orc):
+    cfg, U0 = cases.air_helium_2d(nbx=4, nby=2)
+    _announce("tEnd clamp")
+    o = orc.Oracle(cfg)
+    o.set_state(U0)
+    t_end = 0.0123
+    t_o, n_o = o.run(1000, 0.0, t_end)
+    with g5.Context(cfg) as ctx:
+        ctx.upload(U0)
+        t_g, n_g = ctx.run(n_o + 7, 0.0, t_end)
+        U_g = ctx.download()
+    assert n_g == n_o and t_g == t_o
+    check_parity(U_g, o.get_state(), 2, "tEnd clamp")
