@@ -1,0 +1,438 @@
+// godunov_five_eq.hpp -- C++ host layer above the C ABI (header only).
+//
+// Mirrors the reference's plugin surface so that a C++ caller written against
+// GodunovImplemBase / SolverGodunovFiveEq finds the same names and meanings:
+//   - ConfigMap: .ini reader with the getReal/getInteger/getBool/getString/getRealVector accessors
+//     the reference reads its parameters with (same sections and keys);
+//   - HydroSettings (models/HydroSettings.{h,cpp}): cfl, slope_type, smallr, smallc, smallp, ...;
+//   - config_from_ini: the .ini keys of a fixed single-level brick mesh -> g5_config;
+//   - GodunovImplemHip<dim>: resize_auxiliary_data, total_mem_size_in_bytes, do_time_step(U, U2, dt)
+//     and do_time_step(U, U2, t, dt) of scheme/GodunovImplemBase.h:93-107;
+//   - SolverGodunovFiveEq<dim>: compute_dt_local (SolverGodunovFiveEq.cpp:368-390),
+//     godunov_unsplit (:481-526), next_iteration_impl (:395-476) and the run loop with the
+//     Mcell-updates/s accounting of :474 and :585-590.
+// Errors are reported with std::runtime_error carrying g5_last_error().
+#pragma once
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "godunov_five_eq.h"
+
+namespace g5cpp
+{
+
+// ---------------------------------------------------------------------------------------------
+// .ini reader: "[section]" headers, "key=value" lines, '#' or ';' comments
+// ---------------------------------------------------------------------------------------------
+class ConfigMap
+{
+public:
+  ConfigMap() = default;
+
+  static ConfigMap
+  from_string(std::string const & text)
+  {
+    ConfigMap          m;
+    std::istringstream in(text);
+    std::string        line, section;
+    while (std::getline(in, line))
+    {
+      const auto hash = line.find_first_of("#;");
+      if (hash != std::string::npos)
+        line = line.substr(0, hash);
+      line = trim(line);
+      if (line.empty())
+        continue;
+      if (line.front() == '[' and line.back() == ']')
+      {
+        section = trim(line.substr(1, line.size() - 2));
+        continue;
+      }
+      const auto eq = line.find('=');
+      if (eq == std::string::npos)
+        continue;
+      m.m_values[section + "/" + trim(line.substr(0, eq))] = trim(line.substr(eq + 1));
+    }
+    return m;
+  }
+
+  static ConfigMap
+  from_file(std::string const & path)
+  {
+    std::ifstream f(path);
+    if (not f)
+      throw std::runtime_error("cannot open " + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    return from_string(ss.str());
+  }
+
+  bool
+  has(std::string const & section, std::string const & key) const
+  {
+    return m_values.count(section + "/" + key) != 0;
+  }
+
+  std::string
+  getString(std::string const & section, std::string const & key, std::string const & def) const
+  {
+    const auto it = m_values.find(section + "/" + key);
+    return it == m_values.end() ? def : it->second;
+  }
+
+  double
+  getReal(std::string const & section, std::string const & key, double def) const
+  {
+    const auto it = m_values.find(section + "/" + key);
+    return it == m_values.end() ? def : std::stod(it->second);
+  }
+
+  int
+  getInteger(std::string const & section, std::string const & key, int def) const
+  {
+    const auto it = m_values.find(section + "/" + key);
+    return it == m_values.end() ? def : std::stoi(it->second);
+  }
+
+  bool
+  getBool(std::string const & section, std::string const & key, bool def) const
+  {
+    const auto it = m_values.find(section + "/" + key);
+    if (it == m_values.end())
+      return def;
+    std::string v = it->second;
+    std::transform(v.begin(), v.end(), v.begin(), ::tolower);
+    return v == "1" or v == "yes" or v == "true" or v == "on";
+  }
+
+  std::vector<double>
+  getRealVector(std::string const & section, std::string const & key, std::vector<double> const & def) const
+  {
+    const auto it = m_values.find(section + "/" + key);
+    if (it == m_values.end())
+      return def;
+    std::vector<double> out;
+    std::stringstream   ss(it->second);
+    std::string         item;
+    while (std::getline(ss, item, ','))
+      if (not trim(item).empty())
+        out.push_back(std::stod(trim(item)));
+    return out;
+  }
+
+private:
+  static std::string
+  trim(std::string const & s)
+  {
+    const auto b = s.find_first_not_of(" \t\r\n");
+    if (b == std::string::npos)
+      return "";
+    const auto e = s.find_last_not_of(" \t\r\n");
+    return s.substr(b, e - b + 1);
+  }
+  std::map<std::string, std::string> m_values;
+};
+
+// models/HydroSettings.cpp:18-29 (same keys and defaults)
+struct HydroSettings
+{
+  double      cfl, slope_type, smallr, smallc, smallp, smallpp;
+  int         niter_riemann;
+  std::string riemann;
+  explicit HydroSettings(ConfigMap const & c)
+    : cfl(c.getReal("hydro", "cfl", 0.5))
+    , slope_type(c.getReal("hydro", "slope_type", 1.0))
+    , smallr(c.getReal("hydro", "smallr", 1e-10))
+    , smallc(c.getReal("hydro", "smallc", 1e-10))
+    , smallp(smallc * smallc)
+    , smallpp(smallr * smallp)
+    , niter_riemann(c.getInteger("hydro", "niter_riemann", 10))
+    , riemann(c.getString("hydro", "riemann", "UNKNOWN"))
+  {}
+};
+
+inline int
+bc_from_name(std::string const & s)
+{
+  if (s == "ZERO_GRADIENT")
+    return G5_BC_ZERO_GRADIENT;
+  if (s == "WALL")
+    return G5_BC_WALL;
+  if (s == "PERIODIC")
+    return G5_BC_PERIODIC;
+  return G5_BC_NONE;
+}
+
+// The .ini keys of a fixed single-level brick mesh (level_min == level_max) as a g5_config:
+// [run] dimension, [amr] bx (=by=bz) level_min level_max time_integrator,
+// [p4est_connectivity] nbrick_{x,y,z} periodic_{x,y,z}, [mesh] xmin ymin zmin scaling_factor
+// boundary_type_*, [hydro] cfl slope_type smallr smallc riemann, [THINC] enabled beta epsilon,
+// [material{0,1}] mg_eos_name with [material{k}_mie_gruneisen_{ig,sg}] gamma pinf.
+inline g5_config
+config_from_ini(ConfigMap const & c)
+{
+  g5_config cfg{};
+  cfg.dim = c.getInteger("run", "dimension", 2);
+  cfg.bsize = c.getInteger("amr", "bx", 16);
+  const int lmin = c.getInteger("amr", "level_min", 0);
+  const int lmax = c.getInteger("amr", "level_max", lmin);
+  if (lmin != lmax)
+    throw std::runtime_error("only a fixed single-level mesh is supported (level_min == level_max)");
+  const char * ax[3] = { "x", "y", "z" };
+  for (int d = 0; d < 3; ++d)
+  {
+    const int nbrick = c.getInteger("p4est_connectivity", std::string("nbrick_") + ax[d], 1);
+    cfg.nb[d] = d < cfg.dim ? nbrick * (1 << lmin) : 1;
+    cfg.periodic[d] = d < cfg.dim ? c.getInteger("p4est_connectivity", std::string("periodic_") + ax[d], 0) : 0;
+    cfg.xmin[d] = c.getReal("mesh", std::string(ax[d]) + "min", 0.0);
+    for (int s = 0; s < 2; ++s)
+    {
+      const std::string key = std::string("boundary_type_") + ax[d] + (s == 0 ? "min" : "max");
+      cfg.bc[2 * d + s] = cfg.periodic[d] ? G5_BC_PERIODIC : bc_from_name(c.getString("mesh", key, "ZERO_GRADIENT"));
+    }
+  }
+  const double scaling = c.getReal("mesh", "scaling_factor", 1.0);
+  cfg.dx = scaling / (static_cast<double>(cfg.bsize) * static_cast<double>(1 << lmin));
+  const HydroSettings hs(c);
+  if (hs.riemann != "HLLC")
+    throw std::runtime_error("only riemann=HLLC is supported (models/RiemannSolvers.h:489-499)");
+  cfg.cfl = hs.cfl;
+  cfg.slope_type = hs.slope_type;
+  cfg.smallr = hs.smallr;
+  cfg.smallc = hs.smallc;
+  const std::string ti = c.getString("amr", "time_integrator", "HANCOCK");
+  cfg.integrator = ti == "RK2_SSP" ? G5_TI_RK2_SSP : (ti == "RK1" ? G5_TI_RK1 : G5_TI_HANCOCK);
+  cfg.thinc_enabled = c.getBool("THINC", "enabled", false) ? 1 : 0;
+  cfg.thinc_beta = c.getReal("THINC", "beta", 1.5);
+  cfg.thinc_epsilon = c.getReal("THINC", "epsilon", 1e-3);
+  for (int k = 0; k < 2; ++k)
+  {
+    const std::string mat = "material" + std::to_string(k);
+    const std::string eos = c.getString(mat, "mg_eos_name", "MG_IDEAL_GAS");
+    const std::string sec = mat + (eos == "MG_STIFFENED_GAS" ? "_mie_gruneisen_sg" : "_mie_gruneisen_ig");
+    cfg.gamma[k] = c.getReal(sec, "gamma", 1.4);
+    cfg.pinf[k] = eos == "MG_STIFFENED_GAS" ? c.getReal(sec, "pinf", 0.0) : 0.0;
+  }
+  cfg.precision = c.getString("hydro", "precision", "FP64") == "MIXED" ? G5_PREC_MIXED : G5_PREC_FP64;
+  return cfg;
+}
+
+// [regionN] keys of common.h:80-109
+inline g5_region
+region_from_ini(ConfigMap const & c, int i)
+{
+  const std::string s = "region" + std::to_string(i);
+  g5_region         r{};
+  const auto        ar = c.getRealVector(s, "alpha_rho", { 0.0, 0.0 });
+  if (ar.size() != 2)
+    throw std::runtime_error("alpha_rho needs two values");
+  r.alpha0 = c.getReal(s, "alpha0", 1.0);
+  r.alpha_rho[0] = ar[0];
+  r.alpha_rho[1] = ar[1];
+  r.p = c.getReal(s, "p", 1.0);
+  r.u = c.getReal(s, "u", 0.0);
+  r.v = c.getReal(s, "v", 0.0);
+  r.w = c.getReal(s, "w", 0.0);
+  return r;
+}
+
+inline void
+check(int rc, g5_ctx const * ctx)
+{
+  if (rc != G5_OK)
+    throw std::runtime_error(std::string("g5: ") + g5_last_error(ctx));
+}
+
+// ---------------------------------------------------------------------------------------------
+// State handle: the device-resident conservative state of a context (the solver's U / U2).
+// ---------------------------------------------------------------------------------------------
+struct DataArrayBlock
+{
+  g5_ctx * ctx = nullptr;
+};
+
+// scheme/GodunovImplemBase.h:59-198 surface, implemented by the HIP kernels
+template <int dim>
+class GodunovImplemHip
+{
+public:
+  explicit GodunovImplemHip(g5_config const & cfg)
+  {
+    if (cfg.dim != dim)
+      throw std::runtime_error("dimension mismatch");
+    g5_ctx * c = nullptr;
+    check(g5_create(&cfg, &c), nullptr);
+    m_ctx.reset(c);
+  }
+
+  // static factory, scheme/GodunovImplemV0.h:103-114
+  static GodunovImplemHip *
+  create(ConfigMap const & config_map)
+  {
+    return new GodunovImplemHip(config_from_ini(config_map));
+  }
+
+  // scratch is sized at creation for the fixed mesh: nothing to resize
+  void
+  resize_auxiliary_data()
+  {}
+
+  uint64_t
+  total_mem_size_in_bytes()
+  {
+    uint64_t b = 0;
+    check(g5_mem_bytes(m_ctx.get(), &b), m_ctx.get());
+    return b;
+  }
+
+  // U -> U2 is kept inside the context (U2 becomes the current state, as after the reference's
+  // deep_copy U <- U2 in godunov_unsplit)
+  void
+  do_time_step(DataArrayBlock const &, DataArrayBlock const &, double dt)
+  {
+    check(g5_do_time_step(m_ctx.get(), dt), m_ctx.get());
+  }
+
+  void
+  do_time_step(DataArrayBlock const & U, DataArrayBlock const & U2, double /*t*/, double dt)
+  {
+    do_time_step(U, U2, dt);
+  }
+
+  g5_ctx *
+  ctx() const
+  {
+    return m_ctx.get();
+  }
+
+private:
+  struct Del
+  {
+    void
+    operator()(g5_ctx * c) const
+    {
+      g5_destroy(c);
+    }
+  };
+  std::unique_ptr<g5_ctx, Del> m_ctx;
+};
+
+// SolverGodunovFiveEq: time loop around the implementation (fixed mesh, one GPU)
+template <int dim>
+class SolverGodunovFiveEq
+{
+public:
+  explicit SolverGodunovFiveEq(ConfigMap const & config_map)
+    : m_cfg(config_from_ini(config_map))
+    , m_impl(m_cfg)
+    , m_tEnd(config_map.getReal("run", "tEnd", 1.0))
+    , m_nStepmax(config_map.getInteger("run", "nStepmax", 1000))
+  {}
+
+  void
+  set_state(std::vector<double> const & U)
+  {
+    check(g5_upload_state(m_impl.ctx(), U.data(), U.size()), m_impl.ctx());
+  }
+
+  std::vector<double>
+  get_state()
+  {
+    int32_t nb, nv, nc;
+    check(g5_shape(m_impl.ctx(), &nb, &nv, &nc), m_impl.ctx());
+    std::vector<double> U(static_cast<size_t>(nb) * nv * nc);
+    check(g5_download_state(m_impl.ctx(), U.data(), U.size()), m_impl.ctx());
+    return U;
+  }
+
+  double
+  compute_dt_local()
+  {
+    double dt = 0.0;
+    check(g5_compute_dt(m_impl.ctx(), &dt), m_impl.ctx());
+    return dt;
+  }
+
+  void
+  godunov_unsplit(double dt)
+  {
+    m_impl.do_time_step(DataArrayBlock{ m_impl.ctx() }, DataArrayBlock{ m_impl.ctx() }, m_t, dt);
+  }
+
+  // one iteration: dt (clamped to tEnd), step, cell-update accounting
+  void
+  next_iteration_impl()
+  {
+    double dt = compute_dt_local();
+    if (m_t + dt > m_tEnd)
+      dt = m_tEnd - m_t;
+    godunov_unsplit(dt);
+    m_t += dt;
+    ++m_iteration;
+    m_cell_updates += num_cells();
+  }
+
+  // the whole run with the device-resident time loop; returns Mcell-updates/s
+  double
+  run()
+  {
+    const auto t0 = std::chrono::steady_clock::now();
+    int32_t    done = 0;
+    check(g5_run(m_impl.ctx(), m_nStepmax - m_iteration, &m_t, m_tEnd, &done), m_impl.ctx());
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    m_iteration += done;
+    m_cell_updates += static_cast<uint64_t>(done) * num_cells();
+    return static_cast<double>(done) * num_cells() / s / 1e6;
+  }
+
+  uint64_t
+  num_cells() const
+  {
+    uint64_t n = 1;
+    for (int d = 0; d < dim; ++d)
+      n *= static_cast<uint64_t>(m_cfg.nb[d]) * m_cfg.bsize;
+    return n;
+  }
+
+  double
+  time() const
+  {
+    return m_t;
+  }
+  int
+  iteration() const
+  {
+    return m_iteration;
+  }
+  uint64_t
+  total_num_cell_updates() const
+  {
+    return m_cell_updates;
+  }
+  g5_config const &
+  config() const
+  {
+    return m_cfg;
+  }
+
+private:
+  g5_config              m_cfg;
+  GodunovImplemHip<dim>  m_impl;
+  double                 m_tEnd;
+  int                    m_nStepmax;
+  double                 m_t = 0.0;
+  int                    m_iteration = 0;
+  uint64_t               m_cell_updates = 0;
+};
+
+} // namespace g5cpp
