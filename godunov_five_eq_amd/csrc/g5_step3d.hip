@@ -14,6 +14,8 @@
 // Threads 0..255 (waves 0-3) own the interior columns (t%16, t/16); threads 256..319 (wave 4)
 // own the 64 ring columns whose single face state the block needs and convert the 80 outer
 // columns of the 20x20 plane.
+#include <cstdlib>
+
 #include "g5_device.h"
 #include "g5_kernels.h"
 
@@ -196,14 +198,18 @@ logical_block3(StepArgs const & A)
   return (w & 7) * A.per_xcd + (w >> 3);
 }
 
-template <bool THINC, bool HANCOCK>
-__global__ void __launch_bounds__(320)
+// NTH = 320: waves 0-3 own the interior columns, wave 4 owns the ring columns and converts the
+//            outer cells (capped at 256 registers per lane by the 5-wave workgroup);
+// NTH = 256: every thread owns an interior column; threads 0..63 also own a ring column and
+//            threads 64..143 convert one outer cell (one wave per SIMD, up to 512 registers).
+template <bool THINC, bool HANCOCK, int NTH>
+__global__ void __launch_bounds__(NTH)
 k_step3d(StepArgs A)
 {
   __shared__ double sQ[7 * NT3];
   __shared__ double sA[THINC ? 3 * NT3 : 1];
   __shared__ double sE[18 * NF3];
-  __shared__ double sRed[5];
+  __shared__ double sRed[NTH / 64];
 
   const int t = threadIdx.x;
   const int blk = logical_block3(A);
@@ -217,7 +223,7 @@ k_step3d(StepArgs A)
   const bool      host_dt = A.dt_fixed > 0.0;
   if (!host_dt and C->active == 0)
   {
-    for (int c = t; c < NC; c += 320)
+    for (int c = t; c < NC; c += NTH)
       for (int v = 0; v < 9; ++v)
         A.U2[blk_off + v * NC + c] = A.U[blk_off + v * NC + c];
     return;
@@ -226,29 +232,29 @@ k_step3d(StepArgs A)
   const double dtdx = dt / S.dx;
   const double st = S.slope_type;
 
-  const bool interior = t < 256;
-  const int  i = t & 15, j = (t >> 4) & 15; // interior column (valid for t < 256)
-  const int  r = t - 256;                   // ring index (valid for t >= 256)
+  // work items of this thread
+  const bool has_int = t < 256;
+  const bool has_ring = NTH == 320 ? t >= 256 : t < 64;
+  const int  r = NTH == 320 ? t - 256 : t; // ring index when has_ring
   const int  rs = (r >> 4) & 3, rr = r & 15;
+  const int  n_out = NTH == 320 ? (t >= 256 ? (r < 16 ? 2 : 1) : 0) : (t >= 64 and t < 144 ? 1 : 0);
+  const int  i = t & 15, j = (t >> 4) & 15; // interior column (valid when has_int)
   int        rgi = 0, rgj = 0, ogi0 = 0, ogj0 = 0, ogi1 = 0, ogj1 = 0;
-  if (!interior)
-  {
+  if (has_ring)
     ring_cell3(r, rgi, rgj);
-    outer_cell3(r, ogi0, ogj0);
-    if (r < 16)
-      outer_cell3(r + 64, ogi1, ogj1);
-  }
-  const int cgi = interior ? i : rgi, cgj = interior ? j : rgj; // column owned (interior or ring)
+  if (n_out >= 1)
+    outer_cell3(NTH == 320 ? r : t - 64, ogi0, ogj0);
+  if (n_out >= 2)
+    outer_cell3(r + 64, ogi1, ogj1);
 
   double * XL = sE;
   double * YL = sE + 7 * NF3;
   double * FX = sE;
   double * FY = sE + 9 * NF3;
 
-  // per-thread plane history of the owned column
-  Prim qm, q0, qp;
-  // wave 4: outer cells of the plane being converted
-  Prim qo0, qo1;
+  // plane history of the owned interior column (q*) and ring column (r*)
+  Prim qm, q0, qp, rm, r0, rp;
+  Prim qo0, qo1; // outer cells of the plane being converted
   // interior: carried state of plane P-1
   Cons   u2prev;
   double uaprev0 = 0.0, uaprev1 = 0.0;
@@ -256,81 +262,59 @@ k_step3d(StepArgs A)
   Face   zL; // +z face state of plane P-1
   double inv_max = 0.0;
 
-  // ---- prologue: planes -2 and -1 ----
-  qm = convert_at(S, A.U, bd, cgi, cgj, -2);
-  if (!interior)
-  {
-    qo0 = convert_at(S, A.U, bd, ogi0, ogj0, -2);
-    if (r < 16)
-      qo1 = convert_at(S, A.U, bd, ogi1, ogj1, -2);
-  }
-  if (THINC)
-  {
-    double * plane = sA + aslot(-2) * NT3;
-    plane[tidx3(cgi, cgj)] = qm.a;
-    if (!interior)
+  auto convert_plane = [&](int P, Prim & qi, Prim & qr) {
+    if (has_int)
+      qi = convert_at(S, A.U, bd, i, j, P);
+    if (has_ring)
+      qr = convert_at(S, A.U, bd, rgi, rgj, P);
+    if (n_out >= 1)
+      qo0 = convert_at(S, A.U, bd, ogi0, ogj0, P);
+    if (n_out >= 2)
+      qo1 = convert_at(S, A.U, bd, ogi1, ogj1, P);
+    if (THINC)
     {
-      plane[tidx3(ogi0, ogj0)] = qo0.a;
-      if (r < 16)
+      double * plane = sA + aslot(P) * NT3;
+      if (has_int)
+        plane[tidx3(i, j)] = qi.a;
+      if (has_ring)
+        plane[tidx3(rgi, rgj)] = qr.a;
+      if (n_out >= 1)
+        plane[tidx3(ogi0, ogj0)] = qo0.a;
+      if (n_out >= 2)
         plane[tidx3(ogi1, ogj1)] = qo1.a;
     }
-  }
-  q0 = convert_at(S, A.U, bd, cgi, cgj, -1);
-  sq_store3(sQ, tidx3(cgi, cgj), q0);
-  if (!interior)
-  {
-    qo0 = convert_at(S, A.U, bd, ogi0, ogj0, -1);
-    sq_store3(sQ, tidx3(ogi0, ogj0), qo0);
-    if (r < 16)
-    {
-      qo1 = convert_at(S, A.U, bd, ogi1, ogj1, -1);
+  };
+  auto store_plane = [&](Prim const & qi, Prim const & qr) {
+    if (has_int)
+      sq_store3(sQ, tidx3(i, j), qi);
+    if (has_ring)
+      sq_store3(sQ, tidx3(rgi, rgj), qr);
+    if (n_out >= 1)
+      sq_store3(sQ, tidx3(ogi0, ogj0), qo0);
+    if (n_out >= 2)
       sq_store3(sQ, tidx3(ogi1, ogj1), qo1);
-    }
-  }
-  if (THINC)
-  {
-    double * plane = sA + aslot(-1) * NT3;
-    plane[tidx3(cgi, cgj)] = q0.a;
-    if (!interior)
-    {
-      plane[tidx3(ogi0, ogj0)] = qo0.a;
-      if (r < 16)
-        plane[tidx3(ogi1, ogj1)] = qo1.a;
-    }
-  }
+  };
+
+  // ---- prologue: planes -2 and -1 ----
+  convert_plane(-2, qm, rm);
+  convert_plane(-1, q0, r0);
+  store_plane(q0, r0);
 
   for (int P = -1; P <= 16; ++P)
   {
     const bool inner_plane = P >= 0 and P <= 15;
     // (a) convert plane P+1
-    qp = convert_at(S, A.U, bd, cgi, cgj, P + 1);
-    if (!interior)
-    {
-      qo0 = convert_at(S, A.U, bd, ogi0, ogj0, P + 1);
-      if (r < 16)
-        qo1 = convert_at(S, A.U, bd, ogi1, ogj1, P + 1);
-    }
-    if (THINC)
-    {
-      double * plane = sA + aslot(P + 1) * NT3;
-      plane[tidx3(cgi, cgj)] = qp.a;
-      if (!interior)
-      {
-        plane[tidx3(ogi0, ogj0)] = qo0.a;
-        if (r < 16)
-          plane[tidx3(ogi1, ogj1)] = qo1.a;
-      }
-    }
+    convert_plane(P + 1, qp, rp);
     __syncthreads(); // (b)
 
     // (c) predictor of plane P; z face P and the completion of plane P-1 (all column-local)
     Face fxm, fym, fring;
-    if (interior or inner_plane)
+    if (has_int)
     {
-      const Prim  qxm = sq_load3(sQ, tidx3(cgi - 1, cgj));
-      const Prim  qxp = sq_load3(sQ, tidx3(cgi + 1, cgj));
-      const Prim  qym = sq_load3(sQ, tidx3(cgi, cgj - 1));
-      const Prim  qyp = sq_load3(sQ, tidx3(cgi, cgj + 1));
+      const Prim  qxm = sq_load3(sQ, tidx3(i - 1, j));
+      const Prim  qxp = sq_load3(sQ, tidx3(i + 1, j));
+      const Prim  qym = sq_load3(sQ, tidx3(i, j - 1));
+      const Prim  qyp = sq_load3(sQ, tidx3(i, j + 1));
       const Slope sx = slopes_of(q0, qxp, qxm, st);
       const Slope sy = slopes_of(q0, qyp, qym, st);
       const Slope sz = slopes_of(q0, qp, qm, st);
@@ -340,85 +324,89 @@ k_step3d(StepArgs A)
       Face fzm, fzp;
       if (!THINC)
       {
-        if (interior)
+        fzm = face_muscl<3, 2, HANCOCK>(S, q0, sx, sy, sz, h, -0.5);
+        fzp = face_muscl<3, 2, HANCOCK>(S, q0, sx, sy, sz, h, 0.5);
+        if (inner_plane)
         {
-          fzm = face_muscl<3, 2, HANCOCK>(S, q0, sx, sy, sz, h, -0.5);
-          fzp = face_muscl<3, 2, HANCOCK>(S, q0, sx, sy, sz, h, 0.5);
-          if (inner_plane)
-          {
-            fxm = face_muscl<3, 0, HANCOCK>(S, q0, sx, sy, sz, h, -0.5);
-            fym = face_muscl<3, 1, HANCOCK>(S, q0, sx, sy, sz, h, -0.5);
-            st_store3(XL, fx3(i + 1, j), face_muscl<3, 0, HANCOCK>(S, q0, sx, sy, sz, h, 0.5));
-            st_store3(YL, fy3(i, j + 1), face_muscl<3, 1, HANCOCK>(S, q0, sx, sy, sz, h, 0.5));
-          }
-        }
-        else
-        {
-          const double o = (rs & 1) ? -0.5 : 0.5;
-          fring = rs < 2 ? face_muscl<3, 0, HANCOCK>(S, q0, sx, sy, sz, h, o)
-                         : face_muscl<3, 1, HANCOCK>(S, q0, sx, sy, sz, h, o);
+          fxm = face_muscl<3, 0, HANCOCK>(S, q0, sx, sy, sz, h, -0.5);
+          fym = face_muscl<3, 1, HANCOCK>(S, q0, sx, sy, sz, h, -0.5);
+          st_store3(XL, fx3(i + 1, j), face_muscl<3, 0, HANCOCK>(S, q0, sx, sy, sz, h, 0.5));
+          st_store3(YL, fy3(i, j + 1), face_muscl<3, 1, HANCOCK>(S, q0, sx, sy, sz, h, 0.5));
         }
       }
       else
       {
         double nx = 0.0, ny = 0.0, nz = 0.0;
         if (q0.a > S.thinc_eps and q0.a < 1.0 - S.thinc_eps)
-          youngs_at(sA, P, cgi, cgj, nx, ny, nz);
-        if (interior)
+          youngs_at(sA, P, i, j, nx, ny, nz);
+        const ThincAxis tz = thinc_axis(S, q0.a, qm.a, q0.a, qp.a, fabs(nz));
+        fzm = face_thinc<3, 2, HANCOCK>(q0, sx, sy, sz, h, tz, -1);
+        fzp = face_thinc<3, 2, HANCOCK>(q0, sx, sy, sz, h, tz, 1);
+        if (inner_plane)
         {
-          const ThincAxis tz = thinc_axis(S, q0.a, qm.a, q0.a, qp.a, fabs(nz));
-          fzm = face_thinc<3, 2, HANCOCK>(q0, sx, sy, sz, h, tz, -1);
-          fzp = face_thinc<3, 2, HANCOCK>(q0, sx, sy, sz, h, tz, 1);
-          if (inner_plane)
-          {
-            const ThincAxis tx = thinc_axis(S, q0.a, qxm.a, q0.a, qxp.a, fabs(nx));
-            const ThincAxis ty = thinc_axis(S, q0.a, qym.a, q0.a, qyp.a, fabs(ny));
-            fxm = face_thinc<3, 0, HANCOCK>(q0, sx, sy, sz, h, tx, -1);
-            fym = face_thinc<3, 1, HANCOCK>(q0, sx, sy, sz, h, ty, -1);
-            st_store3(XL, fx3(i + 1, j), face_thinc<3, 0, HANCOCK>(q0, sx, sy, sz, h, tx, 1));
-            st_store3(YL, fy3(i, j + 1), face_thinc<3, 1, HANCOCK>(q0, sx, sy, sz, h, ty, 1));
-          }
-        }
-        else
-        {
-          const int ib = (rs & 1) ? -1 : 1;
-          if (rs < 2)
-          {
-            const ThincAxis tx = thinc_axis(S, q0.a, qxm.a, q0.a, qxp.a, fabs(nx));
-            fring = face_thinc<3, 0, HANCOCK>(q0, sx, sy, sz, h, tx, ib);
-          }
-          else
-          {
-            const ThincAxis ty = thinc_axis(S, q0.a, qym.a, q0.a, qyp.a, fabs(ny));
-            fring = face_thinc<3, 1, HANCOCK>(q0, sx, sy, sz, h, ty, ib);
-          }
+          const ThincAxis tx = thinc_axis(S, q0.a, qxm.a, q0.a, qxp.a, fabs(nx));
+          const ThincAxis ty = thinc_axis(S, q0.a, qym.a, q0.a, qyp.a, fabs(ny));
+          fxm = face_thinc<3, 0, HANCOCK>(q0, sx, sy, sz, h, tx, -1);
+          fym = face_thinc<3, 1, HANCOCK>(q0, sx, sy, sz, h, ty, -1);
+          st_store3(XL, fx3(i + 1, j), face_thinc<3, 0, HANCOCK>(q0, sx, sy, sz, h, tx, 1));
+          st_store3(YL, fy3(i, j + 1), face_thinc<3, 1, HANCOCK>(q0, sx, sy, sz, h, ty, 1));
         }
       }
-      if (interior)
+      if (P >= 0)
       {
-        if (P >= 0)
+        // z face P between planes P-1 and P
+        const Flux Fz = hllc<3>(S, zL, fzm, dtdx, 2);
+        if (P >= 1)
         {
-          // z face P between planes P-1 and P
-          const Flux Fz = hllc<3>(S, zL, fzm, dtdx, 2);
-          if (P >= 1)
-          {
-            // plane P-1 gets its z update, normalisation, store and CFL
-            apply_flux_pair<3>(u2prev, zFprev, Fz, uaprev0, uaprev1);
-            fix_volume_fractions(u2prev);
-            store_cons<3, B3>(A.U2, blk_off + i + 16 * j + 256 * (P - 1), u2prev);
-            inv_max = fmax(inv_max, cell_inv_dt<3>(S, u2prev));
-          }
-          zFprev = Fz;
+          // plane P-1 gets its z update, normalisation, store and CFL
+          apply_flux_pair<3>(u2prev, zFprev, Fz, uaprev0, uaprev1);
+          fix_volume_fractions(u2prev);
+          store_cons<3, B3>(A.U2, blk_off + i + 16 * j + 256 * (P - 1), u2prev);
+          inv_max = fmax(inv_max, cell_inv_dt<3>(S, u2prev));
         }
-        zL = fzp;
+        zFprev = Fz;
+      }
+      zL = fzp;
+    }
+    if (has_ring and inner_plane)
+    {
+      const Prim  qxm = sq_load3(sQ, tidx3(rgi - 1, rgj));
+      const Prim  qxp = sq_load3(sQ, tidx3(rgi + 1, rgj));
+      const Prim  qym = sq_load3(sQ, tidx3(rgi, rgj - 1));
+      const Prim  qyp = sq_load3(sQ, tidx3(rgi, rgj + 1));
+      const Slope sx = slopes_of(r0, qxp, qxm, st);
+      const Slope sy = slopes_of(r0, qyp, qym, st);
+      const Slope sz = slopes_of(r0, rp, rm, st);
+      Hancock     h = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+      if (HANCOCK)
+        h = hancock_terms<3>(S, r0, sx, sy, sz, dtdx);
+      if (!THINC)
+      {
+        const double o = (rs & 1) ? -0.5 : 0.5;
+        fring = rs < 2 ? face_muscl<3, 0, HANCOCK>(S, r0, sx, sy, sz, h, o)
+                       : face_muscl<3, 1, HANCOCK>(S, r0, sx, sy, sz, h, o);
       }
       else
       {
-        if (rs == 0)
-          st_store3(XL, fx3(0, rr), fring);
-        else if (rs == 2)
-          st_store3(YL, fy3(rr, 0), fring);
+        double nx = 0.0, ny = 0.0, nz = 0.0;
+        if (r0.a > S.thinc_eps and r0.a < 1.0 - S.thinc_eps)
+          youngs_at(sA, P, rgi, rgj, nx, ny, nz);
+        const int ib = (rs & 1) ? -1 : 1;
+        if (rs < 2)
+        {
+          const ThincAxis tx = thinc_axis(S, r0.a, qxm.a, r0.a, qxp.a, fabs(nx));
+          fring = face_thinc<3, 0, HANCOCK>(r0, sx, sy, sz, h, tx, ib);
+        }
+        else
+        {
+          const ThincAxis ty = thinc_axis(S, r0.a, qym.a, r0.a, qyp.a, fabs(ny));
+          fring = face_thinc<3, 1, HANCOCK>(r0, sx, sy, sz, h, ty, ib);
+        }
       }
+      if (rs == 0)
+        st_store3(XL, fx3(0, rr), fring);
+      else if (rs == 2)
+        st_store3(YL, fy3(rr, 0), fring);
     }
     __syncthreads(); // (d)
 
@@ -426,12 +414,12 @@ k_step3d(StepArgs A)
     Flux Fx, Fy, Fr;
     if (inner_plane)
     {
-      if (interior)
+      if (has_int)
       {
         Fx = hllc<3>(S, st_load3(XL, fx3(i, j)), fxm, dtdx, 0);
         Fy = hllc<3>(S, st_load3(YL, fy3(i, j)), fym, dtdx, 1);
       }
-      else if (rs & 1)
+      if (has_ring and (rs & 1))
       {
         const int  ax = rs == 1 ? 0 : 1;
         const Face L = ax == 0 ? st_load3(XL, fx3(16, rr)) : st_load3(YL, fy3(rr, 16));
@@ -443,12 +431,12 @@ k_step3d(StepArgs A)
     // (g) flux exchange, next plane into LDS
     if (inner_plane)
     {
-      if (interior)
+      if (has_int)
       {
         fl_store3(FX, fx3(i, j), Fx);
         fl_store3(FY, fy3(i, j), Fy);
       }
-      else if (rs & 1)
+      if (has_ring and (rs & 1))
       {
         if (rs == 1)
           fl_store3(FX, fx3(16, rr), Fr);
@@ -456,17 +444,11 @@ k_step3d(StepArgs A)
           fl_store3(FY, fy3(rr, 16), Fr);
       }
     }
-    sq_store3(sQ, tidx3(cgi, cgj), qp);
-    if (!interior)
-    {
-      sq_store3(sQ, tidx3(ogi0, ogj0), qo0);
-      if (r < 16)
-        sq_store3(sQ, tidx3(ogi1, ogj1), qo1);
-    }
+    store_plane(qp, rp);
     __syncthreads(); // (h)
 
     // (i) x then y update of plane P
-    if (interior and inner_plane)
+    if (has_int and inner_plane)
     {
       const Cons uc = load_cons<3, B3>(A.U, blk_off + i + 16 * j + 256 * P, -1);
       u2prev = uc;
@@ -477,6 +459,8 @@ k_step3d(StepArgs A)
     }
     qm = q0;
     q0 = qp;
+    rm = r0;
+    r0 = rp;
   }
 
   // CFL of the new state
@@ -486,29 +470,46 @@ k_step3d(StepArgs A)
   __syncthreads();
   if (t == 0)
   {
-    const double m = fmax(fmax(fmax(sRed[0], sRed[1]), fmax(sRed[2], sRed[3])), sRed[4]);
+    double m = sRed[0];
+    for (int w = 1; w < NTH / 64; ++w)
+      m = fmax(m, sRed[w]);
     atomicMax(&C->invdt_bits, static_cast<unsigned long long>(__double_as_longlong(m)));
   }
+}
+
+// block shape of the 3-D step: 256 (default) or 320 threads, G5_STEP3D_THREADS overrides
+static int
+step3d_threads()
+{
+  static int n = [] {
+    const char * e = getenv("G5_STEP3D_THREADS");
+    return (e and atoi(e) == 320) ? 320 : 256;
+  }();
+  return n;
+}
+
+template <int NTH>
+static void
+launch3(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
+{
+  const dim3 grid(8 * A.per_xcd), block(NTH);
+  if (thinc and hancock)
+    hipLaunchKernelGGL((k_step3d<true, true, NTH>), grid, block, 0, stream, A);
+  else if (thinc)
+    hipLaunchKernelGGL((k_step3d<true, false, NTH>), grid, block, 0, stream, A);
+  else if (hancock)
+    hipLaunchKernelGGL((k_step3d<false, true, NTH>), grid, block, 0, stream, A);
+  else
+    hipLaunchKernelGGL((k_step3d<false, false, NTH>), grid, block, 0, stream, A);
 }
 
 void
 launch_step3d(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
 {
-  const dim3 grid(8 * A.per_xcd), block(320);
-  if (thinc)
-  {
-    if (hancock)
-      hipLaunchKernelGGL((k_step3d<true, true>), grid, block, 0, stream, A);
-    else
-      hipLaunchKernelGGL((k_step3d<true, false>), grid, block, 0, stream, A);
-  }
+  if (step3d_threads() == 320)
+    launch3<320>(A, thinc, hancock, stream);
   else
-  {
-    if (hancock)
-      hipLaunchKernelGGL((k_step3d<false, true>), grid, block, 0, stream, A);
-    else
-      hipLaunchKernelGGL((k_step3d<false, false>), grid, block, 0, stream, A);
-  }
+    launch3<256>(A, thinc, hancock, stream);
 }
 
 } // namespace g5
