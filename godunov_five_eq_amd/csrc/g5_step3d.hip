@@ -101,28 +101,31 @@ sq_store3(double * sQ, int idx, Prim const & p)
   sQ[5 * NT3 + idx] = p.v;
   sQ[6 * NT3 + idx] = p.w;
 }
+// face states: 7 doubles at slot k of a [7][N] array (N = 272 faces, or 16 ring faces)
+template <int N = NF3>
 G5D void
 st_store3(double * A, int k, Face const & f)
 {
-  A[0 * NF3 + k] = f.r0;
-  A[1 * NF3 + k] = f.r1;
-  A[2 * NF3 + k] = f.a;
-  A[3 * NF3 + k] = f.p;
-  A[4 * NF3 + k] = f.u;
-  A[5 * NF3 + k] = f.v;
-  A[6 * NF3 + k] = f.w;
+  A[0 * N + k] = f.r0;
+  A[1 * N + k] = f.r1;
+  A[2 * N + k] = f.a;
+  A[3 * N + k] = f.p;
+  A[4 * N + k] = f.u;
+  A[5 * N + k] = f.v;
+  A[6 * N + k] = f.w;
 }
+template <int N = NF3>
 G5D Face
 st_load3(double const * A, int k)
 {
   Face f;
-  f.r0 = A[0 * NF3 + k];
-  f.r1 = A[1 * NF3 + k];
-  f.a = A[2 * NF3 + k];
-  f.p = A[3 * NF3 + k];
-  f.u = A[4 * NF3 + k];
-  f.v = A[5 * NF3 + k];
-  f.w = A[6 * NF3 + k];
+  f.r0 = A[0 * N + k];
+  f.r1 = A[1 * N + k];
+  f.a = A[2 * N + k];
+  f.p = A[3 * N + k];
+  f.u = A[4 * N + k];
+  f.v = A[5 * N + k];
+  f.w = A[6 * N + k];
   return f;
 }
 G5D void
@@ -477,13 +480,306 @@ k_step3d(StepArgs A)
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// v3 layout: one workgroup of 4 wavefronts per block and per CU (the register demand of the
+// FP64 z-march allows one wavefront per SIMD), so the whole 160 KB of LDS is available:
+//   - a ring of 3 primitive planes (P-1, P, P+1) for every stencil, THINC alpha included;
+//   - separate face-state (XL/YL, ring XR/YR) and flux (FX/FY) buffers, 3 barriers per plane;
+//   - the raw state of the owned column is carried from its conversion to its update, and the
+//     global loads of plane P+2 are issued one phase ahead of their use.
+// Extra work is spread so that every wavefront has one extra item per plane: wave 0 converts the
+// 64 ring cells, wave 1 computes their face states, wave 2 solves the 32 ring Riemann problems
+// and converts 16 outer cells, wave 3 converts 64 outer cells.  With ZSPLIT the z Riemann solve
+// and the completion of plane P-1 run before the x/y exchange on waves 0, 2, 3 and after it on
+// wave 1, which carries the ring predictor.
+// ---------------------------------------------------------------------------------------------
+namespace
+{
+constexpr int QPL = 7 * NT3; // doubles per primitive plane
+
+G5D Prim
+q3_load(double const * sQ, int slot, int idx)
+{
+  return sq_load3(sQ + slot * QPL, idx);
+}
+
+G5D void
+youngs_ring(double const * sQ, int P, int gi, int gj, double & nx, double & ny, double & nz)
+{
+  double a27[27];
+#pragma unroll
+  for (int dk = -1; dk <= 1; ++dk)
+  {
+    double const * plane = sQ + aslot(P + dk) * QPL + 2 * NT3; // alpha of plane P+dk
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int di = -1; di <= 1; ++di)
+        a27[(di + 1) + 3 * (dj + 1) + 9 * (dk + 1)] = plane[tidx3(gi + di, gj + dj)];
+  }
+  youngs3(a27, nx, ny, nz);
+}
+
+// slopes, Hancock terms and the requested face states of tile cell (gi,gj) in plane P
+template <bool THINC, bool HANCOCK>
+struct Pred3
+{
+  Prim    q;
+  Slope   sx, sy, sz;
+  Hancock h;
+  double  nx, ny, nz;
+  double  axm, axp, aym, ayp, azm, azp;
+
+  G5D void
+  compute(SolveDesc const & S, double const * sQ, int P, int gi, int gj, double dtdx)
+  {
+    const int  sp = aslot(P);
+    const int  c = tidx3(gi, gj);
+    q = q3_load(sQ, sp, c);
+    const Prim qxm = q3_load(sQ, sp, tidx3(gi - 1, gj));
+    const Prim qxp = q3_load(sQ, sp, tidx3(gi + 1, gj));
+    const Prim qym = q3_load(sQ, sp, tidx3(gi, gj - 1));
+    const Prim qyp = q3_load(sQ, sp, tidx3(gi, gj + 1));
+    const Prim qzm = q3_load(sQ, aslot(P - 1), c);
+    const Prim qzp = q3_load(sQ, aslot(P + 1), c);
+    sx = slopes_of(q, qxp, qxm, S.slope_type);
+    sy = slopes_of(q, qyp, qym, S.slope_type);
+    sz = slopes_of(q, qzp, qzm, S.slope_type);
+    axm = qxm.a;
+    axp = qxp.a;
+    aym = qym.a;
+    ayp = qyp.a;
+    azm = qzm.a;
+    azp = qzp.a;
+    h = Hancock{ 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+    if (HANCOCK)
+      h = hancock_terms<3>(S, q, sx, sy, sz, dtdx);
+    nx = ny = nz = 0.0;
+    if (THINC and q.a > S.thinc_eps and q.a < 1.0 - S.thinc_eps)
+      youngs_ring(sQ, P, gi, gj, nx, ny, nz);
+  }
+
+  template <int AX>
+  G5D Face
+  face(SolveDesc const & S, int ib) const
+  {
+    if (!THINC)
+      return face_muscl<3, AX, HANCOCK>(S, q, sx, sy, sz, h, 0.5 * static_cast<double>(ib));
+    const double am = AX == 0 ? axm : (AX == 1 ? aym : azm);
+    const double ap = AX == 0 ? axp : (AX == 1 ? ayp : azp);
+    const double nd = AX == 0 ? nx : (AX == 1 ? ny : nz);
+    const ThincAxis ta = thinc_axis(S, q.a, am, q.a, ap, fabs(nd));
+    return face_thinc<3, AX, HANCOCK>(q, sx, sy, sz, h, ta, ib);
+  }
+};
+} // namespace
+
+template <bool THINC, bool HANCOCK, bool ZSPLIT>
+__global__ void __launch_bounds__(256)
+k_step3d_v3(StepArgs A)
+{
+  __shared__ double sQ[3 * QPL];
+  __shared__ double sXL[7 * NF3];
+  __shared__ double sYL[7 * NF3];
+  __shared__ double sXR[7 * 16];
+  __shared__ double sYR[7 * 16];
+  __shared__ double sFX[9 * NF3];
+  __shared__ double sFY[9 * NF3];
+  __shared__ double sRed[4];
+
+  const int t = threadIdx.x;
+  const int blk = logical_block3(A);
+  if (blk >= A.nblocks)
+    return;
+  const BlockDesc bd = A.blocks[blk];
+  const SolveDesc S = A.solves[bd.solve];
+  SolveCtl *      C = A.ctl + bd.solve;
+  constexpr int   NC = 4096;
+  const int64_t   blk_off = static_cast<int64_t>(blk) * 9 * NC;
+  const bool      host_dt = A.dt_fixed > 0.0;
+  if (!host_dt and C->active == 0)
+  {
+    for (int c = t; c < NC; c += 256)
+      for (int v = 0; v < 9; ++v)
+        A.U2[blk_off + v * NC + c] = A.U[blk_off + v * NC + c];
+    return;
+  }
+  const double dt = host_dt ? A.dt_fixed : C->dt;
+  const double dtdx = dt / S.dx;
+
+  const int  wave = t >> 6, lane = t & 63;
+  const int  i = t & 15, j = t >> 4;
+  // extra conversion item
+  const bool has_x = wave == 0 or wave == 3 or (wave == 2 and lane < 16);
+  int        xgi = 0, xgj = 0;
+  if (wave == 0)
+    ring_cell3(lane, xgi, xgj);
+  else if (wave == 3)
+    outer_cell3(lane, xgi, xgj);
+  else if (wave == 2 and lane < 16)
+    outer_cell3(64 + lane, xgi, xgj);
+  // ring predictor (wave 1) and ring Riemann problems (wave 2)
+  const int rs = (lane >> 4) & 3, rr = lane & 15;
+  int       rgi = 0, rgj = 0;
+  ring_cell3(lane, rgi, rgj);
+  const bool z_early = !ZSPLIT or wave != 1;
+
+  auto fetch = [&](int gi, int gj, int gk) {
+    int           neg;
+    const int64_t off = resolve_cell<3, B3>(S, bd.bx, bd.by, bd.bz, gi, gj, gk, neg);
+    return load_cons<3, B3>(A.U, off, neg);
+  };
+  auto put = [&](int P, int gi, int gj, Cons const & u) {
+    Prim q;
+    cons_to_prim<3>(S, u, q);
+    sq_store3(sQ + aslot(P) * QPL, tidx3(gi, gj), q);
+  };
+
+  // carried state
+  Cons   pf_i, pf_x;   // prefetched raw state of the next plane to convert
+  Cons   uraw_cur;     // raw state of the owned column in plane P
+  Cons   uraw_next;    // ... in plane P+1
+  Cons   u2prev;       // plane P-1 after its x and y updates
+  double uaprev0 = 0.0, uaprev1 = 0.0;
+  Flux   zFprev;       // z flux on face P-1
+  Face   zL;           // +z face state of plane P-1
+  Face   zLnext;       // +z face state of plane P (wave 1 with ZSPLIT, until its z solve)
+  double inv_max = 0.0;
+
+  // prologue: planes -2 and -1 into LDS, plane 0 prefetched
+  {
+    const Cons a = fetch(i, j, -2);
+    put(-2, i, j, a);
+    if (has_x)
+      put(-2, xgi, xgj, fetch(xgi, xgj, -2));
+    const Cons b = fetch(i, j, -1);
+    put(-1, i, j, b);
+    if (has_x)
+      put(-1, xgi, xgj, fetch(xgi, xgj, -1));
+    pf_i = fetch(i, j, 0);
+    if (has_x)
+      pf_x = fetch(xgi, xgj, 0);
+  }
+
+  auto z_solve = [&](int P, Face const & fzm) {
+    // z face P between planes P-1 and P, then plane P-1 is complete
+    const Flux Fz = hllc<3>(S, zL, fzm, dtdx, 2);
+    if (P >= 1)
+    {
+      apply_flux_pair<3>(u2prev, zFprev, Fz, uaprev0, uaprev1);
+      fix_volume_fractions(u2prev);
+      store_cons<3, B3>(A.U2, blk_off + i + 16 * j + 256 * (P - 1), u2prev);
+      inv_max = fmax(inv_max, cell_inv_dt<3>(S, u2prev));
+    }
+    zFprev = Fz;
+  };
+
+  for (int P = -1; P <= 16; ++P)
+  {
+    const bool inner_plane = P >= 0 and P <= 15;
+    // (a) plane P+1 from the prefetched raw state
+    put(P + 1, i, j, pf_i);
+    uraw_next = pf_i;
+    if (has_x)
+      put(P + 1, xgi, xgj, pf_x);
+    __syncthreads(); // (b)
+
+    // (c) prefetch plane P+2, predictor of plane P
+    if (P + 2 <= 17)
+    {
+      pf_i = fetch(i, j, P + 2);
+      if (has_x)
+        pf_x = fetch(xgi, xgj, P + 2);
+    }
+    Face fxm, fym, fzm;
+    {
+      Pred3<THINC, HANCOCK> pr;
+      pr.compute(S, sQ, P, i, j, dtdx);
+      fzm = pr.template face<2>(S, -1);
+      const Face fzp = pr.template face<2>(S, 1);
+      if (inner_plane)
+      {
+        fxm = pr.template face<0>(S, -1);
+        fym = pr.template face<1>(S, -1);
+        st_store3(sXL, fx3(i + 1, j), pr.template face<0>(S, 1));
+        st_store3(sYL, fy3(i, j + 1), pr.template face<1>(S, 1));
+      }
+      if (P >= 0 and z_early)
+        z_solve(P, fzm);
+      if (z_early or P < 0)
+        zL = fzp;
+      else
+        zLnext = fzp; // wave 1 keeps both z faces until after the exchange
+    }
+    if (wave == 1 and inner_plane)
+    {
+      Pred3<THINC, HANCOCK> pr;
+      pr.compute(S, sQ, P, rgi, rgj, dtdx);
+      if (rs == 0)
+        st_store3(sXL, fx3(0, rr), pr.template face<0>(S, 1));
+      else if (rs == 1)
+        st_store3<16>(sXR, rr, pr.template face<0>(S, -1));
+      else if (rs == 2)
+        st_store3(sYL, fy3(rr, 0), pr.template face<1>(S, 1));
+      else
+        st_store3<16>(sYR, rr, pr.template face<1>(S, -1));
+    }
+    __syncthreads(); // (d)
+
+    // (e) Riemann problems on the x and y faces of plane P
+    if (inner_plane)
+    {
+      fl_store3(sFX, fx3(i, j), hllc<3>(S, st_load3(sXL, fx3(i, j)), fxm, dtdx, 0));
+      fl_store3(sFY, fy3(i, j), hllc<3>(S, st_load3(sYL, fy3(i, j)), fym, dtdx, 1));
+      if (wave == 2 and (rs & 1))
+      {
+        if (rs == 1)
+          fl_store3(sFX, fx3(16, rr), hllc<3>(S, st_load3(sXL, fx3(16, rr)), st_load3<16>(sXR, rr), dtdx, 0));
+        else
+          fl_store3(sFY, fy3(rr, 16), hllc<3>(S, st_load3(sYL, fy3(rr, 16)), st_load3<16>(sYR, rr), dtdx, 1));
+      }
+    }
+    if (!z_early)
+    {
+      if (P >= 0)
+      {
+        z_solve(P, fzm);
+        zL = zLnext;
+      }
+    }
+    __syncthreads(); // (f)
+
+    // (i) x then y update of plane P
+    if (inner_plane)
+    {
+      u2prev = uraw_cur;
+      apply_flux_pair<3>(u2prev, fl_load3(sFX, fx3(i, j)), fl_load3(sFX, fx3(i + 1, j)), uraw_cur.a0, uraw_cur.a1);
+      apply_flux_pair<3>(u2prev, fl_load3(sFY, fy3(i, j)), fl_load3(sFY, fy3(i, j + 1)), uraw_cur.a0, uraw_cur.a1);
+      uaprev0 = uraw_cur.a0;
+      uaprev1 = uraw_cur.a1;
+    }
+    uraw_cur = uraw_next;
+  }
+
+  const double inv = wave_max(inv_max);
+  if (lane == 0)
+    sRed[wave] = inv;
+  __syncthreads();
+  if (t == 0)
+  {
+    const double m = fmax(fmax(sRed[0], sRed[1]), fmax(sRed[2], sRed[3]));
+    atomicMax(&C->invdt_bits, static_cast<unsigned long long>(__double_as_longlong(m)));
+  }
+}
+
 // block shape of the 3-D step: 256 (default) or 320 threads, G5_STEP3D_THREADS overrides
 static int
-step3d_threads()
+step3d_variant()
 {
+  // 0: v3 layout with the split z solve (default), 1: v3 without split, 256/320: older layouts
   static int n = [] {
-    const char * e = getenv("G5_STEP3D_THREADS");
-    return (e and atoi(e) == 320) ? 320 : 256;
+    const char * e = getenv("G5_STEP3D_VARIANT");
+    return e ? atoi(e) : 0;
   }();
   return n;
 }
@@ -503,13 +799,38 @@ launch3(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
     hipLaunchKernelGGL((k_step3d<false, false, NTH>), grid, block, 0, stream, A);
 }
 
+template <bool ZSPLIT>
+static void
+launch3v3(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
+{
+  const dim3 grid(8 * A.per_xcd), block(256);
+  if (thinc and hancock)
+    hipLaunchKernelGGL((k_step3d_v3<true, true, ZSPLIT>), grid, block, 0, stream, A);
+  else if (thinc)
+    hipLaunchKernelGGL((k_step3d_v3<true, false, ZSPLIT>), grid, block, 0, stream, A);
+  else if (hancock)
+    hipLaunchKernelGGL((k_step3d_v3<false, true, ZSPLIT>), grid, block, 0, stream, A);
+  else
+    hipLaunchKernelGGL((k_step3d_v3<false, false, ZSPLIT>), grid, block, 0, stream, A);
+}
+
 void
 launch_step3d(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
 {
-  if (step3d_threads() == 320)
-    launch3<320>(A, thinc, hancock, stream);
-  else
-    launch3<256>(A, thinc, hancock, stream);
+  switch (step3d_variant())
+  {
+    case 320:
+      launch3<320>(A, thinc, hancock, stream);
+      break;
+    case 256:
+      launch3<256>(A, thinc, hancock, stream);
+      break;
+    case 1:
+      launch3v3<false>(A, thinc, hancock, stream);
+      break;
+    default:
+      launch3v3<true>(A, thinc, hancock, stream);
+  }
 }
 
 } // namespace g5
