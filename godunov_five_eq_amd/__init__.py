@@ -88,6 +88,21 @@ def lib():
             "g5_last_run_ms": (ctypes.c_int, [vp, P(d)]),
             "g5_time_step_kernel": (ctypes.c_int, [vp, i32, d, P(d)]),
             "g5_init_shock_bubble": (ctypes.c_int, [P(G5Config), P(G5Region), d, P(d), d, P(d), sz]),
+            "g5_pool_create": (ctypes.c_int, [i32, i32, i32, ctypes.c_uint64, P(vp)]),
+            "g5_pool_destroy": (ctypes.c_int, [vp]),
+            "g5_pool_last_error": (ctypes.c_char_p, [vp]),
+            "g5_pool_admit": (ctypes.c_int, [vp, P(G5Config), d, i32, P(i32)]),
+            "g5_pool_release": (ctypes.c_int, [vp, i32]),
+            "g5_pool_upload": (ctypes.c_int, [vp, i32, P(d), sz]),
+            "g5_pool_download": (ctypes.c_int, [vp, i32, P(d), sz]),
+            "g5_pool_init_shock_bubble": (ctypes.c_int, [vp, i32, P(G5Region), i32, d, P(d), P(d)]),
+            "g5_pool_init_triple_point": (ctypes.c_int, [vp, i32, P(G5Region), d, d]),
+            "g5_pool_start": (ctypes.c_int, [vp, P(i32), i32]),
+            "g5_pool_step": (ctypes.c_int, [vp, i32]),
+            "g5_pool_status": (ctypes.c_int, [vp, i32, P(d), P(i32), P(i32)]),
+            "g5_pool_finished": (ctypes.c_int, [vp, P(i32), i32, P(i32)]),
+            "g5_pool_usage": (ctypes.c_int, [vp, P(ctypes.c_int64), P(ctypes.c_int64), P(i32)]),
+            "g5_pool_last_step_ms": (ctypes.c_int, [vp, P(d)]),
             "g5_rankine_hugoniot": (ctypes.c_int, [d, d, d, d, d, d, P(d), P(d), P(d), P(d)]),
         }
         for name, (res, args) in sig.items():
@@ -105,6 +120,10 @@ def exported_symbols():
         "g5_upload_state", "g5_download_state", "g5_state_device_ptr", "g5_compute_dt",
         "g5_fill_outside", "g5_outside_coords", "g5_do_time_step", "g5_run", "g5_synchronize", "g5_last_run_ms",
         "g5_time_step_kernel", "g5_init_shock_bubble", "g5_rankine_hugoniot",
+        "g5_pool_create", "g5_pool_destroy", "g5_pool_last_error", "g5_pool_admit", "g5_pool_release",
+        "g5_pool_upload", "g5_pool_download", "g5_pool_init_shock_bubble", "g5_pool_init_triple_point",
+        "g5_pool_start", "g5_pool_step", "g5_pool_status", "g5_pool_finished", "g5_pool_usage",
+        "g5_pool_last_step_ms",
     ]
 
 
@@ -276,4 +295,106 @@ class Context:
     def time_step_kernel(self, nsteps, dt):
         ms = ctypes.c_double()
         self._check(lib().g5_time_step_kernel(self._h, nsteps, dt, ctypes.byref(ms)))
+        return ms.value
+
+
+class Pool:
+    """Many independent solves of one dimension stepped together (g5_pool_* of the C ABI)."""
+
+    def __init__(self, dim, thinc=False, precision=PREC_FP64, budget_bytes=8 << 30):
+        self.dim = dim
+        self._h = ctypes.c_void_p()
+        rc = lib().g5_pool_create(dim, 1 if thinc else 0, precision, int(budget_bytes), ctypes.byref(self._h))
+        if rc != G5_OK:
+            raise G5Error(rc, lib().g5_pool_last_error(None).decode())
+        self._cfg = {}
+
+    def _check(self, rc):
+        if rc != G5_OK:
+            raise G5Error(rc, lib().g5_pool_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().g5_pool_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def admit(self, cfg, t_end=1e300, max_steps=2 ** 31 - 1):
+        sid = ctypes.c_int32()
+        self._check(lib().g5_pool_admit(self._h, ctypes.byref(cfg), t_end, max_steps, ctypes.byref(sid)))
+        self._cfg[sid.value] = cfg
+        return sid.value
+
+    def try_admit(self, cfg, t_end=1e300, max_steps=2 ** 31 - 1):
+        """admit, or None when the HBM budget has no room"""
+        sid = ctypes.c_int32()
+        rc = lib().g5_pool_admit(self._h, ctypes.byref(cfg), t_end, max_steps, ctypes.byref(sid))
+        if rc == G5_ERR_NOMEM:
+            return None
+        self._check(rc)
+        self._cfg[sid.value] = cfg
+        return sid.value
+
+    def release(self, sid):
+        self._check(lib().g5_pool_release(self._h, sid))
+        self._cfg.pop(sid, None)
+
+    def upload(self, sid, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).ravel()
+        self._check(lib().g5_pool_upload(self._h, sid, _dptr(U), U.size))
+
+    def download(self, sid):
+        U = np.empty(state_size(self._cfg[sid]), dtype=np.float64)
+        self._check(lib().g5_pool_download(self._h, sid, _dptr(U), U.size))
+        return U
+
+    def init_shock_bubble(self, sid, regions, x_front, center, size, shape=0):
+        regs = (G5Region * 3)(*regions)
+        c3 = (ctypes.c_double * 3)(*(list(center) + [0.0] * (3 - len(center))))
+        sz = list(size) if hasattr(size, "__len__") else [size]
+        s3 = (ctypes.c_double * 3)(*(sz + [sz[-1]] * (3 - len(sz))))
+        self._check(lib().g5_pool_init_shock_bubble(self._h, sid, regs, shape, x_front, c3, s3))
+
+    def init_triple_point(self, sid, regions, xd, yd):
+        regs = (G5Region * 3)(*regions)
+        self._check(lib().g5_pool_init_triple_point(self._h, sid, regs, xd, yd))
+
+    def start(self, sids):
+        arr = (ctypes.c_int32 * len(sids))(*sids)
+        self._check(lib().g5_pool_start(self._h, arr, len(sids)))
+
+    def step(self, nsteps=1):
+        self._check(lib().g5_pool_step(self._h, nsteps))
+
+    def status(self, sid):
+        t, n, a = ctypes.c_double(), ctypes.c_int32(), ctypes.c_int32()
+        self._check(lib().g5_pool_status(self._h, sid, ctypes.byref(t), ctypes.byref(n), ctypes.byref(a)))
+        return t.value, n.value, bool(a.value)
+
+    def finished(self):
+        n = ctypes.c_int32()
+        self._check(lib().g5_pool_finished(self._h, None, 0, ctypes.byref(n)))
+        ids = (ctypes.c_int32 * max(1, n.value))()
+        self._check(lib().g5_pool_finished(self._h, ids, n.value, ctypes.byref(n)))
+        return list(ids[: n.value])
+
+    def usage(self):
+        u, c, l = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        self._check(lib().g5_pool_usage(self._h, ctypes.byref(u), ctypes.byref(c), ctypes.byref(l)))
+        return u.value, c.value, l.value
+
+    def last_step_ms(self):
+        ms = ctypes.c_double()
+        self._check(lib().g5_pool_last_step_ms(self._h, ctypes.byref(ms)))
         return ms.value

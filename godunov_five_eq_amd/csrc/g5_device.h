@@ -641,8 +641,8 @@ cell_inv_dt(SolveDesc const & S, Cons const & u)
 
 // ---------------------------------------------------------------------------------------------
 // ghost/border resolution.  Block N (brick coordinates, already wrapped in periodic directions,
-// possibly one step outside the brick) and in-block cell c map to (source block, in-block cell,
-// negated momentum axis): same-level neighbour lookup of StencilHelper plus the outside-block fill
+// possibly one step outside the brick) and in-block cell c map to (offset of the source cell in the
+// state buffer through the solve's slot table, negated momentum axis): same-level neighbour lookup of StencilHelper plus the outside-block fill
 // of border_conditions/FillOutside.cpp:69-170 -- the inside block is the outside block moved back
 // along its outside normal, the faces are visited in order xmin..zmax and the last one that
 // applies defines the copy (ZERO_GRADIENT: edge cell; WALL: mirrored cell, normal momentum
@@ -650,7 +650,8 @@ cell_inv_dt(SolveDesc const & S, Cons const & u)
 // ---------------------------------------------------------------------------------------------
 template <int DIM, int B>
 G5D int64_t
-resolve_block_cell(SolveDesc const & S, int const (&N)[3], int const (&c)[3], int & neg)
+resolve_block_cell(SolveDesc const & S, int32_t const * __restrict__ slots, int const (&N)[3], int const (&c)[3],
+                   int & neg)
 {
   int  I[3], n[3];
   bool outside = false;
@@ -692,7 +693,7 @@ resolve_block_cell(SolveDesc const & S, int const (&N)[3], int const (&c)[3], in
       }
     }
   }
-  const int             blk = S.base + I[0] + S.nb[0] * (I[1] + S.nb[1] * I[2]);
+  const int             blk = slots[S.base + I[0] + S.nb[0] * (I[1] + S.nb[1] * I[2])];
   const int             cell = cc[0] + B * (cc[1] + (DIM == 3 ? B * cc[2] : 0));
   constexpr int64_t     NV = DIM == 2 ? 8 : 9;
   constexpr int64_t     NC = DIM == 2 ? B * B : B * B * B;
@@ -702,7 +703,8 @@ resolve_block_cell(SolveDesc const & S, int const (&N)[3], int const (&c)[3], in
 // the cell at tile coordinates g (in [-2, B+1]^d) of owned block (bx,by,bz)
 template <int DIM, int B>
 G5D int64_t
-resolve_cell(SolveDesc const & S, int bx, int by, int bz, int gi, int gj, int gk, int & neg)
+resolve_cell(SolveDesc const & S, int32_t const * __restrict__ slots, int bx, int by, int bz, int gi, int gj, int gk,
+             int & neg)
 {
   const int g[3] = { gi, gj, gk };
   const int b3[3] = { bx, by, bz };
@@ -723,7 +725,7 @@ resolve_cell(SolveDesc const & S, int bx, int by, int bz, int gi, int gj, int gk
       Nd = Nd < 0 ? Nd + S.nb[d] : (Nd >= S.nb[d] ? Nd - S.nb[d] : Nd);
     N[d] = Nd;
   }
-  return resolve_block_cell<DIM, B>(S, N, c, neg);
+  return resolve_block_cell<DIM, B>(S, slots, N, c, neg);
 }
 
 // load the conservative state of one cell: base = offset of (block, var 0, cell)

@@ -12,24 +12,36 @@ namespace g5
 void launch_step2d(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream);
 void launch_step3d(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream);
 
-// max over owned cells of the CFL rate of U (ComputeDtFiveEqFunctor) into ctl[solve].invdt_bits
-void launch_inv_dt(int dim, double const * U, BlockDesc const * blocks, SolveDesc const * solves,
+// max over the listed blocks of the CFL rate of the current state into ctl[solve].invdt_bits
+void launch_inv_dt(int dim, double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves,
                    SolveCtl * ctl, int nblocks, hipStream_t stream);
 
-// dt of the first step from invdt_bits (clamped to tend); marks solves active
-void launch_start(SolveCtl * ctl, SolveDesc const * solves, int nsolves, hipStream_t stream);
+// dt of the first step of solves ids[0..n) (all when ids is null) from invdt_bits; marks active
+void launch_start(SolveCtl * ctl, SolveDesc const * solves, int32_t const * ids, int n, hipStream_t stream);
 
-// after a step: t += dt, steps++, next dt from invdt_bits, invdt_bits = 0
+// after a device-dt step: t += dt, steps++, parity flip, next dt, invdt_bits = 0
 void launch_finalize(SolveCtl * ctl, SolveDesc const * solves, int nsolves, hipStream_t stream);
+
+// after a host-dt step of every solve: parity flip
+void launch_flip(SolveCtl * ctl, int nsolves, hipStream_t stream);
 
 // zero the invdt accumulators
 void launch_reset_invdt(SolveCtl * ctl, int nsolves, hipStream_t stream);
 
-// gather the outside ring (FillOutsideCellFunctor) into ring[n_out][nvar][cells]
-void launch_fill_outside(int dim, double const * U, SolveDesc const * solve, int32_t const * coords,
-                         int n_out, double * ring, hipStream_t stream);
+// copy a solve's blocks between their slots and a contiguous array (to_slots: array -> slots)
+void launch_gather(double * buf0, double * buf1, SolveCtl const * ctl, int solve, int32_t const * slots, int base,
+                   int nblk, int block_doubles, double * out, bool to_slots, hipStream_t stream);
+
+// gather the outside ring of one solve (FillOutsideCellFunctor) into ring[n_out][nvar][cells]
+void launch_fill_outside(int dim, double * buf0, double * buf1, SolveCtl const * ctl, int solve,
+                         SolveDesc const * solves, int32_t const * slots, int32_t const * coords, int n_out,
+                         double * ring, hipStream_t stream);
 
 // U2 = a*U + b*U2 over n doubles (RK2_SSP LinearCombination)
 void launch_lincomb(double a, double const * U, double b, double * U2, size_t n, hipStream_t stream);
+
+// initial condition of one solve into its slots of the current buffer
+void launch_init_ic(int dim, double * buf0, double * buf1, SolveCtl const * ctl, int solve, SolveDesc const * solves,
+                    int32_t const * slots, IcDesc const * ic, int nblk, hipStream_t stream);
 
 } // namespace g5

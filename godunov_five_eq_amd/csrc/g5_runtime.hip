@@ -1,8 +1,13 @@
-// g5_runtime.hip -- the C ABI (include/godunov_five_eq.h): context, state transfer, step
-// orchestration (the GodunovImplemV0::do_time_step / godunov_unsplit surface), device-resident
-// time loop.
+// g5_runtime.hip -- the C ABI (include/godunov_five_eq.h).
+//
+// Everything is built on one internal object, the Pool: two state buffers of block slots sized by
+// an HBM budget, a table of solves (SolveDesc + device-resident SolveCtl), a slot table per solve
+// and the launch list of the blocks of all live solves.  A single solve (g5_ctx, the
+// GodunovImplemV0 surface) is a pool of one; g5_pool_* exposes the pool itself for batched and
+// served solves that are admitted, stepped together and released independently.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -14,52 +19,130 @@
 
 using namespace g5;
 
-struct g5_ctx
-{
-  g5_config   cfg{};
-  SolveDesc   sd{};
-  int         nblocks = 0;
-  int         nvar = 0;
-  int         ncell = 0;
-  size_t      nstate = 0; // doubles per state copy
-  double *    dU[2] = { nullptr, nullptr };
-  double *    dU3 = nullptr; // RK2_SSP stage buffer
-  int         cur = 0;
-  BlockDesc * d_blocks = nullptr;
-  SolveDesc * d_solve = nullptr;
-  SolveCtl *  d_ctl = nullptr;
-  SolveCtl *  h_ctl = nullptr; // pinned
-  int32_t *   d_outside = nullptr;
-  double *    d_ring = nullptr;
-  std::vector<int32_t> outside;
-  hipStream_t stream = nullptr;
-  bool        has_state = false;
-  int         per_xcd = 0;
-  double      last_run_ms = 0.0; // HIP-event time of the last g5_run step loop
-  std::string err;
-};
-
 namespace
 {
 
 std::string g_create_error;
 
-int
-fail(g5_ctx * ctx, int code, std::string const & msg)
+struct SolveRec
 {
-  if (ctx)
-    ctx->err = msg;
-  else
-    g_create_error = msg;
+  bool                 live = false;
+  g5_config            cfg{};
+  int32_t              nblk = 0;
+  int32_t              table_base = 0;
+  std::vector<int32_t> slots;
+};
+
+// first-fit allocator of contiguous index ranges (slot-table segments)
+struct RangeAlloc
+{
+  std::vector<std::pair<int32_t, int32_t>> free; // (start, length), sorted by start
+  int32_t                                  end = 0;
+  int32_t
+  take(int32_t n)
+  {
+    for (size_t k = 0; k < free.size(); ++k)
+      if (free[k].second >= n)
+      {
+        const int32_t s = free[k].first;
+        free[k].first += n;
+        free[k].second -= n;
+        if (free[k].second == 0)
+          free.erase(free.begin() + static_cast<long>(k));
+        return s;
+      }
+    const int32_t s = end;
+    end += n;
+    return s;
+  }
+  void
+  give(int32_t s, int32_t n)
+  {
+    free.emplace_back(s, n);
+    std::sort(free.begin(), free.end());
+    std::vector<std::pair<int32_t, int32_t>> merged;
+    for (auto const & r : free)
+    {
+      if (!merged.empty() and merged.back().first + merged.back().second == r.first)
+        merged.back().second += r.second;
+      else
+        merged.push_back(r);
+    }
+    free.swap(merged);
+    if (!free.empty() and free.back().first + free.back().second == end)
+    {
+      end = free.back().first;
+      free.pop_back();
+    }
+  }
+};
+
+} // namespace
+
+struct g5_pool
+{
+  int32_t     dim = 2, thinc = 0, precision = G5_PREC_FP64;
+  int32_t     nvar = 8, ncell = 256, block_doubles = 2048;
+  int64_t     nslots = 0;
+  double *    buf[2] = { nullptr, nullptr };
+  double *    stage = nullptr; // RK2_SSP stage buffer (single-solve contexts only)
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  std::vector<int32_t>  free_slots; // stack
+  std::vector<SolveRec> solves;
+  std::vector<int32_t>  free_ids;
+  RangeAlloc            table;
+
+  // device mirrors
+  SolveDesc * d_solves = nullptr;
+  SolveCtl *  d_ctl = nullptr;
+  int32_t     cap_solves = 0;
+  int32_t *   d_slots = nullptr;
+  int32_t     cap_table = 0;
+  BlockDesc * d_blocks = nullptr;
+  int64_t     cap_blocks = 0;
+  int32_t     nlist = 0;
+  bool        list_dirty = true;
+  SolveCtl *  h_ctl = nullptr; // pinned mirror of the control records
+  int32_t     cap_hctl = 0;
+
+  // HIP graph of one device-dt step over the current launch list
+  hipGraphExec_t graph = nullptr;
+  bool           graph_valid = false;
+
+  double last_run_ms = 0.0;
+};
+
+struct g5_ctx
+{
+  g5_pool *            pool = nullptr;
+  g5_config            cfg{};
+  int32_t              id = 0;
+  bool                 has_state = false;
+  std::vector<int32_t> outside;
+  int32_t *            d_outside = nullptr;
+  double *             d_ring = nullptr;
+  double *             d_staging = nullptr; // contiguous [block][var][cell] buffer for transfers
+  std::string          err;
+};
+
+namespace
+{
+
+int
+fail(std::string & err, int code, std::string const & msg)
+{
+  err = msg;
   return code;
 }
 
-#define G5_HIP(ctx, call)                                                                          \
+#define G5_HIP(errstr, call)                                                                       \
   do                                                                                               \
   {                                                                                                \
     hipError_t e_ = (call);                                                                        \
     if (e_ != hipSuccess)                                                                          \
-      return fail(ctx, G5_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));            \
+      return fail(errstr, G5_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
   } while (0)
 
 int
@@ -71,8 +154,8 @@ validate(g5_config const & c, std::string & msg)
     return msg = "the HIP step kernels take bsize = 16", G5_ERR_UNSUPPORTED;
   for (int d = 0; d < c.dim; ++d)
   {
-    if (c.nb[d] < 1)
-      return msg = "nb must be >= 1", G5_ERR_INVALID;
+    if (c.nb[d] < 1 or c.nb[d] > 32767)
+      return msg = "nb must be in [1, 32767]", G5_ERR_INVALID;
     if (not c.periodic[d])
     {
       for (int s = 0; s < 2; ++s)
@@ -96,42 +179,437 @@ validate(g5_config const & c, std::string & msg)
   return G5_OK;
 }
 
-void
-launch_step(g5_ctx * ctx, double const * U, double * U2, double dt_fixed, bool hancock)
+int64_t
+nblocks_of(g5_config const & c)
 {
-  StepArgs A;
-  A.U = U;
-  A.U2 = U2;
-  A.blocks = ctx->d_blocks;
-  A.solves = ctx->d_solve;
-  A.ctl = ctx->d_ctl;
-  A.nblocks = ctx->nblocks;
-  A.per_xcd = ctx->per_xcd;
-  A.dt_fixed = dt_fixed;
-  if (ctx->cfg.dim == 2)
-    launch_step2d(A, ctx->cfg.thinc_enabled != 0, hancock, ctx->stream);
-  else
-    launch_step3d(A, ctx->cfg.thinc_enabled != 0, hancock, ctx->stream);
+  return static_cast<int64_t>(c.nb[0]) * c.nb[1] * (c.dim == 3 ? c.nb[2] : 1);
 }
 
-// one godunov_unsplit iteration with a host dt
+template <typename T>
 int
-host_step(g5_ctx * ctx, double dt)
+grow(g5_pool * p, T *& ptr, int64_t & cap, int64_t need, hipStream_t s)
 {
-  const int c = ctx->cur;
-  if (ctx->cfg.integrator == G5_TI_RK2_SSP)
+  if (need <= cap)
+    return G5_OK;
+  int64_t nc = std::max<int64_t>(need, cap * 2);
+  nc = std::max<int64_t>(nc, 16);
+  T *     np = nullptr;
+  G5_HIP(p->err, hipMalloc(&np, nc * sizeof(T)));
+  if (ptr)
   {
-    // stage 1: U_RK = U + dt L(U); stage 2: U2 = U_RK + dt L(U_RK); U = 1/2 U + 1/2 U2
-    launch_step(ctx, ctx->dU[c], ctx->dU3, dt, false);
-    launch_step(ctx, ctx->dU3, ctx->dU[1 - c], dt, false);
-    launch_lincomb(0.5, ctx->dU[c], 0.5, ctx->dU[1 - c], ctx->nstate, ctx->stream);
+    G5_HIP(p->err, hipMemcpyAsync(np, ptr, cap * sizeof(T), hipMemcpyDeviceToDevice, s));
+    G5_HIP(p->err, hipStreamSynchronize(s));
+    hipFree(ptr);
+  }
+  ptr = np;
+  cap = nc;
+  return G5_OK;
+}
+
+int
+pool_create(int dim, int thinc, int precision, uint64_t budget, bool rk2_stage, g5_pool ** out, std::string & err)
+{
+  g5_pool * p = new g5_pool();
+  p->dim = dim;
+  p->thinc = thinc;
+  p->precision = precision;
+  p->nvar = dim == 2 ? 8 : 9;
+  p->ncell = dim == 2 ? 256 : 4096;
+  p->block_doubles = p->nvar * p->ncell;
+  const uint64_t per_slot = static_cast<uint64_t>(p->block_doubles) * sizeof(double) * (rk2_stage ? 3 : 2);
+  p->nslots = static_cast<int64_t>(budget / per_slot);
+  if (p->nslots < 1)
+  {
+    delete p;
+    return fail(err, G5_ERR_NOMEM, "HBM budget smaller than one block");
+  }
+  auto bail = [&](int code, std::string const & m) {
+    g5_pool_destroy(p);
+    return fail(err, code, m);
+  };
+  hipError_t e = hipStreamCreate(&p->stream);
+  if (e != hipSuccess)
+    return bail(G5_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  const size_t bytes = static_cast<size_t>(p->nslots) * p->block_doubles * sizeof(double);
+  for (int k = 0; k < 2; ++k)
+    if ((e = hipMalloc(&p->buf[k], bytes)) != hipSuccess)
+      return bail(G5_ERR_NOMEM, std::string("hipMalloc state: ") + hipGetErrorString(e));
+  if (rk2_stage and (e = hipMalloc(&p->stage, bytes)) != hipSuccess)
+    return bail(G5_ERR_NOMEM, std::string("hipMalloc stage: ") + hipGetErrorString(e));
+  p->free_slots.resize(p->nslots);
+  for (int64_t k = 0; k < p->nslots; ++k)
+    p->free_slots[k] = static_cast<int32_t>(p->nslots - 1 - k); // pop_back gives 0, 1, 2, ...
+  *out = p;
+  return G5_OK;
+}
+
+int
+ensure_solve_capacity(g5_pool * p, int32_t need)
+{
+  if (need <= p->cap_solves)
+    return G5_OK;
+  int64_t cs = p->cap_solves, cc = p->cap_solves;
+  int     rc = grow(p, p->d_solves, cs, need, p->stream);
+  if (rc != G5_OK)
+    return rc;
+  rc = grow(p, p->d_ctl, cc, need, p->stream);
+  if (rc != G5_OK)
+    return rc;
+  p->cap_solves = static_cast<int32_t>(std::min(cs, cc));
+  p->graph_valid = false;
+  return G5_OK;
+}
+
+int
+pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, int32_t * id)
+{
+  std::string msg;
+  int         rc = validate(cfg, msg);
+  if (rc != G5_OK)
+    return fail(p->err, rc, msg);
+  if (cfg.dim != p->dim or (cfg.thinc_enabled != 0) != (p->thinc != 0) or cfg.precision != p->precision)
+    return fail(p->err, G5_ERR_INVALID, "solve does not match the pool (dim, THINC, precision)");
+  if (p->stage == nullptr and cfg.integrator == G5_TI_RK2_SSP)
+    return fail(p->err, G5_ERR_UNSUPPORTED, "RK2_SSP needs a single-solve context");
+  const int64_t nblk = nblocks_of(cfg);
+  if (nblk > static_cast<int64_t>(p->free_slots.size()))
+    return fail(p->err, G5_ERR_NOMEM, "HBM budget exhausted");
+  int32_t sid;
+  if (!p->free_ids.empty())
+  {
+    sid = p->free_ids.back();
+    p->free_ids.pop_back();
   }
   else
   {
-    launch_step(ctx, ctx->dU[c], ctx->dU[1 - c], dt, ctx->cfg.integrator == G5_TI_HANCOCK);
+    sid = static_cast<int32_t>(p->solves.size());
+    p->solves.emplace_back();
   }
-  ctx->cur = 1 - c;
-  G5_HIP(ctx, hipGetLastError());
+  rc = ensure_solve_capacity(p, static_cast<int32_t>(p->solves.size()));
+  if (rc != G5_OK)
+    return rc;
+  SolveRec & r = p->solves[sid];
+  r.live = true;
+  r.cfg = cfg;
+  for (int d = cfg.dim; d < 3; ++d)
+    r.cfg.nb[d] = 1;
+  r.nblk = static_cast<int32_t>(nblk);
+  r.slots.resize(nblk);
+  for (int64_t k = 0; k < nblk; ++k)
+  {
+    r.slots[k] = p->free_slots.back();
+    p->free_slots.pop_back();
+  }
+  r.table_base = p->table.take(r.nblk);
+  int64_t ct = p->cap_table;
+  rc = grow(p, p->d_slots, ct, p->table.end, p->stream);
+  if (rc != G5_OK)
+    return rc;
+  p->cap_table = static_cast<int32_t>(ct);
+  SolveDesc sd;
+  make_solve_desc(r.cfg, r.table_base, sd);
+  SolveCtl c{};
+  c.tend = tend;
+  c.max_steps = max_steps;
+  G5_HIP(p->err, hipMemcpyAsync(p->d_slots + r.table_base, r.slots.data(), nblk * sizeof(int32_t),
+                                hipMemcpyHostToDevice, p->stream));
+  G5_HIP(p->err, hipMemcpyAsync(p->d_solves + sid, &sd, sizeof(SolveDesc), hipMemcpyHostToDevice, p->stream));
+  G5_HIP(p->err, hipMemcpyAsync(p->d_ctl + sid, &c, sizeof(SolveCtl), hipMemcpyHostToDevice, p->stream));
+  G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  p->list_dirty = true;
+  *id = sid;
+  return G5_OK;
+}
+
+int
+pool_release(g5_pool * p, int32_t id)
+{
+  if (id < 0 or id >= static_cast<int32_t>(p->solves.size()) or !p->solves[id].live)
+    return fail(p->err, G5_ERR_INVALID, "no such solve");
+  G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  SolveRec & r = p->solves[id];
+  for (auto s : r.slots)
+    p->free_slots.push_back(s);
+  p->table.give(r.table_base, r.nblk);
+  r = SolveRec();
+  p->free_ids.push_back(id);
+  SolveCtl c{}; // inactive
+  G5_HIP(p->err, hipMemcpyAsync(p->d_ctl + id, &c, sizeof(SolveCtl), hipMemcpyHostToDevice, p->stream));
+  G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  p->list_dirty = true;
+  return G5_OK;
+}
+
+void
+append_blocks(g5_pool const * p, int32_t sid, std::vector<BlockDesc> & out)
+{
+  SolveRec const & r = p->solves[sid];
+  const int        nx = r.cfg.nb[0], ny = r.cfg.nb[1], nz = r.cfg.nb[2];
+  int              k = 0;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x, ++k)
+        out.push_back(BlockDesc{ sid, r.slots[k], static_cast<int16_t>(x), static_cast<int16_t>(y),
+                                 static_cast<int16_t>(z), 0 });
+}
+
+int
+rebuild_list(g5_pool * p)
+{
+  if (!p->list_dirty)
+    return G5_OK;
+  std::vector<BlockDesc> list;
+  for (int32_t s = 0; s < static_cast<int32_t>(p->solves.size()); ++s)
+    if (p->solves[s].live)
+      append_blocks(p, s, list);
+  int64_t cb = p->cap_blocks;
+  int     rc = grow(p, p->d_blocks, cb, static_cast<int64_t>(list.size()), p->stream);
+  if (rc != G5_OK)
+    return rc;
+  p->cap_blocks = cb;
+  if (!list.empty())
+    G5_HIP(p->err, hipMemcpyAsync(p->d_blocks, list.data(), list.size() * sizeof(BlockDesc), hipMemcpyHostToDevice,
+                                  p->stream));
+  G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  p->nlist = static_cast<int32_t>(list.size());
+  p->list_dirty = false;
+  p->graph_valid = false;
+  return G5_OK;
+}
+
+StepArgs
+step_args(g5_pool * p, BlockDesc const * blocks, int32_t n, double dt_fixed)
+{
+  StepArgs A;
+  A.buf0 = p->buf[0];
+  A.buf1 = p->buf[1];
+  A.U_in = nullptr;
+  A.U_out = nullptr;
+  A.blocks = blocks;
+  A.solves = p->d_solves;
+  A.ctl = p->d_ctl;
+  A.slots = p->d_slots;
+  A.nblocks = n;
+  A.per_xcd = (n + 7) / 8;
+  A.dt_fixed = dt_fixed;
+  return A;
+}
+
+void
+launch_step(g5_pool * p, StepArgs const & A, bool hancock)
+{
+  if (A.nblocks <= 0)
+    return;
+  if (p->dim == 2)
+    launch_step2d(A, p->thinc != 0, hancock, p->stream);
+  else
+    launch_step3d(A, p->thinc != 0, hancock, p->stream);
+}
+
+bool
+pool_hancock(g5_pool const * p)
+{
+  for (auto const & r : p->solves)
+    if (r.live)
+      return r.cfg.integrator == G5_TI_HANCOCK;
+  return true;
+}
+
+// one device-dt step of every active solve: step kernel + finalize, as a HIP graph
+int
+pool_step_once(g5_pool * p)
+{
+  if (p->nlist == 0)
+    return G5_OK;
+  if (!p->graph_valid)
+  {
+    if (p->graph)
+    {
+      hipGraphExecDestroy(p->graph);
+      p->graph = nullptr;
+    }
+    hipGraph_t g;
+    G5_HIP(p->err, hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    launch_step(p, step_args(p, p->d_blocks, p->nlist, 0.0), pool_hancock(p));
+    launch_finalize(p->d_ctl, p->d_solves, static_cast<int>(p->solves.size()), p->stream);
+    G5_HIP(p->err, hipStreamEndCapture(p->stream, &g));
+    G5_HIP(p->err, hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0));
+    hipGraphDestroy(g);
+    p->graph_valid = true;
+  }
+  G5_HIP(p->err, hipGraphLaunch(p->graph, p->stream));
+  return G5_OK;
+}
+
+int
+pool_step(g5_pool * p, int32_t nsteps)
+{
+  int rc = rebuild_list(p);
+  if (rc != G5_OK)
+    return rc;
+  hipEvent_t e0, e1;
+  G5_HIP(p->err, hipEventCreate(&e0));
+  G5_HIP(p->err, hipEventCreate(&e1));
+  G5_HIP(p->err, hipEventRecord(e0, p->stream));
+  for (int32_t n = 0; n < nsteps; ++n)
+  {
+    rc = pool_step_once(p);
+    if (rc != G5_OK)
+      return rc;
+  }
+  G5_HIP(p->err, hipEventRecord(e1, p->stream));
+  G5_HIP(p->err, hipEventSynchronize(e1));
+  float ms = 0.f;
+  G5_HIP(p->err, hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  p->last_run_ms = ms;
+  G5_HIP(p->err, hipGetLastError());
+  return G5_OK;
+}
+
+// first dt of the given solves from their current state
+int
+pool_start(g5_pool * p, int32_t const * ids, int32_t n)
+{
+  std::vector<BlockDesc> list;
+  for (int32_t k = 0; k < n; ++k)
+  {
+    if (ids[k] < 0 or ids[k] >= static_cast<int32_t>(p->solves.size()) or !p->solves[ids[k]].live)
+      return fail(p->err, G5_ERR_INVALID, "no such solve");
+    append_blocks(p, ids[k], list);
+  }
+  BlockDesc * d_list = nullptr;
+  int32_t *   d_ids = nullptr;
+  G5_HIP(p->err, hipMalloc(&d_list, std::max<size_t>(1, list.size()) * sizeof(BlockDesc)));
+  G5_HIP(p->err, hipMalloc(&d_ids, std::max<int32_t>(1, n) * sizeof(int32_t)));
+  G5_HIP(p->err, hipMemcpyAsync(d_list, list.data(), list.size() * sizeof(BlockDesc), hipMemcpyHostToDevice, p->stream));
+  G5_HIP(p->err, hipMemcpyAsync(d_ids, ids, n * sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
+  // invdt of these solves is 0 (admit and finalize leave it cleared)
+  launch_inv_dt(p->dim, p->buf[0], p->buf[1], d_list, p->d_solves, p->d_ctl, static_cast<int>(list.size()), p->stream);
+  launch_start(p->d_ctl, p->d_solves, d_ids, n, p->stream);
+  G5_HIP(p->err, hipGetLastError());
+  G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  hipFree(d_list);
+  hipFree(d_ids);
+  return G5_OK;
+}
+
+int
+pool_transfer(g5_pool * p, int32_t id, double * host, size_t n, bool upload, double ** staging)
+{
+  if (id < 0 or id >= static_cast<int32_t>(p->solves.size()) or !p->solves[id].live)
+    return fail(p->err, G5_ERR_INVALID, "no such solve");
+  SolveRec const & r = p->solves[id];
+  const size_t     need = static_cast<size_t>(r.nblk) * p->block_doubles;
+  if (n != need)
+    return fail(p->err, G5_ERR_INVALID, "state size mismatch");
+  double * st = staging ? *staging : nullptr;
+  if (!st)
+  {
+    G5_HIP(p->err, hipMalloc(&st, need * sizeof(double)));
+    if (staging)
+      *staging = st;
+  }
+  if (upload)
+  {
+    G5_HIP(p->err, hipMemcpyAsync(st, host, need * sizeof(double), hipMemcpyHostToDevice, p->stream));
+    launch_gather(p->buf[0], p->buf[1], p->d_ctl, id, p->d_slots, r.table_base, r.nblk, p->block_doubles, st, true,
+                  p->stream);
+  }
+  else
+  {
+    launch_gather(p->buf[0], p->buf[1], p->d_ctl, id, p->d_slots, r.table_base, r.nblk, p->block_doubles, st, false,
+                  p->stream);
+    G5_HIP(p->err, hipMemcpyAsync(host, st, need * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  }
+  G5_HIP(p->err, hipGetLastError());
+  G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  if (!staging)
+    hipFree(st);
+  return G5_OK;
+}
+
+// host-side description of the shock-bubble / triple-point initial conditions
+void
+region_cons(g5_config const & c, g5_region const & r, double * U)
+{
+  double g1[2], gp[2];
+  for (int k = 0; k < 2; ++k)
+  {
+    g1[k] = 1.0 / (c.gamma[k] - 1.0);
+    gp[k] = c.gamma[k] * c.pinf[k] / (c.gamma[k] - 1.0);
+  }
+  const int    dim = c.dim;
+  const double rho = r.alpha_rho[0] + r.alpha_rho[1];
+  const double a0 = r.alpha0, a1 = 1.0 - r.alpha0;
+  const double e = (r.p * (a0 * g1[0] + a1 * g1[1]) + (a0 * gp[0] + a1 * gp[1])) / rho;
+  const double ekin = dim == 2 ? 0.5 * (r.u * r.u + r.v * r.v) : 0.5 * (r.u * r.u + r.v * r.v + r.w * r.w);
+  const int    IA0 = dim == 2 ? 4 : 5;
+  for (int v = 0; v < 9; ++v)
+    U[v] = 0.0;
+  U[0] = rho;
+  U[1] = (e + ekin) * rho;
+  U[2] = r.u * rho;
+  U[3] = r.v * rho;
+  if (dim == 3)
+    U[4] = r.w * rho;
+  U[IA0] = r.alpha0;
+  U[IA0 + 1] = r.alpha_rho[0];
+  U[IA0 + 2] = 1.0 - r.alpha0;
+  U[IA0 + 3] = r.alpha_rho[1];
+}
+
+void
+fill_ic(g5_config const & c, g5_region const * regions3, IcDesc & ic)
+{
+  for (int r = 0; r < 3; ++r)
+    region_cons(c, regions3[r], ic.S[r]);
+  for (int d = 0; d < 3; ++d)
+    ic.xmin[d] = c.xmin[d];
+  const int dim = c.dim;
+  auto      eint_of = [&](double const * s) {
+    double ekin = s[2] * s[2] + s[3] * s[3];
+    if (dim == 3)
+      ekin += s[4] * s[4];
+    ekin /= (2.0 * s[0]);
+    return s[1] - ekin;
+  };
+  ic.eint_pre = eint_of(ic.S[1]);
+  ic.eint_bub = eint_of(ic.S[2]);
+}
+
+int
+pool_init_ic(g5_pool * p, int32_t id, IcDesc const & ic)
+{
+  if (id < 0 or id >= static_cast<int32_t>(p->solves.size()) or !p->solves[id].live)
+    return fail(p->err, G5_ERR_INVALID, "no such solve");
+  IcDesc * d_ic = nullptr;
+  G5_HIP(p->err, hipMalloc(&d_ic, sizeof(IcDesc)));
+  G5_HIP(p->err, hipMemcpyAsync(d_ic, &ic, sizeof(IcDesc), hipMemcpyHostToDevice, p->stream));
+  launch_init_ic(p->dim, p->buf[0], p->buf[1], p->d_ctl, id, p->d_solves, p->d_slots, d_ic, p->solves[id].nblk,
+                 p->stream);
+  G5_HIP(p->err, hipGetLastError());
+  G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  hipFree(d_ic);
+  return G5_OK;
+}
+
+int
+read_ctl(g5_pool * p)
+{
+  const int32_t n = static_cast<int32_t>(p->solves.size());
+  if (n > p->cap_hctl)
+  {
+    if (p->h_ctl)
+      hipHostFree(p->h_ctl);
+    p->h_ctl = nullptr;
+    G5_HIP(p->err, hipHostMalloc(&p->h_ctl, std::max(n, 1) * sizeof(SolveCtl), hipHostMallocDefault));
+    p->cap_hctl = std::max(n, 1);
+  }
+  if (n > 0)
+    G5_HIP(p->err, hipMemcpyAsync(p->h_ctl, p->d_ctl, n * sizeof(SolveCtl), hipMemcpyDeviceToHost, p->stream));
+  G5_HIP(p->err, hipStreamSynchronize(p->stream));
   return G5_OK;
 }
 
@@ -143,7 +621,7 @@ extern "C"
 const char *
 g5_version(void)
 {
-  return "godunov_five_eq_amd 0.1 (gfx950, FP64 fused block step)";
+  return "godunov_five_eq_amd 0.2 (gfx950, fused block step, pooled solves)";
 }
 
 const char *
@@ -152,34 +630,241 @@ g5_last_error(const g5_ctx * ctx)
   return ctx ? ctx->err.c_str() : g_create_error.c_str();
 }
 
+// =============================================================================================
+// pools
+// =============================================================================================
+int
+g5_pool_create(int32_t dim, int32_t thinc_enabled, int32_t precision, uint64_t hbm_budget_bytes, g5_pool ** out)
+{
+  if (not out)
+    return fail(g_create_error, G5_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (dim != 2 and dim != 3)
+    return fail(g_create_error, G5_ERR_INVALID, "dim must be 2 or 3");
+  if (precision == G5_PREC_MIXED)
+    return fail(g_create_error, G5_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+  if (precision != G5_PREC_FP64)
+    return fail(g_create_error, G5_ERR_INVALID, "unknown precision");
+  return pool_create(dim, thinc_enabled ? 1 : 0, precision, hbm_budget_bytes, false, out, g_create_error);
+}
+
+int
+g5_pool_destroy(g5_pool * p)
+{
+  if (not p)
+    return G5_OK;
+  if (p->stream)
+    hipStreamSynchronize(p->stream);
+  if (p->graph)
+    hipGraphExecDestroy(p->graph);
+  hipFree(p->buf[0]);
+  hipFree(p->buf[1]);
+  hipFree(p->stage);
+  hipFree(p->d_solves);
+  hipFree(p->d_ctl);
+  hipFree(p->d_slots);
+  hipFree(p->d_blocks);
+  if (p->h_ctl)
+    hipHostFree(p->h_ctl);
+  if (p->stream)
+    hipStreamDestroy(p->stream);
+  delete p;
+  return G5_OK;
+}
+
+const char *
+g5_pool_last_error(const g5_pool * p)
+{
+  return p ? p->err.c_str() : g_create_error.c_str();
+}
+
+int
+g5_pool_admit(g5_pool * p, const g5_config * cfg, double t_end, int32_t max_steps, int32_t * solve_id)
+{
+  if (not p or not cfg or not solve_id)
+    return G5_ERR_INVALID;
+  return pool_admit(p, *cfg, t_end, max_steps, solve_id);
+}
+
+int
+g5_pool_release(g5_pool * p, int32_t id)
+{
+  if (not p)
+    return G5_ERR_INVALID;
+  return pool_release(p, id);
+}
+
+int
+g5_pool_upload(g5_pool * p, int32_t id, const double * U, size_t n)
+{
+  if (not p or not U)
+    return G5_ERR_INVALID;
+  return pool_transfer(p, id, const_cast<double *>(U), n, true, nullptr);
+}
+
+int
+g5_pool_download(g5_pool * p, int32_t id, double * U, size_t n)
+{
+  if (not p or not U)
+    return G5_ERR_INVALID;
+  return pool_transfer(p, id, U, n, false, nullptr);
+}
+
+int
+g5_pool_init_shock_bubble(g5_pool * p, int32_t id, const g5_region * regions3, int32_t shape, double x_front,
+                          const double * center3, const double * size3)
+{
+  if (not p or not regions3 or not center3 or not size3)
+    return G5_ERR_INVALID;
+  if (id < 0 or id >= static_cast<int32_t>(p->solves.size()) or !p->solves[id].live)
+    return fail(p->err, G5_ERR_INVALID, "no such solve");
+  IcDesc ic{};
+  ic.kind = shape == 0 ? 0 : 1;
+  ic.x_front = x_front;
+  for (int d = 0; d < 3; ++d)
+  {
+    ic.center[d] = center3[d];
+    ic.half[d] = size3[d];
+  }
+  ic.radius = size3[0];
+  fill_ic(p->solves[id].cfg, regions3, ic);
+  return pool_init_ic(p, id, ic);
+}
+
+int
+g5_pool_init_triple_point(g5_pool * p, int32_t id, const g5_region * regions3, double xd, double yd)
+{
+  if (not p or not regions3)
+    return G5_ERR_INVALID;
+  if (id < 0 or id >= static_cast<int32_t>(p->solves.size()) or !p->solves[id].live)
+    return fail(p->err, G5_ERR_INVALID, "no such solve");
+  IcDesc ic{};
+  ic.kind = 2;
+  ic.x_front = xd;
+  ic.yd = yd;
+  fill_ic(p->solves[id].cfg, regions3, ic);
+  return pool_init_ic(p, id, ic);
+}
+
+int
+g5_pool_start(g5_pool * p, const int32_t * ids, int32_t n)
+{
+  if (not p or (n > 0 and not ids))
+    return G5_ERR_INVALID;
+  return pool_start(p, ids, n);
+}
+
+int
+g5_pool_step(g5_pool * p, int32_t nsteps)
+{
+  if (not p or nsteps < 0)
+    return G5_ERR_INVALID;
+  return pool_step(p, nsteps);
+}
+
+int
+g5_pool_status(g5_pool * p, int32_t id, double * t, int32_t * steps, int32_t * active)
+{
+  if (not p)
+    return G5_ERR_INVALID;
+  if (id < 0 or id >= static_cast<int32_t>(p->solves.size()) or !p->solves[id].live)
+    return fail(p->err, G5_ERR_INVALID, "no such solve");
+  const int rc = read_ctl(p);
+  if (rc != G5_OK)
+    return rc;
+  if (t)
+    *t = p->h_ctl[id].t;
+  if (steps)
+    *steps = p->h_ctl[id].steps;
+  if (active)
+    *active = p->h_ctl[id].active;
+  return G5_OK;
+}
+
+int
+g5_pool_finished(g5_pool * p, int32_t * ids, int32_t cap, int32_t * n_done)
+{
+  if (not p or not n_done)
+    return G5_ERR_INVALID;
+  const int rc = read_ctl(p);
+  if (rc != G5_OK)
+    return rc;
+  int32_t n = 0;
+  for (int32_t s = 0; s < static_cast<int32_t>(p->solves.size()); ++s)
+    if (p->solves[s].live and p->h_ctl[s].active == 0)
+    {
+      if (ids and n < cap)
+        ids[n] = s;
+      ++n;
+    }
+  *n_done = n;
+  return G5_OK;
+}
+
+int
+g5_pool_usage(const g5_pool * p, int64_t * used_blocks, int64_t * capacity_blocks, int32_t * live_solves)
+{
+  if (not p)
+    return G5_ERR_INVALID;
+  if (used_blocks)
+    *used_blocks = p->nslots - static_cast<int64_t>(p->free_slots.size());
+  if (capacity_blocks)
+    *capacity_blocks = p->nslots;
+  if (live_solves)
+  {
+    int32_t n = 0;
+    for (auto const & r : p->solves)
+      n += r.live ? 1 : 0;
+    *live_solves = n;
+  }
+  return G5_OK;
+}
+
+int
+g5_pool_last_step_ms(const g5_pool * p, double * ms)
+{
+  if (not p or not ms)
+    return G5_ERR_INVALID;
+  *ms = p->last_run_ms;
+  return G5_OK;
+}
+
+// =============================================================================================
+// single solve: a pool of one whose slots are the lexicographic block order
+// =============================================================================================
 int
 g5_create(const g5_config * cfg, g5_ctx ** out)
 {
   if (not cfg or not out)
-    return fail(nullptr, G5_ERR_INVALID, "null argument");
+    return fail(g_create_error, G5_ERR_INVALID, "null argument");
   *out = nullptr;
   std::string msg;
-  const int   rc = validate(*cfg, msg);
+  int         rc = validate(*cfg, msg);
   if (rc != G5_OK)
-    return fail(nullptr, rc, msg);
+    return fail(g_create_error, rc, msg);
   g5_ctx * ctx = new g5_ctx();
   ctx->cfg = *cfg;
   for (int d = cfg->dim; d < 3; ++d)
     ctx->cfg.nb[d] = 1;
-  make_solve_desc(ctx->cfg, 0, ctx->sd);
-  const int nx = ctx->cfg.nb[0], ny = ctx->cfg.nb[1], nz = ctx->cfg.nb[2];
-  ctx->nblocks = nx * ny * nz;
-  ctx->nvar = cfg->dim == 2 ? 8 : 9;
-  ctx->ncell = cfg->dim == 2 ? 256 : 4096;
-  ctx->nstate = static_cast<size_t>(ctx->nblocks) * ctx->nvar * ctx->ncell;
-  ctx->per_xcd = (ctx->nblocks + 7) / 8;
-
-  std::vector<BlockDesc> blocks(ctx->nblocks);
-  for (int z = 0, b = 0; z < nz; ++z)
-    for (int y = 0; y < ny; ++y)
-      for (int x = 0; x < nx; ++x, ++b)
-        blocks[b] = BlockDesc{ 0, x, y, z };
+  const int64_t  nblk = nblocks_of(ctx->cfg);
+  const int      bd = (cfg->dim == 2 ? 8 * 256 : 9 * 4096);
+  const bool     rk2 = cfg->integrator == G5_TI_RK2_SSP;
+  const uint64_t budget = static_cast<uint64_t>(nblk) * bd * sizeof(double) * (rk2 ? 3 : 2);
+  rc = pool_create(cfg->dim, cfg->thinc_enabled ? 1 : 0, cfg->precision, budget, rk2, &ctx->pool, g_create_error);
+  if (rc != G5_OK)
+  {
+    delete ctx;
+    return rc;
+  }
+  rc = pool_admit(ctx->pool, ctx->cfg, 1e300, 0x7fffffff, &ctx->id);
+  if (rc != G5_OK)
+  {
+    g_create_error = ctx->pool->err;
+    g5_destroy(ctx);
+    return rc;
+  }
   // outside ring: blocks leaving the brick along at least one non-periodic direction
+  const int nx = ctx->cfg.nb[0], ny = ctx->cfg.nb[1], nz = ctx->cfg.nb[2];
   const int zlo = cfg->dim == 3 ? -1 : 0, zhi = cfg->dim == 3 ? nz : 0;
   for (int z = zlo; z <= zhi; ++z)
     for (int y = -1; y <= ny; ++y)
@@ -197,44 +882,6 @@ g5_create(const g5_config * cfg, g5_ctx ** out)
           ctx->outside.push_back(z);
         }
       }
-
-  auto cleanup = [&](int code, std::string const & m) {
-    g5_destroy(ctx);
-    return fail(nullptr, code, m);
-  };
-  hipError_t e = hipStreamCreate(&ctx->stream);
-  if (e != hipSuccess)
-    return cleanup(G5_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-  for (int k = 0; k < 2; ++k)
-  {
-    e = hipMalloc(&ctx->dU[k], ctx->nstate * sizeof(double));
-    if (e != hipSuccess)
-      return cleanup(G5_ERR_NOMEM, std::string("hipMalloc state: ") + hipGetErrorString(e));
-  }
-  if (cfg->integrator == G5_TI_RK2_SSP)
-  {
-    e = hipMalloc(&ctx->dU3, ctx->nstate * sizeof(double));
-    if (e != hipSuccess)
-      return cleanup(G5_ERR_NOMEM, std::string("hipMalloc stage: ") + hipGetErrorString(e));
-  }
-  if ((e = hipMalloc(&ctx->d_blocks, blocks.size() * sizeof(BlockDesc))) != hipSuccess or
-      (e = hipMalloc(&ctx->d_solve, sizeof(SolveDesc))) != hipSuccess or
-      (e = hipMalloc(&ctx->d_ctl, sizeof(SolveCtl))) != hipSuccess or
-      (e = hipHostMalloc(&ctx->h_ctl, sizeof(SolveCtl), hipHostMallocDefault)) != hipSuccess)
-    return cleanup(G5_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-  if (not ctx->outside.empty())
-  {
-    if ((e = hipMalloc(&ctx->d_outside, ctx->outside.size() * sizeof(int32_t))) != hipSuccess)
-      return cleanup(G5_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    hipMemcpy(ctx->d_outside, ctx->outside.data(), ctx->outside.size() * sizeof(int32_t), hipMemcpyHostToDevice);
-  }
-  hipMemcpy(ctx->d_blocks, blocks.data(), blocks.size() * sizeof(BlockDesc), hipMemcpyHostToDevice);
-  hipMemcpy(ctx->d_solve, &ctx->sd, sizeof(SolveDesc), hipMemcpyHostToDevice);
-  SolveCtl c0{};
-  hipMemcpy(ctx->d_ctl, &c0, sizeof(SolveCtl), hipMemcpyHostToDevice);
-  e = hipGetLastError();
-  if (e != hipSuccess)
-    return cleanup(G5_ERR_HIP, std::string("setup: ") + hipGetErrorString(e));
   *out = ctx;
   return G5_OK;
 }
@@ -244,20 +891,14 @@ g5_destroy(g5_ctx * ctx)
 {
   if (not ctx)
     return G5_OK;
-  if (ctx->stream)
-    hipStreamSynchronize(ctx->stream);
-  hipFree(ctx->dU[0]);
-  hipFree(ctx->dU[1]);
-  hipFree(ctx->dU3);
-  hipFree(ctx->d_blocks);
-  hipFree(ctx->d_solve);
-  hipFree(ctx->d_ctl);
-  hipFree(ctx->d_outside);
-  hipFree(ctx->d_ring);
-  if (ctx->h_ctl)
-    hipHostFree(ctx->h_ctl);
-  if (ctx->stream)
-    hipStreamDestroy(ctx->stream);
+  if (ctx->pool)
+  {
+    hipStreamSynchronize(ctx->pool->stream);
+    hipFree(ctx->d_outside);
+    hipFree(ctx->d_ring);
+    hipFree(ctx->d_staging);
+    g5_pool_destroy(ctx->pool);
+  }
   delete ctx;
   return G5_OK;
 }
@@ -268,11 +909,11 @@ g5_shape(const g5_ctx * ctx, int32_t * n_blocks, int32_t * nvar, int32_t * cells
   if (not ctx)
     return G5_ERR_INVALID;
   if (n_blocks)
-    *n_blocks = ctx->nblocks;
+    *n_blocks = static_cast<int32_t>(nblocks_of(ctx->cfg));
   if (nvar)
-    *nvar = ctx->nvar;
+    *nvar = ctx->pool->nvar;
   if (cells_per_block)
-    *cells_per_block = ctx->ncell;
+    *cells_per_block = ctx->pool->ncell;
   return G5_OK;
 }
 
@@ -281,13 +922,15 @@ g5_mem_bytes(const g5_ctx * ctx, uint64_t * bytes)
 {
   if (not ctx or not bytes)
     return G5_ERR_INVALID;
-  uint64_t b = 2ull * ctx->nstate * sizeof(double);
-  if (ctx->dU3)
-    b += ctx->nstate * sizeof(double);
-  b += ctx->nblocks * sizeof(BlockDesc) + sizeof(SolveDesc) + sizeof(SolveCtl);
+  g5_pool const * p = ctx->pool;
+  uint64_t        b = static_cast<uint64_t>(p->nslots) * p->block_doubles * sizeof(double) * (p->stage ? 3 : 2);
+  b += p->cap_solves * (sizeof(SolveDesc) + sizeof(SolveCtl)) + p->cap_table * sizeof(int32_t) +
+       p->cap_blocks * sizeof(BlockDesc);
   b += ctx->outside.size() * sizeof(int32_t);
   if (ctx->d_ring)
-    b += ctx->outside.size() / 3 * ctx->nvar * ctx->ncell * sizeof(double);
+    b += ctx->outside.size() / 3 * p->block_doubles * sizeof(double);
+  if (ctx->d_staging)
+    b += static_cast<uint64_t>(nblocks_of(ctx->cfg)) * p->block_doubles * sizeof(double);
   *bytes = b;
   return G5_OK;
 }
@@ -296,11 +939,10 @@ int
 g5_upload_state(g5_ctx * ctx, const double * U_host, size_t n)
 {
   if (not ctx or not U_host)
-    return fail(ctx, G5_ERR_INVALID, "null argument");
-  if (n != ctx->nstate)
-    return fail(ctx, G5_ERR_INVALID, "state size mismatch");
-  G5_HIP(ctx, hipMemcpyAsync(ctx->dU[ctx->cur], U_host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-  G5_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return ctx ? fail(ctx->err, G5_ERR_INVALID, "null argument") : G5_ERR_INVALID;
+  const int rc = pool_transfer(ctx->pool, ctx->id, const_cast<double *>(U_host), n, true, &ctx->d_staging);
+  if (rc != G5_OK)
+    return fail(ctx->err, rc, ctx->pool->err);
   ctx->has_state = true;
   return G5_OK;
 }
@@ -309,22 +951,23 @@ int
 g5_download_state(g5_ctx * ctx, double * U_host, size_t n)
 {
   if (not ctx or not U_host)
-    return fail(ctx, G5_ERR_INVALID, "null argument");
-  if (n != ctx->nstate)
-    return fail(ctx, G5_ERR_INVALID, "state size mismatch");
+    return ctx ? fail(ctx->err, G5_ERR_INVALID, "null argument") : G5_ERR_INVALID;
   if (not ctx->has_state)
-    return fail(ctx, G5_ERR_STATE, "no state uploaded");
-  G5_HIP(ctx, hipMemcpyAsync(U_host, ctx->dU[ctx->cur], n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  G5_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return G5_OK;
+    return fail(ctx->err, G5_ERR_STATE, "no state uploaded");
+  const int rc = pool_transfer(ctx->pool, ctx->id, U_host, n, false, &ctx->d_staging);
+  return rc == G5_OK ? G5_OK : fail(ctx->err, rc, ctx->pool->err);
 }
 
 int
 g5_state_device_ptr(g5_ctx * ctx, double ** U_dev)
 {
   if (not ctx or not U_dev)
-    return fail(ctx, G5_ERR_INVALID, "null argument");
-  *U_dev = ctx->dU[ctx->cur];
+    return ctx ? fail(ctx->err, G5_ERR_INVALID, "null argument") : G5_ERR_INVALID;
+  const int rc = read_ctl(ctx->pool);
+  if (rc != G5_OK)
+    return fail(ctx->err, rc, ctx->pool->err);
+  // a pool of one uses slots 0..n-1 in lexicographic order: the buffer is the brick's U array
+  *U_dev = ctx->pool->buf[ctx->pool->h_ctl[ctx->id].parity];
   return G5_OK;
 }
 
@@ -332,17 +975,23 @@ int
 g5_compute_dt(g5_ctx * ctx, double * dt)
 {
   if (not ctx or not dt)
-    return fail(ctx, G5_ERR_INVALID, "null argument");
+    return ctx ? fail(ctx->err, G5_ERR_INVALID, "null argument") : G5_ERR_INVALID;
   if (not ctx->has_state)
-    return fail(ctx, G5_ERR_STATE, "no state uploaded");
-  launch_reset_invdt(ctx->d_ctl, 1, ctx->stream);
-  launch_inv_dt(ctx->cfg.dim, ctx->dU[ctx->cur], ctx->d_blocks, ctx->d_solve, ctx->d_ctl, ctx->nblocks, ctx->stream);
-  G5_HIP(ctx, hipGetLastError());
-  G5_HIP(ctx, hipMemcpyAsync(ctx->h_ctl, ctx->d_ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, ctx->stream));
-  G5_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return fail(ctx->err, G5_ERR_STATE, "no state uploaded");
+  g5_pool * p = ctx->pool;
+  int       rc = rebuild_list(p);
+  if (rc != G5_OK)
+    return fail(ctx->err, rc, p->err);
+  launch_reset_invdt(p->d_ctl, 1, p->stream);
+  launch_inv_dt(p->dim, p->buf[0], p->buf[1], p->d_blocks, p->d_solves, p->d_ctl, p->nlist, p->stream);
+  G5_HIP(ctx->err, hipGetLastError());
+  rc = read_ctl(p);
+  if (rc != G5_OK)
+    return fail(ctx->err, rc, p->err);
   double inv;
-  std::memcpy(&inv, &ctx->h_ctl->invdt_bits, sizeof(double));
+  std::memcpy(&inv, &p->h_ctl[0].invdt_bits, sizeof(double));
   *dt = ctx->cfg.cfl / inv;
+  launch_reset_invdt(p->d_ctl, 1, p->stream);
   return G5_OK;
 }
 
@@ -350,22 +999,30 @@ int
 g5_fill_outside(g5_ctx * ctx, double * out_host, size_t n)
 {
   if (not ctx)
-    return fail(ctx, G5_ERR_INVALID, "null argument");
+    return G5_ERR_INVALID;
   if (not ctx->has_state)
-    return fail(ctx, G5_ERR_STATE, "no state uploaded");
+    return fail(ctx->err, G5_ERR_STATE, "no state uploaded");
+  g5_pool *    p = ctx->pool;
   const int    n_out = static_cast<int>(ctx->outside.size() / 3);
-  const size_t need = static_cast<size_t>(n_out) * ctx->nvar * ctx->ncell;
+  const size_t need = static_cast<size_t>(n_out) * p->block_doubles;
   if (out_host and n != need)
-    return fail(ctx, G5_ERR_INVALID, "ring size mismatch");
+    return fail(ctx->err, G5_ERR_INVALID, "ring size mismatch");
   if (n_out == 0)
     return G5_OK;
+  if (not ctx->d_outside)
+  {
+    G5_HIP(ctx->err, hipMalloc(&ctx->d_outside, ctx->outside.size() * sizeof(int32_t)));
+    G5_HIP(ctx->err, hipMemcpy(ctx->d_outside, ctx->outside.data(), ctx->outside.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice));
+  }
   if (not ctx->d_ring)
-    G5_HIP(ctx, hipMalloc(&ctx->d_ring, need * sizeof(double)));
-  launch_fill_outside(ctx->cfg.dim, ctx->dU[ctx->cur], ctx->d_solve, ctx->d_outside, n_out, ctx->d_ring, ctx->stream);
-  G5_HIP(ctx, hipGetLastError());
+    G5_HIP(ctx->err, hipMalloc(&ctx->d_ring, need * sizeof(double)));
+  launch_fill_outside(p->dim, p->buf[0], p->buf[1], p->d_ctl, ctx->id, p->d_solves, p->d_slots, ctx->d_outside, n_out,
+                      ctx->d_ring, p->stream);
+  G5_HIP(ctx->err, hipGetLastError());
   if (out_host)
-    G5_HIP(ctx, hipMemcpyAsync(out_host, ctx->d_ring, need * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  G5_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    G5_HIP(ctx->err, hipMemcpyAsync(out_host, ctx->d_ring, need * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  G5_HIP(ctx->err, hipStreamSynchronize(p->stream));
   return G5_OK;
 }
 
@@ -384,15 +1041,48 @@ g5_outside_coords(const g5_ctx * ctx, int32_t * n_outside, int32_t * coords, siz
   return G5_OK;
 }
 
+// one godunov_unsplit iteration with a host dt
+static int
+host_step(g5_ctx * ctx, double dt)
+{
+  g5_pool * p = ctx->pool;
+  int       rc = rebuild_list(p);
+  if (rc != G5_OK)
+    return fail(ctx->err, rc, p->err);
+  if (ctx->cfg.integrator == G5_TI_RK2_SSP)
+  {
+    // stage 1: U_RK = U + dt L(U); stage 2: U2 = U_RK + dt L(U_RK); U = 1/2 U + 1/2 U2
+    rc = read_ctl(p);
+    if (rc != G5_OK)
+      return fail(ctx->err, rc, p->err);
+    const int par = p->h_ctl[ctx->id].parity;
+    StepArgs  A = step_args(p, p->d_blocks, p->nlist, dt);
+    A.U_out = p->stage;
+    launch_step(p, A, false);
+    A.U_in = p->stage;
+    A.U_out = p->buf[1 - par];
+    launch_step(p, A, false);
+    launch_lincomb(0.5, p->buf[par], 0.5, p->buf[1 - par], static_cast<size_t>(p->nslots) * p->block_doubles,
+                   p->stream);
+  }
+  else
+  {
+    launch_step(p, step_args(p, p->d_blocks, p->nlist, dt), ctx->cfg.integrator == G5_TI_HANCOCK);
+  }
+  launch_flip(p->d_ctl, 1, p->stream);
+  G5_HIP(ctx->err, hipGetLastError());
+  return G5_OK;
+}
+
 int
 g5_do_time_step(g5_ctx * ctx, double dt)
 {
   if (not ctx)
-    return fail(ctx, G5_ERR_INVALID, "null argument");
+    return G5_ERR_INVALID;
   if (not ctx->has_state)
-    return fail(ctx, G5_ERR_STATE, "no state uploaded");
+    return fail(ctx->err, G5_ERR_STATE, "no state uploaded");
   if (not(dt > 0.0))
-    return fail(ctx, G5_ERR_INVALID, "dt must be > 0");
+    return fail(ctx->err, G5_ERR_INVALID, "dt must be > 0");
   return host_step(ctx, dt);
 }
 
@@ -400,11 +1090,12 @@ int
 g5_run(g5_ctx * ctx, int32_t nsteps, double * t, double tEnd, int32_t * steps_done)
 {
   if (not ctx or not t)
-    return fail(ctx, G5_ERR_INVALID, "null argument");
+    return ctx ? fail(ctx->err, G5_ERR_INVALID, "null argument") : G5_ERR_INVALID;
   if (not ctx->has_state)
-    return fail(ctx, G5_ERR_STATE, "no state uploaded");
+    return fail(ctx->err, G5_ERR_STATE, "no state uploaded");
   if (nsteps < 0)
-    return fail(ctx, G5_ERR_INVALID, "nsteps < 0");
+    return fail(ctx->err, G5_ERR_INVALID, "nsteps < 0");
+  g5_pool * p = ctx->pool;
   if (ctx->cfg.integrator == G5_TI_RK2_SSP)
   {
     // host-driven: the stage structure needs the dt of the step before the first stage
@@ -426,38 +1117,25 @@ g5_run(g5_ctx * ctx, int32_t nsteps, double * t, double tEnd, int32_t * steps_do
       *steps_done = n;
     return g5_synchronize(ctx);
   }
+  int rc = read_ctl(p);
+  if (rc != G5_OK)
+    return fail(ctx->err, rc, p->err);
   SolveCtl c0{};
   c0.t = *t;
   c0.tend = tEnd;
   c0.max_steps = nsteps;
-  c0.active = 1;
-  G5_HIP(ctx, hipMemcpyAsync(ctx->d_ctl, &c0, sizeof(SolveCtl), hipMemcpyHostToDevice, ctx->stream));
-  launch_inv_dt(ctx->cfg.dim, ctx->dU[ctx->cur], ctx->d_blocks, ctx->d_solve, ctx->d_ctl, ctx->nblocks, ctx->stream);
-  launch_start(ctx->d_ctl, ctx->d_solve, 1, ctx->stream);
-  const bool hancock = ctx->cfg.integrator == G5_TI_HANCOCK;
-  hipEvent_t e0, e1;
-  G5_HIP(ctx, hipEventCreate(&e0));
-  G5_HIP(ctx, hipEventCreate(&e1));
-  G5_HIP(ctx, hipEventRecord(e0, ctx->stream));
-  for (int n = 0; n < nsteps; ++n)
-  {
-    launch_step(ctx, ctx->dU[ctx->cur], ctx->dU[1 - ctx->cur], 0.0, hancock);
-    launch_finalize(ctx->d_ctl, ctx->d_solve, 1, ctx->stream);
-    ctx->cur = 1 - ctx->cur;
-  }
-  G5_HIP(ctx, hipEventRecord(e1, ctx->stream));
-  G5_HIP(ctx, hipGetLastError());
-  G5_HIP(ctx, hipEventSynchronize(e1));
-  float ms = 0.f;
-  G5_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  ctx->last_run_ms = ms;
-  G5_HIP(ctx, hipMemcpyAsync(ctx->h_ctl, ctx->d_ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, ctx->stream));
-  G5_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  *t = ctx->h_ctl->t;
+  c0.parity = p->h_ctl[ctx->id].parity;
+  G5_HIP(ctx->err, hipMemcpyAsync(p->d_ctl + ctx->id, &c0, sizeof(SolveCtl), hipMemcpyHostToDevice, p->stream));
+  rc = pool_start(p, &ctx->id, 1);
+  if (rc == G5_OK)
+    rc = pool_step(p, nsteps);
+  if (rc == G5_OK)
+    rc = read_ctl(p);
+  if (rc != G5_OK)
+    return fail(ctx->err, rc, p->err);
+  *t = p->h_ctl[ctx->id].t;
   if (steps_done)
-    *steps_done = ctx->h_ctl->steps;
+    *steps_done = p->h_ctl[ctx->id].steps;
   return G5_OK;
 }
 
@@ -466,7 +1144,7 @@ g5_last_run_ms(const g5_ctx * ctx, double * ms)
 {
   if (not ctx or not ms)
     return G5_ERR_INVALID;
-  *ms = ctx->last_run_ms;
+  *ms = ctx->pool->last_run_ms;
   return G5_OK;
 }
 
@@ -474,9 +1152,9 @@ int
 g5_synchronize(g5_ctx * ctx)
 {
   if (not ctx)
-    return fail(ctx, G5_ERR_INVALID, "null argument");
-  G5_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  G5_HIP(ctx, hipGetLastError());
+    return G5_ERR_INVALID;
+  G5_HIP(ctx->err, hipStreamSynchronize(ctx->pool->stream));
+  G5_HIP(ctx->err, hipGetLastError());
   return G5_OK;
 }
 
@@ -484,25 +1162,29 @@ int
 g5_time_step_kernel(g5_ctx * ctx, int32_t nsteps, double dt, double * mean_ms)
 {
   if (not ctx or not mean_ms)
-    return fail(ctx, G5_ERR_INVALID, "null argument");
+    return ctx ? fail(ctx->err, G5_ERR_INVALID, "null argument") : G5_ERR_INVALID;
   if (not ctx->has_state)
-    return fail(ctx, G5_ERR_STATE, "no state uploaded");
+    return fail(ctx->err, G5_ERR_STATE, "no state uploaded");
   if (nsteps <= 0 or not(dt > 0.0))
-    return fail(ctx, G5_ERR_INVALID, "nsteps and dt must be > 0");
+    return fail(ctx->err, G5_ERR_INVALID, "nsteps and dt must be > 0");
+  g5_pool * p = ctx->pool;
+  int       rc = rebuild_list(p);
+  if (rc != G5_OK)
+    return fail(ctx->err, rc, p->err);
   hipEvent_t e0, e1;
-  G5_HIP(ctx, hipEventCreate(&e0));
-  G5_HIP(ctx, hipEventCreate(&e1));
+  G5_HIP(ctx->err, hipEventCreate(&e0));
+  G5_HIP(ctx->err, hipEventCreate(&e1));
   const bool hancock = ctx->cfg.integrator == G5_TI_HANCOCK;
-  G5_HIP(ctx, hipEventRecord(e0, ctx->stream));
+  G5_HIP(ctx->err, hipEventRecord(e0, p->stream));
   for (int n = 0; n < nsteps; ++n)
   {
-    launch_step(ctx, ctx->dU[ctx->cur], ctx->dU[1 - ctx->cur], dt, hancock);
-    ctx->cur = 1 - ctx->cur;
+    launch_step(p, step_args(p, p->d_blocks, p->nlist, dt), hancock);
+    launch_flip(p->d_ctl, 1, p->stream);
   }
-  G5_HIP(ctx, hipEventRecord(e1, ctx->stream));
-  G5_HIP(ctx, hipEventSynchronize(e1));
+  G5_HIP(ctx->err, hipEventRecord(e1, p->stream));
+  G5_HIP(ctx->err, hipEventSynchronize(e1));
   float ms = 0.f;
-  G5_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+  G5_HIP(ctx->err, hipEventElapsedTime(&ms, e0, e1));
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   *mean_ms = static_cast<double>(ms) / nsteps;

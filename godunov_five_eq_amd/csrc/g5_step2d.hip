@@ -250,21 +250,19 @@ k_step2d(StepArgs A)
   const BlockDesc bd = A.blocks[blk];
   const SolveDesc S = A.solves[bd.solve];
   SolveCtl *      C = A.ctl + bd.solve;
-  const int64_t   own = static_cast<int64_t>(blk) * 8 * 256 + t;
+  const int64_t   own = static_cast<int64_t>(bd.slot) * 8 * 256 + t;
   const bool      host_dt = A.dt_fixed > 0.0;
   if (!host_dt and C->active == 0)
-  {
-    // finished solve: carry the state over so both buffers hold it
-    for (int v = 0; v < 8; ++v)
-      A.U2[own + v * 256] = A.U[own + v * 256];
-    return;
-  }
-  const double dt = host_dt ? A.dt_fixed : C->dt;
+    return; // finished solve: its state stays in buffer `parity`
+  const int      par = C->parity;
+  double const * U = A.U_in ? A.U_in : (par ? A.buf1 : A.buf0);
+  double *       U2 = A.U_out ? A.U_out : (par ? A.buf0 : A.buf1);
+  const double   dt = host_dt ? A.dt_fixed : C->dt;
   const double dtdx = dt / S.dx;
   const int    i = t & 15, j = t >> 4;
 
   // ---- 1. gather + primitives ----
-  Cons uown = load_cons<2, B2>(A.U, own, -1);
+  Cons uown = load_cons<2, B2>(U, own, -1);
   {
     Prim q;
     cons_to_prim<2>(S, uown, q);
@@ -274,8 +272,8 @@ k_step2d(StepArgs A)
   {
     int gi, gj, neg;
     halo_cell2(t, gi, gj);
-    const int64_t off = resolve_cell<2, B2>(S, bd.bx, bd.by, 0, gi, gj, 0, neg);
-    const Cons    u = load_cons<2, B2>(A.U, off, neg);
+    const int64_t off = resolve_cell<2, B2>(S, A.slots, bd.bx, bd.by, 0, gi, gj, 0, neg);
+    const Cons    u = load_cons<2, B2>(U, off, neg);
     Prim          q;
     cons_to_prim<2>(S, u, q);
     sq_store2(sQ, tidx2(gi, gj), q);
@@ -336,7 +334,7 @@ k_step2d(StepArgs A)
   apply_flux_pair<2>(u2, Fx, fl_load2(FX, fx2(i + 1, j)), uown.a0, uown.a1);
   apply_flux_pair<2>(u2, Fy, fl_load2(FY, fy2(i, j + 1)), uown.a0, uown.a1);
   fix_volume_fractions(u2);
-  store_cons<2, B2>(A.U2, own, u2);
+  store_cons<2, B2>(U2, own, u2);
 
   // ---- 5. CFL of the new state ----
   double inv = wave_max(cell_inv_dt<2>(S, u2));

@@ -8,11 +8,13 @@
 namespace g5
 {
 
-// One solve (one brick of blocks) as the kernels see it.  Several solves can share one launch:
-// their blocks are flattened into one block list (BlockDesc) and each block carries its solve id.
+// One solve (one brick of blocks) as the kernels see it.  Many solves share one launch: their
+// blocks are flattened into one block list (BlockDesc) and each block carries its solve id.  The
+// state of block (x,y,z) of a solve lives in memory slot slots[base + x + nb0*(y + nb1*z)], so
+// solves can be admitted and released without moving other solves' data.
 struct SolveDesc
 {
-  int32_t base;        // index of the solve's first block in the flattened block list
+  int32_t base;        // index of the solve's first entry in the slot table
   int32_t nb[3];       // blocks per direction
   int32_t periodic[3]; // periodicity per direction
   int32_t bc[6];       // G5_BC_* per face
@@ -40,23 +42,27 @@ struct SolveCtl
   int32_t            active;     // 1 while t < tend and steps < max_steps
   int32_t            steps;      // steps done
   int32_t            max_steps;  // step budget
-  int32_t            parity;     // index (0/1) of the buffer holding the latest state
+  int32_t            parity;     // which of the two state buffers holds the current state
 };
 
-// block descriptor: solve id and brick coordinates
+// block of the launch list: solve id, memory slot, brick coordinates
 struct BlockDesc
 {
   int32_t solve;
-  int32_t bx, by, bz;
+  int32_t slot;
+  int16_t bx, by, bz, pad;
 };
 
 struct StepArgs
 {
-  const double *    U;   // state at time n (all blocks)
-  double *          U2;  // state at time n+1
+  double *          buf0;    // state buffer 0
+  double *          buf1;    // state buffer 1
+  const double *    U_in;    // non-null: read the step input here instead of buf[parity]
+  double *          U_out;   // non-null: write the step output here instead of buf[1 - parity]
   const BlockDesc * blocks;
   const SolveDesc * solves;
   SolveCtl *        ctl;
+  const int32_t *   slots;   // slot table
   int32_t           nblocks;
   int32_t           per_xcd; // workgroups per XCD chunk (grid = 8 * per_xcd)
   double            dt_fixed; // > 0: use this dt for every solve (host-driven step)
@@ -90,5 +96,25 @@ make_solve_desc(g5_config const & c, int32_t base, SolveDesc & s)
   s.thinc_beta = c.thinc_beta;
   s.thinc_eps = c.thinc_epsilon;
 }
+
+// Initial-condition description for the device generator (InitShockBubble / InitTriplePoint):
+// kind 0: region 0 for x <= x_front else region 1, with a region-2 sphere (circle) inclusion;
+// kind 1: same with an axis-aligned box (rectangle) inclusion;
+// kind 2: triple point, region 0 for x <= x_front, else region 1 where y <= yd
+//         (sqrt(y^2+z^2) <= yd in 3-D) and region 2 elsewhere.
+struct IcDesc
+{
+  int32_t kind;
+  int32_t pad;
+  double  x_front;
+  double  yd;
+  double  center[3];
+  double  radius;
+  double  half[3];
+  double  xmin[3];
+  double  S[3][9];  // conservative region states
+  double  eint_pre; // InitShockBubble.cpp:183-207 specific energies of regions 1 and 2
+  double  eint_bub;
+};
 
 } // namespace g5

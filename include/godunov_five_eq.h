@@ -189,6 +189,55 @@ int g5_init_shock_bubble(const g5_config * cfg, const g5_region * regions3, doub
 int g5_rankine_hugoniot(double gamma, double pinf, double rho1, double p1, double mach,
                         double p2_in, double * rho2, double * u2, double * p2, double * shock_speed);
 
+/* ---- pools: many independent solves stepped together (batching and serving) ---- */
+
+/* A pool holds the state of many independent solves of one dimension, THINC flag and precision
+ * in device memory sized by an HBM budget (two state copies).  The blocks of all live solves form
+ * one launch list; every solve keeps its own dt, time, step budget and buffer parity on the
+ * device, so solves can be admitted and released while others are in flight (continuous
+ * batching).  No counterpart in the reference (one solve per process there); the per-solve step
+ * is the same GodunovImplemV0::do_time_step + compute_dt_local iteration as g5_run. */
+typedef struct g5_pool g5_pool;
+
+int g5_pool_create(int32_t dim, int32_t thinc_enabled, int32_t precision, uint64_t hbm_budget_bytes,
+                   g5_pool ** out);
+int g5_pool_destroy(g5_pool * pool);
+const char * g5_pool_last_error(const g5_pool * pool);
+
+/* reserve memory for one solve (G5_ERR_NOMEM when the budget has no room); it starts inactive */
+int g5_pool_admit(g5_pool * pool, const g5_config * cfg, double t_end, int32_t max_steps, int32_t * solve_id);
+
+/* free the solve's memory and id */
+int g5_pool_release(g5_pool * pool, int32_t solve_id);
+
+/* state of one solve in the g5_upload_state layout */
+int g5_pool_upload(g5_pool * pool, int32_t solve_id, const double * U, size_t n);
+int g5_pool_download(g5_pool * pool, int32_t solve_id, double * U, size_t n);
+
+/* device initial conditions (init/InitShockBubble.cpp:50-225, init/InitTriplePoint.cpp:46-133):
+ * shape 0: sphere/circle of radius size3[0] at center3; shape 1: box of half-sizes size3 */
+int g5_pool_init_shock_bubble(g5_pool * pool, int32_t solve_id, const g5_region * regions3, int32_t shape,
+                              double x_front, const double * center3, const double * size3);
+int g5_pool_init_triple_point(g5_pool * pool, int32_t solve_id, const g5_region * regions3, double xd, double yd);
+
+/* compute the first dt of solves whose state is set and mark them active */
+int g5_pool_start(g5_pool * pool, const int32_t * solve_ids, int32_t n);
+
+/* nsteps device-dt steps of every active solve (one HIP graph per step: fused step + finalise) */
+int g5_pool_step(g5_pool * pool, int32_t nsteps);
+
+/* time, steps done and activity of one solve (synchronises) */
+int g5_pool_status(g5_pool * pool, int32_t solve_id, double * t, int32_t * steps, int32_t * active);
+
+/* live solves that reached t_end or their step budget: up to cap ids, *n_done = their number */
+int g5_pool_finished(g5_pool * pool, int32_t * solve_ids, int32_t cap, int32_t * n_done);
+
+/* blocks in use, block capacity of the budget, live solves */
+int g5_pool_usage(const g5_pool * pool, int64_t * used_blocks, int64_t * capacity_blocks, int32_t * live_solves);
+
+/* HIP-event duration of the last g5_pool_step call (ms) */
+int g5_pool_last_step_ms(const g5_pool * pool, double * ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,146 @@
+"""Batched solves in one pool (the batching and serving forms): each solve is bit-identical to
+the same solve run alone and to the oracle, whatever else shares the launch, including solves
+admitted while others are in flight; memory is reused after release; the budget is enforced;
+the device initial condition equals the host one."""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def mixed_2d_solves(g5):
+    a = cases.air_helium_2d(nbx=8, nby=4)
+    b = cases.air_helium_2d(nbx=5, nby=3, slope_type=2.0, periodic=(1, 0, 0),
+                            bc=("PERIODIC", "PERIODIC", "WALL", "WALL", "NONE", "NONE"))
+    rho2, u2, p2, _ = g5.rankine_hugoniot(4.4, 6e8, 1000.0, 1e5, p2=1e9)
+    cfg = g5.make_config(2, (6, 6), 1.0 / 96, bc=("ZERO_GRADIENT",) * 4 + ("NONE", "NONE"), cfl=0.5,
+                         gamma=(1.4, 4.4), pinf=(0.0, 6e8))
+    regs = [g5.region(0.0, (0.0, rho2), p2, u2), g5.region(0.0, (0.0, 1000.0), 1e5),
+            g5.region(1.0, (1.0, 0.0), 1e5)]
+    c = (cfg, g5.init_shock_bubble(cfg, regs, 0.25, (0.5, 0.5), 0.2))
+    return [a, b, c]
+
+
+def oracle_run(orc, cfg, U0, n):
+    o = orc.Oracle(cfg)
+    o.set_state(U0)
+    o.run(n)
+    return o.get_state()
+
+
+def test_pool_mix_matches_alone_and_oracle(g5, orc):
+    say("pool: three 2-D solves in one launch")
+    solves = mixed_2d_solves(g5)
+    nsteps = 12
+    with g5.Pool(2, budget_bytes=1 << 28) as pool:
+        ids = []
+        for cfg, U0 in solves:
+            sid = pool.admit(cfg, max_steps=nsteps)
+            pool.upload(sid, U0)
+            ids.append(sid)
+        pool.start(ids)
+        pool.step(nsteps + 3)  # extra steps: finished solves must not move
+        for sid, (cfg, U0) in zip(ids, solves):
+            t, n, active = pool.status(sid)
+            assert n == nsteps and not active
+            U = pool.download(sid)
+            assert np.array_equal(U, oracle_run(orc, cfg, U0, nsteps)), sid
+    for cfg, U0 in solves[:1]:
+        with g5.Context(cfg) as ctx:
+            ctx.upload(U0)
+            ctx.run(nsteps)
+            assert np.array_equal(ctx.download(), oracle_run(orc, cfg, U0, nsteps))
+
+
+def test_pool_admission_in_flight_and_reuse(g5, orc):
+    say("pool: admission while another solve runs, release and reuse")
+    (ca, Ua), (cb, Ub), (cc, Uc) = mixed_2d_solves(g5)
+    na, nb_, nc = (c.nb[0] * c.nb[1] for c in (ca, cb, cc))
+    with g5.Pool(2, budget_bytes=max(na + nb_, nb_ + nc) * 2 * 2048 * 8) as pool:
+        a = pool.admit(ca, max_steps=10)
+        pool.upload(a, Ua)
+        pool.start([a])
+        pool.step(4)
+        b = pool.admit(cb, max_steps=6)     # joins while a is in flight
+        pool.upload(b, Ub)
+        pool.start([b])
+        assert pool.try_admit(cc) is None   # budget full
+        pool.step(6)
+        assert sorted(pool.finished()) == sorted([a, b])
+        assert np.array_equal(pool.download(a), oracle_run(orc, ca, Ua, 10))
+        assert np.array_equal(pool.download(b), oracle_run(orc, cb, Ub, 6))
+        pool.release(a)
+        c = pool.admit(cc, max_steps=5)     # reuses a's memory (scattered slots)
+        pool.upload(c, Uc)
+        pool.start([c])
+        pool.step(5)
+        assert np.array_equal(pool.download(c), oracle_run(orc, cc, Uc, 5))
+        assert np.array_equal(pool.download(b), oracle_run(orc, cb, Ub, 6))  # untouched while finished
+
+
+def test_pool_3d_mix(g5, orc):
+    say("pool: 3-D THINC solves")
+    a = cases.air_water_3d(nb=2)
+    b = cases.air_water_3d(nb=3)
+    with g5.Pool(3, thinc=True, budget_bytes=1 << 30) as pool:
+        ids = []
+        for cfg, U0 in (a, b):
+            sid = pool.admit(cfg, max_steps=4)
+            pool.upload(sid, U0)
+            ids.append(sid)
+        pool.start(ids)
+        pool.step(4)
+        for sid, (cfg, U0) in zip(ids, (a, b)):
+            assert np.array_equal(pool.download(sid), oracle_run(orc, cfg, U0, 4))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_device_initial_condition_matches_host(g5, dim):
+    say(f"device IC dim={dim}")
+    cfg, U0 = cases.air_helium_2d(nbx=4, nby=3) if dim == 2 else cases.air_water_3d(nb=2)
+    if dim == 2:
+        rho2, u2, p2, _ = g5.rankine_hugoniot(1.4, 0.0, 1.0, 1.0, mach=1.22)
+        regs = [g5.region(1.0, (rho2, 0.0), p2, u2), g5.region(1.0, (1.0, 0.0), 1.0),
+                g5.region(0.0, (0.0, 0.138), 1.0)]
+        args = (0.6, (1.0, 0.5 * 3 * 16 * cfg.dx), 0.25)
+    else:
+        rho2, u2, p2, _ = g5.rankine_hugoniot(4.4, 6e8, 1000.0, 1e5, p2=1e9)
+        regs = [g5.region(0.0, (0.0, rho2), p2, u2), g5.region(0.0, (0.0, 1000.0), 1e5),
+                g5.region(1.0, (1.0, 0.0), 1e5)]
+        args = (0.25, (0.5, 0.5, 0.5), 0.2)
+    host = g5.init_shock_bubble(cfg, regs, *args)
+    with g5.Pool(dim, thinc=bool(cfg.thinc_enabled), budget_bytes=1 << 28) as pool:
+        sid = pool.admit(cfg)
+        pool.init_shock_bubble(sid, regs, args[0], args[1], args[2], shape=0)
+        dev = pool.download(sid)
+    assert np.array_equal(dev, host)
+
+
+def test_device_triple_point_and_box(g5):
+    say("device IC triple point / box")
+    cfg = g5.make_config(2, (7, 3), 1.0 / 48, gamma=(1.5, 1.4))
+    regs = [g5.region(1.0, (1.0, 0.0), 1.0), g5.region(1.0, (1.0, 0.0), 0.1), g5.region(0.0, (0.0, 0.125), 0.1)]
+    with g5.Pool(2, budget_bytes=1 << 26) as pool:
+        sid = pool.admit(cfg)
+        pool.init_triple_point(sid, regs, 1.0, 1.5 / 2)
+        U = pool.download(sid).reshape(7 * 3, 8, 256)
+        x = np.zeros((7 * 3, 256)); y = np.zeros((7 * 3, 256))
+        for b in range(21):
+            bx, by = b % 7, b // 7
+            for c in range(256):
+                x[b, c] = (bx * 16 + c % 16 + 0.5) / 48
+                y[b, c] = (by * 16 + c // 16 + 0.5) / 48
+        a0 = U[:, 4]
+        assert np.all(a0[x <= 1.0] == 1.0) and np.all(a0[(x > 1.0) & (y <= 0.75)] == 1.0)
+        assert np.all(a0[(x > 1.0) & (y > 0.75)] == 0.0)
+        sid2 = pool.admit(cfg)
+        pool.init_shock_bubble(sid2, regs, 0.3, (1.5, 0.5), (0.25, 0.2), shape=1)
+        V = pool.download(sid2).reshape(21, 8, 256)
+        area = (1 - V[:, 4]).sum() / 48 ** 2
+        assert area == pytest.approx(0.5 * 0.4, rel=1e-12)
