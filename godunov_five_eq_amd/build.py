@@ -22,6 +22,9 @@ ARCH = os.environ.get("G5_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FP32 division/sqrt of the mixed-precision path use the hardware reciprocal and square root
+    # (FP64 division and sqrt stay IEEE correctly rounded)
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]

@@ -14,8 +14,9 @@ cur_buf(double * b0, double * b1, int parity)
 
 // ComputeDtFiveEqFunctor (scheme/ComputeDtFiveEqFunctor.cpp:45-134): one workgroup per block,
 // per-cell CFL rate, wavefront butterfly max (__shfl_xor), LDS max over the 4 wavefronts,
-// one atomicMax per block on the bit pattern of the (non-negative) double.
-template <int DIM>
+// one atomicMax per block on the bit pattern of the (non-negative) double.  T selects the FP64
+// rate or the mixed-path rate (FP32 sound speed from the FP64-recovered pressure).
+template <int DIM, typename T>
 __global__ void __launch_bounds__(256)
 k_inv_dt(double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves, SolveCtl * ctl,
          int nblocks)
@@ -33,7 +34,7 @@ k_inv_dt(double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const
   for (int c = threadIdx.x; c < NC; c += 256)
   {
     const Cons u = load_cons<DIM, 16>(U, static_cast<int64_t>(bd.slot) * NV * NC + c, -1);
-    inv = fmax(inv, cell_inv_dt<DIM>(S, u));
+    inv = fmax(inv, cell_rate<DIM, T>(S, u));
   }
   inv = wave_max(inv);
   if ((threadIdx.x & 63) == 0)
@@ -47,15 +48,26 @@ k_inv_dt(double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const
 }
 
 void
-launch_inv_dt(int dim, double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves,
+launch_inv_dt(int dim, bool mixed, double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves,
               SolveCtl * ctl, int nblocks, hipStream_t stream)
 {
   if (nblocks <= 0)
     return;
+  const dim3 grid(nblocks), block(256);
   if (dim == 2)
-    hipLaunchKernelGGL(k_inv_dt<2>, dim3(nblocks), dim3(256), 0, stream, buf0, buf1, blocks, solves, ctl, nblocks);
+  {
+    if (mixed)
+      hipLaunchKernelGGL((k_inv_dt<2, float>), grid, block, 0, stream, buf0, buf1, blocks, solves, ctl, nblocks);
+    else
+      hipLaunchKernelGGL((k_inv_dt<2, double>), grid, block, 0, stream, buf0, buf1, blocks, solves, ctl, nblocks);
+  }
   else
-    hipLaunchKernelGGL(k_inv_dt<3>, dim3(nblocks), dim3(256), 0, stream, buf0, buf1, blocks, solves, ctl, nblocks);
+  {
+    if (mixed)
+      hipLaunchKernelGGL((k_inv_dt<3, float>), grid, block, 0, stream, buf0, buf1, blocks, solves, ctl, nblocks);
+    else
+      hipLaunchKernelGGL((k_inv_dt<3, double>), grid, block, 0, stream, buf0, buf1, blocks, solves, ctl, nblocks);
+  }
 }
 
 // SolverGodunovFiveEq::compute_dt_local (SolverGodunovFiveEq.cpp:386): dt = cfl / invDt, then

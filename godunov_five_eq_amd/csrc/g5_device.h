@@ -1,8 +1,14 @@
-// g5_device.h -- device-side physics of the five-equation step (FP64 path).
+// g5_device.h -- device-side physics of the five-equation step (FP64 and mixed paths).
 //
 // Each function restates one reference function; the FP64 operation order is the reference's
 // (the file is compiled with -ffp-contract=off, and FP64 division and sqrt are IEEE correctly
 // rounded on gfx950), so results are reproducible to the bit on the host.
+//
+// The reconstruction functions are templates on the arithmetic T of the primitive/face path.
+// T = double is the FP64 path.  T = float is the mixed path: slopes, Hancock terms, THINC (with a
+// hardware-exponential tanh), Youngs normals and the HLLC wave speeds run in FP32 (FP32 division
+// and sqrt are the fast approximate forms), while the pressure recovery from the total energy,
+// the HLLC energy flux, the conservative update and the dt reduction stay FP64.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,40 +29,80 @@ struct Cons // conservative state, models/FiveEq.h:35-50 order
   double d, e, mu, mv, mw, a0, ar0, a1, ar1;
 };
 
-struct Prim // the primitive values the flux path reads (get_prim_variables .h:196-241)
+// T is the arithmetic of the primitive/face path: double for the FP64 path, float for the mixed
+// path (the conservative state and the fluxes added to it stay double in both)
+template <typename T>
+struct PrimT // the primitive values the flux path reads (get_prim_variables .h:196-241)
 {
-  double r0, r1, a, p, u, v, w;
+  T r0, r1, a, p, u, v, w;
 };
 
-struct Slope // limited slopes of the reconstructed variables
+template <typename T>
+struct SlopeT // limited slopes of the reconstructed variables
 {
-  double r0, r1, p, u, v, w;
+  T r0, r1, p, u, v, w;
 };
 
-struct Face // reconstructed face state (IA0 in a)
+template <typename T>
+struct FaceT // reconstructed face state (IA0 in a)
 {
-  double r0, r1, a, p, u, v, w;
+  T r0, r1, a, p, u, v, w;
 };
 
-struct Flux // face flux scaled by dt/dx (set_flux, ComputeFluxesAndStoreFunctor.h:253-307)
-{
-  double ar0, ar1, a0, a1, e, mu, mv, mw, us;
+using Prim = PrimT<double>;
+using Slope = SlopeT<double>;
+using Face = FaceT<double>;
+
+template <typename T>
+struct FluxT // face flux scaled by dt/dx (set_flux, ComputeFluxesAndStoreFunctor.h:253-307); the
+{            // energy flux stays double in both paths (it carries rho e ~ Pi, large when pinf is)
+  T      ar0, ar1, a0, a1;
+  double e;
+  T      mu, mv, mw, us;
 };
+using Flux = FluxT<double>;
 
 // ---------------------------------------------------------------------------------------------
 // mixture stiffened-gas EOS (Allaire et al. 2002 isobaric closure):
 //   xi = sum alpha_k/(gamma_k-1), Pi = sum alpha_k gamma_k pinf_k/(gamma_k-1)
 //   p = (rho e - Pi)/xi ; e = (p xi + Pi)/rho ; rho c^2 = ((xi+1) p + Pi)/xi
 // ---------------------------------------------------------------------------------------------
-G5D double
-eos_xi(SolveDesc const & S, double a0, double a1)
+template <typename T>
+struct Par; // solve parameters in the arithmetic T
+template <>
+struct Par<double>
 {
-  return a0 * S.g1[0] + a1 * S.g1[1];
+  G5D static double g1(SolveDesc const & S, int k) { return S.g1[k]; }
+  G5D static double gp(SolveDesc const & S, int k) { return S.gp[k]; }
+  G5D static double smallr(SolveDesc const & S) { return S.smallr; }
+  G5D static double smallp(SolveDesc const & S) { return S.smallp; }
+  G5D static double smallc(SolveDesc const & S) { return S.smallc; }
+  G5D static double beta(SolveDesc const & S) { return S.thinc_beta; }
+  G5D static double eps(SolveDesc const & S) { return S.thinc_eps; }
+};
+template <>
+struct Par<float>
+{
+  G5D static float g1(SolveDesc const & S, int k) { return S.g1f[k]; }
+  G5D static float gp(SolveDesc const & S, int k) { return S.gpf[k]; }
+  G5D static float smallr(SolveDesc const & S) { return S.smallrf; }
+  G5D static float smallp(SolveDesc const & S) { return S.smallpf; }
+  G5D static float smallc(SolveDesc const & S) { return S.smallcf; }
+  G5D static float beta(SolveDesc const & S) { return S.thinc_betaf; }
+  G5D static float eps(SolveDesc const & S) { return S.thinc_epsf; }
+};
+
+template <typename T>
+G5D T
+eos_xi(SolveDesc const & S, T a0, T a1)
+{
+  return a0 * Par<T>::g1(S, 0) + a1 * Par<T>::g1(S, 1);
 }
-G5D double
-eos_pi(SolveDesc const & S, double a0, double a1)
+template <typename T>
+G5D T
+eos_pi(SolveDesc const & S, T a0, T a1)
 {
-  return a0 * S.gp[0] + a1 * S.gp[1];
+  return a0 * Par<T>::gp(S, 0) + a1 * Par<T>::gp(S, 1);
 }
 G5D double
 eos_pressure(SolveDesc const & S, double rho, double eint, double a0, double a1)
@@ -68,11 +114,12 @@ eos_eint(SolveDesc const & S, double rho, double p, double a0, double a1)
 {
   return (p * eos_xi(S, a0, a1) + eos_pi(S, a0, a1)) / rho;
 }
-G5D double
-eos_sound_speed(SolveDesc const & S, double rho, double p, double a0, double a1)
+template <typename T>
+G5D T
+eos_sound_speed(SolveDesc const & S, T rho, T p, T a0, T a1)
 {
-  const double x = eos_xi(S, a0, a1);
-  const double rc2 = ((x + 1.0) * p + eos_pi(S, a0, a1)) / x;
+  const T x = eos_xi(S, a0, a1);
+  const T rc2 = ((x + T(1.0)) * p + eos_pi(S, a0, a1)) / x;
   return sqrt(rc2 / rho);
 }
 
@@ -121,28 +168,85 @@ cons_to_prim(SolveDesc const & S, Cons const & U, Prim & q)
   return eos_sound_speed(S, d, p, a, 1.0 - a);
 }
 
-// scheme/ComputeLimitedSlopesFunctor.cpp:114-137 slope_unsplit_scalar
-G5D double
-slope1(double q, double qp, double qm, double slope_type)
+// Mixed-precision conversion: the pressure is recovered in FP64 from the total energy
+// (p xi = rho e_int - Pi cancels Pi ~ 1e9 against p xi ~ 1e4 when pinf ~ 6e8), then every
+// primitive is rounded to FP32.  Same sanitising as cons_to_prim; returns the FP32 sound speed.
+template <int DIM>
+G5D float
+cons_to_prim_mixed(SolveDesc const & S, Cons const & U, PrimT<float> & q)
 {
-  double dq = 0.0;
-  if (slope_type == 1.0 or slope_type == 2.0)
+  const double d = fmax(U.d, S.smallr);
+  const double inv_d = 1.0 / d;
+  const double ux = U.mu * inv_d;
+  const double uy = U.mv * inv_d;
+  double       m2 = U.mu * ux + U.mv * uy;
+  double       uz = 0.0;
+  if (DIM == 3)
   {
-    const double dlft = slope_type * (q - qm);
-    const double drgt = slope_type * (qp - q);
-    const double dcen = 0.5 * (qp - qm);
-    const double dsgn = (dcen >= 0.0) ? 1.0 : -1.0;
-    const double slop = fmin(fabs(dlft), fabs(drgt));
-    const double dlim = (dlft * drgt) <= 0.0 ? 0.0 : slop;
+    uz = U.mw * inv_d;
+    m2 += U.mw * uz;
+  }
+  const double pxi = (U.e - 0.5 * m2) - eos_pi(S, U.a0, 1.0 - U.a0);
+  const float  p = static_cast<float>(pxi) / static_cast<float>(eos_xi(S, U.a0, 1.0 - U.a0));
+  double       a = fmax(0.0, fmin(1.0, U.a0));
+  double       r0 = U.ar0, r1 = U.ar1;
+  const double eps = 1e-12;
+  if (r0 < eps)
+    r0 = 0.0;
+  if (r1 < eps)
+    r1 = 0.0;
+  if (a < eps)
+    a = 0.0;
+  if (a > 1.0 - eps)
+    a = 1.0;
+  q.r0 = static_cast<float>(r0);
+  q.r1 = static_cast<float>(r1);
+  q.a = static_cast<float>(a);
+  q.p = p;
+  q.u = static_cast<float>(ux);
+  q.v = static_cast<float>(uy);
+  q.w = static_cast<float>(uz);
+  return eos_sound_speed(S, static_cast<float>(d), p, q.a, 1.0f - q.a);
+}
+
+// one name for both paths
+template <int DIM>
+G5D double
+to_prim(SolveDesc const & S, Cons const & U, PrimT<double> & q)
+{
+  return cons_to_prim<DIM>(S, U, q);
+}
+template <int DIM>
+G5D float
+to_prim(SolveDesc const & S, Cons const & U, PrimT<float> & q)
+{
+  return cons_to_prim_mixed<DIM>(S, U, q);
+}
+
+// scheme/ComputeLimitedSlopesFunctor.cpp:114-137 slope_unsplit_scalar
+template <typename T>
+G5D T
+slope1(T q, T qp, T qm, T slope_type)
+{
+  T dq = T(0.0);
+  if (slope_type == T(1.0) or slope_type == T(2.0))
+  {
+    const T dlft = slope_type * (q - qm);
+    const T drgt = slope_type * (qp - q);
+    const T dcen = T(0.5) * (qp - qm);
+    const T dsgn = (dcen >= T(0.0)) ? T(1.0) : T(-1.0);
+    const T slop = fmin(fabs(dlft), fabs(drgt));
+    const T dlim = (dlft * drgt) <= T(0.0) ? T(0.0) : slop;
     dq = dsgn * fmin(dlim, fabs(dcen));
   }
   return dq;
 }
 
-G5D Slope
-slopes_of(Prim const & c, Prim const & p, Prim const & m, double st)
+template <typename T>
+G5D SlopeT<T>
+slopes_of(PrimT<T> const & c, PrimT<T> const & p, PrimT<T> const & m, T st)
 {
-  Slope s;
+  SlopeT<T> s;
   s.r0 = slope1(c.r0, p.r0, m.r0, st);
   s.r1 = slope1(c.r1, p.r1, m.r1, st);
   s.p = slope1(c.p, p.p, m.p, st);
@@ -158,86 +262,86 @@ slopes_of(Prim const & c, Prim const & p, Prim const & m, double st)
 // source terms depend on the cell only, so they are computed once per cell and reused by every
 // face of the cell (the reference recomputes the same values per face).
 // ---------------------------------------------------------------------------------------------
-struct Hancock // 0.5 * (sr0_0, sr0_1, sp0, su0, sv0, sw0)
+template <typename T>
+struct HancockT // 0.5 * (sr0_0, sr0_1, sp0, su0, sv0, sw0)
 {
-  double r0, r1, p, u, v, w;
+  T r0, r1, p, u, v, w;
 };
+using Hancock = HancockT<double>;
 
-template <int DIM>
-G5D Hancock
-hancock_terms(SolveDesc const & S, Prim const & q, Slope const & sx, Slope const & sy, Slope const & sz,
-              double dtdx)
+template <int DIM, typename T>
+G5D HancockT<T>
+hancock_terms(SolveDesc const & S, PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy,
+              SlopeT<T> const & sz, T dtdx)
 {
-  Hancock      h;
-  const double r = q.r0 + q.r1;
-  const double c = eos_sound_speed(S, r, q.p, q.a, 1.0 - q.a);
-  const double dtdy = dtdx, dtdz = dtdx;
-  const double uu = q.u, vv = q.v, ww = q.w;
+  HancockT<T> h;
+  const T     r = q.r0 + q.r1;
+  const T     c = eos_sound_speed(S, r, q.p, q.a, T(1.0) - q.a);
+  const T     dtdy = dtdx, dtdz = dtdx;
+  const T     uu = q.u, vv = q.v, ww = q.w;
   if (DIM == 2)
   {
-    const double sr0 =
-      (-uu * (sx.r0 + sx.r1) - sx.u * r) * dtdx + (-vv * (sy.r0 + sy.r1) - sy.v * r) * dtdy;
-    const double sr0_0 = sr0 * q.a;
-    const double sr0_1 = sr0 * (1.0 - q.a);
-    const double su0 = (-uu * sx.u - sx.p / r) * dtdx + (-vv * sy.u) * dtdy;
-    const double sv0 = (-uu * sx.v) * dtdx + (-vv * sy.v - sy.p / r) * dtdy;
-    const double sp0 = (-uu * sx.p - sx.u * r * c * c) * dtdx + (-vv * sy.p - sy.v * r * c * c) * dtdy;
-    h.r0 = 0.5 * sr0_0;
-    h.r1 = 0.5 * sr0_1;
-    h.p = 0.5 * sp0;
-    h.u = 0.5 * su0;
-    h.v = 0.5 * sv0;
-    h.w = 0.0;
+    const T sr0 = (-uu * (sx.r0 + sx.r1) - sx.u * r) * dtdx + (-vv * (sy.r0 + sy.r1) - sy.v * r) * dtdy;
+    const T sr0_0 = sr0 * q.a;
+    const T sr0_1 = sr0 * (T(1.0) - q.a);
+    const T su0 = (-uu * sx.u - sx.p / r) * dtdx + (-vv * sy.u) * dtdy;
+    const T sv0 = (-uu * sx.v) * dtdx + (-vv * sy.v - sy.p / r) * dtdy;
+    const T sp0 = (-uu * sx.p - sx.u * r * c * c) * dtdx + (-vv * sy.p - sy.v * r * c * c) * dtdy;
+    h.r0 = T(0.5) * sr0_0;
+    h.r1 = T(0.5) * sr0_1;
+    h.p = T(0.5) * sp0;
+    h.u = T(0.5) * su0;
+    h.v = T(0.5) * sv0;
+    h.w = T(0.0);
   }
   else
   {
-    const double sr0 = (-uu * (sx.r0 + sx.r1) - sx.u * r) * dtdx +
-                       (-vv * (sy.r0 + sy.r1) - sy.v * r) * dtdy +
-                       (-ww * (sz.r0 + sz.r1) - sz.w * r) * dtdz;
-    const double sr0_0 = sr0 * q.a;
-    const double sr0_1 = sr0 * (1.0 - q.a);
-    const double su0 = (-uu * sx.u - sx.p / r) * dtdx + (-vv * sy.u) * dtdy + (-ww * sz.u) * dtdz;
-    const double sv0 = (-uu * sx.v) * dtdx + (-vv * sy.v - sy.p / r) * dtdy + (-ww * sz.v) * dtdz;
-    const double sw0 = (-uu * sx.w) * dtdx + (-vv * sy.w) * dtdy + (-ww * sz.w - sz.p / r) * dtdz;
-    const double sp0 = (-uu * sx.p - sx.u * r * c * c) * dtdx + (-vv * sy.p - sy.v * r * c * c) * dtdy +
-                       (-ww * sz.p - sz.w * r * c * c) * dtdz;
-    h.r0 = 0.5 * sr0_0;
-    h.r1 = 0.5 * sr0_1;
-    h.p = 0.5 * sp0;
-    h.u = 0.5 * su0;
-    h.v = 0.5 * sv0;
-    h.w = 0.5 * sw0;
+    const T sr0 = (-uu * (sx.r0 + sx.r1) - sx.u * r) * dtdx + (-vv * (sy.r0 + sy.r1) - sy.v * r) * dtdy +
+                  (-ww * (sz.r0 + sz.r1) - sz.w * r) * dtdz;
+    const T sr0_0 = sr0 * q.a;
+    const T sr0_1 = sr0 * (T(1.0) - q.a);
+    const T su0 = (-uu * sx.u - sx.p / r) * dtdx + (-vv * sy.u) * dtdy + (-ww * sz.u) * dtdz;
+    const T sv0 = (-uu * sx.v) * dtdx + (-vv * sy.v - sy.p / r) * dtdy + (-ww * sz.v) * dtdz;
+    const T sw0 = (-uu * sx.w) * dtdx + (-vv * sy.w) * dtdy + (-ww * sz.w - sz.p / r) * dtdz;
+    const T sp0 = (-uu * sx.p - sx.u * r * c * c) * dtdx + (-vv * sy.p - sy.v * r * c * c) * dtdy +
+                  (-ww * sz.p - sz.w * r * c * c) * dtdz;
+    h.r0 = T(0.5) * sr0_0;
+    h.r1 = T(0.5) * sr0_1;
+    h.p = T(0.5) * sp0;
+    h.u = T(0.5) * su0;
+    h.v = T(0.5) * sv0;
+    h.w = T(0.5) * sw0;
   }
   return h;
 }
 
 // q + off_x*sx + off_y*sy (+ off_z*sz), one offset +-0.5 on axis AX, the others 0.0 (kept so that
 // the operation sequence is the reference's).
-template <int DIM, int AX>
-G5D double
-muscl_extrap(double qv, double sxv, double syv, double szv, double o)
+template <int DIM, int AX, typename T>
+G5D T
+muscl_extrap(T qv, T sxv, T syv, T szv, T o)
 {
-  const double ox = AX == 0 ? o : 0.0;
-  const double oy = AX == 1 ? o : 0.0;
-  const double oz = AX == 2 ? o : 0.0;
+  const T ox = AX == 0 ? o : T(0.0);
+  const T oy = AX == 1 ? o : T(0.0);
+  const T oz = AX == 2 ? o : T(0.0);
   if (DIM == 2)
     return qv + ox * sxv + oy * syv;
   return qv + ox * sxv + oy * syv + oz * szv;
 }
 
 // reconstruct_state_{2d,3d} face part (scheme/ComputeFluxesAndStoreFunctor.cpp:164-211)
-template <int DIM, int AX, bool HANCOCK>
-G5D Face
-face_muscl(SolveDesc const & S, Prim const & q, Slope const & sx, Slope const & sy, Slope const & sz,
-           Hancock const & h, double o)
+template <int DIM, int AX, bool HANCOCK, typename T>
+G5D FaceT<T>
+face_muscl(SolveDesc const & S, PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy,
+           SlopeT<T> const & sz, HancockT<T> const & h, T o)
 {
-  Face f;
+  FaceT<T> f;
   f.r0 = muscl_extrap<DIM, AX>(q.r0, sx.r0, sy.r0, sz.r0, o);
   f.r1 = muscl_extrap<DIM, AX>(q.r1, sx.r1, sy.r1, sz.r1, o);
   f.p = muscl_extrap<DIM, AX>(q.p, sx.p, sy.p, sz.p, o);
   f.u = muscl_extrap<DIM, AX>(q.u, sx.u, sy.u, sz.u, o);
   f.v = muscl_extrap<DIM, AX>(q.v, sx.v, sy.v, sz.v, o);
-  f.w = DIM == 3 ? muscl_extrap<DIM, AX>(q.w, sx.w, sy.w, sz.w, o) : 0.0;
+  f.w = DIM == 3 ? muscl_extrap<DIM, AX>(q.w, sx.w, sy.w, sz.w, o) : T(0.0);
   if (HANCOCK)
   {
     f.r0 += h.r0;
@@ -248,8 +352,8 @@ face_muscl(SolveDesc const & S, Prim const & q, Slope const & sx, Slope const & 
     if (DIM == 3)
       f.w += h.w;
   }
-  f.r0 = fmax(S.smallr, f.r0);
-  f.r1 = fmax(S.smallr, f.r1);
+  f.r0 = fmax(Par<T>::smallr(S), f.r0);
+  f.r1 = fmax(Par<T>::smallr(S), f.r1);
   f.a = q.a;
   return f;
 }
@@ -299,41 +403,61 @@ det_tanh(double x)
   return x < 0 ? -t : t;
 }
 
+// FP32 tanh of the mixed path: 1 - 2/(exp(2|x|)+1) with the hardware exponential
+G5D float
+fast_tanh(float x)
+{
+  const float e = __expf(-2.0f * fabsf(x));
+  const float t = (1.0f - e) / (1.0f + e);
+  return x < 0.0f ? -t : t;
+}
+G5D double
+tanh_of(double x)
+{
+  return det_tanh(x);
+}
+G5D float
+tanh_of(float x)
+{
+  return fast_tanh(x);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Youngs (1982) normal from the 3^d alpha stencil (same weights and order as the host oracle).
 // ---------------------------------------------------------------------------------------------
+template <typename T>
 G5D void
-youngs2(double const (&a)[9], double & nx, double & ny)
+youngs2(T const (&a)[9], T & nx, T & ny)
 {
   // a[(di+1) + 3*(dj+1)]
-  const double gx = ((a[2] + 2.0 * a[5]) + a[8]) - ((a[0] + 2.0 * a[3]) + a[6]);
-  const double gy = ((a[6] + 2.0 * a[7]) + a[8]) - ((a[0] + 2.0 * a[1]) + a[2]);
-  const double nrm = sqrt(gx * gx + gy * gy);
-  if (nrm > 0.0)
+  const T gx = ((a[2] + T(2.0) * a[5]) + a[8]) - ((a[0] + T(2.0) * a[3]) + a[6]);
+  const T gy = ((a[6] + T(2.0) * a[7]) + a[8]) - ((a[0] + T(2.0) * a[1]) + a[2]);
+  const T nrm = sqrt(gx * gx + gy * gy);
+  if (nrm > T(0.0))
   {
     nx = gx / nrm;
     ny = gy / nrm;
   }
   else
   {
-    nx = 0.0;
-    ny = 0.0;
+    nx = T(0.0);
+    ny = T(0.0);
   }
 }
 
 // 3-D: a[(di+1) + 3*(dj+1) + 9*(dk+1)]; plane sums in the order t2 outer, t1 inner with
 // o[d]=s, o[(d+1)%3]=t1, o[(d+2)%3]=t2 and weights (1,2,1)x(1,2,1)
-template <int D>
-G5D double
-youngs_plane(double const (&a)[27], int s)
+template <int D, typename T>
+G5D T
+youngs_plane(T const (&a)[27], int s)
 {
-  double acc = 0.0;
+  T acc = T(0.0);
 #pragma unroll
   for (int t2 = -1; t2 <= 1; ++t2)
 #pragma unroll
     for (int t1 = -1; t1 <= 1; ++t1)
     {
-      const double w = (t1 == 0 ? 2.0 : 1.0) * (t2 == 0 ? 2.0 : 1.0);
+      const T w = (t1 == 0 ? T(2.0) : T(1.0)) * (t2 == 0 ? T(2.0) : T(1.0));
       int          o[3];
       o[D] = s;
       o[(D + 1) % 3] = t1;
@@ -343,14 +467,15 @@ youngs_plane(double const (&a)[27], int s)
   return acc;
 }
 
+template <typename T>
 G5D void
-youngs3(double const (&a)[27], double & nx, double & ny, double & nz)
+youngs3(T const (&a)[27], T & nx, T & ny, T & nz)
 {
-  const double gx = youngs_plane<0>(a, 1) - youngs_plane<0>(a, -1);
-  const double gy = youngs_plane<1>(a, 1) - youngs_plane<1>(a, -1);
-  const double gz = youngs_plane<2>(a, 1) - youngs_plane<2>(a, -1);
-  const double nrm = sqrt(gx * gx + gy * gy + gz * gz);
-  if (nrm > 0.0)
+  const T gx = youngs_plane<0>(a, 1) - youngs_plane<0>(a, -1);
+  const T gy = youngs_plane<1>(a, 1) - youngs_plane<1>(a, -1);
+  const T gz = youngs_plane<2>(a, 1) - youngs_plane<2>(a, -1);
+  const T nrm = sqrt(gx * gx + gy * gy + gz * gz);
+  if (nrm > T(0.0))
   {
     nx = gx / nrm;
     ny = gy / nrm;
@@ -358,7 +483,7 @@ youngs3(double const (&a)[27], double & nx, double & ny, double & nz)
   }
   else
   {
-    nx = ny = nz = 0.0;
+    nx = ny = nz = T(0.0);
   }
 }
 
@@ -366,45 +491,48 @@ youngs3(double const (&a)[27], double & nx, double & ny, double & nz)
 // :140-222 / :310-395).  am, ac, ap: alpha at -1, 0, +1 along the face axis; nd: |normal[axis]|
 // (only used when THINC is requested); ib: +1 for the right face of the cell, -1 for the left;
 // o: the MUSCL offset (0.5*ib) on the face axis (MUSCL fallback).
-struct ThincAxis // per-cell, per-axis THINC quantities shared by the two faces
+template <typename T>
+struct ThincAxisT // per-cell, per-axis THINC quantities shared by the two faces
 {
-  bool   req;
-  double A, B, C, D;
+  bool req;
+  T    A, B, C, D;
 };
+using ThincAxis = ThincAxisT<double>;
 
-G5D ThincAxis
-thinc_axis(SolveDesc const & S, double qa, double am, double ac, double ap, double nd)
+template <typename T>
+G5D ThincAxisT<T>
+thinc_axis(SolveDesc const & S, T qa, T am, T ac, T ap, T nd)
 {
-  ThincAxis    t;
-  const double monotonicity = (ap - ac) * (ac - am);
-  t.req = (qa > S.thinc_eps) and (qa < 1.0 - S.thinc_eps) and monotonicity > 0;
-  t.A = t.B = t.C = t.D = 0.0;
+  ThincAxisT<T> t;
+  const T       monotonicity = (ap - ac) * (ac - am);
+  t.req = (qa > Par<T>::eps(S)) and (qa < T(1.0) - Par<T>::eps(S)) and monotonicity > 0;
+  t.A = t.B = t.C = t.D = T(0.0);
   if (t.req)
   {
-    const double beta = S.thinc_beta * nd + 0.01;
+    const T beta = Par<T>::beta(S) * nd + T(0.01);
     t.A = (ap + am) / 2;
     t.B = (ap - am) / 2;
-    t.C = det_tanh(0.5 * beta);
-    const double phi = ac;
-    t.D = det_tanh(0.5 * beta * (phi - t.A) / t.B);
+    t.C = tanh_of(T(0.5) * beta);
+    const T phi = ac;
+    t.D = tanh_of(T(0.5) * beta * (phi - t.A) / t.B);
   }
   return t;
 }
 
 // reconstruct_state_{2d,3d} of the THINC functor: alpha-related part and MUSCL part for p, u, v, w
-template <int DIM, int AX, bool HANCOCK>
-G5D Face
-face_thinc(Prim const & q, Slope const & sx, Slope const & sy, Slope const & sz, Hancock const & h,
-           ThincAxis const & th, int ib)
+template <int DIM, int AX, bool HANCOCK, typename T>
+G5D FaceT<T>
+face_thinc(PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy, SlopeT<T> const & sz,
+           HancockT<T> const & h, ThincAxisT<T> const & th, int ib)
 {
-  Face         f;
-  const double o = 0.5 * static_cast<double>(ib); // HALF_F * offsets[AX]
+  FaceT<T> f;
+  const T  o = T(0.5) * static_cast<T>(ib); // HALF_F * offsets[AX]
   if (th.req)
   {
-    const double bb = static_cast<double>(ib);
-    f.a = th.A + bb * th.B * (th.C + bb * th.D / th.C) / (1.0 + bb * th.D);
+    const T bb = static_cast<T>(ib);
+    f.a = th.A + bb * th.B * (th.C + bb * th.D / th.C) / (T(1.0) + bb * th.D);
     f.r0 = q.r0 + q.r0 / q.a * (f.a - q.a);
-    f.r1 = q.r1 + q.r1 / (1.0 - q.a) * (q.a - f.a);
+    f.r1 = q.r1 + q.r1 / (T(1.0) - q.a) * (q.a - f.a);
   }
   else
   {
@@ -415,7 +543,7 @@ face_thinc(Prim const & q, Slope const & sx, Slope const & sy, Slope const & sz,
   f.p = muscl_extrap<DIM, AX>(q.p, sx.p, sy.p, sz.p, o);
   f.u = muscl_extrap<DIM, AX>(q.u, sx.u, sy.u, sz.u, o);
   f.v = muscl_extrap<DIM, AX>(q.v, sx.v, sy.v, sz.v, o);
-  f.w = DIM == 3 ? muscl_extrap<DIM, AX>(q.w, sx.w, sy.w, sz.w, o) : 0.0;
+  f.w = DIM == 3 ? muscl_extrap<DIM, AX>(q.w, sx.w, sy.w, sz.w, o) : T(0.0);
   if (HANCOCK)
   {
     f.r0 += h.r0;
@@ -558,36 +686,171 @@ hllc(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
   return F;
 }
 
+// Mixed-precision HLLC on FP32 face states: wave speeds, star velocity/pressure, mass, momentum
+// and volume-fraction fluxes in FP32; the total energies, their star values and the energy flux in
+// FP64 (rho e = p xi + Pi is dominated by Pi when pinf is large, and the update takes differences
+// of energy fluxes).  The reference's e(p) -> p(e) round trip on each side is the identity in exact
+// arithmetic and is not repeated in FP32, where it would lose the pressure to the cancellation.
+template <int DIM>
+G5D FluxT<float>
+hllc_mixed(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, double dtdx, int AX)
+{
+  const float ulN = AX == 0 ? L.u : (AX == 1 ? L.v : L.w);
+  const float ulT1 = AX == 1 ? L.u : L.v;
+  const float ulT2 = AX == 2 ? L.u : L.w;
+  const float urN = AX == 0 ? R.u : (AX == 1 ? R.v : R.w);
+  const float urT1 = AX == 1 ? R.u : R.v;
+  const float urT2 = AX == 2 ? R.u : R.w;
+
+  const float smallr = S.smallrf, smallp = S.smallpf, smallc = S.smallcf;
+
+  const float rl = fmaxf(L.r0 + L.r1, smallr);
+  const float pl = fmaxf(L.p, rl * smallp);
+  const float ul = ulN;
+  float       kl = ul * ul + ulT1 * ulT1;
+  if (DIM == 3)
+    kl += ulT2 * ulT2;
+  const float  cl = eos_sound_speed(S, rl, pl, L.a, 1.0f - L.a);
+  const double al = L.a;
+  const double etotl = (static_cast<double>(pl) * eos_xi(S, al, 1.0 - al) + eos_pi(S, al, 1.0 - al)) +
+                       static_cast<double>(0.5f * rl * kl);
+
+  const float rr = fmaxf(R.r0 + R.r1, smallr);
+  const float pr = fmaxf(R.p, rr * smallp);
+  const float ur = urN;
+  float       kr = ur * ur + urT1 * urT1;
+  if (DIM == 3)
+    kr += urT2 * urT2;
+  const float  cr = eos_sound_speed(S, rr, pr, R.a, 1.0f - R.a);
+  const double ar = R.a;
+  const double etotr = (static_cast<double>(pr) * eos_xi(S, ar, 1.0 - ar) + eos_pi(S, ar, 1.0 - ar)) +
+                       static_cast<double>(0.5f * rr * kr);
+
+  const float cfastl = fmaxf(cl, smallc);
+  const float cfastr = fmaxf(cr, smallc);
+  const float SL = fminf(ul, ur) - fmaxf(cfastl, cfastr);
+  const float SR = fmaxf(ul, ur) + fmaxf(cfastl, cfastr);
+
+  const float rcl = rl * (ul - SL);
+  const float rcr = rr * (SR - ur);
+  const float inv_rc = 1.0f / (rcr + rcl);
+  const float ustar0 = (rcr * ur + rcl * ul + (pl - pr)) * inv_rc;
+  const float ptotstar = (rcr * pl + rcl * pr + rcl * rcr * (ul - ur)) * inv_rc;
+
+  float  ro, ro_0, ro_1, uo, ptoto;
+  double etoto;
+  if (SL > 0.0f)
+  {
+    ro_0 = L.r0;
+    ro_1 = L.r1;
+    ro = rl;
+    uo = ul;
+    ptoto = pl;
+    etoto = etotl;
+  }
+  else if (ustar0 > 0.0f)
+  {
+    // etot* = f etot + (p* u* - p u)/(S - u*), f = (S - u)/(S - u*)
+    const float inv = 1.0f / (SL - ustar0);
+    const float factor = (SL - ul) * inv;
+    ro_0 = L.r0 * factor;
+    ro_1 = L.r1 * factor;
+    ro = rl * factor;
+    uo = ustar0;
+    ptoto = ptotstar;
+    etoto = static_cast<double>(factor) * etotl + static_cast<double>((ptotstar * ustar0 - pl * ul) * inv);
+  }
+  else if (SR > 0.0f)
+  {
+    const float inv = 1.0f / (SR - ustar0);
+    const float factor = (SR - ur) * inv;
+    ro_0 = R.r0 * factor;
+    ro_1 = R.r1 * factor;
+    ro = rr * factor;
+    uo = ustar0;
+    ptoto = ptotstar;
+    etoto = static_cast<double>(factor) * etotr + static_cast<double>((ptotstar * ustar0 - pr * ur) * inv);
+  }
+  else
+  {
+    ro_0 = R.r0;
+    ro_1 = R.r1;
+    ro = rr;
+    uo = ur;
+    ptoto = pr;
+    etoto = etotr;
+  }
+
+  const float  dtdxf = static_cast<float>(dtdx);
+  const float  fd = (ro_0 + ro_1) * uo;
+  const float  fn = ro * uo * uo + ptoto;
+  const double fe = (etoto + static_cast<double>(ptoto)) * static_cast<double>(uo);
+  const bool   up = fd > 0.0f;
+  const float  ft1 = fd * (up ? ulT1 : urT1);
+  const float  ft2 = DIM == 3 ? fd * (up ? ulT2 : urT2) : 0.0f;
+  const float  phi0 = up ? L.a : R.a;
+  const float  phi1 = up ? 1.0f - L.a : 1.0f - R.a;
+
+  FluxT<float> F;
+  F.ar0 = (ro_0 * uo) * dtdxf;
+  F.ar1 = (ro_1 * uo) * dtdxf;
+  F.a0 = (uo * phi0) * dtdxf;
+  F.a1 = (uo * phi1) * dtdxf;
+  F.e = fe * dtdx;
+  const float fu = AX == 0 ? fn : (AX == 1 ? ft1 : ft2);
+  const float fv = AX == 1 ? fn : ft1;
+  const float fw = AX == 2 ? fn : ft2;
+  F.mu = fu * dtdxf;
+  F.mv = fv * dtdxf;
+  F.mw = DIM == 3 ? fw * dtdxf : 0.0f;
+  F.us = uo * dtdxf;
+  return F;
+}
+
+template <int DIM>
+G5D Flux
+riemann(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
+{
+  return hllc<DIM>(S, L, R, dtdx, AX);
+}
+template <int DIM>
+G5D FluxT<float>
+riemann(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, double dtdx, int AX)
+{
+  return hllc_mixed<DIM>(S, L, R, dtdx, AX);
+}
+
 // ---------------------------------------------------------------------------------------------
 // scheme/ReadFluxesAndConservativeUpdateFunctor.cpp:146-212 (same-level branch) + update_U
 // (.h:319-393): accumulate left minus right flux with the Allaire-Kokh alpha source term (time-n
 // alpha of the cell), then add to the running U2 with the 1e-12 clamps.
 // ---------------------------------------------------------------------------------------------
-template <int DIM>
+// (the flux fields are widened to double: the state accumulation is FP64 in both paths)
+template <int DIM, typename T>
 G5D void
-apply_flux_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double ua1)
+apply_flux_pair(Cons & u2, FluxT<T> const & FL, FluxT<T> const & FR, double ua0, double ua1)
 {
   const double eps = 1e-12;
-  double       acc_r0 = 0.0 + FL.ar0;
-  double       acc_r1 = 0.0 + FL.ar1;
-  double       acc_a0 = 0.0 + FL.a0;
-  double       acc_a1 = 0.0 + FL.a1;
+  double       acc_r0 = 0.0 + static_cast<double>(FL.ar0);
+  double       acc_r1 = 0.0 + static_cast<double>(FL.ar1);
+  double       acc_a0 = 0.0 + static_cast<double>(FL.a0);
+  double       acc_a1 = 0.0 + static_cast<double>(FL.a1);
   double       acc_e = 0.0 + FL.e;
-  double       acc_mu = 0.0 + FL.mu;
-  double       acc_mv = 0.0 + FL.mv;
-  double       acc_mw = 0.0 + FL.mw;
-  acc_a0 -= 1.0 * FL.us * ua0;
-  acc_a1 -= 1.0 * FL.us * ua1;
-  acc_r0 = acc_r0 - FR.ar0;
-  acc_r1 = acc_r1 - FR.ar1;
-  acc_a0 = acc_a0 - FR.a0;
-  acc_a1 = acc_a1 - FR.a1;
+  double       acc_mu = 0.0 + static_cast<double>(FL.mu);
+  double       acc_mv = 0.0 + static_cast<double>(FL.mv);
+  double       acc_mw = 0.0 + static_cast<double>(FL.mw);
+  acc_a0 -= 1.0 * static_cast<double>(FL.us) * ua0;
+  acc_a1 -= 1.0 * static_cast<double>(FL.us) * ua1;
+  acc_r0 = acc_r0 - static_cast<double>(FR.ar0);
+  acc_r1 = acc_r1 - static_cast<double>(FR.ar1);
+  acc_a0 = acc_a0 - static_cast<double>(FR.a0);
+  acc_a1 = acc_a1 - static_cast<double>(FR.a1);
   acc_e = acc_e - FR.e;
-  acc_mu = acc_mu - FR.mu;
-  acc_mv = acc_mv - FR.mv;
-  acc_mw = acc_mw - FR.mw;
-  acc_a0 += 1.0 * FR.us * ua0;
-  acc_a1 += 1.0 * FR.us * ua1;
+  acc_mu = acc_mu - static_cast<double>(FR.mu);
+  acc_mv = acc_mv - static_cast<double>(FR.mv);
+  acc_mw = acc_mw - static_cast<double>(FR.mw);
+  acc_a0 += 1.0 * static_cast<double>(FR.us) * ua0;
+  acc_a1 += 1.0 * static_cast<double>(FR.us) * ua1;
 
   u2.ar0 += acc_r0;
   if (u2.ar0 < eps)
@@ -637,6 +900,30 @@ cell_inv_dt(SolveDesc const & S, Cons const & u)
     return vx / S.dx + vy / S.dx;
   const double vz = c + fabs(q.w);
   return vx / S.dx + vy / S.dx + vz / S.dx;
+}
+
+// mixed path: FP32 sound speed and velocities from the FP64-recovered pressure; the rate and its
+// max reduction are FP64
+template <int DIM>
+G5D double
+cell_inv_dt_mixed(SolveDesc const & S, Cons const & u)
+{
+  PrimT<float> q;
+  const float  c = cons_to_prim_mixed<DIM>(S, u, q);
+  float        v = (c + fabsf(q.u)) + (c + fabsf(q.v));
+  if (DIM == 3)
+    v += c + fabsf(q.w);
+  return static_cast<double>(v) / S.dx;
+}
+
+template <int DIM, typename T>
+G5D double
+cell_rate(SolveDesc const & S, Cons const & u)
+{
+  if constexpr (sizeof(T) == sizeof(double))
+    return cell_inv_dt<DIM>(S, u);
+  else
+    return cell_inv_dt_mixed<DIM>(S, u);
 }
 
 // ---------------------------------------------------------------------------------------------

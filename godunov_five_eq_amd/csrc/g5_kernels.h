@@ -8,12 +8,14 @@
 namespace g5
 {
 
-// fused step (primitives, slopes, fluxes, update, normalisation, next-dt contribution)
-void launch_step2d(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream);
-void launch_step3d(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream);
+// fused step (primitives, slopes, fluxes, update, normalisation, next-dt contribution); mixed
+// selects the FP32 primitive/face path
+void launch_step2d(StepArgs const & A, bool thinc, bool hancock, bool mixed, hipStream_t stream);
+void launch_step3d(StepArgs const & A, bool thinc, bool hancock, bool mixed, hipStream_t stream);
 
 // max over the listed blocks of the CFL rate of the current state into ctl[solve].invdt_bits
-void launch_inv_dt(int dim, double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves,
+// (mixed: the rate of the mixed-precision path)
+void launch_inv_dt(int dim, bool mixed, double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves,
                    SolveCtl * ctl, int nblocks, hipStream_t stream);
 
 // dt of the first step of solves ids[0..n) (all when ids is null) from invdt_bits; marks active

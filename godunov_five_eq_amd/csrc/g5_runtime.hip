@@ -174,8 +174,6 @@ validate(g5_config const & c, std::string & msg)
     return msg = "unknown time integrator", G5_ERR_INVALID;
   if (c.precision != G5_PREC_FP64 and c.precision != G5_PREC_MIXED)
     return msg = "unknown precision", G5_ERR_INVALID;
-  if (c.precision == G5_PREC_MIXED)
-    return msg = "mixed precision is not available in this build", G5_ERR_UNSUPPORTED;
   return G5_OK;
 }
 
@@ -400,10 +398,11 @@ launch_step(g5_pool * p, StepArgs const & A, bool hancock)
 {
   if (A.nblocks <= 0)
     return;
+  const bool mixed = p->precision == G5_PREC_MIXED;
   if (p->dim == 2)
-    launch_step2d(A, p->thinc != 0, hancock, p->stream);
+    launch_step2d(A, p->thinc != 0, hancock, mixed, p->stream);
   else
-    launch_step3d(A, p->thinc != 0, hancock, p->stream);
+    launch_step3d(A, p->thinc != 0, hancock, mixed, p->stream);
 }
 
 bool
@@ -486,7 +485,7 @@ pool_start(g5_pool * p, int32_t const * ids, int32_t n)
   G5_HIP(p->err, hipMemcpyAsync(d_list, list.data(), list.size() * sizeof(BlockDesc), hipMemcpyHostToDevice, p->stream));
   G5_HIP(p->err, hipMemcpyAsync(d_ids, ids, n * sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
   // invdt of these solves is 0 (admit and finalize leave it cleared)
-  launch_inv_dt(p->dim, p->buf[0], p->buf[1], d_list, p->d_solves, p->d_ctl, static_cast<int>(list.size()), p->stream);
+  launch_inv_dt(p->dim, p->precision == G5_PREC_MIXED, p->buf[0], p->buf[1], d_list, p->d_solves, p->d_ctl, static_cast<int>(list.size()), p->stream);
   launch_start(p->d_ctl, p->d_solves, d_ids, n, p->stream);
   G5_HIP(p->err, hipGetLastError());
   G5_HIP(p->err, hipStreamSynchronize(p->stream));
@@ -641,9 +640,7 @@ g5_pool_create(int32_t dim, int32_t thinc_enabled, int32_t precision, uint64_t h
   *out = nullptr;
   if (dim != 2 and dim != 3)
     return fail(g_create_error, G5_ERR_INVALID, "dim must be 2 or 3");
-  if (precision == G5_PREC_MIXED)
-    return fail(g_create_error, G5_ERR_UNSUPPORTED, "mixed precision is not available in this build");
-  if (precision != G5_PREC_FP64)
+  if (precision != G5_PREC_FP64 and precision != G5_PREC_MIXED)
     return fail(g_create_error, G5_ERR_INVALID, "unknown precision");
   return pool_create(dim, thinc_enabled ? 1 : 0, precision, hbm_budget_bytes, false, out, g_create_error);
 }
@@ -983,7 +980,7 @@ g5_compute_dt(g5_ctx * ctx, double * dt)
   if (rc != G5_OK)
     return fail(ctx->err, rc, p->err);
   launch_reset_invdt(p->d_ctl, 1, p->stream);
-  launch_inv_dt(p->dim, p->buf[0], p->buf[1], p->d_blocks, p->d_solves, p->d_ctl, p->nlist, p->stream);
+  launch_inv_dt(p->dim, p->precision == G5_PREC_MIXED, p->buf[0], p->buf[1], p->d_blocks, p->d_solves, p->d_ctl, p->nlist, p->stream);
   G5_HIP(ctx->err, hipGetLastError());
   rc = read_ctl(p);
   if (rc != G5_OK)

@@ -1,4 +1,5 @@
-// g5_step2d.hip -- fused 2-D five-equation step, one 16x16 block per workgroup (FP64).
+// g5_step2d.hip -- fused 2-D five-equation step, one 16x16 block per workgroup (FP64 or mixed:
+// the primitive/face path in the arithmetic T, the state accumulation in FP64).
 //
 // One launch does, for every owned block, what the reference does in seven launches
 // (scheme/GodunovImplemV0.cpp:83-151 plus the border fill and the next dt reduction):
@@ -76,22 +77,24 @@ halo_cell2(int h, int & gi, int & gj)
   gj = (c % 18) - 1;
 }
 
-G5D Prim
-sq_load2(double const * sQ, int idx)
+template <typename T>
+G5D PrimT<T>
+sq_load2(T const * sQ, int idx)
 {
-  Prim p;
+  PrimT<T> p;
   p.r0 = sQ[0 * NT2 + idx];
   p.r1 = sQ[1 * NT2 + idx];
   p.a = sQ[2 * NT2 + idx];
   p.p = sQ[3 * NT2 + idx];
   p.u = sQ[4 * NT2 + idx];
   p.v = sQ[5 * NT2 + idx];
-  p.w = 0.0;
+  p.w = T(0.0);
   return p;
 }
 
+template <typename T>
 G5D void
-sq_store2(double * sQ, int idx, Prim const & p)
+sq_store2(T * sQ, int idx, PrimT<T> const & p)
 {
   sQ[0 * NT2 + idx] = p.r0;
   sQ[1 * NT2 + idx] = p.r1;
@@ -101,9 +104,10 @@ sq_store2(double * sQ, int idx, Prim const & p)
   sQ[5 * NT2 + idx] = p.v;
 }
 
-// face states: 6 doubles (r0, r1, a, p, u, v) at slot k of a [6][NF2] array
+// face states: 6 values (r0, r1, a, p, u, v) at slot k of a [6][NF2] array
+template <typename T>
 G5D void
-st_store2(double * A, int k, Face const & f)
+st_store2(T * A, int k, FaceT<T> const & f)
 {
   A[0 * NF2 + k] = f.r0;
   A[1 * NF2 + k] = f.r1;
@@ -112,87 +116,91 @@ st_store2(double * A, int k, Face const & f)
   A[4 * NF2 + k] = f.u;
   A[5 * NF2 + k] = f.v;
 }
-G5D Face
-st_load2(double const * A, int k)
+template <typename T>
+G5D FaceT<T>
+st_load2(T const * A, int k)
 {
-  Face f;
+  FaceT<T> f;
   f.r0 = A[0 * NF2 + k];
   f.r1 = A[1 * NF2 + k];
   f.a = A[2 * NF2 + k];
   f.p = A[3 * NF2 + k];
   f.u = A[4 * NF2 + k];
   f.v = A[5 * NF2 + k];
-  f.w = 0.0;
+  f.w = T(0.0);
   return f;
 }
-// fluxes: 8 doubles (ar0, ar1, a0, a1, e, mu, mv, us) at slot k of a [8][NF2] array
+// fluxes: 7 values (ar0, ar1, a0, a1, mu, mv, us) at slot k of a [7][NF2] array, energy in E[k]
+template <typename T>
 G5D void
-fl_store2(double * A, int k, Flux const & F)
+fl_store2(T * A, double * E, int k, FluxT<T> const & F)
 {
   A[0 * NF2 + k] = F.ar0;
   A[1 * NF2 + k] = F.ar1;
   A[2 * NF2 + k] = F.a0;
   A[3 * NF2 + k] = F.a1;
-  A[4 * NF2 + k] = F.e;
-  A[5 * NF2 + k] = F.mu;
-  A[6 * NF2 + k] = F.mv;
-  A[7 * NF2 + k] = F.us;
+  A[4 * NF2 + k] = F.mu;
+  A[5 * NF2 + k] = F.mv;
+  A[6 * NF2 + k] = F.us;
+  E[k] = F.e;
 }
-G5D Flux
-fl_load2(double const * A, int k)
+template <typename T>
+G5D FluxT<T>
+fl_load2(T const * A, double const * E, int k)
 {
-  Flux F;
+  FluxT<T> F;
   F.ar0 = A[0 * NF2 + k];
   F.ar1 = A[1 * NF2 + k];
   F.a0 = A[2 * NF2 + k];
   F.a1 = A[3 * NF2 + k];
-  F.e = A[4 * NF2 + k];
-  F.mu = A[5 * NF2 + k];
-  F.mv = A[6 * NF2 + k];
-  F.mw = 0.0;
-  F.us = A[7 * NF2 + k];
+  F.mu = A[4 * NF2 + k];
+  F.mv = A[5 * NF2 + k];
+  F.mw = T(0.0);
+  F.us = A[6 * NF2 + k];
+  F.e = E[k];
   return F;
 }
 
 // the reconstructed face state of tile cell (gi,gj) on axis ax with side ib (+1 right, -1 left)
-template <bool THINC, bool HANCOCK>
+template <bool THINC, bool HANCOCK, typename T>
 G5D void
-predict_faces2(SolveDesc const & S, double const * sQ, int gi, int gj, double dtdx, bool all,
-               int ax1, int ib1, Face & fxm, Face & fxp, Face & fym, Face & fyp, Face & f1)
+predict_faces2(SolveDesc const & S, T const * sQ, int gi, int gj, T dtdx, bool all, int ax1, int ib1,
+               FaceT<T> & fxm, FaceT<T> & fxp, FaceT<T> & fym, FaceT<T> & fyp, FaceT<T> & f1)
 {
-  const Prim  qc = sq_load2(sQ, tidx2(gi, gj));
-  const Prim  qxm = sq_load2(sQ, tidx2(gi - 1, gj));
-  const Prim  qxp = sq_load2(sQ, tidx2(gi + 1, gj));
-  const Prim  qym = sq_load2(sQ, tidx2(gi, gj - 1));
-  const Prim  qyp = sq_load2(sQ, tidx2(gi, gj + 1));
-  const Slope sx = slopes_of(qc, qxp, qxm, S.slope_type);
-  const Slope sy = slopes_of(qc, qyp, qym, S.slope_type);
-  const Slope sz = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
-  Hancock     h = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+  const PrimT<T>  qc = sq_load2(sQ, tidx2(gi, gj));
+  const PrimT<T>  qxm = sq_load2(sQ, tidx2(gi - 1, gj));
+  const PrimT<T>  qxp = sq_load2(sQ, tidx2(gi + 1, gj));
+  const PrimT<T>  qym = sq_load2(sQ, tidx2(gi, gj - 1));
+  const PrimT<T>  qyp = sq_load2(sQ, tidx2(gi, gj + 1));
+  const T         st = static_cast<T>(S.slope_type);
+  const SlopeT<T> sx = slopes_of(qc, qxp, qxm, st);
+  const SlopeT<T> sy = slopes_of(qc, qyp, qym, st);
+  const SlopeT<T> sz = { T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
+  HancockT<T>     h = { T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
   if (HANCOCK)
     h = hancock_terms<2>(S, qc, sx, sy, sz, dtdx);
   if (!THINC)
   {
     if (all)
     {
-      fxm = face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, -0.5);
-      fxp = face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, 0.5);
-      fym = face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, -0.5);
-      fyp = face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, 0.5);
+      fxm = face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, T(-0.5));
+      fxp = face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, T(0.5));
+      fym = face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, T(-0.5));
+      fyp = face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, T(0.5));
     }
     else
     {
-      const double o = 0.5 * static_cast<double>(ib1);
+      const T o = T(0.5) * static_cast<T>(ib1);
       f1 = ax1 == 0 ? face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, o)
                     : face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, o);
     }
     return;
   }
   // THINC: Youngs normal from the 3x3 alpha stencil, needed only in mixed cells
-  double nx = 0.0, ny = 0.0;
-  if (qc.a > S.thinc_eps and qc.a < 1.0 - S.thinc_eps)
+  T nx = T(0.0), ny = T(0.0);
+  if (qc.a > Par<T>::eps(S) and qc.a < T(1.0) - Par<T>::eps(S))
   {
-    double a9[9];
+    T a9[9];
 #pragma unroll
     for (int dj = -1; dj <= 1; ++dj)
 #pragma unroll
@@ -202,8 +210,8 @@ predict_faces2(SolveDesc const & S, double const * sQ, int gi, int gj, double dt
   }
   if (all)
   {
-    const ThincAxis tx = thinc_axis(S, qc.a, qxm.a, qc.a, qxp.a, fabs(nx));
-    const ThincAxis ty = thinc_axis(S, qc.a, qym.a, qc.a, qyp.a, fabs(ny));
+    const ThincAxisT<T> tx = thinc_axis(S, qc.a, qxm.a, qc.a, qxp.a, fabs(nx));
+    const ThincAxisT<T> ty = thinc_axis(S, qc.a, qym.a, qc.a, qyp.a, fabs(ny));
     fxm = face_thinc<2, 0, HANCOCK>(qc, sx, sy, sz, h, tx, -1);
     fxp = face_thinc<2, 0, HANCOCK>(qc, sx, sy, sz, h, tx, 1);
     fym = face_thinc<2, 1, HANCOCK>(qc, sx, sy, sz, h, ty, -1);
@@ -213,12 +221,12 @@ predict_faces2(SolveDesc const & S, double const * sQ, int gi, int gj, double dt
   {
     if (ax1 == 0)
     {
-      const ThincAxis tx = thinc_axis(S, qc.a, qxm.a, qc.a, qxp.a, fabs(nx));
+      const ThincAxisT<T> tx = thinc_axis(S, qc.a, qxm.a, qc.a, qxp.a, fabs(nx));
       f1 = face_thinc<2, 0, HANCOCK>(qc, sx, sy, sz, h, tx, ib1);
     }
     else
     {
-      const ThincAxis ty = thinc_axis(S, qc.a, qym.a, qc.a, qyp.a, fabs(ny));
+      const ThincAxisT<T> ty = thinc_axis(S, qc.a, qym.a, qc.a, qyp.a, fabs(ny));
       f1 = face_thinc<2, 1, HANCOCK>(qc, sx, sy, sz, h, ty, ib1);
     }
   }
@@ -235,12 +243,17 @@ logical_block(StepArgs const & A)
   return (w & 7) * A.per_xcd + (w >> 3);
 }
 
-template <bool THINC, bool HANCOCK>
+// LDS bytes of the face-state / flux exchange area: 12 face-state rows or 14 flux rows of T plus
+// 2 energy-flux rows of double
+template <typename T>
+constexpr int SE2_DOUBLES = (14 * NF2 * static_cast<int>(sizeof(T)) + 2 * NF2 * 8) / 8;
+
+template <bool THINC, bool HANCOCK, typename T>
 __global__ void __launch_bounds__(256)
 k_step2d(StepArgs A)
 {
-  __shared__ double sQ[6 * NT2];
-  __shared__ double sE[16 * NF2];
+  __shared__ T      sQ[6 * NT2];
+  __shared__ double sE[SE2_DOUBLES<T>];
   __shared__ double sRed[4];
 
   const int t = threadIdx.x;
@@ -258,14 +271,15 @@ k_step2d(StepArgs A)
   double const * U = A.U_in ? A.U_in : (par ? A.buf1 : A.buf0);
   double *       U2 = A.U_out ? A.U_out : (par ? A.buf0 : A.buf1);
   const double   dt = host_dt ? A.dt_fixed : C->dt;
-  const double dtdx = dt / S.dx;
-  const int    i = t & 15, j = t >> 4;
+  const double   dtdx = dt / S.dx;
+  const T        dtdxT = static_cast<T>(dtdx);
+  const int      i = t & 15, j = t >> 4;
 
   // ---- 1. gather + primitives ----
   Cons uown = load_cons<2, B2>(U, own, -1);
   {
-    Prim q;
-    cons_to_prim<2>(S, uown, q);
+    PrimT<T> q;
+    to_prim<2>(S, uown, q);
     sq_store2(sQ, tidx2(i, j), q);
   }
   if (t < 144)
@@ -274,17 +288,17 @@ k_step2d(StepArgs A)
     halo_cell2(t, gi, gj);
     const int64_t off = resolve_cell<2, B2>(S, A.slots, bd.bx, bd.by, 0, gi, gj, 0, neg);
     const Cons    u = load_cons<2, B2>(U, off, neg);
-    Prim          q;
-    cons_to_prim<2>(S, u, q);
+    PrimT<T>      q;
+    to_prim<2>(S, u, q);
     sq_store2(sQ, tidx2(gi, gj), q);
   }
   __syncthreads();
 
   // ---- 2. slopes, Hancock terms, face states ----
-  double * XL = sE;           // [6][NF2] left states of the x faces
-  double * YL = sE + 6 * NF2; // [6][NF2] left states of the y faces
-  Face     fxm, fxp, fym, fyp, fdummy, fring;
-  predict_faces2<THINC, HANCOCK>(S, sQ, i, j, dtdx, true, 0, 0, fxm, fxp, fym, fyp, fdummy);
+  T *      XL = reinterpret_cast<T *>(sE); // [6][NF2] left states of the x faces
+  T *      YL = XL + 6 * NF2;              // [6][NF2] left states of the y faces
+  FaceT<T> fxm, fxp, fym, fyp, fdummy, fring;
+  predict_faces2<THINC, HANCOCK>(S, sQ, i, j, dtdxT, true, 0, 0, fxm, fxp, fym, fyp, fdummy);
   st_store2(XL, fx2(i + 1, j), fxp);
   st_store2(YL, fy2(i, j + 1), fyp);
   // ring cells: s=0 (-1,r) +x ; s=1 (16,r) -x ; s=2 (r,-1) +y ; s=3 (r,16) -y
@@ -295,8 +309,8 @@ k_step2d(StepArgs A)
     halo_cell2(t, gi, gj);
     const int ax = rs < 2 ? 0 : 1;
     const int ib = (rs & 1) ? -1 : 1;
-    predict_faces2<THINC, HANCOCK>(S, sQ, gi, gj, dtdx, false, ax, ib, fdummy, fdummy, fdummy,
-                                   fdummy, fring);
+    predict_faces2<THINC, HANCOCK>(S, sQ, gi, gj, dtdxT, false, ax, ib, fdummy, fdummy, fdummy, fdummy,
+                                   fring);
     if (rs == 0)
       st_store2(XL, fx2(0, rr), fring);
     else if (rs == 2)
@@ -305,39 +319,41 @@ k_step2d(StepArgs A)
   __syncthreads();
 
   // ---- 3. HLLC ----
-  const Flux Fx = hllc<2>(S, st_load2(XL, fx2(i, j)), fxm, dtdx, 0);
-  const Flux Fy = hllc<2>(S, st_load2(YL, fy2(i, j)), fym, dtdx, 1);
-  Flux       Fr;
-  const bool ring_face = t < 64 and (rs & 1);
+  const FluxT<T> Fx = riemann<2>(S, st_load2(XL, fx2(i, j)), fxm, dtdx, 0);
+  const FluxT<T> Fy = riemann<2>(S, st_load2(YL, fy2(i, j)), fym, dtdx, 1);
+  FluxT<T>       Fr;
+  const bool     ring_face = t < 64 and (rs & 1);
   if (ring_face)
   {
-    const int  ax = rs == 1 ? 0 : 1;
-    const Face L = ax == 0 ? st_load2(XL, fx2(16, rr)) : st_load2(YL, fy2(rr, 16));
-    Fr = hllc<2>(S, L, fring, dtdx, ax);
+    const int      ax = rs == 1 ? 0 : 1;
+    const FaceT<T> L = ax == 0 ? st_load2(XL, fx2(16, rr)) : st_load2(YL, fy2(rr, 16));
+    Fr = riemann<2>(S, L, fring, dtdx, ax);
   }
   __syncthreads();
 
   // ---- 4. flux exchange and update ----
-  double * FX = sE;           // [8][NF2]
-  double * FY = sE + 8 * NF2; // [8][NF2]
-  fl_store2(FX, fx2(i, j), Fx);
-  fl_store2(FY, fy2(i, j), Fy);
+  T *      FX = reinterpret_cast<T *>(sE); // [7][NF2]
+  T *      FY = FX + 7 * NF2;              // [7][NF2]
+  double * FXE = reinterpret_cast<double *>(FY + 7 * NF2);
+  double * FYE = FXE + NF2;
+  fl_store2(FX, FXE, fx2(i, j), Fx);
+  fl_store2(FY, FYE, fy2(i, j), Fy);
   if (ring_face)
   {
     if (rs == 1)
-      fl_store2(FX, fx2(16, rr), Fr);
+      fl_store2(FX, FXE, fx2(16, rr), Fr);
     else
-      fl_store2(FY, fy2(rr, 16), Fr);
+      fl_store2(FY, FYE, fy2(rr, 16), Fr);
   }
   __syncthreads();
   Cons u2 = uown;
-  apply_flux_pair<2>(u2, Fx, fl_load2(FX, fx2(i + 1, j)), uown.a0, uown.a1);
-  apply_flux_pair<2>(u2, Fy, fl_load2(FY, fy2(i, j + 1)), uown.a0, uown.a1);
+  apply_flux_pair<2>(u2, Fx, fl_load2(FX, FXE, fx2(i + 1, j)), uown.a0, uown.a1);
+  apply_flux_pair<2>(u2, Fy, fl_load2(FY, FYE, fy2(i, j + 1)), uown.a0, uown.a1);
   fix_volume_fractions(u2);
   store_cons<2, B2>(U2, own, u2);
 
   // ---- 5. CFL of the new state ----
-  double inv = wave_max(cell_inv_dt<2>(S, u2));
+  double inv = wave_max(cell_rate<2, T>(S, u2));
   if ((t & 63) == 0)
     sRed[t >> 6] = inv;
   __syncthreads();
@@ -348,24 +364,28 @@ k_step2d(StepArgs A)
   }
 }
 
-void
-launch_step2d(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
+template <typename T>
+static void
+launch2(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
 {
   const dim3 grid(8 * A.per_xcd), block(256);
-  if (thinc)
-  {
-    if (hancock)
-      hipLaunchKernelGGL((k_step2d<true, true>), grid, block, 0, stream, A);
-    else
-      hipLaunchKernelGGL((k_step2d<true, false>), grid, block, 0, stream, A);
-  }
+  if (thinc and hancock)
+    hipLaunchKernelGGL((k_step2d<true, true, T>), grid, block, 0, stream, A);
+  else if (thinc)
+    hipLaunchKernelGGL((k_step2d<true, false, T>), grid, block, 0, stream, A);
+  else if (hancock)
+    hipLaunchKernelGGL((k_step2d<false, true, T>), grid, block, 0, stream, A);
   else
-  {
-    if (hancock)
-      hipLaunchKernelGGL((k_step2d<false, true>), grid, block, 0, stream, A);
-    else
-      hipLaunchKernelGGL((k_step2d<false, false>), grid, block, 0, stream, A);
-  }
+    hipLaunchKernelGGL((k_step2d<false, false, T>), grid, block, 0, stream, A);
+}
+
+void
+launch_step2d(StepArgs const & A, bool thinc, bool hancock, bool mixed, hipStream_t stream)
+{
+  if (mixed)
+    launch2<float>(A, thinc, hancock, stream);
+  else
+    launch2<double>(A, thinc, hancock, stream);
 }
 
 } // namespace g5

@@ -1,19 +1,26 @@
 // g5_step3d.hip -- fused 3-D five-equation step, one 16^3 block per workgroup, marching in z.
 //
-// A 20^3 FP64 primitive tile with ghost width 2 (576 KB for 9 variables) does not fit in LDS, so
-// the block is swept plane by plane (2.5-D): LDS holds the primitives of the current z plane
-// (20x20) for the x/y stencils, the z neighbours live in registers of the thread that owns the
-// column.  Per plane P = -1..16 the workgroup
-//   a. gathers and converts plane P+1 (ghost/border resolution on the fly),
+// A 20^3 primitive tile with ghost width 2 does not fit in LDS, so the block is swept plane by
+// plane (2.5-D): LDS holds a ring of three primitive planes (P-1, P, P+1; 20x20 each) for the x,
+// y and z stencils and the THINC normals.  Per plane P = -1..16 the workgroup
+//   a. converts plane P+1 (gathered one phase earlier, ghost/border resolution on the fly),
 //   b. computes slopes, sound speed and Hancock terms of plane P and its face states,
 //   c. solves HLLC on the x and y faces of plane P and on the z face between P-1 and P,
 //   d. exchanges the x/y fluxes through LDS, applies the x then y update to plane P and the z
 //      update (with normalisation, store and CFL contribution) to plane P-1.
 // Same numerics as scheme/GodunovImplemV0.cpp:83-151 (see g5_step2d.hip for the per-function
 // citations); the z face flux of plane P-1 is kept in registers until plane P is done.
-// Threads 0..255 (waves 0-3) own the interior columns (t%16, t/16); threads 256..319 (wave 4)
-// own the 64 ring columns whose single face state the block needs and convert the 80 outer
-// columns of the 20x20 plane.
+//
+// Layout: one workgroup of 4 wavefronts per block; thread t owns the interior column
+// (t%16, t/16).  The register demand of the FP64 z-march allows one wavefront per SIMD, so the
+// FP64 form uses up to 139 KB of LDS; the mixed form (T = float primitives, face states and
+// fluxes, FP64 energy flux and state) halves it.  Extra work is spread so that every wavefront
+// has one extra item per plane: wave 0 converts the 64 ring cells, wave 1 computes their face
+// states, wave 2 solves the 32 ring Riemann problems and converts 16 outer cells, wave 3
+// converts 64 outer cells.  The raw state of the owned column is carried from its conversion to
+// its update, and the global loads of plane P+2 are issued one phase ahead of their use.  With
+// ZSPLIT the z Riemann solve and the completion of plane P-1 run before the x/y exchange on
+// waves 0, 2, 3 and after it on wave 1, which carries the ring predictor.
 #include <cstdlib>
 
 #include "g5_device.h"
@@ -77,10 +84,11 @@ ring_cell3(int r, int & gi, int & gj)
   gj = s == 0 ? rr : (s == 1 ? rr : (s == 2 ? -1 : 16));
 }
 
-G5D Prim
-sq_load3(double const * sQ, int idx)
+template <typename T>
+G5D PrimT<T>
+sq_load3(T const * sQ, int idx)
 {
-  Prim p;
+  PrimT<T> p;
   p.r0 = sQ[0 * NT3 + idx];
   p.r1 = sQ[1 * NT3 + idx];
   p.a = sQ[2 * NT3 + idx];
@@ -90,8 +98,9 @@ sq_load3(double const * sQ, int idx)
   p.w = sQ[6 * NT3 + idx];
   return p;
 }
+template <typename T>
 G5D void
-sq_store3(double * sQ, int idx, Prim const & p)
+sq_store3(T * sQ, int idx, PrimT<T> const & p)
 {
   sQ[0 * NT3 + idx] = p.r0;
   sQ[1 * NT3 + idx] = p.r1;
@@ -101,10 +110,10 @@ sq_store3(double * sQ, int idx, Prim const & p)
   sQ[5 * NT3 + idx] = p.v;
   sQ[6 * NT3 + idx] = p.w;
 }
-// face states: 7 doubles at slot k of a [7][N] array (N = 272 faces, or 16 ring faces)
-template <int N = NF3>
+// face states: 7 values at slot k of a [7][N] array (N = 272 faces, or 16 ring faces)
+template <int N = NF3, typename T>
 G5D void
-st_store3(double * A, int k, Face const & f)
+st_store3(T * A, int k, FaceT<T> const & f)
 {
   A[0 * N + k] = f.r0;
   A[1 * N + k] = f.r1;
@@ -114,11 +123,11 @@ st_store3(double * A, int k, Face const & f)
   A[5 * N + k] = f.v;
   A[6 * N + k] = f.w;
 }
-template <int N = NF3>
-G5D Face
-st_load3(double const * A, int k)
+template <int N = NF3, typename T>
+G5D FaceT<T>
+st_load3(T const * A, int k)
 {
-  Face f;
+  FaceT<T> f;
   f.r0 = A[0 * N + k];
   f.r1 = A[1 * N + k];
   f.a = A[2 * N + k];
@@ -128,32 +137,35 @@ st_load3(double const * A, int k)
   f.w = A[6 * N + k];
   return f;
 }
+// fluxes: 8 values (ar0, ar1, a0, a1, mu, mv, mw, us) at slot k of a [8][NF3] array, energy in E[k]
+template <typename T>
 G5D void
-fl_store3(double * A, int k, Flux const & F)
+fl_store3(T * A, double * E, int k, FluxT<T> const & F)
 {
   A[0 * NF3 + k] = F.ar0;
   A[1 * NF3 + k] = F.ar1;
   A[2 * NF3 + k] = F.a0;
   A[3 * NF3 + k] = F.a1;
-  A[4 * NF3 + k] = F.e;
-  A[5 * NF3 + k] = F.mu;
-  A[6 * NF3 + k] = F.mv;
-  A[7 * NF3 + k] = F.mw;
-  A[8 * NF3 + k] = F.us;
+  A[4 * NF3 + k] = F.mu;
+  A[5 * NF3 + k] = F.mv;
+  A[6 * NF3 + k] = F.mw;
+  A[7 * NF3 + k] = F.us;
+  E[k] = F.e;
 }
-G5D Flux
-fl_load3(double const * A, int k)
+template <typename T>
+G5D FluxT<T>
+fl_load3(T const * A, double const * E, int k)
 {
-  Flux F;
+  FluxT<T> F;
   F.ar0 = A[0 * NF3 + k];
   F.ar1 = A[1 * NF3 + k];
   F.a0 = A[2 * NF3 + k];
   F.a1 = A[3 * NF3 + k];
-  F.e = A[4 * NF3 + k];
-  F.mu = A[5 * NF3 + k];
-  F.mv = A[6 * NF3 + k];
-  F.mw = A[7 * NF3 + k];
-  F.us = A[8 * NF3 + k];
+  F.mu = A[4 * NF3 + k];
+  F.mv = A[5 * NF3 + k];
+  F.mw = A[6 * NF3 + k];
+  F.us = A[7 * NF3 + k];
+  F.e = E[k];
   return F;
 }
 
@@ -172,37 +184,26 @@ logical_block3(StepArgs const & A)
   return (w & 7) * A.per_xcd + (w >> 3);
 }
 
-// ---------------------------------------------------------------------------------------------
-// v3 layout: one workgroup of 4 wavefronts per block and per CU (the register demand of the
-// FP64 z-march allows one wavefront per SIMD), so the whole 160 KB of LDS is available:
-//   - a ring of 3 primitive planes (P-1, P, P+1) for every stencil, THINC alpha included;
-//   - separate face-state (XL/YL, ring XR/YR) and flux (FX/FY) buffers, 3 barriers per plane;
-//   - the raw state of the owned column is carried from its conversion to its update, and the
-//     global loads of plane P+2 are issued one phase ahead of their use.
-// Extra work is spread so that every wavefront has one extra item per plane: wave 0 converts the
-// 64 ring cells, wave 1 computes their face states, wave 2 solves the 32 ring Riemann problems
-// and converts 16 outer cells, wave 3 converts 64 outer cells.  With ZSPLIT the z Riemann solve
-// and the completion of plane P-1 run before the x/y exchange on waves 0, 2, 3 and after it on
-// wave 1, which carries the ring predictor.
-// ---------------------------------------------------------------------------------------------
 namespace
 {
-constexpr int QPL = 7 * NT3; // doubles per primitive plane
+constexpr int QPL = 7 * NT3; // values per primitive plane
 
-G5D Prim
-q3_load(double const * sQ, int slot, int idx)
+template <typename T>
+G5D PrimT<T>
+q3_load(T const * sQ, int slot, int idx)
 {
   return sq_load3(sQ + slot * QPL, idx);
 }
 
+template <typename T>
 G5D void
-youngs_ring(double const * sQ, int P, int gi, int gj, double & nx, double & ny, double & nz)
+youngs_ring(T const * sQ, int P, int gi, int gj, T & nx, T & ny, T & nz)
 {
-  double a27[27];
+  T a27[27];
 #pragma unroll
   for (int dk = -1; dk <= 1; ++dk)
   {
-    double const * plane = sQ + aslot(P + dk) * QPL + 2 * NT3; // alpha of plane P+dk
+    T const * plane = sQ + aslot(P + dk) * QPL + 2 * NT3; // alpha of plane P+dk
 #pragma unroll
     for (int dj = -1; dj <= 1; ++dj)
 #pragma unroll
@@ -213,70 +214,73 @@ youngs_ring(double const * sQ, int P, int gi, int gj, double & nx, double & ny, 
 }
 
 // slopes, Hancock terms and the requested face states of tile cell (gi,gj) in plane P
-template <bool THINC, bool HANCOCK>
+template <bool THINC, bool HANCOCK, typename T>
 struct Pred3
 {
-  Prim    q;
-  Slope   sx, sy, sz;
-  Hancock h;
-  double  nx, ny, nz;
-  double  axm, axp, aym, ayp, azm, azp;
+  PrimT<T>    q;
+  SlopeT<T>   sx, sy, sz;
+  HancockT<T> h;
+  T           nx, ny, nz;
+  T           axm, axp, aym, ayp, azm, azp;
 
   G5D void
-  compute(SolveDesc const & S, double const * sQ, int P, int gi, int gj, double dtdx)
+  compute(SolveDesc const & S, T const * sQ, int P, int gi, int gj, T dtdx)
   {
-    const int  sp = aslot(P);
-    const int  c = tidx3(gi, gj);
+    const int      sp = aslot(P);
+    const int      c = tidx3(gi, gj);
     q = q3_load(sQ, sp, c);
-    const Prim qxm = q3_load(sQ, sp, tidx3(gi - 1, gj));
-    const Prim qxp = q3_load(sQ, sp, tidx3(gi + 1, gj));
-    const Prim qym = q3_load(sQ, sp, tidx3(gi, gj - 1));
-    const Prim qyp = q3_load(sQ, sp, tidx3(gi, gj + 1));
-    const Prim qzm = q3_load(sQ, aslot(P - 1), c);
-    const Prim qzp = q3_load(sQ, aslot(P + 1), c);
-    sx = slopes_of(q, qxp, qxm, S.slope_type);
-    sy = slopes_of(q, qyp, qym, S.slope_type);
-    sz = slopes_of(q, qzp, qzm, S.slope_type);
+    const PrimT<T> qxm = q3_load(sQ, sp, tidx3(gi - 1, gj));
+    const PrimT<T> qxp = q3_load(sQ, sp, tidx3(gi + 1, gj));
+    const PrimT<T> qym = q3_load(sQ, sp, tidx3(gi, gj - 1));
+    const PrimT<T> qyp = q3_load(sQ, sp, tidx3(gi, gj + 1));
+    const PrimT<T> qzm = q3_load(sQ, aslot(P - 1), c);
+    const PrimT<T> qzp = q3_load(sQ, aslot(P + 1), c);
+    const T        st = static_cast<T>(S.slope_type);
+    sx = slopes_of(q, qxp, qxm, st);
+    sy = slopes_of(q, qyp, qym, st);
+    sz = slopes_of(q, qzp, qzm, st);
     axm = qxm.a;
     axp = qxp.a;
     aym = qym.a;
     ayp = qyp.a;
     azm = qzm.a;
     azp = qzp.a;
-    h = Hancock{ 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+    h = HancockT<T>{ T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
     if (HANCOCK)
       h = hancock_terms<3>(S, q, sx, sy, sz, dtdx);
-    nx = ny = nz = 0.0;
-    if (THINC and q.a > S.thinc_eps and q.a < 1.0 - S.thinc_eps)
+    nx = ny = nz = T(0.0);
+    if (THINC and q.a > Par<T>::eps(S) and q.a < T(1.0) - Par<T>::eps(S))
       youngs_ring(sQ, P, gi, gj, nx, ny, nz);
   }
 
   template <int AX>
-  G5D Face
+  G5D FaceT<T>
   face(SolveDesc const & S, int ib) const
   {
     if (!THINC)
-      return face_muscl<3, AX, HANCOCK>(S, q, sx, sy, sz, h, 0.5 * static_cast<double>(ib));
-    const double am = AX == 0 ? axm : (AX == 1 ? aym : azm);
-    const double ap = AX == 0 ? axp : (AX == 1 ? ayp : azp);
-    const double nd = AX == 0 ? nx : (AX == 1 ? ny : nz);
-    const ThincAxis ta = thinc_axis(S, q.a, am, q.a, ap, fabs(nd));
+      return face_muscl<3, AX, HANCOCK>(S, q, sx, sy, sz, h, T(0.5) * static_cast<T>(ib));
+    const T             am = AX == 0 ? axm : (AX == 1 ? aym : azm);
+    const T             ap = AX == 0 ? axp : (AX == 1 ? ayp : azp);
+    const T             nd = AX == 0 ? nx : (AX == 1 ? ny : nz);
+    const ThincAxisT<T> ta = thinc_axis(S, q.a, am, q.a, ap, fabs(nd));
     return face_thinc<3, AX, HANCOCK>(q, sx, sy, sz, h, ta, ib);
   }
 };
 } // namespace
 
-template <bool THINC, bool HANCOCK, bool ZSPLIT>
+template <bool THINC, bool HANCOCK, bool ZSPLIT, typename T>
 __global__ void __launch_bounds__(256)
 k_step3d_v3(StepArgs A)
 {
-  __shared__ double sQ[3 * QPL];
-  __shared__ double sXL[7 * NF3];
-  __shared__ double sYL[7 * NF3];
-  __shared__ double sXR[7 * 16];
-  __shared__ double sYR[7 * 16];
-  __shared__ double sFX[9 * NF3];
-  __shared__ double sFY[9 * NF3];
+  __shared__ T      sQ[3 * QPL];
+  __shared__ T      sXL[7 * NF3];
+  __shared__ T      sYL[7 * NF3];
+  __shared__ T      sXR[7 * 16];
+  __shared__ T      sYR[7 * 16];
+  __shared__ T      sFX[8 * NF3];
+  __shared__ T      sFY[8 * NF3];
+  __shared__ double sFXE[NF3];
+  __shared__ double sFYE[NF3];
   __shared__ double sRed[4];
 
   const int t = threadIdx.x;
@@ -295,7 +299,8 @@ k_step3d_v3(StepArgs A)
   double const * U = A.U_in ? A.U_in : (par ? A.buf1 : A.buf0);
   double *       U2 = A.U_out ? A.U_out : (par ? A.buf0 : A.buf1);
   const double   dt = host_dt ? A.dt_fixed : C->dt;
-  const double dtdx = dt / S.dx;
+  const double   dtdx = dt / S.dx;
+  const T        dtdxT = static_cast<T>(dtdx);
 
   const int  wave = t >> 6, lane = t & 63;
   const int  i = t & 15, j = t >> 4;
@@ -320,21 +325,21 @@ k_step3d_v3(StepArgs A)
     return load_cons<3, B3>(U, off, neg);
   };
   auto put = [&](int P, int gi, int gj, Cons const & u) {
-    Prim q;
-    cons_to_prim<3>(S, u, q);
+    PrimT<T> q;
+    to_prim<3>(S, u, q);
     sq_store3(sQ + aslot(P) * QPL, tidx3(gi, gj), q);
   };
 
   // carried state
-  Cons   pf_i, pf_x;   // prefetched raw state of the next plane to convert
-  Cons   uraw_cur;     // raw state of the owned column in plane P
-  Cons   uraw_next;    // ... in plane P+1
-  Cons   u2prev;       // plane P-1 after its x and y updates
-  double uaprev0 = 0.0, uaprev1 = 0.0;
-  Flux   zFprev;       // z flux on face P-1
-  Face   zL;           // +z face state of plane P-1
-  Face   zLnext;       // +z face state of plane P (wave 1 with ZSPLIT, until its z solve)
-  double inv_max = 0.0;
+  Cons     pf_i, pf_x;   // prefetched raw state of the next plane to convert
+  Cons     uraw_cur;     // raw state of the owned column in plane P
+  Cons     uraw_next;    // ... in plane P+1
+  Cons     u2prev;       // plane P-1 after its x and y updates
+  double   uaprev0 = 0.0, uaprev1 = 0.0;
+  FluxT<T> zFprev;       // z flux on face P-1
+  FaceT<T> zL;           // +z face state of plane P-1
+  FaceT<T> zLnext;       // +z face state of plane P (wave 1 with ZSPLIT, until its z solve)
+  double   inv_max = 0.0;
 
   // prologue: planes -2 and -1 into LDS, plane 0 prefetched
   {
@@ -351,15 +356,15 @@ k_step3d_v3(StepArgs A)
       pf_x = fetch(xgi, xgj, 0);
   }
 
-  auto z_solve = [&](int P, Face const & fzm) {
+  auto z_solve = [&](int P, FaceT<T> const & fzm) {
     // z face P between planes P-1 and P, then plane P-1 is complete
-    const Flux Fz = hllc<3>(S, zL, fzm, dtdx, 2);
+    const FluxT<T> Fz = riemann<3>(S, zL, fzm, dtdx, 2);
     if (P >= 1)
     {
       apply_flux_pair<3>(u2prev, zFprev, Fz, uaprev0, uaprev1);
       fix_volume_fractions(u2prev);
       store_cons<3, B3>(U2, blk_off + i + 16 * j + 256 * (P - 1), u2prev);
-      inv_max = fmax(inv_max, cell_inv_dt<3>(S, u2prev));
+      inv_max = fmax(inv_max, cell_rate<3, T>(S, u2prev));
     }
     zFprev = Fz;
   };
@@ -381,12 +386,12 @@ k_step3d_v3(StepArgs A)
       if (has_x)
         pf_x = fetch(xgi, xgj, P + 2);
     }
-    Face fxm, fym, fzm;
+    FaceT<T> fxm, fym, fzm;
     {
-      Pred3<THINC, HANCOCK> pr;
-      pr.compute(S, sQ, P, i, j, dtdx);
+      Pred3<THINC, HANCOCK, T> pr;
+      pr.compute(S, sQ, P, i, j, dtdxT);
       fzm = pr.template face<2>(S, -1);
-      const Face fzp = pr.template face<2>(S, 1);
+      const FaceT<T> fzp = pr.template face<2>(S, 1);
       if (inner_plane)
       {
         fxm = pr.template face<0>(S, -1);
@@ -403,8 +408,8 @@ k_step3d_v3(StepArgs A)
     }
     if (wave == 1 and inner_plane)
     {
-      Pred3<THINC, HANCOCK> pr;
-      pr.compute(S, sQ, P, rgi, rgj, dtdx);
+      Pred3<THINC, HANCOCK, T> pr;
+      pr.compute(S, sQ, P, rgi, rgj, dtdxT);
       if (rs == 0)
         st_store3(sXL, fx3(0, rr), pr.template face<0>(S, 1));
       else if (rs == 1)
@@ -419,14 +424,16 @@ k_step3d_v3(StepArgs A)
     // (e) Riemann problems on the x and y faces of plane P
     if (inner_plane)
     {
-      fl_store3(sFX, fx3(i, j), hllc<3>(S, st_load3(sXL, fx3(i, j)), fxm, dtdx, 0));
-      fl_store3(sFY, fy3(i, j), hllc<3>(S, st_load3(sYL, fy3(i, j)), fym, dtdx, 1));
+      fl_store3(sFX, sFXE, fx3(i, j), riemann<3>(S, st_load3(sXL, fx3(i, j)), fxm, dtdx, 0));
+      fl_store3(sFY, sFYE, fy3(i, j), riemann<3>(S, st_load3(sYL, fy3(i, j)), fym, dtdx, 1));
       if (wave == 2 and (rs & 1))
       {
         if (rs == 1)
-          fl_store3(sFX, fx3(16, rr), hllc<3>(S, st_load3(sXL, fx3(16, rr)), st_load3<16>(sXR, rr), dtdx, 0));
+          fl_store3(sFX, sFXE, fx3(16, rr),
+                    riemann<3>(S, st_load3(sXL, fx3(16, rr)), st_load3<16>(sXR, rr), dtdx, 0));
         else
-          fl_store3(sFY, fy3(rr, 16), hllc<3>(S, st_load3(sYL, fy3(rr, 16)), st_load3<16>(sYR, rr), dtdx, 1));
+          fl_store3(sFY, sFYE, fy3(rr, 16),
+                    riemann<3>(S, st_load3(sYL, fy3(rr, 16)), st_load3<16>(sYR, rr), dtdx, 1));
       }
     }
     if (!z_early)
@@ -443,8 +450,10 @@ k_step3d_v3(StepArgs A)
     if (inner_plane)
     {
       u2prev = uraw_cur;
-      apply_flux_pair<3>(u2prev, fl_load3(sFX, fx3(i, j)), fl_load3(sFX, fx3(i + 1, j)), uraw_cur.a0, uraw_cur.a1);
-      apply_flux_pair<3>(u2prev, fl_load3(sFY, fy3(i, j)), fl_load3(sFY, fy3(i, j + 1)), uraw_cur.a0, uraw_cur.a1);
+      apply_flux_pair<3>(u2prev, fl_load3(sFX, sFXE, fx3(i, j)), fl_load3(sFX, sFXE, fx3(i + 1, j)), uraw_cur.a0,
+                         uraw_cur.a1);
+      apply_flux_pair<3>(u2prev, fl_load3(sFY, sFYE, fy3(i, j)), fl_load3(sFY, sFYE, fy3(i, j + 1)), uraw_cur.a0,
+                         uraw_cur.a1);
       uaprev0 = uraw_cur.a0;
       uaprev1 = uraw_cur.a1;
     }
@@ -462,11 +471,11 @@ k_step3d_v3(StepArgs A)
   }
 }
 
-// block shape of the 3-D step: 256 (default) or 320 threads, G5_STEP3D_THREADS overrides
+// 0: split z solve (default), 1: z solve before the exchange on every wavefront
+// (G5_STEP3D_VARIANT selects)
 static int
 step3d_variant()
 {
-  // 0: split z solve (default), 1: z solve before the exchange on every wavefront
   static int n = [] {
     const char * e = getenv("G5_STEP3D_VARIANT");
     return e ? atoi(e) : 0;
@@ -474,31 +483,38 @@ step3d_variant()
   return n;
 }
 
-template <bool ZSPLIT>
+template <bool ZSPLIT, typename T>
 static void
 launch3v3(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
 {
   const dim3 grid(8 * A.per_xcd), block(256);
   if (thinc and hancock)
-    hipLaunchKernelGGL((k_step3d_v3<true, true, ZSPLIT>), grid, block, 0, stream, A);
+    hipLaunchKernelGGL((k_step3d_v3<true, true, ZSPLIT, T>), grid, block, 0, stream, A);
   else if (thinc)
-    hipLaunchKernelGGL((k_step3d_v3<true, false, ZSPLIT>), grid, block, 0, stream, A);
+    hipLaunchKernelGGL((k_step3d_v3<true, false, ZSPLIT, T>), grid, block, 0, stream, A);
   else if (hancock)
-    hipLaunchKernelGGL((k_step3d_v3<false, true, ZSPLIT>), grid, block, 0, stream, A);
+    hipLaunchKernelGGL((k_step3d_v3<false, true, ZSPLIT, T>), grid, block, 0, stream, A);
   else
-    hipLaunchKernelGGL((k_step3d_v3<false, false, ZSPLIT>), grid, block, 0, stream, A);
+    hipLaunchKernelGGL((k_step3d_v3<false, false, ZSPLIT, T>), grid, block, 0, stream, A);
 }
 
 void
-launch_step3d(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
+launch_step3d(StepArgs const & A, bool thinc, bool hancock, bool mixed, hipStream_t stream)
 {
-  switch (step3d_variant())
+  const bool split = step3d_variant() != 1;
+  if (mixed)
   {
-    case 1:
-      launch3v3<false>(A, thinc, hancock, stream);
-      break;
-    default:
-      launch3v3<true>(A, thinc, hancock, stream);
+    if (split)
+      launch3v3<true, float>(A, thinc, hancock, stream);
+    else
+      launch3v3<false, float>(A, thinc, hancock, stream);
+  }
+  else
+  {
+    if (split)
+      launch3v3<true, double>(A, thinc, hancock, stream);
+    else
+      launch3v3<false, double>(A, thinc, hancock, stream);
   }
 }
 

@@ -30,6 +30,15 @@ struct SolveDesc
   double  gp[2];      // gamma_k pinf_k/(gamma_k - 1)
   double  thinc_beta;
   double  thinc_eps;
+  // FP32 copies for the mixed-precision path
+  float g1f[2];
+  float gpf[2];
+  float smallrf;
+  float smallcf;
+  float smallpf;
+  float thinc_betaf;
+  float thinc_epsf;
+  float padf;
 };
 
 // per-solve time-stepping control, resident on the device
@@ -95,6 +104,17 @@ make_solve_desc(g5_config const & c, int32_t base, SolveDesc & s)
   }
   s.thinc_beta = c.thinc_beta;
   s.thinc_eps = c.thinc_epsilon;
+  for (int k = 0; k < 2; ++k)
+  {
+    s.g1f[k] = static_cast<float>(s.g1[k]);
+    s.gpf[k] = static_cast<float>(s.gp[k]);
+  }
+  s.smallrf = static_cast<float>(s.smallr);
+  s.smallcf = static_cast<float>(s.smallc);
+  s.smallpf = static_cast<float>(s.smallp);
+  s.thinc_betaf = static_cast<float>(s.thinc_beta);
+  s.thinc_epsf = static_cast<float>(s.thinc_eps);
+  s.padf = 0.0f;
 }
 
 // Initial-condition description for the device generator (InitShockBubble / InitTriplePoint):
