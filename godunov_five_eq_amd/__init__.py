@@ -56,6 +56,63 @@ class G5Region(ctypes.Structure):
     ]
 
 
+class G5Launch(ctypes.Structure):
+    _fields_ = [
+        ("wg_per_cu", ctypes.c_int32),
+        ("xcd_remap", ctypes.c_int32),
+        ("zsplit", ctypes.c_int32),
+        ("autotuned", ctypes.c_int32),
+        ("tuned_ms", ctypes.c_double),
+    ]
+
+
+class G5SolveSpec(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("nb", ctypes.c_int32 * 3),
+        ("kind", ctypes.c_int32),
+        ("shape", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("gamma", ctypes.c_double * 2),
+        ("pinf", ctypes.c_double * 2),
+        ("mach", ctypes.c_double),
+        ("rho1", ctypes.c_double),
+        ("p1", ctypes.c_double),
+        ("rho0", ctypes.c_double),
+        ("arrival", ctypes.c_double),
+    ]
+
+
+class G5ServeConfig(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("n_requests", ctypes.c_int32),
+        ("steps_per_poll", ctypes.c_int32),
+        ("rate_hz", ctypes.c_double),
+        ("frac_3d", ctypes.c_double),
+        ("share_3d", ctypes.c_double),
+        ("hbm_budget_bytes", ctypes.c_uint64),
+        ("precision", ctypes.c_int32),
+        ("thinc_enabled", ctypes.c_int32),
+    ]
+
+
+class G5ServeReport(ctypes.Structure):
+    _fields_ = [
+        ("completed", ctypes.c_int32),
+        ("max_inflight", ctypes.c_int32),
+        ("wall_s", ctypes.c_double),
+        ("cell_updates", ctypes.c_double),
+        ("cell_updates_per_s", ctypes.c_double),
+        ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double),
+        ("mean_ms", ctypes.c_double),
+        ("max_ms", ctypes.c_double),
+        ("last_arrival_s", ctypes.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -103,6 +160,16 @@ def lib():
             "g5_pool_finished": (ctypes.c_int, [vp, P(i32), i32, P(i32)]),
             "g5_pool_usage": (ctypes.c_int, [vp, P(ctypes.c_int64), P(ctypes.c_int64), P(i32)]),
             "g5_pool_last_step_ms": (ctypes.c_int, [vp, P(d)]),
+            "g5_pool_set_launch": (ctypes.c_int, [vp, ctypes.POINTER(G5Launch)]),
+            "g5_pool_get_launch": (ctypes.c_int, [vp, ctypes.POINTER(G5Launch)]),
+            "g5_pool_set_autotune": (ctypes.c_int, [vp, i32]),
+            "g5_pool_properties": (ctypes.c_int, [vp, P(i32), P(i32), P(i32)]),
+            "g5_batch_spec": (ctypes.c_int, [ctypes.c_uint64, i32, P(G5SolveSpec)]),
+            "g5_serve_specs": (ctypes.c_int, [ctypes.c_uint64, i32, d, d, P(G5SolveSpec)]),
+            "g5_spec_config": (ctypes.c_int, [P(G5SolveSpec), i32, i32, P(G5Config)]),
+            "g5_spec_regions": (ctypes.c_int, [P(G5SolveSpec), P(G5Region)]),
+            "g5_pool_admit_spec": (ctypes.c_int, [vp, P(G5SolveSpec), P(i32)]),
+            "g5_serve": (ctypes.c_int, [P(G5ServeConfig), P(G5ServeReport), P(d), sz]),
             "g5_rankine_hugoniot": (ctypes.c_int, [d, d, d, d, d, d, P(d), P(d), P(d), P(d)]),
         }
         for name, (res, args) in sig.items():
@@ -123,7 +190,9 @@ def exported_symbols():
         "g5_pool_create", "g5_pool_destroy", "g5_pool_last_error", "g5_pool_admit", "g5_pool_release",
         "g5_pool_upload", "g5_pool_download", "g5_pool_init_shock_bubble", "g5_pool_init_triple_point",
         "g5_pool_start", "g5_pool_step", "g5_pool_status", "g5_pool_finished", "g5_pool_usage",
-        "g5_pool_last_step_ms",
+        "g5_pool_last_step_ms", "g5_pool_set_launch", "g5_pool_get_launch", "g5_pool_set_autotune",
+        "g5_pool_properties", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
+        "g5_pool_admit_spec", "g5_serve",
     ]
 
 
@@ -298,6 +367,54 @@ class Context:
         return ms.value
 
 
+def batch_spec(seed, index):
+    """Solve `index` of the seeded batching workload (g5_batch_spec)."""
+    sp = G5SolveSpec()
+    rc = lib().g5_batch_spec(seed, index, ctypes.byref(sp))
+    if rc != G5_OK:
+        raise G5Error(rc, "g5_batch_spec")
+    return sp
+
+
+def serve_specs(seed, n, rate_hz, frac_3d=0.1):
+    """Requests of the seeded serving workload (g5_serve_specs)."""
+    arr = (G5SolveSpec * n)()
+    rc = lib().g5_serve_specs(seed, n, rate_hz, frac_3d, arr)
+    if rc != G5_OK:
+        raise G5Error(rc, "g5_serve_specs")
+    return list(arr)
+
+
+def spec_config(spec, thinc=False, precision=PREC_FP64):
+    cfg = G5Config()
+    rc = lib().g5_spec_config(ctypes.byref(spec), 1 if thinc else 0, precision, ctypes.byref(cfg))
+    if rc != G5_OK:
+        raise G5Error(rc, "g5_spec_config")
+    return cfg
+
+
+def spec_regions(spec):
+    regs = (G5Region * 3)()
+    rc = lib().g5_spec_regions(ctypes.byref(spec), regs)
+    if rc != G5_OK:
+        raise G5Error(rc, "g5_spec_regions")
+    return list(regs)
+
+
+def serve(seed, n_requests, rate_hz, *, frac_3d=0.1, share_3d=0.3, budget_bytes=200 << 30,
+          precision=PREC_MIXED, thinc=False, steps_per_poll=4):
+    """Run the serving loop (g5_serve); returns (report dict, per-request latencies in ms)."""
+    c = G5ServeConfig(seed, n_requests, steps_per_poll, rate_hz, frac_3d, share_3d, int(budget_bytes),
+                      precision, 1 if thinc else 0)
+    r = G5ServeReport()
+    lat = np.zeros(n_requests)
+    rc = lib().g5_serve(ctypes.byref(c), ctypes.byref(r), _dptr(lat), lat.size)
+    if rc != G5_OK:
+        raise G5Error(rc, "g5_serve: " + lib().g5_pool_last_error(None).decode())
+    rep = {k: getattr(r, k) for k, _ in G5ServeReport._fields_}
+    return rep, lat
+
+
 class Pool:
     """Many independent solves of one dimension stepped together (g5_pool_* of the C ABI)."""
 
@@ -398,3 +515,41 @@ class Pool:
         ms = ctypes.c_double()
         self._check(lib().g5_pool_last_step_ms(self._h, ctypes.byref(ms)))
         return ms.value
+
+    def set_launch(self, wg_per_cu=None, xcd_remap=1, zsplit=1):
+        """Fix the step kernels' launch geometry; wg_per_cu=None returns to autotuning."""
+        if wg_per_cu is None:
+            self._check(lib().g5_pool_set_launch(self._h, None))
+            return
+        L = G5Launch(wg_per_cu, xcd_remap, zsplit, 0, 0.0)
+        self._check(lib().g5_pool_set_launch(self._h, ctypes.byref(L)))
+
+    def launch(self):
+        L = G5Launch()
+        self._check(lib().g5_pool_get_launch(self._h, ctypes.byref(L)))
+        return dict(wg_per_cu=L.wg_per_cu, xcd_remap=L.xcd_remap, zsplit=L.zsplit,
+                    autotuned=bool(L.autotuned), tuned_ms=L.tuned_ms)
+
+    def set_autotune(self, enable):
+        self._check(lib().g5_pool_set_autotune(self._h, 1 if enable else 0))
+
+    def _spec_cfg(self, spec):
+        d, th, pr = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._check(lib().g5_pool_properties(self._h, ctypes.byref(d), ctypes.byref(th), ctypes.byref(pr)))
+        return spec_config(spec, bool(th.value), pr.value)
+
+    def admit_spec(self, spec):
+        """Admit a workload spec and generate its initial condition on the device."""
+        sid = ctypes.c_int32(-1)
+        self._check(lib().g5_pool_admit_spec(self._h, ctypes.byref(spec), ctypes.byref(sid)))
+        self._cfg[sid.value] = self._spec_cfg(spec)
+        return sid.value
+
+    def try_admit_spec(self, spec):
+        sid = ctypes.c_int32(-1)
+        rc = lib().g5_pool_admit_spec(self._h, ctypes.byref(spec), ctypes.byref(sid))
+        if rc == G5_ERR_NOMEM:
+            return None
+        self._check(rc)
+        self._cfg[sid.value] = self._spec_cfg(spec)
+        return sid.value

@@ -14,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libg5.so")
 
 HIP_SOURCES = ["g5_step2d.hip", "g5_step3d.hip", "g5_aux.hip", "g5_runtime.hip"]
-CXX_SOURCES = ["g5_init.cpp"]
+CXX_SOURCES = ["g5_init.cpp", "g5_workloads.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("G5_OFFLOAD_ARCH", "gfx950")

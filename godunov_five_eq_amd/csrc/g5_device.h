@@ -926,6 +926,36 @@ cell_rate(SolveDesc const & S, Cons const & u)
     return cell_inv_dt_mixed<DIM>(S, u);
 }
 
+// Blocks of the launch list handled by this workgroup: [b, end) with the given stride.  One
+// workgroup per block, or persistent workgroups looping over a range; with remap, the
+// workgroups of XCD x (workgroup ids are dealt round-robin to the 8 XCDs) take the contiguous
+// chunk [x * per_xcd, (x + 1) * per_xcd), so neighbouring blocks, which re-read each other's
+// halo, share an L2.
+G5D void
+block_range(StepArgs const & A, int & b, int & end, int & stride)
+{
+  const int w = blockIdx.x;
+  if (A.wg_per_xcd == 0)
+  {
+    b = A.remap ? (w & 7) * A.per_xcd + (w >> 3) : w;
+    end = min(b + 1, A.nblocks);
+    stride = 1;
+  }
+  else if (A.remap)
+  {
+    const int x = w & 7;
+    b = x * A.per_xcd + (w >> 3);
+    end = min((x + 1) * A.per_xcd, A.nblocks);
+    stride = A.wg_per_xcd;
+  }
+  else
+  {
+    b = w;
+    end = A.nblocks;
+    stride = static_cast<int>(gridDim.x);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // ghost/border resolution.  Block N (brick coordinates, already wrapped in periodic directions,
 // possibly one step outside the brick) and in-block cell c map to (offset of the source cell in the

@@ -10,8 +10,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "g5_kernels.h"
@@ -27,6 +31,7 @@ std::string g_create_error;
 struct SolveRec
 {
   bool                 live = false;
+  bool                 started = false; // g5_pool_start ran for it
   g5_config            cfg{};
   int32_t              nblk = 0;
   int32_t              table_base = 0;
@@ -110,6 +115,21 @@ struct g5_pool
   // HIP graph of one device-dt step over the current launch list
   hipGraphExec_t graph = nullptr;
   bool           graph_valid = false;
+
+  // scratch of g5_pool_start (block list and ids of the solves being started)
+  BlockDesc * d_start_list = nullptr;
+  int64_t     cap_start_list = 0;
+  int32_t *   d_start_ids = nullptr;
+  int64_t     cap_start_ids = 0;
+
+  // launch geometry of the step kernels: fixed by the caller, or chosen by the autotuner for the
+  // current launch-list size class
+  LaunchCfg launch{ 0, 1, 1 };
+  bool      launch_fixed = false;
+  bool      autotune = true;
+  int32_t   tuned_class = -1; // size class the current launch was tuned (or looked up) for
+  double    tuned_ms = 0.0;
+  int32_t   cus = 256;
 
   double last_run_ms = 0.0;
 };
@@ -228,6 +248,12 @@ pool_create(int dim, int thinc, int precision, uint64_t budget, bool rk2_stage, 
   hipError_t e = hipStreamCreate(&p->stream);
   if (e != hipSuccess)
     return bail(G5_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess and
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess and cus > 0)
+    p->cus = cus;
+  if (char const * at = std::getenv("G5_AUTOTUNE"))
+    p->autotune = std::atoi(at) != 0;
   const size_t bytes = static_cast<size_t>(p->nslots) * p->block_doubles * sizeof(double);
   for (int k = 0; k < 2; ++k)
     if ((e = hipMalloc(&p->buf[k], bytes)) != hipSuccess)
@@ -288,6 +314,7 @@ pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, i
     return rc;
   SolveRec & r = p->solves[sid];
   r.live = true;
+  r.started = false;
   r.cfg = cfg;
   for (int d = cfg.dim; d < 3; ++d)
     r.cfg.nb[d] = 1;
@@ -326,8 +353,8 @@ pool_release(g5_pool * p, int32_t id)
     return fail(p->err, G5_ERR_INVALID, "no such solve");
   G5_HIP(p->err, hipStreamSynchronize(p->stream));
   SolveRec & r = p->solves[id];
-  for (auto s : r.slots)
-    p->free_slots.push_back(s);
+  for (auto it = r.slots.rbegin(); it != r.slots.rend(); ++it)
+    p->free_slots.push_back(*it); // the next admission pops them in ascending order
   p->table.give(r.table_base, r.nblk);
   r = SolveRec();
   p->free_ids.push_back(id);
@@ -389,6 +416,8 @@ step_args(g5_pool * p, BlockDesc const * blocks, int32_t n, double dt_fixed)
   A.slots = p->d_slots;
   A.nblocks = n;
   A.per_xcd = (n + 7) / 8;
+  A.wg_per_xcd = (p->dim == 2 and p->launch.wg_per_cu > 0) ? p->launch.wg_per_cu * std::max(1, p->cus / 8) : 0;
+  A.remap = p->launch.remap;
   A.dt_fixed = dt_fixed;
   return A;
 }
@@ -402,7 +431,123 @@ launch_step(g5_pool * p, StepArgs const & A, bool hancock)
   if (p->dim == 2)
     launch_step2d(A, p->thinc != 0, hancock, mixed, p->stream);
   else
-    launch_step3d(A, p->thinc != 0, hancock, mixed, p->stream);
+    launch_step3d(A, p->thinc != 0, hancock, mixed, p->launch.zsplit != 0, p->stream);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Autotuner of the launch geometry.  Candidates: one workgroup per block or persistent
+// workgroups (1, 2, 4 or 8 per CU, within what the kernel's occupancy allows), XCD-contiguous
+// block ranges or plain order, and for 3-D the two z-solve placements.  Each candidate runs the
+// device-dt step kernel of the live launch list a few times; a step kernel only writes the next
+// state buffer and max-accumulates the next CFL rate, both functions of the current state alone,
+// so the repeated launches leave exactly what the real step will write.  The best candidate is
+// kept per (dim, THINC, precision, integrator, list-size class) for the life of the process.
+// ---------------------------------------------------------------------------------------------
+struct TuneKey
+{
+  int32_t dim, thinc, precision, hancock, size_class;
+  bool
+  operator<(TuneKey const & o) const
+  {
+    return std::tie(dim, thinc, precision, hancock, size_class) <
+           std::tie(o.dim, o.thinc, o.precision, o.hancock, o.size_class);
+  }
+};
+struct TuneVal
+{
+  LaunchCfg cfg;
+  double    ms;
+};
+std::mutex                    g_tune_mutex;
+std::map<TuneKey, TuneVal>    g_tune_cache;
+
+int32_t
+size_class(int32_t nblocks)
+{
+  int32_t c = 0;
+  while ((1 << (c + 1)) <= nblocks)
+    ++c;
+  return c; // floor(log2(nblocks))
+}
+
+std::vector<LaunchCfg>
+tune_candidates(g5_pool const * p)
+{
+  std::vector<LaunchCfg> c;
+  // resident workgroups per CU: 2-D about 2 (FP64) or 5 (mixed); 3-D 1 (FP64) or 2 (mixed)
+  const int max_wg = p->dim == 2 ? (p->precision == G5_PREC_MIXED ? 8 : 2) : (p->precision == G5_PREC_MIXED ? 2 : 1);
+  for (int z = (p->dim == 3 ? 0 : 1); z <= 1; ++z)
+  {
+    c.push_back({ 0, 1, z });
+    c.push_back({ 0, 0, z });
+    if (p->dim == 2) // the 3-D step has no persistent form
+      for (int w = 1; w <= max_wg; w *= 2)
+        c.push_back({ w, 1, z });
+  }
+  return c;
+}
+
+int
+autotune(g5_pool * p, bool hancock)
+{
+  const TuneKey key{ p->dim, p->thinc, p->precision, hancock ? 1 : 0, size_class(p->nlist) };
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end())
+    {
+      p->launch = it->second.cfg;
+      p->tuned_ms = it->second.ms;
+      p->tuned_class = key.size_class;
+      return G5_OK;
+    }
+  }
+  hipEvent_t e0, e1;
+  G5_HIP(p->err, hipEventCreate(&e0));
+  G5_HIP(p->err, hipEventCreate(&e1));
+  LaunchCfg best = p->launch;
+  double    best_ms = 1e300;
+  for (LaunchCfg const & c : tune_candidates(p))
+  {
+    p->launch = c;
+    const StepArgs A = step_args(p, p->d_blocks, p->nlist, 0.0);
+    launch_step(p, A, hancock); // warm-up
+    G5_HIP(p->err, hipEventRecord(e0, p->stream));
+    constexpr int reps = 3;
+    for (int r = 0; r < reps; ++r)
+      launch_step(p, A, hancock);
+    G5_HIP(p->err, hipEventRecord(e1, p->stream));
+    G5_HIP(p->err, hipEventSynchronize(e1));
+    float ms = 0.f;
+    G5_HIP(p->err, hipEventElapsedTime(&ms, e0, e1));
+    if (ms / reps < best_ms)
+    {
+      best_ms = ms / reps;
+      best = c;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  G5_HIP(p->err, hipGetLastError());
+  p->launch = best;
+  p->tuned_ms = best_ms;
+  p->tuned_class = key.size_class;
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  g_tune_cache[key] = TuneVal{ best, best_ms };
+  return G5_OK;
+}
+
+// before a device-dt step over a new launch list: tune (or look up) its launch geometry
+int
+ensure_tuned(g5_pool * p, bool hancock)
+{
+  if (p->launch_fixed or !p->autotune or p->nlist == 0 or p->tuned_class == size_class(p->nlist))
+    return G5_OK;
+  const LaunchCfg before = p->launch;
+  const int       rc = autotune(p, hancock);
+  if (rc == G5_OK and std::memcmp(&before, &p->launch, sizeof(LaunchCfg)) != 0)
+    p->graph_valid = false;
+  return rc;
 }
 
 bool
@@ -446,6 +591,9 @@ pool_step(g5_pool * p, int32_t nsteps)
   int rc = rebuild_list(p);
   if (rc != G5_OK)
     return rc;
+  rc = ensure_tuned(p, pool_hancock(p));
+  if (rc != G5_OK)
+    return rc;
   hipEvent_t e0, e1;
   G5_HIP(p->err, hipEventCreate(&e0));
   G5_HIP(p->err, hipEventCreate(&e1));
@@ -478,19 +626,22 @@ pool_start(g5_pool * p, int32_t const * ids, int32_t n)
       return fail(p->err, G5_ERR_INVALID, "no such solve");
     append_blocks(p, ids[k], list);
   }
-  BlockDesc * d_list = nullptr;
-  int32_t *   d_ids = nullptr;
-  G5_HIP(p->err, hipMalloc(&d_list, std::max<size_t>(1, list.size()) * sizeof(BlockDesc)));
-  G5_HIP(p->err, hipMalloc(&d_ids, std::max<int32_t>(1, n) * sizeof(int32_t)));
-  G5_HIP(p->err, hipMemcpyAsync(d_list, list.data(), list.size() * sizeof(BlockDesc), hipMemcpyHostToDevice, p->stream));
-  G5_HIP(p->err, hipMemcpyAsync(d_ids, ids, n * sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
+  int rc = grow(p, p->d_start_list, p->cap_start_list, std::max<int64_t>(1, list.size()), p->stream);
+  if (rc == G5_OK)
+    rc = grow(p, p->d_start_ids, p->cap_start_ids, std::max<int64_t>(1, n), p->stream);
+  if (rc != G5_OK)
+    return rc;
+  G5_HIP(p->err, hipMemcpyAsync(p->d_start_list, list.data(), list.size() * sizeof(BlockDesc), hipMemcpyHostToDevice,
+                                p->stream));
+  G5_HIP(p->err, hipMemcpyAsync(p->d_start_ids, ids, n * sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
   // invdt of these solves is 0 (admit and finalize leave it cleared)
-  launch_inv_dt(p->dim, p->precision == G5_PREC_MIXED, p->buf[0], p->buf[1], d_list, p->d_solves, p->d_ctl, static_cast<int>(list.size()), p->stream);
-  launch_start(p->d_ctl, p->d_solves, d_ids, n, p->stream);
+  launch_inv_dt(p->dim, p->precision == G5_PREC_MIXED, p->buf[0], p->buf[1], p->d_start_list, p->d_solves, p->d_ctl,
+                static_cast<int>(list.size()), p->stream);
+  launch_start(p->d_ctl, p->d_solves, p->d_start_ids, n, p->stream);
   G5_HIP(p->err, hipGetLastError());
   G5_HIP(p->err, hipStreamSynchronize(p->stream));
-  hipFree(d_list);
-  hipFree(d_ids);
+  for (int32_t k = 0; k < n; ++k)
+    p->solves[ids[k]].started = true;
   return G5_OK;
 }
 
@@ -661,6 +812,8 @@ g5_pool_destroy(g5_pool * p)
   hipFree(p->d_ctl);
   hipFree(p->d_slots);
   hipFree(p->d_blocks);
+  hipFree(p->d_start_list);
+  hipFree(p->d_start_ids);
   if (p->h_ctl)
     hipHostFree(p->h_ctl);
   if (p->stream)
@@ -788,7 +941,7 @@ g5_pool_finished(g5_pool * p, int32_t * ids, int32_t cap, int32_t * n_done)
     return rc;
   int32_t n = 0;
   for (int32_t s = 0; s < static_cast<int32_t>(p->solves.size()); ++s)
-    if (p->solves[s].live and p->h_ctl[s].active == 0)
+    if (p->solves[s].live and p->solves[s].started and p->h_ctl[s].active == 0)
     {
       if (ids and n < cap)
         ids[n] = s;
@@ -823,6 +976,79 @@ g5_pool_last_step_ms(const g5_pool * p, double * ms)
   if (not p or not ms)
     return G5_ERR_INVALID;
   *ms = p->last_run_ms;
+  return G5_OK;
+}
+
+namespace
+{
+void
+to_launch(LaunchCfg const & c, g5_pool const * p, g5_launch * out)
+{
+  out->wg_per_cu = c.wg_per_cu;
+  out->xcd_remap = c.remap;
+  out->zsplit = c.zsplit;
+  out->autotuned = (!p->launch_fixed and p->autotune and p->tuned_class >= 0) ? 1 : 0;
+  out->tuned_ms = p->tuned_ms;
+}
+int
+set_launch(g5_pool * p, g5_launch const * l)
+{
+  if (l == nullptr)
+  {
+    p->launch_fixed = false;
+    p->tuned_class = -1; // re-tune (or look up) at the next step
+    return G5_OK;
+  }
+  if (l->wg_per_cu < 0 or l->wg_per_cu > 16 or (l->xcd_remap != 0 and l->xcd_remap != 1) or
+      (l->zsplit != 0 and l->zsplit != 1))
+    return fail(p->err, G5_ERR_INVALID, "launch: wg_per_cu in [0,16], xcd_remap and zsplit in {0,1}");
+  if (p->dim == 3 and l->wg_per_cu != 0)
+    return fail(p->err, G5_ERR_UNSUPPORTED, "the 3-D step has no persistent form (wg_per_cu must be 0)");
+  p->launch = LaunchCfg{ l->wg_per_cu, l->xcd_remap, l->zsplit };
+  p->launch_fixed = true;
+  p->graph_valid = false;
+  return G5_OK;
+}
+} // namespace
+
+int
+g5_pool_set_launch(g5_pool * p, const g5_launch * launch)
+{
+  if (not p)
+    return G5_ERR_INVALID;
+  return set_launch(p, launch);
+}
+
+int
+g5_pool_get_launch(const g5_pool * p, g5_launch * launch)
+{
+  if (not p or not launch)
+    return G5_ERR_INVALID;
+  to_launch(p->launch, p, launch);
+  return G5_OK;
+}
+
+int
+g5_pool_properties(const g5_pool * p, int32_t * dim, int32_t * thinc_enabled, int32_t * precision)
+{
+  if (not p)
+    return G5_ERR_INVALID;
+  if (dim)
+    *dim = p->dim;
+  if (thinc_enabled)
+    *thinc_enabled = p->thinc;
+  if (precision)
+    *precision = p->precision;
+  return G5_OK;
+}
+
+int
+g5_pool_set_autotune(g5_pool * p, int32_t enable)
+{
+  if (not p)
+    return G5_ERR_INVALID;
+  p->autotune = enable != 0;
+  p->tuned_class = -1;
   return G5_OK;
 }
 

@@ -234,32 +234,20 @@ predict_faces2(SolveDesc const & S, T const * sQ, int gi, int gj, T dtdx, bool a
 
 } // namespace
 
-G5D int
-logical_block(StepArgs const & A)
-{
-  // workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous run of blocks so
-  // that neighbouring blocks (which re-read each other's halo) share an L2
-  const int w = blockIdx.x;
-  return (w & 7) * A.per_xcd + (w >> 3);
-}
-
 // LDS bytes of the face-state / flux exchange area: 12 face-state rows or 14 flux rows of T plus
 // 2 energy-flux rows of double
 template <typename T>
 constexpr int SE2_DOUBLES = (14 * NF2 * static_cast<int>(sizeof(T)) + 2 * NF2 * 8) / 8;
 
 template <bool THINC, bool HANCOCK, typename T>
-__global__ void __launch_bounds__(256)
-k_step2d(StepArgs A)
+G5D void
+step2d_block(StepArgs const & A, int blk)
 {
   __shared__ T      sQ[6 * NT2];
   __shared__ double sE[SE2_DOUBLES<T>];
   __shared__ double sRed[4];
 
-  const int t = threadIdx.x;
-  const int blk = logical_block(A);
-  if (blk >= A.nblocks)
-    return;
+  const int       t = threadIdx.x;
   const BlockDesc bd = A.blocks[blk];
   const SolveDesc S = A.solves[bd.solve];
   SolveCtl *      C = A.ctl + bd.solve;
@@ -364,19 +352,54 @@ k_step2d(StepArgs A)
   }
 }
 
+// one workgroup per block
+template <bool THINC, bool HANCOCK, typename T>
+__global__ void __launch_bounds__(256)
+k_step2d(StepArgs A)
+{
+  int b, end, stride;
+  block_range(A, b, end, stride);
+  if (b < end)
+    step2d_block<THINC, HANCOCK, T>(A, b);
+}
+
+// persistent workgroups looping over their range of the block list
+template <bool THINC, bool HANCOCK, typename T>
+__global__ void __launch_bounds__(256)
+k_step2d_persistent(StepArgs A)
+{
+  int b, end, stride;
+  block_range(A, b, end, stride);
+  for (; b < end; b += stride)
+  {
+    step2d_block<THINC, HANCOCK, T>(A, b);
+    __syncthreads(); // the next block reuses the LDS
+  }
+}
+
+template <bool THINC, bool HANCOCK, typename T>
+static void
+launch2k(StepArgs const & A, hipStream_t stream)
+{
+  const dim3 block(256);
+  if (A.wg_per_xcd > 0)
+    hipLaunchKernelGGL((k_step2d_persistent<THINC, HANCOCK, T>), dim3(8 * A.wg_per_xcd), block, 0, stream, A);
+  else
+    hipLaunchKernelGGL((k_step2d<THINC, HANCOCK, T>), dim3(8 * A.per_xcd), block, 0, stream, A);
+}
+
 template <typename T>
 static void
 launch2(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
 {
-  const dim3 grid(8 * A.per_xcd), block(256);
   if (thinc and hancock)
-    hipLaunchKernelGGL((k_step2d<true, true, T>), grid, block, 0, stream, A);
+    launch2k<true, true, T>(A, stream);
   else if (thinc)
-    hipLaunchKernelGGL((k_step2d<true, false, T>), grid, block, 0, stream, A);
+    launch2k<true, false, T>(A, stream);
   else if (hancock)
-    hipLaunchKernelGGL((k_step2d<false, true, T>), grid, block, 0, stream, A);
+    launch2k<false, true, T>(A, stream);
   else
-    hipLaunchKernelGGL((k_step2d<false, false, T>), grid, block, 0, stream, A);
+    launch2k<false, false, T>(A, stream);
 }
 
 void

@@ -21,8 +21,6 @@
 // its update, and the global loads of plane P+2 are issued one phase ahead of their use.  With
 // ZSPLIT the z Riemann solve and the completion of plane P-1 run before the x/y exchange on
 // waves 0, 2, 3 and after it on wave 1, which carries the ring predictor.
-#include <cstdlib>
-
 #include "g5_device.h"
 #include "g5_kernels.h"
 
@@ -177,13 +175,6 @@ aslot(int plane)
 
 } // namespace
 
-G5D int
-logical_block3(StepArgs const & A)
-{
-  const int w = blockIdx.x;
-  return (w & 7) * A.per_xcd + (w >> 3);
-}
-
 namespace
 {
 constexpr int QPL = 7 * NT3; // values per primitive plane
@@ -269,8 +260,8 @@ struct Pred3
 } // namespace
 
 template <bool THINC, bool HANCOCK, bool ZSPLIT, typename T>
-__global__ void __launch_bounds__(256)
-k_step3d_v3(StepArgs A)
+G5D void
+step3d_body(StepArgs const & A, int blk)
 {
   __shared__ T      sQ[3 * QPL];
   __shared__ T      sXL[7 * NF3];
@@ -283,10 +274,7 @@ k_step3d_v3(StepArgs A)
   __shared__ double sFYE[NF3];
   __shared__ double sRed[4];
 
-  const int t = threadIdx.x;
-  const int blk = logical_block3(A);
-  if (blk >= A.nblocks)
-    return;
+  const int       t = threadIdx.x;
   const BlockDesc bd = A.blocks[blk];
   const SolveDesc S = A.solves[bd.solve];
   SolveCtl *      C = A.ctl + bd.solve;
@@ -471,16 +459,33 @@ k_step3d_v3(StepArgs A)
   }
 }
 
-// 0: split z solve (default), 1: z solve before the exchange on every wavefront
-// (G5_STEP3D_VARIANT selects)
-static int
-step3d_variant()
+// FP64: one wavefront per SIMD (about 330 registers); mixed: two wavefronts per SIMD (at most
+// 256 registers, and 72 KB of LDS per workgroup)
+// One workgroup per block.  (A persistent form, workgroups looping over several blocks, made
+// an illegal memory access on the GPU with no cause found in the code; the runtime never asks
+// for it in 3-D: the launch must have wg_per_xcd == 0.)
+template <bool THINC, bool HANCOCK, bool ZSPLIT, typename T>
+G5D void
+step3d_blocks(StepArgs const & A)
 {
-  static int n = [] {
-    const char * e = getenv("G5_STEP3D_VARIANT");
-    return e ? atoi(e) : 0;
-  }();
-  return n;
+  int b, end, stride;
+  block_range(A, b, end, stride);
+  if (b < end)
+    step3d_body<THINC, HANCOCK, ZSPLIT, T>(A, b);
+}
+
+template <bool THINC, bool HANCOCK, bool ZSPLIT>
+__global__ void __launch_bounds__(256)
+k_step3d_v3(StepArgs A)
+{
+  step3d_blocks<THINC, HANCOCK, ZSPLIT, double>(A);
+}
+
+template <bool THINC, bool HANCOCK, bool ZSPLIT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_step3d_mixed(StepArgs A)
+{
+  step3d_blocks<THINC, HANCOCK, ZSPLIT, float>(A);
 }
 
 template <bool ZSPLIT, typename T>
@@ -488,20 +493,33 @@ static void
 launch3v3(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
 {
   const dim3 grid(8 * A.per_xcd), block(256);
-  if (thinc and hancock)
-    hipLaunchKernelGGL((k_step3d_v3<true, true, ZSPLIT, T>), grid, block, 0, stream, A);
-  else if (thinc)
-    hipLaunchKernelGGL((k_step3d_v3<true, false, ZSPLIT, T>), grid, block, 0, stream, A);
-  else if (hancock)
-    hipLaunchKernelGGL((k_step3d_v3<false, true, ZSPLIT, T>), grid, block, 0, stream, A);
+  if constexpr (sizeof(T) == sizeof(double))
+  {
+    if (thinc and hancock)
+      hipLaunchKernelGGL((k_step3d_v3<true, true, ZSPLIT>), grid, block, 0, stream, A);
+    else if (thinc)
+      hipLaunchKernelGGL((k_step3d_v3<true, false, ZSPLIT>), grid, block, 0, stream, A);
+    else if (hancock)
+      hipLaunchKernelGGL((k_step3d_v3<false, true, ZSPLIT>), grid, block, 0, stream, A);
+    else
+      hipLaunchKernelGGL((k_step3d_v3<false, false, ZSPLIT>), grid, block, 0, stream, A);
+  }
   else
-    hipLaunchKernelGGL((k_step3d_v3<false, false, ZSPLIT, T>), grid, block, 0, stream, A);
+  {
+    if (thinc and hancock)
+      hipLaunchKernelGGL((k_step3d_mixed<true, true, ZSPLIT>), grid, block, 0, stream, A);
+    else if (thinc)
+      hipLaunchKernelGGL((k_step3d_mixed<true, false, ZSPLIT>), grid, block, 0, stream, A);
+    else if (hancock)
+      hipLaunchKernelGGL((k_step3d_mixed<false, true, ZSPLIT>), grid, block, 0, stream, A);
+    else
+      hipLaunchKernelGGL((k_step3d_mixed<false, false, ZSPLIT>), grid, block, 0, stream, A);
+  }
 }
 
 void
-launch_step3d(StepArgs const & A, bool thinc, bool hancock, bool mixed, hipStream_t stream)
+launch_step3d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool split, hipStream_t stream)
 {
-  const bool split = step3d_variant() != 1;
   if (mixed)
   {
     if (split)

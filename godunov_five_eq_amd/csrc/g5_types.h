@@ -73,8 +73,20 @@ struct StepArgs
   SolveCtl *        ctl;
   const int32_t *   slots;   // slot table
   int32_t           nblocks;
-  int32_t           per_xcd; // workgroups per XCD chunk (grid = 8 * per_xcd)
-  double            dt_fixed; // > 0: use this dt for every solve (host-driven step)
+  int32_t           per_xcd;    // blocks per XCD chunk, ceil(nblocks / 8)
+  int32_t           wg_per_xcd; // 0: one workgroup per block (grid 8 * per_xcd); > 0: persistent
+                                // workgroups per XCD, each looping over its XCD's chunk
+  int32_t           remap;      // 1: XCD x works on blocks [x * per_xcd, (x + 1) * per_xcd)
+  double            dt_fixed;   // > 0: use this dt for every solve (host-driven step)
+};
+
+// launch geometry of the step kernels (chosen by the autotuner or set by the caller); every
+// choice computes the same bits
+struct LaunchCfg
+{
+  int32_t wg_per_cu; // 0: one workgroup per block; > 0: persistent, this many workgroups per CU
+  int32_t remap;     // XCD-contiguous block ranges
+  int32_t zsplit;    // 3-D: z Riemann solve split around the x/y exchange
 };
 
 // filling the host-side SolveDesc from a g5_config

@@ -1,0 +1,416 @@
+// g5_workloads.cpp -- seeded workloads of the batching and serving forms, and the serving
+// request loop (continuous batching over two pools).
+//
+// A solve spec describes one independent 2-D or 3-D run: grid, initial-condition family
+// (shock-bubble with a circle/sphere or rectangle/box inclusion, or triple point), the two
+// stiffened-gas materials, the shock Mach number and the step budget.  Specs are drawn from
+// std::mt19937_64 (whose output sequence the C++ standard fixes) through this file's own
+// uniform mappings, so a (seed, index) pair names the same solve on every platform.
+//
+// Initial conditions follow the reference's generators: init/InitShockBubble.cpp:50-225
+// (post-shock region for x <= x_front, pre-shock region, inclusion of material 0) and
+// init/InitTriplePoint.cpp:46-133 (left region for x <= xd, right regions split at y = yd).
+// The post-shock state comes from the stiffened-gas Rankine-Hugoniot relations
+// (g5_rankine_hugoniot, app/tools/moving_shock.py:79-163 for the ideal-gas form).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/godunov_five_eq.h"
+
+namespace
+{
+
+// uniform double in [0, 1) from the top 53 bits
+double
+u01(std::mt19937_64 & g)
+{
+  return static_cast<double>(g() >> 11) * 0x1.0p-53;
+}
+double
+uniform(std::mt19937_64 & g, double a, double b)
+{
+  return a + (b - a) * u01(g);
+}
+// uniform integer in [0, n)
+int64_t
+below(std::mt19937_64 & g, int64_t n)
+{
+  return static_cast<int64_t>(u01(g) * static_cast<double>(n));
+}
+// uniform multiple of 16 in [lo, hi] (both multiples of 16)
+int32_t
+side16(std::mt19937_64 & g, int32_t lo, int32_t hi)
+{
+  return lo + 16 * static_cast<int32_t>(below(g, (hi - lo) / 16 + 1));
+}
+
+std::mt19937_64
+stream_of(uint64_t seed, uint64_t index)
+{
+  // one generator per (seed, index): splitmix64 of the pair
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * (index + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return std::mt19937_64(z);
+}
+
+// materials, Mach number and IC family shared by both workloads
+void
+draw_physics(std::mt19937_64 & g, g5_solve_spec & s)
+{
+  s.kind = static_cast<int32_t>(below(g, 2));
+  s.shape = static_cast<int32_t>(below(g, 2));
+  s.gamma[1] = uniform(g, 1.1, 4.4); // surrounding material (carries the shock)
+  s.pinf[1] = below(g, 2) ? 6e8 : 0.0;
+  s.gamma[0] = uniform(g, 1.1, 4.4); // inclusion (gas)
+  s.pinf[0] = 0.0;
+  s.mach = uniform(g, 1.1, 3.0);
+  if (s.pinf[1] > 0.0)
+  {
+    // liquid-like surrounding material (water: rho 1000 kg/m^3, 1 bar) with a gas inclusion
+    s.rho1 = 1000.0;
+    s.p1 = 1e5;
+    s.rho0 = uniform(g, 1.0, 10.0);
+  }
+  else
+  {
+    // gas-gas (normalised units, as the air-helium shock-bubble)
+    s.rho1 = 1.0;
+    s.p1 = 1.0;
+    s.rho0 = uniform(g, 0.1, 5.0);
+  }
+}
+
+} // namespace
+
+extern "C" {
+
+int
+g5_batch_spec(uint64_t seed, int32_t index, g5_solve_spec * out)
+{
+  if (not out or index < 0)
+    return G5_ERR_INVALID;
+  std::memset(out, 0, sizeof(*out));
+  std::mt19937_64 g = stream_of(seed, static_cast<uint64_t>(index));
+  static const int32_t sides[4] = { 64, 128, 256, 512 };
+  const int32_t        side = sides[below(g, 4)];
+  const double         aspect = uniform(g, 1.0, 4.0);
+  const int32_t        lng = std::max<int32_t>(side, 16 * static_cast<int32_t>(std::lround(side * aspect / 16.0)));
+  out->dim = 2;
+  out->nb[0] = lng / 16;
+  out->nb[1] = side / 16;
+  out->nb[2] = 1;
+  out->steps = 100;
+  draw_physics(g, *out);
+  out->arrival = 0.0;
+  return G5_OK;
+}
+
+int
+g5_serve_specs(uint64_t seed, int32_t n, double rate_hz, double frac_3d, g5_solve_spec * out)
+{
+  if (not out or n < 0 or not(rate_hz > 0.0) or frac_3d < 0.0 or frac_3d > 1.0)
+    return G5_ERR_INVALID;
+  std::mt19937_64 arrivals = stream_of(seed, 0xA11ull << 40);
+  double          t = 0.0;
+  for (int32_t k = 0; k < n; ++k)
+  {
+    g5_solve_spec & s = out[k];
+    std::memset(&s, 0, sizeof(s));
+    // Poisson process: exponential inter-arrival times of mean 1/rate
+    t += -std::log(1.0 - u01(arrivals)) / rate_hz;
+    s.arrival = t;
+    std::mt19937_64 g = stream_of(seed, static_cast<uint64_t>(k));
+    if (u01(g) < frac_3d)
+    {
+      s.dim = 3;
+      for (int d = 0; d < 3; ++d)
+        s.nb[d] = side16(g, 64, 128) / 16;
+    }
+    else
+    {
+      s.dim = 2;
+      s.nb[0] = side16(g, 64, 1024) / 16;
+      s.nb[1] = side16(g, 64, 1024) / 16;
+      s.nb[2] = 1;
+    }
+    s.steps = 50 + static_cast<int32_t>(below(g, 451));
+    draw_physics(g, s);
+  }
+  return G5_OK;
+}
+
+int
+g5_spec_config(const g5_solve_spec * s, int32_t thinc, int32_t precision, g5_config * cfg)
+{
+  if (not s or not cfg or (s->dim != 2 and s->dim != 3))
+    return G5_ERR_INVALID;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->dim = s->dim;
+  cfg->bsize = 16;
+  for (int d = 0; d < 3; ++d)
+    cfg->nb[d] = d < s->dim ? s->nb[d] : 1;
+  // unit height: the shortest transverse direction holds 1.0
+  int32_t nshort = s->nb[1];
+  if (s->dim == 3)
+    nshort = std::min(nshort, s->nb[2]);
+  cfg->dx = 1.0 / (16.0 * nshort);
+  const bool triple = s->kind == 1;
+  for (int f = 0; f < 6; ++f)
+  {
+    const int d = f / 2;
+    if (d >= s->dim)
+      cfg->bc[f] = G5_BC_NONE;
+    else if (d == 0 or triple)
+      cfg->bc[f] = G5_BC_ZERO_GRADIENT; // inflow/outflow along the shock (all sides: triple point)
+    else
+      cfg->bc[f] = G5_BC_WALL; // shock tube walls of the shock-bubble
+  }
+  cfg->integrator = G5_TI_HANCOCK;
+  cfg->thinc_enabled = thinc;
+  cfg->precision = precision;
+  cfg->cfl = 0.5;
+  cfg->slope_type = triple ? 2.0 : 1.0; // MC for the triple point (test_triple_point_2d.ini), else minmod
+  cfg->smallr = 1e-10;
+  cfg->smallc = 1e-10;
+  for (int k = 0; k < 2; ++k)
+  {
+    cfg->gamma[k] = s->gamma[k];
+    cfg->pinf[k] = s->pinf[k];
+  }
+  cfg->thinc_beta = 1.6;
+  cfg->thinc_epsilon = 1e-3;
+  return G5_OK;
+}
+
+int
+g5_spec_regions(const g5_solve_spec * s, g5_region * regions3)
+{
+  if (not s or not regions3)
+    return G5_ERR_INVALID;
+  double    rho2, u2, p2, ws;
+  const int rc = g5_rankine_hugoniot(s->gamma[1], s->pinf[1], s->rho1, s->p1, s->mach, 0.0, &rho2, &u2, &p2, &ws);
+  if (rc != G5_OK)
+    return rc;
+  std::memset(regions3, 0, 3 * sizeof(g5_region));
+  // region 0: post-shock material 1; region 1: pre-shock material 1; region 2: material 0
+  regions3[0].alpha0 = 0.0;
+  regions3[0].alpha_rho[1] = rho2;
+  regions3[0].p = p2;
+  regions3[0].u = u2;
+  regions3[1].alpha0 = 0.0;
+  regions3[1].alpha_rho[1] = s->rho1;
+  regions3[1].p = s->p1;
+  regions3[2].alpha0 = 1.0;
+  regions3[2].alpha_rho[0] = s->rho0;
+  regions3[2].p = s->p1;
+  return G5_OK;
+}
+
+int
+g5_pool_admit_spec(g5_pool * pool, const g5_solve_spec * s, int32_t * solve_id)
+{
+  if (not pool or not s or not solve_id)
+    return G5_ERR_INVALID;
+  int32_t dim = 0, thinc = 0, precision = 0;
+  int     rc = g5_pool_properties(pool, &dim, &thinc, &precision);
+  if (rc != G5_OK)
+    return rc;
+  if (s->dim != dim)
+    return G5_ERR_INVALID;
+  g5_config cfg;
+  g5_region reg[3];
+  if ((rc = g5_spec_config(s, thinc, precision, &cfg)) != G5_OK or (rc = g5_spec_regions(s, reg)) != G5_OK)
+    return rc;
+  int32_t id = -1;
+  rc = g5_pool_admit(pool, &cfg, 1e300, s->steps, &id);
+  if (rc != G5_OK)
+    return rc;
+  if (s->kind == 1)
+    rc = g5_pool_init_triple_point(pool, id, reg, 1.0 / 3.0, 0.5);
+  else
+  {
+    // inclusion centred at (0.55, 0.5, 0.5): circle/sphere of radius 0.25 or box of half-sizes
+    // (0.2, 0.3, 0.3); the shock starts at x = 0.2
+    const double center[3] = { 0.55, 0.5, 0.5 };
+    const double size_circle[3] = { 0.25, 0.25, 0.25 };
+    const double size_box[3] = { 0.2, 0.3, 0.3 };
+    rc = g5_pool_init_shock_bubble(pool, id, reg, s->shape, 0.2, center, s->shape ? size_box : size_circle);
+  }
+  if (rc != G5_OK)
+  {
+    g5_pool_release(pool, id);
+    return rc;
+  }
+  *solve_id = id;
+  return G5_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Serving loop: requests arrive on a Poisson clock; each is admitted into the pool of its
+// dimension as soon as the pool's HBM share holds it (first come, first served), its initial
+// condition is generated on the device, and it is stepped together with every other solve in
+// flight; a finished solve is downloaded (the result) and released, which frees its memory for
+// the queue.  Latency = result on the host - arrival.
+// ---------------------------------------------------------------------------------------------
+int
+g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, size_t cap)
+{
+  if (not c or not r or c->n_requests <= 0 or c->steps_per_poll <= 0 or c->share_3d < 0.0 or c->share_3d > 1.0)
+    return G5_ERR_INVALID;
+  std::memset(r, 0, sizeof(*r));
+  std::vector<g5_solve_spec> specs(c->n_requests);
+  int rc = g5_serve_specs(c->seed, c->n_requests, c->rate_hz, c->frac_3d, specs.data());
+  if (rc != G5_OK)
+    return rc;
+  g5_pool *      pools[2] = { nullptr, nullptr };
+  const uint64_t budget3 = static_cast<uint64_t>(static_cast<double>(c->hbm_budget_bytes) * c->share_3d);
+  const uint64_t budget2 = c->hbm_budget_bytes - budget3;
+  rc = g5_pool_create(2, c->thinc_enabled, c->precision, budget2, &pools[0]);
+  if (rc == G5_OK and c->frac_3d > 0.0)
+    rc = g5_pool_create(3, c->thinc_enabled, c->precision, budget3, &pools[1]);
+  if (rc != G5_OK)
+  {
+    g5_pool_destroy(pools[0]);
+    g5_pool_destroy(pools[1]);
+    return rc;
+  }
+  struct Live
+  {
+    int32_t req;
+    int32_t id;
+  };
+  std::vector<Live>    live[2];
+  std::vector<int32_t> queue; // arrived, not admitted
+  std::vector<double>  lat(c->n_requests, 0.0);
+  std::vector<double>  host_result;
+  size_t               next = 0;
+  int32_t              done = 0;
+  double               cell_updates = 0.0;
+  int32_t              max_inflight = 0;
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  auto       now = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
+
+  while (done < c->n_requests and rc == G5_OK)
+  {
+    const double tn = now();
+    while (next < specs.size() and specs[next].arrival <= tn)
+      queue.push_back(static_cast<int32_t>(next++));
+    // admission, first come first served per dimension
+    std::vector<int32_t> started[2];
+    size_t               q = 0;
+    bool                 blocked[2] = { false, false };
+    std::vector<int32_t> rest;
+    for (; q < queue.size(); ++q)
+    {
+      const int32_t req = queue[q];
+      const int     k = specs[req].dim == 2 ? 0 : 1;
+      if (blocked[k] or pools[k] == nullptr)
+      {
+        rest.push_back(req);
+        continue;
+      }
+      int32_t id = -1;
+      const int a = g5_pool_admit_spec(pools[k], &specs[req], &id);
+      if (a == G5_ERR_NOMEM)
+      {
+        blocked[k] = true; // keep the arrival order within a dimension
+        rest.push_back(req);
+        continue;
+      }
+      if (a != G5_OK)
+      {
+        rc = a;
+        break;
+      }
+      live[k].push_back({ req, id });
+      started[k].push_back(id);
+    }
+    queue.swap(rest);
+    for (int k = 0; k < 2 and rc == G5_OK; ++k)
+      if (!started[k].empty())
+        rc = g5_pool_start(pools[k], started[k].data(), static_cast<int32_t>(started[k].size()));
+    const int32_t inflight = static_cast<int32_t>(live[0].size() + live[1].size());
+    max_inflight = std::max(max_inflight, inflight);
+    if (inflight == 0)
+    {
+      if (next < specs.size())
+      {
+        const double wait = specs[next].arrival - now();
+        if (wait > 0.0)
+          std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
+      }
+      continue;
+    }
+    // step everything in flight
+    for (int k = 0; k < 2 and rc == G5_OK; ++k)
+      if (!live[k].empty())
+        rc = g5_pool_step(pools[k], c->steps_per_poll);
+    // collect finished solves
+    for (int k = 0; k < 2 and rc == G5_OK; ++k)
+    {
+      if (live[k].empty())
+        continue;
+      std::vector<int32_t> fin(live[k].size());
+      int32_t              nf = 0;
+      rc = g5_pool_finished(pools[k], fin.data(), static_cast<int32_t>(fin.size()), &nf);
+      for (int32_t f = 0; f < nf and rc == G5_OK; ++f)
+      {
+        auto it = std::find_if(live[k].begin(), live[k].end(), [&](Live const & l) { return l.id == fin[f]; });
+        if (it == live[k].end())
+          continue;
+        g5_solve_spec const & s = specs[it->req];
+        const int             nvar = s.dim == 2 ? 8 : 9;
+        const size_t          cells = static_cast<size_t>(s.nb[0]) * s.nb[1] * (s.dim == 3 ? s.nb[2] : 1) *
+                             (s.dim == 2 ? 256 : 4096);
+        host_result.resize(cells * nvar);
+        rc = g5_pool_download(pools[k], it->id, host_result.data(), host_result.size());
+        if (rc != G5_OK)
+          break;
+        lat[it->req] = (now() - s.arrival) * 1e3;
+        cell_updates += static_cast<double>(cells) * s.steps;
+        rc = g5_pool_release(pools[k], it->id);
+        live[k].erase(it);
+        ++done;
+      }
+    }
+  }
+  const double wall = now();
+  for (int k = 0; k < 2; ++k)
+    g5_pool_destroy(pools[k]);
+  if (rc != G5_OK)
+    return rc;
+  std::vector<double> sorted = lat;
+  std::sort(sorted.begin(), sorted.end());
+  auto pct = [&](double q) {
+    const size_t i = static_cast<size_t>(std::ceil(q * sorted.size())) - 1;
+    return sorted[std::min(i, sorted.size() - 1)];
+  };
+  r->completed = done;
+  r->max_inflight = max_inflight;
+  r->wall_s = wall;
+  r->cell_updates = cell_updates;
+  r->cell_updates_per_s = cell_updates / wall;
+  r->p50_ms = pct(0.50);
+  r->p99_ms = pct(0.99);
+  r->max_ms = sorted.back();
+  double sum = 0.0;
+  for (double v : lat)
+    sum += v;
+  r->mean_ms = sum / lat.size();
+  r->last_arrival_s = specs.back().arrival;
+  if (latencies_ms)
+    for (size_t k = 0; k < std::min(cap, lat.size()); ++k)
+      latencies_ms[k] = lat[k];
+  return G5_OK;
+}
+
+} // extern "C"

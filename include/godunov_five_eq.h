@@ -238,6 +238,113 @@ int g5_pool_usage(const g5_pool * pool, int64_t * used_blocks, int64_t * capacit
 /* HIP-event duration of the last g5_pool_step call (ms) */
 int g5_pool_last_step_ms(const g5_pool * pool, double * ms);
 
+/* Launch geometry of the step kernels.  No reference counterpart: the reference launches Kokkos
+ * team policies whose shape is fixed by the block size (scheme/GodunovImplemV0.cpp:83-151).
+ * Every geometry computes the same bits; only the time differs.  By default a pool tunes it on
+ * its first step over a launch list of a new size class (floor(log2(blocks))): each candidate
+ * runs the step kernel of that list a few times (the kernel only writes the next state and the
+ * next CFL rate, so the extra launches change nothing) and the fastest is kept for the process,
+ * per dimension, THINC flag, precision, integrator and size class.  G5_AUTOTUNE=0 in the
+ * environment turns tuning off (default geometry: one workgroup per block, XCD ranges, z split). */
+typedef struct g5_launch
+{
+  int32_t wg_per_cu; /* 0: one workgroup per block; 1..16: persistent workgroups per CU (2-D only) */
+  int32_t xcd_remap; /* 1: each XCD takes a contiguous range of the block list */
+  int32_t zsplit;    /* 3-D: z Riemann solve split around the x/y flux exchange (1) or not (0) */
+  int32_t autotuned; /* out: 1 when the geometry came from the autotuner */
+  double  tuned_ms;  /* out: best step-kernel time measured by the autotuner (ms) */
+} g5_launch;
+
+/* dimension, THINC flag and precision of a pool */
+int g5_pool_properties(const g5_pool * pool, int32_t * dim, int32_t * thinc_enabled, int32_t * precision);
+
+/* fix the geometry (launch != NULL; autotuning is then bypassed) or return to autotuning (NULL) */
+int g5_pool_set_launch(g5_pool * pool, const g5_launch * launch);
+int g5_pool_get_launch(const g5_pool * pool, g5_launch * launch);
+/* enable (1) or disable (0) autotuning for this pool */
+int g5_pool_set_autotune(g5_pool * pool, int32_t enable);
+
+/* ---- seeded workloads of the batching and serving forms ---- */
+
+/* One independent solve: 2-D or 3-D grid of 16^d blocks with unit height (dx = 1/(16 * the
+ * smallest transverse block count)), an initial-condition family and two stiffened-gas
+ * materials: material 1 surrounds and carries a shock of Mach `mach` into the pre-shock state
+ * (rho1, p1); material 0 forms the inclusion (density rho0 at p1).  Shock-bubble
+ * (init/InitShockBubble.cpp:50-225): shock front at x = 0.2, inclusion centred at
+ * (0.55, 0.5, 0.5), a circle/sphere of radius 0.25 (shape 0) or a box of half-sizes
+ * (0.2, 0.3, 0.3) (shape 1), ZERO_GRADIENT in x and walls elsewhere, minmod.  Triple point
+ * (init/InitTriplePoint.cpp:46-133): post-shock state for x <= 1/3, pre-shock material 1 for
+ * y <= 0.5 (sqrt(y^2+z^2) <= 0.5 in 3-D), material 0 elsewhere, ZERO_GRADIENT borders, MC slopes.
+ * HANCOCK, HLLC, CFL 0.5.  No reference counterpart: the reference reads one such setup from an
+ * .ini file per run. */
+typedef struct g5_solve_spec
+{
+  int32_t dim;
+  int32_t nb[3];
+  int32_t kind;  /* 0 shock-bubble, 1 triple point */
+  int32_t shape; /* shock-bubble inclusion: 0 circle/sphere, 1 rectangle/box */
+  int32_t steps; /* step budget */
+  int32_t pad;
+  double  gamma[2];
+  double  pinf[2];
+  double  mach;
+  double  rho1, p1, rho0;
+  double  arrival; /* serving: arrival time (s) after the start */
+} g5_solve_spec;
+
+/* Batching workload: solve `index` of the seeded set.  Short side uniform in {64,128,256,512},
+ * aspect ratio uniform in [1,4] (long side rounded to 16), gamma uniform in [1.1,4.4] for both
+ * materials, pinf of material 1 in {0, 6e8} (6e8: rho1 1000, p1 1e5, rho0 in [1,10]; 0: rho1 1,
+ * p1 1, rho0 in [0.1,5]), Mach uniform in [1.1,3], family and shape each with probability 1/2,
+ * 100 steps.  Generator: std::mt19937_64 seeded per (seed, index) by splitmix64. */
+int g5_batch_spec(uint64_t seed, int32_t index, g5_solve_spec * out);
+
+/* Serving workload: n requests with Poisson arrivals at rate_hz; a fraction frac_3d are 3-D with
+ * sides uniform in multiples of 16 in [64,128], the rest 2-D with sides in [64,1024]; 50-500
+ * steps; physics drawn as for the batching workload. */
+int g5_serve_specs(uint64_t seed, int32_t n, double rate_hz, double frac_3d, g5_solve_spec * out);
+
+/* the configuration and the three region states (post-shock, pre-shock, inclusion) of a spec */
+int g5_spec_config(const g5_solve_spec * spec, int32_t thinc_enabled, int32_t precision, g5_config * cfg);
+int g5_spec_regions(const g5_solve_spec * spec, g5_region * regions3);
+
+/* admit a spec into a pool (step budget = spec->steps, no end time) and generate its initial
+ * condition on the device; G5_ERR_NOMEM when the pool's budget cannot hold it now */
+int g5_pool_admit_spec(g5_pool * pool, const g5_solve_spec * spec, int32_t * solve_id);
+
+/* Serving request loop (continuous batching).  Requests are admitted first come first served
+ * into a 2-D and a 3-D pool that split the HBM budget (share_3d to the 3-D pool) as soon as
+ * memory frees up, every in-flight solve is stepped together (steps_per_poll device-dt steps per
+ * loop turn, one HIP graph per step and pool), finished solves are downloaded and released.
+ * Reports throughput (cell-updates of completed solves / wall time) and the latency of each
+ * request (result on the host - arrival). */
+typedef struct g5_serve_config
+{
+  uint64_t seed;
+  int32_t  n_requests;
+  int32_t  steps_per_poll;
+  double   rate_hz;
+  double   frac_3d;
+  double   share_3d;
+  uint64_t hbm_budget_bytes;
+  int32_t  precision;
+  int32_t  thinc_enabled;
+} g5_serve_config;
+
+typedef struct g5_serve_report
+{
+  int32_t completed;
+  int32_t max_inflight;
+  double  wall_s;
+  double  cell_updates;
+  double  cell_updates_per_s;
+  double  p50_ms, p99_ms, mean_ms, max_ms;
+  double  last_arrival_s;
+} g5_serve_report;
+
+/* latencies_ms (optional, cap entries) receives the latency of each request in arrival order */
+int g5_serve(const g5_serve_config * config, g5_serve_report * report, double * latencies_ms, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

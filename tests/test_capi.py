@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -65,3 +66,27 @@ def test_initial_condition_host_side(g5):
     assert np.all(np.isfinite(V))
     assert V[:, 5].min() == 0.0 and V[:, 5].max() == 1.0  # air fraction
     assert np.allclose(V[:, 0], V[:, 6] + V[:, 8])
+
+
+def _block_range(w, nblocks, wg_per_xcd, remap, grid):
+    """Host restatement of the step kernels' block_range (blocks handled by workgroup w)."""
+    per_xcd = (nblocks + 7) // 8
+    if wg_per_xcd == 0:
+        b = (w & 7) * per_xcd + (w >> 3) if remap else w
+        return list(range(b, min(b + 1, nblocks)))
+    if remap:
+        x = w & 7
+        b = x * per_xcd + (w >> 3)
+        return list(range(b, min((x + 1) * per_xcd, nblocks), wg_per_xcd))
+    return list(range(w, nblocks, grid))
+
+
+@pytest.mark.parametrize("nblocks", [1, 7, 27, 256, 1920, 4097])
+def test_launch_geometries_cover_every_block_once(nblocks):
+    per_xcd = (nblocks + 7) // 8
+    for wg_per_xcd, remap in [(0, 1), (0, 0), (32, 1), (64, 1), (128, 0), (3, 1)]:
+        grid = 8 * wg_per_xcd if wg_per_xcd else 8 * per_xcd
+        seen = []
+        for w in range(grid):
+            seen += _block_range(w, nblocks, wg_per_xcd, remap, grid)
+        assert sorted(seen) == list(range(nblocks)), (nblocks, wg_per_xcd, remap)

@@ -144,3 +144,45 @@ def test_device_triple_point_and_box(g5):
         V = pool.download(sid2).reshape(21, 8, 256)
         area = (1 - V[:, 4]).sum() / 48 ** 2
         assert area == pytest.approx(0.5 * 0.4, rel=1e-12)
+
+
+def test_launch_geometries_bit_identical(g5):
+    # every launch geometry the autotuner may pick computes the same bits; the 2-D persistent
+    # geometries loop over several blocks per workgroup here (1920 blocks, 256..1024 workgroups)
+    say("launch geometries: 2-D FP64/mixed, 3-D FP64/mixed")
+    for dim, prec in ((2, g5.PREC_FP64), (2, g5.PREC_MIXED), (3, g5.PREC_FP64), (3, g5.PREC_MIXED)):
+        if dim == 2:
+            cfg, U0 = cases.air_helium_2d(nbx=48, nby=40, thinc=True, thinc_beta=1.6, precision=prec)
+            geoms = [(0, 1, 1), (0, 0, 1), (1, 1, 1), (2, 1, 1), (4, 0, 1)]
+        else:
+            cfg, U0 = cases.air_water_3d(nb=3, precision=prec)
+            geoms = [(0, 1, 1), (0, 0, 1), (0, 1, 0), (0, 0, 0)]
+        ref = None
+        for wg, remap, zs in geoms:
+            say(f"dim={dim} prec={prec} launch wg_per_cu={wg} remap={remap} zsplit={zs}")
+            with g5.Pool(dim, thinc=True, precision=prec, budget_bytes=1 << 28) as pool:
+                pool.set_launch(wg, remap, zs)
+                sid = pool.admit(cfg, max_steps=6)
+                pool.upload(sid, U0)
+                pool.start([sid])
+                pool.step(6)
+                U = pool.download(sid)
+                assert pool.launch()["wg_per_cu"] == wg
+            if ref is None:
+                ref = U
+            else:
+                assert np.array_equal(U, ref), (dim, prec, wg, remap, zs)
+        # the autotuned pool gives the same bits as well
+        with g5.Pool(dim, thinc=True, precision=prec, budget_bytes=1 << 28) as pool:
+            sid = pool.admit(cfg, max_steps=6)
+            pool.upload(sid, U0)
+            pool.start([sid])
+            pool.step(6)
+            assert np.array_equal(pool.download(sid), ref)
+            say(f"dim={dim} prec={prec} autotuned launch {pool.launch()}")
+
+
+def test_3d_persistent_launch_is_rejected(g5):
+    with g5.Pool(3, thinc=True, budget_bytes=1 << 26) as pool:
+        with pytest.raises(g5.G5Error):
+            pool.set_launch(1, 1, 1)
