@@ -1,16 +1,25 @@
-"""Benchmark: cell-updates/s of the fused FP64 HIP step on one MI355X.
+"""Benchmark: cell-updates/s of the fused HIP step on one MI355X.
 
-Workload (default): 3-D air-water shock-bubble with THINC at 256^3 cells (4096 blocks of 16^3),
-FP64, HANCOCK, HLLC, CFL 0.5 -- the plain FP64 THINC configuration of the reference's metric.
-A step is one godunov_unsplit iteration (device-resident dt, fused step kernel + dt finalise).
-value = owned cells * steps / wall time of the timed loop (state resident in HBM).
+Workloads (--workload):
+  air_water_3d  (default) 3-D air-water shock-bubble with THINC at 256^3 cells (4096 blocks of
+                16^3), FP64, HANCOCK, HLLC, CFL 0.5, 50 steps: the plain FP64 THINC configuration.
+  air_helium_2d 2-D air-helium shock-bubble, 512x128 cells, FP64, minmod, CFL 0.8, 200 steps (the
+                reference's CPU-runnable case; the state fits in the last-level cache).
+  mixed_3d      the air-water THINC case at 512x256x256 cells in mixed precision, 100 steps, with
+                the per-field relative L1 error against the FP64 run of the same case.
+  batch         4096 independent 2-D solves of the seeded batching workload (g5_batch_spec),
+                mixed precision, one pool, 100 steps each.
+  serve         the native serving loop (g5_serve): Poisson arrivals, 2-D and 10% 3-D requests,
+                HBM budget 200 GB, mixed precision; reports p50/p99 latency.
 
-roofline: algorithmic bytes per cell-update = 2 * nvar * 8 B (read and write the FP64 state
-once; 144 B in 3-D), divided by the HIP-event duration of the step loop per step.
-cpu_baseline: the CPU FP64 oracle (a restatement of the reference V0 step) on a bounded sample of
-the same workload, on the host cores of this machine.
+A step is one godunov_unsplit iteration over every solve (device-resident dt, fused step kernel
++ dt finalise, one HIP graph).  value = owned cells * steps / wall time of the timed loop, state
+resident in HBM.  roofline: algorithmic bytes per cell-update = 2 * nvar * 8 B (the FP64 state
+read and written once: 128 B in 2-D, 144 B in 3-D; the mixed form keeps the FP64 state), divided
+by the HIP-event time per step of the timed loop.  cpu_baseline: the CPU FP64 oracle (a
+restatement of the reference V0 step) on a bounded sample of the same workload, on this host.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nb 16] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 """
 import argparse
 import json
@@ -21,21 +30,32 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
 
+import numpy as np  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-updates/s on 1 GPU vs plain FP64 at fixed L1 err; p50/p99 solve latency"
+SEED = 20251203
+DEFAULT_STEPS = {"air_water_3d": 50, "air_helium_2d": 200, "mixed_3d": 100, "batch": 100, "serve": 0}
 
 
-def traffic_from_profile(nb):
+def traffic_from_profile(name):
     """HBM bytes per step-kernel launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE,
-    scaled by a copy-kernel calibration); None when no measurement for this size is committed."""
-    path = os.path.join(ROOT, "profiles", f"traffic_step3d_air_water_thinc_{16 * nb}.json")
+    scaled by a copy-kernel calibration); None when no measurement is committed."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if os.path.exists(path):
         with open(path) as f:
             return json.load(f).get("bytes_per_launch")
     return None
 
 
-def cpu_baseline(cfg, U0, nsteps):
+def rel_l1(a, b, nvar, ncell):
+    A = a.reshape(-1, nvar, ncell)
+    B = b.reshape(-1, nvar, ncell)
+    return [float(np.abs(A[:, v] - B[:, v]).sum() / max(np.abs(B[:, v]).sum(), 1e-300)) for v in range(nvar)]
+
+
+def cpu_rate(cfg, U0, nsteps):
+    """Cell-updates/s of the CPU FP64 oracle on one configuration (bounded sample)."""
     import oracle
 
     o = oracle.Oracle(cfg)
@@ -43,91 +63,218 @@ def cpu_baseline(cfg, U0, nsteps):
     t0 = time.perf_counter()
     _, n = o.run(nsteps)
     dt = time.perf_counter() - t0
-    cells = o.nowned * o.ncell
-    return cells * n / dt, oracle.max_threads()
+    return o.nowned * o.ncell * n, dt, oracle.max_threads()
+
+
+def roofline(cells_per_step, nvar, ms_per_step, traffic):
+    b_alg = 2 * nvar * 8 * cells_per_step
+    achieved = b_alg / (ms_per_step * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic}
+
+
+def run_context(g5, cfg, U0, steps, warmup):
+    """Warm-up, then `steps` timed steps bracketed by synchronisations; returns
+    (wall s, HIP-event ms per step, state, steps done)."""
+    ctx = g5.Context(cfg)
+    ctx.upload(U0)
+    if warmup > 0:
+        ctx.run(warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    _, n = ctx.run(steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ctx.last_run_ms() / max(n, 1)
+    U = ctx.download()
+    ctx.close()
+    return wall, ev_ms, U, n
+
+
+def bench_single(g5, args, name):
+    import cases
+
+    if name == "air_water_3d":
+        cfg, U0 = cases.air_water_3d(nb=args.nb, thinc=True)
+        grid = [16 * args.nb] * 3
+        wl = "shock_bubble_3d_air_water_thinc"
+    else:
+        cfg, U0 = cases.air_helium_2d()
+        grid = [512, 128]
+        wl = "shock_bubble_2d_air_helium"
+    nvar = g5.nvar_of(cfg.dim)
+    cells = int(np.prod(grid))
+    wall, ev_ms, U, n = run_context(g5, cfg, U0, args.steps, args.warmup)
+    assert n == args.steps and np.all(np.isfinite(U)), "non-finite state"
+    out = base_line(args, cells * n / wall, wall * 1e3 / n, n, "f64")
+    out["config"] = {"workload": wl, "cells": cells, "grid": grid, "blocks": int(np.prod(cfg.nb[:cfg.dim])),
+                     "block": [16] * cfg.dim, "precision": "fp64", "integrator": "HANCOCK", "riemann": "HLLC",
+                     "thinc": bool(cfg.thinc_enabled), "cfl": cfg.cfl}
+    out["roofline"] = roofline(cells, nvar, ev_ms, traffic_from_profile(
+        f"step3d_air_water_thinc_{grid[0]}" if cfg.dim == 3 else "step2d_air_helium_512x128"))
+    if not args.no_cpu:
+        k = args.cpu_steps if cfg.dim == 3 else 200
+        cu, dt, th = cpu_rate(cfg, U0, k)
+        out["cpu_baseline"] = {"value": cu / dt, "unit": "cell-updates/s", "cores": th, "kind": "port",
+                               "sample": f"{k} steps of the same {'x'.join(map(str, grid))} workload, CPU FP64 "
+                                         f"oracle ({th} OpenMP threads, {dt:.1f} s)"}
+    return out
+
+
+def bench_mixed3d(g5, args):
+    import cases
+
+    nbx = 2 * args.nb
+    cfg64, U0 = cases.air_water_3d(nb=args.nb, thinc=True)
+    # 512 x 256 x 256: brick 2x1x1 of the unit-cube case, same dx
+    cfg64.nb[0] = nbx
+    U0 = g5.init_shock_bubble(cfg64, air_water_regions(g5), 0.25, (0.5, 0.5, 0.5), 0.2)
+    cfgm = g5.G5Config.from_buffer_copy(cfg64)
+    cfgm.precision = g5.PREC_MIXED
+    cells = nbx * args.nb * args.nb * 4096
+    wall64, ev64, U64, n64 = run_context(g5, cfg64, U0, args.steps, args.warmup)
+    wall, ev, Um, n = run_context(g5, cfgm, U0, args.steps, args.warmup)
+    assert n == n64 == args.steps and np.all(np.isfinite(Um)) and np.all(np.isfinite(U64))
+    # both runs include the warm-up steps: the states compared are after warmup + steps steps
+    l1 = rel_l1(Um, U64, 9, 4096)
+    names = ["rho", "E", "rho_u", "rho_v", "rho_w", "alpha0", "alpha_rho0", "alpha1", "alpha_rho1"]
+    out = base_line(args, cells * n / wall, wall * 1e3 / n, n, "mixed-f32-f64")
+    out["config"] = {"workload": "shock_bubble_3d_air_water_thinc_mixed", "cells": cells,
+                     "grid": [16 * nbx, 16 * args.nb, 16 * args.nb], "blocks": nbx * args.nb ** 2,
+                     "precision": "mixed", "l1_vs_fp64_after_steps": args.steps + args.warmup,
+                     "rel_l1_vs_fp64": dict(zip(names, l1)),
+                     "fp64_value": cells * n64 / wall64, "speedup_vs_fp64": wall64 / wall}
+    out["roofline"] = roofline(cells, 9, ev, traffic_from_profile(f"step3d_mixed_air_water_thinc_{16 * nbx}"))
+    if not args.no_cpu:
+        k = max(1, args.cpu_steps // 2)
+        cu, dt, th = cpu_rate(cfg64, U0, k)
+        out["cpu_baseline"] = {"value": cu / dt, "unit": "cell-updates/s", "cores": th, "kind": "port",
+                               "sample": f"{k} steps of the same 512x256x256 case, CPU FP64 oracle "
+                                         f"({th} OpenMP threads, {dt:.1f} s)"}
+    return out
+
+
+def air_water_regions(g5):
+    rho2, u2, p2, _ = g5.rankine_hugoniot(4.4, 6e8, 1000.0, 1e5, p2=1e9)
+    return [g5.region(0.0, (0.0, rho2), p2, u2), g5.region(0.0, (0.0, 1000.0), 1e5),
+            g5.region(1.0, (1.0, 0.0), 1e5)]
+
+
+def bench_batch(g5, args):
+    prec = g5.PREC_FP64 if args.fp64 else g5.PREC_MIXED
+    n = args.nsolves
+    specs = [g5.batch_spec(SEED, k) for k in range(n)]
+    cells = sum(256 * s.nb[0] * s.nb[1] for s in specs)
+    bytes_needed = cells * 8 * 8 * 2
+    budget = int(bytes_needed * 1.02) + (64 << 20)
+    steps = args.steps
+    pool = g5.Pool(2, thinc=False, precision=prec, budget_bytes=budget)
+    t_adm = time.perf_counter()
+    ids = [pool.admit_spec(s) for s in specs]
+    t_adm = time.perf_counter() - t_adm
+    pool.start(ids)
+    # launch list, autotuning of the launch geometry and step-graph capture before the timed loop
+    pool.prepare()
+    if args.warmup > 0:
+        pool.step(args.warmup)
+    pool.status(ids[0])
+    t0 = time.perf_counter()
+    pool.step(steps)
+    done = pool.status(ids[0])  # synchronises
+    wall = time.perf_counter() - t0
+    ev_ms = pool.last_step_ms() / steps
+    launch = pool.launch()
+    # every solve advanced by warmup + steps (each has a 100-step budget: the timed loop ends
+    # exactly at the budget with the default warm-up of 0)
+    finite = all(np.isfinite(pool.download(i)).all() for i in ids[:: max(1, n // 64)])
+    stepped = [pool.status(i)[1] for i in ids]
+    pool.close()
+    assert done is not None and finite
+    out = base_line(args, cells * steps / wall, wall * 1e3 / steps, steps, "mixed-f32-f64" if prec else "f64")
+    out["config"] = {"workload": "batch_2d_seeded", "seed": SEED, "solves": n, "cells": cells,
+                     "precision": "mixed" if prec else "fp64", "block": [16, 16], "blocks": cells // 256,
+                     "steps_per_solve": min(stepped), "admission_s": t_adm, "launch": launch,
+                     "hip_graph": True}
+    out["roofline"] = roofline(cells, 8, ev_ms, None)
+    if not args.no_cpu:
+        # bounded sample: the first solves of the same batch (device-generated initial states)
+        cu = dt = 0.0
+        th = 1
+        with g5.Pool(2, precision=g5.PREC_FP64, budget_bytes=1 << 30) as p64:
+            k = 0
+            while dt < 10.0 and k < n:
+                sid = p64.try_admit_spec(specs[k])
+                if sid is None:
+                    break
+                U0 = p64.download(sid)
+                c, d, th = cpu_rate(g5.spec_config(specs[k]), U0, 10)
+                p64.release(sid)
+                cu += c
+                dt += d
+                k += 1
+        out["cpu_baseline"] = {"value": cu / dt, "unit": "cell-updates/s", "cores": th, "kind": "port",
+                               "sample": f"10 steps of each of the first {k} solves of the same batch, CPU "
+                                         f"FP64 oracle ({th} OpenMP threads, {dt:.1f} s)"}
+    return out
+
+
+def bench_serve(g5, args):
+    prec = g5.PREC_FP64 if args.fp64 else g5.PREC_MIXED
+    rep, lat = g5.serve(SEED, args.requests, args.rate, frac_3d=0.1, share_3d=0.3, budget_bytes=200 << 30,
+                        precision=prec, steps_per_poll=args.poll)
+    assert rep["completed"] == args.requests
+    out = base_line(args, rep["cell_updates_per_s"], None, None, "mixed-f32-f64" if prec else "f64")
+    out["higher_is_better"] = True
+    out["config"] = {"workload": "serve_poisson", "seed": SEED, "requests": args.requests, "rate_hz": args.rate,
+                     "frac_3d": 0.1, "hbm_budget_gb": 200, "precision": "mixed" if prec else "fp64",
+                     "steps_per_poll": args.poll, "p50_ms": rep["p50_ms"], "p99_ms": rep["p99_ms"],
+                     "mean_ms": rep["mean_ms"], "max_ms": rep["max_ms"], "wall_s": rep["wall_s"],
+                     "max_inflight": rep["max_inflight"], "cell_updates": rep["cell_updates"],
+                     "last_arrival_s": rep["last_arrival_s"]}
+    out["roofline"] = None
+    return out
+
+
+def base_line(args, value, ms_per_step, steps, dtype):
+    return {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nb", type=int, default=16, help="blocks per direction (16 -> 256^3 cells)")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="air_water_3d", choices=sorted(DEFAULT_STEPS))
+    ap.add_argument("--nb", type=int, default=16, help="blocks per direction of the 3-D cases (16 -> 256)")
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64", action="store_true", help="batch/serve: FP64 instead of mixed precision")
+    ap.add_argument("--nsolves", type=int, default=4096)
+    ap.add_argument("--requests", type=int, default=400)
+    ap.add_argument("--rate", type=float, default=100.0)
+    ap.add_argument("--poll", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are out of scope "
-              "(the halo-exchange design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: this build runs on one GPU; runs on several GPUs are out of scope "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.steps is None:
+        args.steps = DEFAULT_STEPS[args.workload]
+    if args.warmup is None:
+        args.warmup = 0 if args.workload == "batch" else 5
 
-    import cases
     import godunov_five_eq_amd as g5
 
-    cfg, U0 = cases.air_water_3d(nb=args.nb, thinc=True)
-    cells = (16 * args.nb) ** 3
-    ctx = g5.Context(cfg)
-    ctx.upload(U0)
-    if args.warmup > 0:
-        ctx.run(args.warmup)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    _, n = ctx.run(args.steps)
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = ctx.last_run_ms() / max(n, 1)
-    U = ctx.download()
-    ctx.close()
-    import numpy as np
-
-    assert n == args.steps and np.all(np.isfinite(U)), "non-finite state"
-    value = cells * n / wall
-    b_alg = 2 * 9 * 8 * cells  # bytes per launch
-    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": n,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / n,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": "shock_bubble_3d_air_water_thinc",
-            "cells": cells,
-            "grid": [16 * args.nb] * 3,
-            "blocks": args.nb ** 3,
-            "block": [16, 16, 16],
-            "precision": "fp64",
-            "integrator": "HANCOCK",
-            "riemann": "HLLC",
-            "thinc_beta": 1.6,
-            "cfl": 0.5,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profile(args.nb),
-        },
-    }
-    if not args.no_cpu:
-        rate, threads = cpu_baseline(cfg, U0, args.cpu_steps)
-        out["cpu_baseline"] = {
-            "value": rate,
-            "unit": "cell-updates/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{args.cpu_steps} steps of the same {16 * args.nb}^3 workload, CPU FP64 oracle "
-                      f"(OpenMP, {threads} threads)",
-        }
+    if args.workload in ("air_water_3d", "air_helium_2d"):
+        out = bench_single(g5, args, args.workload)
+    elif args.workload == "mixed_3d":
+        out = bench_mixed3d(g5, args)
+    elif args.workload == "batch":
+        out = bench_batch(g5, args)
+    else:
+        out = bench_serve(g5, args)
     print(json.dumps(out))
 
 

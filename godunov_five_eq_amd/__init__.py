@@ -164,6 +164,7 @@ def lib():
             "g5_pool_get_launch": (ctypes.c_int, [vp, ctypes.POINTER(G5Launch)]),
             "g5_pool_set_autotune": (ctypes.c_int, [vp, i32]),
             "g5_pool_properties": (ctypes.c_int, [vp, P(i32), P(i32), P(i32)]),
+            "g5_pool_prepare": (ctypes.c_int, [vp]),
             "g5_batch_spec": (ctypes.c_int, [ctypes.c_uint64, i32, P(G5SolveSpec)]),
             "g5_serve_specs": (ctypes.c_int, [ctypes.c_uint64, i32, d, d, P(G5SolveSpec)]),
             "g5_spec_config": (ctypes.c_int, [P(G5SolveSpec), i32, i32, P(G5Config)]),
@@ -191,7 +192,7 @@ def exported_symbols():
         "g5_pool_upload", "g5_pool_download", "g5_pool_init_shock_bubble", "g5_pool_init_triple_point",
         "g5_pool_start", "g5_pool_step", "g5_pool_status", "g5_pool_finished", "g5_pool_usage",
         "g5_pool_last_step_ms", "g5_pool_set_launch", "g5_pool_get_launch", "g5_pool_set_autotune",
-        "g5_pool_properties", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
+        "g5_pool_properties", "g5_pool_prepare", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
         "g5_pool_admit_spec", "g5_serve",
     ]
 
@@ -537,6 +538,10 @@ class Pool:
         d, th, pr = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         self._check(lib().g5_pool_properties(self._h, ctypes.byref(d), ctypes.byref(th), ctypes.byref(pr)))
         return spec_config(spec, bool(th.value), pr.value)
+
+    def prepare(self):
+        """Launch list, launch-geometry tuning and step-graph capture, without stepping."""
+        self._check(lib().g5_pool_prepare(self._h))
 
     def admit_spec(self, spec):
         """Admit a workload spec and generate its initial condition on the device."""

@@ -559,28 +559,37 @@ pool_hancock(g5_pool const * p)
   return true;
 }
 
+// (re)capture the HIP graph of one device-dt step over the current launch list
+int
+pool_capture(g5_pool * p)
+{
+  if (p->nlist == 0 or p->graph_valid)
+    return G5_OK;
+  if (p->graph)
+  {
+    hipGraphExecDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  hipGraph_t g;
+  G5_HIP(p->err, hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+  launch_step(p, step_args(p, p->d_blocks, p->nlist, 0.0), pool_hancock(p));
+  launch_finalize(p->d_ctl, p->d_solves, static_cast<int>(p->solves.size()), p->stream);
+  G5_HIP(p->err, hipStreamEndCapture(p->stream, &g));
+  G5_HIP(p->err, hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0));
+  hipGraphDestroy(g);
+  p->graph_valid = true;
+  return G5_OK;
+}
+
 // one device-dt step of every active solve: step kernel + finalize, as a HIP graph
 int
 pool_step_once(g5_pool * p)
 {
   if (p->nlist == 0)
     return G5_OK;
-  if (!p->graph_valid)
-  {
-    if (p->graph)
-    {
-      hipGraphExecDestroy(p->graph);
-      p->graph = nullptr;
-    }
-    hipGraph_t g;
-    G5_HIP(p->err, hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-    launch_step(p, step_args(p, p->d_blocks, p->nlist, 0.0), pool_hancock(p));
-    launch_finalize(p->d_ctl, p->d_solves, static_cast<int>(p->solves.size()), p->stream);
-    G5_HIP(p->err, hipStreamEndCapture(p->stream, &g));
-    G5_HIP(p->err, hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0));
-    hipGraphDestroy(g);
-    p->graph_valid = true;
-  }
+  const int rc = pool_capture(p);
+  if (rc != G5_OK)
+    return rc;
   G5_HIP(p->err, hipGraphLaunch(p->graph, p->stream));
   return G5_OK;
 }
@@ -1026,6 +1035,19 @@ g5_pool_get_launch(const g5_pool * p, g5_launch * launch)
     return G5_ERR_INVALID;
   to_launch(p->launch, p, launch);
   return G5_OK;
+}
+
+int
+g5_pool_prepare(g5_pool * p)
+{
+  if (not p)
+    return G5_ERR_INVALID;
+  int rc = rebuild_list(p);
+  if (rc == G5_OK)
+    rc = ensure_tuned(p, pool_hancock(p));
+  if (rc == G5_OK)
+    rc = pool_capture(p);
+  return rc;
 }
 
 int

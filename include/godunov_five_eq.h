@@ -255,6 +255,10 @@ typedef struct g5_launch
   double  tuned_ms;  /* out: best step-kernel time measured by the autotuner (ms) */
 } g5_launch;
 
+/* build the launch list of the live solves, tune its launch geometry and capture the step
+ * graph now, so that the next g5_pool_step does none of it (no state changes) */
+int g5_pool_prepare(g5_pool * pool);
+
 /* dimension, THINC flag and precision of a pool */
 int g5_pool_properties(const g5_pool * pool, int32_t * dim, int32_t * thinc_enabled, int32_t * precision);
 
