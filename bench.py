@@ -186,15 +186,36 @@ def bench_batch(g5, args):
     launch = pool.launch()
     # every solve advanced by warmup + steps (each has a 100-step budget: the timed loop ends
     # exactly at the budget with the default warm-up of 0)
-    finite = all(np.isfinite(pool.download(i)).all() for i in ids[:: max(1, n // 64)])
+    sample = list(range(0, n, max(1, n // 64)))
+    results = {k: pool.download(ids[k]) for k in sample}
+    finite = all(np.isfinite(u).all() for u in results.values())
     stepped = [pool.status(i)[1] for i in ids]
     pool.close()
     assert done is not None and finite
+    l1 = None
+    if prec == g5.PREC_MIXED:
+        # accuracy at this speed: the sampled solves rerun in FP64 (bit-identical to the oracle)
+        num = np.zeros(8)
+        den = np.zeros(8)
+        with g5.Pool(2, precision=g5.PREC_FP64, budget_bytes=16 << 30) as p64:
+            for k in sample:
+                sid = p64.admit_spec(specs[k])
+                p64.start([sid])
+                p64.step(args.warmup + steps)
+                ref = p64.download(sid).reshape(-1, 8, 256)
+                p64.release(sid)
+                got = results[k].reshape(-1, 8, 256)
+                num += np.abs(got - ref).sum(axis=(0, 2))
+                den += np.abs(ref).sum(axis=(0, 2))
+        mom = den[2] + den[3]
+        den[2] = den[3] = mom
+        names = ["rho", "E", "rho_u", "rho_v", "alpha0", "alpha_rho0", "alpha1", "alpha_rho1"]
+        l1 = dict(zip(names, (num / den).tolist()))
     out = base_line(args, cells * steps / wall, wall * 1e3 / steps, steps, "mixed-f32-f64" if prec else "f64")
     out["config"] = {"workload": "batch_2d_seeded", "seed": SEED, "solves": n, "cells": cells,
                      "precision": "mixed" if prec else "fp64", "block": [16, 16], "blocks": cells // 256,
                      "steps_per_solve": min(stepped), "admission_s": t_adm, "launch": launch,
-                     "hip_graph": True}
+                     "hip_graph": True, "rel_l1_vs_fp64_sample": l1, "l1_sample_solves": len(sample)}
     out["roofline"] = roofline(cells, 8, ev_ms, None)
     if not args.no_cpu:
         # bounded sample: the first solves of the same batch (device-generated initial states)

@@ -95,6 +95,8 @@ class G5ServeConfig(ctypes.Structure):
         ("hbm_budget_bytes", ctypes.c_uint64),
         ("precision", ctypes.c_int32),
         ("thinc_enabled", ctypes.c_int32),
+        ("autotune", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
     ]
 
 
@@ -165,6 +167,7 @@ def lib():
             "g5_pool_set_autotune": (ctypes.c_int, [vp, i32]),
             "g5_pool_properties": (ctypes.c_int, [vp, P(i32), P(i32), P(i32)]),
             "g5_pool_prepare": (ctypes.c_int, [vp]),
+            "g5_pool_step_async": (ctypes.c_int, [vp, i32]),
             "g5_batch_spec": (ctypes.c_int, [ctypes.c_uint64, i32, P(G5SolveSpec)]),
             "g5_serve_specs": (ctypes.c_int, [ctypes.c_uint64, i32, d, d, P(G5SolveSpec)]),
             "g5_spec_config": (ctypes.c_int, [P(G5SolveSpec), i32, i32, P(G5Config)]),
@@ -192,7 +195,7 @@ def exported_symbols():
         "g5_pool_upload", "g5_pool_download", "g5_pool_init_shock_bubble", "g5_pool_init_triple_point",
         "g5_pool_start", "g5_pool_step", "g5_pool_status", "g5_pool_finished", "g5_pool_usage",
         "g5_pool_last_step_ms", "g5_pool_set_launch", "g5_pool_get_launch", "g5_pool_set_autotune",
-        "g5_pool_properties", "g5_pool_prepare", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
+        "g5_pool_properties", "g5_pool_prepare", "g5_pool_step_async", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
         "g5_pool_admit_spec", "g5_serve",
     ]
 
@@ -403,10 +406,10 @@ def spec_regions(spec):
 
 
 def serve(seed, n_requests, rate_hz, *, frac_3d=0.1, share_3d=0.3, budget_bytes=200 << 30,
-          precision=PREC_MIXED, thinc=False, steps_per_poll=4):
+          precision=PREC_MIXED, thinc=False, steps_per_poll=4, autotune=False):
     """Run the serving loop (g5_serve); returns (report dict, per-request latencies in ms)."""
     c = G5ServeConfig(seed, n_requests, steps_per_poll, rate_hz, frac_3d, share_3d, int(budget_bytes),
-                      precision, 1 if thinc else 0)
+                      precision, 1 if thinc else 0, 1 if autotune else 0, 0)
     r = G5ServeReport()
     lat = np.zeros(n_requests)
     rc = lib().g5_serve(ctypes.byref(c), ctypes.byref(r), _dptr(lat), lat.size)

@@ -53,12 +53,24 @@ using Prim = PrimT<double>;
 using Slope = SlopeT<double>;
 using Face = FaceT<double>;
 
+// What a face exchanges between the two cells that share it.  FP64 path: the face flux scaled by
+// dt/dx (set_flux, ComputeFluxesAndStoreFunctor.h:253-307).  Mixed path: the upwinded HLLC star
+// state in FP32 (partial and total densities, normal velocity, pressure, transverse velocities,
+// volume fraction) with its total energy in FP64; each cell assembles the FP64 flux from it
+// (assemble_flux), so both neighbours apply bit-identical fluxes and the mass, momentum, energy
+// and volume-fraction fluxes are mutually consistent in FP64.
 template <typename T>
-struct FluxT // face flux scaled by dt/dx (set_flux, ComputeFluxesAndStoreFunctor.h:253-307); the
-{            // energy flux stays double in both paths (it carries rho e ~ Pi, large when pinf is)
-  T      ar0, ar1, a0, a1;
-  double e;
-  T      mu, mv, mw, us;
+struct FluxT;
+template <>
+struct FluxT<double>
+{
+  double ar0, ar1, a0, a1, e, mu, mv, mw, us;
+};
+template <>
+struct FluxT<float>
+{
+  float  ro0, ro1, ro, uo, pt, t1, t2, phi0;
+  double etot;
 };
 using Flux = FluxT<double>;
 
@@ -693,7 +705,7 @@ hllc(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
 // arithmetic and is not repeated in FP32, where it would lose the pressure to the cancellation.
 template <int DIM>
 G5D FluxT<float>
-hllc_mixed(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, double dtdx, int AX)
+hllc_mixed(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, double, int AX)
 {
   const float ulN = AX == 0 ? L.u : (AX == 1 ? L.v : L.w);
   const float ulT1 = AX == 1 ? L.u : L.v;
@@ -704,38 +716,42 @@ hllc_mixed(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, 
 
   const float smallr = S.smallrf, smallp = S.smallpf, smallc = S.smallcf;
 
-  const float rl = fmaxf(L.r0 + L.r1, smallr);
-  const float pl = fmaxf(L.p, rl * smallp);
-  const float ul = ulN;
-  float       kl = ul * ul + ulT1 * ulT1;
-  if (DIM == 3)
-    kl += ulT2 * ulT2;
-  const float  cl = eos_sound_speed(S, rl, pl, L.a, 1.0f - L.a);
-  const double al = L.a;
-  const double etotl = (static_cast<double>(pl) * eos_xi(S, al, 1.0 - al) + eos_pi(S, al, 1.0 - al)) +
-                       static_cast<double>(0.5f * rl * kl);
+  // total energies in FP64 from the FP32 face states: rho e = p xi + Pi, plus the kinetic energy
+  auto etot_of = [&](float r, float p, float a, float un, float t1, float t2) {
+    const double ad = a, ud = un, t1d = t1, t2d = t2;
+    double       k = ud * ud + t1d * t1d;
+    if (DIM == 3)
+      k += t2d * t2d;
+    return (static_cast<double>(p) * eos_xi(S, ad, 1.0 - ad) + eos_pi(S, ad, 1.0 - ad)) +
+           0.5 * static_cast<double>(r) * k;
+  };
 
-  const float rr = fmaxf(R.r0 + R.r1, smallr);
-  const float pr = fmaxf(R.p, rr * smallp);
-  const float ur = urN;
-  float       kr = ur * ur + urT1 * urT1;
-  if (DIM == 3)
-    kr += urT2 * urT2;
+  const float  rl = fmaxf(L.r0 + L.r1, smallr);
+  const float  pl = fmaxf(L.p, rl * smallp);
+  const float  ul = ulN;
+  const float  cl = eos_sound_speed(S, rl, pl, L.a, 1.0f - L.a);
+  const double etotl = etot_of(rl, pl, L.a, ul, ulT1, ulT2);
+
+  const float  rr = fmaxf(R.r0 + R.r1, smallr);
+  const float  pr = fmaxf(R.p, rr * smallp);
+  const float  ur = urN;
   const float  cr = eos_sound_speed(S, rr, pr, R.a, 1.0f - R.a);
-  const double ar = R.a;
-  const double etotr = (static_cast<double>(pr) * eos_xi(S, ar, 1.0 - ar) + eos_pi(S, ar, 1.0 - ar)) +
-                       static_cast<double>(0.5f * rr * kr);
+  const double etotr = etot_of(rr, pr, R.a, ur, urT1, urT2);
 
   const float cfastl = fmaxf(cl, smallc);
   const float cfastr = fmaxf(cr, smallc);
   const float SL = fminf(ul, ur) - fmaxf(cfastl, cfastr);
   const float SR = fmaxf(ul, ur) + fmaxf(cfastl, cfastr);
 
+  // star velocity and pressure written as corrections of the left state, so that a face with
+  // equal velocities and pressures on both sides gets u* = u and p* = p exactly in FP32 (the
+  // HLLC contact property survives the rounding: an interface advected at uniform u and p stays
+  // in equilibrium)
   const float rcl = rl * (ul - SL);
   const float rcr = rr * (SR - ur);
   const float inv_rc = 1.0f / (rcr + rcl);
-  const float ustar0 = (rcr * ur + rcl * ul + (pl - pr)) * inv_rc;
-  const float ptotstar = (rcr * pl + rcl * pr + rcl * rcr * (ul - ur)) * inv_rc;
+  const float ustar0 = ul + (rcr * (ur - ul) + (pl - pr)) * inv_rc;
+  const float ptotstar = pl + (rcl * (pr - pl) + rcl * rcr * (ul - ur)) * inv_rc;
 
   float  ro, ro_0, ro_1, uo, ptoto;
   double etoto;
@@ -750,9 +766,9 @@ hllc_mixed(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, 
   }
   else if (ustar0 > 0.0f)
   {
-    // etot* = f etot + (p* u* - p u)/(S - u*), f = (S - u)/(S - u*)
+    // etot* = f etot + (p* u* - p u)/(S - u*), f = (S - u)/(S - u*) (exactly 1 when u* = u)
     const float inv = 1.0f / (SL - ustar0);
-    const float factor = (SL - ul) * inv;
+    const float factor = ustar0 == ul ? 1.0f : (SL - ul) * inv;
     ro_0 = L.r0 * factor;
     ro_1 = L.r1 * factor;
     ro = rl * factor;
@@ -763,7 +779,7 @@ hllc_mixed(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, 
   else if (SR > 0.0f)
   {
     const float inv = 1.0f / (SR - ustar0);
-    const float factor = (SR - ur) * inv;
+    const float factor = ustar0 == ur ? 1.0f : (SR - ur) * inv;
     ro_0 = R.r0 * factor;
     ro_1 = R.r1 * factor;
     ro = rr * factor;
@@ -781,29 +797,17 @@ hllc_mixed(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, 
     etoto = etotr;
   }
 
-  const float  dtdxf = static_cast<float>(dtdx);
-  const float  fd = (ro_0 + ro_1) * uo;
-  const float  fn = ro * uo * uo + ptoto;
-  const double fe = (etoto + static_cast<double>(ptoto)) * static_cast<double>(uo);
-  const bool   up = fd > 0.0f;
-  const float  ft1 = fd * (up ? ulT1 : urT1);
-  const float  ft2 = DIM == 3 ? fd * (up ? ulT2 : urT2) : 0.0f;
-  const float  phi0 = up ? L.a : R.a;
-  const float  phi1 = up ? 1.0f - L.a : 1.0f - R.a;
-
+  const bool   up = (ro_0 + ro_1) * uo > 0.0f;
   FluxT<float> F;
-  F.ar0 = (ro_0 * uo) * dtdxf;
-  F.ar1 = (ro_1 * uo) * dtdxf;
-  F.a0 = (uo * phi0) * dtdxf;
-  F.a1 = (uo * phi1) * dtdxf;
-  F.e = fe * dtdx;
-  const float fu = AX == 0 ? fn : (AX == 1 ? ft1 : ft2);
-  const float fv = AX == 1 ? fn : ft1;
-  const float fw = AX == 2 ? fn : ft2;
-  F.mu = fu * dtdxf;
-  F.mv = fv * dtdxf;
-  F.mw = DIM == 3 ? fw * dtdxf : 0.0f;
-  F.us = uo * dtdxf;
+  F.ro0 = ro_0;
+  F.ro1 = ro_1;
+  F.ro = ro;
+  F.uo = uo;
+  F.pt = ptoto;
+  F.t1 = up ? ulT1 : urT1;
+  F.t2 = DIM == 3 ? (up ? ulT2 : urT2) : 0.0f;
+  F.phi0 = up ? L.a : R.a;
+  F.etot = etoto;
   return F;
 }
 
@@ -825,32 +829,31 @@ riemann(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, dou
 // (.h:319-393): accumulate left minus right flux with the Allaire-Kokh alpha source term (time-n
 // alpha of the cell), then add to the running U2 with the 1e-12 clamps.
 // ---------------------------------------------------------------------------------------------
-// (the flux fields are widened to double: the state accumulation is FP64 in both paths)
-template <int DIM, typename T>
+template <int DIM>
 G5D void
-apply_flux_pair(Cons & u2, FluxT<T> const & FL, FluxT<T> const & FR, double ua0, double ua1)
+apply_flux_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double ua1)
 {
   const double eps = 1e-12;
-  double       acc_r0 = 0.0 + static_cast<double>(FL.ar0);
-  double       acc_r1 = 0.0 + static_cast<double>(FL.ar1);
-  double       acc_a0 = 0.0 + static_cast<double>(FL.a0);
-  double       acc_a1 = 0.0 + static_cast<double>(FL.a1);
+  double       acc_r0 = 0.0 + FL.ar0;
+  double       acc_r1 = 0.0 + FL.ar1;
+  double       acc_a0 = 0.0 + FL.a0;
+  double       acc_a1 = 0.0 + FL.a1;
   double       acc_e = 0.0 + FL.e;
-  double       acc_mu = 0.0 + static_cast<double>(FL.mu);
-  double       acc_mv = 0.0 + static_cast<double>(FL.mv);
-  double       acc_mw = 0.0 + static_cast<double>(FL.mw);
-  acc_a0 -= 1.0 * static_cast<double>(FL.us) * ua0;
-  acc_a1 -= 1.0 * static_cast<double>(FL.us) * ua1;
-  acc_r0 = acc_r0 - static_cast<double>(FR.ar0);
-  acc_r1 = acc_r1 - static_cast<double>(FR.ar1);
-  acc_a0 = acc_a0 - static_cast<double>(FR.a0);
-  acc_a1 = acc_a1 - static_cast<double>(FR.a1);
+  double       acc_mu = 0.0 + FL.mu;
+  double       acc_mv = 0.0 + FL.mv;
+  double       acc_mw = 0.0 + FL.mw;
+  acc_a0 -= 1.0 * FL.us * ua0;
+  acc_a1 -= 1.0 * FL.us * ua1;
+  acc_r0 = acc_r0 - FR.ar0;
+  acc_r1 = acc_r1 - FR.ar1;
+  acc_a0 = acc_a0 - FR.a0;
+  acc_a1 = acc_a1 - FR.a1;
   acc_e = acc_e - FR.e;
-  acc_mu = acc_mu - static_cast<double>(FR.mu);
-  acc_mv = acc_mv - static_cast<double>(FR.mv);
-  acc_mw = acc_mw - static_cast<double>(FR.mw);
-  acc_a0 += 1.0 * static_cast<double>(FR.us) * ua0;
-  acc_a1 += 1.0 * static_cast<double>(FR.us) * ua1;
+  acc_mu = acc_mu - FR.mu;
+  acc_mv = acc_mv - FR.mv;
+  acc_mw = acc_mw - FR.mw;
+  acc_a0 += 1.0 * FR.us * ua0;
+  acc_a1 += 1.0 * FR.us * ua1;
 
   u2.ar0 += acc_r0;
   if (u2.ar0 < eps)
@@ -873,6 +876,51 @@ apply_flux_pair(Cons & u2, FluxT<T> const & FL, FluxT<T> const & FR, double ua0,
   u2.mv += acc_mv;
   if (DIM == 3)
     u2.mw += acc_mw;
+}
+
+// Mixed path: the FP64 face flux (scaled by dt/dx, rotated back to x, y, z momentum) from the
+// FP32 star state of an AX face; the same operations as the flux part of hllc.
+template <int DIM>
+G5D Flux
+assemble_flux(FluxT<float> const & s, double dtdx, int AX)
+{
+  const double ro0 = s.ro0, ro1 = s.ro1, ro = s.ro, uo = s.uo, pt = s.pt;
+  const double fd = (ro0 + ro1) * uo;
+  const double fn = ro * uo * uo + pt;
+  const double fe = (s.etot + pt) * uo;
+  const double ft1 = fd * static_cast<double>(s.t1);
+  const double ft2 = DIM == 3 ? fd * static_cast<double>(s.t2) : 0.0;
+  const double phi0 = s.phi0;
+  const double phi1 = 1.0 - phi0;
+  Flux         F;
+  F.ar0 = (ro0 * uo) * dtdx;
+  F.ar1 = (ro1 * uo) * dtdx;
+  F.a0 = (uo * phi0) * dtdx;
+  F.a1 = (uo * phi1) * dtdx;
+  F.e = fe * dtdx;
+  const double fu = AX == 0 ? fn : (AX == 1 ? ft1 : ft2);
+  const double fv = AX == 1 ? fn : ft1;
+  const double fw = AX == 2 ? fn : ft2;
+  F.mu = fu * dtdx;
+  F.mv = fv * dtdx;
+  F.mw = DIM == 3 ? fw * dtdx : 0.0;
+  F.us = uo * dtdx;
+  return F;
+}
+
+// the update with the face records of either path (dtdx and AX are used by the mixed path)
+template <int DIM>
+G5D void
+apply_face_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double ua1, double, int)
+{
+  apply_flux_pair<DIM>(u2, FL, FR, ua0, ua1);
+}
+template <int DIM>
+G5D void
+apply_face_pair(Cons & u2, FluxT<float> const & FL, FluxT<float> const & FR, double ua0, double ua1, double dtdx,
+                int AX)
+{
+  apply_flux_pair<DIM>(u2, assemble_flux<DIM>(FL, dtdx, AX), assemble_flux<DIM>(FR, dtdx, AX), ua0, ua1);
 }
 
 // scheme/FixVolumeFractionsNormalization.cpp:58-70

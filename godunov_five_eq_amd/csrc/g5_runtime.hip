@@ -131,7 +131,9 @@ struct g5_pool
   double    tuned_ms = 0.0;
   int32_t   cus = 256;
 
-  double last_run_ms = 0.0;
+  double     last_run_ms = 0.0;
+  hipEvent_t ev[2] = { nullptr, nullptr }; // around the last stepping call
+  bool       timing_pending = false;
 };
 
 struct g5_ctx
@@ -581,6 +583,8 @@ pool_capture(g5_pool * p)
   return G5_OK;
 }
 
+int pool_timing(g5_pool * p);
+
 // one device-dt step of every active solve: step kernel + finalize, as a HIP graph
 int
 pool_step_once(g5_pool * p)
@@ -595,7 +599,7 @@ pool_step_once(g5_pool * p)
 }
 
 int
-pool_step(g5_pool * p, int32_t nsteps)
+pool_step(g5_pool * p, int32_t nsteps, bool sync)
 {
   int rc = rebuild_list(p);
   if (rc != G5_OK)
@@ -603,24 +607,35 @@ pool_step(g5_pool * p, int32_t nsteps)
   rc = ensure_tuned(p, pool_hancock(p));
   if (rc != G5_OK)
     return rc;
-  hipEvent_t e0, e1;
-  G5_HIP(p->err, hipEventCreate(&e0));
-  G5_HIP(p->err, hipEventCreate(&e1));
-  G5_HIP(p->err, hipEventRecord(e0, p->stream));
+  if (p->ev[0] == nullptr)
+  {
+    G5_HIP(p->err, hipEventCreate(&p->ev[0]));
+    G5_HIP(p->err, hipEventCreate(&p->ev[1]));
+  }
+  G5_HIP(p->err, hipEventRecord(p->ev[0], p->stream));
   for (int32_t n = 0; n < nsteps; ++n)
   {
     rc = pool_step_once(p);
     if (rc != G5_OK)
       return rc;
   }
-  G5_HIP(p->err, hipEventRecord(e1, p->stream));
-  G5_HIP(p->err, hipEventSynchronize(e1));
-  float ms = 0.f;
-  G5_HIP(p->err, hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  p->last_run_ms = ms;
+  G5_HIP(p->err, hipEventRecord(p->ev[1], p->stream));
   G5_HIP(p->err, hipGetLastError());
+  p->timing_pending = true;
+  return sync ? pool_timing(p) : G5_OK;
+}
+
+// wait for the last stepping call and read its HIP-event duration
+int
+pool_timing(g5_pool * p)
+{
+  if (!p->timing_pending)
+    return G5_OK;
+  G5_HIP(p->err, hipEventSynchronize(p->ev[1]));
+  float ms = 0.f;
+  G5_HIP(p->err, hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
+  p->last_run_ms = ms;
+  p->timing_pending = false;
   return G5_OK;
 }
 
@@ -814,6 +829,9 @@ g5_pool_destroy(g5_pool * p)
     hipStreamSynchronize(p->stream);
   if (p->graph)
     hipGraphExecDestroy(p->graph);
+  for (hipEvent_t e : p->ev)
+    if (e)
+      hipEventDestroy(e);
   hipFree(p->buf[0]);
   hipFree(p->buf[1]);
   hipFree(p->stage);
@@ -918,7 +936,15 @@ g5_pool_step(g5_pool * p, int32_t nsteps)
 {
   if (not p or nsteps < 0)
     return G5_ERR_INVALID;
-  return pool_step(p, nsteps);
+  return pool_step(p, nsteps, true);
+}
+
+int
+g5_pool_step_async(g5_pool * p, int32_t nsteps)
+{
+  if (not p or nsteps < 0)
+    return G5_ERR_INVALID;
+  return pool_step(p, nsteps, false);
 }
 
 int
@@ -980,10 +1006,13 @@ g5_pool_usage(const g5_pool * p, int64_t * used_blocks, int64_t * capacity_block
 }
 
 int
-g5_pool_last_step_ms(const g5_pool * p, double * ms)
+g5_pool_last_step_ms(g5_pool * p, double * ms)
 {
   if (not p or not ms)
     return G5_ERR_INVALID;
+  const int rc = pool_timing(p);
+  if (rc != G5_OK)
+    return rc;
   *ms = p->last_run_ms;
   return G5_OK;
 }
@@ -1373,7 +1402,7 @@ g5_run(g5_ctx * ctx, int32_t nsteps, double * t, double tEnd, int32_t * steps_do
   G5_HIP(ctx->err, hipMemcpyAsync(p->d_ctl + ctx->id, &c0, sizeof(SolveCtl), hipMemcpyHostToDevice, p->stream));
   rc = pool_start(p, &ctx->id, 1);
   if (rc == G5_OK)
-    rc = pool_step(p, nsteps);
+    rc = pool_step(p, nsteps, true);
   if (rc == G5_OK)
     rc = read_ctl(p);
   if (rc != G5_OK)

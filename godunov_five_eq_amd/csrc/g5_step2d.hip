@@ -130,10 +130,10 @@ st_load2(T const * A, int k)
   f.w = T(0.0);
   return f;
 }
-// fluxes: 7 values (ar0, ar1, a0, a1, mu, mv, us) at slot k of a [7][NF2] array, energy in E[k]
-template <typename T>
+// face records at slot k of a [7][NF2] array plus E[k]: the FP64 flux (ar0, ar1, a0, a1, mu, mv,
+// us; energy in E) or the mixed path's star state (ro0, ro1, ro, uo, pt, t1, phi0; etot in E)
 G5D void
-fl_store2(T * A, double * E, int k, FluxT<T> const & F)
+fl_store2(double * A, double * E, int k, Flux const & F)
 {
   A[0 * NF2 + k] = F.ar0;
   A[1 * NF2 + k] = F.ar1;
@@ -144,20 +144,46 @@ fl_store2(T * A, double * E, int k, FluxT<T> const & F)
   A[6 * NF2 + k] = F.us;
   E[k] = F.e;
 }
-template <typename T>
-G5D FluxT<T>
-fl_load2(T const * A, double const * E, int k)
+G5D Flux
+fl_load2(double const * A, double const * E, int k)
 {
-  FluxT<T> F;
+  Flux F;
   F.ar0 = A[0 * NF2 + k];
   F.ar1 = A[1 * NF2 + k];
   F.a0 = A[2 * NF2 + k];
   F.a1 = A[3 * NF2 + k];
   F.mu = A[4 * NF2 + k];
   F.mv = A[5 * NF2 + k];
-  F.mw = T(0.0);
+  F.mw = 0.0;
   F.us = A[6 * NF2 + k];
   F.e = E[k];
+  return F;
+}
+G5D void
+fl_store2(float * A, double * E, int k, FluxT<float> const & F)
+{
+  A[0 * NF2 + k] = F.ro0;
+  A[1 * NF2 + k] = F.ro1;
+  A[2 * NF2 + k] = F.ro;
+  A[3 * NF2 + k] = F.uo;
+  A[4 * NF2 + k] = F.pt;
+  A[5 * NF2 + k] = F.t1;
+  A[6 * NF2 + k] = F.phi0;
+  E[k] = F.etot;
+}
+G5D FluxT<float>
+fl_load2(float const * A, double const * E, int k)
+{
+  FluxT<float> F;
+  F.ro0 = A[0 * NF2 + k];
+  F.ro1 = A[1 * NF2 + k];
+  F.ro = A[2 * NF2 + k];
+  F.uo = A[3 * NF2 + k];
+  F.pt = A[4 * NF2 + k];
+  F.t1 = A[5 * NF2 + k];
+  F.t2 = 0.0f;
+  F.phi0 = A[6 * NF2 + k];
+  F.etot = E[k];
   return F;
 }
 
@@ -335,8 +361,8 @@ step2d_block(StepArgs const & A, int blk)
   }
   __syncthreads();
   Cons u2 = uown;
-  apply_flux_pair<2>(u2, Fx, fl_load2(FX, FXE, fx2(i + 1, j)), uown.a0, uown.a1);
-  apply_flux_pair<2>(u2, Fy, fl_load2(FY, FYE, fy2(i, j + 1)), uown.a0, uown.a1);
+  apply_face_pair<2>(u2, Fx, fl_load2(FX, FXE, fx2(i + 1, j)), uown.a0, uown.a1, dtdx, 0);
+  apply_face_pair<2>(u2, Fy, fl_load2(FY, FYE, fy2(i, j + 1)), uown.a0, uown.a1, dtdx, 1);
   fix_volume_fractions(u2);
   store_cons<2, B2>(U2, own, u2);
 

@@ -135,10 +135,10 @@ st_load3(T const * A, int k)
   f.w = A[6 * N + k];
   return f;
 }
-// fluxes: 8 values (ar0, ar1, a0, a1, mu, mv, mw, us) at slot k of a [8][NF3] array, energy in E[k]
-template <typename T>
+// face records at slot k of a [8][NF3] array plus E[k]: the FP64 flux (ar0, ar1, a0, a1, mu, mv,
+// mw, us; energy in E) or the mixed path's star state (ro0, ro1, ro, uo, pt, t1, t2, phi0; etot)
 G5D void
-fl_store3(T * A, double * E, int k, FluxT<T> const & F)
+fl_store3(double * A, double * E, int k, Flux const & F)
 {
   A[0 * NF3 + k] = F.ar0;
   A[1 * NF3 + k] = F.ar1;
@@ -150,11 +150,10 @@ fl_store3(T * A, double * E, int k, FluxT<T> const & F)
   A[7 * NF3 + k] = F.us;
   E[k] = F.e;
 }
-template <typename T>
-G5D FluxT<T>
-fl_load3(T const * A, double const * E, int k)
+G5D Flux
+fl_load3(double const * A, double const * E, int k)
 {
-  FluxT<T> F;
+  Flux F;
   F.ar0 = A[0 * NF3 + k];
   F.ar1 = A[1 * NF3 + k];
   F.a0 = A[2 * NF3 + k];
@@ -164,6 +163,34 @@ fl_load3(T const * A, double const * E, int k)
   F.mw = A[6 * NF3 + k];
   F.us = A[7 * NF3 + k];
   F.e = E[k];
+  return F;
+}
+G5D void
+fl_store3(float * A, double * E, int k, FluxT<float> const & F)
+{
+  A[0 * NF3 + k] = F.ro0;
+  A[1 * NF3 + k] = F.ro1;
+  A[2 * NF3 + k] = F.ro;
+  A[3 * NF3 + k] = F.uo;
+  A[4 * NF3 + k] = F.pt;
+  A[5 * NF3 + k] = F.t1;
+  A[6 * NF3 + k] = F.t2;
+  A[7 * NF3 + k] = F.phi0;
+  E[k] = F.etot;
+}
+G5D FluxT<float>
+fl_load3(float const * A, double const * E, int k)
+{
+  FluxT<float> F;
+  F.ro0 = A[0 * NF3 + k];
+  F.ro1 = A[1 * NF3 + k];
+  F.ro = A[2 * NF3 + k];
+  F.uo = A[3 * NF3 + k];
+  F.pt = A[4 * NF3 + k];
+  F.t1 = A[5 * NF3 + k];
+  F.t2 = A[6 * NF3 + k];
+  F.phi0 = A[7 * NF3 + k];
+  F.etot = E[k];
   return F;
 }
 
@@ -349,7 +376,7 @@ step3d_body(StepArgs const & A, int blk)
     const FluxT<T> Fz = riemann<3>(S, zL, fzm, dtdx, 2);
     if (P >= 1)
     {
-      apply_flux_pair<3>(u2prev, zFprev, Fz, uaprev0, uaprev1);
+      apply_face_pair<3>(u2prev, zFprev, Fz, uaprev0, uaprev1, dtdx, 2);
       fix_volume_fractions(u2prev);
       store_cons<3, B3>(U2, blk_off + i + 16 * j + 256 * (P - 1), u2prev);
       inv_max = fmax(inv_max, cell_rate<3, T>(S, u2prev));
@@ -438,10 +465,10 @@ step3d_body(StepArgs const & A, int blk)
     if (inner_plane)
     {
       u2prev = uraw_cur;
-      apply_flux_pair<3>(u2prev, fl_load3(sFX, sFXE, fx3(i, j)), fl_load3(sFX, sFXE, fx3(i + 1, j)), uraw_cur.a0,
-                         uraw_cur.a1);
-      apply_flux_pair<3>(u2prev, fl_load3(sFY, sFYE, fy3(i, j)), fl_load3(sFY, sFYE, fy3(i, j + 1)), uraw_cur.a0,
-                         uraw_cur.a1);
+      apply_face_pair<3>(u2prev, fl_load3(sFX, sFXE, fx3(i, j)), fl_load3(sFX, sFXE, fx3(i + 1, j)), uraw_cur.a0,
+                         uraw_cur.a1, dtdx, 0);
+      apply_face_pair<3>(u2prev, fl_load3(sFY, sFYE, fy3(i, j)), fl_load3(sFY, sFYE, fy3(i, j + 1)), uraw_cur.a0,
+                         uraw_cur.a1, dtdx, 1);
       uaprev0 = uraw_cur.a0;
       uaprev1 = uraw_cur.a1;
     }

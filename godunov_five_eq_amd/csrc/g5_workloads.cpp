@@ -282,6 +282,9 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
     g5_pool_destroy(pools[1]);
     return rc;
   }
+  for (g5_pool * p : pools)
+    if (p)
+      g5_pool_set_autotune(p, c->autotune);
   struct Live
   {
     int32_t req;
@@ -350,10 +353,10 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
       }
       continue;
     }
-    // step everything in flight
+    // step everything in flight: both pools' steps are enqueued before either is waited for
     for (int k = 0; k < 2 and rc == G5_OK; ++k)
       if (!live[k].empty())
-        rc = g5_pool_step(pools[k], c->steps_per_poll);
+        rc = g5_pool_step_async(pools[k], c->steps_per_poll);
     // collect finished solves
     for (int k = 0; k < 2 and rc == G5_OK; ++k)
     {

@@ -235,8 +235,12 @@ int g5_pool_finished(g5_pool * pool, int32_t * solve_ids, int32_t cap, int32_t *
 /* blocks in use, block capacity of the budget, live solves */
 int g5_pool_usage(const g5_pool * pool, int64_t * used_blocks, int64_t * capacity_blocks, int32_t * live_solves);
 
-/* HIP-event duration of the last g5_pool_step call (ms) */
-int g5_pool_last_step_ms(const g5_pool * pool, double * ms);
+/* enqueue nsteps steps on the pool's stream and return without waiting (g5_pool_finished,
+ * g5_pool_status, g5_pool_download and g5_pool_last_step_ms wait for them) */
+int g5_pool_step_async(g5_pool * pool, int32_t nsteps);
+
+/* HIP-event duration of the last stepping call (ms; waits for it) */
+int g5_pool_last_step_ms(g5_pool * pool, double * ms);
 
 /* Launch geometry of the step kernels.  No reference counterpart: the reference launches Kokkos
  * team policies whose shape is fixed by the block size (scheme/GodunovImplemV0.cpp:83-151).
@@ -333,6 +337,8 @@ typedef struct g5_serve_config
   uint64_t hbm_budget_bytes;
   int32_t  precision;
   int32_t  thinc_enabled;
+  int32_t  autotune; /* 0: default launch geometry (no tuning trials while serving) */
+  int32_t  pad;
 } g5_serve_config;
 
 typedef struct g5_serve_report

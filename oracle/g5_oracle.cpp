@@ -495,6 +495,21 @@ youngs_normal(int dim, double const * a, double * n)
 // =============================================================================================
 // Solver: mesh, arrays, V0 step
 // =============================================================================================
+// THINC face value of alpha (ComputeFluxesAndStoreTHINCFunctor.cpp:176-196, Zhang JCP 2025
+// eq. 43): am, ac, ap = alpha at -1, 0, +1 along the face axis, nd = |normal component|,
+// bb = +1 for the right face of the cell, -1 for the left one.
+static double
+thinc_face_alpha(double am, double ac, double ap, double nd, double thinc_beta, int bb)
+{
+  const double beta = thinc_beta * nd + 0.01;
+  const double A = (ap + am) / 2;
+  const double B = (ap - am) / 2;
+  const double C = det_tanh(0.5 * beta);
+  const double phi = ac;
+  const double D = det_tanh(0.5 * beta * (phi - A) / B);
+  return A + bb * B * (C + bb * D / C) / (1.0 + bb * D);
+}
+
 struct Solver
 {
   g5o_params P;
@@ -947,14 +962,8 @@ struct Solver
               st[(di + 1) + 3 * (dj + 1) + 9 * (dk + 1)] = q(ib, H.IA0, is + di, js + dj, ks + dk);
       }
       youngs_normal(dim, st, n);
-      const int    bb = ioff[dir] < 0 ? -1 : 1;
-      const double beta = P.thinc_beta * std::fabs(n[dir]) + 0.01;
-      const double A = (ap + am) / 2;
-      const double B = (ap - am) / 2;
-      const double C = det_tanh(0.5 * beta);
-      const double phi = a0c;
-      const double D = det_tanh(0.5 * beta * (phi - A) / B);
-      qr[H.IA0] = A + bb * B * (C + bb * D / C) / (1.0 + bb * D);
+      const int bb = ioff[dir] < 0 ? -1 : 1;
+      qr[H.IA0] = thinc_face_alpha(am, a0c, ap, std::fabs(n[dir]), P.thinc_beta, bb);
       qr[H.IAD0] = r0 + r0 / qc[H.IA0] * (qr[H.IA0] - qc[H.IA0]);
       qr[H.IAD1] = r1 + r1 / (1.0 - qc[H.IA0]) * (qc[H.IA0] - qr[H.IA0]);
     }
@@ -1456,6 +1465,12 @@ double
 g5o_slope(double q, double qp, double qm, double slope_type)
 {
   return slope_unsplit_scalar(q, qp, qm, slope_type);
+}
+
+double
+g5o_thinc_alpha(double am, double ac, double ap, double nd, double thinc_beta, int bb)
+{
+  return thinc_face_alpha(am, ac, ap, nd, thinc_beta, bb);
 }
 
 double

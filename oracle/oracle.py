@@ -73,6 +73,7 @@ def lib():
             "g5o_riemann_hllc": (None, [P(OracleParams), P(d), P(d), P(d), P(d)]),
             "g5o_slope": (d, [d, d, d, d]),
             "g5o_tanh": (d, [d]),
+            "g5o_thinc_alpha": (d, [d, d, d, d, d, i32]),
             "g5o_youngs": (None, [i32, P(d), P(d)]),
         }
         for name, (res, args) in sig.items():
@@ -174,6 +175,11 @@ def max_threads():
 
 def tanh(x):
     return lib().g5o_tanh(x)
+
+
+def thinc_alpha(am, ac, ap, nd, beta, side):
+    """THINC face value of alpha (side +1: right face, -1: left face)."""
+    return lib().g5o_thinc_alpha(am, ac, ap, nd, beta, side)
 
 
 def slope(q, qp, qm, slope_type):

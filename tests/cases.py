@@ -53,3 +53,48 @@ def air_helium_3d(nb=4, **kw):
             g5.region(0.0, (0.0, 0.138), 1.0)]
     U0 = g5.init_shock_bubble(cfg, regs, 0.2, (0.5, 0.5, 0.5), 0.25)
     return cfg, U0
+
+
+def interface_advection_2d(alpha_min=0.1, alpha_max=0.9, pinf=0.0, rho0=1000.0, rho1=1000.0,
+                           velocity=(100.0, 50.0), p0=1e5, nb=(4, 4), **kw):
+    """Periodic 2-D box with uniform pressure p0 and velocity, and a smooth material blob
+    (alpha0 from alpha_min outside to alpha_max inside a circle of radius 0.25, tanh width 0.03):
+    air (gamma 1.4) and a stiffened gas (gamma 4.4, pinf) with phase densities rho0 and rho1.
+    Returns (cfg, U0, pressure(U) -> per-cell pressure of the mixture EOS)."""
+    import numpy as np
+
+    dx = 1.0 / (16 * nb[1])
+    opts = dict(periodic=(1, 1, 0), bc=("PERIODIC",) * 6, cfl=0.5, gamma=(1.4, 4.4), pinf=(0.0, pinf),
+                thinc_beta=1.6, thinc_epsilon=1e-3)
+    opts.update(kw)
+    cfg = g5.make_config(2, nb, dx, **opts)
+    g1 = np.array([1 / 0.4, 1 / 3.4])
+    gp = np.array([0.0, 4.4 * pinf / 3.4])
+    ux, uy = velocity
+    nblk = nb[0] * nb[1]
+    i = np.arange(16)
+    X = (np.arange(nb[0])[:, None] * 16 + i[None, :] + 0.5) * dx  # [bx, i]
+    Y = (np.arange(nb[1])[:, None] * 16 + i[None, :] + 0.5) * dx  # [by, j]
+    U = np.zeros((nb[1], nb[0], 8, 16, 16))  # [by, bx, var, j, i]
+    x = X[None, :, None, :]
+    y = Y[:, None, :, None]
+    r = np.hypot(x - 0.5 * nb[0] * 16 * dx, y - 0.5)
+    a0 = alpha_min + (alpha_max - alpha_min) * 0.5 * (1 + np.tanh((0.25 - r) / 0.03))
+    ar0, ar1 = a0 * rho0, (1 - a0) * rho1
+    rho = ar0 + ar1
+    xi = a0 * g1[0] + (1 - a0) * g1[1]
+    Pi = a0 * gp[0] + (1 - a0) * gp[1]
+    E = p0 * xi + Pi + 0.5 * rho * (ux * ux + uy * uy)
+    for v, f in enumerate([rho, E, rho * ux, rho * uy, a0, ar0, 1 - a0, ar1]):
+        U[:, :, v] = f
+    U0 = U.reshape(nblk, 8, 256).ravel()
+
+    def pressure(S):
+        V = S.reshape(-1, 8, 256)
+        d, a = V[:, 0], V[:, 4]
+        vx, vy = V[:, 2] / d, V[:, 3] / d
+        xi_ = a * g1[0] + (1 - a) * g1[1]
+        Pi_ = a * gp[0] + (1 - a) * gp[1]
+        return (V[:, 1] - 0.5 * d * (vx * vx + vy * vy) - Pi_) / xi_
+
+    return cfg, U0, pressure

@@ -341,3 +341,123 @@ def test_oracle_deterministic(orc):
         res.append(o.get_state())
     orc.set_num_threads(max(1, orc.max_threads()))
     assert np.array_equal(res[0], res[1])
+
+
+@pytest.mark.parametrize("thinc", [False, True])
+@pytest.mark.parametrize("pinf,bound", [(0.0, 1e-12), (6e8, 5e-11)])
+def test_interface_advection_keeps_pressure_and_velocity(orc, thinc, pinf, bound):
+    # Abgrall criterion: a material interface advected at uniform pressure and velocity between
+    # two stiffened gases of equal phase density stays in pressure and velocity equilibrium to
+    # round-off (40 steps, diffuse interface with alpha in [0.1, 0.9], no clamps active).  It
+    # ties the mixture EOS, the HLLC energy flux, the alpha flux and the non-conservative alpha
+    # source together: a wrong sign or a missing term in any of them breaks it at O(1e-2).
+    # The pinf = 6e8 bound is the FP64 round-off of rho*e ~ Pi ~ 1.4e9 seen through the
+    # recovery p = (rho*e - Pi)/xi at p = 1e5 (cancellation factor ~3e4).
+    cfg, U0, pressure = cases.interface_advection_2d(pinf=pinf, thinc=thinc)
+    o = orc.Oracle(cfg)
+    o.set_state(U0)
+    o.run(40)
+    U = o.get_state()
+    V = U.reshape(-1, 8, 256)
+    assert np.abs(pressure(U) / 1e5 - 1).max() <= bound
+    assert np.abs(V[:, 2] / V[:, 0] / 100.0 - 1).max() <= 1e-13
+    assert np.abs(V[:, 3] / V[:, 0] / 50.0 - 1).max() <= 1e-13
+    assert V[:, 4].min() > 0.1 - 1e-9 and V[:, 4].max() < 0.9 + 1e-9
+
+
+def test_hancock_alpha_split_at_density_jumps(orc):
+    # Reference behaviour, kept: the Hancock predictor splits the total-density change between
+    # the phases by alpha (sr0_0 = sr0 * alpha, ComputeFluxesAndStoreFunctor.cpp:186-190), not by
+    # the phases' own gradients.  Across a diffuse interface with a density ratio of 1000 this
+    # drives the face alpha_rho of the light phase negative; the update then clamps it to zero
+    # (ReadFluxesAndConservativeUpdateFunctor.h:325-331) and pressure equilibrium is lost.  The
+    # same case without the Hancock predictor (RK1) keeps it to round-off.
+    cfg, U0, pressure = cases.interface_advection_2d(pinf=0.0, rho0=1.0, rho1=1000.0)
+    o = orc.Oracle(cfg)
+    o.set_state(U0)
+    o.run(5)
+    U = o.get_state()
+    assert np.abs(pressure(U) / 1e5 - 1).max() > 1e-2
+    assert U.reshape(-1, 8, 256)[:, 5].min() == 0.0
+    cfg, U0, pressure = cases.interface_advection_2d(pinf=0.0, rho0=1.0, rho1=1000.0, integrator="RK1")
+    o = orc.Oracle(cfg)
+    o.set_state(U0)
+    o.run(5)
+    assert np.abs(pressure(o.get_state()) / 1e5 - 1).max() <= 1e-12
+
+
+def test_exact_riemann_helper_matches_published_sod_star_state():
+    # Toro, table 4.2 (test 1): p* = 0.30313, u* = 0.92745, rho*L = 0.42632, rho*R = 0.26557
+    import exact_riemann as er
+
+    p, u = er.star_state(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4)
+    assert p == pytest.approx(0.30313, abs=5e-6) and u == pytest.approx(0.92745, abs=5e-6)
+    s = er.sample(np.array([0.5 + 0.2 * (u - 1e-3), 0.5 + 0.2 * (u + 1e-3)]), 0.2, 0.5,
+                  (1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 1.4)
+    assert s[0, 0] == pytest.approx(0.42632, abs=5e-6) and s[0, 1] == pytest.approx(0.26557, abs=5e-6)
+
+
+def _sod_l1(orc, g5, ncells, integrator="HANCOCK"):
+    import exact_riemann as er
+
+    nbx = ncells // 16
+    dx = 1.0 / ncells
+    cfg = g5.make_config(2, (nbx, 1), dx, bc=("ZERO_GRADIENT", "ZERO_GRADIENT", "WALL", "WALL", "NONE", "NONE"),
+                         cfl=0.8, slope_type=1.0, gamma=(1.4, 1.4), pinf=(0.0, 0.0), integrator=integrator)
+    U = np.zeros((nbx, 8, 16, 16))  # [block, var, j, i]
+    x = (np.arange(nbx)[:, None] * 16 + np.arange(16)[None, :] + 0.5) * dx  # [block, i]
+    left = x < 0.5
+    rho = np.where(left, 1.0, 0.125)[:, None, :].repeat(16, axis=1)
+    p = np.where(left, 1.0, 0.1)[:, None, :].repeat(16, axis=1)
+    U[:, 0] = rho
+    U[:, 1] = p / 0.4
+    U[:, 4] = 1.0
+    U[:, 5] = rho
+    o = orc.Oracle(cfg)
+    o.set_state(U.ravel())
+    t, n = o.run(100000, 0.0, 0.2)
+    assert t == pytest.approx(0.2, rel=1e-14)
+    V = o.get_state().reshape(nbx, 8, 16, 16)
+    num = V[:, 0, 8, :].ravel()  # one row; every row is identical
+    assert np.all(V[:, 0] == V[:, 0, 8:9, :])
+    exact = er.sample(x.ravel(), 0.2, 0.5, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 1.4)[0]
+    return np.abs(num - exact).mean()
+
+
+def test_sod_shock_tube_converges_to_exact_solution(orc, g5):
+    # HLLC + MUSCL-Hancock (minmod) on the Sod problem at t = 0.2: the density L1 error against
+    # the exact solution is small and falls with resolution at the rate a second-order scheme
+    # shows on a solution with a shock and a contact (first order in L1, ~0.8 observed).
+    e1 = _sod_l1(orc, g5, 256)
+    e2 = _sod_l1(orc, g5, 512)
+    assert e1 < 4e-3 and e2 < 2.2e-3, (e1, e2)  # measured 3.00e-3 and 1.76e-3
+    assert np.log2(e1 / e2) > 0.65, (e1, e2)
+
+
+def test_thinc_face_alpha_closed_form_bounds_and_symmetry(orc):
+    # Zhang JCP 2025 eq. 43 as used by the reference (ComputeFluxesAndStoreTHINCFunctor.cpp
+    # :176-196).  Closed form at the cell-centre value alpha_c = A = (am + ap)/2: D = 0 and the
+    # face value is A +- B tanh(beta/2), B = (ap - am)/2, beta = beta0*|n| + 0.01.
+    for am, ap, nd, b0 in [(0.1, 0.9, 1.0, 1.6), (0.8, 0.2, 0.6, 2.0), (0.0, 1.0, 0.3, 1.6)]:
+        A, B = 0.5 * (ap + am), 0.5 * (ap - am)
+        beta = b0 * nd + 0.01
+        for side in (1, -1):
+            got = orc.thinc_alpha(am, A, ap, nd, b0, side)
+            assert got == pytest.approx(A + side * B * np.tanh(0.5 * beta), rel=1e-14, abs=1e-15)
+    rng = np.random.default_rng(3)
+    for _ in range(2000):
+        am, ap = rng.uniform(0, 1, 2)
+        if abs(ap - am) < 1e-3:
+            continue
+        ac = am + (ap - am) * rng.uniform(0.01, 0.99)  # monotone triple
+        nd, b0 = rng.uniform(0, 1), rng.uniform(1.0, 2.5)
+        lo, hi = min(am, ap), max(am, ap)
+        fr = orc.thinc_alpha(am, ac, ap, nd, b0, 1)
+        fl = orc.thinc_alpha(am, ac, ap, nd, b0, -1)
+        # bounded by the neighbours, right face closer to ap, mirror symmetry
+        assert lo - 1e-14 <= fr <= hi + 1e-14 and lo - 1e-14 <= fl <= hi + 1e-14
+        assert abs(fr - ap) <= abs(fl - ap) + 1e-14
+        assert orc.thinc_alpha(ap, ac, am, nd, b0, -1) == pytest.approx(fr, abs=1e-14)
+    # monotone in the cell value
+    vals = [orc.thinc_alpha(0.1, ac, 0.9, 0.7, 1.6, 1) for ac in np.linspace(0.11, 0.89, 200)]
+    assert np.all(np.diff(vals) > 0)
