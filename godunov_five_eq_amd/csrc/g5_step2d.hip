@@ -260,18 +260,25 @@ predict_faces2(SolveDesc const & S, T const * sQ, int gi, int gj, T dtdx, bool a
 
 } // namespace
 
-// LDS bytes of the face-state / flux exchange area: 12 face-state rows or 14 flux rows of T plus
-// 2 energy-flux rows of double
+// One LDS arena, in doubles.  Phases 1-3 use [primitives 6 x 400 | face states 12 x 272] (T);
+// phase 4 reuses it from the start for the face records (14 x 272 T + 2 x 272 double), once the
+// barrier after the Riemann solves has retired every read of the primitives and face states.
+// FP64: 45.3 KB, so three workgroups fit in a CU's 160 KB.
 template <typename T>
-constexpr int SE2_DOUBLES = (14 * NF2 * static_cast<int>(sizeof(T)) + 2 * NF2 * 8) / 8;
+constexpr int ARENA2_BYTES_A = (6 * NT2 + 12 * NF2) * static_cast<int>(sizeof(T));
+template <typename T>
+constexpr int ARENA2_BYTES_B = 14 * NF2 * static_cast<int>(sizeof(T)) + 2 * NF2 * 8;
+template <typename T>
+constexpr int ARENA2_DOUBLES =
+  ((ARENA2_BYTES_A<T> > ARENA2_BYTES_B<T> ? ARENA2_BYTES_A<T> : ARENA2_BYTES_B<T>)+7) / 8;
 
 template <bool THINC, bool HANCOCK, typename T>
 G5D void
 step2d_block(StepArgs const & A, int blk)
 {
-  __shared__ T      sQ[6 * NT2];
-  __shared__ double sE[SE2_DOUBLES<T>];
+  __shared__ double sArena[ARENA2_DOUBLES<T>];
   __shared__ double sRed[4];
+  T * const         sQ = reinterpret_cast<T *>(sArena);
 
   const int       t = threadIdx.x;
   const BlockDesc bd = A.blocks[blk];
@@ -309,8 +316,8 @@ step2d_block(StepArgs const & A, int blk)
   __syncthreads();
 
   // ---- 2. slopes, Hancock terms, face states ----
-  T *      XL = reinterpret_cast<T *>(sE); // [6][NF2] left states of the x faces
-  T *      YL = XL + 6 * NF2;              // [6][NF2] left states of the y faces
+  T *      XL = sQ + 6 * NT2; // [6][NF2] left states of the x faces
+  T *      YL = XL + 6 * NF2;  // [6][NF2] left states of the y faces
   FaceT<T> fxm, fxp, fym, fyp, fdummy, fring;
   predict_faces2<THINC, HANCOCK>(S, sQ, i, j, dtdxT, true, 0, 0, fxm, fxp, fym, fyp, fdummy);
   st_store2(XL, fx2(i + 1, j), fxp);
@@ -346,7 +353,7 @@ step2d_block(StepArgs const & A, int blk)
   __syncthreads();
 
   // ---- 4. flux exchange and update ----
-  T *      FX = reinterpret_cast<T *>(sE); // [7][NF2]
+  T *      FX = reinterpret_cast<T *>(sArena); // [7][NF2] (primitives and face states are dead)
   T *      FY = FX + 7 * NF2;              // [7][NF2]
   double * FXE = reinterpret_cast<double *>(FY + 7 * NF2);
   double * FYE = FXE + NF2;
