@@ -112,6 +112,8 @@ class G5ServeReport(ctypes.Structure):
         ("mean_ms", ctypes.c_double),
         ("max_ms", ctypes.c_double),
         ("last_arrival_s", ctypes.c_double),
+        ("step_s_2d", ctypes.c_double),
+        ("step_s_3d", ctypes.c_double),
     ]
 
 
@@ -167,6 +169,7 @@ def lib():
             "g5_pool_set_autotune": (ctypes.c_int, [vp, i32]),
             "g5_pool_properties": (ctypes.c_int, [vp, P(i32), P(i32), P(i32)]),
             "g5_pool_prepare": (ctypes.c_int, [vp]),
+            "g5_pool_set_clock": (ctypes.c_int, [vp, i32, d, i32]),
             "g5_pool_step_async": (ctypes.c_int, [vp, i32]),
             "g5_batch_spec": (ctypes.c_int, [ctypes.c_uint64, i32, P(G5SolveSpec)]),
             "g5_serve_specs": (ctypes.c_int, [ctypes.c_uint64, i32, d, d, P(G5SolveSpec)]),
@@ -195,7 +198,7 @@ def exported_symbols():
         "g5_pool_upload", "g5_pool_download", "g5_pool_init_shock_bubble", "g5_pool_init_triple_point",
         "g5_pool_start", "g5_pool_step", "g5_pool_status", "g5_pool_finished", "g5_pool_usage",
         "g5_pool_last_step_ms", "g5_pool_set_launch", "g5_pool_get_launch", "g5_pool_set_autotune",
-        "g5_pool_properties", "g5_pool_prepare", "g5_pool_step_async", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
+        "g5_pool_properties", "g5_pool_prepare", "g5_pool_set_clock", "g5_pool_step_async", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
         "g5_pool_admit_spec", "g5_serve",
     ]
 
@@ -429,6 +432,7 @@ class Pool:
         if rc != G5_OK:
             raise G5Error(rc, lib().g5_pool_last_error(None).decode())
         self._cfg = {}
+        self._limits = {}
 
     def _check(self, rc):
         if rc != G5_OK:
@@ -455,7 +459,12 @@ class Pool:
         sid = ctypes.c_int32()
         self._check(lib().g5_pool_admit(self._h, ctypes.byref(cfg), t_end, max_steps, ctypes.byref(sid)))
         self._cfg[sid.value] = cfg
+        self._limits[sid.value] = (t_end, max_steps)
         return sid.value
+
+    def limits(self, sid):
+        """(t_end, max_steps) the solve was admitted with"""
+        return self._limits[sid]
 
     def try_admit(self, cfg, t_end=1e300, max_steps=2 ** 31 - 1):
         """admit, or None when the HBM budget has no room"""
@@ -465,11 +474,13 @@ class Pool:
             return None
         self._check(rc)
         self._cfg[sid.value] = cfg
+        self._limits[sid.value] = (t_end, max_steps)
         return sid.value
 
     def release(self, sid):
         self._check(lib().g5_pool_release(self._h, sid))
         self._cfg.pop(sid, None)
+        self._limits.pop(sid, None)
 
     def upload(self, sid, U):
         U = np.ascontiguousarray(U, dtype=np.float64).ravel()
@@ -542,6 +553,12 @@ class Pool:
         self._check(lib().g5_pool_properties(self._h, ctypes.byref(d), ctypes.byref(th), ctypes.byref(pr)))
         return spec_config(spec, bool(th.value), pr.value)
 
+    def set_clock(self, sid, t, steps_done):
+        self._check(lib().g5_pool_set_clock(self._h, sid, t, steps_done))
+
+    def config(self, sid):
+        return self._cfg[sid]
+
     def prepare(self):
         """Launch list, launch-geometry tuning and step-graph capture, without stepping."""
         self._check(lib().g5_pool_prepare(self._h))
@@ -551,6 +568,7 @@ class Pool:
         sid = ctypes.c_int32(-1)
         self._check(lib().g5_pool_admit_spec(self._h, ctypes.byref(spec), ctypes.byref(sid)))
         self._cfg[sid.value] = self._spec_cfg(spec)
+        self._limits[sid.value] = (1e300, spec.steps)
         return sid.value
 
     def try_admit_spec(self, spec):
@@ -560,4 +578,5 @@ class Pool:
             return None
         self._check(rc)
         self._cfg[sid.value] = self._spec_cfg(spec)
+        self._limits[sid.value] = (1e300, spec.steps)
         return sid.value

@@ -948,6 +948,28 @@ g5_pool_step_async(g5_pool * p, int32_t nsteps)
 }
 
 int
+g5_pool_set_clock(g5_pool * p, int32_t id, double t, int32_t steps_done)
+{
+  if (not p)
+    return G5_ERR_INVALID;
+  if (id < 0 or id >= static_cast<int32_t>(p->solves.size()) or !p->solves[id].live)
+    return fail(p->err, G5_ERR_INVALID, "no such solve");
+  if (p->solves[id].started)
+    return fail(p->err, G5_ERR_STATE, "the clock of a started solve cannot be set");
+  if (steps_done < 0 or not std::isfinite(t))
+    return fail(p->err, G5_ERR_INVALID, "t must be finite and steps_done >= 0");
+  int rc = read_ctl(p);
+  if (rc != G5_OK)
+    return rc;
+  SolveCtl c = p->h_ctl[id];
+  c.t = t;
+  c.steps = steps_done;
+  G5_HIP(p->err, hipMemcpyAsync(p->d_ctl + id, &c, sizeof(SolveCtl), hipMemcpyHostToDevice, p->stream));
+  G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  return G5_OK;
+}
+
+int
 g5_pool_status(g5_pool * p, int32_t id, double * t, int32_t * steps, int32_t * active)
 {
   if (not p)

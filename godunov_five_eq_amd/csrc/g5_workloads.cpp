@@ -298,6 +298,7 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
   int32_t              done = 0;
   double               cell_updates = 0.0;
   int32_t              max_inflight = 0;
+  double               step_ms[2] = { 0.0, 0.0 };
   using clk = std::chrono::steady_clock;
   const auto t0 = clk::now();
   auto       now = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
@@ -354,9 +355,20 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
       continue;
     }
     // step everything in flight: both pools' steps are enqueued before either is waited for
+    bool stepped[2] = { false, false };
     for (int k = 0; k < 2 and rc == G5_OK; ++k)
       if (!live[k].empty())
+      {
         rc = g5_pool_step_async(pools[k], c->steps_per_poll);
+        stepped[k] = rc == G5_OK;
+      }
+    for (int k = 0; k < 2 and rc == G5_OK; ++k)
+      if (stepped[k])
+      {
+        double ms = 0.0;
+        rc = g5_pool_last_step_ms(pools[k], &ms);
+        step_ms[k] += ms;
+      }
     // collect finished solves
     for (int k = 0; k < 2 and rc == G5_OK; ++k)
     {
@@ -410,6 +422,8 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
     sum += v;
   r->mean_ms = sum / lat.size();
   r->last_arrival_s = specs.back().arrival;
+  r->step_s_2d = step_ms[0] * 1e-3;
+  r->step_s_3d = step_ms[1] * 1e-3;
   if (latencies_ms)
     for (size_t k = 0; k < std::min(cap, lat.size()); ++k)
       latencies_ms[k] = lat[k];

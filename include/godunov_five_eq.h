@@ -229,6 +229,15 @@ int g5_pool_step(g5_pool * pool, int32_t nsteps);
 /* time, steps done and activity of one solve (synchronises) */
 int g5_pool_status(g5_pool * pool, int32_t solve_id, double * t, int32_t * steps, int32_t * active);
 
+/* Resume support (checkpoint/restart and eviction of a solve to host memory): set the time and
+ * the number of steps already done of an admitted solve before g5_pool_start.  A solve saved as
+ * (g5_pool_download state, g5_pool_status t and steps) and re-admitted with the same
+ * configuration, end time and step budget, uploaded, given its clock and started continues
+ * bit-identically (its first dt is recomputed from the state, as the step would have).
+ * Replaces the restart path of SolverGodunovFiveEq.cpp:811-846 / init/init.cpp:132-207 for
+ * one solve; the on-disk format is the caller's (the Python layer writes .npy + JSON). */
+int g5_pool_set_clock(g5_pool * pool, int32_t solve_id, double t, int32_t steps_done);
+
 /* live solves that reached t_end or their step budget: up to cap ids, *n_done = their number */
 int g5_pool_finished(g5_pool * pool, int32_t * solve_ids, int32_t cap, int32_t * n_done);
 
@@ -350,6 +359,7 @@ typedef struct g5_serve_report
   double  cell_updates_per_s;
   double  p50_ms, p99_ms, mean_ms, max_ms;
   double  last_arrival_s;
+  double  step_s_2d, step_s_3d; /* HIP-event time of the stepping calls of each pool */
 } g5_serve_report;
 
 /* latencies_ms (optional, cap entries) receives the latency of each request in arrival order */

@@ -90,3 +90,16 @@ def test_launch_geometries_cover_every_block_once(nblocks):
         for w in range(grid):
             seen += _block_range(w, nblocks, wg_per_xcd, remap, grid)
         assert sorted(seen) == list(range(nblocks)), (nblocks, wg_per_xcd, remap)
+
+
+def test_checkpoint_header_round_trip(g5, tmp_path):
+    # the JSON header keeps every configuration field exactly (host side only)
+    from godunov_five_eq_amd import checkpoint
+
+    cfg = g5.make_config(3, (3, 4, 5), 1.0 / 48, thinc=True, thinc_beta=1.7, gamma=(1.4, 4.4), pinf=(0.0, 6e8),
+                         precision=g5.PREC_MIXED, integrator="RK1", bc=("WALL",) * 6)
+    d = checkpoint._cfg_to_dict(cfg)
+    import json
+
+    back = checkpoint._cfg_from_dict(json.loads(json.dumps(d)))
+    assert bytes(back) == bytes(cfg)

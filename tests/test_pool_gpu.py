@@ -186,3 +186,35 @@ def test_3d_persistent_launch_is_rejected(g5):
     with g5.Pool(3, thinc=True, budget_bytes=1 << 26) as pool:
         with pytest.raises(g5.G5Error):
             pool.set_launch(1, 1, 1)
+
+
+def test_checkpoint_restart_is_bit_identical(g5, tmp_path):
+    # a solve saved after 7 steps, released, and restored into another pool (different slots)
+    # finishes its 19-step budget with the bits of the uninterrupted run, FP64 and mixed
+    from godunov_five_eq_amd import checkpoint
+
+    for prec in (g5.PREC_FP64, g5.PREC_MIXED):
+        say(f"checkpoint/restart precision={prec}")
+        cfg, U0 = cases.air_helium_2d(nbx=7, nby=5, thinc=True, thinc_beta=1.6, precision=prec)
+        with g5.Pool(2, thinc=True, precision=prec, budget_bytes=1 << 26) as pool:
+            sid = pool.admit(cfg, max_steps=19)
+            pool.upload(sid, U0)
+            pool.start([sid])
+            pool.step(25)
+            ref = pool.download(sid)
+            t_ref, n_ref, _ = pool.status(sid)
+        prefix = str(tmp_path / f"solve{prec}")
+        with g5.Pool(2, thinc=True, precision=prec, budget_bytes=1 << 26) as pool:
+            other = pool.admit(cfg)  # occupies the first slots, so the solve lands elsewhere
+            pool.upload(other, U0)
+            sid = pool.admit(cfg, max_steps=19)
+            pool.upload(sid, U0)
+            pool.start([sid])
+            pool.step(7)
+            checkpoint.save(prefix, pool, sid)
+        with g5.Pool(2, thinc=True, precision=prec, budget_bytes=1 << 26) as pool:
+            sid = checkpoint.load(prefix, pool)
+            pool.step(25)
+            t, n, active = pool.status(sid)
+            assert n == n_ref == 19 and t == t_ref and not active
+            assert np.array_equal(pool.download(sid), ref)
