@@ -216,7 +216,7 @@ def bench_batch(g5, args):
                      "precision": "mixed" if prec else "fp64", "block": [16, 16], "blocks": cells // 256,
                      "steps_per_solve": min(stepped), "admission_s": t_adm, "launch": launch,
                      "hip_graph": True, "rel_l1_vs_fp64_sample": l1, "l1_sample_solves": len(sample)}
-    out["roofline"] = roofline(cells, 8, ev_ms, None)
+    out["roofline"] = roofline(cells, 8, ev_ms, traffic_from_profile("step2d_batch_mixed" if prec else "step2d_batch_fp64"))
     if not args.no_cpu:
         # bounded sample: the first solves of the same batch (device-generated initial states)
         cu = dt = 0.0
