@@ -286,6 +286,32 @@ ensure_solve_capacity(g5_pool * p, int32_t need)
   return G5_OK;
 }
 
+// device side of an admission: descriptor, control record and slot-table segment of solve sid
+int
+admit_device(g5_pool * p, int32_t sid, double tend, int32_t max_steps)
+{
+  int rc = ensure_solve_capacity(p, static_cast<int32_t>(p->solves.size()));
+  if (rc != G5_OK)
+    return rc;
+  SolveRec const & r = p->solves[sid];
+  int64_t          ct = p->cap_table;
+  rc = grow(p, p->d_slots, ct, p->table.end, p->stream);
+  if (rc != G5_OK)
+    return rc;
+  p->cap_table = static_cast<int32_t>(ct);
+  SolveDesc sd;
+  make_solve_desc(r.cfg, r.table_base, sd);
+  SolveCtl c{};
+  c.tend = tend;
+  c.max_steps = max_steps;
+  G5_HIP(p->err, hipMemcpyAsync(p->d_slots + r.table_base, r.slots.data(), r.nblk * sizeof(int32_t),
+                                hipMemcpyHostToDevice, p->stream));
+  G5_HIP(p->err, hipMemcpyAsync(p->d_solves + sid, &sd, sizeof(SolveDesc), hipMemcpyHostToDevice, p->stream));
+  G5_HIP(p->err, hipMemcpyAsync(p->d_ctl + sid, &c, sizeof(SolveCtl), hipMemcpyHostToDevice, p->stream));
+  G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  return G5_OK;
+}
+
 int
 pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, int32_t * id)
 {
@@ -311,9 +337,6 @@ pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, i
     sid = static_cast<int32_t>(p->solves.size());
     p->solves.emplace_back();
   }
-  rc = ensure_solve_capacity(p, static_cast<int32_t>(p->solves.size()));
-  if (rc != G5_OK)
-    return rc;
   SolveRec & r = p->solves[sid];
   r.live = true;
   r.started = false;
@@ -328,21 +351,18 @@ pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, i
     p->free_slots.pop_back();
   }
   r.table_base = p->table.take(r.nblk);
-  int64_t ct = p->cap_table;
-  rc = grow(p, p->d_slots, ct, p->table.end, p->stream);
+  rc = admit_device(p, sid, tend, max_steps);
   if (rc != G5_OK)
+  {
+    // give everything back: the solve was never admitted
+    SolveRec & q = p->solves[sid];
+    for (auto it = q.slots.rbegin(); it != q.slots.rend(); ++it)
+      p->free_slots.push_back(*it);
+    p->table.give(q.table_base, q.nblk);
+    q = SolveRec();
+    p->free_ids.push_back(sid);
     return rc;
-  p->cap_table = static_cast<int32_t>(ct);
-  SolveDesc sd;
-  make_solve_desc(r.cfg, r.table_base, sd);
-  SolveCtl c{};
-  c.tend = tend;
-  c.max_steps = max_steps;
-  G5_HIP(p->err, hipMemcpyAsync(p->d_slots + r.table_base, r.slots.data(), nblk * sizeof(int32_t),
-                                hipMemcpyHostToDevice, p->stream));
-  G5_HIP(p->err, hipMemcpyAsync(p->d_solves + sid, &sd, sizeof(SolveDesc), hipMemcpyHostToDevice, p->stream));
-  G5_HIP(p->err, hipMemcpyAsync(p->d_ctl + sid, &c, sizeof(SolveCtl), hipMemcpyHostToDevice, p->stream));
-  G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  }
   p->list_dirty = true;
   *id = sid;
   return G5_OK;

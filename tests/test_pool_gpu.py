@@ -218,3 +218,33 @@ def test_checkpoint_restart_is_bit_identical(g5, tmp_path):
             t, n, active = pool.status(sid)
             assert n == n_ref == 19 and t == t_ref and not active
             assert np.array_equal(pool.download(sid), ref)
+
+
+def test_pool_error_paths(g5):
+    say("pool error paths")
+    cfg2, U0 = cases.air_helium_2d(nbx=4, nby=2)
+    cfg3, _ = cases.air_helium_3d(nb=2)
+    with g5.Pool(2, budget_bytes=1 << 22) as pool:  # 128 slots of 2-D blocks
+        with pytest.raises(g5.G5Error) as e:
+            pool.admit(cfg3)
+        assert e.value.code == g5.G5_ERR_INVALID
+        rk2, _ = cases.air_helium_2d(nbx=4, nby=2, integrator="RK2_SSP")
+        with pytest.raises(g5.G5Error) as e:
+            pool.admit(rk2)
+        assert e.value.code == g5.G5_ERR_UNSUPPORTED
+        big, _ = cases.air_helium_2d(nbx=16, nby=16)  # 256 blocks > 128 slots
+        assert pool.try_admit(big) is None
+        sid = pool.admit(cfg2, max_steps=3)  # the pool still works after the refusals
+        pool.upload(sid, U0)
+        pool.start([sid])
+        with pytest.raises(g5.G5Error) as e:
+            pool.set_clock(sid, 0.5, 2)
+        assert e.value.code == g5.G5_ERR_STATE
+        pool.step(5)
+        assert pool.status(sid)[1] == 3
+        pool.release(sid)
+        with pytest.raises(g5.G5Error):
+            pool.release(sid)
+        with pytest.raises(g5.G5Error):
+            pool.status(sid)
+        assert pool.usage()[0] == 0
