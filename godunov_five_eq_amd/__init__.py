@@ -114,6 +114,10 @@ class G5ServeReport(ctypes.Structure):
         ("last_arrival_s", ctypes.c_double),
         ("step_s_2d", ctypes.c_double),
         ("step_s_3d", ctypes.c_double),
+        ("admit_s", ctypes.c_double),
+        ("start_s", ctypes.c_double),
+        ("wait_s", ctypes.c_double),
+        ("collect_s", ctypes.c_double),
     ]
 
 
@@ -169,6 +173,8 @@ def lib():
             "g5_pool_set_autotune": (ctypes.c_int, [vp, i32]),
             "g5_pool_properties": (ctypes.c_int, [vp, P(i32), P(i32), P(i32)]),
             "g5_pool_prepare": (ctypes.c_int, [vp]),
+            "g5_host_alloc": (ctypes.c_int, [sz, P(vp)]),
+            "g5_host_free": (ctypes.c_int, [vp]),
             "g5_pool_set_clock": (ctypes.c_int, [vp, i32, d, i32]),
             "g5_pool_step_async": (ctypes.c_int, [vp, i32]),
             "g5_batch_spec": (ctypes.c_int, [ctypes.c_uint64, i32, P(G5SolveSpec)]),
@@ -198,7 +204,7 @@ def exported_symbols():
         "g5_pool_upload", "g5_pool_download", "g5_pool_init_shock_bubble", "g5_pool_init_triple_point",
         "g5_pool_start", "g5_pool_step", "g5_pool_status", "g5_pool_finished", "g5_pool_usage",
         "g5_pool_last_step_ms", "g5_pool_set_launch", "g5_pool_get_launch", "g5_pool_set_autotune",
-        "g5_pool_properties", "g5_pool_prepare", "g5_pool_set_clock", "g5_pool_step_async", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
+        "g5_pool_properties", "g5_pool_prepare", "g5_host_alloc", "g5_host_free", "g5_pool_set_clock", "g5_pool_step_async", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
         "g5_pool_admit_spec", "g5_serve",
     ]
 

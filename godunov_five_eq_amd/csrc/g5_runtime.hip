@@ -116,6 +116,10 @@ struct g5_pool
   hipGraphExec_t graph = nullptr;
   bool           graph_valid = false;
 
+  // staging of g5_pool_upload / g5_pool_download for solves whose slots are not one run
+  double * d_xfer = nullptr;
+  int64_t  cap_xfer = 0;
+
   // scratch of g5_pool_start (block list and ids of the solves being started)
   BlockDesc * d_start_list = nullptr;
   int64_t     cap_start_list = 0;
@@ -548,8 +552,8 @@ autotune(g5_pool * p, bool hancock)
       best = c;
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   G5_HIP(p->err, hipGetLastError());
   p->launch = best;
   p->tuned_ms = best_ms;
@@ -589,7 +593,7 @@ pool_capture(g5_pool * p)
     return G5_OK;
   if (p->graph)
   {
-    hipGraphExecDestroy(p->graph);
+    (void)hipGraphExecDestroy(p->graph);
     p->graph = nullptr;
   }
   hipGraph_t g;
@@ -598,7 +602,7 @@ pool_capture(g5_pool * p)
   launch_finalize(p->d_ctl, p->d_solves, static_cast<int>(p->solves.size()), p->stream);
   G5_HIP(p->err, hipStreamEndCapture(p->stream, &g));
   G5_HIP(p->err, hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0));
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   p->graph_valid = true;
   return G5_OK;
 }
@@ -689,6 +693,8 @@ pool_start(g5_pool * p, int32_t const * ids, int32_t n)
   return G5_OK;
 }
 
+int read_ctl(g5_pool * p);
+
 int
 pool_transfer(g5_pool * p, int32_t id, double * host, size_t n, bool upload, double ** staging)
 {
@@ -698,12 +704,31 @@ pool_transfer(g5_pool * p, int32_t id, double * host, size_t n, bool upload, dou
   const size_t     need = static_cast<size_t>(r.nblk) * p->block_doubles;
   if (n != need)
     return fail(p->err, G5_ERR_INVALID, "state size mismatch");
+  // slots in one ascending run (the usual case in a pool that is not fragmented): the state is
+  // already contiguous in the current buffer, one copy
+  bool contiguous = true;
+  for (int32_t k = 1; k < r.nblk and contiguous; ++k)
+    contiguous = r.slots[k] == r.slots[0] + k;
+  if (contiguous)
+  {
+    int rc = read_ctl(p);
+    if (rc != G5_OK)
+      return rc;
+    double * base = p->buf[p->h_ctl[id].parity] + static_cast<int64_t>(r.slots[0]) * p->block_doubles;
+    if (upload)
+      G5_HIP(p->err, hipMemcpyAsync(base, host, need * sizeof(double), hipMemcpyHostToDevice, p->stream));
+    else
+      G5_HIP(p->err, hipMemcpyAsync(host, base, need * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    G5_HIP(p->err, hipStreamSynchronize(p->stream));
+    return G5_OK;
+  }
   double * st = staging ? *staging : nullptr;
   if (!st)
   {
-    G5_HIP(p->err, hipMalloc(&st, need * sizeof(double)));
-    if (staging)
-      *staging = st;
+    int rc = grow(p, p->d_xfer, p->cap_xfer, static_cast<int64_t>(need), p->stream);
+    if (rc != G5_OK)
+      return rc;
+    st = p->d_xfer;
   }
   if (upload)
   {
@@ -719,8 +744,6 @@ pool_transfer(g5_pool * p, int32_t id, double * host, size_t n, bool upload, dou
   }
   G5_HIP(p->err, hipGetLastError());
   G5_HIP(p->err, hipStreamSynchronize(p->stream));
-  if (!staging)
-    hipFree(st);
   return G5_OK;
 }
 
@@ -848,7 +871,7 @@ g5_pool_destroy(g5_pool * p)
   if (p->stream)
     hipStreamSynchronize(p->stream);
   if (p->graph)
-    hipGraphExecDestroy(p->graph);
+    (void)hipGraphExecDestroy(p->graph);
   for (hipEvent_t e : p->ev)
     if (e)
       hipEventDestroy(e);
@@ -861,6 +884,7 @@ g5_pool_destroy(g5_pool * p)
   hipFree(p->d_blocks);
   hipFree(p->d_start_list);
   hipFree(p->d_start_ids);
+  hipFree(p->d_xfer);
   if (p->h_ctl)
     hipHostFree(p->h_ctl);
   if (p->stream)
@@ -1105,6 +1129,26 @@ g5_pool_get_launch(const g5_pool * p, g5_launch * launch)
   if (not p or not launch)
     return G5_ERR_INVALID;
   to_launch(p->launch, p, launch);
+  return G5_OK;
+}
+
+int
+g5_host_alloc(size_t bytes, void ** ptr)
+{
+  if (not ptr)
+    return G5_ERR_INVALID;
+  *ptr = nullptr;
+  const hipError_t e = hipHostMalloc(ptr, bytes == 0 ? 1 : bytes, hipHostMallocDefault);
+  if (e != hipSuccess)
+    return fail(g_create_error, G5_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  return G5_OK;
+}
+
+int
+g5_host_free(void * ptr)
+{
+  if (ptr)
+    hipHostFree(ptr);
   return G5_OK;
 }
 
@@ -1501,8 +1545,8 @@ g5_time_step_kernel(g5_ctx * ctx, int32_t nsteps, double dt, double * mean_ms)
   G5_HIP(ctx->err, hipEventSynchronize(e1));
   float ms = 0.f;
   G5_HIP(ctx->err, hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   *mean_ms = static_cast<double>(ms) / nsteps;
   return G5_OK;
 }

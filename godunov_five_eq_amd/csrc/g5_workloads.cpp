@@ -293,12 +293,14 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
   std::vector<Live>    live[2];
   std::vector<int32_t> queue; // arrived, not admitted
   std::vector<double>  lat(c->n_requests, 0.0);
-  std::vector<double>  host_result;
+  double *             host_result = nullptr; // pinned: results come back at full PCIe rate
+  size_t               host_cap = 0;
   size_t               next = 0;
   int32_t              done = 0;
   double               cell_updates = 0.0;
   int32_t              max_inflight = 0;
   double               step_ms[2] = { 0.0, 0.0 };
+  double               host_s[4] = { 0.0, 0.0, 0.0, 0.0 }; // admission, start, wait for steps, collection
   using clk = std::chrono::steady_clock;
   const auto t0 = clk::now();
   auto       now = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
@@ -308,6 +310,7 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
     const double tn = now();
     while (next < specs.size() and specs[next].arrival <= tn)
       queue.push_back(static_cast<int32_t>(next++));
+    double ph = now();
     // admission, first come first served per dimension
     std::vector<int32_t> started[2];
     size_t               q = 0;
@@ -339,9 +342,12 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
       started[k].push_back(id);
     }
     queue.swap(rest);
+    host_s[0] += now() - ph;
+    ph = now();
     for (int k = 0; k < 2 and rc == G5_OK; ++k)
       if (!started[k].empty())
         rc = g5_pool_start(pools[k], started[k].data(), static_cast<int32_t>(started[k].size()));
+    host_s[1] += now() - ph;
     const int32_t inflight = static_cast<int32_t>(live[0].size() + live[1].size());
     max_inflight = std::max(max_inflight, inflight);
     if (inflight == 0)
@@ -362,6 +368,7 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
         rc = g5_pool_step_async(pools[k], c->steps_per_poll);
         stepped[k] = rc == G5_OK;
       }
+    ph = now();
     for (int k = 0; k < 2 and rc == G5_OK; ++k)
       if (stepped[k])
       {
@@ -369,6 +376,8 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
         rc = g5_pool_last_step_ms(pools[k], &ms);
         step_ms[k] += ms;
       }
+    host_s[2] += now() - ph;
+    ph = now();
     // collect finished solves
     for (int k = 0; k < 2 and rc == G5_OK; ++k)
     {
@@ -386,8 +395,15 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
         const int             nvar = s.dim == 2 ? 8 : 9;
         const size_t          cells = static_cast<size_t>(s.nb[0]) * s.nb[1] * (s.dim == 3 ? s.nb[2] : 1) *
                              (s.dim == 2 ? 256 : 4096);
-        host_result.resize(cells * nvar);
-        rc = g5_pool_download(pools[k], it->id, host_result.data(), host_result.size());
+        if (cells * nvar > host_cap)
+        {
+          g5_host_free(host_result);
+          host_cap = cells * nvar;
+          rc = g5_host_alloc(host_cap * sizeof(double), reinterpret_cast<void **>(&host_result));
+          if (rc != G5_OK)
+            break;
+        }
+        rc = g5_pool_download(pools[k], it->id, host_result, cells * nvar);
         if (rc != G5_OK)
           break;
         lat[it->req] = (now() - s.arrival) * 1e3;
@@ -397,8 +413,10 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
         ++done;
       }
     }
+    host_s[3] += now() - ph;
   }
   const double wall = now();
+  g5_host_free(host_result);
   for (int k = 0; k < 2; ++k)
     g5_pool_destroy(pools[k]);
   if (rc != G5_OK)
@@ -424,6 +442,10 @@ g5_serve(const g5_serve_config * c, g5_serve_report * r, double * latencies_ms, 
   r->last_arrival_s = specs.back().arrival;
   r->step_s_2d = step_ms[0] * 1e-3;
   r->step_s_3d = step_ms[1] * 1e-3;
+  r->admit_s = host_s[0];
+  r->start_s = host_s[1];
+  r->wait_s = host_s[2];
+  r->collect_s = host_s[3];
   if (latencies_ms)
     for (size_t k = 0; k < std::min(cap, lat.size()); ++k)
       latencies_ms[k] = lat[k];

@@ -268,6 +268,12 @@ typedef struct g5_launch
   double  tuned_ms;  /* out: best step-kernel time measured by the autotuner (ms) */
 } g5_launch;
 
+/* Pinned (page-locked) host memory for state transfers: g5_upload_state / g5_download_state /
+ * g5_pool_upload / g5_pool_download run at full PCIe rate from and to such buffers.  No reference
+ * counterpart (Kokkos deep_copy to host mirrors, SolverGodunovFiveEq.cpp:811-846). */
+int g5_host_alloc(size_t bytes, void ** ptr);
+int g5_host_free(void * ptr);
+
 /* build the launch list of the live solves, tune its launch geometry and capture the step
  * graph now, so that the next g5_pool_step does none of it (no state changes) */
 int g5_pool_prepare(g5_pool * pool);
@@ -360,6 +366,9 @@ typedef struct g5_serve_report
   double  p50_ms, p99_ms, mean_ms, max_ms;
   double  last_arrival_s;
   double  step_s_2d, step_s_3d; /* HIP-event time of the stepping calls of each pool */
+  /* host wall time in the loop's phases: admission (incl. device initial conditions), first dt,
+   * waiting for the enqueued steps, collection of finished solves (download and release) */
+  double  admit_s, start_s, wait_s, collect_s;
 } g5_serve_report;
 
 /* latencies_ms (optional, cap entries) receives the latency of each request in arrival order */
