@@ -9,8 +9,9 @@ Workloads (--workload):
                 the per-field relative L1 error against the FP64 run of the same case.
   batch         4096 independent 2-D solves of the seeded batching workload (g5_batch_spec),
                 mixed precision, one pool, 100 steps each.
-  serve         the native serving loop (g5_serve): Poisson arrivals, 2-D and 10% 3-D requests,
-                HBM budget 200 GB, mixed precision; reports p50/p99 latency.
+  serve         the native serving loop (g5_serve): 400 requests with Poisson arrivals at 160/s
+                (about 80% of the loop's capacity), 2-D and 10% 3-D requests, HBM budget 200 GB,
+                mixed precision; reports p50/p99 latency.
 
 A step is one godunov_unsplit iteration over every solve (device-resident dt, fused step kernel
 + dt finalise, one HIP graph).  value = owned cells * steps / wall time of the timed loop, state
@@ -274,8 +275,8 @@ def main():
     ap.add_argument("--fp64", action="store_true", help="batch/serve: FP64 instead of mixed precision")
     ap.add_argument("--nsolves", type=int, default=4096)
     ap.add_argument("--requests", type=int, default=400)
-    ap.add_argument("--rate", type=float, default=100.0)
-    ap.add_argument("--poll", type=int, default=4)
+    ap.add_argument("--rate", type=float, default=160.0)
+    ap.add_argument("--poll", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; runs on several GPUs are out of scope "
