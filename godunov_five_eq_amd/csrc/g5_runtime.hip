@@ -112,9 +112,12 @@ struct g5_pool
   SolveCtl *  h_ctl = nullptr; // pinned mirror of the control records
   int32_t     cap_hctl = 0;
 
-  // HIP graph of one device-dt step over the current launch list
+  // HIP graphs of one and of kGraphSteps device-dt steps over the current launch list
   hipGraphExec_t graph = nullptr;
-  bool           graph_valid = false;
+  hipGraphExec_t graph_k = nullptr;
+  bool           graph_valid = false;   // both graphs are stale when false
+  bool           graph1_ok = false;     // captured for the current list
+  bool           graphk_ok = false;
 
   // staging of g5_pool_upload / g5_pool_download for solves whose slots are not one run
   double * d_xfer = nullptr;
@@ -585,40 +588,81 @@ pool_hancock(g5_pool const * p)
   return true;
 }
 
-// (re)capture the HIP graph of one device-dt step over the current launch list
+// steps per launch of the multi-step graph (fewer host launches for small, latency-bound lists)
+constexpr int kGraphSteps = 8;
+
+// capture nsteps device-dt steps (step kernel + finalise each) of the current launch list
 int
-pool_capture(g5_pool * p)
+capture_steps(g5_pool * p, int nsteps, hipGraphExec_t * out)
 {
-  if (p->nlist == 0 or p->graph_valid)
-    return G5_OK;
-  if (p->graph)
+  if (*out)
   {
-    (void)hipGraphExecDestroy(p->graph);
-    p->graph = nullptr;
+    (void)hipGraphExecDestroy(*out);
+    *out = nullptr;
   }
   hipGraph_t g;
   G5_HIP(p->err, hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-  launch_step(p, step_args(p, p->d_blocks, p->nlist, 0.0), pool_hancock(p));
-  launch_finalize(p->d_ctl, p->d_solves, static_cast<int>(p->solves.size()), p->stream);
+  const StepArgs A = step_args(p, p->d_blocks, p->nlist, 0.0);
+  const bool     hancock = pool_hancock(p);
+  for (int k = 0; k < nsteps; ++k)
+  {
+    launch_step(p, A, hancock);
+    launch_finalize(p->d_ctl, p->d_solves, static_cast<int>(p->solves.size()), p->stream);
+  }
   G5_HIP(p->err, hipStreamEndCapture(p->stream, &g));
-  G5_HIP(p->err, hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0));
+  G5_HIP(p->err, hipGraphInstantiate(out, g, nullptr, nullptr, 0));
   (void)hipGraphDestroy(g);
-  p->graph_valid = true;
+  return G5_OK;
+}
+
+// (re)capture, when stale, the step graph of 1 or of kGraphSteps steps over the current list
+int
+pool_capture(g5_pool * p, bool multi)
+{
+  if (p->nlist == 0)
+    return G5_OK;
+  if (!p->graph_valid)
+  {
+    p->graph1_ok = p->graphk_ok = false;
+    p->graph_valid = true;
+  }
+  bool & ok = multi ? p->graphk_ok : p->graph1_ok;
+  if (ok)
+    return G5_OK;
+  const int rc = capture_steps(p, multi ? kGraphSteps : 1, multi ? &p->graph_k : &p->graph);
+  if (rc != G5_OK)
+  {
+    p->graph_valid = false;
+    return rc;
+  }
+  ok = true;
   return G5_OK;
 }
 
 int pool_timing(g5_pool * p);
 
-// one device-dt step of every active solve: step kernel + finalize, as a HIP graph
+// n device-dt steps of every active solve, as launches of the captured step graphs
 int
-pool_step_once(g5_pool * p)
+pool_launch_steps(g5_pool * p, int32_t n)
 {
-  if (p->nlist == 0)
+  if (p->nlist == 0 or n <= 0)
     return G5_OK;
-  const int rc = pool_capture(p);
-  if (rc != G5_OK)
-    return rc;
-  G5_HIP(p->err, hipGraphLaunch(p->graph, p->stream));
+  if (n >= kGraphSteps)
+  {
+    const int rc = pool_capture(p, true);
+    if (rc != G5_OK)
+      return rc;
+  }
+  if (n % kGraphSteps != 0)
+  {
+    const int rc = pool_capture(p, false);
+    if (rc != G5_OK)
+      return rc;
+  }
+  for (; n >= kGraphSteps; n -= kGraphSteps)
+    G5_HIP(p->err, hipGraphLaunch(p->graph_k, p->stream));
+  for (; n > 0; --n)
+    G5_HIP(p->err, hipGraphLaunch(p->graph, p->stream));
   return G5_OK;
 }
 
@@ -637,12 +681,9 @@ pool_step(g5_pool * p, int32_t nsteps, bool sync)
     G5_HIP(p->err, hipEventCreate(&p->ev[1]));
   }
   G5_HIP(p->err, hipEventRecord(p->ev[0], p->stream));
-  for (int32_t n = 0; n < nsteps; ++n)
-  {
-    rc = pool_step_once(p);
-    if (rc != G5_OK)
-      return rc;
-  }
+  rc = pool_launch_steps(p, nsteps);
+  if (rc != G5_OK)
+    return rc;
   G5_HIP(p->err, hipEventRecord(p->ev[1], p->stream));
   G5_HIP(p->err, hipGetLastError());
   p->timing_pending = true;
@@ -872,6 +913,8 @@ g5_pool_destroy(g5_pool * p)
     hipStreamSynchronize(p->stream);
   if (p->graph)
     (void)hipGraphExecDestroy(p->graph);
+  if (p->graph_k)
+    (void)hipGraphExecDestroy(p->graph_k);
   for (hipEvent_t e : p->ev)
     if (e)
       hipEventDestroy(e);
@@ -1161,7 +1204,9 @@ g5_pool_prepare(g5_pool * p)
   if (rc == G5_OK)
     rc = ensure_tuned(p, pool_hancock(p));
   if (rc == G5_OK)
-    rc = pool_capture(p);
+    rc = pool_capture(p, true);
+  if (rc == G5_OK)
+    rc = pool_capture(p, false);
   return rc;
 }
 

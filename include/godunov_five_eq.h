@@ -223,7 +223,8 @@ int g5_pool_init_triple_point(g5_pool * pool, int32_t solve_id, const g5_region 
 /* compute the first dt of solves whose state is set and mark them active */
 int g5_pool_start(g5_pool * pool, const int32_t * solve_ids, int32_t n);
 
-/* nsteps device-dt steps of every active solve (one HIP graph per step: fused step + finalise) */
+/* nsteps device-dt steps of every active solve (HIP graphs of 8 steps, then of single steps:
+ * fused step kernel + finalise per step) */
 int g5_pool_step(g5_pool * pool, int32_t nsteps);
 
 /* time, steps done and activity of one solve (synchronises) */
