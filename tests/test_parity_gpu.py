@@ -178,3 +178,19 @@ def test_run_stops_at_tend(g5, orc):
         U_g = ctx.download()
     assert n_g == n_o and t_g == t_o
     check_parity(U_g, o.get_state(), 2, "tEnd clamp")
+
+
+def test_parity_rk1_without_hancock(g5, orc):
+    # integrator RK1: MUSCL reconstruction without the Hancock half step, 2-D MC THINC and 3-D
+    cfg, U0 = cases.air_helium_2d(nbx=6, nby=4, integrator="RK1", slope_type=2.0, thinc=True, thinc_beta=1.6)
+    U_g, U_o = run_both(cfg, U0, 15, g5, orc, "2d RK1 MC THINC")
+    assert check_parity(U_g, U_o, 2, "2d RK1 MC THINC 15 steps")
+    cfg, U0 = cases.air_water_3d(nb=2, integrator="RK1")
+    U_g, U_o = run_both(cfg, U0, 6, g5, orc, "3d RK1 THINC")
+    assert check_parity(U_g, U_o, 3, "3d RK1 THINC 6 steps")
+
+
+def test_parity_3d_mc_slopes_periodic(g5, orc):
+    cfg, U0 = cases.air_helium_3d(nb=2, slope_type=2.0, periodic=(1, 1, 1), bc=("PERIODIC",) * 6)
+    U_g, U_o = run_both(cfg, U0, 8, g5, orc, "3d MC periodic")
+    assert check_parity(U_g, U_o, 3, "3d MC periodic 8 steps")
