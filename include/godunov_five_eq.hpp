@@ -24,6 +24,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "godunov_five_eq.h"
@@ -433,6 +434,110 @@ private:
   double                 m_t = 0.0;
   int                    m_iteration = 0;
   uint64_t               m_cell_updates = 0;
+};
+
+// Many independent solves stepped together (the batching and serving forms): RAII over g5_pool.
+class PoolHip
+{
+public:
+  PoolHip(int dim, bool thinc, int precision, uint64_t hbm_budget_bytes)
+  {
+    g5_pool * p = nullptr;
+    const int rc = g5_pool_create(dim, thinc ? 1 : 0, precision, hbm_budget_bytes, &p);
+    if (rc != G5_OK)
+      throw std::runtime_error(std::string("g5_pool_create: ") + g5_pool_last_error(nullptr));
+    m_pool.reset(p);
+  }
+
+  // admit a solve with its configuration; -1 when the HBM budget has no room now
+  int32_t
+  admit(g5_config const & cfg, double t_end, int32_t max_steps)
+  {
+    int32_t   id = -1;
+    const int rc = g5_pool_admit(m_pool.get(), &cfg, t_end, max_steps, &id);
+    if (rc == G5_ERR_NOMEM)
+      return -1;
+    pcheck(rc);
+    return id;
+  }
+
+  // admit a seeded workload solve, its initial condition generated on the device
+  int32_t
+  admit(g5_solve_spec const & spec)
+  {
+    int32_t   id = -1;
+    const int rc = g5_pool_admit_spec(m_pool.get(), &spec, &id);
+    if (rc == G5_ERR_NOMEM)
+      return -1;
+    pcheck(rc);
+    return id;
+  }
+
+  void
+  upload(int32_t id, std::vector<double> const & U)
+  {
+    pcheck(g5_pool_upload(m_pool.get(), id, U.data(), U.size()));
+  }
+  void
+  download(int32_t id, std::vector<double> & U)
+  {
+    pcheck(g5_pool_download(m_pool.get(), id, U.data(), U.size()));
+  }
+  void
+  start(std::vector<int32_t> const & ids)
+  {
+    pcheck(g5_pool_start(m_pool.get(), ids.data(), static_cast<int32_t>(ids.size())));
+  }
+  void
+  step(int32_t nsteps)
+  {
+    pcheck(g5_pool_step(m_pool.get(), nsteps));
+  }
+  std::vector<int32_t>
+  finished()
+  {
+    int32_t n = 0;
+    pcheck(g5_pool_finished(m_pool.get(), nullptr, 0, &n));
+    std::vector<int32_t> ids(static_cast<size_t>(n));
+    pcheck(g5_pool_finished(m_pool.get(), ids.data(), n, &n));
+    return ids;
+  }
+  void
+  release(int32_t id)
+  {
+    pcheck(g5_pool_release(m_pool.get(), id));
+  }
+  // (t, steps done, active)
+  std::tuple<double, int32_t, bool>
+  status(int32_t id)
+  {
+    double  t = 0.0;
+    int32_t n = 0, a = 0;
+    pcheck(g5_pool_status(m_pool.get(), id, &t, &n, &a));
+    return { t, n, a != 0 };
+  }
+  g5_pool *
+  get() const
+  {
+    return m_pool.get();
+  }
+
+private:
+  void
+  pcheck(int rc)
+  {
+    if (rc != G5_OK)
+      throw std::runtime_error(std::string("g5_pool: ") + g5_pool_last_error(m_pool.get()));
+  }
+  struct Del
+  {
+    void
+    operator()(g5_pool * p) const
+    {
+      g5_pool_destroy(p);
+    }
+  };
+  std::unique_ptr<g5_pool, Del> m_pool;
 };
 
 } // namespace g5cpp

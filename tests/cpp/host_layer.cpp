@@ -31,9 +31,39 @@ run(g5cpp::ConfigMap const & cm, int nsteps)
   return 0;
 }
 
+// a few solves of the seeded batch in one pool through the C++ PoolHip class
+int
+run_pool(int nsolves)
+{
+  g5cpp::PoolHip       pool(2, false, G5_PREC_MIXED, 1ull << 30);
+  std::vector<int32_t> ids;
+  for (int k = 0; k < nsolves; ++k)
+  {
+    g5_solve_spec s;
+    g5cpp::check(g5_batch_spec(20251203, k, &s), nullptr);
+    s.nb[0] = 6 + k % 3;
+    s.nb[1] = 4;
+    s.steps = 20 + k;
+    ids.push_back(pool.admit(s));
+  }
+  pool.start(ids);
+  pool.step(20 + nsolves);
+  const auto fin = pool.finished();
+  int        ok = 0;
+  for (int k = 0; k < nsolves; ++k)
+  {
+    const auto st = pool.status(ids[k]);
+    ok += std::get<1>(st) == 20 + k and !std::get<2>(st);
+  }
+  std::printf("pool solves=%d finished=%zu complete=%d\n", nsolves, fin.size(), ok);
+  return ok == nsolves and static_cast<int>(fin.size()) == nsolves ? 0 : 1;
+}
+
 int
 main(int argc, char ** argv)
 {
+  if (argc >= 3 and std::strcmp(argv[1], "--pool") == 0)
+    return run_pool(std::atoi(argv[2]));
   if (argc < 2)
     return 2;
   const auto cm = g5cpp::ConfigMap::from_file(argv[1]);

@@ -59,3 +59,13 @@ def test_cpp_solver_loop_runs(exe):
     kv = parse(out.stdout.splitlines()[-1])
     assert kv["steps"] == "200"
     assert float(kv["Mcell-updates/s"]) > 0
+
+
+@pytest.mark.gpu
+def test_cpp_pool_class_runs(exe):
+    print("[gpu] C++ PoolHip over seeded batch solves", flush=True)
+    out = subprocess.run([exe, "--pool", "6"], capture_output=True, text=True, timeout=120)
+    sys.stdout.write(out.stdout + out.stderr)
+    assert out.returncode == 0
+    kv = parse(out.stdout.splitlines()[-1])
+    assert kv["complete"] == "6" and kv["finished"] == "6"
