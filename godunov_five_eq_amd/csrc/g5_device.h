@@ -328,14 +328,16 @@ hancock_terms(SolveDesc const & S, PrimT<T> const & q, SlopeT<T> const & sx, Slo
 }
 
 // q + off_x*sx + off_y*sy (+ off_z*sz), one offset +-0.5 on axis AX, the others 0.0 (kept so that
-// the operation sequence is the reference's).
+// the operation sequence is the reference's).  AX = -1: the axis is the run-time value ax (one
+// code path for lanes that reconstruct faces on different axes; the same operations per lane).
 template <int DIM, int AX, typename T>
 G5D T
-muscl_extrap(T qv, T sxv, T syv, T szv, T o)
+muscl_extrap(T qv, T sxv, T syv, T szv, T o, int ax = AX)
 {
-  const T ox = AX == 0 ? o : T(0.0);
-  const T oy = AX == 1 ? o : T(0.0);
-  const T oz = AX == 2 ? o : T(0.0);
+  const int a = AX >= 0 ? AX : ax;
+  const T   ox = a == 0 ? o : T(0.0);
+  const T   oy = a == 1 ? o : T(0.0);
+  const T   oz = a == 2 ? o : T(0.0);
   if (DIM == 2)
     return qv + ox * sxv + oy * syv;
   return qv + ox * sxv + oy * syv + oz * szv;
@@ -345,15 +347,15 @@ muscl_extrap(T qv, T sxv, T syv, T szv, T o)
 template <int DIM, int AX, bool HANCOCK, typename T>
 G5D FaceT<T>
 face_muscl(SolveDesc const & S, PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy,
-           SlopeT<T> const & sz, HancockT<T> const & h, T o)
+           SlopeT<T> const & sz, HancockT<T> const & h, T o, int ax = AX)
 {
   FaceT<T> f;
-  f.r0 = muscl_extrap<DIM, AX>(q.r0, sx.r0, sy.r0, sz.r0, o);
-  f.r1 = muscl_extrap<DIM, AX>(q.r1, sx.r1, sy.r1, sz.r1, o);
-  f.p = muscl_extrap<DIM, AX>(q.p, sx.p, sy.p, sz.p, o);
-  f.u = muscl_extrap<DIM, AX>(q.u, sx.u, sy.u, sz.u, o);
-  f.v = muscl_extrap<DIM, AX>(q.v, sx.v, sy.v, sz.v, o);
-  f.w = DIM == 3 ? muscl_extrap<DIM, AX>(q.w, sx.w, sy.w, sz.w, o) : T(0.0);
+  f.r0 = muscl_extrap<DIM, AX>(q.r0, sx.r0, sy.r0, sz.r0, o, ax);
+  f.r1 = muscl_extrap<DIM, AX>(q.r1, sx.r1, sy.r1, sz.r1, o, ax);
+  f.p = muscl_extrap<DIM, AX>(q.p, sx.p, sy.p, sz.p, o, ax);
+  f.u = muscl_extrap<DIM, AX>(q.u, sx.u, sy.u, sz.u, o, ax);
+  f.v = muscl_extrap<DIM, AX>(q.v, sx.v, sy.v, sz.v, o, ax);
+  f.w = DIM == 3 ? muscl_extrap<DIM, AX>(q.w, sx.w, sy.w, sz.w, o, ax) : T(0.0);
   if (HANCOCK)
   {
     f.r0 += h.r0;
@@ -535,7 +537,7 @@ thinc_axis(SolveDesc const & S, T qa, T am, T ac, T ap, T nd)
 template <int DIM, int AX, bool HANCOCK, typename T>
 G5D FaceT<T>
 face_thinc(PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy, SlopeT<T> const & sz,
-           HancockT<T> const & h, ThincAxisT<T> const & th, int ib)
+           HancockT<T> const & h, ThincAxisT<T> const & th, int ib, int ax = AX)
 {
   FaceT<T> f;
   const T  o = T(0.5) * static_cast<T>(ib); // HALF_F * offsets[AX]
@@ -549,13 +551,13 @@ face_thinc(PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy, Slope
   else
   {
     f.a = q.a;
-    f.r0 = muscl_extrap<DIM, AX>(q.r0, sx.r0, sy.r0, sz.r0, o);
-    f.r1 = muscl_extrap<DIM, AX>(q.r1, sx.r1, sy.r1, sz.r1, o);
+    f.r0 = muscl_extrap<DIM, AX>(q.r0, sx.r0, sy.r0, sz.r0, o, ax);
+    f.r1 = muscl_extrap<DIM, AX>(q.r1, sx.r1, sy.r1, sz.r1, o, ax);
   }
-  f.p = muscl_extrap<DIM, AX>(q.p, sx.p, sy.p, sz.p, o);
-  f.u = muscl_extrap<DIM, AX>(q.u, sx.u, sy.u, sz.u, o);
-  f.v = muscl_extrap<DIM, AX>(q.v, sx.v, sy.v, sz.v, o);
-  f.w = DIM == 3 ? muscl_extrap<DIM, AX>(q.w, sx.w, sy.w, sz.w, o) : T(0.0);
+  f.p = muscl_extrap<DIM, AX>(q.p, sx.p, sy.p, sz.p, o, ax);
+  f.u = muscl_extrap<DIM, AX>(q.u, sx.u, sy.u, sz.u, o, ax);
+  f.v = muscl_extrap<DIM, AX>(q.v, sx.v, sy.v, sz.v, o, ax);
+  f.w = DIM == 3 ? muscl_extrap<DIM, AX>(q.w, sx.w, sy.w, sz.w, o, ax) : T(0.0);
   if (HANCOCK)
   {
     f.r0 += h.r0;

@@ -216,9 +216,8 @@ predict_faces2(SolveDesc const & S, T const * sQ, int gi, int gj, T dtdx, bool a
     }
     else
     {
-      const T o = T(0.5) * static_cast<T>(ib1);
-      f1 = ax1 == 0 ? face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, o)
-                    : face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, o);
+      // one reconstruction on the lane's axis ax1
+      f1 = face_muscl<2, -1, HANCOCK>(S, qc, sx, sy, sz, h, T(0.5) * static_cast<T>(ib1), ax1);
     }
     return;
   }
@@ -245,16 +244,9 @@ predict_faces2(SolveDesc const & S, T const * sQ, int gi, int gj, T dtdx, bool a
   }
   else
   {
-    if (ax1 == 0)
-    {
-      const ThincAxisT<T> tx = thinc_axis(S, qc.a, qxm.a, qc.a, qxp.a, fabs(nx));
-      f1 = face_thinc<2, 0, HANCOCK>(qc, sx, sy, sz, h, tx, ib1);
-    }
-    else
-    {
-      const ThincAxisT<T> ty = thinc_axis(S, qc.a, qym.a, qc.a, qyp.a, fabs(ny));
-      f1 = face_thinc<2, 1, HANCOCK>(qc, sx, sy, sz, h, ty, ib1);
-    }
+    const bool          x = ax1 == 0;
+    const ThincAxisT<T> t1 = thinc_axis(S, qc.a, x ? qxm.a : qym.a, qc.a, x ? qxp.a : qyp.a, fabs(x ? nx : ny));
+    f1 = face_thinc<2, -1, HANCOCK>(qc, sx, sy, sz, h, t1, ib1, ax1);
   }
 }
 

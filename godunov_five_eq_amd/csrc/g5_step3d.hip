@@ -15,12 +15,13 @@
 // (t%16, t/16).  The register demand of the FP64 z-march allows one wavefront per SIMD, so the
 // FP64 form uses up to 139 KB of LDS; the mixed form (T = float primitives, face states and
 // fluxes, FP64 energy flux and state) halves it.  Extra work is spread so that every wavefront
-// has one extra item per plane: wave 0 converts the 64 ring cells, wave 1 computes their face
-// states, wave 2 solves the 32 ring Riemann problems and converts 16 outer cells, wave 3
-// converts 64 outer cells.  The raw state of the owned column is carried from its conversion to
-// its update, and the global loads of plane P+2 are issued one phase ahead of their use.  With
-// ZSPLIT the z Riemann solve and the completion of plane P-1 run before the x/y exchange on
-// waves 0, 2, 3 and after it on wave 1, which carries the ring predictor.
+// has one extra item per plane: wave 0 converts the 64 ring cells and solves the 32 ring Riemann
+// problems, wave 1 computes the ring cells' face states, wave 2 converts 16 outer cells, wave 3
+// converts 64 outer cells (lanes on different axes share one run-time-axis code path).  The raw
+// state of the owned column is carried from its conversion to its update, and the global loads
+// of plane P+2 are issued one phase ahead of their use.  The z Riemann solve runs before the x/y
+// exchange (with ZSPLIT on waves 0, 2, 3 only: wave 1, which carries the ring predictor, solves
+// after it); the completion of plane P-1 follows the last barrier of the plane on every wave.
 #include "g5_device.h"
 #include "g5_kernels.h"
 
@@ -120,6 +121,18 @@ st_store3(T * A, int k, FaceT<T> const & f)
   A[4 * N + k] = f.u;
   A[5 * N + k] = f.v;
   A[6 * N + k] = f.w;
+}
+template <typename T>
+G5D void
+st_store3n(T * A, int n, FaceT<T> const & f) // row stride n
+{
+  A[0 * n] = f.r0;
+  A[1 * n] = f.r1;
+  A[2 * n] = f.a;
+  A[3 * n] = f.p;
+  A[4 * n] = f.u;
+  A[5 * n] = f.v;
+  A[6 * n] = f.w;
 }
 template <int N = NF3, typename T>
 G5D FaceT<T>
@@ -271,17 +284,19 @@ struct Pred3
       youngs_ring(sQ, P, gi, gj, nx, ny, nz);
   }
 
+  // face state on axis AX (AX = -1: on the run-time axis ax), ib = -1 left face, +1 right face
   template <int AX>
   G5D FaceT<T>
-  face(SolveDesc const & S, int ib) const
+  face(SolveDesc const & S, int ib, int ax = AX) const
   {
     if (!THINC)
-      return face_muscl<3, AX, HANCOCK>(S, q, sx, sy, sz, h, T(0.5) * static_cast<T>(ib));
-    const T             am = AX == 0 ? axm : (AX == 1 ? aym : azm);
-    const T             ap = AX == 0 ? axp : (AX == 1 ? ayp : azp);
-    const T             nd = AX == 0 ? nx : (AX == 1 ? ny : nz);
+      return face_muscl<3, AX, HANCOCK>(S, q, sx, sy, sz, h, T(0.5) * static_cast<T>(ib), ax);
+    const int           a = AX >= 0 ? AX : ax;
+    const T             am = a == 0 ? axm : (a == 1 ? aym : azm);
+    const T             ap = a == 0 ? axp : (a == 1 ? ayp : azp);
+    const T             nd = a == 0 ? nx : (a == 1 ? ny : nz);
     const ThincAxisT<T> ta = thinc_axis(S, q.a, am, q.a, ap, fabs(nd));
-    return face_thinc<3, AX, HANCOCK>(q, sx, sy, sz, h, ta, ib);
+    return face_thinc<3, AX, HANCOCK>(q, sx, sy, sz, h, ta, ib, ax);
   }
 };
 } // namespace
@@ -371,17 +386,25 @@ step3d_body(StepArgs const & A, int blk)
       pf_x = fetch(xgi, xgj, 0);
   }
 
-  auto z_solve = [&](int P, FaceT<T> const & fzm) {
-    // z face P between planes P-1 and P, then plane P-1 is complete
-    const FluxT<T> Fz = riemann<3>(S, zL, fzm, dtdx, 2);
+  // z face P (between planes P-1 and P): the Riemann solve, then the completion of plane P-1 --
+  // in FP64 after the plane's last barrier (the solve's flux stays in registers meanwhile), in the
+  // mixed form, whose registers are capped at 256, right after the solve
+  constexpr bool LATE = sizeof(T) == sizeof(double);
+  FluxT<T>       zF; // flux of z face P, from its solve to the completion
+  auto           z_complete = [&](int P) {
     if (P >= 1)
     {
-      apply_face_pair<3>(u2prev, zFprev, Fz, uaprev0, uaprev1, dtdx, 2);
+      apply_face_pair<3>(u2prev, zFprev, zF, uaprev0, uaprev1, dtdx, 2);
       fix_volume_fractions(u2prev);
       store_cons<3, B3>(U2, blk_off + i + 16 * j + 256 * (P - 1), u2prev);
       inv_max = fmax(inv_max, cell_rate<3, T>(S, u2prev));
     }
-    zFprev = Fz;
+    zFprev = zF;
+  };
+  auto z_solve = [&](int P, FaceT<T> const & fzm) {
+    zF = riemann<3>(S, zL, fzm, dtdx, 2);
+    if (!LATE)
+      z_complete(P);
   };
 
   for (int P = -1; P <= 16; ++P)
@@ -423,16 +446,12 @@ step3d_body(StepArgs const & A, int blk)
     }
     if (wave == 1 and inner_plane)
     {
+      // ring cell of side rs: its face on the block boundary, one reconstruction for all lanes
       Pred3<THINC, HANCOCK, T> pr;
       pr.compute(S, sQ, P, rgi, rgj, dtdxT);
-      if (rs == 0)
-        st_store3(sXL, fx3(0, rr), pr.template face<0>(S, 1));
-      else if (rs == 1)
-        st_store3<16>(sXR, rr, pr.template face<0>(S, -1));
-      else if (rs == 2)
-        st_store3(sYL, fy3(rr, 0), pr.template face<1>(S, 1));
-      else
-        st_store3<16>(sYR, rr, pr.template face<1>(S, -1));
+      const FaceT<T> f = pr.template face<-1>(S, (rs & 1) ? -1 : 1, rs >> 1);
+      T *            dst = rs == 0 ? sXL + fx3(0, rr) : (rs == 1 ? sXR + rr : (rs == 2 ? sYL + fy3(rr, 0) : sYR + rr));
+      st_store3n(dst, (rs & 1) ? 16 : NF3, f);
     }
     __syncthreads(); // (d)
 
@@ -441,14 +460,14 @@ step3d_body(StepArgs const & A, int blk)
     {
       fl_store3(sFX, sFXE, fx3(i, j), riemann<3>(S, st_load3(sXL, fx3(i, j)), fxm, dtdx, 0));
       fl_store3(sFY, sFYE, fy3(i, j), riemann<3>(S, st_load3(sYL, fy3(i, j)), fym, dtdx, 1));
-      if (wave == 2 and (rs & 1))
+      if (wave == 0 and (rs & 1))
       {
-        if (rs == 1)
-          fl_store3(sFX, sFXE, fx3(16, rr),
-                    riemann<3>(S, st_load3(sXL, fx3(16, rr)), st_load3<16>(sXR, rr), dtdx, 0));
-        else
-          fl_store3(sFY, sFYE, fy3(rr, 16),
-                    riemann<3>(S, st_load3(sYL, fy3(rr, 16)), st_load3<16>(sYR, rr), dtdx, 1));
+        // the last x face (rs 1) or y face (rs 3) of row/column rr: one solve on the run-time axis
+        const int ax = rs >> 1;
+        const int k = ax == 0 ? fx3(16, rr) : fy3(rr, 16);
+        const FaceT<T> L = st_load3(ax == 0 ? sXL : sYL, k);
+        const FaceT<T> R = st_load3<16>(ax == 0 ? sXR : sYR, rr);
+        fl_store3(ax == 0 ? sFX : sFY, ax == 0 ? sFXE : sFYE, k, riemann<3>(S, L, R, dtdx, ax));
       }
     }
     if (!z_early)
@@ -461,7 +480,10 @@ step3d_body(StepArgs const & A, int blk)
     }
     __syncthreads(); // (f)
 
-    // (i) x then y update of plane P
+    // (g) completion of plane P-1 (z update, normalisation, store, CFL), then (i) the x and y
+    // update of plane P
+    if (LATE and P >= 0)
+      z_complete(P);
     if (inner_plane)
     {
       u2prev = uraw_cur;
