@@ -1015,12 +1015,12 @@ block_range(StepArgs const & A, int & b, int & end, int & stride)
 // applies defines the copy (ZERO_GRADIENT: edge cell; WALL: mirrored cell, normal momentum
 // negated).
 // ---------------------------------------------------------------------------------------------
+// I: the inside block (N itself when N is inside the domain); cc: the cell of I to read.
 template <int DIM, int B>
-G5D int64_t
-resolve_block_cell(SolveDesc const & S, int32_t const * __restrict__ slots, int const (&N)[3], int const (&c)[3],
-                   int & neg)
+G5D void
+remap_block_cell(SolveDesc const & S, int const (&N)[3], int const (&c)[3], int (&I)[3], int (&cc)[3], int & neg)
 {
-  int  I[3], n[3];
+  int  n[3];
   bool outside = false;
   neg = -1;
 #pragma unroll
@@ -1030,7 +1030,9 @@ resolve_block_cell(SolveDesc const & S, int32_t const * __restrict__ slots, int 
     outside = outside or (n[d] != 0);
     I[d] = N[d] - n[d];
   }
-  int cc[3] = { c[0], c[1], c[2] };
+  cc[0] = c[0];
+  cc[1] = c[1];
+  cc[2] = c[2];
   if (outside)
   {
 #pragma unroll
@@ -1060,39 +1062,112 @@ resolve_block_cell(SolveDesc const & S, int32_t const * __restrict__ slots, int 
       }
     }
   }
-  const int             blk = slots[S.base + I[0] + S.nb[0] * (I[1] + S.nb[1] * I[2])];
-  const int             cell = cc[0] + B * (cc[1] + (DIM == 3 ? B * cc[2] : 0));
-  constexpr int64_t     NV = DIM == 2 ? 8 : 9;
-  constexpr int64_t     NC = DIM == 2 ? B * B : B * B * B;
+}
+
+template <int DIM, int B>
+G5D int64_t
+cell_offset(int blk, int const (&cc)[3])
+{
+  const int         cell = cc[0] + B * (cc[1] + (DIM == 3 ? B * cc[2] : 0));
+  constexpr int64_t NV = DIM == 2 ? 8 : 9;
+  constexpr int64_t NC = DIM == 2 ? B * B : B * B * B;
   return static_cast<int64_t>(blk) * NV * NC + cell;
 }
 
-// the cell at tile coordinates g (in [-2, B+1]^d) of owned block (bx,by,bz)
 template <int DIM, int B>
 G5D int64_t
-resolve_cell(SolveDesc const & S, int32_t const * __restrict__ slots, int bx, int by, int bz, int gi, int gj, int gk,
-             int & neg)
+resolve_block_cell(SolveDesc const & S, int32_t const * __restrict__ slots, int const (&N)[3], int const (&c)[3],
+                   int & neg)
 {
-  const int g[3] = { gi, gj, gk };
-  const int b3[3] = { bx, by, bz };
-  int       c[3], N[3];
+  int I[3], cc[3];
+  remap_block_cell<DIM, B>(S, N, c, I, cc, neg);
+  return cell_offset<DIM, B>(slots[S.base + I[0] + S.nb[0] * (I[1] + S.nb[1] * I[2])], cc);
+}
+
+// the cell at tile coordinates g (in [-2, B+1]^d) of owned block (bx,by,bz)
+// neighbour block N (periodic wrap applied) of block b in direction dir
+template <int DIM>
+G5D void
+neighbour_block(SolveDesc const & S, int const (&b3)[3], int const (&dir)[3], int (&N)[3])
+{
 #pragma unroll
   for (int d = 0; d < 3; ++d)
   {
     if (d >= DIM)
     {
-      c[d] = 0;
       N[d] = 0;
       continue;
     }
-    const int dir = g[d] < 0 ? -1 : (g[d] >= B ? 1 : 0);
-    c[d] = g[d] - dir * B;
-    int Nd = b3[d] + dir;
+    int Nd = b3[d] + dir[d];
     if (S.periodic[d])
       Nd = Nd < 0 ? Nd + S.nb[d] : (Nd >= S.nb[d] ? Nd - S.nb[d] : Nd);
     N[d] = Nd;
   }
+}
+
+template <int DIM, int B>
+G5D void
+tile_cell_dir(int gi, int gj, int gk, int (&dir)[3], int (&c)[3])
+{
+  const int g[3] = { gi, gj, gk };
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    dir[d] = d < DIM ? (g[d] < 0 ? -1 : (g[d] >= B ? 1 : 0)) : 0;
+    c[d] = d < DIM ? g[d] - dir[d] * B : 0;
+  }
+}
+
+template <int DIM, int B>
+G5D int64_t
+resolve_cell(SolveDesc const & S, int32_t const * __restrict__ slots, int bx, int by, int bz, int gi, int gj, int gk,
+             int & neg)
+{
+  const int b3[3] = { bx, by, bz };
+  int       dir[3], c[3], N[3];
+  tile_cell_dir<DIM, B>(gi, gj, gk, dir, c);
+  neighbour_block<DIM>(S, b3, dir, N);
   return resolve_block_cell<DIM, B>(S, slots, N, c, neg);
+}
+
+// The slots of the 3^DIM blocks a tile reads, resolved once per block into the table
+// tab[(dx+1) + 3(dy+1) + 9(dz+1)] (entry k by thread k, k < 3^DIM): the inside block of the
+// neighbour in direction (dx,dy,dz) does not depend on the cell.
+template <int DIM>
+G5D void
+fill_slot_table(SolveDesc const & S, int32_t const * __restrict__ slots, int bx, int by, int bz, int k, int * tab)
+{
+  if (k >= (DIM == 2 ? 9 : 27))
+    return;
+  const int b3[3] = { bx, by, bz };
+  const int dir[3] = { k % 3 - 1, (k / 3) % 3 - 1, DIM == 3 ? k / 9 - 1 : 0 };
+  const int c[3] = { 0, 0, 0 };
+  int       N[3], I[3], cc[3], neg;
+  neighbour_block<DIM>(S, b3, dir, N);
+  remap_block_cell<DIM, 16>(S, N, c, I, cc, neg);
+  tab[k] = slots[S.base + I[0] + S.nb[0] * (I[1] + S.nb[1] * I[2])];
+}
+
+// resolve_cell with the block slot taken from a table made by fill_slot_table
+template <int DIM, int B>
+G5D int64_t
+resolve_cell_tab(SolveDesc const & S, int const * tab, int bx, int by, int bz, int gi, int gj, int gk, int & neg)
+{
+  const int b3[3] = { bx, by, bz };
+  int       dir[3], c[3], N[3], I[3], cc[3];
+  tile_cell_dir<DIM, B>(gi, gj, gk, dir, c);
+  neighbour_block<DIM>(S, b3, dir, N);
+  remap_block_cell<DIM, B>(S, N, c, I, cc, neg);
+  return cell_offset<DIM, B>(tab[(dir[0] + 1) + 3 * (dir[1] + 1) + 9 * (dir[2] + 1)], cc);
+}
+
+// the mirrored cell of a WALL border: momentum component neg negated (neg = -1: none)
+G5D void
+negate_momentum(Cons & u, int neg)
+{
+  u.mu = neg == 0 ? u.mu * -1.0 : u.mu;
+  u.mv = neg == 1 ? u.mv * -1.0 : u.mv;
+  u.mw = neg == 2 ? u.mw * -1.0 : u.mw;
 }
 
 // load the conservative state of one cell: base = offset of (block, var 0, cell)
@@ -1122,12 +1197,7 @@ load_cons(double const * __restrict__ U, int64_t base, int neg)
     u.a1 = U[base + 6 * NC];
     u.ar1 = U[base + 7 * NC];
   }
-  if (neg == 0)
-    u.mu *= -1;
-  else if (neg == 1)
-    u.mv *= -1;
-  else if (neg == 2)
-    u.mw *= -1;
+  negate_momentum(u, neg);
   return u;
 }
 

@@ -3,7 +3,8 @@
 // A 20^3 primitive tile with ghost width 2 does not fit in LDS, so the block is swept plane by
 // plane (2.5-D): LDS holds a ring of three primitive planes (P-1, P, P+1; 20x20 each) for the x,
 // y and z stencils and the THINC normals.  Per plane P = -1..16 the workgroup
-//   a. converts plane P+1 (gathered one phase earlier, ghost/border resolution on the fly),
+//   a. converts plane P+1 (gathered one phase earlier, ghost/border resolution on the fly with
+//      the neighbour block slots from a per-block table in LDS),
 //   b. computes slopes, sound speed and Hancock terms of plane P and its face states,
 //   c. solves HLLC on the x and y faces of plane P and on the z face between P-1 and P,
 //   d. exchanges the x/y fluxes through LDS, applies the x then y update to plane P and the z
@@ -315,6 +316,7 @@ step3d_body(StepArgs const & A, int blk)
   __shared__ double sFXE[NF3];
   __shared__ double sFYE[NF3];
   __shared__ double sRed[4];
+  __shared__ int    sSlot[27];
 
   const int       t = threadIdx.x;
   const BlockDesc bd = A.blocks[blk];
@@ -349,10 +351,21 @@ step3d_body(StepArgs const & A, int blk)
   ring_cell3(lane, rgi, rgj);
   const bool z_early = !ZSPLIT or wave != 1;
 
+  // neighbour slots once per block (LDS), so that a fetch issues its loads without a dependent
+  // global read of the slot table
+  fill_slot_table<3>(S, A.slots, bd.bx, bd.by, bd.bz, t, sSlot);
+  __syncthreads();
+  // raw loads; the WALL momentum sign is applied at the conversion, so that the prefetch of a
+  // plane is not waited for in the phase that issues it
+  auto fetch_raw = [&](int gi, int gj, int gk, int & neg) {
+    const int64_t off = resolve_cell_tab<3, B3>(S, sSlot, bd.bx, bd.by, bd.bz, gi, gj, gk, neg);
+    return load_cons<3, B3>(U, off, -1);
+  };
   auto fetch = [&](int gi, int gj, int gk) {
-    int           neg;
-    const int64_t off = resolve_cell<3, B3>(S, A.slots, bd.bx, bd.by, bd.bz, gi, gj, gk, neg);
-    return load_cons<3, B3>(U, off, neg);
+    int  neg;
+    Cons u = fetch_raw(gi, gj, gk, neg);
+    negate_momentum(u, neg);
+    return u;
   };
   auto put = [&](int P, int gi, int gj, Cons const & u) {
     PrimT<T> q;
@@ -362,6 +375,7 @@ step3d_body(StepArgs const & A, int blk)
 
   // carried state
   Cons     pf_i, pf_x;   // prefetched raw state of the next plane to convert
+  int      neg_i = -1, neg_x = -1; // ... and its WALL momentum component
   Cons     uraw_cur;     // raw state of the owned column in plane P
   Cons     uraw_next;    // ... in plane P+1
   Cons     u2prev;       // plane P-1 after its x and y updates
@@ -381,9 +395,9 @@ step3d_body(StepArgs const & A, int blk)
     put(-1, i, j, b);
     if (has_x)
       put(-1, xgi, xgj, fetch(xgi, xgj, -1));
-    pf_i = fetch(i, j, 0);
+    pf_i = fetch_raw(i, j, 0, neg_i);
     if (has_x)
-      pf_x = fetch(xgi, xgj, 0);
+      pf_x = fetch_raw(xgi, xgj, 0, neg_x);
   }
 
   // z face P (between planes P-1 and P): the Riemann solve, then the completion of plane P-1 --
@@ -411,18 +425,22 @@ step3d_body(StepArgs const & A, int blk)
   {
     const bool inner_plane = P >= 0 and P <= 15;
     // (a) plane P+1 from the prefetched raw state
+    negate_momentum(pf_i, neg_i);
     put(P + 1, i, j, pf_i);
     uraw_next = pf_i;
     if (has_x)
+    {
+      negate_momentum(pf_x, neg_x);
       put(P + 1, xgi, xgj, pf_x);
+    }
     __syncthreads(); // (b)
 
     // (c) prefetch plane P+2, predictor of plane P
     if (P + 2 <= 17)
     {
-      pf_i = fetch(i, j, P + 2);
+      pf_i = fetch_raw(i, j, P + 2, neg_i);
       if (has_x)
-        pf_x = fetch(xgi, xgj, P + 2);
+        pf_x = fetch_raw(xgi, xgj, P + 2, neg_x);
     }
     FaceT<T> fxm, fym, fzm;
     {
