@@ -617,8 +617,21 @@ hllc(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
   const double etotr = rr * eintr + ecinr;
   const double ptotr = pr;
 
-  const double cfastl = sqrt(fmax(cl * cl, smallc * smallc));
-  const double cfastr = sqrt(fmax(cr * cr, smallc * smallc));
+  // cfast = sqrt(max(c*c, smallc*smallc)).  With correctly rounded multiplication and square root
+  // in radix 2, sqrt(RN(x*x)) == x for every x whose square neither overflows nor underflows, and
+  // RN(x*x) is monotone in x, so the expression equals max(c, smallc) bit for bit when smallc^2 is
+  // normal and c < 1e150.  The reference form is kept for the other cases (and for NaN).
+  double cfastl, cfastr;
+  if (smallc >= 1.5e-154 and cl < 1e150 and cr < 1e150)
+  {
+    cfastl = fmax(cl, smallc);
+    cfastr = fmax(cr, smallc);
+  }
+  else
+  {
+    cfastl = sqrt(fmax(cl * cl, smallc * smallc));
+    cfastr = sqrt(fmax(cr * cr, smallc * smallc));
+  }
   const double SL = fmin(ul, ur) - fmax(cfastl, cfastr);
   const double SR = fmax(ul, ur) + fmax(cfastl, cfastr);
 

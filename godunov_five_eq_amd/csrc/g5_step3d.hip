@@ -285,18 +285,28 @@ struct Pred3
       youngs_ring(sQ, P, gi, gj, nx, ny, nz);
   }
 
-  // face state on axis AX (AX = -1: on the run-time axis ax), ib = -1 left face, +1 right face
+  // THINC quantities of axis AX (AX = -1: of the run-time axis ax), shared by its two faces
+  template <int AX>
+  G5D ThincAxisT<T>
+  axis(SolveDesc const & S, int ax = AX) const
+  {
+    if (!THINC)
+      return ThincAxisT<T>{ false, T(0.0), T(0.0), T(0.0), T(0.0) };
+    const int a = AX >= 0 ? AX : ax;
+    const T   am = a == 0 ? axm : (a == 1 ? aym : azm);
+    const T   ap = a == 0 ? axp : (a == 1 ? ayp : azp);
+    const T   nd = a == 0 ? nx : (a == 1 ? ny : nz);
+    return thinc_axis(S, q.a, am, q.a, ap, fabs(nd));
+  }
+
+  // face state on axis AX (AX = -1: on the run-time axis ax), ib = -1 left face, +1 right face;
+  // ta: axis<AX>(S, ax)
   template <int AX>
   G5D FaceT<T>
-  face(SolveDesc const & S, int ib, int ax = AX) const
+  face(SolveDesc const & S, int ib, ThincAxisT<T> const & ta, int ax = AX) const
   {
     if (!THINC)
       return face_muscl<3, AX, HANCOCK>(S, q, sx, sy, sz, h, T(0.5) * static_cast<T>(ib), ax);
-    const int           a = AX >= 0 ? AX : ax;
-    const T             am = a == 0 ? axm : (a == 1 ? aym : azm);
-    const T             ap = a == 0 ? axp : (a == 1 ? ayp : azp);
-    const T             nd = a == 0 ? nx : (a == 1 ? ny : nz);
-    const ThincAxisT<T> ta = thinc_axis(S, q.a, am, q.a, ap, fabs(nd));
     return face_thinc<3, AX, HANCOCK>(q, sx, sy, sz, h, ta, ib, ax);
   }
 };
@@ -385,6 +395,14 @@ step3d_body(StepArgs const & A, int blk)
   FaceT<T> zLnext;       // +z face state of plane P (wave 1 with ZSPLIT, until its z solve)
   double   inv_max = 0.0;
 
+  // column offsets at plane 0 of the own column and of the extra conversion item: within the
+  // block's z range (dz = 0) no border transform acts on z, so plane k is at offset + 256 k
+  const int64_t own_off0 = blk_off + i + 16 * j;
+  int           x_neg0 = -1;
+  int64_t       x_off0 = 0;
+  if (has_x)
+    x_off0 = resolve_cell_tab<3, B3>(S, sSlot, bd.bx, bd.by, bd.bz, xgi, xgj, 0, x_neg0);
+
   // prologue: planes -2 and -1 into LDS, plane 0 prefetched
   {
     const Cons a = fetch(i, j, -2);
@@ -436,7 +454,18 @@ step3d_body(StepArgs const & A, int blk)
     __syncthreads(); // (b)
 
     // (c) prefetch plane P+2, predictor of plane P
-    if (P + 2 <= 17)
+    if (P + 2 <= 15)
+    {
+      // planes of the block's own z range: each lane's column offset is linear in the plane
+      pf_i = load_cons<3, B3>(U, own_off0 + 256 * (P + 2), -1);
+      neg_i = -1;
+      if (has_x)
+      {
+        pf_x = load_cons<3, B3>(U, x_off0 + 256 * (P + 2), -1);
+        neg_x = x_neg0;
+      }
+    }
+    else if (P + 2 <= 17)
     {
       pf_i = fetch_raw(i, j, P + 2, neg_i);
       if (has_x)
@@ -446,14 +475,17 @@ step3d_body(StepArgs const & A, int blk)
     {
       Pred3<THINC, HANCOCK, T> pr;
       pr.compute(S, sQ, P, i, j, dtdxT);
-      fzm = pr.template face<2>(S, -1);
-      const FaceT<T> fzp = pr.template face<2>(S, 1);
+      const ThincAxisT<T> tz = pr.template axis<2>(S);
+      fzm = pr.template face<2>(S, -1, tz);
+      const FaceT<T> fzp = pr.template face<2>(S, 1, tz);
       if (inner_plane)
       {
-        fxm = pr.template face<0>(S, -1);
-        fym = pr.template face<1>(S, -1);
-        st_store3(sXL, fx3(i + 1, j), pr.template face<0>(S, 1));
-        st_store3(sYL, fy3(i, j + 1), pr.template face<1>(S, 1));
+        const ThincAxisT<T> tx = pr.template axis<0>(S);
+        const ThincAxisT<T> ty = pr.template axis<1>(S);
+        fxm = pr.template face<0>(S, -1, tx);
+        fym = pr.template face<1>(S, -1, ty);
+        st_store3(sXL, fx3(i + 1, j), pr.template face<0>(S, 1, tx));
+        st_store3(sYL, fy3(i, j + 1), pr.template face<1>(S, 1, ty));
       }
       if (P >= 0 and z_early)
         z_solve(P, fzm);
@@ -467,7 +499,7 @@ step3d_body(StepArgs const & A, int blk)
       // ring cell of side rs: its face on the block boundary, one reconstruction for all lanes
       Pred3<THINC, HANCOCK, T> pr;
       pr.compute(S, sQ, P, rgi, rgj, dtdxT);
-      const FaceT<T> f = pr.template face<-1>(S, (rs & 1) ? -1 : 1, rs >> 1);
+      const FaceT<T> f = pr.template face<-1>(S, (rs & 1) ? -1 : 1, pr.template axis<-1>(S, rs >> 1), rs >> 1);
       T *            dst = rs == 0 ? sXL + fx3(0, rr) : (rs == 1 ? sXR + rr : (rs == 2 ? sYL + fy3(rr, 0) : sYR + rr));
       st_store3n(dst, (rs & 1) ? 16 : NF3, f);
     }

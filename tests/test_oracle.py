@@ -461,3 +461,20 @@ def test_thinc_face_alpha_closed_form_bounds_and_symmetry(orc):
     # monotone in the cell value
     vals = [orc.thinc_alpha(0.1, ac, 0.9, 0.7, 1.6, 1) for ac in np.linspace(0.11, 0.89, 200)]
     assert np.all(np.diff(vals) > 0)
+
+
+def test_hllc_fast_speed_identity():
+    """The FP64 HLLC kernel evaluates the reference's cfast = sqrt(max(c*c, smallc*smallc)) as
+    max(c, smallc) when smallc**2 is normal and c < 1e150.  Check the identity it rests on
+    (sqrt(RN(x*x)) == x in binary64 with correct rounding) on random doubles over the whole range
+    and on runs of consecutive doubles around typical sound speeds."""
+    rng = np.random.default_rng(20251204)
+    s = 1e-10
+    c = np.ldexp(1.0 + rng.random(2_000_000), rng.integers(-500, 498, 2_000_000))
+    assert np.array_equal(np.sqrt(np.maximum(c * c, s * s)), np.maximum(c, s))
+    for base in (1.0, 2.0 ** 0.5, 340.29, 1482.7, 5.2e3):
+        run = base + np.arange(200_000) * np.spacing(base)
+        assert np.array_equal(np.sqrt(run * run), run)
+    # below smallc the reference form gives smallc, as max(c, smallc) does
+    tiny = np.ldexp(1.0 + rng.random(10_000), rng.integers(-510, -34, 10_000))
+    assert np.array_equal(np.sqrt(np.maximum(tiny * tiny, s * s)), np.full(tiny.size, s))
