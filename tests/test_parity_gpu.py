@@ -106,6 +106,15 @@ def test_parity_3d_periodic_mixed_bc(g5, orc):
     assert check_parity(U_g, U_o, 3, "3d mixed borders 8 steps")
 
 
+def test_parity_3d_thinc_walls_periodic(g5, orc):
+    # THINC with WALL borders on x and z (mirrored halo columns and planes, negated momentum) and a
+    # periodic y wrap: the halo offsets of the plane prefetch and the per-block slot table
+    cfg, U0 = cases.air_water_3d(nb=3, periodic=(0, 1, 0),
+                                 bc=("WALL", "WALL", "PERIODIC", "PERIODIC", "WALL", "ZERO_GRADIENT"))
+    U_g, U_o = run_both(cfg, U0, 8, g5, orc, "3d air-water THINC walls")
+    assert check_parity(U_g, U_o, 3, "3d air-water THINC walls + periodic y 8 steps")
+
+
 def test_compute_dt_and_host_steps(g5, orc):
     # compute_dt + do_time_step (the drop-in GodunovImplemV0 surface) against the oracle
     cfg, U0 = cases.air_helium_2d(nbx=8, nby=4)
