@@ -959,9 +959,14 @@ cell_inv_dt(SolveDesc const & S, Cons const & u)
   const double c = cons_to_prim<DIM>(S, u, q);
   const double vx = c + fabs(q.u);
   const double vy = c + fabs(q.v);
+  const double vz = DIM == 3 ? c + fabs(q.w) : 0.0;
+  if (S.inv_dx_p2 > 0.0) // dx = 2^k: the quotients by dx are the exact products by 2^-k
+  {
+    const double r = S.inv_dx_p2;
+    return DIM == 2 ? vx * r + vy * r : vx * r + vy * r + vz * r;
+  }
   if (DIM == 2)
     return vx / S.dx + vy / S.dx;
-  const double vz = c + fabs(q.w);
   return vx / S.dx + vy / S.dx + vz / S.dx;
 }
 
@@ -976,7 +981,7 @@ cell_inv_dt_mixed(SolveDesc const & S, Cons const & u)
   float        v = (c + fabsf(q.u)) + (c + fabsf(q.v));
   if (DIM == 3)
     v += c + fabsf(q.w);
-  return static_cast<double>(v) / S.dx;
+  return S.inv_dx_p2 > 0.0 ? static_cast<double>(v) * S.inv_dx_p2 : static_cast<double>(v) / S.dx;
 }
 
 template <int DIM, typename T>

@@ -3,6 +3,8 @@
 
 #include <stdint.h>
 
+#include <cmath>
+
 #include "../../include/godunov_five_eq.h"
 
 namespace g5
@@ -30,6 +32,7 @@ struct SolveDesc
   double  gp[2];      // gamma_k pinf_k/(gamma_k - 1)
   double  thinc_beta;
   double  thinc_eps;
+  double  inv_dx_p2;  // 1/dx when dx is a power of two (x/dx == x*inv_dx_p2 exactly), else 0
   // FP32 copies for the mixed-precision path
   float g1f[2];
   float gpf[2];
@@ -116,6 +119,9 @@ make_solve_desc(g5_config const & c, int32_t base, SolveDesc & s)
   }
   s.thinc_beta = c.thinc_beta;
   s.thinc_eps = c.thinc_epsilon;
+  int          e2 = 0;
+  const double m2 = std::frexp(c.dx, &e2); // dx = m2 * 2^e2, m2 in [0.5, 1)
+  s.inv_dx_p2 = (m2 == 0.5 and e2 > -1000 and e2 < 1000) ? std::ldexp(1.0, 1 - e2) : 0.0;
   for (int k = 0; k < 2; ++k)
   {
     s.g1f[k] = static_cast<float>(s.g1[k]);
