@@ -131,7 +131,7 @@ struct g5_pool
 
   // launch geometry of the step kernels: fixed by the caller, or chosen by the autotuner for the
   // current launch-list size class
-  LaunchCfg launch{ 0, 1, 1 };
+  LaunchCfg launch{ 0, 1, 1, 0 };
   bool      launch_fixed = false;
   bool      autotune = true;
   int32_t   tuned_class = -1; // size class the current launch was tuned (or looked up) for
@@ -458,7 +458,7 @@ launch_step(g5_pool * p, StepArgs const & A, bool hancock)
     return;
   const bool mixed = p->precision == G5_PREC_MIXED;
   if (p->dim == 2)
-    launch_step2d(A, p->thinc != 0, hancock, mixed, p->stream);
+    launch_step2d(A, p->thinc != 0, hancock, mixed, p->launch.ring_wave != 0, p->stream);
   else
     launch_step3d(A, p->thinc != 0, hancock, mixed, p->launch.zsplit != 0, p->stream);
 }
@@ -507,11 +507,15 @@ tune_candidates(g5_pool const * p)
   const int max_wg = p->dim == 2 ? (p->precision == G5_PREC_MIXED ? 8 : 2) : (p->precision == G5_PREC_MIXED ? 2 : 1);
   for (int z = (p->dim == 3 ? 0 : 1); z <= 1; ++z)
   {
-    c.push_back({ 0, 1, z });
-    c.push_back({ 0, 0, z });
+    c.push_back({ 0, 1, z, 0 });
+    c.push_back({ 0, 0, z, 0 });
     if (p->dim == 2) // the 3-D step has no persistent form
+    {
       for (int w = 1; w <= max_wg; w *= 2)
-        c.push_back({ w, 1, z });
+        c.push_back({ w, 1, z, 0 });
+      // a fifth wavefront for the ring work: shorter per-block latency, fewer registers per lane
+      c.push_back({ 0, 1, z, 1 });
+    }
   }
   return c;
 }
@@ -1134,6 +1138,7 @@ to_launch(LaunchCfg const & c, g5_pool const * p, g5_launch * out)
   out->wg_per_cu = c.wg_per_cu;
   out->xcd_remap = c.remap;
   out->zsplit = c.zsplit;
+  out->ring_wave = c.ring_wave;
   out->autotuned = (!p->launch_fixed and p->autotune and p->tuned_class >= 0) ? 1 : 0;
   out->tuned_ms = p->tuned_ms;
 }
@@ -1147,11 +1152,13 @@ set_launch(g5_pool * p, g5_launch const * l)
     return G5_OK;
   }
   if (l->wg_per_cu < 0 or l->wg_per_cu > 16 or (l->xcd_remap != 0 and l->xcd_remap != 1) or
-      (l->zsplit != 0 and l->zsplit != 1))
-    return fail(p->err, G5_ERR_INVALID, "launch: wg_per_cu in [0,16], xcd_remap and zsplit in {0,1}");
+      (l->zsplit != 0 and l->zsplit != 1) or (l->ring_wave != 0 and l->ring_wave != 1))
+    return fail(p->err, G5_ERR_INVALID, "launch: wg_per_cu in [0,16], xcd_remap, zsplit and ring_wave in {0,1}");
   if (p->dim == 3 and l->wg_per_cu != 0)
     return fail(p->err, G5_ERR_UNSUPPORTED, "the 3-D step has no persistent form (wg_per_cu must be 0)");
-  p->launch = LaunchCfg{ l->wg_per_cu, l->xcd_remap, l->zsplit };
+  if (l->ring_wave != 0 and (p->dim != 2 or l->wg_per_cu != 0))
+    return fail(p->err, G5_ERR_UNSUPPORTED, "ring_wave is a 2-D form with one workgroup per block");
+  p->launch = LaunchCfg{ l->wg_per_cu, l->xcd_remap, l->zsplit, l->ring_wave };
   p->launch_fixed = true;
   p->graph_valid = false;
   return G5_OK;

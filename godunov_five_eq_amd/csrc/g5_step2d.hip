@@ -264,15 +264,21 @@ template <typename T>
 constexpr int ARENA2_DOUBLES =
   ((ARENA2_BYTES_A<T> > ARENA2_BYTES_B<T> ? ARENA2_BYTES_A<T> : ARENA2_BYTES_B<T>)+7) / 8;
 
-template <bool THINC, bool HANCOCK, typename T>
+// RW: a fifth wavefront (threads 256..319) does the 64 ring-cell face states and the 32 ring
+// Riemann problems in parallel with the owned cells instead of after them on wave 0 -- the form
+// for launch lists too short to fill the GPU, where a block's latency is the step time.
+template <bool THINC, bool HANCOCK, typename T, bool RW = false>
 G5D void
 step2d_block(StepArgs const & A, int blk)
 {
   __shared__ double sArena[ARENA2_DOUBLES<T>];
-  __shared__ double sRed[4];
+  __shared__ double sRed[5];
   T * const         sQ = reinterpret_cast<T *>(sArena);
 
   const int       t = threadIdx.x;
+  const bool      owner = !RW or t < 256;       // lane with an owned cell
+  const int       tr = RW ? t - 256 : t;        // ring lane index when in [0, 64)
+  const bool      ring = RW ? !owner : t < 64; // lane with a ring cell
   const BlockDesc bd = A.blocks[blk];
   const SolveDesc S = A.solves[bd.solve];
   SolveCtl *      C = A.ctl + bd.solve;
@@ -289,7 +295,10 @@ step2d_block(StepArgs const & A, int blk)
   const int      i = t & 15, j = t >> 4;
 
   // ---- 1. gather + primitives ----
-  Cons uown = load_cons<2, B2>(U, own, -1);
+  Cons uown{};
+  if (owner)
+    uown = load_cons<2, B2>(U, own, -1);
+  if (owner)
   {
     PrimT<T> q;
     to_prim<2>(S, uown, q);
@@ -311,15 +320,18 @@ step2d_block(StepArgs const & A, int blk)
   T *      XL = sQ + 6 * NT2; // [6][NF2] left states of the x faces
   T *      YL = XL + 6 * NF2;  // [6][NF2] left states of the y faces
   FaceT<T> fxm, fxp, fym, fyp, fdummy, fring;
-  predict_faces2<THINC, HANCOCK>(S, sQ, i, j, dtdxT, true, 0, 0, fxm, fxp, fym, fyp, fdummy);
-  st_store2(XL, fx2(i + 1, j), fxp);
-  st_store2(YL, fy2(i, j + 1), fyp);
+  if (owner)
+  {
+    predict_faces2<THINC, HANCOCK>(S, sQ, i, j, dtdxT, true, 0, 0, fxm, fxp, fym, fyp, fdummy);
+    st_store2(XL, fx2(i + 1, j), fxp);
+    st_store2(YL, fy2(i, j + 1), fyp);
+  }
   // ring cells: s=0 (-1,r) +x ; s=1 (16,r) -x ; s=2 (r,-1) +y ; s=3 (r,16) -y
-  const int rs = t >> 4, rr = t & 15;
-  if (t < 64)
+  const int rs = (tr >> 4) & 3, rr = tr & 15;
+  if (ring)
   {
     int gi, gj;
-    halo_cell2(t, gi, gj);
+    halo_cell2(tr, gi, gj);
     const int ax = rs < 2 ? 0 : 1;
     const int ib = (rs & 1) ? -1 : 1;
     predict_faces2<THINC, HANCOCK>(S, sQ, gi, gj, dtdxT, false, ax, ib, fdummy, fdummy, fdummy, fdummy,
@@ -332,10 +344,14 @@ step2d_block(StepArgs const & A, int blk)
   __syncthreads();
 
   // ---- 3. HLLC ----
-  const FluxT<T> Fx = riemann<2>(S, st_load2(XL, fx2(i, j)), fxm, dtdx, 0);
-  const FluxT<T> Fy = riemann<2>(S, st_load2(YL, fy2(i, j)), fym, dtdx, 1);
-  FluxT<T>       Fr;
-  const bool     ring_face = t < 64 and (rs & 1);
+  FluxT<T> Fx, Fy;
+  if (owner)
+  {
+    Fx = riemann<2>(S, st_load2(XL, fx2(i, j)), fxm, dtdx, 0);
+    Fy = riemann<2>(S, st_load2(YL, fy2(i, j)), fym, dtdx, 1);
+  }
+  FluxT<T>   Fr;
+  const bool ring_face = ring and (rs & 1);
   if (ring_face)
   {
     const int      ax = rs == 1 ? 0 : 1;
@@ -349,8 +365,11 @@ step2d_block(StepArgs const & A, int blk)
   T *      FY = FX + 7 * NF2;              // [7][NF2]
   double * FXE = reinterpret_cast<double *>(FY + 7 * NF2);
   double * FYE = FXE + NF2;
-  fl_store2(FX, FXE, fx2(i, j), Fx);
-  fl_store2(FY, FYE, fy2(i, j), Fy);
+  if (owner)
+  {
+    fl_store2(FX, FXE, fx2(i, j), Fx);
+    fl_store2(FY, FYE, fy2(i, j), Fy);
+  }
   if (ring_face)
   {
     if (rs == 1)
@@ -359,14 +378,19 @@ step2d_block(StepArgs const & A, int blk)
       fl_store2(FY, FYE, fy2(rr, 16), Fr);
   }
   __syncthreads();
-  Cons u2 = uown;
-  apply_face_pair<2>(u2, Fx, fl_load2(FX, FXE, fx2(i + 1, j)), uown.a0, uown.a1, dtdx, 0);
-  apply_face_pair<2>(u2, Fy, fl_load2(FY, FYE, fy2(i, j + 1)), uown.a0, uown.a1, dtdx, 1);
-  fix_volume_fractions(u2);
-  store_cons<2, B2>(U2, own, u2);
+  double rate = 0.0;
+  if (owner)
+  {
+    Cons u2 = uown;
+    apply_face_pair<2>(u2, Fx, fl_load2(FX, FXE, fx2(i + 1, j)), uown.a0, uown.a1, dtdx, 0);
+    apply_face_pair<2>(u2, Fy, fl_load2(FY, FYE, fy2(i, j + 1)), uown.a0, uown.a1, dtdx, 1);
+    fix_volume_fractions(u2);
+    store_cons<2, B2>(U2, own, u2);
+    rate = cell_rate<2, T>(S, u2);
+  }
 
-  // ---- 5. CFL of the new state ----
-  double inv = wave_max(cell_rate<2, T>(S, u2));
+  // ---- 5. CFL of the new state (the ring wavefront has no cell) ----
+  double inv = wave_max(rate);
   if ((t & 63) == 0)
     sRed[t >> 6] = inv;
   __syncthreads();
@@ -388,6 +412,17 @@ k_step2d(StepArgs A)
     step2d_block<THINC, HANCOCK, T>(A, b);
 }
 
+// one workgroup of five wavefronts per block (ring work on the fifth)
+template <bool THINC, bool HANCOCK, typename T>
+__global__ void __launch_bounds__(320)
+k_step2d_ringwave(StepArgs A)
+{
+  int b, end, stride;
+  block_range(A, b, end, stride);
+  if (b < end)
+    step2d_block<THINC, HANCOCK, T, true>(A, b);
+}
+
 // persistent workgroups looping over their range of the block list
 template <bool THINC, bool HANCOCK, typename T>
 __global__ void __launch_bounds__(256)
@@ -404,36 +439,38 @@ k_step2d_persistent(StepArgs A)
 
 template <bool THINC, bool HANCOCK, typename T>
 static void
-launch2k(StepArgs const & A, hipStream_t stream)
+launch2k(StepArgs const & A, bool ring_wave, hipStream_t stream)
 {
   const dim3 block(256);
   if (A.wg_per_xcd > 0)
     hipLaunchKernelGGL((k_step2d_persistent<THINC, HANCOCK, T>), dim3(8 * A.wg_per_xcd), block, 0, stream, A);
+  else if (ring_wave)
+    hipLaunchKernelGGL((k_step2d_ringwave<THINC, HANCOCK, T>), dim3(8 * A.per_xcd), dim3(320), 0, stream, A);
   else
     hipLaunchKernelGGL((k_step2d<THINC, HANCOCK, T>), dim3(8 * A.per_xcd), block, 0, stream, A);
 }
 
 template <typename T>
 static void
-launch2(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
+launch2(StepArgs const & A, bool thinc, bool hancock, bool ring_wave, hipStream_t stream)
 {
   if (thinc and hancock)
-    launch2k<true, true, T>(A, stream);
+    launch2k<true, true, T>(A, ring_wave, stream);
   else if (thinc)
-    launch2k<true, false, T>(A, stream);
+    launch2k<true, false, T>(A, ring_wave, stream);
   else if (hancock)
-    launch2k<false, true, T>(A, stream);
+    launch2k<false, true, T>(A, ring_wave, stream);
   else
-    launch2k<false, false, T>(A, stream);
+    launch2k<false, false, T>(A, ring_wave, stream);
 }
 
 void
-launch_step2d(StepArgs const & A, bool thinc, bool hancock, bool mixed, hipStream_t stream)
+launch_step2d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool ring_wave, hipStream_t stream)
 {
   if (mixed)
-    launch2<float>(A, thinc, hancock, stream);
+    launch2<float>(A, thinc, hancock, ring_wave, stream);
   else
-    launch2<double>(A, thinc, hancock, stream);
+    launch2<double>(A, thinc, hancock, ring_wave, stream);
 }
 
 } // namespace g5

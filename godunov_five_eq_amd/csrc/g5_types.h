@@ -90,6 +90,7 @@ struct LaunchCfg
   int32_t wg_per_cu; // 0: one workgroup per block; > 0: persistent, this many workgroups per CU
   int32_t remap;     // XCD-contiguous block ranges
   int32_t zsplit;    // 3-D: z Riemann solve split around the x/y exchange
+  int32_t ring_wave; // 2-D, one workgroup per block: a fifth wavefront for the ring work
 };
 
 // filling the host-side SolveDesc from a g5_config

@@ -265,6 +265,7 @@ typedef struct g5_launch
   int32_t wg_per_cu; /* 0: one workgroup per block; 1..16: persistent workgroups per CU (2-D only) */
   int32_t xcd_remap; /* 1: each XCD takes a contiguous range of the block list */
   int32_t zsplit;    /* 3-D: z Riemann solve split around the x/y flux exchange (1) or not (0) */
+  int32_t ring_wave; /* 2-D, wg_per_cu 0: a fifth wavefront per workgroup for the ring cells (1) */
   int32_t autotuned; /* out: 1 when the geometry came from the autotuner */
   double  tuned_ms;  /* out: best step-kernel time measured by the autotuner (ms) */
 } g5_launch;

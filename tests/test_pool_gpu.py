@@ -153,15 +153,15 @@ def test_launch_geometries_bit_identical(g5):
     for dim, prec in ((2, g5.PREC_FP64), (2, g5.PREC_MIXED), (3, g5.PREC_FP64), (3, g5.PREC_MIXED)):
         if dim == 2:
             cfg, U0 = cases.air_helium_2d(nbx=48, nby=40, thinc=True, thinc_beta=1.6, precision=prec)
-            geoms = [(0, 1, 1), (0, 0, 1), (1, 1, 1), (2, 1, 1), (4, 0, 1)]
+            geoms = [(0, 1, 1, 0), (0, 0, 1, 0), (1, 1, 1, 0), (2, 1, 1, 0), (4, 0, 1, 0), (0, 1, 1, 1), (0, 0, 1, 1)]
         else:
             cfg, U0 = cases.air_water_3d(nb=3, precision=prec)
-            geoms = [(0, 1, 1), (0, 0, 1), (0, 1, 0), (0, 0, 0)]
+            geoms = [(0, 1, 1, 0), (0, 0, 1, 0), (0, 1, 0, 0), (0, 0, 0, 0)]
         ref = None
-        for wg, remap, zs in geoms:
-            say(f"dim={dim} prec={prec} launch wg_per_cu={wg} remap={remap} zsplit={zs}")
+        for wg, remap, zs, rw in geoms:
+            say(f"dim={dim} prec={prec} launch wg_per_cu={wg} remap={remap} zsplit={zs} ring_wave={rw}")
             with g5.Pool(dim, thinc=True, precision=prec, budget_bytes=1 << 28) as pool:
-                pool.set_launch(wg, remap, zs)
+                pool.set_launch(wg, remap, zs, rw)
                 sid = pool.admit(cfg, max_steps=6)
                 pool.upload(sid, U0)
                 pool.start([sid])
@@ -171,7 +171,7 @@ def test_launch_geometries_bit_identical(g5):
             if ref is None:
                 ref = U
             else:
-                assert np.array_equal(U, ref), (dim, prec, wg, remap, zs)
+                assert np.array_equal(U, ref), (dim, prec, wg, remap, zs, rw)
         # the autotuned pool gives the same bits as well
         with g5.Pool(dim, thinc=True, precision=prec, budget_bytes=1 << 28) as pool:
             sid = pool.admit(cfg, max_steps=6)
@@ -186,6 +186,11 @@ def test_3d_persistent_launch_is_rejected(g5):
     with g5.Pool(3, thinc=True, budget_bytes=1 << 26) as pool:
         with pytest.raises(g5.G5Error):
             pool.set_launch(1, 1, 1)
+        with pytest.raises(g5.G5Error):  # the ring wavefront is a 2-D form
+            pool.set_launch(0, 1, 1, 1)
+    with g5.Pool(2, thinc=True, budget_bytes=1 << 26) as pool:
+        with pytest.raises(g5.G5Error):  # ... with one workgroup per block
+            pool.set_launch(2, 1, 1, 1)
 
 
 def test_checkpoint_restart_is_bit_identical(g5, tmp_path):
