@@ -279,8 +279,8 @@ def main():
     ap.add_argument("--poll", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; runs on several GPUs are out of scope "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: the step runs on one GPU; sharding blocks over several GPUs (one halo "
+              "exchange and one dt min-reduction per step) is not built", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
         args.steps = DEFAULT_STEPS[args.workload]
