@@ -19,6 +19,8 @@ TI_HANCOCK, TI_RK1, TI_RK2_SSP = range(3)
 PREC_FP64, PREC_MIXED = range(2)
 
 BC_NAMES = {"NONE": BC_NONE, "PERIODIC": BC_PERIODIC, "ZERO_GRADIENT": BC_ZERO_GRADIENT, "WALL": BC_WALL}
+INLET_NONE, INLET_SHOCK_BUBBLE, INLET_JET = 0, 1, 2
+INLET_NAMES = {"none": INLET_NONE, "shock_bubble": INLET_SHOCK_BUBBLE, "jet": INLET_JET}
 TI_NAMES = {"HANCOCK": TI_HANCOCK, "RK1": TI_RK1, "RK2_SSP": TI_RK2_SSP}
 
 
@@ -42,6 +44,10 @@ class G5Config(ctypes.Structure):
         ("thinc_beta", ctypes.c_double),
         ("thinc_epsilon", ctypes.c_double),
         ("xmin", ctypes.c_double * 3),
+        ("inlet", ctypes.c_int32),
+        ("inlet_pad", ctypes.c_int32),
+        ("inlet_u", ctypes.c_double * 9),
+        ("jet", ctypes.c_double * 3),
     ]
 
 
@@ -154,6 +160,7 @@ def lib():
             "g5_last_run_ms": (ctypes.c_int, [vp, P(d)]),
             "g5_time_step_kernel": (ctypes.c_int, [vp, i32, d, P(d)]),
             "g5_init_shock_bubble": (ctypes.c_int, [P(G5Config), P(G5Region), d, P(d), d, P(d), sz]),
+            "g5_region_conservative": (ctypes.c_int, [P(G5Config), P(G5Region), P(d), sz]),
             "g5_pool_create": (ctypes.c_int, [i32, i32, i32, ctypes.c_uint64, P(vp)]),
             "g5_pool_destroy": (ctypes.c_int, [vp]),
             "g5_pool_last_error": (ctypes.c_char_p, [vp]),
@@ -200,7 +207,7 @@ def exported_symbols():
         "g5_version", "g5_last_error", "g5_create", "g5_destroy", "g5_shape", "g5_mem_bytes",
         "g5_upload_state", "g5_download_state", "g5_state_device_ptr", "g5_compute_dt",
         "g5_fill_outside", "g5_outside_coords", "g5_do_time_step", "g5_run", "g5_synchronize", "g5_last_run_ms",
-        "g5_time_step_kernel", "g5_init_shock_bubble", "g5_rankine_hugoniot",
+        "g5_time_step_kernel", "g5_init_shock_bubble", "g5_rankine_hugoniot", "g5_region_conservative",
         "g5_pool_create", "g5_pool_destroy", "g5_pool_last_error", "g5_pool_admit", "g5_pool_release",
         "g5_pool_upload", "g5_pool_download", "g5_pool_init_shock_bubble", "g5_pool_init_triple_point",
         "g5_pool_start", "g5_pool_step", "g5_pool_status", "g5_pool_finished", "g5_pool_usage",
@@ -227,8 +234,10 @@ def nvar_of(dim):
 def make_config(dim, nb, dx, *, bsize=16, periodic=(0, 0, 0), bc=("ZERO_GRADIENT",) * 6,
                 integrator="HANCOCK", thinc=False, precision=PREC_FP64, cfl=0.5, slope_type=1.0,
                 smallr=1e-10, smallc=1e-10, gamma=(1.4, 1.4), pinf=(0.0, 0.0), thinc_beta=1.5,
-                thinc_epsilon=1e-3, xmin=(0.0, 0.0, 0.0)):
-    """Build a G5Config; defaults follow models/HydroSettings.cpp:18-29."""
+                thinc_epsilon=1e-3, xmin=(0.0, 0.0, 0.0), inlet=None, jet=(0.5, 0.5, 0.05)):
+    """Build a G5Config; defaults follow models/HydroSettings.cpp:18-29.  inlet: None, or
+    ("shock_bubble", region) / ("jet", region) -- outside cells with x < xmin (for the jet those
+    whose centre is inside it, jet = (centre y, centre z, width)) take the region's state."""
     c = G5Config()
     c.dim = dim
     c.bsize = bsize
@@ -252,7 +261,25 @@ def make_config(dim, nb, dx, *, bsize=16, periodic=(0, 0, 0), bc=("ZERO_GRADIENT
     c.pinf[0], c.pinf[1] = pinf
     c.thinc_beta = thinc_beta
     c.thinc_epsilon = thinc_epsilon
+    for k in range(3):
+        c.jet[k] = jet[k]
+    if inlet is not None:
+        kind, reg = inlet
+        c.inlet = INLET_NAMES[kind]
+        u = region_conservative(c, reg)
+        for k in range(u.size):
+            c.inlet_u[k] = u[k]
     return c
+
+
+def region_conservative(cfg, reg):
+    """Conservative state (nvar values) of a region (common.h:68-144 get_region_init_state)."""
+    u = np.zeros(nvar_of(cfg.dim), dtype=np.float64)
+    rc = lib().g5_region_conservative(ctypes.byref(cfg), ctypes.byref(reg),
+                                      u.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), u.size)
+    if rc != 0:
+        raise G5Error(rc, "g5_region_conservative")
+    return u
 
 
 def state_size(cfg):

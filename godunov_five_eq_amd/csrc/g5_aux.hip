@@ -213,7 +213,8 @@ k_fill_outside(double * buf0, double * buf1, SolveCtl const * ctl, int solve, So
     const int     c[3] = { cell & 15, (cell >> 4) & 15, DIM == 3 ? cell >> 8 : 0 };
     int           neg;
     const int64_t off = resolve_block_cell<DIM, 16>(S, slots, N, c, neg);
-    const Cons    u = load_cons<DIM, 16>(U, off, neg);
+    Cons          u = load_cons<DIM, 16>(U, off, -1);
+    apply_border<DIM>(S, u, neg);
     store_cons<DIM, 16>(ring, static_cast<int64_t>(o) * NV * NC + cell, u);
   }
 }

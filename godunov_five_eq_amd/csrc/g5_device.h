@@ -1034,7 +1034,8 @@ block_range(StepArgs const & A, int & b, int & end, int & stride)
 // negated).
 // ---------------------------------------------------------------------------------------------
 // I: the inside block (N itself when N is inside the domain); cc: the cell of I to read.
-template <int DIM, int B>
+// INLET false: the caller knows that no solve it serves has an inflow border.
+template <int DIM, int B, bool INLET = true>
 G5D void
 remap_block_cell(SolveDesc const & S, int const (&N)[3], int const (&c)[3], int (&I)[3], int (&cc)[3], int & neg)
 {
@@ -1079,6 +1080,26 @@ remap_block_cell(SolveDesc const & S, int const (&N)[3], int const (&c)[3], int 
         }
       }
     }
+  }
+  // inflow override (SolverGodunovFiveEq.cpp:997-1018, after the border fill): outside cells with
+  // x < xmin, for the jet those whose centre is inside it; neg = -2 marks them
+  if (INLET and S.inlet != G5_INLET_NONE and N[0] < 0)
+  {
+    bool in = true;
+    if (S.inlet == G5_INLET_JET)
+    {
+      const double y = S.origin[1] + (static_cast<double>(N[1] * B + c[1]) + 0.5) * S.dx;
+      if (DIM == 2)
+        in = y > S.jet[0] - S.jet[2] / 2 and y < S.jet[0] + S.jet[2] / 2;
+      else
+      {
+        const double z = S.origin[2] + (static_cast<double>(N[2] * B + c[2]) + 0.5) * S.dx;
+        const double r2 = (y - S.jet[0]) * (y - S.jet[0]) + (z - S.jet[1]) * (z - S.jet[1]);
+        in = r2 < S.jet[2] * S.jet[2] / 4;
+      }
+    }
+    if (in)
+      neg = -2;
   }
 }
 
@@ -1162,12 +1183,12 @@ fill_slot_table(SolveDesc const & S, int32_t const * __restrict__ slots, int bx,
   const int c[3] = { 0, 0, 0 };
   int       N[3], I[3], cc[3], neg;
   neighbour_block<DIM>(S, b3, dir, N);
-  remap_block_cell<DIM, 16>(S, N, c, I, cc, neg);
+  remap_block_cell<DIM, 16, false>(S, N, c, I, cc, neg); // the inside block only
   tab[k] = slots[S.base + I[0] + S.nb[0] * (I[1] + S.nb[1] * I[2])];
 }
 
 // resolve_cell with the block slot taken from a table made by fill_slot_table
-template <int DIM, int B>
+template <int DIM, int B, bool INLET = true>
 G5D int64_t
 resolve_cell_tab(SolveDesc const & S, int const * tab, int bx, int by, int bz, int gi, int gj, int gk, int & neg)
 {
@@ -1175,7 +1196,7 @@ resolve_cell_tab(SolveDesc const & S, int const * tab, int bx, int by, int bz, i
   int       dir[3], c[3], N[3], I[3], cc[3];
   tile_cell_dir<DIM, B>(gi, gj, gk, dir, c);
   neighbour_block<DIM>(S, b3, dir, N);
-  remap_block_cell<DIM, B>(S, N, c, I, cc, neg);
+  remap_block_cell<DIM, B, INLET>(S, N, c, I, cc, neg);
   return cell_offset<DIM, B>(tab[(dir[0] + 1) + 3 * (dir[1] + 1) + 9 * (dir[2] + 1)], cc);
 }
 
@@ -1186,6 +1207,28 @@ negate_momentum(Cons & u, int neg)
   u.mu = neg == 0 ? u.mu * -1.0 : u.mu;
   u.mv = neg == 1 ? u.mv * -1.0 : u.mv;
   u.mw = neg == 2 ? u.mw * -1.0 : u.mw;
+}
+
+// the outside-cell transform a loaded source cell still needs: the inflow state (neg = -2) or the
+// WALL momentum sign (neg = 0, 1, 2)
+template <int DIM, bool INLET = true>
+G5D void
+apply_border(SolveDesc const & S, Cons & u, int neg)
+{
+  if (INLET and neg == -2)
+  {
+    u.d = S.inlet_u[0];
+    u.e = S.inlet_u[1];
+    u.mu = S.inlet_u[2];
+    u.mv = S.inlet_u[3];
+    u.mw = DIM == 3 ? S.inlet_u[4] : 0.0;
+    u.a0 = S.inlet_u[DIM == 3 ? 5 : 4];
+    u.ar0 = S.inlet_u[DIM == 3 ? 6 : 5];
+    u.a1 = S.inlet_u[DIM == 3 ? 7 : 6];
+    u.ar1 = S.inlet_u[DIM == 3 ? 8 : 7];
+  }
+  else
+    negate_momentum(u, neg);
 }
 
 // load the conservative state of one cell: base = offset of (block, var 0, cell)

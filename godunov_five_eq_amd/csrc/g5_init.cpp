@@ -89,6 +89,18 @@ sphere_fraction(int dim, double const * lo, double dx, double const * c, double 
 extern "C"
 {
 
+// common.h:68-144 get_region_init_state
+int
+g5_region_conservative(const g5_config * cfg, const g5_region * region, double * U, size_t n)
+{
+  if (not cfg or not region or not U or (cfg->dim != 2 and cfg->dim != 3))
+    return G5_ERR_INVALID;
+  if (n != static_cast<size_t>(cfg->dim == 2 ? 8 : 9))
+    return G5_ERR_INVALID;
+  region_to_cons(*cfg, Mix(*cfg), *region, U);
+  return G5_OK;
+}
+
 // init/InitShockBubble.cpp:50-225 InitShockBubbleDataFunctor (one bubble)
 int
 g5_init_shock_bubble(const g5_config * cfg, const g5_region * regions3, double x_front,

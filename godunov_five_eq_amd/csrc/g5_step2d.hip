@@ -309,7 +309,8 @@ step2d_block(StepArgs const & A, int blk)
     int gi, gj, neg;
     halo_cell2(t, gi, gj);
     const int64_t off = resolve_cell<2, B2>(S, A.slots, bd.bx, bd.by, 0, gi, gj, 0, neg);
-    const Cons    u = load_cons<2, B2>(U, off, neg);
+    Cons          u = load_cons<2, B2>(U, off, -1);
+    apply_border<2>(S, u, neg);
     PrimT<T>      q;
     to_prim<2>(S, u, q);
     sq_store2(sQ, tidx2(gi, gj), q);

@@ -312,7 +312,7 @@ struct Pred3
 };
 } // namespace
 
-template <bool THINC, bool HANCOCK, bool ZSPLIT, typename T>
+template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, typename T>
 G5D void
 step3d_body(StepArgs const & A, int blk)
 {
@@ -368,13 +368,13 @@ step3d_body(StepArgs const & A, int blk)
   // raw loads; the WALL momentum sign is applied at the conversion, so that the prefetch of a
   // plane is not waited for in the phase that issues it
   auto fetch_raw = [&](int gi, int gj, int gk, int & neg) {
-    const int64_t off = resolve_cell_tab<3, B3>(S, sSlot, bd.bx, bd.by, bd.bz, gi, gj, gk, neg);
+    const int64_t off = resolve_cell_tab<3, B3, INLET>(S, sSlot, bd.bx, bd.by, bd.bz, gi, gj, gk, neg);
     return load_cons<3, B3>(U, off, -1);
   };
   auto fetch = [&](int gi, int gj, int gk) {
     int  neg;
     Cons u = fetch_raw(gi, gj, gk, neg);
-    negate_momentum(u, neg);
+    apply_border<3, INLET>(S, u, neg);
     return u;
   };
   auto put = [&](int P, int gi, int gj, Cons const & u) {
@@ -398,10 +398,11 @@ step3d_body(StepArgs const & A, int blk)
   // column offsets at plane 0 of the own column and of the extra conversion item: within the
   // block's z range (dz = 0) no border transform acts on z, so plane k is at offset + 256 k
   const int64_t own_off0 = blk_off + i + 16 * j;
+  const bool    inlet_block = INLET and S.inlet != G5_INLET_NONE and bd.bx == 0; // halo cells may be inflow cells
   int           x_neg0 = -1;
   int64_t       x_off0 = 0;
   if (has_x)
-    x_off0 = resolve_cell_tab<3, B3>(S, sSlot, bd.bx, bd.by, bd.bz, xgi, xgj, 0, x_neg0);
+    x_off0 = resolve_cell_tab<3, B3, INLET>(S, sSlot, bd.bx, bd.by, bd.bz, xgi, xgj, 0, x_neg0);
 
   // prologue: planes -2 and -1 into LDS, plane 0 prefetched
   {
@@ -443,18 +444,18 @@ step3d_body(StepArgs const & A, int blk)
   {
     const bool inner_plane = P >= 0 and P <= 15;
     // (a) plane P+1 from the prefetched raw state
-    negate_momentum(pf_i, neg_i);
+    negate_momentum(pf_i, neg_i); // the own column is never left of x-min: no inflow cell
     put(P + 1, i, j, pf_i);
     uraw_next = pf_i;
     if (has_x)
     {
-      negate_momentum(pf_x, neg_x);
+      apply_border<3, INLET>(S, pf_x, neg_x);
       put(P + 1, xgi, xgj, pf_x);
     }
     __syncthreads(); // (b)
 
     // (c) prefetch plane P+2, predictor of plane P
-    if (P + 2 <= 15)
+    if (P + 2 <= 15 and not inlet_block)
     {
       // planes of the block's own z range: each lane's column offset is linear in the plane
       pf_i = load_cons<3, B3>(U, own_off0 + 256 * (P + 2), -1);
@@ -563,31 +564,34 @@ step3d_body(StepArgs const & A, int blk)
 // One workgroup per block.  (A persistent form, workgroups looping over several blocks, made
 // an illegal memory access on the GPU with no cause found in the code; the runtime never asks
 // for it in 3-D: the launch must have wg_per_xcd == 0.)
-template <bool THINC, bool HANCOCK, bool ZSPLIT, typename T>
+template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, typename T>
 G5D void
 step3d_blocks(StepArgs const & A)
 {
   int b, end, stride;
   block_range(A, b, end, stride);
   if (b < end)
-    step3d_body<THINC, HANCOCK, ZSPLIT, T>(A, b);
+    step3d_body<THINC, HANCOCK, ZSPLIT, INLET, T>(A, b);
 }
 
-template <bool THINC, bool HANCOCK, bool ZSPLIT>
+// INLET: the launch list holds a solve with an inflow border (the resolver then marks inflow
+// cells, whose state the conversion replaces); without it the border code is the WALL /
+// ZERO_GRADIENT / periodic resolution alone
+template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET>
 __global__ void __launch_bounds__(256)
 k_step3d_v3(StepArgs A)
 {
-  step3d_blocks<THINC, HANCOCK, ZSPLIT, double>(A);
+  step3d_blocks<THINC, HANCOCK, ZSPLIT, INLET, double>(A);
 }
 
-template <bool THINC, bool HANCOCK, bool ZSPLIT>
+template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_step3d_mixed(StepArgs A)
 {
-  step3d_blocks<THINC, HANCOCK, ZSPLIT, float>(A);
+  step3d_blocks<THINC, HANCOCK, ZSPLIT, INLET, float>(A);
 }
 
-template <bool ZSPLIT, typename T>
+template <bool ZSPLIT, bool INLET, typename T>
 static void
 launch3v3(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
 {
@@ -595,44 +599,54 @@ launch3v3(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
   if constexpr (sizeof(T) == sizeof(double))
   {
     if (thinc and hancock)
-      hipLaunchKernelGGL((k_step3d_v3<true, true, ZSPLIT>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_v3<true, true, ZSPLIT, INLET>), grid, block, 0, stream, A);
     else if (thinc)
-      hipLaunchKernelGGL((k_step3d_v3<true, false, ZSPLIT>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_v3<true, false, ZSPLIT, INLET>), grid, block, 0, stream, A);
     else if (hancock)
-      hipLaunchKernelGGL((k_step3d_v3<false, true, ZSPLIT>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_v3<false, true, ZSPLIT, INLET>), grid, block, 0, stream, A);
     else
-      hipLaunchKernelGGL((k_step3d_v3<false, false, ZSPLIT>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_v3<false, false, ZSPLIT, INLET>), grid, block, 0, stream, A);
   }
   else
   {
     if (thinc and hancock)
-      hipLaunchKernelGGL((k_step3d_mixed<true, true, ZSPLIT>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_mixed<true, true, ZSPLIT, INLET>), grid, block, 0, stream, A);
     else if (thinc)
-      hipLaunchKernelGGL((k_step3d_mixed<true, false, ZSPLIT>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_mixed<true, false, ZSPLIT, INLET>), grid, block, 0, stream, A);
     else if (hancock)
-      hipLaunchKernelGGL((k_step3d_mixed<false, true, ZSPLIT>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_mixed<false, true, ZSPLIT, INLET>), grid, block, 0, stream, A);
     else
-      hipLaunchKernelGGL((k_step3d_mixed<false, false, ZSPLIT>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_mixed<false, false, ZSPLIT, INLET>), grid, block, 0, stream, A);
+  }
+}
+
+template <typename T>
+static void
+launch3p(StepArgs const & A, bool thinc, bool hancock, bool split, bool inlet, hipStream_t stream)
+{
+  if (split)
+  {
+    if (inlet)
+      launch3v3<true, true, T>(A, thinc, hancock, stream);
+    else
+      launch3v3<true, false, T>(A, thinc, hancock, stream);
+  }
+  else
+  {
+    if (inlet)
+      launch3v3<false, true, T>(A, thinc, hancock, stream);
+    else
+      launch3v3<false, false, T>(A, thinc, hancock, stream);
   }
 }
 
 void
-launch_step3d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool split, hipStream_t stream)
+launch_step3d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool split, bool inlet, hipStream_t stream)
 {
   if (mixed)
-  {
-    if (split)
-      launch3v3<true, float>(A, thinc, hancock, stream);
-    else
-      launch3v3<false, float>(A, thinc, hancock, stream);
-  }
+    launch3p<float>(A, thinc, hancock, split, inlet, stream);
   else
-  {
-    if (split)
-      launch3v3<true, double>(A, thinc, hancock, stream);
-    else
-      launch3v3<false, double>(A, thinc, hancock, stream);
-  }
+    launch3p<double>(A, thinc, hancock, split, inlet, stream);
 }
 
 } // namespace g5

@@ -21,7 +21,7 @@ struct SolveDesc
   int32_t periodic[3]; // periodicity per direction
   int32_t bc[6];       // G5_BC_* per face
   int32_t integrator;  // G5_TI_*
-  int32_t pad;
+  int32_t inlet;       // G5_INLET_*
   double  dx;
   double  cfl;
   double  slope_type;
@@ -33,6 +33,9 @@ struct SolveDesc
   double  thinc_beta;
   double  thinc_eps;
   double  inv_dx_p2;  // 1/dx when dx is a power of two (x/dx == x*inv_dx_p2 exactly), else 0
+  double  inlet_u[9]; // conservative inflow state of the overridden outside cells
+  double  origin[3];  // domain origin
+  double  jet[3];     // jet centre y, z and width
   // FP32 copies for the mixed-precision path
   float g1f[2];
   float gpf[2];
@@ -106,7 +109,14 @@ make_solve_desc(g5_config const & c, int32_t base, SolveDesc & s)
   for (int f = 0; f < 6; ++f)
     s.bc[f] = c.bc[f];
   s.integrator = c.integrator;
-  s.pad = 0;
+  s.inlet = c.inlet;
+  for (int k = 0; k < 9; ++k)
+    s.inlet_u[k] = c.inlet_u[k];
+  for (int d = 0; d < 3; ++d)
+  {
+    s.origin[d] = c.xmin[d];
+    s.jet[d] = c.jet[d];
+  }
   s.dx = c.dx;
   s.cfl = c.cfl;
   s.slope_type = c.slope_type;

@@ -62,6 +62,15 @@ enum
   G5_PREC_MIXED = 1 /* FP32 slopes, wave speeds and THINC; FP64 state, EOS pressure recovery, dt */
 };
 
+/* inflow overrides of the outside cells, applied after the border fill
+ * (SolverGodunovFiveEq.cpp:997-1018) */
+enum
+{
+  G5_INLET_NONE = 0,
+  G5_INLET_SHOCK_BUBBLE = 1, /* every outside cell with x < xmin (border_conditions/BCShockBubble.h:62-74) */
+  G5_INLET_JET = 2           /* outside cells with x < xmin whose centre is in the jet (BCJet.h:71-88) */
+};
+
 /*
  * Solve configuration.  Field meaning follows the reference .ini keys:
  * [hydro] cfl slope_type smallr smallc (models/HydroSettings.cpp:18-29),
@@ -89,7 +98,15 @@ typedef struct g5_config
   double  pinf[2];  /* per material (0 for an ideal gas) */
   double  thinc_beta;
   double  thinc_epsilon;
-  double  xmin[3]; /* domain origin, used by the initial-condition generators */
+  double  xmin[3]; /* domain origin, used by the initial-condition generators and the inlets */
+  /* inflow override: G5_INLET_*; the overridden outside cells take the conservative state
+   * inlet_u (variable ids of models/FiveEq.h, nvar values); jet = (centre y, centre z, width):
+   * 2-D |y - jet[0]| < jet[2]/2, 3-D (y - jet[0])^2 + (z - jet[1])^2 < jet[2]^2/4 at the cell
+   * centre ([jet] y, z, width) */
+  int32_t inlet;
+  int32_t inlet_pad;
+  double  inlet_u[9];
+  double  jet[3];
 } g5_config;
 
 typedef struct g5_ctx g5_ctx;
@@ -176,6 +193,10 @@ typedef struct g5_region
   double p;
   double u, v, w;
 } g5_region;
+
+/* Conservative state of a region (the inflow state of the inlets, ...): replaces
+ * get_region_init_state (common.h:68-144).  U receives nvar values. */
+int g5_region_conservative(const g5_config * cfg, const g5_region * region, double * U, size_t n);
 
 /* Shock-bubble initial condition of InitShockBubbleDataFunctor: region 0 post-shock
  * (x <= x_front), region 1 pre-shock, region 2 bubble of radius r at center c, with the bubble

@@ -179,6 +179,8 @@ bc_from_name(std::string const & s)
 // [p4est_connectivity] nbrick_{x,y,z} periodic_{x,y,z}, [mesh] xmin ymin zmin scaling_factor
 // boundary_type_*, [hydro] cfl slope_type smallr smallc riemann, [THINC] enabled beta epsilon,
 // [material{0,1}] mg_eos_name with [material{k}_mie_gruneisen_{ig,sg}] gamma pinf.
+inline g5_region region_from_ini(ConfigMap const & c, int i);
+
 inline g5_config
 config_from_ini(ConfigMap const & c)
 {
@@ -225,6 +227,30 @@ config_from_ini(ConfigMap const & c)
     cfg.pinf[k] = eos == "MG_STIFFENED_GAS" ? c.getReal(sec, "pinf", 0.0) : 0.0;
   }
   cfg.precision = c.getString("hydro", "precision", "FP64") == "MIXED" ? G5_PREC_MIXED : G5_PREC_FP64;
+  // inflow overrides of fill_outside_quadrants (SolverGodunovFiveEq.cpp:997-1018): the jet takes
+  // region 1 ([border] enable_jet, [jet] y, z, width: border_conditions/BCJet.h:62-69,
+  // BCJet.cpp:55), the shock-bubble inlet region 0 ([shock_bubble] use_inlet_bc, a
+  // ShockBubbleParams key of the absent core: the key name is this layer's choice)
+  cfg.jet[0] = c.getReal("jet", "y", 0.5);
+  cfg.jet[1] = c.getReal("jet", "z", 0.5);
+  cfg.jet[2] = c.getReal("jet", "width", 0.05);
+  int inlet_region = -1;
+  if (c.getBool("border", "enable_jet", false))
+  {
+    cfg.inlet = G5_INLET_JET;
+    inlet_region = 1;
+  }
+  else if (c.getString("hydro", "problem", "") == "shock_bubble" and c.getBool("shock_bubble", "use_inlet_bc", false))
+  {
+    cfg.inlet = G5_INLET_SHOCK_BUBBLE;
+    inlet_region = 0;
+  }
+  if (inlet_region >= 0)
+  {
+    const g5_region r = region_from_ini(c, inlet_region);
+    if (g5_region_conservative(&cfg, &r, cfg.inlet_u, cfg.dim == 2 ? 8 : 9) != G5_OK)
+      throw std::runtime_error("inlet region state: " + std::string(g5_last_error(nullptr)));
+  }
   return cfg;
 }
 

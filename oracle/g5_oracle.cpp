@@ -63,6 +63,13 @@ extern "C" struct g5o_params
   double  pinf[2];     // material p_inf
   double  thinc_beta;
   double  thinc_epsilon;
+  // inflow override of SolverGodunovFiveEq.cpp:997-1018: 0 none, 1 shock-bubble inlet
+  // (border_conditions/BCShockBubble.h), 2 jet (border_conditions/BCJet.h)
+  int32_t inlet;
+  int32_t pad1;
+  double  inlet_u[9]; // conservative inflow state
+  double  xmin[3];    // domain origin ([mesh] xmin, ymin, zmin)
+  double  jet[3];     // [jet] y, z, width
 };
 
 enum
@@ -750,6 +757,44 @@ struct Solver
             }
           }
         }
+      }
+    }
+    if (P.inlet != 0)
+      override_inlet(A);
+  }
+
+  // FillOutsideShockBubble / FillOutsideJet (border_conditions/BCShockBubble.cpp:72-172,
+  // BCJet.cpp), applied after the border fill (SolverGodunovFiveEq.cpp:997-1018): outside cells
+  // whose centre has x < xmin (for the jet: and lies in the jet, BCJet.h:71-88) take the inflow
+  // state.  The cell centre is xmin + (global cell index + 1/2) dx (the reference's coordinate
+  // helpers are in the absent core).
+  void
+  override_inlet(std::vector<double> & A)
+  {
+#pragma omp parallel for schedule(static)
+    for (int io = nowned; io < ntotal; ++io)
+    {
+      const int co[3] = { bcoord[3 * io], bcoord[3 * io + 1], bcoord[3 * io + 2] };
+      for (int cell = 0; cell < ncell; ++cell)
+      {
+        const int    cout[3] = { cell % b, (cell / b) % b, dim == 3 ? cell / (b * b) : 0 };
+        double       xc[3];
+        for (int d = 0; d < 3; ++d)
+          xc[d] = P.xmin[d] + (static_cast<double>(co[d] * b + cout[d]) + 0.5) * P.dx;
+        bool in = xc[0] < P.xmin[0];
+        if (in and P.inlet == 2)
+        {
+          if (dim == 2)
+            in = xc[1] > P.jet[0] - P.jet[2] / 2 and xc[1] < P.jet[0] + P.jet[2] / 2;
+          else
+          {
+            const double r2 = (xc[1] - P.jet[0]) * (xc[1] - P.jet[0]) + (xc[2] - P.jet[1]) * (xc[2] - P.jet[1]);
+            in = r2 < P.jet[2] * P.jet[2] / 4;
+          }
+        }
+        if (in)
+          for (int v = 0; v < nvar; ++v)
+            u(A, io, v, cell) = P.inlet_u[v];
       }
     }
   }

@@ -31,6 +31,11 @@ class OracleParams(ctypes.Structure):
         ("pinf", ctypes.c_double * 2),
         ("thinc_beta", ctypes.c_double),
         ("thinc_epsilon", ctypes.c_double),
+        ("inlet", ctypes.c_int32),
+        ("pad1", ctypes.c_int32),
+        ("inlet_u", ctypes.c_double * 9),
+        ("xmin", ctypes.c_double * 3),
+        ("jet", ctypes.c_double * 3),
     ]
 
 
@@ -103,6 +108,12 @@ def params_from(cfg):
     p.gamma[0], p.gamma[1] = cfg.gamma[0], cfg.gamma[1]
     p.pinf[0], p.pinf[1] = cfg.pinf[0], cfg.pinf[1]
     p.thinc_beta, p.thinc_epsilon = cfg.thinc_beta, cfg.thinc_epsilon
+    p.inlet = getattr(cfg, "inlet", 0)
+    for k in range(9):
+        p.inlet_u[k] = cfg.inlet_u[k] if p.inlet else 0.0
+    for k in range(3):
+        p.xmin[k] = cfg.xmin[k]
+        p.jet[k] = cfg.jet[k] if p.inlet else 0.0
     return p
 
 

@@ -98,3 +98,26 @@ def interface_advection_2d(alpha_min=0.1, alpha_max=0.9, pinf=0.0, rho0=1000.0, 
         return (V[:, 1] - 0.5 * d * (vx * vx + vy * vy) - Pi_) / xi_
 
     return cfg, U0, pressure
+
+
+def jet(dim=2, nb=None, inlet="jet", jet_geom=(0.5, 0.5, 0.25), **kw):
+    """Fuel jet into nitrogen after app/test_jet_2d_fuel_N2.ini (gammas, p_inf, region states),
+    on a fixed single-level mesh with WALL borders: region 0 (nitrogen, u = 2) everywhere, and
+    the outside cells with x < xmin take region 1 (fuel, u = 18) inside the jet (inlet "jet")
+    or all of them take region 0 (inlet "shock_bubble").  Returns (cfg, U0)."""
+    import numpy as np
+
+    nb = nb or ((6, 2) if dim == 2 else (3, 2, 2))
+    dx = 1.0 / (16 * nb[1])
+    r0 = g5.region(1.0, (32.2, 0.0), 3.83982e5, 2.0)
+    r1 = g5.region(0.0, (0.0, 623.0), 3.83982e5, 18.0)
+    opts = dict(bc=("WALL",) * 6, cfl=0.8, slope_type=2.0, gamma=(1.4, 4.4), pinf=(0.0, 8e7),
+                thinc=True, thinc_beta=1.5, thinc_epsilon=1e-3)
+    opts.update(kw)
+    base = g5.make_config(dim, nb, dx, **opts)
+    cfg = g5.make_config(dim, nb, dx, inlet=(inlet, r1 if inlet == "jet" else r0), jet=jet_geom, **opts)
+    u0 = g5.region_conservative(base, r0)
+    ncell = 16 ** dim
+    nblocks = int(np.prod(nb))
+    U0 = np.repeat(u0[None, :, None], nblocks, axis=0).repeat(ncell, axis=2).ravel()
+    return cfg, U0

@@ -49,6 +49,27 @@ def test_ini_3d_config(exe):
     assert kv["pinf"] == "0,600000000"
 
 
+def test_ini_jet_inlet(exe):
+    # [border] enable_jet: the jet inlet with the region-1 state and the [jet] geometry (defaults
+    # of border_conditions/BCJet.h:62-69 for the keys the file leaves out)
+    out = subprocess.run([exe, os.path.join(GOLD, "jet_fuel_n2_2d_fixed.ini")], check=True,
+                         capture_output=True, text=True).stdout
+    kv = parse(out)
+    assert kv["inlet"] == "2" and kv["nb"] == "12,4,1"
+    y, z, w = (float(x) for x in kv["jet"].split(","))
+    assert (y, z, w) == (0.5, 0.5, 0.25)
+    u = [float(x) for x in kv["inlet_u"].split(",")]
+    # region 1: alpha0 = 0, alpha_rho = (0, 623), p = 3.83982e5, u = 18 (fuel: gamma 4.4, pinf 8e7)
+    rho_e = (3.83982e5 + 4.4 * 8e7) / 3.4
+    assert u[0] == 623.0 and u[2] == 623.0 * 18.0 and u[3] == 0.0
+    assert u[4:] == [0.0, 0.0, 1.0, 623.0]
+    assert abs(u[1] - (rho_e + 0.5 * 623.0 * 18.0 ** 2)) <= 1e-12 * u[1]
+    # without the key no inlet is set
+    out = subprocess.run([exe, os.path.join(GOLD, "shock_bubble_air_helium_2d_512x128.ini")], check=True,
+                         capture_output=True, text=True).stdout
+    assert parse(out)["inlet"] == "0"
+
+
 @pytest.mark.gpu
 def test_cpp_solver_loop_runs(exe):
     print("[gpu] C++ SolverGodunovFiveEq on the 2-D fixture", flush=True)

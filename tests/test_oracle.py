@@ -478,3 +478,46 @@ def test_hllc_fast_speed_identity():
     # below smallc the reference form gives smallc, as max(c, smallc) does
     tiny = np.ldexp(1.0 + rng.random(10_000), rng.integers(-510, -34, 10_000))
     assert np.array_equal(np.sqrt(np.maximum(tiny * tiny, s * s)), np.full(tiny.size, s))
+
+
+@pytest.mark.parametrize("dim,kind", [(2, "jet"), (3, "jet"), (2, "shock_bubble"), (3, "shock_bubble")])
+def test_inlet_override_of_the_outside_cells(orc, dim, kind):
+    """fill_outside_quadrants (SolverGodunovFiveEq.cpp:997-1018): after the border fill, the outside
+    cells with x < xmin take the inflow state -- all of them for the shock-bubble inlet, those whose
+    centre lies in the jet (BCJet.h:71-88: |y - yj| < w/2 in 2-D, (y-yj)^2 + (z-zj)^2 < w^2/4 in
+    3-D) for the jet -- and every other outside cell keeps the plain border fill."""
+    geom = (0.4, 0.55, 0.3)
+    cfg, U0 = cases.jet(dim, inlet=kind, jet_geom=geom)
+    plain, _ = cases.jet(dim, inlet=kind, jet_geom=geom)
+    plain.inlet = 0
+    o = orc.Oracle(cfg)
+    o.set_state(U0)
+    o.fill_outside()
+    p = orc.Oracle(plain)
+    p.set_state(U0)
+    p.fill_outside()
+    nvar, b = o.nvar, 16
+    A = o.get_state_all().reshape(-1, nvar, b ** dim)
+    B = p.get_state_all().reshape(-1, nvar, b ** dim)
+    coords = o.block_coords()
+    assert np.array_equal(A[: o.nowned], B[: o.nowned])
+    cell = np.arange(b ** dim)
+    local = [cell % b, (cell // b) % b, cell // (b * b)]
+    inflow = np.array(cfg.inlet_u[:nvar])
+    n_over = 0
+    for ib in range(o.nowned, o.nowned + o.noutside):
+        centre = [cfg.xmin[d] + (coords[ib, d] * b + local[d] + 0.5) * cfg.dx for d in range(dim)]
+        expect = centre[0] < cfg.xmin[0]
+        if kind == "jet":
+            if dim == 2:
+                expect &= np.abs(centre[1] - geom[0]) < geom[2] / 2
+            else:
+                expect &= (centre[1] - geom[0]) ** 2 + (centre[2] - geom[1]) ** 2 < geom[2] ** 2 / 4
+        expect = np.broadcast_to(expect, cell.shape)
+        n_over += int(expect.sum())
+        assert np.array_equal(A[ib][:, expect], np.repeat(inflow[:, None], expect.sum(), axis=1))
+        assert np.array_equal(A[ib][:, ~expect], B[ib][:, ~expect])
+    if kind == "shock_bubble":  # every cell of the outside blocks left of x-min, corners included
+        assert n_over == (cfg.nb[1] + 2) * ((cfg.nb[2] + 2) if dim == 3 else 1) * b ** dim
+    else:
+        assert n_over > 0

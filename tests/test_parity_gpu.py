@@ -115,6 +115,19 @@ def test_parity_3d_thinc_walls_periodic(g5, orc):
     assert check_parity(U_g, U_o, 3, "3d air-water THINC walls + periodic y 8 steps")
 
 
+def test_parity_inlets(g5, orc):
+    # the jet inlet (slot in 2-D, disc in 3-D) and the shock-bubble inlet, with WALL borders:
+    # ghost cells left of x-min that take the inflow state, in the step kernels' on-the-fly
+    # border resolution (2-D halo gather, 3-D plane fetches of the x-min blocks)
+    for dim, inlet, steps in ((2, "jet", 40), (3, "jet", 10), (2, "shock_bubble", 30), (3, "shock_bubble", 6)):
+        cfg, U0 = cases.jet(dim, inlet=inlet)
+        U_g, U_o = run_both(cfg, U0, steps, g5, orc, f"{dim}d {inlet} inlet")
+        assert check_parity(U_g, U_o, dim, f"{dim}d {inlet} inlet {steps} steps")
+        if inlet == "jet":  # the fuel has entered the domain (alpha0 < 1 somewhere)
+            A = U_g.reshape(-1, 8 if dim == 2 else 9, 16 ** dim)
+            assert A[:, 4 if dim == 2 else 5].min() < 1.0 - 1e-6
+
+
 def test_compute_dt_and_host_steps(g5, orc):
     # compute_dt + do_time_step (the drop-in GodunovImplemV0 surface) against the oracle
     cfg, U0 = cases.air_helium_2d(nbx=8, nby=4)
@@ -134,12 +147,15 @@ def test_compute_dt_and_host_steps(g5, orc):
 
 
 def test_fill_outside_matches_oracle(g5, orc):
-    for dim in (2, 3):
-        if dim == 2:
+    runs = [(2, None), (3, None), (2, "jet"), (3, "jet"), (2, "shock_bubble")]
+    for dim, inlet in runs:
+        if inlet is not None:  # the inflow overrides after the border fill
+            cfg, U0 = cases.jet(dim, inlet=inlet, jet_geom=(0.4, 0.55, 0.3))
+        elif dim == 2:
             cfg, U0 = cases.air_helium_2d(nbx=3, nby=2, bc=("WALL", "ZERO_GRADIENT", "WALL", "ZERO_GRADIENT", "NONE", "NONE"))
         else:
             cfg, U0 = cases.air_helium_3d(nb=2, bc=("WALL", "ZERO_GRADIENT", "ZERO_GRADIENT", "WALL", "WALL", "ZERO_GRADIENT"))
-        _announce(f"fill_outside dim={dim}")
+        _announce(f"fill_outside dim={dim} inlet={inlet}")
         o = orc.Oracle(cfg)
         o.set_state(U0)
         o.fill_outside()
