@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libg5.so")
 
 # return codes and enums of include/godunov_five_eq.h
-G5_OK, G5_ERR_INVALID, G5_ERR_HIP, G5_ERR_UNSUPPORTED, G5_ERR_NOMEM, G5_ERR_STATE = range(6)
+G5_OK, G5_ERR_INVALID, G5_ERR_HIP, G5_ERR_UNSUPPORTED, G5_ERR_NOMEM, G5_ERR_STATE, G5_ERR_IO = range(7)
 BC_NONE, BC_PERIODIC, BC_ZERO_GRADIENT, BC_WALL = range(4)
 TI_HANCOCK, TI_RK1, TI_RK2_SSP = range(3)
 PREC_FP64, PREC_MIXED = range(2)

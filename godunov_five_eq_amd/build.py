@@ -7,11 +7,15 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libg5.so")
+H5LIB = os.path.join(HERE, "libg5h5.so")
+# HDF5 1.10 C library of the image (headers, libhdf5 and h5dump under /opt/conda)
+HDF5_PREFIX = os.environ.get("G5_HDF5_PREFIX", "/opt/conda")
 
 HIP_SOURCES = ["g5_step2d.hip", "g5_step3d.hip", "g5_aux.hip", "g5_runtime.hip"]
 CXX_SOURCES = ["g5_init.cpp", "g5_workloads.cpp"]
@@ -41,20 +45,36 @@ def _run(cmd, verbose):
         print(r.stdout)
 
 
+def build_hdf5(verbose=False):
+    """libg5h5.so: HDF5 + XDMF output and restart (host code, kept out of libg5.so)."""
+    inc = os.path.join(HDF5_PREFIX, "include")
+    libdir = os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        raise RuntimeError(f"hdf5.h not found under {inc}: set G5_HDF5_PREFIX")
+    tmp = H5LIB + ".tmp"
+    _run(["g++", *CXX_FLAGS, "-fopenmp", "-shared", "-I", inc, os.path.join(CSRC, "g5_hdf5.cpp"), "-o", tmp,
+          "-L", libdir, "-lhdf5", "-Wl,-rpath," + libdir], verbose)
+    shutil.move(tmp, H5LIB)
+    return H5LIB
+
+
 def build(verbose=False, extra_hip_flags=()):
     os.makedirs(OBJ, exist_ok=True)
-    objs = []
+    jobs = []
     for src in HIP_SOURCES:
         obj = os.path.join(OBJ, src + ".o")
-        _run([HIPCC, *HIP_FLAGS, *extra_hip_flags, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        jobs.append((obj, [HIPCC, *HIP_FLAGS, *extra_hip_flags, "-c", os.path.join(CSRC, src), "-o", obj]))
     for src in CXX_SOURCES:
         obj = os.path.join(OBJ, src + ".o")
-        _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        jobs.append((obj, ["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]))
+    # translation units in parallel (the 3-D step file dominates)
+    with ThreadPoolExecutor(max_workers=min(len(jobs), max(1, min(8, os.cpu_count() or 1)))) as ex:
+        list(ex.map(lambda j: _run(j[1], verbose), jobs))
+    objs = [o for o, _ in jobs]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     shutil.move(tmp, LIB)
+    build_hdf5(verbose)
     return LIB
 
 

@@ -1,8 +1,11 @@
-"""Per-solve checkpoint and restart: the state as .npy (U[block][var][cell], FP64) and a JSON
-header (configuration, time, steps done, end time, step budget).  A restored solve continues
-bit-identically (g5_pool_set_clock).  This restates, for one solve, what the reference's HDF5
-restart does (SolverGodunovFiveEq.cpp:811-846, init/init.cpp:132-207); the files are not HDF5
-(no HDF5 library is available to this build)."""
+"""Per-solve checkpoint and restart of a pool solve.  A restored solve continues bit-identically
+(g5_pool_set_clock).  This restates, for one solve, what the reference's HDF5 restart does
+(SolverGodunovFiveEq.cpp:811-846, init/init.cpp:132-207), in two forms:
+  save / load: the state as .npy (U[block][var][cell], FP64) and a JSON header (configuration,
+      time, steps done, end time, step budget) -- self-contained;
+  save_hdf5 / load_hdf5: the HDF5 + XDMF file of hdf5io (variables by name, run time and
+      iteration); the configuration and the limits come from the caller, as the reference takes
+      them from the .ini."""
 import json
 
 import numpy as np
@@ -57,5 +60,25 @@ def load(prefix, pool, start=True):
     pool.upload(sid, U)
     pool.set_clock(sid, h["t"], h["steps"])
     if start and h["active"]:
+        pool.start([sid])
+    return sid
+
+
+def save_hdf5(directory, basename, pool, sid):
+    """Write directory/basename.h5 and .xmf for solve `sid` of `pool` (waits for its steps)."""
+    from . import hdf5io
+    t, steps, _ = pool.status(sid)
+    return hdf5io.write(directory, basename, pool.config(sid), pool.download(sid), t, steps)
+
+
+def load_hdf5(path, pool, cfg, t_end, max_steps, start=True):
+    """Admit a solve of configuration `cfg` into `pool` and restore state, time and step count from
+    an HDF5 checkpoint of the same mesh; start it unless asked not to.  Returns the solve id."""
+    from . import hdf5io
+    U, t, steps = hdf5io.read(path, cfg)
+    sid = pool.admit(cfg, t_end, max_steps)
+    pool.upload(sid, U)
+    pool.set_clock(sid, t, steps)
+    if start:
         pool.start([sid])
     return sid

@@ -35,7 +35,8 @@ enum
   G5_ERR_HIP = 2,         /* a HIP runtime call failed */
   G5_ERR_UNSUPPORTED = 3, /* valid configuration this build does not implement */
   G5_ERR_NOMEM = 4,       /* allocation failed or HBM budget exceeded */
-  G5_ERR_STATE = 5        /* call out of order (e.g. step before upload) */
+  G5_ERR_STATE = 5,       /* call out of order (e.g. step before upload) */
+  G5_ERR_IO = 6           /* file input/output failed (godunov_five_eq_io.h) */
 };
 
 /* border conditions, border_conditions/FillOutside.h:51-55 (BC_HYDRO) */

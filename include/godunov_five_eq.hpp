@@ -10,7 +10,8 @@
 //     and do_time_step(U, U2, t, dt) of scheme/GodunovImplemBase.h:93-107;
 //   - SolverGodunovFiveEq<dim>: compute_dt_local (SolverGodunovFiveEq.cpp:368-390),
 //     godunov_unsplit (:481-526), next_iteration_impl (:395-476) and the run loop with the
-//     Mcell-updates/s accounting of :474 and :585-590.
+//     Mcell-updates/s accounting of :474 and :585-590; save_solution_hdf5 (:773-917) and
+//     init_restart (init/init.cpp:132-207) over libg5h5.so (godunov_five_eq_io.h).
 // Errors are reported with std::runtime_error carrying g5_last_error().
 #pragma once
 
@@ -18,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <fstream>
 #include <map>
 #include <memory>
@@ -28,6 +30,7 @@
 #include <vector>
 
 #include "godunov_five_eq.h"
+#include "godunov_five_eq_io.h" // save_solution_hdf5 / init_restart need libg5h5.so at link time
 
 namespace g5cpp
 {
@@ -78,6 +81,12 @@ public:
     return from_string(ss.str());
   }
 
+  void
+  setString(std::string const & section, std::string const & key, std::string const & value)
+  {
+    m_values[section + "/" + key] = value;
+  }
+
   bool
   has(std::string const & section, std::string const & key) const
   {
@@ -114,6 +123,26 @@ public:
     std::string v = it->second;
     std::transform(v.begin(), v.end(), v.begin(), ::tolower);
     return v == "1" or v == "yes" or v == "true" or v == "on";
+  }
+
+  // comma-separated list, blanks around the items removed ([output] write_variables=a,b,c)
+  std::vector<std::string>
+  getStringVector(std::string const & section, std::string const & key, std::vector<std::string> const & def) const
+  {
+    const auto it = m_values.find(section + "/" + key);
+    if (it == m_values.end())
+      return def;
+    std::vector<std::string> out;
+    std::stringstream        ss(it->second);
+    std::string              item;
+    while (std::getline(ss, item, ','))
+    {
+      const auto a = item.find_first_not_of(" \t");
+      const auto b = item.find_last_not_of(" \t");
+      if (a != std::string::npos)
+        out.push_back(item.substr(a, b - a + 1));
+    }
+    return out;
   }
 
   std::vector<double>
@@ -364,7 +393,52 @@ public:
     , m_impl(m_cfg)
     , m_tEnd(config_map.getReal("run", "tEnd", 1.0))
     , m_nStepmax(config_map.getInteger("run", "nStepmax", 1000))
+    , m_outputDir(config_map.getString("output", "outputDir", "./"))
+    , m_outputPrefix(config_map.getString("output", "outputPrefix", "output"))
+    , m_write_variables(config_map.getStringVector("output", "write_variables", {}))
+    , m_restart_prefix(config_map.getString("restart", "filePrefix", ""))
   {}
+
+  // HDF5 + XDMF output of the state (save_solution_hdf5, SolverGodunovFiveEq.cpp:773-917):
+  // <outputDir>/<outputPrefix>_<iteration, 7 digits>.h5 and .xmf.  A checkpoint holds every
+  // conservative variable; otherwise the conservative variables named in [output]
+  // write_variables (derived quantities are not written; with none named, every variable).
+  // Returns the .h5 path.
+  std::string
+  save_solution_hdf5(bool checkpoint = true)
+  {
+    uint32_t mask = 0;
+    if (!checkpoint)
+      for (int v = 0; v < (dim == 2 ? 8 : 9); ++v)
+        if (std::find(m_write_variables.begin(), m_write_variables.end(), g5_h5_var_name(dim, v)) !=
+            m_write_variables.end())
+          mask |= 1u << v;
+    char it[16];
+    std::snprintf(it, sizeof(it), "%07d", m_iteration);
+    const std::string base = m_outputPrefix + "_" + it;
+    const auto        U = get_state();
+    if (g5_h5_write(m_outputDir.c_str(), base.c_str(), &m_cfg, U.data(), U.size(), m_t, m_iteration, mask) != G5_OK)
+      throw std::runtime_error(std::string("save_solution_hdf5: ") + g5_h5_last_error());
+    return m_outputDir + "/" + base + ".h5";
+  }
+
+  // restart (init_restart, init/init.cpp:132-207): state, time and iteration from
+  // <outputDir>/<[restart] filePrefix>.h5, a checkpoint of the same mesh
+  void
+  init_restart()
+  {
+    if (m_restart_prefix.empty())
+      throw std::runtime_error("init_restart: [restart] filePrefix is not set");
+    const std::string path = m_outputDir + "/" + m_restart_prefix + ".h5";
+    std::vector<double> U(num_cells() * (dim == 2 ? 8 : 9));
+    double              t = 0.0;
+    int64_t             it = 0;
+    if (g5_h5_read(path.c_str(), &m_cfg, U.data(), U.size(), &t, &it) != G5_OK)
+      throw std::runtime_error(std::string("init_restart: ") + g5_h5_last_error());
+    set_state(U);
+    m_t = t;
+    m_iteration = static_cast<int>(it);
+  }
 
   void
   set_state(std::vector<double> const & U)
@@ -457,6 +531,9 @@ private:
   GodunovImplemHip<dim>  m_impl;
   double                 m_tEnd;
   int                    m_nStepmax;
+  std::string              m_outputDir, m_outputPrefix;
+  std::vector<std::string> m_write_variables;
+  std::string              m_restart_prefix;
   double                 m_t = 0.0;
   int                    m_iteration = 0;
   uint64_t               m_cell_updates = 0;

@@ -31,6 +31,45 @@ run(g5cpp::ConfigMap const & cm, int nsteps)
   return 0;
 }
 
+// checkpoint after n1 iterations, restart a second solver from the file, continue both n2
+// iterations: the states, times and iterations must be identical
+template <int dim>
+int
+run_restart(g5cpp::ConfigMap cm, std::string const & dir, int n1, int n2)
+{
+  cm.setString("output", "outputDir", dir);
+  cm.setString("output", "outputPrefix", "ckpt");
+  g5cpp::SolverGodunovFiveEq<dim> a(cm);
+  g5_config const &               cfg = a.config();
+  g5_region                       regs[3];
+  for (int r = 0; r < 3; ++r)
+    regs[r] = g5cpp::region_from_ini(cm, r);
+  double const c[3] = { cm.getReal("shock_bubble", "bubble_x", 0.5), cm.getReal("shock_bubble", "bubble_y", 0.5),
+                        cm.getReal("shock_bubble", "bubble_z", 0.5) };
+  std::vector<double> U(a.num_cells() * (dim == 2 ? 8 : 9));
+  g5cpp::check(g5_init_shock_bubble(&cfg, regs, cm.getReal("shock_bubble", "x_front", 0.2), c,
+                                    cm.getReal("shock_bubble", "bubble_radius", 0.1), U.data(), U.size()),
+               nullptr);
+  a.set_state(U);
+  for (int n = 0; n < n1; ++n)
+    a.next_iteration_impl();
+  const std::string h5 = a.save_solution_hdf5(true);
+  for (int n = 0; n < n2; ++n)
+    a.next_iteration_impl();
+  char prefix[32];
+  std::snprintf(prefix, sizeof(prefix), "ckpt_%07d", n1);
+  cm.setString("restart", "filePrefix", prefix);
+  g5cpp::SolverGodunovFiveEq<dim> b(cm);
+  b.init_restart();
+  const int it_restart = b.iteration();
+  for (int n = 0; n < n2; ++n)
+    b.next_iteration_impl();
+  const bool same = a.get_state() == b.get_state() and a.time() == b.time() and a.iteration() == b.iteration();
+  std::printf("file=%s restart_iteration=%d iteration=%d t=%.17g identical=%d\n", h5.c_str(), it_restart,
+              b.iteration(), b.time(), same ? 1 : 0);
+  return same ? 0 : 1;
+}
+
 // a few solves of the seeded batch in one pool through the C++ PoolHip class
 int
 run_pool(int nsolves)
@@ -83,6 +122,8 @@ main(int argc, char ** argv)
   const auto r0 = g5cpp::region_from_ini(cm, 0);
   std::printf("region0 alpha0=%g alpha_rho=%.17g,%.17g p=%.17g u=%.17g\n", r0.alpha0, r0.alpha_rho[0], r0.alpha_rho[1],
               r0.p, r0.u);
+  if (argc >= 4 and std::strcmp(argv[2], "--restart") == 0)
+    return cfg.dim == 2 ? run_restart<2>(cm, argv[3], 30, 20) : run_restart<3>(cm, argv[3], 6, 4);
   if (argc >= 4 and std::strcmp(argv[2], "--run") == 0)
     return cfg.dim == 2 ? run<2>(cm, std::atoi(argv[3])) : run<3>(cm, std::atoi(argv[3]));
   return 0;

@@ -10,10 +10,10 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+def declared_functions(headers=None):
     names = set()
     for fn in os.listdir(os.path.join(ROOT, "include")):
-        if fn.endswith(".h"):
+        if fn.endswith(".h") and (headers is None or fn in headers):
             src = open(os.path.join(ROOT, "include", fn)).read()
             src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
             names |= set(re.findall(r"\b(g5_[a-z0-9_]+)\s*\(", src))
@@ -21,13 +21,23 @@ def declared_functions():
 
 
 def test_library_exports_every_declared_symbol(g5):
+    # libg5.so: godunov_five_eq.h; libg5h5.so: godunov_five_eq_io.h
+    from godunov_five_eq_amd import hdf5io
+    assert set(declared_functions()) == set(declared_functions(["godunov_five_eq.h"])) | set(
+        declared_functions(["godunov_five_eq_io.h"]))
     L = g5.lib()
-    declared = declared_functions()
+    declared = declared_functions(["godunov_five_eq.h"])
     assert len(declared) >= 15
     for name in declared:
         assert hasattr(L, name), name
     assert set(g5.exported_symbols()) <= set(declared)
     assert b"gfx950" in L.g5_version()
+    H = hdf5io.lib()
+    declared_io = declared_functions(["godunov_five_eq_io.h"])
+    assert len(declared_io) == 4
+    for name in declared_io:
+        assert hasattr(H, name), name
+    assert set(hdf5io.exported_symbols()) == set(declared_io)
 
 
 def test_invalid_configs_are_rejected(g5):

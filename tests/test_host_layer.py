@@ -16,7 +16,7 @@ def exe():
     os.makedirs(os.path.dirname(EXE), exist_ok=True)
     libdir = os.path.join(ROOT, "godunov_five_eq_amd")
     subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-o", EXE, os.path.join(ROOT, "tests", "cpp", "host_layer.cpp"),
-                    "-L", libdir, "-lg5", "-Wl,-rpath," + libdir], check=True)
+                    "-L", libdir, "-lg5", "-lg5h5", "-Wl,-rpath," + libdir], check=True)
     return EXE
 
 
@@ -80,6 +80,20 @@ def test_cpp_solver_loop_runs(exe):
     kv = parse(out.stdout.splitlines()[-1])
     assert kv["steps"] == "200"
     assert float(kv["Mcell-updates/s"]) > 0
+
+
+@pytest.mark.gpu
+def test_cpp_checkpoint_restart(exe, tmp_path):
+    # save_solution_hdf5 after 30 iterations, init_restart of a second solver from that file, 20
+    # more iterations on each: identical states, times and iteration counts
+    print("[gpu] C++ save_solution_hdf5 + init_restart on the 2-D fixture", flush=True)
+    out = subprocess.run([exe, os.path.join(GOLD, "shock_bubble_air_helium_2d_512x128.ini"), "--restart", str(tmp_path)],
+                         capture_output=True, text=True, timeout=120)
+    sys.stdout.write(out.stdout + out.stderr)
+    assert out.returncode == 0
+    kv = parse(out.stdout.splitlines()[-1])
+    assert kv["identical"] == "1" and kv["restart_iteration"] == "30" and kv["iteration"] == "50"
+    assert os.path.exists(os.path.join(tmp_path, "ckpt_0000030.xmf"))
 
 
 @pytest.mark.gpu

@@ -225,6 +225,36 @@ def test_checkpoint_restart_is_bit_identical(g5, tmp_path):
             assert np.array_equal(pool.download(sid), ref)
 
 
+def test_hdf5_checkpoint_restart_is_bit_identical(g5, tmp_path):
+    # the HDF5 form: a 3-D THINC solve saved after 5 steps (variables by name, time and step count
+    # in the file) and restored into a fresh pool finishes its 12-step budget with the bits of the
+    # uninterrupted run
+    from godunov_five_eq_amd import checkpoint
+
+    say("HDF5 checkpoint/restart 3d")
+    cfg, U0 = cases.air_water_3d(nb=3)
+    with g5.Pool(3, thinc=True, budget_bytes=1 << 28) as pool:
+        sid = pool.admit(cfg, max_steps=12)
+        pool.upload(sid, U0)
+        pool.start([sid])
+        pool.step(15)
+        ref = pool.download(sid)
+        t_ref, n_ref, _ = pool.status(sid)
+    with g5.Pool(3, thinc=True, budget_bytes=1 << 28) as pool:
+        sid = pool.admit(cfg, max_steps=12)
+        pool.upload(sid, U0)
+        pool.start([sid])
+        pool.step(5)
+        path = checkpoint.save_hdf5(str(tmp_path), "solve_0000005", pool, sid)
+    with g5.Pool(3, thinc=True, budget_bytes=1 << 28) as pool:
+        sid = checkpoint.load_hdf5(path, pool, cfg, 1e300, 12)
+        assert pool.status(sid)[1] == 5
+        pool.step(15)
+        t, n, active = pool.status(sid)
+        assert n == n_ref == 12 and t == t_ref and not active
+        assert np.array_equal(pool.download(sid), ref)
+
+
 def test_pool_error_paths(g5):
     say("pool error paths")
     cfg2, U0 = cases.air_helium_2d(nbx=4, nby=2)
