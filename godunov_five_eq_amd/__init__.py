@@ -160,6 +160,7 @@ def lib():
             "g5_last_run_ms": (ctypes.c_int, [vp, P(d)]),
             "g5_time_step_kernel": (ctypes.c_int, [vp, i32, d, P(d)]),
             "g5_init_shock_bubble": (ctypes.c_int, [P(G5Config), P(G5Region), d, P(d), d, P(d), sz]),
+            "g5_init_triple_point": (ctypes.c_int, [P(G5Config), P(G5Region), d, d, P(d), sz]),
             "g5_region_conservative": (ctypes.c_int, [P(G5Config), P(G5Region), P(d), sz]),
             "g5_pool_create": (ctypes.c_int, [i32, i32, i32, ctypes.c_uint64, P(vp)]),
             "g5_pool_destroy": (ctypes.c_int, [vp]),
@@ -207,7 +208,8 @@ def exported_symbols():
         "g5_version", "g5_last_error", "g5_create", "g5_destroy", "g5_shape", "g5_mem_bytes",
         "g5_upload_state", "g5_download_state", "g5_state_device_ptr", "g5_compute_dt",
         "g5_fill_outside", "g5_outside_coords", "g5_do_time_step", "g5_run", "g5_synchronize", "g5_last_run_ms",
-        "g5_time_step_kernel", "g5_init_shock_bubble", "g5_rankine_hugoniot", "g5_region_conservative",
+        "g5_time_step_kernel", "g5_init_shock_bubble", "g5_init_triple_point", "g5_rankine_hugoniot",
+        "g5_region_conservative",
         "g5_pool_create", "g5_pool_destroy", "g5_pool_last_error", "g5_pool_admit", "g5_pool_release",
         "g5_pool_upload", "g5_pool_download", "g5_pool_init_shock_bubble", "g5_pool_init_triple_point",
         "g5_pool_start", "g5_pool_step", "g5_pool_status", "g5_pool_finished", "g5_pool_usage",
@@ -304,6 +306,17 @@ def init_shock_bubble(cfg, regions, x_front, center, radius):
     rc = lib().g5_init_shock_bubble(ctypes.byref(cfg), regs, x_front, c3, radius, _dptr(U), n)
     if rc != G5_OK:
         raise G5Error(rc, "g5_init_shock_bubble")
+    return U
+
+
+def init_triple_point(cfg, regions, xd, yd):
+    """Host initial condition of InitTriplePointDataFunctor -> state array [block][var][cell]."""
+    n = state_size(cfg)
+    U = np.empty(n, dtype=np.float64)
+    regs = (G5Region * 3)(*regions)
+    rc = lib().g5_init_triple_point(ctypes.byref(cfg), regs, xd, yd, _dptr(U), n)
+    if rc != G5_OK:
+        raise G5Error(rc, "g5_init_triple_point")
     return U
 
 

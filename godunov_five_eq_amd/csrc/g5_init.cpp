@@ -180,6 +180,61 @@ g5_init_shock_bubble(const g5_config * cfg, const g5_region * regions3, double x
   return G5_OK;
 }
 
+// init/InitTriplePoint.cpp:30-90 (InitTriplePointDataFunctor): region 0 for x <= xd, else
+// region 1 where y <= yd (sqrt(y^2 + z^2) <= yd in 3-D) and region 2 elsewhere, at the cell
+// centre; the same arithmetic as the device generator (kind 2 of k_init_ic)
+int
+g5_init_triple_point(const g5_config * cfg, const g5_region * regions3, double xd, double yd, double * U_host,
+                     size_t n)
+{
+  if (not cfg or not regions3 or not U_host)
+    return G5_ERR_INVALID;
+  const int dim = cfg->dim;
+  if ((dim != 2 and dim != 3) or cfg->bsize < 1)
+    return G5_ERR_INVALID;
+  const int    b = cfg->bsize;
+  const int    nvar = dim == 2 ? 8 : 9;
+  const int    ncell = dim == 2 ? b * b : b * b * b;
+  const int    nx = cfg->nb[0], ny = cfg->nb[1], nz = dim == 3 ? cfg->nb[2] : 1;
+  const size_t need = static_cast<size_t>(nx) * ny * nz * nvar * ncell;
+  if (n != need)
+    return G5_ERR_INVALID;
+  const Mix m(*cfg);
+  double    S[3][9];
+  for (int r = 0; r < 3; ++r)
+    region_to_cons(*cfg, m, regions3[r], S[r]);
+  const int IA0 = dim == 2 ? 4 : 5, IAD0 = IA0 + 1, IA1 = IA0 + 2, IAD1 = IA0 + 3, ID = 0;
+  for (long blk = 0; blk < static_cast<long>(nx) * ny * nz; ++blk)
+  {
+    const int bx = static_cast<int>(blk % nx), by = static_cast<int>((blk / nx) % ny),
+              bz = static_cast<int>(blk / (static_cast<long>(nx) * ny));
+    double * Ub = U_host + static_cast<size_t>(blk) * nvar * ncell;
+    for (int cell = 0; cell < ncell; ++cell)
+    {
+      const int    i = cell % b, j = (cell / b) % b, k = dim == 3 ? cell / (b * b) : 0;
+      const double lo[3] = { cfg->xmin[0] + (bx * b + i) * cfg->dx, cfg->xmin[1] + (by * b + j) * cfg->dx,
+                             cfg->xmin[2] + (bz * b + k) * cfg->dx };
+      const double xc = lo[0] + 0.5 * cfg->dx;
+      int          r = 0;
+      if (xc > xd)
+      {
+        const double yc = lo[1] + 0.5 * cfg->dx;
+        const double zc = lo[2] + 0.5 * cfg->dx;
+        const bool   in1 = dim == 2 ? (yc <= yd) : (std::sqrt(yc * yc + zc * zc) <= yd);
+        r = in1 ? 1 : 2;
+      }
+      double u[9];
+      for (int v = 0; v < nvar; ++v)
+        u[v] = S[r][v];
+      u[ID] = u[IAD0] + u[IAD1];
+      u[IA1] = 1.0 - u[IA0];
+      for (int v = 0; v < nvar; ++v)
+        Ub[v * ncell + cell] = u[v];
+    }
+  }
+  return G5_OK;
+}
+
 // Rankine-Hugoniot jump of a shock moving into a stiffened gas at rest.  With P = p + pinf the
 // stiffened gas obeys the ideal-gas relations in P (app/tools/moving_shock.py:79-163 is the
 // pinf = 0 case): P2/P1 = 1 + 2 gamma/(gamma+1) (M^2 - 1), rho2/rho1 = ((g+1)P2 + (g-1)P1) /

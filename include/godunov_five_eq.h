@@ -205,6 +205,13 @@ int g5_region_conservative(const g5_config * cfg, const g5_region * region, doub
 int g5_init_shock_bubble(const g5_config * cfg, const g5_region * regions3, double x_front,
                          const double * center3, double radius, double * U_host, size_t n);
 
+/* Triple-point initial condition of InitTriplePointDataFunctor (init/InitTriplePoint.cpp:30-90):
+ * region 0 for x <= xd, else region 1 where y <= yd (sqrt(y^2 + z^2) <= yd in 3-D), region 2
+ * elsewhere, at cell centres.  Fills U_host (owned blocks); the same states as the device
+ * generator g5_pool_init_triple_point. */
+int g5_init_triple_point(const g5_config * cfg, const g5_region * regions3, double xd, double yd, double * U_host,
+                         size_t n);
+
 /* Post-shock state of a planar shock moving into a resting stiffened gas (Rankine-Hugoniot);
  * for p_inf = 0 this is app/tools/moving_shock.py:79-163.  Either mach > 0 (shock Mach number) or
  * p2 > 0 (post-shock pressure) is given; outputs rho2, u2, p2 and the shock speed. */

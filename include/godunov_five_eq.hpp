@@ -309,6 +309,54 @@ check(int rc, g5_ctx const * ctx)
     throw std::runtime_error(std::string("g5: ") + g5_last_error(ctx));
 }
 
+// The initial state of [hydro] problem on the owned blocks, U[block][var][cell] (the data functors
+// dispatched by init/init.cpp:37-118, host side):
+//   shock_bubble  InitShockBubble.cpp:50-225, [shock_bubble] x_front, bubble_x, bubble_y,
+//                 bubble_z, bubble_radius, regions 0 (post-shock), 1 (pre-shock), 2 (bubble);
+//   triple_point  InitTriplePoint.cpp:30-90, [triple_point] xd, yd (defaults 1 and 1.5, the
+//                 7x3 domain of app/test_triple_point_2d.ini), regions 0, 1, 2;
+//   jet           InitJet.cpp:45-72, region 0 everywhere (the jet enters through the inlet).
+// The parameter structs of the other problems are in the kalypsso core, which is not part of the
+// reference tree: they raise.
+inline std::vector<double>
+initial_state(ConfigMap const & c, g5_config const & cfg)
+{
+  const std::string problem = c.getString("hydro", "problem", "");
+  const int         nvar = cfg.dim == 2 ? 8 : 9;
+  const size_t      ncell = static_cast<size_t>(cfg.dim == 2 ? cfg.bsize * cfg.bsize : cfg.bsize * cfg.bsize * cfg.bsize);
+  size_t            n = static_cast<size_t>(nvar) * ncell;
+  for (int d = 0; d < cfg.dim; ++d)
+    n *= static_cast<size_t>(cfg.nb[d]);
+  std::vector<double> U(n);
+  g5_region           regs[3];
+  if (problem == "shock_bubble" or problem == "triple_point")
+    for (int r = 0; r < 3; ++r)
+      regs[r] = region_from_ini(c, r);
+  if (problem == "shock_bubble")
+  {
+    const double ctr[3] = { c.getReal("shock_bubble", "bubble_x", 0.5), c.getReal("shock_bubble", "bubble_y", 0.5),
+                            c.getReal("shock_bubble", "bubble_z", 0.5) };
+    check(g5_init_shock_bubble(&cfg, regs, c.getReal("shock_bubble", "x_front", 0.2), ctr,
+                               c.getReal("shock_bubble", "bubble_radius", 0.1), U.data(), U.size()),
+          nullptr);
+  }
+  else if (problem == "triple_point")
+    check(g5_init_triple_point(&cfg, regs, c.getReal("triple_point", "xd", 1.0), c.getReal("triple_point", "yd", 1.5),
+                               U.data(), U.size()),
+          nullptr);
+  else if (problem == "jet")
+  {
+    const g5_region r0 = region_from_ini(c, 0);
+    double          u[9];
+    check(g5_region_conservative(&cfg, &r0, u, nvar), nullptr);
+    for (size_t k = 0; k < n; ++k)
+      U[k] = u[(k / ncell) % nvar];
+  }
+  else
+    throw std::runtime_error("problem '" + problem + "' is not available on this path (shock_bubble, triple_point, jet)");
+  return U;
+}
+
 // ---------------------------------------------------------------------------------------------
 // State handle: the device-resident conservative state of a context (the solver's U / U2).
 // ---------------------------------------------------------------------------------------------
@@ -389,7 +437,8 @@ class SolverGodunovFiveEq
 {
 public:
   explicit SolverGodunovFiveEq(ConfigMap const & config_map)
-    : m_cfg(config_from_ini(config_map))
+    : m_config_map(config_map)
+    , m_cfg(config_from_ini(config_map))
     , m_impl(m_cfg)
     , m_tEnd(config_map.getReal("run", "tEnd", 1.0))
     , m_nStepmax(config_map.getInteger("run", "nStepmax", 1000))
@@ -420,6 +469,17 @@ public:
     if (g5_h5_write(m_outputDir.c_str(), base.c_str(), &m_cfg, U.data(), U.size(), m_t, m_iteration, mask) != G5_OK)
       throw std::runtime_error(std::string("save_solution_hdf5: ") + g5_h5_last_error());
     return m_outputDir + "/" + base + ".h5";
+  }
+
+  // init (init/init.cpp:37-118): the restart when [restart] enabled, else the initial state of
+  // [hydro] problem (initial_state)
+  void
+  init()
+  {
+    if (m_config_map.getBool("restart", "enabled", false))
+      init_restart();
+    else
+      set_state(initial_state(m_config_map, m_cfg));
   }
 
   // restart (init_restart, init/init.cpp:132-207): state, time and iteration from
@@ -527,6 +587,7 @@ public:
   }
 
 private:
+  ConfigMap              m_config_map;
   g5_config              m_cfg;
   GodunovImplemHip<dim>  m_impl;
   double                 m_tEnd;

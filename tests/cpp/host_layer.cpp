@@ -12,17 +12,7 @@ int
 run(g5cpp::ConfigMap const & cm, int nsteps)
 {
   g5cpp::SolverGodunovFiveEq<dim> solver(cm);
-  g5_config const &               cfg = solver.config();
-  g5_region                       regs[3];
-  for (int r = 0; r < 3; ++r)
-    regs[r] = g5cpp::region_from_ini(cm, r);
-  double const c[3] = { cm.getReal("shock_bubble", "bubble_x", 0.5), cm.getReal("shock_bubble", "bubble_y", 0.5),
-                        cm.getReal("shock_bubble", "bubble_z", 0.5) };
-  std::vector<double> U(solver.num_cells() * (dim == 2 ? 8 : 9));
-  g5cpp::check(g5_init_shock_bubble(&cfg, regs, cm.getReal("shock_bubble", "x_front", 0.2), c,
-                                    cm.getReal("shock_bubble", "bubble_radius", 0.1), U.data(), U.size()),
-               nullptr);
-  solver.set_state(U);
+  solver.init();
   for (int n = 0; n < 2; ++n)
     solver.next_iteration_impl();
   const double mcups = solver.run();
@@ -40,17 +30,7 @@ run_restart(g5cpp::ConfigMap cm, std::string const & dir, int n1, int n2)
   cm.setString("output", "outputDir", dir);
   cm.setString("output", "outputPrefix", "ckpt");
   g5cpp::SolverGodunovFiveEq<dim> a(cm);
-  g5_config const &               cfg = a.config();
-  g5_region                       regs[3];
-  for (int r = 0; r < 3; ++r)
-    regs[r] = g5cpp::region_from_ini(cm, r);
-  double const c[3] = { cm.getReal("shock_bubble", "bubble_x", 0.5), cm.getReal("shock_bubble", "bubble_y", 0.5),
-                        cm.getReal("shock_bubble", "bubble_z", 0.5) };
-  std::vector<double> U(a.num_cells() * (dim == 2 ? 8 : 9));
-  g5cpp::check(g5_init_shock_bubble(&cfg, regs, cm.getReal("shock_bubble", "x_front", 0.2), c,
-                                    cm.getReal("shock_bubble", "bubble_radius", 0.1), U.data(), U.size()),
-               nullptr);
-  a.set_state(U);
+  a.init();
   for (int n = 0; n < n1; ++n)
     a.next_iteration_impl();
   const std::string h5 = a.save_solution_hdf5(true);
@@ -59,8 +39,9 @@ run_restart(g5cpp::ConfigMap cm, std::string const & dir, int n1, int n2)
   char prefix[32];
   std::snprintf(prefix, sizeof(prefix), "ckpt_%07d", n1);
   cm.setString("restart", "filePrefix", prefix);
+  cm.setString("restart", "enabled", "true");
   g5cpp::SolverGodunovFiveEq<dim> b(cm);
-  b.init_restart();
+  b.init();
   const int it_restart = b.iteration();
   for (int n = 0; n < n2; ++n)
     b.next_iteration_impl();
@@ -122,6 +103,23 @@ main(int argc, char ** argv)
   const auto r0 = g5cpp::region_from_ini(cm, 0);
   std::printf("region0 alpha0=%g alpha_rho=%.17g,%.17g p=%.17g u=%.17g\n", r0.alpha0, r0.alpha_rho[0], r0.alpha_rho[1],
               r0.p, r0.u);
+  if (argc >= 3 and std::strcmp(argv[2], "--init") == 0)
+  {
+    // host only: the initial state of [hydro] problem, written raw to argv[3] when given
+    const auto U = g5cpp::initial_state(cm, cfg);
+    double     sum = 0.0;
+    for (double x : U)
+      sum += x;
+    std::printf("problem=%s values=%zu sum=%.17g\n", cm.getString("hydro", "problem", "").c_str(), U.size(), sum);
+    if (argc >= 4)
+    {
+      FILE * f = std::fopen(argv[3], "wb");
+      if (!f or std::fwrite(U.data(), sizeof(double), U.size(), f) != U.size())
+        return 3;
+      std::fclose(f);
+    }
+    return 0;
+  }
   if (argc >= 4 and std::strcmp(argv[2], "--restart") == 0)
     return cfg.dim == 2 ? run_restart<2>(cm, argv[3], 30, 20) : run_restart<3>(cm, argv[3], 6, 4);
   if (argc >= 4 and std::strcmp(argv[2], "--run") == 0)

@@ -78,6 +78,28 @@ def test_initial_condition_host_side(g5):
     assert np.allclose(V[:, 0], V[:, 6] + V[:, 8])
 
 
+def test_triple_point_3d_host_geometry(g5):
+    # 3-D triple point: region 1 is the cylinder sqrt(y^2 + z^2) <= yd right of xd
+    # (InitTriplePoint.cpp:50-60), region 2 the rest right of xd; restated at the cell centres
+    cfg = g5.make_config(3, (3, 2, 2), 1.0 / 16, gamma=(1.5, 1.4))
+    cfg.xmin[1] = -1.0
+    regs = [g5.region(1.0, (1.0, 0.0), 1.0), g5.region(0.0, (0.0, 1.0), 0.1), g5.region(1.0, (0.125, 0.0), 0.1)]
+    U = g5.init_triple_point(cfg, regs, 1.0, 0.75).reshape(2, 2, 3, 9, 16, 16, 16)  # bz, by, bx, var, cz, cy, cx
+    c = np.arange(16) + 0.5
+    x = (np.arange(3)[:, None] * 16 + c[None, :]) / 16                 # (bx, cx)
+    y = -1.0 + (np.arange(2)[:, None] * 16 + c[None, :]) / 16          # (by, cy)
+    z = (np.arange(2)[:, None] * 16 + c[None, :]) / 16                 # (bz, cz)
+    X = x[None, None, :, None, None, :]
+    Y = y[None, :, None, None, :, None]
+    Z = z[:, None, None, :, None, None]
+    region = np.where(X <= 1.0, 0, np.where(np.sqrt(Y * Y + Z * Z) <= 0.75, 1, 2))
+    region = np.broadcast_to(region, (2, 2, 3, 16, 16, 16))
+    S = [g5.region_conservative(cfg, r) for r in regs]
+    for v in range(9):
+        assert np.array_equal(U[:, :, :, v], np.choose(region, [S[0][v], S[1][v], S[2][v]])), v
+    assert (region == 1).sum() > 0 and (region == 2).sum() > 0
+
+
 def _block_range(w, nblocks, wg_per_xcd, remap, grid):
     """Host restatement of the step kernels' block_range (blocks handled by workgroup w)."""
     per_xcd = (nblocks + 7) // 8

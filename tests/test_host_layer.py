@@ -4,6 +4,7 @@ import os
 import subprocess
 import sys
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -68,6 +69,51 @@ def test_ini_jet_inlet(exe):
     out = subprocess.run([exe, os.path.join(GOLD, "shock_bubble_air_helium_2d_512x128.ini")], check=True,
                          capture_output=True, text=True).stdout
     assert parse(out)["inlet"] == "0"
+
+
+def _init_state(exe, ini, tmp_path):
+    raw = tmp_path / "u.bin"
+    out = subprocess.run([exe, os.path.join(GOLD, ini), "--init", str(raw)], check=True, capture_output=True,
+                         text=True).stdout
+    return parse(out.splitlines()[-1]), np.fromfile(raw, dtype=np.float64)
+
+
+def test_init_triple_point(exe, tmp_path):
+    # init/init.cpp dispatch of [hydro] problem=triple_point: region 0 for x <= 1, region 1 for
+    # y <= 1.5, region 2 above (InitTriplePoint.cpp:30-90), at the cell centres of the 224x96 mesh;
+    # restated here from the geometry and the [region*] states
+    import godunov_five_eq_amd as g5
+    kv, U = _init_state(exe, "triple_point_2d_fixed.ini", tmp_path)
+    assert kv["problem"] == "triple_point" and int(kv["values"]) == U.size == 14 * 6 * 8 * 256
+    cfg = g5.make_config(2, (14, 6), 1 / 32, gamma=(1.5, 1.4))
+    regs = [g5.region(1.0, (1.0, 0.0), 1.0), g5.region(0.0, (0.0, 1.0), 0.1), g5.region(1.0, (0.125, 0.0), 0.1)]
+    S = [g5.region_conservative(cfg, r) for r in regs]
+    A = U.reshape(6, 14, 8, 16, 16)          # (by, bx, var, cy, cx)
+    xc = (np.arange(14)[:, None] * 16 + np.arange(16)[None, :] + 0.5) / 32   # (bx, cx)
+    yc = (np.arange(6)[:, None] * 16 + np.arange(16)[None, :] + 0.5) / 32    # (by, cy)
+    X = xc[None, :, None, :]                 # (1, bx, 1, cx)
+    Y = yc[:, None, :, None]                 # (by, 1, cy, 1)
+    region = np.where(X <= 1.0, 0, np.where(Y <= 1.5, 1, 2))
+    region = np.broadcast_to(region, (6, 14, 16, 16))
+    for v in range(8):
+        want = np.choose(region, [S[0][v], S[1][v], S[2][v]])
+        assert np.array_equal(A[:, :, v], want), v
+    assert (region == 1).sum() == 6 * 32 * 48 and (region == 0).sum() == 32 * 96
+
+
+def test_init_jet_and_unknown_problem(exe, tmp_path):
+    import godunov_five_eq_amd as g5
+    kv, U = _init_state(exe, "jet_fuel_n2_2d_fixed.ini", tmp_path)  # region 0 everywhere (InitJet.cpp:45-72)
+    cfg = g5.make_config(2, (12, 4), 1 / 64, gamma=(1.4, 4.4), pinf=(0.0, 8e7))
+    s0 = g5.region_conservative(cfg, g5.region(1.0, (32.2, 0.0), 3.83982e5, 2.0))
+    A = U.reshape(-1, 8, 256)
+    for v in range(8):
+        assert np.all(A[:, v] == s0[v]), v
+    ini = tmp_path / "rm.ini"
+    text = open(os.path.join(GOLD, "jet_fuel_n2_2d_fixed.ini")).read().replace("problem=jet", "problem=richtmyer_meshkov")
+    ini.write_text(text)
+    out = subprocess.run([exe, str(ini), "--init"], capture_output=True, text=True)
+    assert out.returncode != 0 and "richtmyer_meshkov" in out.stderr
 
 
 @pytest.mark.gpu

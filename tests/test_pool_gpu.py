@@ -139,11 +139,24 @@ def test_device_triple_point_and_box(g5):
         a0 = U[:, 4]
         assert np.all(a0[x <= 1.0] == 1.0) and np.all(a0[(x > 1.0) & (y <= 0.75)] == 1.0)
         assert np.all(a0[(x > 1.0) & (y > 0.75)] == 0.0)
+        # the host generator of the single-solve path gives the same bits
+        assert np.array_equal(pool.download(sid), g5.init_triple_point(cfg, regs, 1.0, 1.5 / 2))
         sid2 = pool.admit(cfg)
         pool.init_shock_bubble(sid2, regs, 0.3, (1.5, 0.5), (0.25, 0.2), shape=1)
         V = pool.download(sid2).reshape(21, 8, 256)
         area = (1 - V[:, 4]).sum() / 48 ** 2
         assert area == pytest.approx(0.5 * 0.4, rel=1e-12)
+
+
+def test_device_triple_point_3d_matches_host(g5):
+    say("device IC triple point 3d vs host")
+    cfg = g5.make_config(3, (4, 2, 2), 1.0 / 16, gamma=(1.5, 1.4))
+    regs = [g5.region(1.0, (1.0, 0.0), 1.0), g5.region(0.0, (0.0, 1.0), 0.1), g5.region(1.0, (0.125, 0.0), 0.1)]
+    with g5.Pool(3, budget_bytes=1 << 27) as pool:
+        sid = pool.admit(cfg)
+        pool.init_triple_point(sid, regs, 1.0, 1.5)
+        dev = pool.download(sid)
+    assert np.array_equal(dev, g5.init_triple_point(cfg, regs, 1.0, 1.5))
 
 
 def test_launch_geometries_bit_identical(g5):
