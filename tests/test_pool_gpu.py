@@ -100,6 +100,27 @@ def test_pool_3d_mix(g5, orc):
             assert np.array_equal(pool.download(sid), oracle_run(orc, cfg, U0, 4))
 
 
+def test_pool_3d_inlet_and_plain_solves(g5, orc):
+    # a 3-D jet (inflow border) and a plain solve in one launch list: the step kernel with the
+    # inflow code runs both; after the jet is released the list has no inlet and the plain kernel
+    # continues the other solve -- every result bit-identical to the oracle
+    say("pool: 3-D inlet + plain solves, then plain alone")
+    a = cases.jet(3, inlet="jet")
+    b = cases.air_water_3d(nb=2)
+    with g5.Pool(3, thinc=True, budget_bytes=1 << 30) as pool:
+        ida = pool.admit(a[0], max_steps=6)
+        pool.upload(ida, a[1])
+        idb = pool.admit(b[0], max_steps=10)
+        pool.upload(idb, b[1])
+        pool.start([ida, idb])
+        pool.step(6)
+        assert np.array_equal(pool.download(ida), oracle_run(orc, a[0], a[1], 6))
+        pool.release(ida)
+        pool.step(4)
+        assert pool.status(idb)[1] == 10
+        assert np.array_equal(pool.download(idb), oracle_run(orc, b[0], b[1], 10))
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_device_initial_condition_matches_host(g5, dim):
     say(f"device IC dim={dim}")
