@@ -121,3 +121,56 @@ def jet(dim=2, nb=None, inlet="jet", jet_geom=(0.5, 0.5, 0.25), **kw):
     nblocks = int(np.prod(nb))
     U0 = np.repeat(u0[None, :, None], nblocks, axis=0).repeat(ncell, axis=2).ravel()
     return cfg, U0
+
+
+def random_case(seed):
+    """A seeded small configuration for randomized parity: 2-D or 3-D, block counts from 1,
+    per-face ZERO_GRADIENT / WALL or periodic directions, THINC or MUSCL, minmod or MC, HANCOCK /
+    RK1 / RK2_SSP, ideal or stiffened gases, a shock-bubble initial state with transverse
+    velocity, and sometimes an inflow border.  Returns (cfg, U0, steps, label)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    dim = 3 if rng.random() < 0.4 else 2
+    nb = [int(rng.integers(1, 6 if dim == 2 else 4)) for _ in range(dim)]
+    periodic = [int(rng.random() < 0.3) for _ in range(dim)] + [0] * (3 - dim)
+    bc = []
+    for d in range(3):
+        for _ in range(2):
+            if d >= dim:
+                bc.append("NONE")
+            elif periodic[d]:
+                bc.append("PERIODIC")
+            else:
+                bc.append("WALL" if rng.random() < 0.5 else "ZERO_GRADIENT")
+    thinc = bool(rng.random() < 0.5)
+    slope = 2.0 if rng.random() < 0.4 else 1.0
+    integrator = str(rng.choice(["HANCOCK", "HANCOCK", "RK1", "RK2_SSP"]))
+    water = bool(rng.random() < 0.4)
+    dx = 1.0 / (16 * max(nb))
+    ext = [16 * n * dx for n in nb]
+    if water:
+        gamma, pinf = (float(rng.uniform(1.2, 1.7)), 4.4), (0.0, 6e8)
+        rho2, u2, p2, _ = g5.rankine_hugoniot(4.4, 6e8, 1000.0, 1e5, p2=float(rng.uniform(2e8, 1e9)))
+        regs = [g5.region(0.0, (0.0, rho2), p2, u2, float(rng.uniform(-20, 20)), float(rng.uniform(-20, 20))),
+                g5.region(0.0, (0.0, 1000.0), 1e5), g5.region(1.0, (1.0, 0.0), 1e5)]
+    else:
+        gamma, pinf = (1.4, float(rng.uniform(1.1, 1.9))), (0.0, 0.0)
+        rho2, u2, p2, _ = g5.rankine_hugoniot(1.4, 0.0, 1.0, 1.0, mach=float(rng.uniform(1.1, 3.0)))
+        regs = [g5.region(1.0, (rho2, 0.0), p2, u2, float(rng.uniform(-0.3, 0.3)), float(rng.uniform(-0.3, 0.3))),
+                g5.region(1.0, (1.0, 0.0), 1.0), g5.region(0.0, (0.0, float(rng.uniform(0.1, 3.0))), 1.0)]
+    inlet = None
+    if not periodic[0] and rng.random() < 0.25:
+        inlet = (str(rng.choice(["jet", "shock_bubble"])), regs[0])
+    cfg = g5.make_config(dim, nb, dx, periodic=periodic, bc=tuple(bc), integrator=integrator, thinc=thinc,
+                         cfl=float(rng.uniform(0.3, 0.8)), slope_type=slope, gamma=gamma, pinf=pinf,
+                         thinc_beta=float(rng.uniform(1.2, 2.0)), inlet=inlet,
+                         jet=(0.5 * ext[1], 0.5 * ext[2] if dim == 3 else 0.5, 0.4 * ext[1]))
+    centre = [float(rng.uniform(0.3, 0.7)) * e for e in ext]
+    radius = float(rng.uniform(0.15, 0.35)) * min(ext)
+    x_front = float(rng.uniform(0.05, 0.35)) * ext[0]
+    U0 = g5.init_shock_bubble(cfg, regs, x_front, centre, radius)
+    steps = int(rng.integers(5, 13)) if dim == 2 else int(rng.integers(3, 7))
+    label = (f"seed {seed}: {dim}d nb={nb} periodic={periodic[:dim]} bc={bc[:2 * dim]} thinc={thinc} "
+             f"slope={slope} {integrator} water={water} inlet={inlet[0] if inlet else None} steps={steps}")
+    return cfg, U0, steps, label

@@ -128,6 +128,42 @@ def test_parity_inlets(g5, orc):
             assert A[:, 4 if dim == 2 else 5].min() < 1.0 - 1e-6
 
 
+@pytest.mark.parametrize("group", range(4))
+def test_parity_random_configs(g5, orc, group):
+    # 40 seeded configurations (cases.random_case): block counts down to 1 per direction (the
+    # ghost resolution then wraps or mirrors onto the block itself), mixed border faces, periodic
+    # directions, THINC / MUSCL, minmod / MC, HANCOCK / RK1 / RK2_SSP, stiffened gases, inflow
+    # borders -- each bit-identical to the oracle
+    for seed in range(10 * group, 10 * group + 10):
+        cfg, U0, steps, label = cases.random_case(seed)
+        U_g, U_o = run_both(cfg, U0, steps, g5, orc, label)
+        assert check_parity(U_g, U_o, cfg.dim, label), label
+
+
+def test_mixed_random_configs(g5, orc):
+    # the same 40 configurations in mixed precision (FP32 slopes, wave speeds and THINC; FP64
+    # state, pressure recovery and dt), each against the FP64 oracle with its own dt sequence:
+    # density and total energy within relative L1 1e-5 after the run
+    worst = 0.0
+    for seed in range(40):
+        cfg, U0, steps, label = cases.random_case(seed)
+        cfg.precision = g5.PREC_MIXED
+        _announce(f"mixed {label}")
+        o = orc.Oracle(cfg)
+        o.set_state(U0)
+        o.run(steps)
+        with g5.Context(cfg) as ctx:
+            ctx.upload(U0)
+            _, n = ctx.run(steps)
+            U_g = ctx.download()
+        assert n == steps and np.all(np.isfinite(U_g)), label
+        err = rel_l1(U_g, o.get_state(), cfg.dim)
+        _announce(f"  rel L1 rho {err[0]:.2e} E {err[1]:.2e} max {err.max():.2e}")
+        assert err[0] <= 1e-5 and err[1] <= 1e-5, (label, err)
+        worst = max(worst, err[0])
+    _announce(f"worst density L1 {worst:.2e}")
+
+
 def test_compute_dt_and_host_steps(g5, orc):
     # compute_dt + do_time_step (the drop-in GodunovImplemV0 surface) against the oracle
     cfg, U0 = cases.air_helium_2d(nbx=8, nby=4)
