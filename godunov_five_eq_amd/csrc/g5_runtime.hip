@@ -407,12 +407,19 @@ append_blocks(g5_pool const * p, int32_t sid, std::vector<BlockDesc> & out)
 {
   SolveRec const & r = p->solves[sid];
   const int        nx = r.cfg.nb[0], ny = r.cfg.nb[1], nz = r.cfg.nb[2];
-  int              k = 0;
+  // 2-D: 4x4-block tiles in x-fastest tile order, so that a block's y neighbours (whose halo rows
+  // it reads) run a few workgroups apart and are still in L2; 3-D: plain brick order
+  const int T = r.cfg.dim == 2 ? 4 : 1;
   for (int z = 0; z < nz; ++z)
-    for (int y = 0; y < ny; ++y)
-      for (int x = 0; x < nx; ++x, ++k)
-        out.push_back(BlockDesc{ sid, r.slots[k], static_cast<int16_t>(x), static_cast<int16_t>(y),
-                                 static_cast<int16_t>(z), 0 });
+    for (int ty = 0; ty < ny; ty += T)
+      for (int tx = 0; tx < nx; tx += T)
+        for (int y = ty; y < std::min(ty + T, ny); ++y)
+          for (int x = tx; x < std::min(tx + T, nx); ++x)
+          {
+            const int k = x + nx * (y + ny * z);
+            out.push_back(BlockDesc{ sid, r.slots[k], static_cast<int16_t>(x), static_cast<int16_t>(y),
+                                     static_cast<int16_t>(z), 0 });
+          }
 }
 
 int
