@@ -408,7 +408,8 @@ append_blocks(g5_pool const * p, int32_t sid, std::vector<BlockDesc> & out)
   SolveRec const & r = p->solves[sid];
   const int        nx = r.cfg.nb[0], ny = r.cfg.nb[1], nz = r.cfg.nb[2];
   // 2-D: 4x4-block tiles in x-fastest tile order, so that a block's y neighbours (whose halo rows
-  // it reads) run a few workgroups apart and are still in L2; 3-D: plain brick order
+  // it reads) run a few workgroups apart (measured 2.6% faster on a 4096-solve batch, although the
+  // fetched HBM volume did not drop); 3-D: plain brick order
   const int T = r.cfg.dim == 2 ? 4 : 1;
   for (int z = 0; z < nz; ++z)
     for (int ty = 0; ty < ny; ty += T)
