@@ -20,6 +20,8 @@ PREC_FP64, PREC_MIXED = range(2)
 
 BC_NAMES = {"NONE": BC_NONE, "PERIODIC": BC_PERIODIC, "ZERO_GRADIENT": BC_ZERO_GRADIENT, "WALL": BC_WALL}
 INLET_NONE, INLET_SHOCK_BUBBLE, INLET_JET = 0, 1, 2
+SOURCE_ALLAIRE_KOKH, SOURCE_MILLER_PUCKETT = 0, 1
+SOURCE_NAMES = {"ALLAIRE_KOKH": SOURCE_ALLAIRE_KOKH, "MILLER_PUCKETT": SOURCE_MILLER_PUCKETT}
 INLET_NAMES = {"none": INLET_NONE, "shock_bubble": INLET_SHOCK_BUBBLE, "jet": INLET_JET}
 TI_NAMES = {"HANCOCK": TI_HANCOCK, "RK1": TI_RK1, "RK2_SSP": TI_RK2_SSP}
 
@@ -48,6 +50,8 @@ class G5Config(ctypes.Structure):
         ("inlet_pad", ctypes.c_int32),
         ("inlet_u", ctypes.c_double * 9),
         ("jet", ctypes.c_double * 3),
+        ("source_term", ctypes.c_int32),
+        ("source_pad", ctypes.c_int32),
     ]
 
 
@@ -236,10 +240,12 @@ def nvar_of(dim):
 def make_config(dim, nb, dx, *, bsize=16, periodic=(0, 0, 0), bc=("ZERO_GRADIENT",) * 6,
                 integrator="HANCOCK", thinc=False, precision=PREC_FP64, cfl=0.5, slope_type=1.0,
                 smallr=1e-10, smallc=1e-10, gamma=(1.4, 1.4), pinf=(0.0, 0.0), thinc_beta=1.5,
-                thinc_epsilon=1e-3, xmin=(0.0, 0.0, 0.0), inlet=None, jet=(0.5, 0.5, 0.05)):
+                thinc_epsilon=1e-3, xmin=(0.0, 0.0, 0.0), inlet=None, jet=(0.5, 0.5, 0.05),
+                source_term="ALLAIRE_KOKH"):
     """Build a G5Config; defaults follow models/HydroSettings.cpp:18-29.  inlet: None, or
     ("shock_bubble", region) / ("jet", region) -- outside cells with x < xmin (for the jet those
-    whose centre is inside it, jet = (centre y, centre z, width)) take the region's state."""
+    whose centre is inside it, jet = (centre y, centre z, width)) take the region's state.
+    source_term: "ALLAIRE_KOKH" or "MILLER_PUCKETT" (volume-fraction source term of the update)."""
     c = G5Config()
     c.dim = dim
     c.bsize = bsize
@@ -265,6 +271,7 @@ def make_config(dim, nb, dx, *, bsize=16, periodic=(0, 0, 0), bc=("ZERO_GRADIENT
     c.thinc_epsilon = thinc_epsilon
     for k in range(3):
         c.jet[k] = jet[k]
+    c.source_term = SOURCE_NAMES[source_term] if isinstance(source_term, str) else int(source_term)
     if inlet is not None:
         kind, reg = inlet
         c.inlet = INLET_NAMES[kind]

@@ -844,9 +844,11 @@ riemann(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, dou
 // (.h:319-393): accumulate left minus right flux with the Allaire-Kokh alpha source term (time-n
 // alpha of the cell), then add to the running U2 with the 1e-12 clamps.
 // ---------------------------------------------------------------------------------------------
+// st0, st1: the source term factors (1 for ALLAIRE_KOKH, source_factors for MILLER_PUCKETT)
 template <int DIM>
 G5D void
-apply_flux_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double ua1)
+apply_flux_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double ua1, double st0 = 1.0,
+                double st1 = 1.0)
 {
   const double eps = 1e-12;
   double       acc_r0 = 0.0 + FL.ar0;
@@ -857,8 +859,8 @@ apply_flux_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double 
   double       acc_mu = 0.0 + FL.mu;
   double       acc_mv = 0.0 + FL.mv;
   double       acc_mw = 0.0 + FL.mw;
-  acc_a0 -= 1.0 * FL.us * ua0;
-  acc_a1 -= 1.0 * FL.us * ua1;
+  acc_a0 -= st0 * FL.us * ua0;
+  acc_a1 -= st1 * FL.us * ua1;
   acc_r0 = acc_r0 - FR.ar0;
   acc_r1 = acc_r1 - FR.ar1;
   acc_a0 = acc_a0 - FR.a0;
@@ -867,8 +869,8 @@ apply_flux_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double 
   acc_mu = acc_mu - FR.mu;
   acc_mv = acc_mv - FR.mv;
   acc_mw = acc_mw - FR.mw;
-  acc_a0 += 1.0 * FR.us * ua0;
-  acc_a1 += 1.0 * FR.us * ua1;
+  acc_a0 += st0 * FR.us * ua0;
+  acc_a1 += st1 * FR.us * ua1;
 
   u2.ar0 += acc_r0;
   if (u2.ar0 < eps)
@@ -926,16 +928,50 @@ assemble_flux(FluxT<float> const & s, double dtdx, int AX)
 // the update with the face records of either path (dtdx and AX are used by the mixed path)
 template <int DIM>
 G5D void
-apply_face_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double ua1, double, int)
+apply_face_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double ua1, double, int, double st0 = 1.0,
+                double st1 = 1.0)
 {
-  apply_flux_pair<DIM>(u2, FL, FR, ua0, ua1);
+  apply_flux_pair<DIM>(u2, FL, FR, ua0, ua1, st0, st1);
 }
 template <int DIM>
 G5D void
 apply_face_pair(Cons & u2, FluxT<float> const & FL, FluxT<float> const & FR, double ua0, double ua1, double dtdx,
-                int AX)
+                int AX, double st0 = 1.0, double st1 = 1.0)
 {
-  apply_flux_pair<DIM>(u2, assemble_flux<DIM>(FL, dtdx, AX), assemble_flux<DIM>(FR, dtdx, AX), ua0, ua1);
+  apply_flux_pair<DIM>(u2, assemble_flux<DIM>(FL, dtdx, AX), assemble_flux<DIM>(FR, dtdx, AX), ua0, ua1, st0, st1);
+}
+
+// scheme/ReadFluxesAndConservativeUpdateFunctor.h:412-443 get_source_term_factor for
+// MILLER_PUCKETT, from the cell's time-n state U: the pressure as compute_mixture_pressure
+// (models/utils.h:160-182: density alpha_rho0 + alpha_rho1, kinetic energy with U[ID],
+// alpha1 = 1 - alpha0), the material bulk moduli of the stiffened gas K_k = gamma_k (p + pinf_k),
+// the mixture bulk modulus by Wood's formula K = 1 / (alpha0/K0 + alpha1/K1) with the stored
+// alpha1, and st_k = 1 - K/K_k.  (The reference's bulk moduli come from its EOS class, which is
+// not in the reference tree: these two formulas are the textbook ones for a stiffened gas at
+// pressure equilibrium.)
+template <int DIM>
+G5D void
+source_factors(SolveDesc const & S, Cons const & U, double & st0, double & st1)
+{
+  const double d = fmax(U.ar0 + U.ar1, S.smallr);
+  const double dk = fmax(U.d, S.smallr);
+  const double ux = U.mu / dk;
+  const double uy = U.mv / dk;
+  double       ekin;
+  if (DIM == 2)
+    ekin = 0.5 * (ux * ux + uy * uy);
+  else
+  {
+    const double uz = U.mw / dk;
+    ekin = 0.5 * (ux * ux + uy * uy + uz * uz);
+  }
+  const double eint = U.e / d - ekin;
+  const double p = eos_pressure(S, d, eint, U.a0, 1.0 - U.a0);
+  const double k0 = S.gamma[0] * (p + S.pinf[0]);
+  const double k1 = S.gamma[1] * (p + S.pinf[1]);
+  const double kbar = 1.0 / (U.a0 / k0 + U.a1 / k1);
+  st0 = 1.0 - kbar / k0;
+  st1 = 1.0 - kbar / k1;
 }
 
 // scheme/FixVolumeFractionsNormalization.cpp:58-70

@@ -10,8 +10,9 @@ namespace g5
 
 // fused step (primitives, slopes, fluxes, update, normalisation, next-dt contribution); mixed
 // selects the FP32 primitive/face path
-void launch_step2d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool ring_wave, hipStream_t stream);
-void launch_step3d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool zsplit, bool inlet,
+void launch_step2d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool ring_wave, bool mp,
+                   hipStream_t stream);
+void launch_step3d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool zsplit, bool inlet, bool mp,
                    hipStream_t stream);
 
 // max over the listed blocks of the CFL rate of the current state into ctl[solve].invdt_bits

@@ -110,6 +110,7 @@ struct g5_pool
   int32_t     nlist = 0;
   bool        list_dirty = true;
   bool        any_inlet = false; // a live solve has an inflow border: the 3-D step kernels resolve it
+  bool        any_mp = false;    // a live solve has the MILLER_PUCKETT source term
   SolveCtl *  h_ctl = nullptr; // pinned mirror of the control records
   int32_t     cap_hctl = 0;
 
@@ -204,6 +205,8 @@ validate(g5_config const & c, std::string & msg)
     return msg = "unknown time integrator", G5_ERR_INVALID;
   if (c.precision != G5_PREC_FP64 and c.precision != G5_PREC_MIXED)
     return msg = "unknown precision", G5_ERR_INVALID;
+  if (c.source_term != G5_SOURCE_ALLAIRE_KOKH and c.source_term != G5_SOURCE_MILLER_PUCKETT)
+    return msg = "unknown volume-fraction source term", G5_ERR_INVALID;
   if (c.inlet != G5_INLET_NONE and c.inlet != G5_INLET_SHOCK_BUBBLE and c.inlet != G5_INLET_JET)
     return msg = "unknown inlet", G5_ERR_INVALID;
   if (c.inlet != G5_INLET_NONE and c.periodic[0])
@@ -356,6 +359,7 @@ pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, i
   r.started = false;
   r.cfg = cfg;
   p->any_inlet = p->any_inlet or cfg.inlet != G5_INLET_NONE; // before any launch that lists it
+  p->any_mp = p->any_mp or cfg.source_term == G5_SOURCE_MILLER_PUCKETT;
   for (int d = cfg.dim; d < 3; ++d)
     r.cfg.nb[d] = 1;
   r.nblk = static_cast<int32_t>(nblk);
@@ -429,14 +433,16 @@ rebuild_list(g5_pool * p)
   if (!p->list_dirty)
     return G5_OK;
   std::vector<BlockDesc> list;
-  bool                   inlet = false;
+  bool                   inlet = false, mp = false;
   for (int32_t s = 0; s < static_cast<int32_t>(p->solves.size()); ++s)
     if (p->solves[s].live)
     {
       append_blocks(p, s, list);
       inlet = inlet or p->solves[s].cfg.inlet != G5_INLET_NONE;
+      mp = mp or p->solves[s].cfg.source_term == G5_SOURCE_MILLER_PUCKETT;
     }
   p->any_inlet = inlet;
+  p->any_mp = mp;
   int64_t cb = p->cap_blocks;
   int     rc = grow(p, p->d_blocks, cb, static_cast<int64_t>(list.size()), p->stream);
   if (rc != G5_OK)
@@ -479,9 +485,9 @@ launch_step(g5_pool * p, StepArgs const & A, bool hancock)
     return;
   const bool mixed = p->precision == G5_PREC_MIXED;
   if (p->dim == 2)
-    launch_step2d(A, p->thinc != 0, hancock, mixed, p->launch.ring_wave != 0, p->stream);
+    launch_step2d(A, p->thinc != 0, hancock, mixed, p->launch.ring_wave != 0, p->any_mp, p->stream);
   else
-    launch_step3d(A, p->thinc != 0, hancock, mixed, p->launch.zsplit != 0, p->any_inlet, p->stream);
+    launch_step3d(A, p->thinc != 0, hancock, mixed, p->launch.zsplit != 0, p->any_inlet, p->any_mp, p->stream);
 }
 
 // ---------------------------------------------------------------------------------------------

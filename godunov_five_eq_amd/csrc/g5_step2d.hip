@@ -267,7 +267,7 @@ constexpr int ARENA2_DOUBLES =
 // RW: a fifth wavefront (threads 256..319) does the 64 ring-cell face states and the 32 ring
 // Riemann problems in parallel with the owned cells instead of after them on wave 0 -- the form
 // for launch lists too short to fill the GPU, where a block's latency is the step time.
-template <bool THINC, bool HANCOCK, typename T, bool RW = false>
+template <bool THINC, bool HANCOCK, typename T, bool RW = false, bool MP = false>
 G5D void
 step2d_block(StepArgs const & A, int blk)
 {
@@ -382,9 +382,13 @@ step2d_block(StepArgs const & A, int blk)
   double rate = 0.0;
   if (owner)
   {
-    Cons u2 = uown;
-    apply_face_pair<2>(u2, Fx, fl_load2(FX, FXE, fx2(i + 1, j)), uown.a0, uown.a1, dtdx, 0);
-    apply_face_pair<2>(u2, Fy, fl_load2(FY, FYE, fy2(i, j + 1)), uown.a0, uown.a1, dtdx, 1);
+    Cons   u2 = uown;
+    double st0 = 1.0, st1 = 1.0; // volume-fraction source factors (MILLER_PUCKETT: from the time-n cell)
+    if constexpr (MP) // the list holds a MILLER_PUCKETT solve; this block's solve decides
+      if (S.source_term == G5_SOURCE_MILLER_PUCKETT)
+        source_factors<2>(S, uown, st0, st1);
+    apply_face_pair<2>(u2, Fx, fl_load2(FX, FXE, fx2(i + 1, j)), uown.a0, uown.a1, dtdx, 0, st0, st1);
+    apply_face_pair<2>(u2, Fy, fl_load2(FY, FYE, fy2(i, j + 1)), uown.a0, uown.a1, dtdx, 1, st0, st1);
     fix_volume_fractions(u2);
     store_cons<2, B2>(U2, own, u2);
     rate = cell_rate<2, T>(S, u2);
@@ -403,29 +407,29 @@ step2d_block(StepArgs const & A, int blk)
 }
 
 // one workgroup per block
-template <bool THINC, bool HANCOCK, typename T>
+template <bool THINC, bool HANCOCK, typename T, bool MP>
 __global__ void __launch_bounds__(256)
 k_step2d(StepArgs A)
 {
   int b, end, stride;
   block_range(A, b, end, stride);
   if (b < end)
-    step2d_block<THINC, HANCOCK, T>(A, b);
+    step2d_block<THINC, HANCOCK, T, false, MP>(A, b);
 }
 
 // one workgroup of five wavefronts per block (ring work on the fifth)
-template <bool THINC, bool HANCOCK, typename T>
+template <bool THINC, bool HANCOCK, typename T, bool MP>
 __global__ void __launch_bounds__(320)
 k_step2d_ringwave(StepArgs A)
 {
   int b, end, stride;
   block_range(A, b, end, stride);
   if (b < end)
-    step2d_block<THINC, HANCOCK, T, true>(A, b);
+    step2d_block<THINC, HANCOCK, T, true, MP>(A, b);
 }
 
 // persistent workgroups looping over their range of the block list
-template <bool THINC, bool HANCOCK, typename T>
+template <bool THINC, bool HANCOCK, typename T, bool MP>
 __global__ void __launch_bounds__(256)
 k_step2d_persistent(StepArgs A)
 {
@@ -433,45 +437,48 @@ k_step2d_persistent(StepArgs A)
   block_range(A, b, end, stride);
   for (; b < end; b += stride)
   {
-    step2d_block<THINC, HANCOCK, T>(A, b);
+    step2d_block<THINC, HANCOCK, T, false, MP>(A, b);
     __syncthreads(); // the next block reuses the LDS
   }
 }
 
-template <bool THINC, bool HANCOCK, typename T>
+// MP: the launch list holds a solve with the MILLER_PUCKETT source term
+template <bool THINC, bool HANCOCK, typename T, bool MP>
 static void
 launch2k(StepArgs const & A, bool ring_wave, hipStream_t stream)
 {
   const dim3 block(256);
   if (A.wg_per_xcd > 0)
-    hipLaunchKernelGGL((k_step2d_persistent<THINC, HANCOCK, T>), dim3(8 * A.wg_per_xcd), block, 0, stream, A);
+    hipLaunchKernelGGL((k_step2d_persistent<THINC, HANCOCK, T, MP>), dim3(8 * A.wg_per_xcd), block, 0, stream, A);
   else if (ring_wave)
-    hipLaunchKernelGGL((k_step2d_ringwave<THINC, HANCOCK, T>), dim3(8 * A.per_xcd), dim3(320), 0, stream, A);
+    hipLaunchKernelGGL((k_step2d_ringwave<THINC, HANCOCK, T, MP>), dim3(8 * A.per_xcd), dim3(320), 0, stream, A);
   else
-    hipLaunchKernelGGL((k_step2d<THINC, HANCOCK, T>), dim3(8 * A.per_xcd), block, 0, stream, A);
+    hipLaunchKernelGGL((k_step2d<THINC, HANCOCK, T, MP>), dim3(8 * A.per_xcd), block, 0, stream, A);
 }
 
-template <typename T>
+template <typename T, bool MP>
 static void
 launch2(StepArgs const & A, bool thinc, bool hancock, bool ring_wave, hipStream_t stream)
 {
   if (thinc and hancock)
-    launch2k<true, true, T>(A, ring_wave, stream);
+    launch2k<true, true, T, MP>(A, ring_wave, stream);
   else if (thinc)
-    launch2k<true, false, T>(A, ring_wave, stream);
+    launch2k<true, false, T, MP>(A, ring_wave, stream);
   else if (hancock)
-    launch2k<false, true, T>(A, ring_wave, stream);
+    launch2k<false, true, T, MP>(A, ring_wave, stream);
   else
-    launch2k<false, false, T>(A, ring_wave, stream);
+    launch2k<false, false, T, MP>(A, ring_wave, stream);
 }
 
 void
-launch_step2d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool ring_wave, hipStream_t stream)
+launch_step2d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool ring_wave, bool mp, hipStream_t stream)
 {
   if (mixed)
-    launch2<float>(A, thinc, hancock, ring_wave, stream);
+    mp ? launch2<float, true>(A, thinc, hancock, ring_wave, stream)
+       : launch2<float, false>(A, thinc, hancock, ring_wave, stream);
   else
-    launch2<double>(A, thinc, hancock, ring_wave, stream);
+    mp ? launch2<double, true>(A, thinc, hancock, ring_wave, stream)
+       : launch2<double, false>(A, thinc, hancock, ring_wave, stream);
 }
 
 } // namespace g5

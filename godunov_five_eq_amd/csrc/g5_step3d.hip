@@ -312,7 +312,7 @@ struct Pred3
 };
 } // namespace
 
-template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, typename T>
+template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, bool MP, typename T>
 G5D void
 step3d_body(StepArgs const & A, int blk)
 {
@@ -390,6 +390,7 @@ step3d_body(StepArgs const & A, int blk)
   Cons     uraw_next;    // ... in plane P+1
   Cons     u2prev;       // plane P-1 after its x and y updates
   double   uaprev0 = 0.0, uaprev1 = 0.0;
+  double   stprev0 = 1.0, stprev1 = 1.0; // source factors of plane P-1 (1 unless MILLER_PUCKETT)
   FluxT<T> zFprev;       // z flux on face P-1
   FaceT<T> zL;           // +z face state of plane P-1
   FaceT<T> zLnext;       // +z face state of plane P (wave 1 with ZSPLIT, until its z solve)
@@ -427,7 +428,7 @@ step3d_body(StepArgs const & A, int blk)
   auto           z_complete = [&](int P) {
     if (P >= 1)
     {
-      apply_face_pair<3>(u2prev, zFprev, zF, uaprev0, uaprev1, dtdx, 2);
+      apply_face_pair<3>(u2prev, zFprev, zF, uaprev0, uaprev1, dtdx, 2, stprev0, stprev1);
       fix_volume_fractions(u2prev);
       store_cons<3, B3>(U2, blk_off + i + 16 * j + 256 * (P - 1), u2prev);
       inv_max = fmax(inv_max, cell_rate<3, T>(S, u2prev));
@@ -538,12 +539,21 @@ step3d_body(StepArgs const & A, int blk)
     if (inner_plane)
     {
       u2prev = uraw_cur;
+      double stc0 = 1.0, stc1 = 1.0;
+      if constexpr (MP) // the list holds a MILLER_PUCKETT solve; this block's solve decides
+        if (S.source_term == G5_SOURCE_MILLER_PUCKETT)
+          source_factors<3>(S, uraw_cur, stc0, stc1);
       apply_face_pair<3>(u2prev, fl_load3(sFX, sFXE, fx3(i, j)), fl_load3(sFX, sFXE, fx3(i + 1, j)), uraw_cur.a0,
-                         uraw_cur.a1, dtdx, 0);
+                         uraw_cur.a1, dtdx, 0, stc0, stc1);
       apply_face_pair<3>(u2prev, fl_load3(sFY, sFYE, fy3(i, j)), fl_load3(sFY, sFYE, fy3(i, j + 1)), uraw_cur.a0,
-                         uraw_cur.a1, dtdx, 1);
+                         uraw_cur.a1, dtdx, 1, stc0, stc1);
       uaprev0 = uraw_cur.a0;
       uaprev1 = uraw_cur.a1;
+      if constexpr (MP)
+      {
+        stprev0 = stc0;
+        stprev1 = stc1;
+      }
     }
     uraw_cur = uraw_next;
   }
@@ -564,34 +574,35 @@ step3d_body(StepArgs const & A, int blk)
 // One workgroup per block.  (A persistent form, workgroups looping over several blocks, made
 // an illegal memory access on the GPU with no cause found in the code; the runtime never asks
 // for it in 3-D: the launch must have wg_per_xcd == 0.)
-template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, typename T>
+template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, bool MP, typename T>
 G5D void
 step3d_blocks(StepArgs const & A)
 {
   int b, end, stride;
   block_range(A, b, end, stride);
   if (b < end)
-    step3d_body<THINC, HANCOCK, ZSPLIT, INLET, T>(A, b);
+    step3d_body<THINC, HANCOCK, ZSPLIT, INLET, MP, T>(A, b);
 }
 
 // INLET: the launch list holds a solve with an inflow border (the resolver then marks inflow
 // cells, whose state the conversion replaces); without it the border code is the WALL /
-// ZERO_GRADIENT / periodic resolution alone
-template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET>
+// ZERO_GRADIENT / periodic resolution alone.  MP: a solve with the MILLER_PUCKETT source term
+// (instantiated with the split z solve only).
+template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, bool MP>
 __global__ void __launch_bounds__(256)
 k_step3d_v3(StepArgs A)
 {
-  step3d_blocks<THINC, HANCOCK, ZSPLIT, INLET, double>(A);
+  step3d_blocks<THINC, HANCOCK, ZSPLIT, INLET, MP, double>(A);
 }
 
-template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET>
+template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, bool MP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_step3d_mixed(StepArgs A)
 {
-  step3d_blocks<THINC, HANCOCK, ZSPLIT, INLET, float>(A);
+  step3d_blocks<THINC, HANCOCK, ZSPLIT, INLET, MP, float>(A);
 }
 
-template <bool ZSPLIT, bool INLET, typename T>
+template <bool ZSPLIT, bool INLET, bool MP, typename T>
 static void
 launch3v3(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
 {
@@ -599,54 +610,62 @@ launch3v3(StepArgs const & A, bool thinc, bool hancock, hipStream_t stream)
   if constexpr (sizeof(T) == sizeof(double))
   {
     if (thinc and hancock)
-      hipLaunchKernelGGL((k_step3d_v3<true, true, ZSPLIT, INLET>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_v3<true, true, ZSPLIT, INLET, MP>), grid, block, 0, stream, A);
     else if (thinc)
-      hipLaunchKernelGGL((k_step3d_v3<true, false, ZSPLIT, INLET>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_v3<true, false, ZSPLIT, INLET, MP>), grid, block, 0, stream, A);
     else if (hancock)
-      hipLaunchKernelGGL((k_step3d_v3<false, true, ZSPLIT, INLET>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_v3<false, true, ZSPLIT, INLET, MP>), grid, block, 0, stream, A);
     else
-      hipLaunchKernelGGL((k_step3d_v3<false, false, ZSPLIT, INLET>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_v3<false, false, ZSPLIT, INLET, MP>), grid, block, 0, stream, A);
   }
   else
   {
     if (thinc and hancock)
-      hipLaunchKernelGGL((k_step3d_mixed<true, true, ZSPLIT, INLET>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_mixed<true, true, ZSPLIT, INLET, MP>), grid, block, 0, stream, A);
     else if (thinc)
-      hipLaunchKernelGGL((k_step3d_mixed<true, false, ZSPLIT, INLET>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_mixed<true, false, ZSPLIT, INLET, MP>), grid, block, 0, stream, A);
     else if (hancock)
-      hipLaunchKernelGGL((k_step3d_mixed<false, true, ZSPLIT, INLET>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_mixed<false, true, ZSPLIT, INLET, MP>), grid, block, 0, stream, A);
     else
-      hipLaunchKernelGGL((k_step3d_mixed<false, false, ZSPLIT, INLET>), grid, block, 0, stream, A);
+      hipLaunchKernelGGL((k_step3d_mixed<false, false, ZSPLIT, INLET, MP>), grid, block, 0, stream, A);
   }
 }
 
 template <typename T>
 static void
-launch3p(StepArgs const & A, bool thinc, bool hancock, bool split, bool inlet, hipStream_t stream)
+launch3p(StepArgs const & A, bool thinc, bool hancock, bool split, bool inlet, bool mp, hipStream_t stream)
 {
-  if (split)
+  if (mp) // the split z solve (same bits as the other placement)
   {
     if (inlet)
-      launch3v3<true, true, T>(A, thinc, hancock, stream);
+      launch3v3<true, true, true, T>(A, thinc, hancock, stream);
     else
-      launch3v3<true, false, T>(A, thinc, hancock, stream);
+      launch3v3<true, false, true, T>(A, thinc, hancock, stream);
+  }
+  else if (split)
+  {
+    if (inlet)
+      launch3v3<true, true, false, T>(A, thinc, hancock, stream);
+    else
+      launch3v3<true, false, false, T>(A, thinc, hancock, stream);
   }
   else
   {
     if (inlet)
-      launch3v3<false, true, T>(A, thinc, hancock, stream);
+      launch3v3<false, true, false, T>(A, thinc, hancock, stream);
     else
-      launch3v3<false, false, T>(A, thinc, hancock, stream);
+      launch3v3<false, false, false, T>(A, thinc, hancock, stream);
   }
 }
 
 void
-launch_step3d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool split, bool inlet, hipStream_t stream)
+launch_step3d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool split, bool inlet, bool mp,
+              hipStream_t stream)
 {
   if (mixed)
-    launch3p<float>(A, thinc, hancock, split, inlet, stream);
+    launch3p<float>(A, thinc, hancock, split, inlet, mp, stream);
   else
-    launch3p<double>(A, thinc, hancock, split, inlet, stream);
+    launch3p<double>(A, thinc, hancock, split, inlet, mp, stream);
 }
 
 } // namespace g5

@@ -22,6 +22,8 @@ struct SolveDesc
   int32_t bc[6];       // G5_BC_* per face
   int32_t integrator;  // G5_TI_*
   int32_t inlet;       // G5_INLET_*
+  int32_t source_term; // G5_SOURCE_*
+  int32_t pad_st;
   double  dx;
   double  cfl;
   double  slope_type;
@@ -36,6 +38,8 @@ struct SolveDesc
   double  inlet_u[9]; // conservative inflow state of the overridden outside cells
   double  origin[3];  // domain origin
   double  jet[3];     // jet centre y, z and width
+  double  gamma[2];   // gamma_k and p_inf_k (bulk moduli of the Miller-Puckett source term)
+  double  pinf[2];
   // FP32 copies for the mixed-precision path
   float g1f[2];
   float gpf[2];
@@ -110,6 +114,8 @@ make_solve_desc(g5_config const & c, int32_t base, SolveDesc & s)
     s.bc[f] = c.bc[f];
   s.integrator = c.integrator;
   s.inlet = c.inlet;
+  s.source_term = c.source_term;
+  s.pad_st = 0;
   for (int k = 0; k < 9; ++k)
     s.inlet_u[k] = c.inlet_u[k];
   for (int d = 0; d < 3; ++d)
@@ -127,6 +133,8 @@ make_solve_desc(g5_config const & c, int32_t base, SolveDesc & s)
   {
     s.g1[k] = 1.0 / (c.gamma[k] - 1.0);
     s.gp[k] = c.gamma[k] * c.pinf[k] / (c.gamma[k] - 1.0);
+    s.gamma[k] = c.gamma[k];
+    s.pinf[k] = c.pinf[k];
   }
   s.thinc_beta = c.thinc_beta;
   s.thinc_eps = c.thinc_epsilon;

@@ -63,6 +63,15 @@ enum
   G5_PREC_MIXED = 1 /* FP32 slopes, wave speeds and THINC; FP64 state, EOS pressure recovery, dt */
 };
 
+/* volume-fraction source term factor st_k of the update (alpha_k source st_k u* alpha_k) */
+enum
+{
+  G5_SOURCE_ALLAIRE_KOKH = 0, /* st_k = 1 */
+  G5_SOURCE_MILLER_PUCKETT = 1 /* st_k = 1 - K/K_k: K_k = gamma_k (p + pinf_k) of the stiffened gas,
+                                  K = 1/(alpha_0/K_0 + alpha_1/K_1) (Wood), p from the cell's
+                                  time-n state */
+};
+
 /* inflow overrides of the outside cells, applied after the border fill
  * (SolverGodunovFiveEq.cpp:997-1018) */
 enum
@@ -108,6 +117,10 @@ typedef struct g5_config
   int32_t inlet_pad;
   double  inlet_u[9];
   double  jet[3];
+  /* volume-fraction source term of the update ([volume_fraction_advection] source_term_type,
+   * scheme/ReadFluxesAndConservativeUpdateFunctor.h:412-443): G5_SOURCE_* */
+  int32_t source_term;
+  int32_t source_pad;
 } g5_config;
 
 typedef struct g5_ctx g5_ctx;

@@ -256,6 +256,14 @@ config_from_ini(ConfigMap const & c)
     cfg.pinf[k] = eos == "MG_STIFFENED_GAS" ? c.getReal(sec, "pinf", 0.0) : 0.0;
   }
   cfg.precision = c.getString("hydro", "precision", "FP64") == "MIXED" ? G5_PREC_MIXED : G5_PREC_FP64;
+  // [volume_fraction_advection] source_term_type (app/test_shock_bubble_air_water_2d.ini:106-108)
+  const std::string st = c.getString("volume_fraction_advection", "source_term_type", "ALLAIRE_KOKH");
+  if (st == "MILLER_PUCKETT")
+    cfg.source_term = G5_SOURCE_MILLER_PUCKETT;
+  else if (st == "ALLAIRE_KOKH")
+    cfg.source_term = G5_SOURCE_ALLAIRE_KOKH;
+  else
+    throw std::runtime_error("unknown [volume_fraction_advection] source_term_type " + st);
   // inflow overrides of fill_outside_quadrants (SolverGodunovFiveEq.cpp:997-1018): the jet takes
   // region 1 ([border] enable_jet, [jet] y, z, width: border_conditions/BCJet.h:62-69,
   // BCJet.cpp:55), the shock-bubble inlet region 0 ([shock_bubble] use_inlet_bc, a

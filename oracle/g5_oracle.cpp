@@ -21,6 +21,9 @@
 //    Exact operation order versus the absent library: parity unpinned.
 //  - Youngs normal: the reference calls vof::youngs_normal_inplace (absent).  Restated from
 //    Youngs (1982); pinned by planar-interface invariants only.  Parity unpinned.
+//  - MILLER_PUCKETT source factors: the bulk moduli come from the same absent EOS class;
+//    restated as the stiffened-gas K_k = gamma_k (p + pinf_k) and Wood's mixture rule, pinned by
+//    the sum rule, pure-phase limits and equilibrium tests only.  Parity unpinned.
 //  - tanh: a deterministic expm1-based tanh (Cody-Waite reduction + Taylor polynomial) used
 //    identically by the HIP path; pinned against libm tanh to a few ulp.
 //
@@ -70,6 +73,10 @@ extern "C" struct g5o_params
   double  inlet_u[9]; // conservative inflow state
   double  xmin[3];    // domain origin ([mesh] xmin, ymin, zmin)
   double  jet[3];     // [jet] y, z, width
+  // volume-fraction source term ([volume_fraction_advection] source_term_type):
+  // 0 ALLAIRE_KOKH, 1 MILLER_PUCKETT (scheme/ReadFluxesAndConservativeUpdateFunctor.h:412-443)
+  int32_t source_term;
+  int32_t pad2;
 };
 
 enum
@@ -125,14 +132,33 @@ struct Eos
 {
   double g1[2]; // 1/(gamma_k - 1)
   double gp[2]; // gamma_k p_inf_k / (gamma_k - 1)
+  double gamma[2], pinf[2];
   void
-  init(double const gamma[2], double const pinf[2])
+  init(double const gamma_[2], double const pinf_[2])
   {
     for (int k = 0; k < 2; ++k)
     {
-      g1[k] = 1.0 / (gamma[k] - 1.0);
-      gp[k] = gamma[k] * pinf[k] / (gamma[k] - 1.0);
+      g1[k] = 1.0 / (gamma_[k] - 1.0);
+      gp[k] = gamma_[k] * pinf_[k] / (gamma_[k] - 1.0);
+      gamma[k] = gamma_[k];
+      pinf[k] = pinf_[k];
     }
+  }
+  // Bulk moduli for the MILLER_PUCKETT source term (call sites
+  // scheme/ReadFluxesAndConservativeUpdateFunctor.h:427-437; the reference's EOS class is not in
+  // its tree -- PARITY UNPINNED, restated from the stiffened-gas closure): material
+  // K_k = rho_k c_k^2 = gamma_k (p + pinf_k), mixture by Wood's formula 1/K = sum alpha_k / K_k.
+  double
+  material_bulk_modulus(int k, double p, double /*alpha_k*/, double /*alpha_rho_k*/) const
+  {
+    return gamma[k] * (p + pinf[k]);
+  }
+  double
+  mixture_bulk_modulus(double /*rho*/, double p, double a0, double a1, double ar0, double ar1) const
+  {
+    const double k0 = material_bulk_modulus(0, p, a0, ar0);
+    const double k1 = material_bulk_modulus(1, p, a1, ar1);
+    return 1.0 / (a0 / k0 + a1 / k1);
   }
   double
   xi(double a0, double a1) const
@@ -256,6 +282,31 @@ compute_mixture_ekin(State const & u, Settings const & s, int dim, Ids const & H
   const double uy = u[H.IV] / d;
   const double uz = u[H.IW] / d;
   return 0.5 * (ux * ux + uy * uy + uz * uz);
+}
+
+// models/utils.h:160-182 compute_mixture_pressure: density alpha_rho0 + alpha_rho1, kinetic
+// energy with U[ID] (compute_mixture_ekin), alpha1 = 1 - alpha0
+static double
+compute_mixture_pressure(State const & u, Settings const & s, Eos const & eos, int dim, Ids const & H)
+{
+  const double d = std::fmax(u[H.IAD0] + u[H.IAD1], s.smallr);
+  const double ekin = compute_mixture_ekin(u, s, dim, H);
+  const double eint = u[H.IE] / d - ekin;
+  return eos.mixture_pressure(d, eint, u[H.IA0], 1.0 - u[H.IA0], u[H.IAD0], u[H.IAD1]);
+}
+
+// scheme/ReadFluxesAndConservativeUpdateFunctor.h:412-443 get_source_term_factor
+static double
+source_term_factor(int source_term, State const & u, Settings const & s, Eos const & eos, int dim, Ids const & H,
+                   int mat)
+{
+  if (source_term != 1)
+    return 1.0; // ALLAIRE_KOKH
+  const double pressure = compute_mixture_pressure(u, s, eos, dim, H);
+  const double kappa0 = mat == 0 ? eos.material_bulk_modulus(0, pressure, u[H.IA0], u[H.IAD0])
+                                 : eos.material_bulk_modulus(1, pressure, u[H.IA1], u[H.IAD1]);
+  const double kappa_bar = eos.mixture_bulk_modulus(u[H.ID], pressure, u[H.IA0], u[H.IA1], u[H.IAD0], u[H.IAD1]);
+  return 1.0 - kappa_bar / kappa0;
 }
 
 // models/utils.h:200-250 computePrimitives ; returns c
@@ -1154,18 +1205,27 @@ struct Solver
   }
 
   // scheme/ReadFluxesAndConservativeUpdateFunctor.cpp:112-592 read_fluxes_and_update_{2d,3d}
-  // (same-level branch) and update_U (.h:319-393); source term factor 1 (ALLAIRE_KOKH, .h:412-443).
+  // (same-level branch) and update_U (.h:319-393); source term factors of .h:412-443 from the
+  // cell's time-n state (1 for ALLAIRE_KOKH).
   void
   update(std::vector<double> & Uin, std::vector<double> & Uout, int dir)
   {
     const long total = static_cast<long>(nowned) * ncell;
     const double eps = 1e-12;
-    const double st0 = 1.0, st1 = 1.0;
 #pragma omp parallel for schedule(static)
     for (long gi = 0; gi < total; ++gi)
     {
       const int ib = static_cast<int>(gi / ncell);
       const int c = static_cast<int>(gi % ncell);
+      double    st0 = 1.0, st1 = 1.0;
+      if (P.source_term == 1)
+      {
+        State uloc{};
+        for (int v = 0; v < nvar; ++v)
+          uloc[v] = u(Uin, ib, v, c);
+        st0 = source_term_factor(1, uloc, s, eos, dim, H, 0);
+        st1 = source_term_factor(1, uloc, s, eos, dim, H, 1);
+      }
       const int i = c % b, j = (c / b) % b, k = dim == 3 ? c / (b * b) : 0;
       int       sh[3] = { 0, 0, 0 };
       sh[dir] = 1;
@@ -1481,6 +1541,20 @@ g5o_compute_primitives(const g5o_params * p, const double * uin, double * qout)
   for (int v = 0; v < H.nbvar; ++v)
     qout[v] = q[v];
   return c;
+}
+
+// source term factors {st0, st1} of one cell's conservative state (p->source_term selects the model)
+void
+g5o_source_factors(const g5o_params * p, const double * uin, double * st2)
+{
+  Ids      H(p->dim);
+  Eos      e = make_eos(p->gamma, p->pinf);
+  Settings s{ p->cfl, p->slope_type, p->smallr, p->smallc, p->smallc * p->smallc };
+  State    u{};
+  for (int v = 0; v < H.nbvar; ++v)
+    u[v] = uin[v];
+  st2[0] = source_term_factor(p->source_term, u, s, e, p->dim, H, 0);
+  st2[1] = source_term_factor(p->source_term, u, s, e, p->dim, H, 1);
 }
 
 // flux out: nbvar values (IA0/IA1 = ustar*phistar), plus out3 = {ustar, phistar0, phistar1}

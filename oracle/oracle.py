@@ -36,6 +36,8 @@ class OracleParams(ctypes.Structure):
         ("inlet_u", ctypes.c_double * 9),
         ("xmin", ctypes.c_double * 3),
         ("jet", ctypes.c_double * 3),
+        ("source_term", ctypes.c_int32),
+        ("pad2", ctypes.c_int32),
     ]
 
 
@@ -76,6 +78,7 @@ def lib():
             "g5o_eos_sound_speed": (d, [P(d), P(d), d, d, d]),
             "g5o_compute_primitives": (d, [P(OracleParams), P(d), P(d)]),
             "g5o_riemann_hllc": (None, [P(OracleParams), P(d), P(d), P(d), P(d)]),
+            "g5o_source_factors": (None, [P(OracleParams), P(d), P(d)]),
             "g5o_slope": (d, [d, d, d, d]),
             "g5o_tanh": (d, [d]),
             "g5o_thinc_alpha": (d, [d, d, d, d, d, i32]),
@@ -114,6 +117,7 @@ def params_from(cfg):
     for k in range(3):
         p.xmin[k] = cfg.xmin[k]
         p.jet[k] = cfg.jet[k] if p.inlet else 0.0
+    p.source_term = getattr(cfg, "source_term", 0)
     return p
 
 
@@ -238,3 +242,12 @@ def riemann_hllc(cfg, ql, qr):
     out3 = np.zeros(3)
     lib().g5o_riemann_hllc(ctypes.byref(p), _dp(a), _dp(b), _dp(flux), _dp(out3))
     return flux, out3[0], out3[1], out3[2]
+
+
+def source_factors(cfg, u):
+    """(st0, st1) of the update's volume-fraction source term for one conservative state."""
+    p = params_from(cfg)
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(2)
+    lib().g5o_source_factors(ctypes.byref(p), _dp(uu), _dp(out))
+    return out[0], out[1]

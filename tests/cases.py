@@ -127,7 +127,8 @@ def random_case(seed):
     """A seeded small configuration for randomized parity: 2-D or 3-D, block counts from 1,
     per-face ZERO_GRADIENT / WALL or periodic directions, THINC or MUSCL, minmod or MC, HANCOCK /
     RK1 / RK2_SSP, ideal or stiffened gases, a shock-bubble initial state with transverse
-    velocity, and sometimes an inflow border.  Returns (cfg, U0, steps, label)."""
+    velocity, sometimes an inflow border and sometimes the MILLER_PUCKETT source term.
+    Returns (cfg, U0, steps, label)."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -171,6 +172,10 @@ def random_case(seed):
     x_front = float(rng.uniform(0.05, 0.35)) * ext[0]
     U0 = g5.init_shock_bubble(cfg, regs, x_front, centre, radius)
     steps = int(rng.integers(5, 13)) if dim == 2 else int(rng.integers(3, 7))
+    mp = bool(rng.random() < 0.25)  # drawn last, so the draws above do not depend on it
+    if mp:
+        cfg.source_term = g5.SOURCE_MILLER_PUCKETT
     label = (f"seed {seed}: {dim}d nb={nb} periodic={periodic[:dim]} bc={bc[:2 * dim]} thinc={thinc} "
-             f"slope={slope} {integrator} water={water} inlet={inlet[0] if inlet else None} steps={steps}")
+             f"slope={slope} {integrator} water={water} inlet={inlet[0] if inlet else None} "
+             f"source={'MILLER_PUCKETT' if mp else 'ALLAIRE_KOKH'} steps={steps}")
     return cfg, U0, steps, label

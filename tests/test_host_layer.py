@@ -71,6 +71,23 @@ def test_ini_jet_inlet(exe):
     assert parse(out)["inlet"] == "0"
 
 
+def test_ini_source_term_type(exe, tmp_path):
+    # [volume_fraction_advection] source_term_type (app/test_shock_bubble_air_water_2d.ini:106-108)
+    base = open(os.path.join(GOLD, "shock_bubble_air_helium_2d_512x128.ini")).read()
+    out = subprocess.run([exe, os.path.join(GOLD, "shock_bubble_air_helium_2d_512x128.ini")], check=True,
+                         capture_output=True, text=True).stdout
+    assert parse(out)["source_term"] == "0"
+    for value, want in (("MILLER_PUCKETT", "1"), ("ALLAIRE_KOKH", "0")):
+        ini = tmp_path / f"{value}.ini"
+        ini.write_text(base + f"\n[volume_fraction_advection]\nsource_term_type={value}\n")
+        out = subprocess.run([exe, str(ini)], check=True, capture_output=True, text=True).stdout
+        assert parse(out)["source_term"] == want
+    ini = tmp_path / "bad.ini"
+    ini.write_text(base + "\n[volume_fraction_advection]\nsource_term_type=KAPILA\n")
+    out = subprocess.run([exe, str(ini)], capture_output=True, text=True)
+    assert out.returncode != 0 and "KAPILA" in out.stderr
+
+
 def _init_state(exe, ini, tmp_path):
     raw = tmp_path / "u.bin"
     out = subprocess.run([exe, os.path.join(GOLD, ini), "--init", str(raw)], check=True, capture_output=True,

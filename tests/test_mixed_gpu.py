@@ -193,3 +193,24 @@ def test_mixed_interface_advection_equilibrium(g5, thinc, pinf):
     _announce(f"mixed interface advection thinc={thinc} pinf={pinf:.0e}: max |p/p0-1| = {dp:.2e}, "
               f"max |u/u0-1| = {du:.2e}")
     assert dp <= MIXED_EQ_P[pinf] and du <= 1e-6
+
+
+def test_mixed_miller_puckett_accuracy(g5, orc):
+    # mixed precision with the MILLER_PUCKETT source term (factors in FP64 from the time-n state)
+    # against the FP64 oracle: density and energy within relative L1 1e-5
+    for cfg, U0, steps in (cases.air_helium_2d(nbx=8, nby=4, source_term="MILLER_PUCKETT", precision=g5.PREC_MIXED) + (40,),
+                           cases.air_water_3d(nb=3, source_term="MILLER_PUCKETT", precision=g5.PREC_MIXED) + (10,)):
+        _announce(f"mixed MILLER_PUCKETT dim={cfg.dim} steps={steps}")
+        o = orc.Oracle(cfg)
+        o.set_state(U0)
+        o.run(steps)
+        with g5.Context(cfg) as ctx:
+            ctx.upload(U0)
+            ctx.run(steps)
+            U_g = ctx.download()
+        nv = 8 if cfg.dim == 2 else 9
+        A = U_g.reshape(-1, nv, 16 ** cfg.dim)
+        B = o.get_state().reshape(-1, nv, 16 ** cfg.dim)
+        err = [np.abs(A[:, v] - B[:, v]).sum() / np.abs(B[:, v]).sum() for v in (0, 1)]
+        _announce(f"  rel L1 rho {err[0]:.2e} E {err[1]:.2e}")
+        assert max(err) <= 1e-5

@@ -521,3 +521,73 @@ def test_inlet_override_of_the_outside_cells(orc, dim, kind):
         assert n_over == (cfg.nb[1] + 2) * ((cfg.nb[2] + 2) if dim == 3 else 1) * b ** dim
     else:
         assert n_over > 0
+
+
+def test_miller_puckett_source_factors(orc):
+    """get_source_term_factor (scheme/ReadFluxesAndConservativeUpdateFunctor.h:412-443):
+    ALLAIRE_KOKH gives st = 1 exactly; MILLER_PUCKETT st_k = 1 - K/K_k.  The bulk moduli live in
+    the reference's absent EOS class (parity unpinned): pinned here are the stiffened-gas K_k =
+    gamma_k (p + pinf_k) with p from the oracle's own pressure recovery (computePrimitives on a
+    consistent state), Wood's mixture modulus, and what they imply -- sum_k alpha_k st_k = 0, a
+    pure phase has st = 0, the stiffer phase st > 0 and the softer st < 0."""
+    import godunov_five_eq_amd as g5
+
+    rng = np.random.default_rng(7)
+    for dim in (2, 3):
+        for pinf in (0.0, 6e8):
+            base = dict(gamma=(1.4, 4.4), pinf=(0.0, pinf))
+            ak = g5.make_config(dim, (1,) * dim, 0.01, **base)
+            mp = g5.make_config(dim, (1,) * dim, 0.01, source_term="MILLER_PUCKETT", **base)
+            for _ in range(50):
+                a0 = float(rng.choice([0.0, 1.0, rng.uniform(0, 1)]))
+                p = float(rng.uniform(1e4, 1e9))
+                u = g5.region_conservative(mp, g5.region(a0, (a0 * rng.uniform(0.5, 2), (1 - a0) * rng.uniform(500, 1500)),
+                                                         p, *rng.uniform(-50, 50, 3)))
+                assert orc.source_factors(ak, u) == (1.0, 1.0)
+                st0, st1 = orc.source_factors(mp, u)
+                q, _ = orc.compute_primitives(mp, u)
+                pr = q[1]
+                k0, k1 = 1.4 * (pr + 0.0), 4.4 * (pr + pinf)
+                kbar = 1.0 / (a0 / k0 + (1 - a0) / k1)
+                assert st0 == pytest.approx(1 - kbar / k0, rel=1e-12, abs=1e-15)
+                assert st1 == pytest.approx(1 - kbar / k1, rel=1e-12, abs=1e-15)
+                assert abs(a0 * st0 + (1 - a0) * st1) <= 1e-14
+                if a0 == 1.0:
+                    assert abs(st0) <= 1e-15
+                elif a0 == 0.0:
+                    assert abs(st1) <= 1e-15
+                else:  # water (material 1) is the stiffer phase
+                    assert st1 > 0 > st0
+    # identical materials: K = K_k, so st = 0 (pure conservative alpha transport)
+    same = g5.make_config(2, (1, 1), 0.01, gamma=(1.4, 1.4), source_term="MILLER_PUCKETT")
+    u = g5.region_conservative(same, g5.region(0.3, (0.3, 0.7), 2.0, 1.0, -1.0))
+    st0, st1 = orc.source_factors(same, u)
+    assert abs(st0) <= 1e-15 and abs(st1) <= 1e-15
+
+
+@pytest.mark.parametrize("thinc", [False, True])
+def test_miller_puckett_interface_advection_and_conservation(orc, thinc):
+    # with the MILLER_PUCKETT factors the Abgrall criterion still holds (at uniform velocity the
+    # source st_k u* alpha_k cancels between the two faces of a cell), and mass, momentum and
+    # energy stay conserved in the periodic box; on a shock (non-uniform velocity) the volume
+    # fractions differ from ALLAIRE_KOKH's and stay in [0, 1]
+    cfg, U0, pressure = cases.interface_advection_2d(pinf=6e8, thinc=thinc, source_term="MILLER_PUCKETT")
+    o = orc.Oracle(cfg)
+    o.set_state(U0)
+    o.run(40)
+    U = o.get_state()
+    V = U.reshape(-1, 8, 256)
+    assert np.abs(pressure(U) / 1e5 - 1).max() <= 5e-11
+    assert np.abs(V[:, 2] / V[:, 0] / 100.0 - 1).max() <= 1e-13
+    V0 = U0.reshape(-1, 8, 256)
+    for v in (0, 1, 2, 3, 5, 7):
+        assert abs(V[:, v].sum() - V0[:, v].sum()) <= 1e-12 * np.abs(V0[:, v]).sum(), v
+    res = {}
+    for st in ("ALLAIRE_KOKH", "MILLER_PUCKETT"):
+        cfg, U0 = cases.air_helium_2d(nbx=6, nby=3, thinc=thinc, source_term=st)
+        o = orc.Oracle(cfg)
+        o.set_state(U0)
+        o.run(25)
+        res[st] = o.get_state().reshape(-1, 8, 256)
+        assert res[st][:, 4].min() >= 0.0 and res[st][:, 4].max() <= 1.0
+    assert np.abs(res["ALLAIRE_KOKH"][:, 4] - res["MILLER_PUCKETT"][:, 4]).max() > 1e-6

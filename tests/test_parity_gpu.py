@@ -164,6 +164,25 @@ def test_mixed_random_configs(g5, orc):
     _announce(f"worst density L1 {worst:.2e}")
 
 
+def test_parity_miller_puckett(g5, orc):
+    # the MILLER_PUCKETT volume-fraction source term (factors from each cell's time-n state) in
+    # the 2-D and 3-D step kernels, FP64, against the oracle: bit-identical
+    MP = dict(source_term="MILLER_PUCKETT")
+    rho2, u2, p2, _ = g5.rankine_hugoniot(4.4, 6e8, 1000.0, 1e5, p2=1e9)
+    cfg = g5.make_config(2, (8, 8), 1.0 / 128, bc=("WALL",) * 4 + ("NONE", "NONE"), cfl=0.5,
+                         gamma=(1.4, 4.4), pinf=(0.0, 6e8), thinc=True, thinc_beta=1.6, **MP)
+    regs = [g5.region(0.0, (0.0, rho2), p2, u2), g5.region(0.0, (0.0, 1000.0), 1e5),
+            g5.region(1.0, (1.0, 0.0), 1e5)]
+    runs = [("2d air-water walls THINC MP", (cfg, g5.init_shock_bubble(cfg, regs, 0.25, (0.5, 0.5), 0.2)), 40),
+            ("2d air-helium MC MP", cases.air_helium_2d(nbx=8, nby=4, slope_type=2.0, **MP), 30),
+            ("2d RK2_SSP MP", cases.air_helium_2d(nbx=6, nby=3, integrator="RK2_SSP", **MP), 10),
+            ("3d air-water THINC MP", cases.air_water_3d(nb=3, **MP), 8),
+            ("3d jet MP", cases.jet(3, inlet="jet", **MP), 6)]
+    for label, (c, U0), steps in runs:
+        U_g, U_o = run_both(c, U0, steps, g5, orc, label)
+        assert check_parity(U_g, U_o, c.dim, label), label
+
+
 def test_compute_dt_and_host_steps(g5, orc):
     # compute_dt + do_time_step (the drop-in GodunovImplemV0 surface) against the oracle
     cfg, U0 = cases.air_helium_2d(nbx=8, nby=4)

@@ -122,6 +122,29 @@ def test_pool_3d_inlet_and_plain_solves(g5, orc):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
+def test_pool_miller_puckett_beside_allaire_kokh(g5, orc, dim):
+    # one launch list with a MILLER_PUCKETT solve and an ALLAIRE_KOKH solve: the kernel variant
+    # with the factors runs both, each block taking its own solve's model; both bit-identical
+    say(f"pool: MILLER_PUCKETT beside ALLAIRE_KOKH, dim={dim}")
+    if dim == 2:
+        a = cases.air_helium_2d(nbx=6, nby=3, source_term="MILLER_PUCKETT")
+        b = cases.air_helium_2d(nbx=4, nby=4)
+    else:
+        a = cases.air_water_3d(nb=2, source_term="MILLER_PUCKETT")
+        b = cases.air_water_3d(nb=2)
+    with g5.Pool(dim, thinc=bool(a[0].thinc_enabled), budget_bytes=1 << 29) as pool:
+        ids = []
+        for cfg, U0 in (a, b):
+            sid = pool.admit(cfg, max_steps=6)
+            pool.upload(sid, U0)
+            ids.append(sid)
+        pool.start(ids)
+        pool.step(6)
+        for sid, (cfg, U0) in zip(ids, (a, b)):
+            assert np.array_equal(pool.download(sid), oracle_run(orc, cfg, U0, 6))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
 def test_device_initial_condition_matches_host(g5, dim):
     say(f"device IC dim={dim}")
     cfg, U0 = cases.air_helium_2d(nbx=4, nby=3) if dim == 2 else cases.air_water_3d(nb=2)
