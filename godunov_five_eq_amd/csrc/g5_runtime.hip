@@ -142,6 +142,8 @@ struct g5_pool
 
   double     last_run_ms = 0.0;
   hipEvent_t ev[2] = { nullptr, nullptr }; // around the last stepping call
+  SolveCtl * h_poll = nullptr;             // pinned: control record copies of run_solve's chunks
+  hipEvent_t ev_poll[2] = { nullptr, nullptr };
   bool       timing_pending = false;
 };
 
@@ -721,6 +723,60 @@ pool_step(g5_pool * p, int32_t nsteps, bool sync)
   return sync ? pool_timing(p) : G5_OK;
 }
 
+// Up to nsteps device-dt steps of solve `id`, stopping once it has finished: the steps go out in
+// chunks (64 steps, doubling up to 4096), each followed by an asynchronous copy of the solve's
+// control record into pinned memory.  The host waits for a chunk's record only after the next
+// chunk is queued, so the device never waits for the host, and stops queueing once a record shows
+// the solve inactive (the chunk still in flight then runs as no-op steps: every block returns at
+// once).  A run to tEnd with a large step budget thus costs the steps it takes.
+int
+pool_run_solve(g5_pool * p, int32_t id, int32_t nsteps)
+{
+  int rc = rebuild_list(p);
+  if (rc != G5_OK)
+    return rc;
+  rc = ensure_tuned(p, pool_hancock(p));
+  if (rc != G5_OK)
+    return rc;
+  if (p->ev[0] == nullptr)
+  {
+    G5_HIP(p->err, hipEventCreate(&p->ev[0]));
+    G5_HIP(p->err, hipEventCreate(&p->ev[1]));
+  }
+  if (p->h_poll == nullptr)
+  {
+    G5_HIP(p->err, hipHostMalloc(&p->h_poll, 2 * sizeof(SolveCtl), hipHostMallocDefault));
+    G5_HIP(p->err, hipEventCreateWithFlags(&p->ev_poll[0], hipEventDisableTiming));
+    G5_HIP(p->err, hipEventCreateWithFlags(&p->ev_poll[1], hipEventDisableTiming));
+  }
+  G5_HIP(p->err, hipEventRecord(p->ev[0], p->stream));
+  int32_t launched = 0, chunk = 64;
+  int     slot = 0;
+  bool    previous = false, done = false;
+  while (not done and launched < nsteps)
+  {
+    const int32_t n = std::min(chunk, nsteps - launched);
+    rc = pool_launch_steps(p, n);
+    if (rc != G5_OK)
+      return rc;
+    launched += n;
+    G5_HIP(p->err, hipMemcpyAsync(p->h_poll + slot, p->d_ctl + id, sizeof(SolveCtl), hipMemcpyDeviceToHost, p->stream));
+    G5_HIP(p->err, hipEventRecord(p->ev_poll[slot], p->stream));
+    if (previous) // the chunk before this one: finished?
+    {
+      G5_HIP(p->err, hipEventSynchronize(p->ev_poll[slot ^ 1]));
+      done = p->h_poll[slot ^ 1].active == 0;
+    }
+    previous = true;
+    slot ^= 1;
+    chunk = std::min(2 * chunk, 4096);
+  }
+  G5_HIP(p->err, hipEventRecord(p->ev[1], p->stream));
+  G5_HIP(p->err, hipGetLastError());
+  p->timing_pending = true;
+  return pool_timing(p);
+}
+
 // wait for the last stepping call and read its HIP-event duration
 int
 pool_timing(g5_pool * p)
@@ -949,6 +1005,11 @@ g5_pool_destroy(g5_pool * p)
   for (hipEvent_t e : p->ev)
     if (e)
       hipEventDestroy(e);
+  for (hipEvent_t e : p->ev_poll)
+    if (e)
+      hipEventDestroy(e);
+  if (p->h_poll)
+    hipHostFree(p->h_poll);
   hipFree(p->buf[0]);
   hipFree(p->buf[1]);
   hipFree(p->stage);
@@ -1567,7 +1628,7 @@ g5_run(g5_ctx * ctx, int32_t nsteps, double * t, double tEnd, int32_t * steps_do
   G5_HIP(ctx->err, hipMemcpyAsync(p->d_ctl + ctx->id, &c0, sizeof(SolveCtl), hipMemcpyHostToDevice, p->stream));
   rc = pool_start(p, &ctx->id, 1);
   if (rc == G5_OK)
-    rc = pool_step(p, nsteps, true);
+    rc = pool_run_solve(p, ctx->id, nsteps);
   if (rc == G5_OK)
     rc = read_ctl(p);
   if (rc != G5_OK)

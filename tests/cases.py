@@ -179,3 +179,66 @@ def random_case(seed):
              f"slope={slope} {integrator} water={water} inlet={inlet[0] if inlet else None} "
              f"source={'MILLER_PUCKETT' if mp else 'ALLAIRE_KOKH'} steps={steps}")
     return cfg, U0, steps, label
+
+
+def haas_sturtevant(level=5, **kw):
+    """The air-helium shock-bubble of app/test_shock_bubble_air_helium_2d/
+    test_shock_bubble_air_helium_2d_measure.ini on a fixed mesh: domain 445 x 89 mm (brick 5x1,
+    scaling 89), 16x16 blocks at `level` (dx = 89 / (16 * 2^level) mm), Mach 1.22 shock at
+    x = 170 mm, helium-air bubble of radius 25 mm at (220, 44.5), MC slopes, HLLC, THINC
+    beta 1.5, CFL 0.5, zero-gradient x borders and walls in y.  Units mm, ms, g and kPa-consistent
+    (velocities in mm/ms = m/s).  Returns (cfg, U0)."""
+    n = 2 ** level
+    dx = 89.0 / (16 * n)
+    opts = dict(bc=("ZERO_GRADIENT", "ZERO_GRADIENT", "WALL", "WALL", "NONE", "NONE"), cfl=0.5, slope_type=2.0,
+                gamma=(1.4, 1.648), pinf=(0.0, 0.0), thinc=True, thinc_beta=1.5, thinc_epsilon=1e-3)
+    opts.update(kw)
+    cfg = g5.make_config(2, (5 * n, n), dx, **opts)
+    regs = [g5.region(1.0, (1.6571422230442021, 0.0), 159059.985, 114.51006445833735),
+            g5.region(1.0, (1.1839, 0.0), 101325.0),
+            g5.region(0.0, (0.0, 0.2193078324225865), 101325.0)]
+    U0 = g5.init_shock_bubble(cfg, regs, 170.0, (220.0, 44.5), 25.0)
+    return cfg, U0
+
+
+def wave_fronts(cfg, U):
+    """Positions (mm) of the five fronts of the reference's plot.py on a 2-D state of the
+    haas_sturtevant case: along the centreline y = 44.5 the upstream ("jet") and downstream
+    interfaces (helium fraction 1/2), the refracted shock inside the bubble and the transmitted
+    shock ahead of it (rightmost pressure rise of 5% above ambient on either side of the
+    downstream interface); along the bottom wall row the incident shock (same criterion)."""
+    import numpy as np
+
+    nbx, nby = cfg.nb[0], cfg.nb[1]
+    A = U.reshape(nby, nbx, 8, 16, 16).transpose(2, 0, 3, 1, 4).reshape(8, nby * 16, nbx * 16)  # var, y, x
+    ny, nx = A.shape[1], A.shape[2]
+    x = (np.arange(nx) + 0.5) * cfg.dx
+    rho, E, mu, mv, a0 = A[0], A[1], A[2], A[3], A[4]
+    g1 = a0 / 0.4 + (1 - a0) / 0.648
+    p = (E - 0.5 * (mu * mu + mv * mv) / rho) / g1
+    jc = ny // 2  # the centreline lies between rows jc-1 and jc
+    he = 1.0 - 0.5 * (a0[jc - 1] + a0[jc])
+    pc = 0.5 * (p[jc - 1] + p[jc])
+
+    def crossing(f, level, i0, i1, rightmost):
+        idx = np.nonzero(f[i0:i1] > level)[0]
+        if idx.size == 0:
+            return float("nan")
+        k = i0 + (idx[-1] if rightmost else idx[0])
+        # linear interpolation to the level between cells k and its outer neighbour
+        kn = k + 1 if rightmost else k - 1
+        if kn < 0 or kn >= nx or f[k] == f[kn]:
+            return float(x[k])
+        return float(x[k] + (x[kn] - x[k]) * (f[k] - level) / (f[k] - f[kn]))
+
+    inside = np.nonzero(he > 0.5)[0]
+    jet = crossing(he, 0.5, 0, nx, False)
+    down = crossing(he, 0.5, 0, nx, True)
+    idown = int(inside[-1]) if inside.size else nx - 1
+    ijet = int(inside[0]) if inside.size else 0
+    pref = 101325.0 * 1.05
+    refracted = crossing(pc, pref, ijet, idown + 1, True)
+    transmitted = crossing(pc, pref, idown + 1, nx, True)
+    shock = crossing(p[0], pref, 0, nx, True)
+    return {"jet": jet, "downstream_interface": down, "refracted": refracted, "transmitted": transmitted,
+            "shock": shock}

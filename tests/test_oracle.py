@@ -591,3 +591,29 @@ def test_miller_puckett_interface_advection_and_conservation(orc, thinc):
         res[st] = o.get_state().reshape(-1, 8, 256)
         assert res[st][:, 4].min() >= 0.0 and res[st][:, 4].max() <= 1.0
     assert np.abs(res["ALLAIRE_KOKH"][:, 4] - res["MILLER_PUCKETT"][:, 4]).max() > 1e-6
+
+
+def test_shock_bubble_fronts_against_reference_measurements(orc):
+    # the oracle on the reference's measured air-helium shock-bubble (app/test_shock_bubble_air_helium_2d/
+    # plot.py:21-74 via tests/golden/shock_bubble_air_helium_fronts.json) at level 3 (dx = 0.70 mm,
+    # 640 x 128 cells, ~1760 steps): every front within 2.1 mm (3 cells), mean deviation within
+    # 0.8 mm.  A wrong wave speed (EOS, HLLC, Rankine-Hugoniot state) moves the fronts by
+    # several mm over the 0.24 ms of the measurements.
+    import json
+    import os
+
+    ref = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                      "shock_bubble_air_helium_fronts.json")))
+    cfg, U0 = cases.haas_sturtevant(level=3)
+    o = orc.Oracle(cfg)
+    o.set_state(U0)
+    t = 0.0
+    fronts = {}
+    for tk in sorted({round(t, 3) for k in ref if isinstance(ref[k], dict) for t in ref[k]["t"]}):
+        t, _ = o.run(10 ** 7, t, tk)
+        assert t == tk
+        fronts[tk] = cases.wave_fronts(cfg, o.get_state())
+    for name in ("shock", "refracted", "jet", "downstream_interface", "transmitted"):
+        d = ref[name]
+        err = np.array([fronts[round(tk, 3)][name] - xk for tk, xk in zip(d["t"], d["x"])])
+        assert np.abs(err).max() <= 2.1 and abs(err.mean()) <= 0.8, (name, err)

@@ -256,6 +256,15 @@ def test_run_stops_at_tend(g5, orc):
         ctx.upload(U0)
         t_g, n_g = ctx.run(n_o + 7, 0.0, t_end)
         U_g = ctx.download()
+        # a step budget far beyond the end time (the reference's nStepmax style): the run stops
+        # at tEnd instead of launching the whole budget
+        import time
+        ctx.upload(U0)
+        t0 = time.perf_counter()
+        t_b, n_b = ctx.run(2 ** 31 - 1, 0.0, t_end)
+        wall = time.perf_counter() - t0
+        assert n_b == n_o and t_b == t_o and np.array_equal(ctx.download(), U_g)
+        assert wall < 10.0, wall
     assert n_g == n_o and t_g == t_o
     check_parity(U_g, o.get_state(), 2, "tEnd clamp")
 
