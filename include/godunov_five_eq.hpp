@@ -454,6 +454,8 @@ public:
     , m_outputPrefix(config_map.getString("output", "outputPrefix", "output"))
     , m_write_variables(config_map.getStringVector("output", "write_variables", {}))
     , m_restart_prefix(config_map.getString("restart", "filePrefix", ""))
+    , m_hdf5_enabled(config_map.getBool("output", "hdf5_enabled", false))
+    , m_nOutput(config_map.getInteger("run", "nOutput", 0))
   {}
 
   // HDF5 + XDMF output of the state (save_solution_hdf5, SolverGodunovFiveEq.cpp:773-917):
@@ -551,17 +553,41 @@ public:
     m_cell_updates += num_cells();
   }
 
-  // the whole run with the device-resident time loop; returns Mcell-updates/s
+  // The whole run with the device-resident time loop, to tEnd or nStepmax; returns the
+  // Mcell-updates/s of the stepping.  With [output] hdf5_enabled and [run] nOutput > 0 the state
+  // is written (save_solution_hdf5, the [output] write_variables) at every iteration that is a
+  // multiple of nOutput and after the last step; nOutput < 0 writes after every step (the
+  // reference's .ini convention).  The scheduling of the reference's absent SolverBase
+  // (should_save_solution) is restated this way.
   double
   run()
   {
-    const auto t0 = std::chrono::steady_clock::now();
-    int32_t    done = 0;
-    check(g5_run(m_impl.ctx(), m_nStepmax - m_iteration, &m_t, m_tEnd, &done), m_impl.ctx());
-    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    m_iteration += done;
-    m_cell_updates += static_cast<uint64_t>(done) * num_cells();
-    return static_cast<double>(done) * num_cells() / s / 1e6;
+    const bool out = m_hdf5_enabled and m_nOutput != 0;
+    const int  every = m_nOutput < 0 ? 1 : m_nOutput;
+    double     s = 0.0;
+    uint64_t   stepped = 0;
+    int        last_saved = -1;
+    while (m_iteration < m_nStepmax and m_t < m_tEnd)
+    {
+      if (out and m_iteration % every == 0)
+      {
+        save_solution_hdf5(false);
+        last_saved = m_iteration;
+      }
+      const int n = out ? std::min(every - m_iteration % every, m_nStepmax - m_iteration) : m_nStepmax - m_iteration;
+      int32_t   done = 0;
+      const auto t0 = std::chrono::steady_clock::now();
+      check(g5_run(m_impl.ctx(), n, &m_t, m_tEnd, &done), m_impl.ctx());
+      s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      m_iteration += done;
+      stepped += static_cast<uint64_t>(done);
+      if (done == 0)
+        break;
+    }
+    if (out and last_saved != m_iteration)
+      save_solution_hdf5(false);
+    m_cell_updates += stepped * num_cells();
+    return s > 0.0 ? static_cast<double>(stepped) * num_cells() / s / 1e6 : 0.0;
   }
 
   uint64_t
@@ -603,6 +629,8 @@ private:
   std::string              m_outputDir, m_outputPrefix;
   std::vector<std::string> m_write_variables;
   std::string              m_restart_prefix;
+  bool                     m_hdf5_enabled;
+  int                      m_nOutput;
   double                 m_t = 0.0;
   int                    m_iteration = 0;
   uint64_t               m_cell_updates = 0;

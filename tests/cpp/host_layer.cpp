@@ -121,6 +121,28 @@ main(int argc, char ** argv)
     }
     return 0;
   }
+  if (argc >= 4 and std::strcmp(argv[2], "--outputs") == 0)
+  {
+    // run() with periodic HDF5 output: 40 steps, nOutput 10, into argv[3]
+    g5cpp::ConfigMap m = cm;
+    m.setString("run", "nStepmax", "40");
+    m.setString("run", "nOutput", "10");
+    m.setString("output", "hdf5_enabled", "true");
+    m.setString("output", "outputDir", argv[3]);
+    m.setString("output", "outputPrefix", "o");
+    m.setString("output", "write_variables", "rho_mix,alpha_0,e_tot");
+    if (cfg.dim != 2)
+      return 2;
+    g5cpp::SolverGodunovFiveEq<2> solver(m);
+    solver.init();
+    solver.run();
+    const auto   U = solver.get_state();
+    double       sum = 0.0;
+    for (double x : U)
+      sum += x;
+    std::printf("steps=%d t=%.17g sum=%.17g\n", solver.iteration(), solver.time(), sum);
+    return 0;
+  }
   if (argc >= 4 and std::strcmp(argv[2], "--restart") == 0)
     return cfg.dim == 2 ? run_restart<2>(cm, argv[3], 30, 20) : run_restart<3>(cm, argv[3], 6, 4);
   if (argc >= 4 and std::strcmp(argv[2], "--run") == 0)

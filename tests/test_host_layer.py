@@ -160,6 +160,27 @@ def test_cpp_checkpoint_restart(exe, tmp_path):
 
 
 @pytest.mark.gpu
+def test_cpp_run_writes_periodic_outputs(exe, tmp_path):
+    # run() with [output] hdf5_enabled and [run] nOutput = 10 over 40 steps: outputs at iterations
+    # 0, 10, 20, 30, 40 holding the [output] write_variables; the last one is the final state
+    print("[gpu] C++ run() with periodic HDF5 output", flush=True)
+    out = subprocess.run([exe, os.path.join(GOLD, "shock_bubble_air_helium_2d_512x128.ini"), "--outputs", str(tmp_path)],
+                         capture_output=True, text=True, timeout=120)
+    sys.stdout.write(out.stdout + out.stderr)
+    assert out.returncode == 0
+    kv = parse(out.stdout.splitlines()[-1])
+    assert kv["steps"] == "40"
+    names = sorted(f for f in os.listdir(tmp_path) if f.endswith(".h5"))
+    assert names == [f"o_{k:07d}.h5" for k in (0, 10, 20, 30, 40)]
+    assert all(os.path.exists(tmp_path / n.replace(".h5", ".xmf")) for n in names)
+    header = subprocess.run(["/opt/conda/bin/h5dump", "-H", str(tmp_path / "o_0000040.h5")], capture_output=True,
+                            text=True).stdout
+    if header:  # h5dump of the image: only the selected variables were written
+        assert 'DATASET "rho_mix"' in header and 'DATASET "alpha_0"' in header and 'DATASET "e_tot"' in header
+        assert 'DATASET "rho_vx"' not in header
+
+
+@pytest.mark.gpu
 def test_cpp_pool_class_runs(exe):
     print("[gpu] C++ PoolHip over seeded batch solves", flush=True)
     out = subprocess.run([exe, "--pool", "6"], capture_output=True, text=True, timeout=120)
