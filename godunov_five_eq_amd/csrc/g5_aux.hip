@@ -96,9 +96,10 @@ k_start(SolveCtl * ctl, SolveDesc const * solves, int32_t const * ids, int n)
   c.active = (c.t < c.tend and c.steps < c.max_steps) ? 1 : 0;
 }
 
-// after a device-dt step: the active solves advance, flip buffers and take their next dt
+// after a device-dt step: the active solves advance, flip buffers (unless the step wrote the new
+// state over the current one: the device RK2_SSP step) and take their next dt
 __global__ void
-k_finalize(SolveCtl * ctl, SolveDesc const * solves, int nsolves)
+k_finalize(SolveCtl * ctl, SolveDesc const * solves, int nsolves, int flip)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsolves)
@@ -111,7 +112,8 @@ k_finalize(SolveCtl * ctl, SolveDesc const * solves, int nsolves)
   }
   c.t = c.t + c.dt;
   c.steps += 1;
-  c.parity ^= 1;
+  if (flip)
+    c.parity ^= 1;
   c.dt = next_dt(c, solves[s]);
   c.invdt_bits = 0ull;
   if (not(c.t < c.tend and c.steps < c.max_steps))
@@ -146,9 +148,10 @@ launch_start(SolveCtl * ctl, SolveDesc const * solves, int32_t const * ids, int 
 }
 
 void
-launch_finalize(SolveCtl * ctl, SolveDesc const * solves, int nsolves, hipStream_t stream)
+launch_finalize(SolveCtl * ctl, SolveDesc const * solves, int nsolves, hipStream_t stream, bool flip)
 {
-  hipLaunchKernelGGL(k_finalize, dim3((nsolves + 255) / 256), dim3(256), 0, stream, ctl, solves, nsolves);
+  hipLaunchKernelGGL(k_finalize, dim3((nsolves + 255) / 256), dim3(256), 0, stream, ctl, solves, nsolves,
+                     flip ? 1 : 0);
 }
 
 void
@@ -248,6 +251,31 @@ launch_lincomb(double a, double const * U, double b, double * U2, size_t n, hipS
   if (nb > 65536)
     nb = 65536;
   hipLaunchKernelGGL(k_lincomb, dim3(static_cast<unsigned>(nb)), dim3(256), 0, stream, a, U, b, U2, n);
+}
+
+// end of a device-dt RK2_SSP step of one solve: U = 1/2 U2 + 1/2 U in place, U the buffer of the
+// solve's parity and U2 the other one (the same bits as 1/2 U + 1/2 U2: the two products are
+// rounded alike and the sum is commutative); nothing once the solve has finished
+__global__ void
+k_rk2_combine(double * buf0, double * buf1, size_t n, SolveCtl const * ctl)
+{
+  if (ctl->active == 0)
+    return;
+  const int            par = ctl->parity;
+  double * __restrict__ U = par ? buf1 : buf0;
+  double const * __restrict__ U2 = par ? buf0 : buf1;
+  for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n;
+       k += static_cast<size_t>(gridDim.x) * blockDim.x)
+    U[k] = 0.5 * U2[k] + 0.5 * U[k];
+}
+
+void
+launch_rk2_combine(double * buf0, double * buf1, size_t n, SolveCtl const * ctl, hipStream_t stream)
+{
+  size_t nb = (n + 255) / 256;
+  if (nb > 65536)
+    nb = 65536;
+  hipLaunchKernelGGL(k_rk2_combine, dim3(static_cast<unsigned>(nb)), dim3(256), 0, stream, buf0, buf1, n, ctl);
 }
 
 // ---------------------------------------------------------------------------------------------

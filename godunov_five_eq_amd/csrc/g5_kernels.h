@@ -24,7 +24,7 @@ void launch_inv_dt(int dim, bool mixed, double * buf0, double * buf1, BlockDesc 
 void launch_start(SolveCtl * ctl, SolveDesc const * solves, int32_t const * ids, int n, hipStream_t stream);
 
 // after a device-dt step: t += dt, steps++, parity flip, next dt, invdt_bits = 0
-void launch_finalize(SolveCtl * ctl, SolveDesc const * solves, int nsolves, hipStream_t stream);
+void launch_finalize(SolveCtl * ctl, SolveDesc const * solves, int nsolves, hipStream_t stream, bool flip = true);
 
 // after a host-dt step of every solve: parity flip
 void launch_flip(SolveCtl * ctl, int nsolves, hipStream_t stream);
@@ -43,6 +43,7 @@ void launch_fill_outside(int dim, double * buf0, double * buf1, SolveCtl const *
 
 // U2 = a*U + b*U2 over n doubles (RK2_SSP LinearCombination)
 void launch_lincomb(double a, double const * U, double b, double * U2, size_t n, hipStream_t stream);
+void launch_rk2_combine(double * buf0, double * buf1, size_t n, SolveCtl const * ctl, hipStream_t stream);
 
 // initial condition of one solve into its slots of the current buffer
 void launch_init_ic(int dim, double * buf0, double * buf1, SolveCtl const * ctl, int solve, SolveDesc const * solves,

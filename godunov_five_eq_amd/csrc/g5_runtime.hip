@@ -612,6 +612,24 @@ ensure_tuned(g5_pool * p, bool hancock)
   return rc;
 }
 
+int32_t
+first_live(g5_pool const * p)
+{
+  for (int32_t s = 0; s < static_cast<int32_t>(p->solves.size()); ++s)
+    if (p->solves[s].live)
+      return s;
+  return 0;
+}
+
+bool
+pool_rk2(g5_pool const * p)
+{
+  for (auto const & r : p->solves)
+    if (r.live)
+      return r.cfg.integrator == G5_TI_RK2_SSP;
+  return false;
+}
+
 bool
 pool_hancock(g5_pool const * p)
 {
@@ -633,14 +651,39 @@ capture_steps(g5_pool * p, int nsteps, hipGraphExec_t * out)
     (void)hipGraphExecDestroy(*out);
     *out = nullptr;
   }
+  // RK2_SSP (single-solve contexts, which have the stage buffer): per step, stage 1
+  // U_RK = U + dt L(U) into the stage buffer, stage 2 U2 = U_RK + dt L(U_RK) into the other
+  // buffer, the CFL rates of the stages discarded, U = 1/2 U2 + 1/2 U in place (the same bits as
+  // 1/2 U + 1/2 U2), the CFL rate of the new U, and the finalisation without a buffer flip; every
+  // buffer is chosen on the device from the solve's parity (SolverGodunovFiveEq.cpp:496-524)
+  const bool rk2 = p->stage != nullptr and pool_rk2(p);
   hipGraph_t g;
   G5_HIP(p->err, hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
   const StepArgs A = step_args(p, p->d_blocks, p->nlist, 0.0);
   const bool     hancock = pool_hancock(p);
+  const int      nsolves = static_cast<int>(p->solves.size());
   for (int k = 0; k < nsteps; ++k)
   {
-    launch_step(p, A, hancock);
-    launch_finalize(p->d_ctl, p->d_solves, static_cast<int>(p->solves.size()), p->stream);
+    if (rk2)
+    {
+      StepArgs S1 = A; // reads the current buffer (by the device parity)
+      S1.U_out = p->stage;
+      launch_step(p, S1, false);
+      StepArgs S2 = A; // writes the other buffer
+      S2.U_in = p->stage;
+      launch_step(p, S2, false);
+      launch_reset_invdt(p->d_ctl, nsolves, p->stream);
+      launch_rk2_combine(p->buf[0], p->buf[1], static_cast<size_t>(p->nslots) * p->block_doubles,
+                         p->d_ctl + first_live(p), p->stream);
+      launch_inv_dt(p->dim, p->precision == G5_PREC_MIXED, p->buf[0], p->buf[1], p->d_blocks, p->d_solves, p->d_ctl,
+                    p->nlist, p->stream);
+      launch_finalize(p->d_ctl, p->d_solves, nsolves, p->stream, false);
+    }
+    else
+    {
+      launch_step(p, A, hancock);
+      launch_finalize(p->d_ctl, p->d_solves, nsolves, p->stream);
+    }
   }
   G5_HIP(p->err, hipStreamEndCapture(p->stream, &g));
   G5_HIP(p->err, hipGraphInstantiate(out, g, nullptr, nullptr, 0));
@@ -1596,28 +1639,7 @@ g5_run(g5_ctx * ctx, int32_t nsteps, double * t, double tEnd, int32_t * steps_do
   if (nsteps < 0)
     return fail(ctx->err, G5_ERR_INVALID, "nsteps < 0");
   g5_pool * p = ctx->pool;
-  if (ctx->cfg.integrator == G5_TI_RK2_SSP)
-  {
-    // host-driven: the stage structure needs the dt of the step before the first stage
-    int n = 0;
-    for (; n < nsteps and *t < tEnd; ++n)
-    {
-      double dt;
-      int    rc = g5_compute_dt(ctx, &dt);
-      if (rc != G5_OK)
-        return rc;
-      if (*t + dt > tEnd)
-        dt = tEnd - *t;
-      rc = host_step(ctx, dt);
-      if (rc != G5_OK)
-        return rc;
-      *t = *t + dt;
-    }
-    if (steps_done)
-      *steps_done = n;
-    return g5_synchronize(ctx);
-  }
-  int rc = read_ctl(p);
+  int       rc = read_ctl(p);
   if (rc != G5_OK)
     return fail(ctx->err, rc, p->err);
   SolveCtl c0{};

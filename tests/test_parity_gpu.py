@@ -228,9 +228,30 @@ def test_fill_outside_matches_oracle(g5, orc):
 
 
 def test_rk2_ssp_parity(g5, orc):
+    # the device-dt RK2_SSP step (two stages, the 1/2 U + 1/2 U2 combination and the CFL rate of
+    # the new state in one graph) in 2-D and 3-D, FP64: bit-identical to the oracle
     cfg, U0 = cases.air_helium_2d(nbx=8, nby=4, integrator="RK2_SSP")
     U_g, U_o = run_both(cfg, U0, 10, g5, orc, "2d RK2_SSP")
-    check_parity(U_g, U_o, 2, "2d RK2_SSP 10 steps")
+    assert check_parity(U_g, U_o, 2, "2d RK2_SSP 10 steps")
+    cfg3, U3 = cases.air_water_3d(nb=2, integrator="RK2_SSP")
+    U_g, U_o = run_both(cfg3, U3, 6, g5, orc, "3d RK2_SSP THINC")
+    assert check_parity(U_g, U_o, 3, "3d RK2_SSP THINC 6 steps")
+    # host-driven steps (compute_dt + do_time_step, which flip the state buffer) followed by the
+    # device loop: the same bits as 10 oracle steps
+    _announce("RK2_SSP host steps then device loop")
+    o = orc.Oracle(cfg)
+    o.set_state(U0)
+    t_o, _ = o.run(10)
+    with g5.Context(cfg) as ctx:
+        ctx.upload(U0)
+        t = 0.0
+        for _ in range(3):
+            dt = ctx.compute_dt()
+            ctx.do_time_step(dt)
+            t += dt
+        t, n = ctx.run(7, t)
+        assert n == 7 and t == t_o
+        assert check_parity(ctx.download(), o.get_state(), 2, "RK2_SSP 3 host + 7 device steps")
 
 
 def test_gpu_deterministic(g5):
