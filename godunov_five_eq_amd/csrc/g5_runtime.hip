@@ -340,6 +340,9 @@ pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, i
     return fail(p->err, rc, msg);
   if (cfg.dim != p->dim or (cfg.thinc_enabled != 0) != (p->thinc != 0) or cfg.precision != p->precision)
     return fail(p->err, G5_ERR_INVALID, "solve does not match the pool (dim, THINC, precision)");
+  for (auto const & r : p->solves) // the step graph applies one integrator to the whole launch list
+    if (r.live and r.cfg.integrator != cfg.integrator)
+      return fail(p->err, G5_ERR_INVALID, "solve does not match the pool's integrator (one integrator per pool)");
   if (p->stage == nullptr and cfg.integrator == G5_TI_RK2_SSP)
     return fail(p->err, G5_ERR_UNSUPPORTED, "RK2_SSP needs a single-solve context");
   const int64_t nblk = nblocks_of(cfg);

@@ -324,6 +324,14 @@ def test_pool_error_paths(g5):
         with pytest.raises(g5.G5Error) as e:
             pool.admit(rk2)
         assert e.value.code == g5.G5_ERR_UNSUPPORTED
+        # one integrator per pool: the step graph applies it to the whole launch list
+        first = pool.admit(cfg2)
+        rk1, _ = cases.air_helium_2d(nbx=4, nby=2, integrator="RK1")
+        with pytest.raises(g5.G5Error) as e:
+            pool.admit(rk1)
+        assert e.value.code == g5.G5_ERR_INVALID and "integrator" in str(e.value)
+        pool.release(first)
+        pool.release(pool.admit(rk1))  # accepted once the pool is empty
         big, _ = cases.air_helium_2d(nbx=16, nby=16)  # 256 blocks > 128 slots
         assert pool.try_admit(big) is None
         sid = pool.admit(cfg2, max_steps=3)  # the pool still works after the refusals
