@@ -34,7 +34,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 # per-file: the 3-D z-march runs one wavefront per SIMD in FP64, where the ILP-oriented scheduler
 # hides more latency (measured 2.6% faster on the 256^3 FP64 step, neutral on the mixed one; the
-# 2-D step, at several wavefronts per SIMD, is 10% slower with it, so it keeps the default)
+# 2-D step, at several wavefronts per SIMD, is 10-21% slower with it, so it keeps the default)
 FILE_FLAGS = {"g5_step3d.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 
 
