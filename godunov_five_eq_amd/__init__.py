@@ -167,6 +167,7 @@ def lib():
             "g5_init_triple_point": (ctypes.c_int, [P(G5Config), P(G5Region), d, d, P(d), sz]),
             "g5_region_conservative": (ctypes.c_int, [P(G5Config), P(G5Region), P(d), sz]),
             "g5_pool_create": (ctypes.c_int, [i32, i32, i32, ctypes.c_uint64, P(vp)]),
+            "g5_pool_create_rk2": (ctypes.c_int, [i32, i32, i32, ctypes.c_uint64, P(vp)]),
             "g5_pool_destroy": (ctypes.c_int, [vp]),
             "g5_pool_last_error": (ctypes.c_char_p, [vp]),
             "g5_pool_admit": (ctypes.c_int, [vp, P(G5Config), d, i32, P(i32)]),
@@ -214,7 +215,7 @@ def exported_symbols():
         "g5_fill_outside", "g5_outside_coords", "g5_do_time_step", "g5_run", "g5_synchronize", "g5_last_run_ms",
         "g5_time_step_kernel", "g5_init_shock_bubble", "g5_init_triple_point", "g5_rankine_hugoniot",
         "g5_region_conservative",
-        "g5_pool_create", "g5_pool_destroy", "g5_pool_last_error", "g5_pool_admit", "g5_pool_release",
+        "g5_pool_create", "g5_pool_create_rk2", "g5_pool_destroy", "g5_pool_last_error", "g5_pool_admit", "g5_pool_release",
         "g5_pool_upload", "g5_pool_download", "g5_pool_init_shock_bubble", "g5_pool_init_triple_point",
         "g5_pool_start", "g5_pool_step", "g5_pool_status", "g5_pool_finished", "g5_pool_usage",
         "g5_pool_last_step_ms", "g5_pool_set_launch", "g5_pool_get_launch", "g5_pool_set_autotune",
@@ -479,10 +480,12 @@ def serve(seed, n_requests, rate_hz, *, frac_3d=0.1, share_3d=0.3, budget_bytes=
 class Pool:
     """Many independent solves of one dimension stepped together (g5_pool_* of the C ABI)."""
 
-    def __init__(self, dim, thinc=False, precision=PREC_FP64, budget_bytes=8 << 30):
+    def __init__(self, dim, thinc=False, precision=PREC_FP64, budget_bytes=8 << 30, rk2=False):
+        """rk2: also hold the RK2_SSP stage buffer (g5_pool_create_rk2), for RK2_SSP solves."""
         self.dim = dim
         self._h = ctypes.c_void_p()
-        rc = lib().g5_pool_create(dim, 1 if thinc else 0, precision, int(budget_bytes), ctypes.byref(self._h))
+        create = lib().g5_pool_create_rk2 if rk2 else lib().g5_pool_create
+        rc = create(dim, 1 if thinc else 0, precision, int(budget_bytes), ctypes.byref(self._h))
         if rc != G5_OK:
             raise G5Error(rc, lib().g5_pool_last_error(None).decode())
         self._cfg = {}

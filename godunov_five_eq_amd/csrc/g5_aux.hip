@@ -253,29 +253,35 @@ launch_lincomb(double a, double const * U, double b, double * U2, size_t n, hipS
   hipLaunchKernelGGL(k_lincomb, dim3(static_cast<unsigned>(nb)), dim3(256), 0, stream, a, U, b, U2, n);
 }
 
-// end of a device-dt RK2_SSP step of one solve: U = 1/2 U2 + 1/2 U in place, U the buffer of the
-// solve's parity and U2 the other one (the same bits as 1/2 U + 1/2 U2: the two products are
-// rounded alike and the sum is commutative); nothing once the solve has finished
-__global__ void
-k_rk2_combine(double * buf0, double * buf1, size_t n, SolveCtl const * ctl)
+// end of a device-dt RK2_SSP step, one workgroup per block of the launch list: U = 1/2 U2 + 1/2 U
+// in place, U the block's slot in the buffer of its solve's parity and U2 the same slot in the
+// other buffer (the same bits as 1/2 U + 1/2 U2: the two products are rounded alike and the sum
+// is commutative); nothing for a solve that has finished
+__global__ void __launch_bounds__(256)
+k_rk2_combine(double * buf0, double * buf1, BlockDesc const * blocks, SolveCtl const * ctl, int nblocks,
+              int block_doubles)
 {
-  if (ctl->active == 0)
+  const int b = blockIdx.x;
+  if (b >= nblocks)
     return;
-  const int            par = ctl->parity;
-  double * __restrict__ U = par ? buf1 : buf0;
-  double const * __restrict__ U2 = par ? buf0 : buf1;
-  for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; k < n;
-       k += static_cast<size_t>(gridDim.x) * blockDim.x)
+  const BlockDesc  bd = blocks[b];
+  SolveCtl const & c = ctl[bd.solve];
+  if (c.active == 0)
+    return;
+  const int64_t         base = static_cast<int64_t>(bd.slot) * block_doubles;
+  double * __restrict__ U = (c.parity ? buf1 : buf0) + base;
+  double const * __restrict__ U2 = (c.parity ? buf0 : buf1) + base;
+  for (int k = threadIdx.x; k < block_doubles; k += 256)
     U[k] = 0.5 * U2[k] + 0.5 * U[k];
 }
 
 void
-launch_rk2_combine(double * buf0, double * buf1, size_t n, SolveCtl const * ctl, hipStream_t stream)
+launch_rk2_combine(double * buf0, double * buf1, BlockDesc const * blocks, SolveCtl const * ctl, int nblocks,
+                   int block_doubles, hipStream_t stream)
 {
-  size_t nb = (n + 255) / 256;
-  if (nb > 65536)
-    nb = 65536;
-  hipLaunchKernelGGL(k_rk2_combine, dim3(static_cast<unsigned>(nb)), dim3(256), 0, stream, buf0, buf1, n, ctl);
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_rk2_combine, dim3(nblocks), dim3(256), 0, stream, buf0, buf1, blocks, ctl, nblocks,
+                       block_doubles);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -43,7 +43,8 @@ void launch_fill_outside(int dim, double * buf0, double * buf1, SolveCtl const *
 
 // U2 = a*U + b*U2 over n doubles (RK2_SSP LinearCombination)
 void launch_lincomb(double a, double const * U, double b, double * U2, size_t n, hipStream_t stream);
-void launch_rk2_combine(double * buf0, double * buf1, size_t n, SolveCtl const * ctl, hipStream_t stream);
+void launch_rk2_combine(double * buf0, double * buf1, BlockDesc const * blocks, SolveCtl const * ctl, int nblocks,
+                        int block_doubles, hipStream_t stream);
 
 // initial condition of one solve into its slots of the current buffer
 void launch_init_ic(int dim, double * buf0, double * buf1, SolveCtl const * ctl, int solve, SolveDesc const * solves,

@@ -344,7 +344,7 @@ pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, i
     if (r.live and r.cfg.integrator != cfg.integrator)
       return fail(p->err, G5_ERR_INVALID, "solve does not match the pool's integrator (one integrator per pool)");
   if (p->stage == nullptr and cfg.integrator == G5_TI_RK2_SSP)
-    return fail(p->err, G5_ERR_UNSUPPORTED, "RK2_SSP needs a single-solve context");
+    return fail(p->err, G5_ERR_UNSUPPORTED, "RK2_SSP needs the stage buffer (a single-solve context or g5_pool_create_rk2)");
   const int64_t nblk = nblocks_of(cfg);
   if (nblk > static_cast<int64_t>(p->free_slots.size()))
     return fail(p->err, G5_ERR_NOMEM, "HBM budget exhausted");
@@ -615,15 +615,6 @@ ensure_tuned(g5_pool * p, bool hancock)
   return rc;
 }
 
-int32_t
-first_live(g5_pool const * p)
-{
-  for (int32_t s = 0; s < static_cast<int32_t>(p->solves.size()); ++s)
-    if (p->solves[s].live)
-      return s;
-  return 0;
-}
-
 bool
 pool_rk2(g5_pool const * p)
 {
@@ -654,7 +645,7 @@ capture_steps(g5_pool * p, int nsteps, hipGraphExec_t * out)
     (void)hipGraphExecDestroy(*out);
     *out = nullptr;
   }
-  // RK2_SSP (single-solve contexts, which have the stage buffer): per step, stage 1
+  // RK2_SSP (pools with the stage buffer: single-solve contexts, g5_pool_create_rk2): per step, stage 1
   // U_RK = U + dt L(U) into the stage buffer, stage 2 U2 = U_RK + dt L(U_RK) into the other
   // buffer, the CFL rates of the stages discarded, U = 1/2 U2 + 1/2 U in place (the same bits as
   // 1/2 U + 1/2 U2), the CFL rate of the new U, and the finalisation without a buffer flip; every
@@ -676,8 +667,7 @@ capture_steps(g5_pool * p, int nsteps, hipGraphExec_t * out)
       S2.U_in = p->stage;
       launch_step(p, S2, false);
       launch_reset_invdt(p->d_ctl, nsolves, p->stream);
-      launch_rk2_combine(p->buf[0], p->buf[1], static_cast<size_t>(p->nslots) * p->block_doubles,
-                         p->d_ctl + first_live(p), p->stream);
+      launch_rk2_combine(p->buf[0], p->buf[1], p->d_blocks, p->d_ctl, p->nlist, p->block_doubles, p->stream);
       launch_inv_dt(p->dim, p->precision == G5_PREC_MIXED, p->buf[0], p->buf[1], p->d_blocks, p->d_solves, p->d_ctl,
                     p->nlist, p->stream);
       launch_finalize(p->d_ctl, p->d_solves, nsolves, p->stream, false);
@@ -1024,6 +1014,19 @@ g5_last_error(const g5_ctx * ctx)
 // =============================================================================================
 // pools
 // =============================================================================================
+int
+g5_pool_create_rk2(int32_t dim, int32_t thinc_enabled, int32_t precision, uint64_t hbm_budget_bytes, g5_pool ** out)
+{
+  if (not out)
+    return fail(g_create_error, G5_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (dim != 2 and dim != 3)
+    return fail(g_create_error, G5_ERR_INVALID, "dim must be 2 or 3");
+  if (precision != G5_PREC_FP64 and precision != G5_PREC_MIXED)
+    return fail(g_create_error, G5_ERR_INVALID, "unknown precision");
+  return pool_create(dim, thinc_enabled ? 1 : 0, precision, hbm_budget_bytes, true, out, g_create_error);
+}
+
 int
 g5_pool_create(int32_t dim, int32_t thinc_enabled, int32_t precision, uint64_t hbm_budget_bytes, g5_pool ** out)
 {

@@ -243,6 +243,11 @@ typedef struct g5_pool g5_pool;
 
 int g5_pool_create(int32_t dim, int32_t thinc_enabled, int32_t precision, uint64_t hbm_budget_bytes,
                    g5_pool ** out);
+/* A pool that also holds an RK2_SSP stage buffer (three state buffers per slot of the budget
+ * instead of two), so that it admits RK2_SSP solves (SolverGodunovFiveEq.cpp:496-524); like the
+ * other integrators, one integrator per pool. */
+int g5_pool_create_rk2(int32_t dim, int32_t thinc_enabled, int32_t precision, uint64_t hbm_budget_bytes,
+                       g5_pool ** out);
 int g5_pool_destroy(g5_pool * pool);
 const char * g5_pool_last_error(const g5_pool * pool);
 

@@ -144,6 +144,29 @@ def test_pool_miller_puckett_beside_allaire_kokh(g5, orc, dim):
             assert np.array_equal(pool.download(sid), oracle_run(orc, cfg, U0, 6))
 
 
+def test_pool_rk2_ssp_solves(g5, orc):
+    # RK2_SSP in a pool with the stage buffer: two solves of different sizes and step budgets
+    # (the first finishes while the second goes on), each bit-identical to the oracle
+    say("pool: RK2_SSP solves")
+    a = cases.air_helium_2d(nbx=6, nby=3, integrator="RK2_SSP")
+    b = cases.air_helium_2d(nbx=4, nby=5, integrator="RK2_SSP", slope_type=2.0, thinc=False)
+    with g5.Pool(2, budget_bytes=1 << 27, rk2=True) as pool:
+        ids = []
+        for (cfg, U0), n in ((a, 5), (b, 9)):
+            sid = pool.admit(cfg, max_steps=n)
+            pool.upload(sid, U0)
+            ids.append(sid)
+        pool.start(ids)
+        pool.step(12)
+        for sid, (cfg, U0), n in zip(ids, (a, b), (5, 9)):
+            assert pool.status(sid)[1] == n
+            assert np.array_equal(pool.download(sid), oracle_run(orc, cfg, U0, n))
+    with g5.Pool(2, budget_bytes=1 << 27) as pool:  # without the stage buffer: refused
+        with pytest.raises(g5.G5Error) as e:
+            pool.admit(a[0])
+        assert e.value.code == g5.G5_ERR_UNSUPPORTED
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_device_initial_condition_matches_host(g5, dim):
     say(f"device IC dim={dim}")
