@@ -640,10 +640,12 @@ private:
 class PoolHip
 {
 public:
-  PoolHip(int dim, bool thinc, int precision, uint64_t hbm_budget_bytes)
+  // rk2: also hold the RK2_SSP stage buffer (g5_pool_create_rk2), for RK2_SSP solves
+  PoolHip(int dim, bool thinc, int precision, uint64_t hbm_budget_bytes, bool rk2 = false)
   {
     g5_pool * p = nullptr;
-    const int rc = g5_pool_create(dim, thinc ? 1 : 0, precision, hbm_budget_bytes, &p);
+    const int rc = rk2 ? g5_pool_create_rk2(dim, thinc ? 1 : 0, precision, hbm_budget_bytes, &p)
+                       : g5_pool_create(dim, thinc ? 1 : 0, precision, hbm_budget_bytes, &p);
     if (rc != G5_OK)
       throw std::runtime_error(std::string("g5_pool_create: ") + g5_pool_last_error(nullptr));
     m_pool.reset(p);
