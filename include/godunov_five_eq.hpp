@@ -251,6 +251,11 @@ config_from_ini(ConfigMap const & c)
   {
     const std::string mat = "material" + std::to_string(k);
     const std::string eos = c.getString(mat, "mg_eos_name", "MG_IDEAL_GAS");
+    // the Mie-Grueneisen closures beyond these two (MG_JWL, MG_SHOCKWAVE2 of the reference's
+    // app files) are defined by the reference's EOS class, which is not in its tree
+    if (eos != "MG_IDEAL_GAS" and eos != "MG_STIFFENED_GAS")
+      throw std::runtime_error("[" + mat + "] mg_eos_name=" + eos +
+                               " is not available on this path (MG_IDEAL_GAS, MG_STIFFENED_GAS)");
     const std::string sec = mat + (eos == "MG_STIFFENED_GAS" ? "_mie_gruneisen_sg" : "_mie_gruneisen_ig");
     cfg.gamma[k] = c.getReal(sec, "gamma", 1.4);
     cfg.pinf[k] = eos == "MG_STIFFENED_GAS" ? c.getReal(sec, "pinf", 0.0) : 0.0;

@@ -88,6 +88,19 @@ def test_ini_source_term_type(exe, tmp_path):
     assert out.returncode != 0 and "KAPILA" in out.stderr
 
 
+def test_ini_unavailable_eos_is_refused(exe, tmp_path):
+    # MG_JWL / MG_SHOCKWAVE2 (app/test_two_fluid_shock_tube_2d/*.ini) are not restated: refused
+    # with the material named, not computed as an ideal gas
+    base = open(os.path.join(GOLD, "shock_bubble_air_helium_2d_512x128.ini")).read()
+    for eos in ("MG_JWL", "MG_SHOCKWAVE2"):
+        ini = tmp_path / f"{eos}.ini"
+        ini.write_text(base.replace("[material1]\nname=helium\nmg_eos_name=MG_IDEAL_GAS",
+                                    f"[material1]\nname=helium\nmg_eos_name={eos}"))
+        assert eos in ini.read_text()
+        out = subprocess.run([exe, str(ini)], capture_output=True, text=True)
+        assert out.returncode != 0 and eos in out.stderr and "material1" in out.stderr
+
+
 def _init_state(exe, ini, tmp_path):
     raw = tmp_path / "u.bin"
     out = subprocess.run([exe, os.path.join(GOLD, ini), "--init", str(raw)], check=True, capture_output=True,
