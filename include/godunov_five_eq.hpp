@@ -243,6 +243,8 @@ config_from_ini(ConfigMap const & c)
   cfg.smallr = hs.smallr;
   cfg.smallc = hs.smallc;
   const std::string ti = c.getString("amr", "time_integrator", "HANCOCK");
+  if (ti != "HANCOCK" and ti != "RK1" and ti != "RK2_SSP")
+    throw std::runtime_error("unknown time_integrator " + ti + " (HANCOCK, RK1, RK2_SSP)");
   cfg.integrator = ti == "RK2_SSP" ? G5_TI_RK2_SSP : (ti == "RK1" ? G5_TI_RK1 : G5_TI_HANCOCK);
   cfg.thinc_enabled = c.getBool("THINC", "enabled", false) ? 1 : 0;
   cfg.thinc_beta = c.getReal("THINC", "beta", 1.5);

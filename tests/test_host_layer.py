@@ -99,6 +99,10 @@ def test_ini_unavailable_eos_is_refused(exe, tmp_path):
         assert eos in ini.read_text()
         out = subprocess.run([exe, str(ini)], capture_output=True, text=True)
         assert out.returncode != 0 and eos in out.stderr and "material1" in out.stderr
+    ini = tmp_path / "rk3.ini"
+    ini.write_text(base.replace("time_integrator=HANCOCK", "time_integrator=RK3"))
+    out = subprocess.run([exe, str(ini)], capture_output=True, text=True)
+    assert out.returncode != 0 and "RK3" in out.stderr
 
 
 def _init_state(exe, ini, tmp_path):
