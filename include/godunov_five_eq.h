@@ -251,7 +251,9 @@ int g5_pool_create_rk2(int32_t dim, int32_t thinc_enabled, int32_t precision, ui
 int g5_pool_destroy(g5_pool * pool);
 const char * g5_pool_last_error(const g5_pool * pool);
 
-/* reserve memory for one solve (G5_ERR_NOMEM when the budget has no room); it starts inactive */
+/* reserve memory for one solve (G5_ERR_NOMEM when the budget has no room); it starts inactive.
+ * G5_ERR_INVALID when dim, THINC, precision or integrator differ from the pool's live solves;
+ * G5_ERR_UNSUPPORTED for RK2_SSP in a pool without the stage buffer (g5_pool_create_rk2). */
 int g5_pool_admit(g5_pool * pool, const g5_config * cfg, double t_end, int32_t max_steps, int32_t * solve_id);
 
 /* free the solve's memory and id */
