@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libg5.so")
 G5_OK, G5_ERR_INVALID, G5_ERR_HIP, G5_ERR_UNSUPPORTED, G5_ERR_NOMEM, G5_ERR_STATE, G5_ERR_IO = range(7)
 BC_NONE, BC_PERIODIC, BC_ZERO_GRADIENT, BC_WALL = range(4)
 TI_HANCOCK, TI_RK1, TI_RK2_SSP = range(3)
-PREC_FP64, PREC_MIXED = range(2)
+PREC_FP64, PREC_MIXED, PREC_FP64_FAST = range(3)
 
 BC_NAMES = {"NONE": BC_NONE, "PERIODIC": BC_PERIODIC, "ZERO_GRADIENT": BC_ZERO_GRADIENT, "WALL": BC_WALL}
 INLET_NONE, INLET_SHOCK_BUBBLE, INLET_JET = 0, 1, 2

@@ -17,7 +17,8 @@ H5LIB = os.path.join(HERE, "libg5h5.so")
 # HDF5 1.10 C library of the image (headers, libhdf5 and h5dump under /opt/conda)
 HDF5_PREFIX = os.environ.get("G5_HDF5_PREFIX", "/opt/conda")
 
-HIP_SOURCES = ["g5_step2d.hip", "g5_step3d.hip", "g5_aux.hip", "g5_runtime.hip"]
+HIP_SOURCES = ["g5_step2d.hip", "g5_step2d_fast.hip", "g5_step3d.hip", "g5_step3d_fast.hip", "g5_aux.hip",
+               "g5_runtime.hip"]
 CXX_SOURCES = ["g5_init.cpp", "g5_workloads.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -35,7 +36,8 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"
 # per-file: the 3-D z-march runs one wavefront per SIMD in FP64, where the ILP-oriented scheduler
 # hides more latency (measured 2.6% faster on the 256^3 FP64 step, neutral on the mixed one; the
 # 2-D step, at several wavefronts per SIMD, is 10-21% slower with it, so it keeps the default)
-FILE_FLAGS = {"g5_step3d.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
+FILE_FLAGS = {"g5_step3d.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+              "g5_step3d_fast.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 
 
 def _run(cmd, verbose):

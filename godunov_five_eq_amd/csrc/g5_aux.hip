@@ -15,8 +15,9 @@ cur_buf(double * b0, double * b1, int parity)
 // ComputeDtFiveEqFunctor (scheme/ComputeDtFiveEqFunctor.cpp:45-134): one workgroup per block,
 // per-cell CFL rate, wavefront butterfly max (__shfl_xor), LDS max over the 4 wavefronts,
 // one atomicMax per block on the bit pattern of the (non-negative) double.  T selects the FP64
-// rate or the mixed-path rate (FP32 sound speed from the FP64-recovered pressure).
-template <int DIM, typename T>
+// rate or the mixed-path rate (FP32 sound speed from the FP64-recovered pressure); FAST the rate of
+// the reassociated FP64 path.
+template <int DIM, typename T, bool FAST = false>
 __global__ void __launch_bounds__(256)
 k_inv_dt(double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves, SolveCtl * ctl,
          int nblocks)
@@ -34,7 +35,7 @@ k_inv_dt(double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const
   for (int c = threadIdx.x; c < NC; c += 256)
   {
     const Cons u = load_cons<DIM, 16>(U, static_cast<int64_t>(bd.slot) * NV * NC + c, -1);
-    inv = fmax(inv, cell_rate<DIM, T>(S, u));
+    inv = fmax(inv, cell_rate<DIM, T, FAST>(S, u));
   }
   inv = wave_max(inv);
   if ((threadIdx.x & 63) == 0)
@@ -47,27 +48,31 @@ k_inv_dt(double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const
   }
 }
 
+template <int DIM>
+static void
+launch_inv_dt_dim(int precision, double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves,
+                  SolveCtl * ctl, int nblocks, hipStream_t stream)
+{
+  const dim3 grid(nblocks), block(256);
+  if (precision == G5_PREC_MIXED)
+    hipLaunchKernelGGL((k_inv_dt<DIM, float>), grid, block, 0, stream, buf0, buf1, blocks, solves, ctl, nblocks);
+  else if (precision == G5_PREC_FP64_FAST)
+    hipLaunchKernelGGL((k_inv_dt<DIM, double, true>), grid, block, 0, stream, buf0, buf1, blocks, solves, ctl,
+                       nblocks);
+  else
+    hipLaunchKernelGGL((k_inv_dt<DIM, double>), grid, block, 0, stream, buf0, buf1, blocks, solves, ctl, nblocks);
+}
+
 void
-launch_inv_dt(int dim, bool mixed, double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves,
+launch_inv_dt(int dim, int precision, double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves,
               SolveCtl * ctl, int nblocks, hipStream_t stream)
 {
   if (nblocks <= 0)
     return;
-  const dim3 grid(nblocks), block(256);
   if (dim == 2)
-  {
-    if (mixed)
-      hipLaunchKernelGGL((k_inv_dt<2, float>), grid, block, 0, stream, buf0, buf1, blocks, solves, ctl, nblocks);
-    else
-      hipLaunchKernelGGL((k_inv_dt<2, double>), grid, block, 0, stream, buf0, buf1, blocks, solves, ctl, nblocks);
-  }
+    launch_inv_dt_dim<2>(precision, buf0, buf1, blocks, solves, ctl, nblocks, stream);
   else
-  {
-    if (mixed)
-      hipLaunchKernelGGL((k_inv_dt<3, float>), grid, block, 0, stream, buf0, buf1, blocks, solves, ctl, nblocks);
-    else
-      hipLaunchKernelGGL((k_inv_dt<3, double>), grid, block, 0, stream, buf0, buf1, blocks, solves, ctl, nblocks);
-  }
+    launch_inv_dt_dim<3>(precision, buf0, buf1, blocks, solves, ctl, nblocks, stream);
 }
 
 // SolverGodunovFiveEq::compute_dt_local (SolverGodunovFiveEq.cpp:386): dt = cfl / invDt, then

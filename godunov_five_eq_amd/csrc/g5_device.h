@@ -104,6 +104,48 @@ struct Par<float>
   G5D static float eps(SolveDesc const & S) { return S.thinc_epsf; }
 };
 
+// ---------------------------------------------------------------------------------------------
+// FP64 arithmetic of the reassociated path (G5_PREC_FP64_FAST).  The hardware reciprocal and
+// reciprocal square root estimates (v_rcp_f64, v_rsq_f64) are refined by Newton-Raphson /
+// Goldschmidt steps, each of which squares the relative error, to within about one ulp -- instead
+// of the IEEE division and square root sequences (scale, estimate, refinement, fused correction
+// and fixup), whose dependent chains are about twice as long.  Arguments are finite and > 0.
+// ---------------------------------------------------------------------------------------------
+G5D double
+rcp64(double b)
+{
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  return fma(r, e, r);
+}
+
+// sqrt(x) for x >= 0: g = x y, h = y/2 from y ~ 1/sqrt(x), one Goldschmidt step, then one
+// Newton correction of g with the residual x - g^2 (0 for x = 0)
+G5D double
+sqrt64(double x)
+{
+  const double y = __builtin_amdgcn_rsq(x);
+  double       g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  const double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  return x == 0.0 ? 0.0 : g;
+}
+
+// 1/sqrt(x) for x > 0: one Goldschmidt step from the estimate
+G5D double
+rsqrt64(double x)
+{
+  const double y = __builtin_amdgcn_rsq(x);
+  const double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  return 2.0 * fma(h, r, h);
+}
+
 template <typename T>
 G5D T
 eos_xi(SolveDesc const & S, T a0, T a1)
@@ -221,14 +263,60 @@ cons_to_prim_mixed(SolveDesc const & S, Cons const & U, PrimT<float> & q)
   return eos_sound_speed(S, static_cast<float>(d), p, q.a, 1.0f - q.a);
 }
 
-// one name for both paths
+// Reassociated FP64 conversion (G5_PREC_FP64_FAST): the same quantities as cons_to_prim with one
+// reciprocal of the density for the velocities, rho e = E - m.u/2 (equal to d (E/d - |u|^2/2) in
+// exact arithmetic), one reciprocal of xi for the pressure and c^2 = ((xi+1) p + Pi)/(xi d) with
+// one reciprocal and the refined square root.  Same sanitising; returns the sound speed.
 template <int DIM>
+G5D double
+cons_to_prim_fast(SolveDesc const & S, Cons const & U, Prim & q)
+{
+  const double d = fmax(U.d, S.smallr);
+  const double inv_d = rcp64(d);
+  const double ux = U.mu * inv_d;
+  const double uy = U.mv * inv_d;
+  double       m2 = U.mu * ux + U.mv * uy;
+  double       uz = 0.0;
+  if (DIM == 3)
+  {
+    uz = U.mw * inv_d;
+    m2 += U.mw * uz;
+  }
+  const double pxi = (U.e - 0.5 * m2) - eos_pi(S, U.a0, 1.0 - U.a0);
+  const double p = pxi * rcp64(eos_xi(S, U.a0, 1.0 - U.a0));
+  double       a = fmax(0.0, fmin(1.0, U.a0));
+  double       r0 = U.ar0, r1 = U.ar1;
+  const double eps = 1e-12;
+  if (r0 < eps)
+    r0 = 0.0;
+  if (r1 < eps)
+    r1 = 0.0;
+  if (a < eps)
+    a = 0.0;
+  if (a > 1.0 - eps)
+    a = 1.0;
+  q.r0 = r0;
+  q.r1 = r1;
+  q.a = a;
+  q.p = p;
+  q.u = ux;
+  q.v = uy;
+  q.w = uz;
+  const double x = eos_xi(S, a, 1.0 - a);
+  return sqrt64(((x + 1.0) * p + eos_pi(S, a, 1.0 - a)) * rcp64(x * d));
+}
+
+// one name for the three paths (FAST: the reassociated FP64 path; ignored by the mixed one)
+template <int DIM, bool FAST = false>
 G5D double
 to_prim(SolveDesc const & S, Cons const & U, PrimT<double> & q)
 {
-  return cons_to_prim<DIM>(S, U, q);
+  if constexpr (FAST)
+    return cons_to_prim_fast<DIM>(S, U, q);
+  else
+    return cons_to_prim<DIM>(S, U, q);
 }
-template <int DIM>
+template <int DIM, bool FAST = false>
 G5D float
 to_prim(SolveDesc const & S, Cons const & U, PrimT<float> & q)
 {
@@ -281,16 +369,44 @@ struct HancockT // 0.5 * (sr0_0, sr0_1, sp0, su0, sv0, sw0)
 };
 using Hancock = HancockT<double>;
 
-template <int DIM, typename T>
+// FAST (reassociated FP64 path): the pressure-gradient terms multiply by one reciprocal of r, and
+// r c^2 is the mixture's rho c^2 = ((xi+1) p + Pi)/xi itself (no square root).
+template <int DIM, typename T, bool FAST = false>
 G5D HancockT<T>
 hancock_terms(SolveDesc const & S, PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy,
               SlopeT<T> const & sz, T dtdx)
 {
   HancockT<T> h;
   const T     r = q.r0 + q.r1;
-  const T     c = eos_sound_speed(S, r, q.p, q.a, T(1.0) - q.a);
   const T     dtdy = dtdx, dtdz = dtdx;
   const T     uu = q.u, vv = q.v, ww = q.w;
+  if constexpr (FAST)
+  {
+    const double x = eos_xi(S, q.a, 1.0 - q.a);
+    const double rc2 = ((x + 1.0) * q.p + eos_pi(S, q.a, 1.0 - q.a)) * rcp64(x);
+    const double ri = rcp64(r);
+    double       sr0 = (-uu * (sx.r0 + sx.r1) - sx.u * r) * dtdx + (-vv * (sy.r0 + sy.r1) - sy.v * r) * dtdy;
+    double       su0 = (-uu * sx.u - sx.p * ri) * dtdx + (-vv * sy.u) * dtdy;
+    double       sv0 = (-uu * sx.v) * dtdx + (-vv * sy.v - sy.p * ri) * dtdy;
+    double       sp0 = (-uu * sx.p - sx.u * rc2) * dtdx + (-vv * sy.p - sy.v * rc2) * dtdy;
+    double       sw0 = 0.0;
+    if (DIM == 3)
+    {
+      sr0 += (-ww * (sz.r0 + sz.r1) - sz.w * r) * dtdz;
+      su0 += (-ww * sz.u) * dtdz;
+      sv0 += (-ww * sz.v) * dtdz;
+      sw0 = (-uu * sx.w) * dtdx + (-vv * sy.w) * dtdy + (-ww * sz.w - sz.p * ri) * dtdz;
+      sp0 += (-ww * sz.p - sz.w * rc2) * dtdz;
+    }
+    h.r0 = 0.5 * (sr0 * q.a);
+    h.r1 = 0.5 * (sr0 * (1.0 - q.a));
+    h.p = 0.5 * sp0;
+    h.u = 0.5 * su0;
+    h.v = 0.5 * sv0;
+    h.w = 0.5 * sw0;
+    return h;
+  }
+  const T c = eos_sound_speed(S, r, q.p, q.a, T(1.0) - q.a);
   if (DIM == 2)
   {
     const T sr0 = (-uu * (sx.r0 + sx.r1) - sx.u * r) * dtdx + (-vv * (sy.r0 + sy.r1) - sy.v * r) * dtdy;
@@ -425,11 +541,27 @@ fast_tanh(float x)
   const float t = (1.0f - e) / (1.0f + e);
   return x < 0.0f ? -t : t;
 }
+// reassociated FP64 path: the same expm1 with the final quotient as a product by rcp64
+G5D double
+det_tanh_fast(double x)
+{
+  const double a = fabs(x);
+  if (a > 22.0)
+    return x > 0 ? 1.0 : -1.0;
+  const double m = det_expm1_neg(-2.0 * a);
+  const double t = -m * rcp64(2.0 + m);
+  return x < 0 ? -t : t;
+}
+template <bool FAST = false>
 G5D double
 tanh_of(double x)
 {
-  return det_tanh(x);
+  if constexpr (FAST)
+    return det_tanh_fast(x);
+  else
+    return det_tanh(x);
 }
+template <bool FAST = false>
 G5D float
 tanh_of(float x)
 {
@@ -439,13 +571,22 @@ tanh_of(float x)
 // ---------------------------------------------------------------------------------------------
 // Youngs (1982) normal from the 3^d alpha stencil (same weights and order as the host oracle).
 // ---------------------------------------------------------------------------------------------
-template <typename T>
+// FAST (reassociated FP64 path): the components multiply by one reciprocal square root
+template <typename T, bool FAST = false>
 G5D void
 youngs2(T const (&a)[9], T & nx, T & ny)
 {
   // a[(di+1) + 3*(dj+1)]
   const T gx = ((a[2] + T(2.0) * a[5]) + a[8]) - ((a[0] + T(2.0) * a[3]) + a[6]);
   const T gy = ((a[6] + T(2.0) * a[7]) + a[8]) - ((a[0] + T(2.0) * a[1]) + a[2]);
+  if constexpr (FAST)
+  {
+    const double n2 = gx * gx + gy * gy;
+    const double rn = n2 > 0.0 ? rsqrt64(n2) : 0.0;
+    nx = gx * rn;
+    ny = gy * rn;
+    return;
+  }
   const T nrm = sqrt(gx * gx + gy * gy);
   if (nrm > T(0.0))
   {
@@ -481,13 +622,22 @@ youngs_plane(T const (&a)[27], int s)
   return acc;
 }
 
-template <typename T>
+template <typename T, bool FAST = false>
 G5D void
 youngs3(T const (&a)[27], T & nx, T & ny, T & nz)
 {
   const T gx = youngs_plane<0>(a, 1) - youngs_plane<0>(a, -1);
   const T gy = youngs_plane<1>(a, 1) - youngs_plane<1>(a, -1);
   const T gz = youngs_plane<2>(a, 1) - youngs_plane<2>(a, -1);
+  if constexpr (FAST)
+  {
+    const double n2 = gx * gx + gy * gy + gz * gz;
+    const double rn = n2 > 0.0 ? rsqrt64(n2) : 0.0;
+    nx = gx * rn;
+    ny = gy * rn;
+    nz = gz * rn;
+    return;
+  }
   const T nrm = sqrt(gx * gx + gy * gy + gz * gz);
   if (nrm > T(0.0))
   {
@@ -513,7 +663,8 @@ struct ThincAxisT // per-cell, per-axis THINC quantities shared by the two faces
 };
 using ThincAxis = ThincAxisT<double>;
 
-template <typename T>
+// FAST (reassociated FP64 path): (phi - A)/B as a product by rcp64(B), the FAST tanh
+template <typename T, bool FAST = false>
 G5D ThincAxisT<T>
 thinc_axis(SolveDesc const & S, T qa, T am, T ac, T ap, T nd)
 {
@@ -526,15 +677,20 @@ thinc_axis(SolveDesc const & S, T qa, T am, T ac, T ap, T nd)
     const T beta = Par<T>::beta(S) * nd + T(0.01);
     t.A = (ap + am) / 2;
     t.B = (ap - am) / 2;
-    t.C = tanh_of(T(0.5) * beta);
+    t.C = tanh_of<FAST>(T(0.5) * beta);
     const T phi = ac;
-    t.D = tanh_of(T(0.5) * beta * (phi - t.A) / t.B);
+    if constexpr (FAST)
+      t.D = tanh_of<FAST>(T(0.5) * beta * (phi - t.A) * rcp64(t.B));
+    else
+      t.D = tanh_of(T(0.5) * beta * (phi - t.A) / t.B);
   }
   return t;
 }
 
-// reconstruct_state_{2d,3d} of the THINC functor: alpha-related part and MUSCL part for p, u, v, w
-template <int DIM, int AX, bool HANCOCK, typename T>
+// reconstruct_state_{2d,3d} of the THINC functor: alpha-related part and MUSCL part for p, u, v, w.
+// FAST (reassociated FP64 path): the quotients by C, 1 + b D, alpha and 1 - alpha as products by
+// reciprocals (those of C, alpha and 1 - alpha are shared by the faces of the cell).
+template <int DIM, int AX, bool HANCOCK, typename T, bool FAST = false>
 G5D FaceT<T>
 face_thinc(PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy, SlopeT<T> const & sz,
            HancockT<T> const & h, ThincAxisT<T> const & th, int ib, int ax = AX)
@@ -544,9 +700,18 @@ face_thinc(PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy, Slope
   if (th.req)
   {
     const T bb = static_cast<T>(ib);
-    f.a = th.A + bb * th.B * (th.C + bb * th.D / th.C) / (T(1.0) + bb * th.D);
-    f.r0 = q.r0 + q.r0 / q.a * (f.a - q.a);
-    f.r1 = q.r1 + q.r1 / (T(1.0) - q.a) * (q.a - f.a);
+    if constexpr (FAST)
+    {
+      f.a = th.A + bb * th.B * ((th.C + bb * th.D * rcp64(th.C)) * rcp64(1.0 + bb * th.D));
+      f.r0 = q.r0 + q.r0 * rcp64(q.a) * (f.a - q.a);
+      f.r1 = q.r1 + q.r1 * rcp64(1.0 - q.a) * (q.a - f.a);
+    }
+    else
+    {
+      f.a = th.A + bb * th.B * (th.C + bb * th.D / th.C) / (T(1.0) + bb * th.D);
+      f.r0 = q.r0 + q.r0 / q.a * (f.a - q.a);
+      f.r1 = q.r1 + q.r1 / (T(1.0) - q.a) * (q.a - f.a);
+    }
   }
   else
   {
@@ -600,7 +765,8 @@ hllc(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
   ecinl += 0.5 * rl * ulT1 * ulT1;
   if (DIM == 3)
     ecinl += 0.5 * rl * ulT2 * ulT2;
-  const double cl = eos_sound_speed(S, rl, pl, L.a, 1.0 - L.a);
+  const double xl = eos_xi(S, L.a, 1.0 - L.a);
+  const double ql = (((xl + 1.0) * pl + eos_pi(S, L.a, 1.0 - L.a)) / xl) / rl; // cl^2 before the root
   const double etotl = rl * eintl + ecinl;
   const double ptotl = pl;
 
@@ -613,27 +779,29 @@ hllc(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
   ecinr += 0.5 * rr * urT1 * urT1;
   if (DIM == 3)
     ecinr += 0.5 * rr * urT2 * urT2;
-  const double cr = eos_sound_speed(S, rr, pr, R.a, 1.0 - R.a);
+  const double xr = eos_xi(S, R.a, 1.0 - R.a);
+  const double qr = (((xr + 1.0) * pr + eos_pi(S, R.a, 1.0 - R.a)) / xr) / rr;
   const double etotr = rr * eintr + ecinr;
   const double ptotr = pr;
 
-  // cfast = sqrt(max(c*c, smallc*smallc)).  With correctly rounded multiplication and square root
-  // in radix 2, sqrt(RN(x*x)) == x for every x whose square neither overflows nor underflows, and
-  // RN(x*x) is monotone in x, so the expression equals max(c, smallc) bit for bit when smallc^2 is
-  // normal and c < 1e150.  The reference form is kept for the other cases (and for NaN).
-  double cfastl, cfastr;
-  if (smallc >= 1.5e-154 and cl < 1e150 and cr < 1e150)
-  {
-    cfastl = fmax(cl, smallc);
-    cfastr = fmax(cr, smallc);
-  }
+  // The wave speeds use max(cfastl, cfastr), cfast = sqrt(max(c*c, smallc*smallc)), c = sqrt(q).
+  // With correctly rounded multiplication and square root in radix 2, sqrt(RN(x*x)) == x for every
+  // x whose square neither overflows nor underflows, and RN(x*x) is monotone in x, so cfast equals
+  // max(c, smallc) bit for bit when smallc^2 is normal and c < 1e150; and since sqrt is correctly
+  // rounded, hence monotone, max(sqrt(ql), sqrt(qr)) == sqrt(max(ql, qr)) bit for bit (fmax skips
+  // a NaN operand on both sides of the identity).  So one square root gives max(cfastl, cfastr).
+  // The reference form is kept for the other cases.
+  const double qm = fmax(ql, qr);
+  double       cfast;
+  if (smallc >= 1.5e-154 and qm < 1e300)
+    cfast = fmax(sqrt(qm), smallc);
   else
   {
-    cfastl = sqrt(fmax(cl * cl, smallc * smallc));
-    cfastr = sqrt(fmax(cr * cr, smallc * smallc));
+    const double cl = sqrt(ql), cr = sqrt(qr);
+    cfast = fmax(sqrt(fmax(cl * cl, smallc * smallc)), sqrt(fmax(cr * cr, smallc * smallc)));
   }
-  const double SL = fmin(ul, ur) - fmax(cfastl, cfastr);
-  const double SR = fmax(ul, ur) + fmax(cfastl, cfastr);
+  const double SL = fmin(ul, ur) - cfast;
+  const double SR = fmax(ul, ur) + cfast;
 
   const double rcl = rl * (ul - SL);
   const double rcr = rr * (SR - ur);
@@ -710,6 +878,123 @@ hllc(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
   F.mv = fv * dtdx;
   F.mw = DIM == 3 ? fw * dtdx : 0.0;
   F.us = ustar * dtdx;
+  return F;
+}
+
+// Reassociated FP64 HLLC (G5_PREC_FP64_FAST): the same wave speeds, star state, sampling, flux,
+// upwinding and scaling as hllc, with
+//   - no e(p) -> p(e) round trip on each side (the identity in exact arithmetic): rho e = p xi + Pi;
+//   - c^2 = ((xi+1) p + Pi)/(xi rho), one reciprocal, and one square root for max(cl, cr);
+//   - u* and p* multiplied by one reciprocal of rcl + rcr, and the star factor and star energy of
+//     the sampled side by one reciprocal of S - u*.
+template <int DIM>
+G5D Flux
+hllc_fast(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
+{
+  const double ulN = AX == 0 ? L.u : (AX == 1 ? L.v : L.w);
+  const double ulT1 = AX == 1 ? L.u : L.v;
+  const double ulT2 = AX == 2 ? L.u : L.w;
+  const double urN = AX == 0 ? R.u : (AX == 1 ? R.v : R.w);
+  const double urT1 = AX == 1 ? R.u : R.v;
+  const double urT2 = AX == 2 ? R.u : R.w;
+  const double smallr = S.smallr, smallp = S.smallp, smallc = S.smallc;
+
+  const double rl = fmax(L.r0 + L.r1, smallr);
+  const double pl = fmax(L.p, rl * smallp);
+  const double ul = ulN;
+  const double xl = eos_xi(S, L.a, 1.0 - L.a), pil = eos_pi(S, L.a, 1.0 - L.a);
+  double       ecinl = 0.5 * rl * ul * ul;
+  ecinl += 0.5 * rl * ulT1 * ulT1;
+  if (DIM == 3)
+    ecinl += 0.5 * rl * ulT2 * ulT2;
+  const double ql = ((xl + 1.0) * pl + pil) * rcp64(xl * rl);
+  const double etotl = (pl * xl + pil) + ecinl;
+
+  const double rr = fmax(R.r0 + R.r1, smallr);
+  const double pr = fmax(R.p, rr * smallp);
+  const double ur = urN;
+  const double xr = eos_xi(S, R.a, 1.0 - R.a), pir = eos_pi(S, R.a, 1.0 - R.a);
+  double       ecinr = 0.5 * rr * ur * ur;
+  ecinr += 0.5 * rr * urT1 * urT1;
+  if (DIM == 3)
+    ecinr += 0.5 * rr * urT2 * urT2;
+  const double qr = ((xr + 1.0) * pr + pir) * rcp64(xr * rr);
+  const double etotr = (pr * xr + pir) + ecinr;
+
+  const double cfast = fmax(sqrt64(fmax(fmax(ql, qr), 0.0)), smallc);
+  const double SL = fmin(ul, ur) - cfast;
+  const double SR = fmax(ul, ur) + cfast;
+
+  const double rcl = rl * (ul - SL);
+  const double rcr = rr * (SR - ur);
+  const double inv_rc = rcp64(rcr + rcl);
+  const double ustar0 = (rcr * ur + rcl * ul + (pl - pr)) * inv_rc;
+  const double ptotstar = (rcr * pl + rcl * pr + rcl * rcr * (ul - ur)) * inv_rc;
+
+  double ro, ro_0, ro_1, uo, ptoto, etoto;
+  if (SL > 0.0)
+  {
+    ro_0 = L.r0;
+    ro_1 = L.r1;
+    ro = rl;
+    uo = ul;
+    ptoto = pl;
+    etoto = etotl;
+  }
+  else if (ustar0 > 0.0)
+  {
+    const double inv = rcp64(SL - ustar0);
+    const double factor = (SL - ul) * inv;
+    ro_0 = L.r0 * factor;
+    ro_1 = L.r1 * factor;
+    ro = rl * factor;
+    uo = ustar0;
+    ptoto = ptotstar;
+    etoto = ((SL - ul) * etotl - pl * ul + ptotstar * ustar0) * inv;
+  }
+  else if (SR > 0.0)
+  {
+    const double inv = rcp64(SR - ustar0);
+    const double factor = (SR - ur) * inv;
+    ro_0 = R.r0 * factor;
+    ro_1 = R.r1 * factor;
+    ro = rr * factor;
+    uo = ustar0;
+    ptoto = ptotstar;
+    etoto = ((SR - ur) * etotr - pr * ur + ptotstar * ustar0) * inv;
+  }
+  else
+  {
+    ro_0 = R.r0;
+    ro_1 = R.r1;
+    ro = rr;
+    uo = ur;
+    ptoto = pr;
+    etoto = etotr;
+  }
+
+  const double fd = (ro_0 + ro_1) * uo;
+  const double fn = ro * uo * uo + ptoto;
+  const double fe = (etoto + ptoto) * uo;
+  const bool   up = fd > 0.0;
+  const double ft1 = fd * (up ? ulT1 : urT1);
+  const double ft2 = DIM == 3 ? fd * (up ? ulT2 : urT2) : 0.0;
+  const double phi0 = up ? L.a : R.a;
+  const double phi1 = up ? 1.0 - L.a : 1.0 - R.a;
+
+  Flux F;
+  F.ar0 = (ro_0 * uo) * dtdx;
+  F.ar1 = (ro_1 * uo) * dtdx;
+  F.a0 = (uo * phi0) * dtdx;
+  F.a1 = (uo * phi1) * dtdx;
+  F.e = fe * dtdx;
+  const double fu = AX == 0 ? fn : (AX == 1 ? ft1 : ft2);
+  const double fv = AX == 1 ? fn : ft1;
+  const double fw = AX == 2 ? fn : ft2;
+  F.mu = fu * dtdx;
+  F.mv = fv * dtdx;
+  F.mw = DIM == 3 ? fw * dtdx : 0.0;
+  F.us = uo * dtdx;
   return F;
 }
 
@@ -826,13 +1111,16 @@ hllc_mixed(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, 
   return F;
 }
 
-template <int DIM>
+template <int DIM, bool FAST = false>
 G5D Flux
 riemann(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
 {
-  return hllc<DIM>(S, L, R, dtdx, AX);
+  if constexpr (FAST)
+    return hllc_fast<DIM>(S, L, R, dtdx, AX);
+  else
+    return hllc<DIM>(S, L, R, dtdx, AX);
 }
-template <int DIM>
+template <int DIM, bool FAST = false>
 G5D FluxT<float>
 riemann(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, double dtdx, int AX)
 {
@@ -974,25 +1262,27 @@ source_factors(SolveDesc const & S, Cons const & U, double & st0, double & st1)
   st1 = 1.0 - kbar / k1;
 }
 
-// scheme/FixVolumeFractionsNormalization.cpp:58-70
+// scheme/FixVolumeFractionsNormalization.cpp:58-70 (FAST: the reciprocal by rcp64)
+template <bool FAST = false>
 G5D void
 fix_volume_fractions(Cons & u)
 {
   const double sum_alpha = u.a0 + u.a1;
   const double sum_alpha_rho = u.ar0 + u.ar1;
-  const double inv = 1.0 / sum_alpha;
+  const double inv = FAST ? rcp64(sum_alpha) : 1.0 / sum_alpha;
   u.a0 *= inv;
   u.a1 *= inv;
   u.d = sum_alpha_rho;
 }
 
-// scheme/ComputeDtFiveEqFunctor.cpp:85-134 compute_cfl (one cell's contribution)
-template <int DIM>
+// scheme/ComputeDtFiveEqFunctor.cpp:85-134 compute_cfl (one cell's contribution; FAST: with the
+// reassociated conversion)
+template <int DIM, bool FAST = false>
 G5D double
 cell_inv_dt(SolveDesc const & S, Cons const & u)
 {
   Prim         q;
-  const double c = cons_to_prim<DIM>(S, u, q);
+  const double c = to_prim<DIM, FAST>(S, u, q);
   const double vx = c + fabs(q.u);
   const double vy = c + fabs(q.v);
   const double vz = DIM == 3 ? c + fabs(q.w) : 0.0;
@@ -1020,12 +1310,12 @@ cell_inv_dt_mixed(SolveDesc const & S, Cons const & u)
   return S.inv_dx_p2 > 0.0 ? static_cast<double>(v) * S.inv_dx_p2 : static_cast<double>(v) / S.dx;
 }
 
-template <int DIM, typename T>
+template <int DIM, typename T, bool FAST = false>
 G5D double
 cell_rate(SolveDesc const & S, Cons const & u)
 {
   if constexpr (sizeof(T) == sizeof(double))
-    return cell_inv_dt<DIM>(S, u);
+    return cell_inv_dt<DIM, FAST>(S, u);
   else
     return cell_inv_dt_mixed<DIM>(S, u);
 }

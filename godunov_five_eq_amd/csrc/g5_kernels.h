@@ -10,14 +10,19 @@ namespace g5
 
 // fused step (primitives, slopes, fluxes, update, normalisation, next-dt contribution); mixed
 // selects the FP32 primitive/face path
-void launch_step2d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool ring_wave, bool mp,
+// precision: G5_PREC_* (FP64 bit-exact, mixed, reassociated FP64)
+void launch_step2d(StepArgs const & A, bool thinc, bool hancock, int precision, bool ring_wave, bool mp,
                    hipStream_t stream);
-void launch_step3d(StepArgs const & A, bool thinc, bool hancock, bool mixed, bool zsplit, bool inlet, bool mp,
+void launch_step3d(StepArgs const & A, bool thinc, bool hancock, int precision, bool zsplit, bool inlet, bool mp,
                    hipStream_t stream);
+// the reassociated FP64 instances (separate translation units)
+void launch_step2d_fast(StepArgs const & A, bool thinc, bool hancock, bool ring_wave, bool mp, hipStream_t stream);
+void launch_step3d_fast(StepArgs const & A, bool thinc, bool hancock, bool zsplit, bool inlet, bool mp,
+                        hipStream_t stream);
 
 // max over the listed blocks of the CFL rate of the current state into ctl[solve].invdt_bits
-// (mixed: the rate of the mixed-precision path)
-void launch_inv_dt(int dim, bool mixed, double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves,
+// (the rate of the path of `precision`, G5_PREC_*)
+void launch_inv_dt(int dim, int precision, double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves,
                    SolveCtl * ctl, int nblocks, hipStream_t stream);
 
 // dt of the first step of solves ids[0..n) (all when ids is null) from invdt_bits; marks active

@@ -205,7 +205,7 @@ validate(g5_config const & c, std::string & msg)
     return msg = "gamma must be > 1", G5_ERR_INVALID;
   if (c.integrator != G5_TI_HANCOCK and c.integrator != G5_TI_RK1 and c.integrator != G5_TI_RK2_SSP)
     return msg = "unknown time integrator", G5_ERR_INVALID;
-  if (c.precision != G5_PREC_FP64 and c.precision != G5_PREC_MIXED)
+  if (c.precision != G5_PREC_FP64 and c.precision != G5_PREC_MIXED and c.precision != G5_PREC_FP64_FAST)
     return msg = "unknown precision", G5_ERR_INVALID;
   if (c.source_term != G5_SOURCE_ALLAIRE_KOKH and c.source_term != G5_SOURCE_MILLER_PUCKETT)
     return msg = "unknown volume-fraction source term", G5_ERR_INVALID;
@@ -488,11 +488,10 @@ launch_step(g5_pool * p, StepArgs const & A, bool hancock)
 {
   if (A.nblocks <= 0)
     return;
-  const bool mixed = p->precision == G5_PREC_MIXED;
   if (p->dim == 2)
-    launch_step2d(A, p->thinc != 0, hancock, mixed, p->launch.ring_wave != 0, p->any_mp, p->stream);
+    launch_step2d(A, p->thinc != 0, hancock, p->precision, p->launch.ring_wave != 0, p->any_mp, p->stream);
   else
-    launch_step3d(A, p->thinc != 0, hancock, mixed, p->launch.zsplit != 0, p->any_inlet, p->any_mp, p->stream);
+    launch_step3d(A, p->thinc != 0, hancock, p->precision, p->launch.zsplit != 0, p->any_inlet, p->any_mp, p->stream);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -668,7 +667,7 @@ capture_steps(g5_pool * p, int nsteps, hipGraphExec_t * out)
       launch_step(p, S2, false);
       launch_reset_invdt(p->d_ctl, nsolves, p->stream);
       launch_rk2_combine(p->buf[0], p->buf[1], p->d_blocks, p->d_ctl, p->nlist, p->block_doubles, p->stream);
-      launch_inv_dt(p->dim, p->precision == G5_PREC_MIXED, p->buf[0], p->buf[1], p->d_blocks, p->d_solves, p->d_ctl,
+      launch_inv_dt(p->dim, p->precision, p->buf[0], p->buf[1], p->d_blocks, p->d_solves, p->d_ctl,
                     p->nlist, p->stream);
       launch_finalize(p->d_ctl, p->d_solves, nsolves, p->stream, false);
     }
@@ -847,7 +846,7 @@ pool_start(g5_pool * p, int32_t const * ids, int32_t n)
                                 p->stream));
   G5_HIP(p->err, hipMemcpyAsync(p->d_start_ids, ids, n * sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
   // invdt of these solves is 0 (admit and finalize leave it cleared)
-  launch_inv_dt(p->dim, p->precision == G5_PREC_MIXED, p->buf[0], p->buf[1], p->d_start_list, p->d_solves, p->d_ctl,
+  launch_inv_dt(p->dim, p->precision, p->buf[0], p->buf[1], p->d_start_list, p->d_solves, p->d_ctl,
                 static_cast<int>(list.size()), p->stream);
   launch_start(p->d_ctl, p->d_solves, p->d_start_ids, n, p->stream);
   G5_HIP(p->err, hipGetLastError());
@@ -1022,7 +1021,7 @@ g5_pool_create_rk2(int32_t dim, int32_t thinc_enabled, int32_t precision, uint64
   *out = nullptr;
   if (dim != 2 and dim != 3)
     return fail(g_create_error, G5_ERR_INVALID, "dim must be 2 or 3");
-  if (precision != G5_PREC_FP64 and precision != G5_PREC_MIXED)
+  if (precision != G5_PREC_FP64 and precision != G5_PREC_MIXED and precision != G5_PREC_FP64_FAST)
     return fail(g_create_error, G5_ERR_INVALID, "unknown precision");
   return pool_create(dim, thinc_enabled ? 1 : 0, precision, hbm_budget_bytes, true, out, g_create_error);
 }
@@ -1035,7 +1034,7 @@ g5_pool_create(int32_t dim, int32_t thinc_enabled, int32_t precision, uint64_t h
   *out = nullptr;
   if (dim != 2 and dim != 3)
     return fail(g_create_error, G5_ERR_INVALID, "dim must be 2 or 3");
-  if (precision != G5_PREC_FP64 and precision != G5_PREC_MIXED)
+  if (precision != G5_PREC_FP64 and precision != G5_PREC_MIXED and precision != G5_PREC_FP64_FAST)
     return fail(g_create_error, G5_ERR_INVALID, "unknown precision");
   return pool_create(dim, thinc_enabled ? 1 : 0, precision, hbm_budget_bytes, false, out, g_create_error);
 }
@@ -1532,7 +1531,7 @@ g5_compute_dt(g5_ctx * ctx, double * dt)
   if (rc != G5_OK)
     return fail(ctx->err, rc, p->err);
   launch_reset_invdt(p->d_ctl, 1, p->stream);
-  launch_inv_dt(p->dim, p->precision == G5_PREC_MIXED, p->buf[0], p->buf[1], p->d_blocks, p->d_solves, p->d_ctl, p->nlist, p->stream);
+  launch_inv_dt(p->dim, p->precision, p->buf[0], p->buf[1], p->d_blocks, p->d_solves, p->d_ctl, p->nlist, p->stream);
   G5_HIP(ctx->err, hipGetLastError());
   rc = read_ctl(p);
   if (rc != G5_OK)

@@ -1,0 +1,478 @@
+// g5_step2d_body.h -- fused 2-D five-equation step, one 16x16 block per workgroup (FP64 or mixed:
+// the primitive/face path in the arithmetic T, the state accumulation in FP64).
+//
+// One launch does, for every owned block, what the reference does in seven launches
+// (scheme/GodunovImplemV0.cpp:83-151 plus the border fill and the next dt reduction):
+//   1. gather the 20x20 tile (ghost width 2) from the neighbour blocks, resolving the outside
+//      border conditions on the fly, and convert it to primitives into LDS
+//      (ConvertToPrimitivesVariablesFunctor + FillOutsideCellFunctor);
+//   2. limited slopes, sound speed and Hancock half-step terms per cell, then the reconstructed
+//      face states (ComputeLimitedSlopesFunctor + reconstruct_state_2d, MUSCL or THINC);
+//   3. HLLC on the 17x16 x-faces and 16x17 y-faces (riemann_hllc);
+//   4. flux exchange through LDS and the x then y conservative update with clamps,
+//      volume-fraction normalisation (ReadFluxesAndConservativeUpdateFunctor,
+//      FixVolumeFractionsNormalization);
+//   5. the CFL contribution of the new state, reduced over the wavefront with shuffles and over
+//      the workgroup in LDS, then one atomicMax per block (ComputeDtFiveEqFunctor).
+// Thread t owns interior cell (t%16, t/16); threads 0..63 (wave 0) also own the 64 ring cells
+// whose single face state the block needs, threads 0..143 convert the 144 halo cells.
+#pragma once
+
+#include "g5_device.h"
+#include "g5_kernels.h"
+
+namespace g5
+{
+
+namespace
+{
+
+constexpr int B2 = 16;
+constexpr int T2 = B2 + 4;   // tile edge
+constexpr int NT2 = T2 * T2; // 400 tile cells
+constexpr int NF2 = 17 * 16; // faces per direction
+
+G5D int
+tidx2(int gi, int gj)
+{
+  return (gi + 2) + T2 * (gj + 2);
+}
+// x-face fi in [0,16] of row j ; y-face fj in [0,16] of column i
+G5D int
+fx2(int fi, int j)
+{
+  return fi + 17 * j;
+}
+G5D int
+fy2(int i, int fj)
+{
+  return i + 16 * fj;
+}
+
+// halo cell h in [0,144): 64 predictor-ring cells then 80 outer cells
+G5D void
+halo_cell2(int h, int & gi, int & gj)
+{
+  if (h < 64)
+  {
+    const int r = h & 15, s = h >> 4;
+    gi = s == 0 ? -1 : (s == 1 ? 16 : r);
+    gj = s == 0 ? r : (s == 1 ? r : (s == 2 ? -1 : 16));
+    return;
+  }
+  int c = h - 64;
+  if (c < 4)
+  {
+    gi = (c & 1) ? 16 : -1;
+    gj = (c & 2) ? 16 : -1;
+    return;
+  }
+  c -= 4;
+  if (c < 40)
+  {
+    gi = (c % 20) - 2;
+    gj = c < 20 ? -2 : 17;
+    return;
+  }
+  c -= 40; // 36 cells: columns -2 and 17, rows -1..16
+  gi = c < 18 ? -2 : 17;
+  gj = (c % 18) - 1;
+}
+
+template <typename T>
+G5D PrimT<T>
+sq_load2(T const * sQ, int idx)
+{
+  PrimT<T> p;
+  p.r0 = sQ[0 * NT2 + idx];
+  p.r1 = sQ[1 * NT2 + idx];
+  p.a = sQ[2 * NT2 + idx];
+  p.p = sQ[3 * NT2 + idx];
+  p.u = sQ[4 * NT2 + idx];
+  p.v = sQ[5 * NT2 + idx];
+  p.w = T(0.0);
+  return p;
+}
+
+template <typename T>
+G5D void
+sq_store2(T * sQ, int idx, PrimT<T> const & p)
+{
+  sQ[0 * NT2 + idx] = p.r0;
+  sQ[1 * NT2 + idx] = p.r1;
+  sQ[2 * NT2 + idx] = p.a;
+  sQ[3 * NT2 + idx] = p.p;
+  sQ[4 * NT2 + idx] = p.u;
+  sQ[5 * NT2 + idx] = p.v;
+}
+
+// face states: 6 values (r0, r1, a, p, u, v) at slot k of a [6][NF2] array
+template <typename T>
+G5D void
+st_store2(T * A, int k, FaceT<T> const & f)
+{
+  A[0 * NF2 + k] = f.r0;
+  A[1 * NF2 + k] = f.r1;
+  A[2 * NF2 + k] = f.a;
+  A[3 * NF2 + k] = f.p;
+  A[4 * NF2 + k] = f.u;
+  A[5 * NF2 + k] = f.v;
+}
+template <typename T>
+G5D FaceT<T>
+st_load2(T const * A, int k)
+{
+  FaceT<T> f;
+  f.r0 = A[0 * NF2 + k];
+  f.r1 = A[1 * NF2 + k];
+  f.a = A[2 * NF2 + k];
+  f.p = A[3 * NF2 + k];
+  f.u = A[4 * NF2 + k];
+  f.v = A[5 * NF2 + k];
+  f.w = T(0.0);
+  return f;
+}
+// face records at slot k of a [7][NF2] array plus E[k]: the FP64 flux (ar0, ar1, a0, a1, mu, mv,
+// us; energy in E) or the mixed path's star state (ro0, ro1, ro, uo, pt, t1, phi0; etot in E)
+G5D void
+fl_store2(double * A, double * E, int k, Flux const & F)
+{
+  A[0 * NF2 + k] = F.ar0;
+  A[1 * NF2 + k] = F.ar1;
+  A[2 * NF2 + k] = F.a0;
+  A[3 * NF2 + k] = F.a1;
+  A[4 * NF2 + k] = F.mu;
+  A[5 * NF2 + k] = F.mv;
+  A[6 * NF2 + k] = F.us;
+  E[k] = F.e;
+}
+G5D Flux
+fl_load2(double const * A, double const * E, int k)
+{
+  Flux F;
+  F.ar0 = A[0 * NF2 + k];
+  F.ar1 = A[1 * NF2 + k];
+  F.a0 = A[2 * NF2 + k];
+  F.a1 = A[3 * NF2 + k];
+  F.mu = A[4 * NF2 + k];
+  F.mv = A[5 * NF2 + k];
+  F.mw = 0.0;
+  F.us = A[6 * NF2 + k];
+  F.e = E[k];
+  return F;
+}
+G5D void
+fl_store2(float * A, double * E, int k, FluxT<float> const & F)
+{
+  A[0 * NF2 + k] = F.ro0;
+  A[1 * NF2 + k] = F.ro1;
+  A[2 * NF2 + k] = F.ro;
+  A[3 * NF2 + k] = F.uo;
+  A[4 * NF2 + k] = F.pt;
+  A[5 * NF2 + k] = F.t1;
+  A[6 * NF2 + k] = F.phi0;
+  E[k] = F.etot;
+}
+G5D FluxT<float>
+fl_load2(float const * A, double const * E, int k)
+{
+  FluxT<float> F;
+  F.ro0 = A[0 * NF2 + k];
+  F.ro1 = A[1 * NF2 + k];
+  F.ro = A[2 * NF2 + k];
+  F.uo = A[3 * NF2 + k];
+  F.pt = A[4 * NF2 + k];
+  F.t1 = A[5 * NF2 + k];
+  F.t2 = 0.0f;
+  F.phi0 = A[6 * NF2 + k];
+  F.etot = E[k];
+  return F;
+}
+
+// the reconstructed face state of tile cell (gi,gj) on axis ax with side ib (+1 right, -1 left)
+template <bool THINC, bool HANCOCK, typename T, bool FAST>
+G5D void
+predict_faces2(SolveDesc const & S, T const * sQ, int gi, int gj, T dtdx, bool all, int ax1, int ib1,
+               FaceT<T> & fxm, FaceT<T> & fxp, FaceT<T> & fym, FaceT<T> & fyp, FaceT<T> & f1)
+{
+  const PrimT<T>  qc = sq_load2(sQ, tidx2(gi, gj));
+  const PrimT<T>  qxm = sq_load2(sQ, tidx2(gi - 1, gj));
+  const PrimT<T>  qxp = sq_load2(sQ, tidx2(gi + 1, gj));
+  const PrimT<T>  qym = sq_load2(sQ, tidx2(gi, gj - 1));
+  const PrimT<T>  qyp = sq_load2(sQ, tidx2(gi, gj + 1));
+  const T         st = static_cast<T>(S.slope_type);
+  const SlopeT<T> sx = slopes_of(qc, qxp, qxm, st);
+  const SlopeT<T> sy = slopes_of(qc, qyp, qym, st);
+  const SlopeT<T> sz = { T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
+  HancockT<T>     h = { T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
+  if (HANCOCK)
+    h = hancock_terms<2, T, FAST>(S, qc, sx, sy, sz, dtdx);
+  if (!THINC)
+  {
+    if (all)
+    {
+      fxm = face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, T(-0.5));
+      fxp = face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, T(0.5));
+      fym = face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, T(-0.5));
+      fyp = face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, T(0.5));
+    }
+    else
+    {
+      // one reconstruction on the lane's axis ax1
+      f1 = face_muscl<2, -1, HANCOCK>(S, qc, sx, sy, sz, h, T(0.5) * static_cast<T>(ib1), ax1);
+    }
+    return;
+  }
+  // THINC: Youngs normal from the 3x3 alpha stencil, needed only in mixed cells
+  T nx = T(0.0), ny = T(0.0);
+  if (qc.a > Par<T>::eps(S) and qc.a < T(1.0) - Par<T>::eps(S))
+  {
+    T a9[9];
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int di = -1; di <= 1; ++di)
+        a9[(di + 1) + 3 * (dj + 1)] = sQ[2 * NT2 + tidx2(gi + di, gj + dj)];
+    youngs2<T, FAST>(a9, nx, ny);
+  }
+  if (all)
+  {
+    const ThincAxisT<T> tx = thinc_axis<T, FAST>(S, qc.a, qxm.a, qc.a, qxp.a, fabs(nx));
+    const ThincAxisT<T> ty = thinc_axis<T, FAST>(S, qc.a, qym.a, qc.a, qyp.a, fabs(ny));
+    fxm = face_thinc<2, 0, HANCOCK, T, FAST>(qc, sx, sy, sz, h, tx, -1);
+    fxp = face_thinc<2, 0, HANCOCK, T, FAST>(qc, sx, sy, sz, h, tx, 1);
+    fym = face_thinc<2, 1, HANCOCK, T, FAST>(qc, sx, sy, sz, h, ty, -1);
+    fyp = face_thinc<2, 1, HANCOCK, T, FAST>(qc, sx, sy, sz, h, ty, 1);
+  }
+  else
+  {
+    const bool          x = ax1 == 0;
+    const ThincAxisT<T> t1 =
+      thinc_axis<T, FAST>(S, qc.a, x ? qxm.a : qym.a, qc.a, x ? qxp.a : qyp.a, fabs(x ? nx : ny));
+    f1 = face_thinc<2, -1, HANCOCK, T, FAST>(qc, sx, sy, sz, h, t1, ib1, ax1);
+  }
+}
+
+} // namespace
+
+// One LDS arena, in doubles.  Phases 1-3 use [primitives 6 x 400 | face states 12 x 272] (T);
+// phase 4 reuses it from the start for the face records (14 x 272 T + 2 x 272 double), once the
+// barrier after the Riemann solves has retired every read of the primitives and face states.
+// FP64: 45.3 KB, so three workgroups fit in a CU's 160 KB.
+template <typename T>
+constexpr int ARENA2_BYTES_A = (6 * NT2 + 12 * NF2) * static_cast<int>(sizeof(T));
+template <typename T>
+constexpr int ARENA2_BYTES_B = 14 * NF2 * static_cast<int>(sizeof(T)) + 2 * NF2 * 8;
+template <typename T>
+constexpr int ARENA2_DOUBLES =
+  ((ARENA2_BYTES_A<T> > ARENA2_BYTES_B<T> ? ARENA2_BYTES_A<T> : ARENA2_BYTES_B<T>)+7) / 8;
+
+// RW: a fifth wavefront (threads 256..319) does the 64 ring-cell face states and the 32 ring
+// Riemann problems in parallel with the owned cells instead of after them on wave 0 -- the form
+// for launch lists too short to fill the GPU, where a block's latency is the step time.
+template <bool THINC, bool HANCOCK, typename T, bool RW = false, bool MP = false, bool FAST = false>
+G5D void
+step2d_block(StepArgs const & A, int blk)
+{
+  __shared__ double sArena[ARENA2_DOUBLES<T>];
+  __shared__ double sRed[5];
+  T * const         sQ = reinterpret_cast<T *>(sArena);
+
+  const int       t = threadIdx.x;
+  const bool      owner = !RW or t < 256;       // lane with an owned cell
+  const int       tr = RW ? t - 256 : t;        // ring lane index when in [0, 64)
+  const bool      ring = RW ? !owner : t < 64; // lane with a ring cell
+  const BlockDesc bd = A.blocks[blk];
+  const SolveDesc S = A.solves[bd.solve];
+  SolveCtl *      C = A.ctl + bd.solve;
+  const int64_t   own = static_cast<int64_t>(bd.slot) * 8 * 256 + t;
+  const bool      host_dt = A.dt_fixed > 0.0;
+  if (!host_dt and C->active == 0)
+    return; // finished solve: its state stays in buffer `parity`
+  const int      par = C->parity;
+  double const * U = A.U_in ? A.U_in : (par ? A.buf1 : A.buf0);
+  double *       U2 = A.U_out ? A.U_out : (par ? A.buf0 : A.buf1);
+  const double   dt = host_dt ? A.dt_fixed : C->dt;
+  const double   dtdx = dt / S.dx;
+  const T        dtdxT = static_cast<T>(dtdx);
+  const int      i = t & 15, j = t >> 4;
+
+  // ---- 1. gather + primitives ----
+  Cons uown{};
+  if (owner)
+    uown = load_cons<2, B2>(U, own, -1);
+  if (owner)
+  {
+    PrimT<T> q;
+    to_prim<2, FAST>(S, uown, q);
+    sq_store2(sQ, tidx2(i, j), q);
+  }
+  if (t < 144)
+  {
+    int gi, gj, neg;
+    halo_cell2(t, gi, gj);
+    const int64_t off = resolve_cell<2, B2>(S, A.slots, bd.bx, bd.by, 0, gi, gj, 0, neg);
+    Cons          u = load_cons<2, B2>(U, off, -1);
+    apply_border<2>(S, u, neg);
+    PrimT<T>      q;
+    to_prim<2, FAST>(S, u, q);
+    sq_store2(sQ, tidx2(gi, gj), q);
+  }
+  __syncthreads();
+
+  // ---- 2. slopes, Hancock terms, face states ----
+  T *      XL = sQ + 6 * NT2; // [6][NF2] left states of the x faces
+  T *      YL = XL + 6 * NF2;  // [6][NF2] left states of the y faces
+  FaceT<T> fxm, fxp, fym, fyp, fdummy, fring;
+  if (owner)
+  {
+    predict_faces2<THINC, HANCOCK, T, FAST>(S, sQ, i, j, dtdxT, true, 0, 0, fxm, fxp, fym, fyp, fdummy);
+    st_store2(XL, fx2(i + 1, j), fxp);
+    st_store2(YL, fy2(i, j + 1), fyp);
+  }
+  // ring cells: s=0 (-1,r) +x ; s=1 (16,r) -x ; s=2 (r,-1) +y ; s=3 (r,16) -y
+  const int rs = (tr >> 4) & 3, rr = tr & 15;
+  if (ring)
+  {
+    int gi, gj;
+    halo_cell2(tr, gi, gj);
+    const int ax = rs < 2 ? 0 : 1;
+    const int ib = (rs & 1) ? -1 : 1;
+    predict_faces2<THINC, HANCOCK, T, FAST>(S, sQ, gi, gj, dtdxT, false, ax, ib, fdummy, fdummy, fdummy, fdummy,
+                                            fring);
+    if (rs == 0)
+      st_store2(XL, fx2(0, rr), fring);
+    else if (rs == 2)
+      st_store2(YL, fy2(rr, 0), fring);
+  }
+  __syncthreads();
+
+  // ---- 3. HLLC ----
+  FluxT<T> Fx, Fy;
+  if (owner)
+  {
+    Fx = riemann<2, FAST>(S, st_load2(XL, fx2(i, j)), fxm, dtdx, 0);
+    Fy = riemann<2, FAST>(S, st_load2(YL, fy2(i, j)), fym, dtdx, 1);
+  }
+  FluxT<T>   Fr;
+  const bool ring_face = ring and (rs & 1);
+  if (ring_face)
+  {
+    const int      ax = rs == 1 ? 0 : 1;
+    const FaceT<T> L = ax == 0 ? st_load2(XL, fx2(16, rr)) : st_load2(YL, fy2(rr, 16));
+    Fr = riemann<2, FAST>(S, L, fring, dtdx, ax);
+  }
+  __syncthreads();
+
+  // ---- 4. flux exchange and update ----
+  T *      FX = reinterpret_cast<T *>(sArena); // [7][NF2] (primitives and face states are dead)
+  T *      FY = FX + 7 * NF2;              // [7][NF2]
+  double * FXE = reinterpret_cast<double *>(FY + 7 * NF2);
+  double * FYE = FXE + NF2;
+  if (owner)
+  {
+    fl_store2(FX, FXE, fx2(i, j), Fx);
+    fl_store2(FY, FYE, fy2(i, j), Fy);
+  }
+  if (ring_face)
+  {
+    if (rs == 1)
+      fl_store2(FX, FXE, fx2(16, rr), Fr);
+    else
+      fl_store2(FY, FYE, fy2(rr, 16), Fr);
+  }
+  __syncthreads();
+  double rate = 0.0;
+  if (owner)
+  {
+    Cons   u2 = uown;
+    double st0 = 1.0, st1 = 1.0; // volume-fraction source factors (MILLER_PUCKETT: from the time-n cell)
+    if constexpr (MP) // the list holds a MILLER_PUCKETT solve; this block's solve decides
+      if (S.source_term == G5_SOURCE_MILLER_PUCKETT)
+        source_factors<2>(S, uown, st0, st1);
+    apply_face_pair<2>(u2, Fx, fl_load2(FX, FXE, fx2(i + 1, j)), uown.a0, uown.a1, dtdx, 0, st0, st1);
+    apply_face_pair<2>(u2, Fy, fl_load2(FY, FYE, fy2(i, j + 1)), uown.a0, uown.a1, dtdx, 1, st0, st1);
+    fix_volume_fractions<FAST>(u2);
+    store_cons<2, B2>(U2, own, u2);
+    rate = cell_rate<2, T, FAST>(S, u2);
+  }
+
+  // ---- 5. CFL of the new state (the ring wavefront has no cell) ----
+  double inv = wave_max(rate);
+  if ((t & 63) == 0)
+    sRed[t >> 6] = inv;
+  __syncthreads();
+  if (t == 0)
+  {
+    inv = fmax(fmax(sRed[0], sRed[1]), fmax(sRed[2], sRed[3]));
+    atomicMax(&C->invdt_bits, static_cast<unsigned long long>(__double_as_longlong(inv)));
+  }
+}
+
+// one workgroup per block
+template <bool THINC, bool HANCOCK, typename T, bool MP, bool FAST>
+__global__ void __launch_bounds__(256)
+k_step2d(StepArgs A)
+{
+  int b, end, stride;
+  block_range(A, b, end, stride);
+  if (b < end)
+    step2d_block<THINC, HANCOCK, T, false, MP, FAST>(A, b);
+}
+
+// one workgroup of five wavefronts per block (ring work on the fifth)
+template <bool THINC, bool HANCOCK, typename T, bool MP, bool FAST>
+__global__ void __launch_bounds__(320)
+k_step2d_ringwave(StepArgs A)
+{
+  int b, end, stride;
+  block_range(A, b, end, stride);
+  if (b < end)
+    step2d_block<THINC, HANCOCK, T, true, MP, FAST>(A, b);
+}
+
+// persistent workgroups looping over their range of the block list
+template <bool THINC, bool HANCOCK, typename T, bool MP, bool FAST>
+__global__ void __launch_bounds__(256)
+k_step2d_persistent(StepArgs A)
+{
+  int b, end, stride;
+  block_range(A, b, end, stride);
+  for (; b < end; b += stride)
+  {
+    step2d_block<THINC, HANCOCK, T, false, MP, FAST>(A, b);
+    __syncthreads(); // the next block reuses the LDS
+  }
+}
+
+// MP: the launch list holds a solve with the MILLER_PUCKETT source term
+template <bool THINC, bool HANCOCK, typename T, bool MP, bool FAST>
+static void
+launch2k(StepArgs const & A, bool ring_wave, hipStream_t stream)
+{
+  const dim3 block(256);
+  if (A.wg_per_xcd > 0)
+    hipLaunchKernelGGL((k_step2d_persistent<THINC, HANCOCK, T, MP, FAST>), dim3(8 * A.wg_per_xcd), block, 0, stream,
+                       A);
+  else if (ring_wave)
+    hipLaunchKernelGGL((k_step2d_ringwave<THINC, HANCOCK, T, MP, FAST>), dim3(8 * A.per_xcd), dim3(320), 0, stream,
+                       A);
+  else
+    hipLaunchKernelGGL((k_step2d<THINC, HANCOCK, T, MP, FAST>), dim3(8 * A.per_xcd), block, 0, stream, A);
+}
+
+template <typename T, bool MP, bool FAST>
+static void
+launch2(StepArgs const & A, bool thinc, bool hancock, bool ring_wave, hipStream_t stream)
+{
+  if (thinc and hancock)
+    launch2k<true, true, T, MP, FAST>(A, ring_wave, stream);
+  else if (thinc)
+    launch2k<true, false, T, MP, FAST>(A, ring_wave, stream);
+  else if (hancock)
+    launch2k<false, true, T, MP, FAST>(A, ring_wave, stream);
+  else
+    launch2k<false, false, T, MP, FAST>(A, ring_wave, stream);
+}
+
+} // namespace g5
