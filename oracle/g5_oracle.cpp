@@ -232,6 +232,9 @@ det_expm1_neg(double y) // y <= 0
 double
 det_tanh(double x)
 {
+#ifdef G5O_LIBM_TANH
+  return std::tanh(x); // the C library's tanh (variant build, see oracle/Makefile)
+#endif
   const double a = std::fabs(x);
   if (a > 22.0)
     return x > 0 ? 1.0 : -1.0;

@@ -10,6 +10,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libg5oracle.so")
+# the same restatement built with the C library's tanh in THINC instead of the deterministic
+# expm1-based one (G5O_LIBM_TANH): measures what the choice of tanh does to whole runs
+LIB_PATHS = {"": LIB_PATH, "libm_tanh": os.path.join(HERE, "_build", "libg5oracle_libm_tanh.so")}
 
 
 class OracleParams(ctypes.Structure):
@@ -41,7 +44,7 @@ class OracleParams(ctypes.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
 def build():
@@ -49,12 +52,12 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} missing: build the oracle (make -C oracle)")
-        L = ctypes.CDLL(LIB_PATH)
+def lib(variant=""):
+    if variant not in _libs:
+        path = LIB_PATHS[variant]
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: build the oracle (make -C oracle)")
+        L = ctypes.CDLL(path)
         P = ctypes.POINTER
         d, i32, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
         sig = {
@@ -88,8 +91,8 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def _dp(a):
@@ -122,18 +125,19 @@ def params_from(cfg):
 
 
 class Oracle:
-    def __init__(self, cfg):
+    def __init__(self, cfg, variant=""):
+        self.L = lib(variant)
         self.p = params_from(cfg)
-        self.h = lib().g5o_create(ctypes.byref(self.p))
-        self.nowned = lib().g5o_num_owned(self.h)
-        self.noutside = lib().g5o_num_outside(self.h)
+        self.h = self.L.g5o_create(ctypes.byref(self.p))
+        self.nowned = self.L.g5o_num_owned(self.h)
+        self.noutside = self.L.g5o_num_outside(self.h)
         self.nvar = 8 if cfg.dim == 2 else 9
         self.ncell = cfg.bsize ** cfg.dim
         self.nstate = self.nowned * self.nvar * self.ncell
 
     def close(self):
         if self.h:
-            lib().g5o_destroy(self.h)
+            self.L.g5o_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -145,38 +149,38 @@ class Oracle:
     def set_state(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64).ravel()
         assert U.size == self.nstate
-        lib().g5o_set_state(self.h, _dp(U))
+        self.L.g5o_set_state(self.h, _dp(U))
 
     def get_state(self):
         U = np.empty(self.nstate, dtype=np.float64)
-        lib().g5o_get_state(self.h, _dp(U))
+        self.L.g5o_get_state(self.h, _dp(U))
         return U
 
     def get_state_all(self):
         U = np.empty((self.nowned + self.noutside) * self.nvar * self.ncell, dtype=np.float64)
-        lib().g5o_get_state_all(self.h, _dp(U))
+        self.L.g5o_get_state_all(self.h, _dp(U))
         return U
 
     def block_coords(self):
         c = np.empty(3 * (self.nowned + self.noutside), dtype=np.int32)
-        lib().g5o_block_coords(self.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        self.L.g5o_block_coords(self.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
         return c.reshape(-1, 3)
 
     def fill_outside(self):
-        lib().g5o_fill_outside(self.h)
+        self.L.g5o_fill_outside(self.h)
 
     def compute_dt(self):
-        return lib().g5o_compute_dt(self.h)
+        return self.L.g5o_compute_dt(self.h)
 
     def compute_inv_dt(self):
-        return lib().g5o_compute_inv_dt(self.h)
+        return self.L.g5o_compute_inv_dt(self.h)
 
     def godunov_unsplit(self, dt):
-        lib().g5o_godunov_unsplit(self.h, dt)
+        self.L.g5o_godunov_unsplit(self.h, dt)
 
     def run(self, nsteps, t=0.0, t_end=1e300):
         tt = ctypes.c_double(t)
-        n = lib().g5o_run(self.h, nsteps, ctypes.byref(tt), t_end)
+        n = self.L.g5o_run(self.h, nsteps, ctypes.byref(tt), t_end)
         return tt.value, n
 
 
