@@ -13,7 +13,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libg5.so")
 
 # return codes and enums of include/godunov_five_eq.h
-G5_OK, G5_ERR_INVALID, G5_ERR_HIP, G5_ERR_UNSUPPORTED, G5_ERR_NOMEM, G5_ERR_STATE, G5_ERR_IO = range(7)
+(G5_OK, G5_ERR_INVALID, G5_ERR_HIP, G5_ERR_UNSUPPORTED, G5_ERR_NOMEM, G5_ERR_STATE, G5_ERR_IO, G5_ERR_DIVERGED,
+ G5_ERR_TOO_LARGE) = range(9)
+FAIL_NONFINITE, FAIL_NEGATIVE_PRESSURE = 1, 2
 BC_NONE, BC_PERIODIC, BC_ZERO_GRADIENT, BC_WALL = range(4)
 TI_HANCOCK, TI_RK1, TI_RK2_SSP = range(3)
 PREC_FP64, PREC_MIXED, PREC_FP64_FAST = range(3)
@@ -51,7 +53,7 @@ class G5Config(ctypes.Structure):
         ("inlet_u", ctypes.c_double * 9),
         ("jet", ctypes.c_double * 3),
         ("source_term", ctypes.c_int32),
-        ("source_pad", ctypes.c_int32),
+        ("abort_negative_pressure", ctypes.c_int32),
     ]
 
 
@@ -129,6 +131,22 @@ class G5ServeReport(ctypes.Structure):
         ("start_s", ctypes.c_double),
         ("wait_s", ctypes.c_double),
         ("collect_s", ctypes.c_double),
+        ("failed", ctypes.c_int32),
+        ("rejected", ctypes.c_int32),
+    ]
+
+
+SERVE_DONE, SERVE_FAILED, SERVE_REJECTED = 0, 1, 2
+
+
+class G5ServeResult(ctypes.Structure):
+    _fields_ = [
+        ("outcome", ctypes.c_int32),
+        ("failure", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("t", ctypes.c_double),
+        ("latency_ms", ctypes.c_double),
     ]
 
 
@@ -179,6 +197,7 @@ def lib():
             "g5_pool_start": (ctypes.c_int, [vp, P(i32), i32]),
             "g5_pool_step": (ctypes.c_int, [vp, i32]),
             "g5_pool_status": (ctypes.c_int, [vp, i32, P(d), P(i32), P(i32)]),
+            "g5_pool_failure": (ctypes.c_int, [vp, i32, P(i32)]),
             "g5_pool_finished": (ctypes.c_int, [vp, P(i32), i32, P(i32)]),
             "g5_pool_usage": (ctypes.c_int, [vp, P(ctypes.c_int64), P(ctypes.c_int64), P(i32)]),
             "g5_pool_last_step_ms": (ctypes.c_int, [vp, P(d)]),
@@ -197,6 +216,9 @@ def lib():
             "g5_spec_regions": (ctypes.c_int, [P(G5SolveSpec), P(G5Region)]),
             "g5_pool_admit_spec": (ctypes.c_int, [vp, P(G5SolveSpec), P(i32)]),
             "g5_serve": (ctypes.c_int, [P(G5ServeConfig), P(G5ServeReport), P(d), sz]),
+            "g5_serve_run": (ctypes.c_int, [P(G5ServeConfig), P(G5SolveSpec), P(G5ServeReport), P(G5ServeResult),
+                                            P(P(d))]),
+            "g5_spec_state_size": (ctypes.c_int, [P(G5SolveSpec), P(sz)]),
             "g5_rankine_hugoniot": (ctypes.c_int, [d, d, d, d, d, d, P(d), P(d), P(d), P(d)]),
         }
         for name, (res, args) in sig.items():
@@ -242,11 +264,12 @@ def make_config(dim, nb, dx, *, bsize=16, periodic=(0, 0, 0), bc=("ZERO_GRADIENT
                 integrator="HANCOCK", thinc=False, precision=PREC_FP64, cfl=0.5, slope_type=1.0,
                 smallr=1e-10, smallc=1e-10, gamma=(1.4, 1.4), pinf=(0.0, 0.0), thinc_beta=1.5,
                 thinc_epsilon=1e-3, xmin=(0.0, 0.0, 0.0), inlet=None, jet=(0.5, 0.5, 0.05),
-                source_term="ALLAIRE_KOKH"):
+                source_term="ALLAIRE_KOKH", abort_negative_pressure=False):
     """Build a G5Config; defaults follow models/HydroSettings.cpp:18-29.  inlet: None, or
     ("shock_bubble", region) / ("jet", region) -- outside cells with x < xmin (for the jet those
     whose centre is inside it, jet = (centre y, centre z, width)) take the region's state.
-    source_term: "ALLAIRE_KOKH" or "MILLER_PUCKETT" (volume-fraction source term of the update)."""
+    source_term: "ALLAIRE_KOKH" or "MILLER_PUCKETT" (volume-fraction source term of the update).
+    abort_negative_pressure: a negative pressure in a new state stops the solve (G5_ERR_DIVERGED)."""
     c = G5Config()
     c.dim = dim
     c.bsize = bsize
@@ -273,6 +296,7 @@ def make_config(dim, nb, dx, *, bsize=16, periodic=(0, 0, 0), bc=("ZERO_GRADIENT
     for k in range(3):
         c.jet[k] = jet[k]
     c.source_term = SOURCE_NAMES[source_term] if isinstance(source_term, str) else int(source_term)
+    c.abort_negative_pressure = 1 if abort_negative_pressure else 0
     if inlet is not None:
         kind, reg = inlet
         c.inlet = INLET_NAMES[kind]
@@ -477,6 +501,34 @@ def serve(seed, n_requests, rate_hz, *, frac_3d=0.1, share_3d=0.3, budget_bytes=
     return rep, lat
 
 
+def serve_run(specs, *, budget_bytes=200 << 30, share_3d=0.3, precision=PREC_MIXED, thinc=False,
+              steps_per_poll=4, autotune=False, keep_states=True):
+    """Serve the given requests (g5_serve_run); returns (report dict, list of per-request result
+    dicts, list of final states or None).  Arrivals must be in non-decreasing order."""
+    n = len(specs)
+    arr = (G5SolveSpec * n)(*specs)
+    c = G5ServeConfig(0, n, steps_per_poll, 0.0, 0.0, share_3d, int(budget_bytes), precision, 1 if thinc else 0,
+                      1 if autotune else 0, 0)
+    r = G5ServeReport()
+    res = (G5ServeResult * n)()
+    states = None
+    ptrs = None
+    if keep_states:
+        states = []
+        for sp in specs:
+            m = ctypes.c_size_t()
+            if lib().g5_spec_state_size(ctypes.byref(sp), ctypes.byref(m)) != G5_OK:
+                raise G5Error(G5_ERR_INVALID, "g5_spec_state_size")
+            states.append(np.zeros(m.value))
+        ptrs = (ctypes.POINTER(ctypes.c_double) * n)(*[_dptr(u) for u in states])
+    rc = lib().g5_serve_run(ctypes.byref(c), arr, ctypes.byref(r), res, ptrs)
+    if rc != G5_OK:
+        raise G5Error(rc, "g5_serve_run: " + lib().g5_pool_last_error(None).decode())
+    rep = {k: getattr(r, k) for k, _ in G5ServeReport._fields_}
+    out = [{k: getattr(x, k) for k, _ in G5ServeResult._fields_ if k != "pad"} for x in res]
+    return rep, out, states
+
+
 class Pool:
     """Many independent solves of one dimension stepped together (g5_pool_* of the C ABI)."""
 
@@ -567,9 +619,16 @@ class Pool:
         self._check(lib().g5_pool_step(self._h, nsteps))
 
     def status(self, sid):
+        """(t, steps, active); raises G5Error (code G5_ERR_DIVERGED) for a failed solve"""
         t, n, a = ctypes.c_double(), ctypes.c_int32(), ctypes.c_int32()
         self._check(lib().g5_pool_status(self._h, sid, ctypes.byref(t), ctypes.byref(n), ctypes.byref(a)))
         return t.value, n.value, bool(a.value)
+
+    def failure(self, sid):
+        """G5_FAIL_* bits of the solve (0: healthy)"""
+        f = ctypes.c_int32()
+        self._check(lib().g5_pool_failure(self._h, sid, ctypes.byref(f)))
+        return f.value
 
     def finished(self):
         n = ctypes.c_int32()

@@ -32,11 +32,16 @@ k_inv_dt(double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const
   constexpr int   NC = DIM == 2 ? 256 : 4096;
   constexpr int   NV = DIM == 2 ? 8 : 9;
   double          inv = 0.0;
+  uint32_t        fail = 0u;
   for (int c = threadIdx.x; c < NC; c += 256)
   {
-    const Cons u = load_cons<DIM, 16>(U, static_cast<int64_t>(bd.slot) * NV * NC + c, -1);
-    inv = fmax(inv, cell_rate<DIM, T, FAST>(S, u));
+    const Cons   u = load_cons<DIM, 16>(U, static_cast<int64_t>(bd.slot) * NV * NC + c, -1);
+    double       p;
+    const double rate = cell_rate<DIM, T, FAST>(S, u, p);
+    inv = fmax(inv, rate);
+    fail |= cell_failure(S, u, rate, p);
   }
+  report_failure(&ctl[bd.solve], fail);
   inv = wave_max(inv);
   if ((threadIdx.x & 63) == 0)
     sRed[threadIdx.x >> 6] = inv;
@@ -87,6 +92,14 @@ next_dt(SolveCtl & c, SolveDesc const & S)
   return dt;
 }
 
+// the CFL maximum of the new state gives a finite, positive dt (0 would be an infinite dt)
+G5D bool
+rate_ok(SolveCtl const & c)
+{
+  const double inv = __longlong_as_double(static_cast<long long>(c.invdt_bits));
+  return inv > 0.0 and inv <= 1.7976931348623157e308;
+}
+
 // first dt of the solves in ids[0..n) (all solves when ids is null)
 __global__ void
 k_start(SolveCtl * ctl, SolveDesc const * solves, int32_t const * ids, int n)
@@ -96,9 +109,11 @@ k_start(SolveCtl * ctl, SolveDesc const * solves, int32_t const * ids, int n)
     return;
   const int  s = ids ? ids[k] : k;
   SolveCtl & c = ctl[s];
+  if (not rate_ok(c))
+    c.failure |= static_cast<uint32_t>(G5_FAIL_NONFINITE);
   c.dt = next_dt(c, solves[s]);
   c.invdt_bits = 0ull;
-  c.active = (c.t < c.tend and c.steps < c.max_steps) ? 1 : 0;
+  c.active = (c.failure == 0u and c.t < c.tend and c.steps < c.max_steps) ? 1 : 0;
 }
 
 // after a device-dt step: the active solves advance, flip buffers (unless the step wrote the new
@@ -119,6 +134,15 @@ k_finalize(SolveCtl * ctl, SolveDesc const * solves, int nsolves, int flip)
   c.steps += 1;
   if (flip)
     c.parity ^= 1;
+  // a failed new state (its cells checked by the step kernels) stops the solve where it failed
+  if (not rate_ok(c))
+    c.failure |= static_cast<uint32_t>(G5_FAIL_NONFINITE);
+  if (c.failure != 0u)
+  {
+    c.active = 0;
+    c.invdt_bits = 0ull;
+    return;
+  }
   c.dt = next_dt(c, solves[s]);
   c.invdt_bits = 0ull;
   if (not(c.t < c.tend and c.steps < c.max_steps))

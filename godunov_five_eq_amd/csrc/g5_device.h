@@ -1279,10 +1279,11 @@ fix_volume_fractions(Cons & u)
 // reassociated conversion)
 template <int DIM, bool FAST = false>
 G5D double
-cell_inv_dt(SolveDesc const & S, Cons const & u)
+cell_inv_dt(SolveDesc const & S, Cons const & u, double & p)
 {
   Prim         q;
   const double c = to_prim<DIM, FAST>(S, u, q);
+  p = q.p;
   const double vx = c + fabs(q.u);
   const double vy = c + fabs(q.v);
   const double vz = DIM == 3 ? c + fabs(q.w) : 0.0;
@@ -1300,24 +1301,54 @@ cell_inv_dt(SolveDesc const & S, Cons const & u)
 // max reduction are FP64
 template <int DIM>
 G5D double
-cell_inv_dt_mixed(SolveDesc const & S, Cons const & u)
+cell_inv_dt_mixed(SolveDesc const & S, Cons const & u, double & p)
 {
   PrimT<float> q;
   const float  c = cons_to_prim_mixed<DIM>(S, u, q);
+  p = q.p;
   float        v = (c + fabsf(q.u)) + (c + fabsf(q.v));
   if (DIM == 3)
     v += c + fabsf(q.w);
   return S.inv_dx_p2 > 0.0 ? static_cast<double>(v) * S.inv_dx_p2 : static_cast<double>(v) / S.dx;
 }
 
+// the CFL rate of a cell of the path of T (and FAST); p receives the cell's mixture pressure
 template <int DIM, typename T, bool FAST = false>
 G5D double
-cell_rate(SolveDesc const & S, Cons const & u)
+cell_rate(SolveDesc const & S, Cons const & u, double & p)
 {
   if constexpr (sizeof(T) == sizeof(double))
-    return cell_inv_dt<DIM, FAST>(S, u);
+    return cell_inv_dt<DIM, FAST>(S, u, p);
   else
-    return cell_inv_dt_mixed<DIM>(S, u);
+    return cell_inv_dt_mixed<DIM>(S, u, p);
+}
+
+// G5_FAIL_* bits of one cell of a new state: a conservative value or a CFL rate that is not finite
+// (the conversion clamps the density and the volume fraction with fmax/fmin, which drop a NaN, so
+// the state itself is tested: the difference s - s of the sum of its values is 0 only when the sum
+// is finite; NaN compares false), and with the solve's check a negative pressure
+G5D uint32_t
+cell_failure(SolveDesc const & S, Cons const & u, double rate, double p)
+{
+  const double sum = ((u.d + u.e) + (u.mu + u.mv) + (u.mw + u.a0)) + ((u.ar0 + u.a1) + u.ar1);
+  const bool   ok = sum - sum == 0.0 and rate <= 1.7976931348623157e308;
+  uint32_t     f = ok ? 0u : static_cast<uint32_t>(G5_FAIL_NONFINITE);
+  if (S.abort_neg_p != 0 and p < 0.0)
+    f |= static_cast<uint32_t>(G5_FAIL_NEGATIVE_PRESSURE);
+  return f;
+}
+
+// OR of the failure bits of a wavefront's lanes into the solve's control record: two ballots,
+// and one atomic by the first failing lane only when a lane failed (every lane of the wavefront
+// calls this)
+G5D void
+report_failure(SolveCtl * C, uint32_t f)
+{
+  const unsigned long long b1 = __ballot((f & 1u) != 0u);
+  const unsigned long long b2 = __ballot((f & 2u) != 0u);
+  const unsigned long long any = b1 | b2;
+  if (any != 0ull and static_cast<int>(threadIdx.x & 63) == __ffsll(static_cast<long long>(any)) - 1)
+    atomicOr(&C->failure, (b1 != 0ull ? 1u : 0u) | (b2 != 0ull ? 2u : 0u));
 }
 
 // Blocks of the launch list handled by this workgroup: [b, end) with the given stride.  One

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -346,6 +347,10 @@ pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, i
   if (p->stage == nullptr and cfg.integrator == G5_TI_RK2_SSP)
     return fail(p->err, G5_ERR_UNSUPPORTED, "RK2_SSP needs the stage buffer (a single-solve context or g5_pool_create_rk2)");
   const int64_t nblk = nblocks_of(cfg);
+  if (nblk > p->nslots)
+    return fail(p->err, G5_ERR_TOO_LARGE,
+                "the solve has " + std::to_string(nblk) + " blocks, more than the pool's whole budget (" +
+                  std::to_string(p->nslots) + ")");
   if (nblk > static_cast<int64_t>(p->free_slots.size()))
     return fail(p->err, G5_ERR_NOMEM, "HBM budget exhausted");
   int32_t sid;
@@ -858,6 +863,30 @@ pool_start(g5_pool * p, int32_t const * ids, int32_t n)
 
 int read_ctl(g5_pool * p);
 
+// a new state of solve id (upload, device initial condition): clear its failure bits
+int
+clear_failure(g5_pool * p, int32_t id)
+{
+  static const uint32_t zero = 0u;
+  G5_HIP(p->err, hipMemcpyAsync(reinterpret_cast<char *>(p->d_ctl + id) + offsetof(SolveCtl, failure), &zero,
+                                sizeof(uint32_t), hipMemcpyHostToDevice, p->stream));
+  G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  return G5_OK;
+}
+
+// message of a failed solve
+std::string
+failure_message(int32_t id, SolveCtl const & c)
+{
+  std::string m = "solve " + std::to_string(id) + " stopped after " + std::to_string(c.steps) + " steps (t = " +
+                  std::to_string(c.t) + "):";
+  if (c.failure & G5_FAIL_NONFINITE)
+    m += " non-finite state or time step";
+  if (c.failure & G5_FAIL_NEGATIVE_PRESSURE)
+    m += " negative pressure";
+  return m;
+}
+
 int
 pool_transfer(g5_pool * p, int32_t id, double * host, size_t n, bool upload, double ** staging)
 {
@@ -883,7 +912,7 @@ pool_transfer(g5_pool * p, int32_t id, double * host, size_t n, bool upload, dou
     else
       G5_HIP(p->err, hipMemcpyAsync(host, base, need * sizeof(double), hipMemcpyDeviceToHost, p->stream));
     G5_HIP(p->err, hipStreamSynchronize(p->stream));
-    return G5_OK;
+    return upload ? clear_failure(p, id) : G5_OK;
   }
   double * st = staging ? *staging : nullptr;
   if (!st)
@@ -907,7 +936,7 @@ pool_transfer(g5_pool * p, int32_t id, double * host, size_t n, bool upload, dou
   }
   G5_HIP(p->err, hipGetLastError());
   G5_HIP(p->err, hipStreamSynchronize(p->stream));
-  return G5_OK;
+  return upload ? clear_failure(p, id) : G5_OK;
 }
 
 // host-side description of the shock-bubble / triple-point initial conditions
@@ -972,7 +1001,7 @@ pool_init_ic(g5_pool * p, int32_t id, IcDesc const & ic)
   G5_HIP(p->err, hipGetLastError());
   G5_HIP(p->err, hipStreamSynchronize(p->stream));
   hipFree(d_ic);
-  return G5_OK;
+  return clear_failure(p, id);
 }
 
 int
@@ -1212,6 +1241,22 @@ g5_pool_status(g5_pool * p, int32_t id, double * t, int32_t * steps, int32_t * a
     *steps = p->h_ctl[id].steps;
   if (active)
     *active = p->h_ctl[id].active;
+  if (p->h_ctl[id].failure != 0u)
+    return fail(p->err, G5_ERR_DIVERGED, failure_message(id, p->h_ctl[id]));
+  return G5_OK;
+}
+
+int
+g5_pool_failure(g5_pool * p, int32_t id, int32_t * failure)
+{
+  if (not p or not failure)
+    return G5_ERR_INVALID;
+  if (id < 0 or id >= static_cast<int32_t>(p->solves.size()) or !p->solves[id].live)
+    return fail(p->err, G5_ERR_INVALID, "no such solve");
+  const int rc = read_ctl(p);
+  if (rc != G5_OK)
+    return rc;
+  *failure = static_cast<int32_t>(p->h_ctl[id].failure);
   return G5_OK;
 }
 
@@ -1540,6 +1585,12 @@ g5_compute_dt(g5_ctx * ctx, double * dt)
   std::memcpy(&inv, &p->h_ctl[0].invdt_bits, sizeof(double));
   *dt = ctx->cfg.cfl / inv;
   launch_reset_invdt(p->d_ctl, 1, p->stream);
+  if (p->h_ctl[0].failure != 0u or not(inv > 0.0 and std::isfinite(inv)))
+  {
+    SolveCtl c = p->h_ctl[0];
+    c.failure |= c.failure == 0u ? static_cast<uint32_t>(G5_FAIL_NONFINITE) : 0u;
+    return fail(ctx->err, G5_ERR_DIVERGED, failure_message(ctx->id, c));
+  }
   return G5_OK;
 }
 
@@ -1663,6 +1714,8 @@ g5_run(g5_ctx * ctx, int32_t nsteps, double * t, double tEnd, int32_t * steps_do
   *t = p->h_ctl[ctx->id].t;
   if (steps_done)
     *steps_done = p->h_ctl[ctx->id].steps;
+  if (p->h_ctl[ctx->id].failure != 0u)
+    return fail(ctx->err, G5_ERR_DIVERGED, failure_message(ctx->id, p->h_ctl[ctx->id]));
   return G5_OK;
 }
 

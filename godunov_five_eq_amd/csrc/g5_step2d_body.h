@@ -382,7 +382,8 @@ step2d_block(StepArgs const & A, int blk)
       fl_store2(FY, FYE, fy2(rr, 16), Fr);
   }
   __syncthreads();
-  double rate = 0.0;
+  double   rate = 0.0;
+  uint32_t fail = 0u;
   if (owner)
   {
     Cons   u2 = uown;
@@ -394,8 +395,11 @@ step2d_block(StepArgs const & A, int blk)
     apply_face_pair<2>(u2, Fy, fl_load2(FY, FYE, fy2(i, j + 1)), uown.a0, uown.a1, dtdx, 1, st0, st1);
     fix_volume_fractions<FAST>(u2);
     store_cons<2, B2>(U2, own, u2);
-    rate = cell_rate<2, T, FAST>(S, u2);
+    double pc;
+    rate = cell_rate<2, T, FAST>(S, u2, pc);
+    fail = cell_failure(S, u2, rate, pc);
   }
+  report_failure(C, fail);
 
   // ---- 5. CFL of the new state (the ring wavefront has no cell) ----
   double inv = wave_max(rate);

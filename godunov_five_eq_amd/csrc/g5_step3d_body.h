@@ -397,6 +397,7 @@ step3d_body(StepArgs const & A, int blk)
   FaceT<T> zL;           // +z face state of plane P-1
   FaceT<T> zLnext;       // +z face state of plane P (wave 1 with ZSPLIT, until its z solve)
   double   inv_max = 0.0;
+  uint32_t fail = 0u; // G5_FAIL_* bits of the lane's new cells
 
   // column offsets at plane 0 of the own column and of the extra conversion item: within the
   // block's z range (dz = 0) no border transform acts on z, so plane k is at offset + 256 k
@@ -433,7 +434,10 @@ step3d_body(StepArgs const & A, int blk)
       apply_face_pair<3>(u2prev, zFprev, zF, uaprev0, uaprev1, dtdx, 2, stprev0, stprev1);
       fix_volume_fractions<FAST>(u2prev);
       store_cons<3, B3>(U2, blk_off + i + 16 * j + 256 * (P - 1), u2prev);
-      inv_max = fmax(inv_max, cell_rate<3, T, FAST>(S, u2prev));
+      double       pc;
+      const double rate = cell_rate<3, T, FAST>(S, u2prev, pc);
+      inv_max = fmax(inv_max, rate);
+      fail |= cell_failure(S, u2prev, rate, pc);
     }
     zFprev = zF;
   };
@@ -560,6 +564,7 @@ step3d_body(StepArgs const & A, int blk)
     uraw_cur = uraw_next;
   }
 
+  report_failure(C, fail);
   const double inv = wave_max(inv_max);
   if (lane == 0)
     sRed[wave] = inv;

@@ -23,7 +23,7 @@ struct SolveDesc
   int32_t integrator;  // G5_TI_*
   int32_t inlet;       // G5_INLET_*
   int32_t source_term; // G5_SOURCE_*
-  int32_t pad_st;
+  int32_t abort_neg_p; // 1: a negative pressure in a new state fails the solve
   double  dx;
   double  cfl;
   double  slope_type;
@@ -62,6 +62,8 @@ struct SolveCtl
   int32_t            steps;      // steps done
   int32_t            max_steps;  // step budget
   int32_t            parity;     // which of the two state buffers holds the current state
+  uint32_t           failure;    // G5_FAIL_* bits, set by the kernels that check a new state
+  int32_t            pad;
 };
 
 // block of the launch list: solve id, memory slot, brick coordinates
@@ -115,7 +117,7 @@ make_solve_desc(g5_config const & c, int32_t base, SolveDesc & s)
   s.integrator = c.integrator;
   s.inlet = c.inlet;
   s.source_term = c.source_term;
-  s.pad_st = 0;
+  s.abort_neg_p = c.abort_negative_pressure != 0 ? 1 : 0;
   for (int k = 0; k < 9; ++k)
     s.inlet_u[k] = c.inlet_u[k];
   for (int d = 0; d < 3; ++d)

@@ -36,7 +36,20 @@ enum
   G5_ERR_UNSUPPORTED = 3, /* valid configuration this build does not implement */
   G5_ERR_NOMEM = 4,       /* allocation failed or HBM budget exceeded */
   G5_ERR_STATE = 5,       /* call out of order (e.g. step before upload) */
-  G5_ERR_IO = 6           /* file input/output failed (godunov_five_eq_io.h) */
+  G5_ERR_IO = 6,          /* file input/output failed (godunov_five_eq_io.h) */
+  G5_ERR_DIVERGED = 7,    /* a solve failed (G5_FAIL_* bits): it was stopped at the failing state */
+  G5_ERR_TOO_LARGE = 8    /* the solve needs more blocks than the whole pool holds (never fits) */
+};
+
+/* why a solve was stopped (bits of g5_pool_failure).  The step kernels and the dt reduction check
+ * every owned cell of each new state: a CFL rate that is not finite (NaN or infinite state, zero
+ * density) or an infinite dt sets G5_FAIL_NONFINITE; with g5_config.abort_negative_pressure a
+ * negative mixture pressure sets G5_FAIL_NEGATIVE_PRESSURE ([hydro] abort_when_negative_pressure,
+ * models/HydroSettings.cpp:28).  The solve then stops stepping; its co-batched solves go on. */
+enum
+{
+  G5_FAIL_NONFINITE = 1,
+  G5_FAIL_NEGATIVE_PRESSURE = 2
 };
 
 /* border conditions, border_conditions/FillOutside.h:51-55 (BC_HYDRO) */
@@ -124,7 +137,9 @@ typedef struct g5_config
   /* volume-fraction source term of the update ([volume_fraction_advection] source_term_type,
    * scheme/ReadFluxesAndConservativeUpdateFunctor.h:412-443): G5_SOURCE_* */
   int32_t source_term;
-  int32_t source_pad;
+  /* 1: a negative pressure in a new state stops the solve with G5_FAIL_NEGATIVE_PRESSURE
+   * ([hydro] abort_when_negative_pressure, models/HydroSettings.cpp:28, default true there) */
+  int32_t abort_negative_pressure;
 } g5_config;
 
 typedef struct g5_ctx g5_ctx;
@@ -255,7 +270,8 @@ int g5_pool_create_rk2(int32_t dim, int32_t thinc_enabled, int32_t precision, ui
 int g5_pool_destroy(g5_pool * pool);
 const char * g5_pool_last_error(const g5_pool * pool);
 
-/* reserve memory for one solve (G5_ERR_NOMEM when the budget has no room); it starts inactive.
+/* reserve memory for one solve (G5_ERR_NOMEM when the budget has no room now, G5_ERR_TOO_LARGE
+ * when the solve has more blocks than the whole budget holds); it starts inactive.
  * G5_ERR_INVALID when dim, THINC, precision or integrator differ from the pool's live solves;
  * G5_ERR_UNSUPPORTED for RK2_SSP in a pool without the stage buffer (g5_pool_create_rk2). */
 int g5_pool_admit(g5_pool * pool, const g5_config * cfg, double t_end, int32_t max_steps, int32_t * solve_id);
@@ -280,8 +296,13 @@ int g5_pool_start(g5_pool * pool, const int32_t * solve_ids, int32_t n);
  * fused step kernel + finalise per step) */
 int g5_pool_step(g5_pool * pool, int32_t nsteps);
 
-/* time, steps done and activity of one solve (synchronises) */
+/* time, steps done and activity of one solve (synchronises).  G5_ERR_DIVERGED (outputs still
+ * filled) when the solve was stopped by a failure (g5_pool_failure). */
 int g5_pool_status(g5_pool * pool, int32_t solve_id, double * t, int32_t * steps, int32_t * active);
+
+/* G5_FAIL_* bits of one solve (0 while it is healthy; synchronises).  No reference counterpart:
+ * there a failing run aborts the process (Kokkos::abort). */
+int g5_pool_failure(g5_pool * pool, int32_t solve_id, int32_t * failure);
 
 /* Resume support (checkpoint/restart and eviction of a solve to host memory): set the time and
  * the number of steps already done of an admitted solve before g5_pool_start.  A solve saved as
@@ -292,7 +313,8 @@ int g5_pool_status(g5_pool * pool, int32_t solve_id, double * t, int32_t * steps
  * one solve; the on-disk format is the caller's (the Python layer writes .npy + JSON). */
 int g5_pool_set_clock(g5_pool * pool, int32_t solve_id, double t, int32_t steps_done);
 
-/* live solves that reached t_end or their step budget: up to cap ids, *n_done = their number */
+/* live solves that reached t_end or their step budget, or were stopped by a failure (see
+ * g5_pool_status): up to cap ids, *n_done = their number */
 int g5_pool_finished(g5_pool * pool, int32_t * solve_ids, int32_t cap, int32_t * n_done);
 
 /* blocks in use, block capacity of the budget, live solves */
@@ -424,10 +446,43 @@ typedef struct g5_serve_report
   /* host wall time in the loop's phases: admission (incl. device initial conditions), first dt,
    * waiting for the enqueued steps, collection of finished solves (download and release) */
   double  admit_s, start_s, wait_s, collect_s;
+  int32_t failed;   /* requests whose solve was stopped by a failure (G5_FAIL_*) */
+  int32_t rejected; /* requests larger than their pool's whole budget (never admissible) */
 } g5_serve_report;
 
 /* latencies_ms (optional, cap entries) receives the latency of each request in arrival order */
 int g5_serve(const g5_serve_config * config, g5_serve_report * report, double * latencies_ms, size_t cap);
+
+/* outcome of one served request */
+enum
+{
+  G5_SERVE_DONE = 0,     /* reached its step budget */
+  G5_SERVE_FAILED = 1,   /* stopped by a failure (failure holds the G5_FAIL_* bits) */
+  G5_SERVE_REJECTED = 2  /* more blocks than its pool's whole budget: never admitted */
+};
+
+typedef struct g5_serve_result
+{
+  int32_t outcome; /* G5_SERVE_* */
+  int32_t failure; /* G5_FAIL_* bits of a failed solve */
+  int32_t steps;   /* steps done */
+  int32_t pad;
+  double  t;          /* final time of the solve */
+  double  latency_ms; /* result on the host (or rejection) - arrival */
+} g5_serve_result;
+
+/* The serving loop of g5_serve over given requests, with per-request results.
+ * specs: config->n_requests requests in order of non-decreasing arrival (NULL: drawn by
+ * g5_serve_specs from config->seed, rate_hz and frac_3d); results (optional): one entry per
+ * request; states (optional): per request a host buffer of g5_spec_state_size doubles (or NULL)
+ * that receives the request's final state in the g5_upload_state layout -- so that each served
+ * result can be checked against the same solve run alone.  Each request is admitted with its step
+ * budget (spec->steps) and no end time, and computes the same bits as when run alone. */
+int g5_serve_run(const g5_serve_config * config, const g5_solve_spec * specs, g5_serve_report * report,
+                 g5_serve_result * results, double * const * states);
+
+/* doubles of the state of a spec's solve (blocks * nvar * cells per block) */
+int g5_spec_state_size(const g5_solve_spec * spec, size_t * n);
 
 #ifdef __cplusplus
 }

@@ -179,6 +179,7 @@ struct HydroSettings
   double      cfl, slope_type, smallr, smallc, smallp, smallpp;
   int         niter_riemann;
   std::string riemann;
+  bool        abort_when_negative_eint, abort_when_negative_pressure;
   explicit HydroSettings(ConfigMap const & c)
     : cfl(c.getReal("hydro", "cfl", 0.5))
     , slope_type(c.getReal("hydro", "slope_type", 1.0))
@@ -188,6 +189,8 @@ struct HydroSettings
     , smallpp(smallr * smallp)
     , niter_riemann(c.getInteger("hydro", "niter_riemann", 10))
     , riemann(c.getString("hydro", "riemann", "UNKNOWN"))
+    , abort_when_negative_eint(c.getBool("hydro", "abort_when_negative_eint", false))
+    , abort_when_negative_pressure(c.getBool("hydro", "abort_when_negative_pressure", true))
   {}
 };
 
@@ -216,6 +219,13 @@ config_from_ini(ConfigMap const & c)
   g5_config cfg{};
   cfg.dim = c.getInteger("run", "dimension", 2);
   cfg.bsize = c.getInteger("amr", "bx", 16);
+  // cubic blocks only: by and bz, when given, must equal bx
+  if (c.getInteger("amr", "by", cfg.bsize) != cfg.bsize or (cfg.dim == 3 and c.getInteger("amr", "bz", cfg.bsize) != cfg.bsize))
+    throw std::runtime_error("[amr] by and bz must equal bx (cubic blocks)");
+  // the gravity source (Hancock predictor term, ComputeFluxesAndStoreFunctor.cpp:197-202, and the
+  // gravity CFL, ComputeDtFiveEqFunctor.cpp:138-201) is not on this path
+  if (c.getBool("gravity", "enabled", false))
+    throw std::runtime_error("[gravity] enabled=true is not available on this path");
   const int lmin = c.getInteger("amr", "level_min", 0);
   const int lmax = c.getInteger("amr", "level_max", lmin);
   if (lmin != lmax)
@@ -242,6 +252,7 @@ config_from_ini(ConfigMap const & c)
   cfg.slope_type = hs.slope_type;
   cfg.smallr = hs.smallr;
   cfg.smallc = hs.smallc;
+  cfg.abort_negative_pressure = hs.abort_when_negative_pressure ? 1 : 0;
   const std::string ti = c.getString("amr", "time_integrator", "HANCOCK");
   if (ti != "HANCOCK" and ti != "RK1" and ti != "RK2_SSP")
     throw std::runtime_error("unknown time_integrator " + ti + " (HANCOCK, RK1, RK2_SSP)");
@@ -415,10 +426,12 @@ public:
   }
 
   // U -> U2 is kept inside the context (U2 becomes the current state, as after the reference's
-  // deep_copy U <- U2 in godunov_unsplit)
+  // deep_copy U <- U2 in godunov_unsplit): the handles must be empty or this context's own
   void
-  do_time_step(DataArrayBlock const &, DataArrayBlock const &, double dt)
+  do_time_step(DataArrayBlock const & U, DataArrayBlock const & U2, double dt)
   {
+    if ((U.ctx and U.ctx != m_ctx.get()) or (U2.ctx and U2.ctx != m_ctx.get()))
+      throw std::invalid_argument("do_time_step: state handle of another context (the state stays in its context)");
     check(g5_do_time_step(m_ctx.get(), dt), m_ctx.get());
   }
 

@@ -96,7 +96,7 @@ main(int argc, char ** argv)
               cfg.smallr, cfg.smallc, hs.smallp);
   std::printf("gamma=%.17g,%.17g pinf=%.17g,%.17g thinc=%d beta=%g eps=%g integrator=%d\n", cfg.gamma[0], cfg.gamma[1],
               cfg.pinf[0], cfg.pinf[1], cfg.thinc_enabled, cfg.thinc_beta, cfg.thinc_epsilon, cfg.integrator);
-  std::printf("source_term=%d\n", cfg.source_term);
+  std::printf("source_term=%d abort_negative_pressure=%d\n", cfg.source_term, cfg.abort_negative_pressure);
   std::printf("inlet=%d jet=%.17g,%.17g,%.17g inlet_u=", cfg.inlet, cfg.jet[0], cfg.jet[1], cfg.jet[2]);
   for (int v = 0; v < (cfg.dim == 2 ? 8 : 9); ++v)
     std::printf(v ? ",%.17g" : "%.17g", cfg.inlet_u[v]);

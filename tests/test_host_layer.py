@@ -105,6 +105,39 @@ def test_ini_unavailable_eos_is_refused(exe, tmp_path):
     assert out.returncode != 0 and "RK3" in out.stderr
 
 
+def test_ini_gravity_and_noncubic_blocks_are_refused(exe, tmp_path):
+    # [gravity] enabled adds a Hancock gravity term and a gravity CFL in the reference
+    # (ComputeFluxesAndStoreFunctor.cpp:197-202, ComputeDtFiveEqFunctor.cpp:138-201): refused, not
+    # run without gravity; by/bz other than bx would change the mesh and dx: refused
+    base = open(os.path.join(GOLD, "shock_bubble_air_helium_2d_512x128.ini")).read()
+    ini = tmp_path / "gravity.ini"
+    ini.write_text(base + "\n[gravity]\nenabled=true\n")
+    out = subprocess.run([exe, str(ini)], capture_output=True, text=True)
+    assert out.returncode != 0 and "gravity" in out.stderr
+    ini.write_text(base + "\n[gravity]\nenabled=false\n")
+    assert subprocess.run([exe, str(ini)], capture_output=True, text=True).returncode == 0
+    ini = tmp_path / "by.ini"
+    assert "\nby=16\n" in base
+    ini.write_text(base.replace("\nby=16\n", "\nby=8\n"))
+    out = subprocess.run([exe, str(ini)], capture_output=True, text=True)
+    assert out.returncode != 0 and "by and bz" in out.stderr
+    ini.write_text(base.replace("\nby=16\n", "\n"))  # by defaults to bx
+    assert subprocess.run([exe, str(ini)], capture_output=True, text=True).returncode == 0
+
+
+def test_ini_abort_when_negative_pressure(exe, tmp_path):
+    # [hydro] abort_when_negative_pressure, default true (models/HydroSettings.cpp:28)
+    base = open(os.path.join(GOLD, "shock_bubble_air_helium_2d_512x128.ini")).read()
+    assert "abort_when_negative_pressure" not in base
+    kv = parse(subprocess.run([exe, os.path.join(GOLD, "shock_bubble_air_helium_2d_512x128.ini")], check=True,
+                              capture_output=True, text=True).stdout)
+    assert kv["abort_negative_pressure"] == "1"
+    ini = tmp_path / "noabort.ini"
+    ini.write_text(base.replace("[hydro]\n", "[hydro]\nabort_when_negative_pressure=false\n"))
+    kv = parse(subprocess.run([exe, str(ini)], check=True, capture_output=True, text=True).stdout)
+    assert kv["abort_negative_pressure"] == "0"
+
+
 def _init_state(exe, ini, tmp_path):
     raw = tmp_path / "u.bin"
     out = subprocess.run([exe, os.path.join(GOLD, ini), "--init", str(raw)], check=True, capture_output=True,

@@ -355,8 +355,10 @@ def test_pool_error_paths(g5):
         assert e.value.code == g5.G5_ERR_INVALID and "integrator" in str(e.value)
         pool.release(first)
         pool.release(pool.admit(rk1))  # accepted once the pool is empty
-        big, _ = cases.air_helium_2d(nbx=16, nby=16)  # 256 blocks > 128 slots
-        assert pool.try_admit(big) is None
+        big, _ = cases.air_helium_2d(nbx=16, nby=16)  # 256 blocks > 128 slots: can never fit
+        with pytest.raises(g5.G5Error) as e:
+            pool.try_admit(big)
+        assert e.value.code == g5.G5_ERR_TOO_LARGE
         sid = pool.admit(cfg2, max_steps=3)  # the pool still works after the refusals
         pool.upload(sid, U0)
         pool.start([sid])

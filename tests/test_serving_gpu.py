@@ -5,6 +5,13 @@
 - Continuous batching: mixed-precision solves admitted while others are in flight, released, and
   their memory reused give exactly the bits they give alone.
 - The native serving loop completes every request and reports consistent latencies.
+- Served results are checkable (g5_serve_run returns each request's final state): in FP64 each
+  served result equals the same request run alone and the oracle run from the same initial state
+  bit for bit; in mixed precision it equals its alone run bit for bit and stays within density
+  relative L1 1e-5 of the oracle.  A request larger than its pool's whole budget is rejected and
+  a failing request is counted as failed, while the loop serves the rest.
+- The serving configuration itself (Poisson stream, 2-D 64^2-1024^2 plus 10% 3-D 64^3-128^3, 50-500
+  steps, 200 GB budget) completes every request.
 """
 import numpy as np
 import pytest
@@ -97,12 +104,130 @@ def test_continuous_batching_alone_equals_mix(g5):
 def test_serve_loop_completes(g5):
     n = 24
     say(f"serving loop: {n} requests")
-    rep, lat = g5.serve(SEED, n, 2000.0, budget_bytes=8 << 30, steps_per_poll=4)
+    specs = g5.serve_specs(SEED, n, 2000.0, 0.1)
+    rep, res, _ = g5.serve_run(specs, budget_bytes=8 << 30, steps_per_poll=4, keep_states=False)
     say(f"serving report {rep}")
-    assert rep["completed"] == n
+    # requests 2, 13 and 19 are triple points in a stiffened gas whose state leaves the EOS domain
+    # (a cell's c^2 turns negative after 271, 148 and 135 steps): they are reported failed
+    failed = [k for k, r in enumerate(res) if r["outcome"] == g5.SERVE_FAILED]
+    assert failed == [2, 13, 19] and all(res[k]["failure"] == g5.FAIL_NONFINITE for k in failed)
+    assert [res[k]["steps"] for k in failed] == [271, 148, 135]
+    assert rep["completed"] == n - 3 and rep["failed"] == 3
+    lat = np.array([r["latency_ms"] for r in res])
     assert np.all(lat > 0)
     assert rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
     assert rep["cell_updates"] > 0 and rep["cell_updates_per_s"] > 0
-    specs = g5.serve_specs(SEED, n, 2000.0, 0.1)
-    cu = sum(s.steps * 16 ** s.dim * int(np.prod(list(s.nb)[:s.dim])) for s in specs)
+    cu = sum(s.steps * 16 ** s.dim * int(np.prod(list(s.nb)[:s.dim])) for k, s in enumerate(specs) if k not in failed)
     assert rep["cell_updates"] == cu
+
+
+def small_serve_specs(g5, n, rate_hz=3000.0):
+    """n requests of the serving workload (Poisson arrivals at rate_hz, a quarter 3-D), downsized
+    so that the oracle checks them in seconds: 2-D 4-8 x 4-7 blocks and 20-40 steps, 3-D 4^3 blocks
+    and 8-12 steps."""
+    specs = g5.serve_specs(SEED, n, rate_hz, 0.25)
+    for k, s in enumerate(specs):
+        if s.dim == 2:
+            s.nb[0], s.nb[1] = 4 + k % 5, 4 + (k // 5) % 4
+            s.steps = 20 + k % 21
+        else:
+            s.nb[0] = s.nb[1] = s.nb[2] = 4
+            s.steps = 8 + k % 5
+    return specs
+
+
+def alone_with_initial(g5, spec, prec):
+    """(initial state, final state, t, steps) of the request run alone in a pool of one"""
+    with g5.Pool(spec.dim, precision=prec, budget_bytes=1 << 28) as pool:
+        sid = pool.admit_spec(spec)
+        U0 = pool.download(sid)
+        pool.start([sid])
+        pool.step(spec.steps + 3)  # the step budget stops it at spec.steps
+        t, n, _ = pool.status(sid)
+        return U0, pool.download(sid), t, n
+
+
+@pytest.mark.parametrize("prec_name", ["fp64", "mixed"])
+def test_served_results_equal_alone_and_oracle(g5, orc, prec_name):
+    prec = g5.PREC_FP64 if prec_name == "fp64" else g5.PREC_MIXED
+    specs = small_serve_specs(g5, 32)
+    n3 = sum(s.dim == 3 for s in specs)
+    # 128 MB: the 3-D pool (half) holds one 64^3 solve at a time, so 3-D requests queue for memory
+    say(f"serve_run {prec_name}: 32 requests ({n3} 3-D), 128 MB budget")
+    rep, res, states = g5.serve_run(specs, budget_bytes=128 << 20, share_3d=0.5, precision=prec, steps_per_poll=4)
+    say(f"report completed={rep['completed']} max_inflight={rep['max_inflight']} p50={rep['p50_ms']:.1f} ms")
+    assert rep["completed"] == 32 and rep["failed"] == 0 and rep["rejected"] == 0
+    assert rep["max_inflight"] > 1 and n3 >= 3
+    worst = 0.0
+    for k, s in enumerate(specs):
+        U0, Ua, ta, na = alone_with_initial(g5, s, prec)
+        r = res[k]
+        assert r["outcome"] == g5.SERVE_DONE and r["steps"] == s.steps == na and r["t"] == ta, k
+        assert np.array_equal(states[k], Ua), k
+        o = orc.Oracle(g5.spec_config(s))
+        o.set_state(U0)
+        t_o, _ = o.run(s.steps)
+        if prec == g5.PREC_FP64:
+            assert t_o == ta and np.array_equal(states[k], o.get_state()), k
+        else:
+            nv = 8 if s.dim == 2 else 9
+            A = states[k].reshape(-1, nv, 16 ** s.dim)[:, 0]
+            B = o.get_state().reshape(-1, nv, 16 ** s.dim)[:, 0]
+            l1 = np.abs(A - B).sum() / np.abs(B).sum()
+            worst = max(worst, l1)
+            assert l1 <= 1e-5, (k, l1)
+    say(f"served results equal alone runs; worst mixed density L1 vs oracle {worst:.1e}")
+
+
+def test_serve_rejects_too_large_and_counts_failed(g5):
+    specs = small_serve_specs(g5, 12)
+    big = next(k for k, s in enumerate(specs) if s.dim == 3)
+    specs[big].nb[0] = specs[big].nb[1] = specs[big].nb[2] = 8  # 128^3: 302 MB, more than the 3-D pool
+    bad = next(k for k, s in enumerate(specs) if s.dim == 2 and k != big)
+    specs[bad].rho0 = float("nan")  # its inclusion is NaN: the solve fails at its first dt
+    say(f"serve_run with request {big} too large for its pool and request {bad} failing")
+    rep, res, states = g5.serve_run(specs, budget_bytes=128 << 20, share_3d=0.5, precision=g5.PREC_FP64)
+    assert rep["rejected"] == 1 and res[big]["outcome"] == g5.SERVE_REJECTED
+    assert rep["failed"] == 1 and res[bad]["outcome"] == g5.SERVE_FAILED
+    assert res[bad]["failure"] == g5.FAIL_NONFINITE and res[bad]["steps"] == 0
+    assert rep["completed"] == 10
+    for k, s in enumerate(specs):
+        if k not in (big, bad):
+            assert res[k]["outcome"] == g5.SERVE_DONE and res[k]["steps"] == s.steps
+
+
+def test_serve_configuration_200gb(g5):
+    # the serving configuration at its real sizes: Poisson arrivals at 160/s, 2-D 64^2-1024^2 and
+    # 10% 3-D 64^3-128^3, 50-500 steps, 200 GB budget split 70/30, mixed precision
+    n = 160
+    say(f"serving configuration: {n} requests at 160/s, 200 GB")
+    rep, lat = g5.serve(SEED, n, 160.0, budget_bytes=200 << 30, steps_per_poll=8)
+    say(f"report completed={rep['completed']} p50={rep['p50_ms']:.1f} p99={rep['p99_ms']:.1f} "
+        f"G cell-updates/s={rep['cell_updates_per_s'] / 1e9:.2f} max_inflight={rep['max_inflight']}")
+    # every request is served: completed, or reported failed when its state leaves the EOS domain
+    assert rep["completed"] + rep["failed"] == n and rep["rejected"] == 0 and rep["failed"] <= n // 5
+    assert np.all(lat > 0) and rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
+    specs = g5.serve_specs(SEED, n, 160.0, 0.1)
+    assert any(s.dim == 3 for s in specs) and max(s.nb[0] for s in specs if s.dim == 2) >= 48
+    cu = sum(s.steps * 16 ** s.dim * int(np.prod(list(s.nb)[:s.dim])) for s in specs)
+    assert 0 < rep["cell_updates"] <= cu
+
+
+def test_cavitating_request_fails_at_the_same_step_in_every_precision(g5):
+    # request 13 of the seeded stream: a Mach 2.83 triple point in a stiffened gas (gamma 2.50,
+    # pinf 6e8) whose expansion drives a cell's c^2 negative at step 148 -- the same step in
+    # FP64, reassociated FP64 and mixed precision; the state itself stays finite
+    spec = g5.serve_specs(SEED, 24, 2000.0, 0.1)[13]
+    assert spec.kind == 1 and spec.pinf[1] == 6e8
+    for prec in (g5.PREC_FP64, g5.PREC_FP64_FAST, g5.PREC_MIXED):
+        with g5.Pool(2, precision=prec, budget_bytes=1 << 30) as pool:
+            sid = pool.admit_spec(spec)
+            pool.start([sid])
+            pool.step(spec.steps)
+            assert pool.failure(sid) == g5.FAIL_NONFINITE
+            U = pool.download(sid)
+            assert np.all(np.isfinite(U))
+            import ctypes
+            t, n, a = ctypes.c_double(), ctypes.c_int32(), ctypes.c_int32()
+            rc = g5.lib().g5_pool_status(pool._h, sid, ctypes.byref(t), ctypes.byref(n), ctypes.byref(a))
+            assert rc == g5.G5_ERR_DIVERGED and n.value == 148 and a.value == 0, (prec, n.value)
