@@ -173,6 +173,9 @@ def lib():
             "g5_upload_state": (ctypes.c_int, [vp, P(d), sz]),
             "g5_download_state": (ctypes.c_int, [vp, P(d), sz]),
             "g5_state_device_ptr": (ctypes.c_int, [vp, P(vp)]),
+            "g5_set_block_order": (ctypes.c_int, [vp, P(i32), i32]),
+            "g5_do_time_step_buffers": (ctypes.c_int, [vp, vp, vp, d]),
+            "g5_compute_dt_buffer": (ctypes.c_int, [vp, vp, P(d)]),
             "g5_compute_dt": (ctypes.c_int, [vp, P(d)]),
             "g5_fill_outside": (ctypes.c_int, [vp, P(d), sz]),
             "g5_outside_coords": (ctypes.c_int, [vp, P(i32), P(i32), sz]),
@@ -242,7 +245,8 @@ def exported_symbols():
         "g5_pool_start", "g5_pool_step", "g5_pool_status", "g5_pool_finished", "g5_pool_usage",
         "g5_pool_last_step_ms", "g5_pool_set_launch", "g5_pool_get_launch", "g5_pool_set_autotune",
         "g5_pool_properties", "g5_pool_prepare", "g5_host_alloc", "g5_host_free", "g5_pool_set_clock", "g5_pool_step_async", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
-        "g5_pool_admit_spec", "g5_serve",
+        "g5_pool_admit_spec", "g5_serve", "g5_serve_run", "g5_spec_state_size", "g5_pool_failure",
+        "g5_set_block_order", "g5_do_time_step_buffers", "g5_compute_dt_buffer",
     ]
 
 
@@ -412,6 +416,25 @@ class Context:
 
     def do_time_step(self, dt):
         self._check(lib().g5_do_time_step(self._h, dt))
+
+    def set_block_order(self, coords):
+        """coords: (n_blocks, 3) brick coordinates of the caller's blocks, in the caller's order"""
+        c = np.ascontiguousarray(coords, dtype=np.int32).reshape(-1, 3)
+        self._check(lib().g5_set_block_order(self._h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), c.shape[0]))
+
+    def device_ptr(self):
+        p = ctypes.c_void_p()
+        self._check(lib().g5_state_device_ptr(self._h, ctypes.byref(p)))
+        return p.value
+
+    def do_time_step_buffers(self, U_dev, U2_dev, dt):
+        """one step from the caller's device array U_dev into U2_dev (integer device addresses)"""
+        self._check(lib().g5_do_time_step_buffers(self._h, ctypes.c_void_p(U_dev), ctypes.c_void_p(U2_dev), dt))
+
+    def compute_dt_buffer(self, U_dev):
+        dt = ctypes.c_double()
+        self._check(lib().g5_compute_dt_buffer(self._h, ctypes.c_void_p(U_dev), ctypes.byref(dt)))
+        return dt.value
 
     def run(self, nsteps, t=0.0, t_end=1e300):
         tt = ctypes.c_double(t)

@@ -158,6 +158,7 @@ struct g5_ctx
   int32_t *            d_outside = nullptr;
   double *             d_ring = nullptr;
   double *             d_staging = nullptr; // contiguous [block][var][cell] buffer for transfers
+  std::vector<int32_t> block_order;         // caller's block order (brick coordinates), empty: lexicographic
   std::string          err;
 };
 
@@ -500,9 +501,9 @@ launch_step(g5_pool * p, StepArgs const & A, bool hancock)
 }
 
 // ---------------------------------------------------------------------------------------------
-// Autotuner of the launch geometry.  Candidates: one workgroup per block or persistent
-// workgroups (1, 2, 4 or 8 per CU, within what the kernel's occupancy allows), XCD-contiguous
-// block ranges or plain order, and for 3-D the two z-solve placements.  Each candidate runs the
+// Autotuner of the launch geometry.  Candidates: one workgroup per block with XCD-contiguous
+// block ranges or plain order, in 2-D the five-wavefront ring form, in 3-D the two z-solve
+// placements.  Each candidate runs the
 // device-dt step kernel of the live launch list a few times; a step kernel only writes the next
 // state buffer and max-accumulates the next CFL rate, both functions of the current state alone,
 // so the repeated launches leave exactly what the real step will write.  The best candidate is
@@ -535,23 +536,19 @@ size_class(int32_t nblocks)
   return c; // floor(log2(nblocks))
 }
 
+// The persistent 2-D form (workgroups looping over a block range) is not a candidate: it measured
+// 1.7-2.4x slower everywhere, and it shares the looping block_range branch with the persistent
+// 3-D form, which faulted; g5_pool_set_launch still selects it explicitly.
 std::vector<LaunchCfg>
 tune_candidates(g5_pool const * p)
 {
   std::vector<LaunchCfg> c;
-  // resident workgroups per CU: 2-D about 2 (FP64) or 5 (mixed); 3-D 1 (FP64) or 2 (mixed)
-  const int max_wg = p->dim == 2 ? (p->precision == G5_PREC_MIXED ? 8 : 2) : (p->precision == G5_PREC_MIXED ? 2 : 1);
   for (int z = (p->dim == 3 ? 0 : 1); z <= 1; ++z)
   {
     c.push_back({ 0, 1, z, 0 });
     c.push_back({ 0, 0, z, 0 });
-    if (p->dim == 2) // the 3-D step has no persistent form
-    {
-      for (int w = 1; w <= max_wg; w *= 2)
-        c.push_back({ w, 1, z, 0 });
-      // a fifth wavefront for the ring work: shorter per-block latency, fewer registers per lane
+    if (p->dim == 2) // a fifth wavefront for the ring work: shorter per-block latency, fewer registers per lane
       c.push_back({ 0, 1, z, 1 });
-    }
   }
   return c;
 }
@@ -1528,12 +1525,35 @@ g5_mem_bytes(const g5_ctx * ctx, uint64_t * bytes)
   return G5_OK;
 }
 
+// a context with the caller's block order: block k of the caller lives in slot k, so the state is
+// the current buffer itself
+static int
+ctx_copy_slots(g5_ctx * ctx, double * host, size_t n, bool upload)
+{
+  g5_pool *    p = ctx->pool;
+  const size_t need = static_cast<size_t>(p->solves[ctx->id].nblk) * p->block_doubles;
+  if (n != need)
+    return fail(p->err, G5_ERR_INVALID, "state size mismatch");
+  int rc = read_ctl(p);
+  if (rc != G5_OK)
+    return rc;
+  double * base = p->buf[p->h_ctl[ctx->id].parity];
+  if (upload)
+    G5_HIP(p->err, hipMemcpyAsync(base, host, need * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  else
+    G5_HIP(p->err, hipMemcpyAsync(host, base, need * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  return upload ? clear_failure(p, ctx->id) : G5_OK;
+}
+
 int
 g5_upload_state(g5_ctx * ctx, const double * U_host, size_t n)
 {
   if (not ctx or not U_host)
     return ctx ? fail(ctx->err, G5_ERR_INVALID, "null argument") : G5_ERR_INVALID;
-  const int rc = pool_transfer(ctx->pool, ctx->id, const_cast<double *>(U_host), n, true, &ctx->d_staging);
+  const int rc = ctx->block_order.empty()
+                   ? pool_transfer(ctx->pool, ctx->id, const_cast<double *>(U_host), n, true, &ctx->d_staging)
+                   : ctx_copy_slots(ctx, const_cast<double *>(U_host), n, true);
   if (rc != G5_OK)
     return fail(ctx->err, rc, ctx->pool->err);
   ctx->has_state = true;
@@ -1547,7 +1567,8 @@ g5_download_state(g5_ctx * ctx, double * U_host, size_t n)
     return ctx ? fail(ctx->err, G5_ERR_INVALID, "null argument") : G5_ERR_INVALID;
   if (not ctx->has_state)
     return fail(ctx->err, G5_ERR_STATE, "no state uploaded");
-  const int rc = pool_transfer(ctx->pool, ctx->id, U_host, n, false, &ctx->d_staging);
+  const int rc = ctx->block_order.empty() ? pool_transfer(ctx->pool, ctx->id, U_host, n, false, &ctx->d_staging)
+                                          : ctx_copy_slots(ctx, U_host, n, false);
   return rc == G5_OK ? G5_OK : fail(ctx->err, rc, ctx->pool->err);
 }
 
@@ -1559,7 +1580,8 @@ g5_state_device_ptr(g5_ctx * ctx, double ** U_dev)
   const int rc = read_ctl(ctx->pool);
   if (rc != G5_OK)
     return fail(ctx->err, rc, ctx->pool->err);
-  // a pool of one uses slots 0..n-1 in lexicographic order: the buffer is the brick's U array
+  // a pool of one uses slots 0..n-1, in lexicographic order or in the caller's block order
+  // (g5_set_block_order): the buffer is the brick's U array in that order
   *U_dev = ctx->pool->buf[ctx->pool->h_ctl[ctx->id].parity];
   return G5_OK;
 }
@@ -1640,6 +1662,53 @@ g5_outside_coords(const g5_ctx * ctx, int32_t * n_outside, int32_t * coords, siz
   return G5_OK;
 }
 
+int
+g5_set_block_order(g5_ctx * ctx, const int32_t * coords, int32_t n_blocks)
+{
+  if (not ctx or not coords)
+    return ctx ? fail(ctx->err, G5_ERR_INVALID, "null argument") : G5_ERR_INVALID;
+  g5_pool *       p = ctx->pool;
+  SolveRec &      r = p->solves[ctx->id];
+  const int       nx = ctx->cfg.nb[0], ny = ctx->cfg.nb[1], nz = ctx->cfg.nb[2];
+  if (n_blocks != r.nblk)
+    return fail(ctx->err, G5_ERR_INVALID, "n_blocks must equal the number of owned blocks");
+  // caller block k goes to slot k: slots[brick index of block k] = k
+  std::vector<int32_t> slots(r.nblk, -1);
+  for (int32_t k = 0; k < n_blocks; ++k)
+  {
+    const int x = coords[3 * k], y = coords[3 * k + 1], z = ctx->cfg.dim == 3 ? coords[3 * k + 2] : 0;
+    if (x < 0 or x >= nx or y < 0 or y >= ny or z < 0 or z >= nz)
+      return fail(ctx->err, G5_ERR_INVALID, "block coordinates outside the brick");
+    const int32_t lex = x + nx * (y + ny * z);
+    if (slots[lex] >= 0)
+      return fail(ctx->err, G5_ERR_INVALID, "block coordinates repeated");
+    slots[lex] = k;
+  }
+  // the state moves with its blocks: gather the current state in the old order, scatter it back
+  std::vector<double> U;
+  if (ctx->has_state)
+  {
+    U.resize(static_cast<size_t>(r.nblk) * p->block_doubles);
+    const int rc = pool_transfer(p, ctx->id, U.data(), U.size(), false, &ctx->d_staging);
+    if (rc != G5_OK)
+      return fail(ctx->err, rc, p->err);
+  }
+  // a context's slots are 0..n-1 (its budget holds exactly its blocks): a permutation of them
+  r.slots = slots;
+  G5_HIP(ctx->err, hipMemcpyAsync(p->d_slots + r.table_base, r.slots.data(), r.nblk * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, p->stream));
+  G5_HIP(ctx->err, hipStreamSynchronize(p->stream));
+  p->list_dirty = true;
+  if (ctx->has_state)
+  {
+    const int rc = pool_transfer(p, ctx->id, U.data(), U.size(), true, &ctx->d_staging);
+    if (rc != G5_OK)
+      return fail(ctx->err, rc, p->err);
+  }
+  ctx->block_order.assign(coords, coords + 3 * static_cast<size_t>(n_blocks));
+  return G5_OK;
+}
+
 // one godunov_unsplit iteration with a host dt
 static int
 host_step(g5_ctx * ctx, double dt)
@@ -1683,6 +1752,77 @@ g5_do_time_step(g5_ctx * ctx, double dt)
   if (not(dt > 0.0))
     return fail(ctx->err, G5_ERR_INVALID, "dt must be > 0");
   return host_step(ctx, dt);
+}
+
+int
+g5_do_time_step_buffers(g5_ctx * ctx, const double * U_dev, double * U2_dev, double dt)
+{
+  if (not ctx)
+    return G5_ERR_INVALID;
+  if (not U_dev or not U2_dev or U_dev == U2_dev)
+    return fail(ctx->err, G5_ERR_INVALID, "U and U2 must be two distinct device arrays");
+  if (not(dt > 0.0))
+    return fail(ctx->err, G5_ERR_INVALID, "dt must be > 0");
+  g5_pool * p = ctx->pool;
+  int       rc = rebuild_list(p);
+  if (rc != G5_OK)
+    return fail(ctx->err, rc, p->err);
+  StepArgs A = step_args(p, p->d_blocks, p->nlist, dt);
+  A.U_in = U_dev;
+  if (ctx->cfg.integrator == G5_TI_RK2_SSP)
+  {
+    // stage 1 into the stage buffer, stage 2 into U2, then U2 = 1/2 U + 1/2 U2
+    A.U_out = p->stage;
+    launch_step(p, A, false);
+    A.U_in = p->stage;
+    A.U_out = U2_dev;
+    launch_step(p, A, false);
+    launch_lincomb(0.5, U_dev, 0.5, U2_dev, static_cast<size_t>(p->nslots) * p->block_doubles, p->stream);
+  }
+  else
+  {
+    A.U_out = U2_dev;
+    launch_step(p, A, ctx->cfg.integrator == G5_TI_HANCOCK);
+  }
+  G5_HIP(ctx->err, hipGetLastError());
+  return G5_OK;
+}
+
+int
+g5_compute_dt_buffer(g5_ctx * ctx, const double * U_dev, double * dt)
+{
+  if (not ctx or not dt)
+    return ctx ? fail(ctx->err, G5_ERR_INVALID, "null argument") : G5_ERR_INVALID;
+  if (not U_dev)
+    return fail(ctx->err, G5_ERR_INVALID, "null device array");
+  g5_pool * p = ctx->pool;
+  int       rc = rebuild_list(p);
+  if (rc != G5_OK)
+    return fail(ctx->err, rc, p->err);
+  static const uint32_t zero = 0u;
+  launch_reset_invdt(p->d_ctl, 1, p->stream);
+  G5_HIP(ctx->err, hipMemcpyAsync(reinterpret_cast<char *>(p->d_ctl) + offsetof(SolveCtl, failure), &zero,
+                                  sizeof(uint32_t), hipMemcpyHostToDevice, p->stream));
+  double * Ud = const_cast<double *>(U_dev); // read only: both buffer arguments are the caller's array
+  launch_inv_dt(p->dim, p->precision, Ud, Ud, p->d_blocks, p->d_solves, p->d_ctl, p->nlist, p->stream);
+  G5_HIP(ctx->err, hipGetLastError());
+  rc = read_ctl(p);
+  if (rc != G5_OK)
+    return fail(ctx->err, rc, p->err);
+  double inv;
+  std::memcpy(&inv, &p->h_ctl[0].invdt_bits, sizeof(double));
+  *dt = ctx->cfg.cfl / inv;
+  SolveCtl c = p->h_ctl[0];
+  launch_reset_invdt(p->d_ctl, 1, p->stream);
+  G5_HIP(ctx->err, hipMemcpyAsync(reinterpret_cast<char *>(p->d_ctl) + offsetof(SolveCtl, failure), &zero,
+                                  sizeof(uint32_t), hipMemcpyHostToDevice, p->stream));
+  G5_HIP(ctx->err, hipStreamSynchronize(p->stream));
+  if (c.failure != 0u or not(inv > 0.0 and std::isfinite(inv)))
+  {
+    c.failure |= c.failure == 0u ? static_cast<uint32_t>(G5_FAIL_NONFINITE) : 0u;
+    return fail(ctx->err, G5_ERR_DIVERGED, failure_message(ctx->id, c));
+  }
+  return G5_OK;
 }
 
 int

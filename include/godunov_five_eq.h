@@ -174,8 +174,32 @@ int g5_mem_bytes(const g5_ctx * ctx, uint64_t * bytes);
 int g5_upload_state(g5_ctx * ctx, const double * U_host, size_t n);
 int g5_download_state(g5_ctx * ctx, double * U_host, size_t n);
 
-/* device pointer of the current state (valid until the next step) */
+/* device pointer of the current state (valid until the next step), in the block order of
+ * g5_set_block_order (lexicographic when it was not called) */
 int g5_state_device_ptr(g5_ctx * ctx, double ** U_dev);
+
+/* The caller's block order: coords[3k..3k+2] are the brick coordinates of the caller's block k
+ * (z ignored in 2-D; a permutation of the brick's blocks, e.g. the Morton order of the AMR
+ * manager's orchard keys).  From then on g5_upload_state, g5_download_state, g5_state_device_ptr
+ * and the *_buffers entry points use U[block][var][cell] with the blocks in this order, as the
+ * reference's DataArrayBlock (cell, var, block) holds them (scheme/GodunovImplemBase.h:93-107,
+ * scheme/GodunovImplemV0.cpp:265-275; the neighbours that StencilHelper finds by key,
+ * ConvertToPrimitivesVariablesFunctor.cpp:876-888, come from the coordinates).  The current state
+ * moves with its blocks. */
+int g5_set_block_order(g5_ctx * ctx, const int32_t * coords, int32_t n_blocks);
+
+/* Replaces GodunovImplemV0::do_time_step(U, U2, dt) (scheme/GodunovImplemV0.cpp:83-151) on the
+ * caller's device arrays: U2 <- one step of U (RK2_SSP: both stages and the combination,
+ * SolverGodunovFiveEq.cpp:496-524).  U and U2 hold the owned blocks in the context's block order;
+ * the caller keeps them (its U <- U2 copy of godunov_unsplit, SolverGodunovFiveEq.cpp:494, stays
+ * the caller's).  Asynchronous on the context's stream: the caller orders its own work on the
+ * arrays against g5_synchronize. */
+int g5_do_time_step_buffers(g5_ctx * ctx, const double * U_dev, double * U2_dev, double dt);
+
+/* compute_dt_local (SolverGodunovFiveEq.cpp:368-390) on the caller's device array U (context block
+ * order); G5_ERR_DIVERGED for a non-finite (or, with the check, negative-pressure) state.
+ * Synchronises the context's stream. */
+int g5_compute_dt_buffer(g5_ctx * ctx, const double * U_dev, double * dt);
 
 /* ---- the step ---- */
 

@@ -388,7 +388,9 @@ initial_state(ConfigMap const & c, g5_config const & cfg)
 // ---------------------------------------------------------------------------------------------
 struct DataArrayBlock
 {
-  g5_ctx * ctx = nullptr;
+  g5_ctx * ctx = nullptr;  // a context's own state ...
+  double * data = nullptr; // ... or the caller's device array U[block][var][cell], blocks in the
+                           // context's block order (GodunovImplemHip::set_block_order)
 };
 
 // scheme/GodunovImplemBase.h:59-198 surface, implemented by the HIP kernels
@@ -425,14 +427,43 @@ public:
     return b;
   }
 
-  // U -> U2 is kept inside the context (U2 becomes the current state, as after the reference's
-  // deep_copy U <- U2 in godunov_unsplit): the handles must be empty or this context's own
+  // With device arrays (U.data, U2.data): U2 <- one step of U on the caller's arrays, which the
+  // caller owns (its U <- U2 copy stays its own, SolverGodunovFiveEq.cpp:494).  With context
+  // handles (or none): U -> U2 is kept inside the context (U2 becomes the current state, as after
+  // the reference's deep_copy U <- U2 in godunov_unsplit); a handle must then be this context's.
   void
   do_time_step(DataArrayBlock const & U, DataArrayBlock const & U2, double dt)
   {
+    if (U.data or U2.data)
+    {
+      if (not U.data or not U2.data)
+        throw std::invalid_argument("do_time_step: U and U2 must both be device arrays");
+      check(g5_do_time_step_buffers(m_ctx.get(), U.data, U2.data, dt), m_ctx.get());
+      return;
+    }
     if ((U.ctx and U.ctx != m_ctx.get()) or (U2.ctx and U2.ctx != m_ctx.get()))
       throw std::invalid_argument("do_time_step: state handle of another context (the state stays in its context)");
     check(g5_do_time_step(m_ctx.get(), dt), m_ctx.get());
+  }
+
+  // the caller's block order (brick coordinates of its blocks, 3 per block): the order of its
+  // DataArrayBlock device arrays and of the context's own state from then on
+  void
+  set_block_order(std::vector<int32_t> const & coords)
+  {
+    check(g5_set_block_order(m_ctx.get(), coords.data(), static_cast<int32_t>(coords.size() / 3)), m_ctx.get());
+  }
+
+  // compute_dt_local on the caller's device array (SolverGodunovFiveEq.cpp:368-390)
+  double
+  compute_dt_local(DataArrayBlock const & U)
+  {
+    double dt = 0.0;
+    if (U.data)
+      check(g5_compute_dt_buffer(m_ctx.get(), U.data, &dt), m_ctx.get());
+    else
+      check(g5_compute_dt(m_ctx.get(), &dt), m_ctx.get());
+    return dt;
   }
 
   void
