@@ -176,6 +176,7 @@ def lib():
             "g5_set_block_order": (ctypes.c_int, [vp, P(i32), i32]),
             "g5_do_time_step_buffers": (ctypes.c_int, [vp, vp, vp, d]),
             "g5_compute_dt_buffer": (ctypes.c_int, [vp, vp, P(d)]),
+            "g5_step_noalloc": (ctypes.c_int, [P(G5Config), vp, vp, d, P(d), P(i32)]),
             "g5_compute_dt": (ctypes.c_int, [vp, P(d)]),
             "g5_fill_outside": (ctypes.c_int, [vp, P(d), sz]),
             "g5_outside_coords": (ctypes.c_int, [vp, P(i32), P(i32), sz]),
@@ -246,7 +247,7 @@ def exported_symbols():
         "g5_pool_last_step_ms", "g5_pool_set_launch", "g5_pool_get_launch", "g5_pool_set_autotune",
         "g5_pool_properties", "g5_pool_prepare", "g5_host_alloc", "g5_host_free", "g5_pool_set_clock", "g5_pool_step_async", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
         "g5_pool_admit_spec", "g5_serve", "g5_serve_run", "g5_spec_state_size", "g5_pool_failure",
-        "g5_set_block_order", "g5_do_time_step_buffers", "g5_compute_dt_buffer",
+        "g5_set_block_order", "g5_do_time_step_buffers", "g5_compute_dt_buffer", "g5_step_noalloc",
     ]
 
 
@@ -308,6 +309,16 @@ def make_config(dim, nb, dx, *, bsize=16, periodic=(0, 0, 0), bc=("ZERO_GRADIENT
         for k in range(u.size):
             c.inlet_u[k] = u[k]
     return c
+
+
+def step_noalloc(cfg, U_dev, U2_dev, dt):
+    """g5_step_noalloc on device addresses U_dev -> U2_dev; returns (next dt, failure bits)"""
+    dn, f = ctypes.c_double(), ctypes.c_int32()
+    rc = lib().g5_step_noalloc(ctypes.byref(cfg), ctypes.c_void_p(U_dev), ctypes.c_void_p(U2_dev), dt,
+                               ctypes.byref(dn), ctypes.byref(f))
+    if rc != G5_OK:
+        raise G5Error(rc, "g5_step_noalloc: " + lib().g5_last_error(None).decode())
+    return dn.value, f.value
 
 
 def region_conservative(cfg, reg):

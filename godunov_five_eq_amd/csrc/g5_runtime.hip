@@ -1019,6 +1019,48 @@ read_ctl(g5_pool * p)
   return G5_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Scratch of g5_step_noalloc: one solve's descriptor, control record (CFL max and failure bits),
+// launch list and slot table, in a file-scope device array instead of hipMalloc'd memory.  Each
+// GPU holds its own copy at its own address, found once per device with hipGetSymbolAddress;
+// host code uses only those addresses (the array's name is a host-side stand-in).
+// ---------------------------------------------------------------------------------------------
+constexpr int64_t kNoallocMaxBlocks = 32768;
+constexpr size_t  kNoallocCtlOff = (sizeof(SolveDesc) + 255) / 256 * 256;
+constexpr size_t  kNoallocBlocksOff = kNoallocCtlOff + 256;
+constexpr size_t  kNoallocSlotsOff = kNoallocBlocksOff + kNoallocMaxBlocks * sizeof(BlockDesc);
+constexpr size_t  kNoallocBytes = kNoallocSlotsOff + kNoallocMaxBlocks * sizeof(int32_t);
+static_assert(sizeof(SolveCtl) <= 256, "control record fits its 256-byte slot");
+
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+constexpr int kNoallocMaxDevices = 64;
+std::mutex    g_noalloc_mutex;
+void *        g_noalloc_ptr[kNoallocMaxDevices] = {};
+// host staging of the launch list and the slot table (no heap use per call)
+BlockDesc     g_noalloc_list[kNoallocMaxBlocks];
+int32_t       g_noalloc_slots[kNoallocMaxBlocks];
+
+// device address of this device's scratch (looked up on the device's first call)
+int
+noalloc_scratch(unsigned char ** out, std::string & err)
+{
+  int dev = 0;
+  G5_HIP(err, hipGetDevice(&dev));
+  if (dev < 0 or dev >= kNoallocMaxDevices)
+    return fail(err, G5_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  if (g_noalloc_ptr[dev] == nullptr)
+  {
+    void * p = nullptr;
+    G5_HIP(err, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (p == nullptr)
+      return fail(err, G5_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = static_cast<unsigned char *>(g_noalloc_ptr[dev]);
+  return G5_OK;
+}
+
 } // namespace
 
 extern "C"
@@ -1908,6 +1950,93 @@ g5_time_step_kernel(g5_ctx * ctx, int32_t nsteps, double dt, double * mean_ms)
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *mean_ms = static_cast<double>(ms) / nsteps;
+  return G5_OK;
+}
+
+
+int
+g5_step_noalloc(const g5_config * cfg, const double * U_dev, double * U2_dev, double dt, double * dt_next,
+                int32_t * failure)
+{
+  std::lock_guard<std::mutex> lk(g_noalloc_mutex);
+  if (not cfg or not U_dev or not U2_dev or not dt_next or U_dev == U2_dev)
+    return fail(g_create_error, G5_ERR_INVALID, "g5_step_noalloc: null or aliased argument");
+  std::string msg;
+  int         rc = validate(*cfg, msg);
+  if (rc != G5_OK)
+    return fail(g_create_error, rc, msg);
+  if (cfg->integrator == G5_TI_RK2_SSP)
+    return fail(g_create_error, G5_ERR_UNSUPPORTED,
+                "g5_step_noalloc: RK2_SSP needs a stage buffer of the whole state (use g5_do_time_step_buffers)");
+  if (not(dt > 0.0))
+    return fail(g_create_error, G5_ERR_INVALID, "dt must be > 0");
+  g5_config c = *cfg;
+  for (int d = c.dim; d < 3; ++d)
+    c.nb[d] = 1;
+  const int64_t nblk = nblocks_of(c);
+  if (nblk > kNoallocMaxBlocks)
+    return fail(g_create_error, G5_ERR_TOO_LARGE,
+                "g5_step_noalloc: " + std::to_string(nblk) + " blocks, beyond the scratch maximum of " +
+                  std::to_string(kNoallocMaxBlocks));
+  unsigned char * base = nullptr;
+  rc = noalloc_scratch(&base, g_create_error);
+  if (rc != G5_OK)
+    return rc;
+  // descriptor, cleared control record (CFL max 0, no failure), brick-ordered launch list and
+  // identity slot table (block k of U in slot k)
+  SolveDesc sd;
+  make_solve_desc(c, 0, sd);
+  SolveCtl ctl{};
+  int64_t  k = 0;
+  for (int z = 0; z < c.nb[2]; ++z)
+    for (int y = 0; y < c.nb[1]; ++y)
+      for (int x = 0; x < c.nb[0]; ++x, ++k)
+      {
+        g_noalloc_list[k] = BlockDesc{ 0, static_cast<int32_t>(k), static_cast<int16_t>(x), static_cast<int16_t>(y),
+                                       static_cast<int16_t>(z), 0 };
+        g_noalloc_slots[k] = static_cast<int32_t>(k);
+      }
+  hipStream_t st = nullptr; // the null stream: the call is ordered after the caller's blocking work
+  G5_HIP(g_create_error, hipMemcpyAsync(base, &sd, sizeof(SolveDesc), hipMemcpyHostToDevice, st));
+  G5_HIP(g_create_error, hipMemcpyAsync(base + kNoallocCtlOff, &ctl, sizeof(SolveCtl), hipMemcpyHostToDevice, st));
+  G5_HIP(g_create_error, hipMemcpyAsync(base + kNoallocBlocksOff, g_noalloc_list, nblk * sizeof(BlockDesc),
+                                        hipMemcpyHostToDevice, st));
+  G5_HIP(g_create_error, hipMemcpyAsync(base + kNoallocSlotsOff, g_noalloc_slots, nblk * sizeof(int32_t),
+                                        hipMemcpyHostToDevice, st));
+  StepArgs A;
+  A.buf0 = nullptr;
+  A.buf1 = nullptr;
+  A.U_in = U_dev;
+  A.U_out = U2_dev;
+  A.blocks = reinterpret_cast<BlockDesc const *>(base + kNoallocBlocksOff);
+  A.solves = reinterpret_cast<SolveDesc const *>(base);
+  A.ctl = reinterpret_cast<SolveCtl *>(base + kNoallocCtlOff);
+  A.slots = reinterpret_cast<int32_t const *>(base + kNoallocSlotsOff);
+  A.nblocks = static_cast<int32_t>(nblk);
+  A.per_xcd = static_cast<int32_t>((nblk + 7) / 8);
+  A.wg_per_xcd = 0;
+  A.remap = 1;
+  A.dt_fixed = dt;
+  const bool hancock = c.integrator == G5_TI_HANCOCK;
+  const bool mp = c.source_term == G5_SOURCE_MILLER_PUCKETT;
+  if (c.dim == 2)
+    launch_step2d(A, c.thinc_enabled != 0, hancock, c.precision, false, mp, st);
+  else
+    launch_step3d(A, c.thinc_enabled != 0, hancock, c.precision, true, c.inlet != G5_INLET_NONE, mp, st);
+  G5_HIP(g_create_error, hipGetLastError());
+  G5_HIP(g_create_error, hipMemcpy(&ctl, base + kNoallocCtlOff, sizeof(SolveCtl), hipMemcpyDeviceToHost));
+  double inv;
+  std::memcpy(&inv, &ctl.invdt_bits, sizeof(double));
+  *dt_next = c.cfl / inv;
+  if (failure)
+    *failure = static_cast<int32_t>(ctl.failure);
+  if (ctl.failure != 0u or not(inv > 0.0 and std::isfinite(inv)))
+  {
+    ctl.failure |= ctl.failure == 0u ? static_cast<uint32_t>(G5_FAIL_NONFINITE) : 0u;
+    if (failure)
+      *failure = static_cast<int32_t>(ctl.failure);
+    return fail(g_create_error, G5_ERR_DIVERGED, failure_message(0, ctl));
+  }
   return G5_OK;
 }
 

@@ -201,6 +201,21 @@ int g5_do_time_step_buffers(g5_ctx * ctx, const double * U_dev, double * U2_dev,
  * Synchronises the context's stream. */
 int g5_compute_dt_buffer(g5_ctx * ctx, const double * U_dev, double * dt);
 
+/* The step of g5_do_time_step_buffers followed by the dt of its result (g5_compute_dt_buffer on U2),
+ * without a context and without allocating: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  U and U2 hold the blocks of cfg's brick in lexicographic order; *dt_next
+ * receives cfl / (CFL maximum of U2), *failure (optional) the G5_FAIL_* bits of U2 (then
+ * G5_ERR_DIVERGED).  The solve's descriptor, control record, launch list and slot table live in a
+ * file-scope device array of 641 KiB per GPU (held until the process ends), so a call takes at most
+ * 32768 blocks (G5_ERR_TOO_LARGE beyond) and HANCOCK or RK1 (RK2_SSP would need a stage buffer of
+ * the whole state: G5_ERR_UNSUPPORTED).  Runs on the null stream and synchronises it.  Every call
+ * on a GPU shares that GPU's array: calls that overlap on one GPU (from several host threads
+ * without the library's own lock, or several processes) race; calls from one process are
+ * serialised by a lock. */
+int g5_step_noalloc(const g5_config * cfg, const double * U_dev, double * U2_dev, double dt, double * dt_next,
+                    int32_t * failure);
+
 /* ---- the step ---- */
 
 /* Replaces SolverGodunovFiveEq::compute_dt_local (SolverGodunovFiveEq.cpp:368-390) and
