@@ -36,8 +36,11 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"
 # per-file: the 3-D z-march runs one wavefront per SIMD in FP64, where the ILP-oriented scheduler
 # hides more latency (measured 2.6% faster on the 256^3 FP64 step, neutral on the mixed one; the
 # 2-D step, at several wavefronts per SIMD, is 10-21% slower with it, so it keeps the default)
+# The reassociated FP64 instances (G5_PREC_FP64_FAST) also let the compiler contract a*b+c into
+# FMA (held to the same relative L1 bar against the bit-exact path as the rest of that path).
 FILE_FLAGS = {"g5_step3d.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
-              "g5_step3d_fast.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
+              "g5_step3d_fast.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=fast"],
+              "g5_step2d_fast.hip": ["-ffp-contract=fast"]}
 
 
 def _run(cmd, verbose):

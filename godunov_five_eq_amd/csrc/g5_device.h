@@ -342,11 +342,35 @@ slope1(T q, T qp, T qm, T slope_type)
   return dq;
 }
 
-template <typename T>
+// Reassociated FP64 path: minmod (slope_type 1) as sign(dc) min(|dl|, |dr|) when dl dr > 0, without
+// the bound by |dc| = |dl + dr|/2 >= min(|dl|, |dr|) that holds in exact arithmetic
+G5D double
+slope1_fast(double q, double qp, double qm, double slope_type)
+{
+  if (slope_type == 1.0)
+  {
+    const double dl = q - qm, dr = qp - q;
+    const double m = fmin(fabs(dl), fabs(dr));
+    return dl * dr <= 0.0 ? 0.0 : ((qp - qm) >= 0.0 ? m : -m);
+  }
+  return slope1(q, qp, qm, slope_type);
+}
+
+template <typename T, bool FAST = false>
 G5D SlopeT<T>
 slopes_of(PrimT<T> const & c, PrimT<T> const & p, PrimT<T> const & m, T st)
 {
   SlopeT<T> s;
+  if constexpr (FAST)
+  {
+    s.r0 = slope1_fast(c.r0, p.r0, m.r0, st);
+    s.r1 = slope1_fast(c.r1, p.r1, m.r1, st);
+    s.p = slope1_fast(c.p, p.p, m.p, st);
+    s.u = slope1_fast(c.u, p.u, m.u, st);
+    s.v = slope1_fast(c.v, p.v, m.v, st);
+    s.w = slope1_fast(c.w, p.w, m.w, st);
+    return s;
+  }
   s.r0 = slope1(c.r0, p.r0, m.r0, st);
   s.r1 = slope1(c.r1, p.r1, m.r1, st);
   s.p = slope1(c.p, p.p, m.p, st);
@@ -446,11 +470,14 @@ hancock_terms(SolveDesc const & S, PrimT<T> const & q, SlopeT<T> const & sx, Slo
 // q + off_x*sx + off_y*sy (+ off_z*sz), one offset +-0.5 on axis AX, the others 0.0 (kept so that
 // the operation sequence is the reference's).  AX = -1: the axis is the run-time value ax (one
 // code path for lanes that reconstruct faces on different axes; the same operations per lane).
-template <int DIM, int AX, typename T>
+// FAST (reassociated FP64 path): q + o * s of the face axis alone.
+template <int DIM, int AX, typename T, bool FAST = false>
 G5D T
 muscl_extrap(T qv, T sxv, T syv, T szv, T o, int ax = AX)
 {
   const int a = AX >= 0 ? AX : ax;
+  if constexpr (FAST)
+    return qv + o * (a == 0 ? sxv : (a == 1 ? syv : szv));
   const T   ox = a == 0 ? o : T(0.0);
   const T   oy = a == 1 ? o : T(0.0);
   const T   oz = a == 2 ? o : T(0.0);
@@ -460,18 +487,18 @@ muscl_extrap(T qv, T sxv, T syv, T szv, T o, int ax = AX)
 }
 
 // reconstruct_state_{2d,3d} face part (scheme/ComputeFluxesAndStoreFunctor.cpp:164-211)
-template <int DIM, int AX, bool HANCOCK, typename T>
+template <int DIM, int AX, bool HANCOCK, typename T, bool FAST = false>
 G5D FaceT<T>
 face_muscl(SolveDesc const & S, PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy,
            SlopeT<T> const & sz, HancockT<T> const & h, T o, int ax = AX)
 {
   FaceT<T> f;
-  f.r0 = muscl_extrap<DIM, AX>(q.r0, sx.r0, sy.r0, sz.r0, o, ax);
-  f.r1 = muscl_extrap<DIM, AX>(q.r1, sx.r1, sy.r1, sz.r1, o, ax);
-  f.p = muscl_extrap<DIM, AX>(q.p, sx.p, sy.p, sz.p, o, ax);
-  f.u = muscl_extrap<DIM, AX>(q.u, sx.u, sy.u, sz.u, o, ax);
-  f.v = muscl_extrap<DIM, AX>(q.v, sx.v, sy.v, sz.v, o, ax);
-  f.w = DIM == 3 ? muscl_extrap<DIM, AX>(q.w, sx.w, sy.w, sz.w, o, ax) : T(0.0);
+  f.r0 = muscl_extrap<DIM, AX, T, FAST>(q.r0, sx.r0, sy.r0, sz.r0, o, ax);
+  f.r1 = muscl_extrap<DIM, AX, T, FAST>(q.r1, sx.r1, sy.r1, sz.r1, o, ax);
+  f.p = muscl_extrap<DIM, AX, T, FAST>(q.p, sx.p, sy.p, sz.p, o, ax);
+  f.u = muscl_extrap<DIM, AX, T, FAST>(q.u, sx.u, sy.u, sz.u, o, ax);
+  f.v = muscl_extrap<DIM, AX, T, FAST>(q.v, sx.v, sy.v, sz.v, o, ax);
+  f.w = DIM == 3 ? muscl_extrap<DIM, AX, T, FAST>(q.w, sx.w, sy.w, sz.w, o, ax) : T(0.0);
   if (HANCOCK)
   {
     f.r0 += h.r0;
@@ -716,13 +743,13 @@ face_thinc(PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy, Slope
   else
   {
     f.a = q.a;
-    f.r0 = muscl_extrap<DIM, AX>(q.r0, sx.r0, sy.r0, sz.r0, o, ax);
-    f.r1 = muscl_extrap<DIM, AX>(q.r1, sx.r1, sy.r1, sz.r1, o, ax);
+    f.r0 = muscl_extrap<DIM, AX, T, FAST>(q.r0, sx.r0, sy.r0, sz.r0, o, ax);
+    f.r1 = muscl_extrap<DIM, AX, T, FAST>(q.r1, sx.r1, sy.r1, sz.r1, o, ax);
   }
-  f.p = muscl_extrap<DIM, AX>(q.p, sx.p, sy.p, sz.p, o, ax);
-  f.u = muscl_extrap<DIM, AX>(q.u, sx.u, sy.u, sz.u, o, ax);
-  f.v = muscl_extrap<DIM, AX>(q.v, sx.v, sy.v, sz.v, o, ax);
-  f.w = DIM == 3 ? muscl_extrap<DIM, AX>(q.w, sx.w, sy.w, sz.w, o, ax) : T(0.0);
+  f.p = muscl_extrap<DIM, AX, T, FAST>(q.p, sx.p, sy.p, sz.p, o, ax);
+  f.u = muscl_extrap<DIM, AX, T, FAST>(q.u, sx.u, sy.u, sz.u, o, ax);
+  f.v = muscl_extrap<DIM, AX, T, FAST>(q.v, sx.v, sy.v, sz.v, o, ax);
+  f.w = DIM == 3 ? muscl_extrap<DIM, AX, T, FAST>(q.w, sx.w, sy.w, sz.w, o, ax) : T(0.0);
   if (HANCOCK)
   {
     f.r0 += h.r0;

@@ -201,8 +201,8 @@ predict_faces2(SolveDesc const & S, T const * sQ, int gi, int gj, T dtdx, bool a
   const PrimT<T>  qym = sq_load2(sQ, tidx2(gi, gj - 1));
   const PrimT<T>  qyp = sq_load2(sQ, tidx2(gi, gj + 1));
   const T         st = static_cast<T>(S.slope_type);
-  const SlopeT<T> sx = slopes_of(qc, qxp, qxm, st);
-  const SlopeT<T> sy = slopes_of(qc, qyp, qym, st);
+  const SlopeT<T> sx = slopes_of<T, FAST>(qc, qxp, qxm, st);
+  const SlopeT<T> sy = slopes_of<T, FAST>(qc, qyp, qym, st);
   const SlopeT<T> sz = { T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
   HancockT<T>     h = { T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
   if (HANCOCK)
@@ -211,15 +211,15 @@ predict_faces2(SolveDesc const & S, T const * sQ, int gi, int gj, T dtdx, bool a
   {
     if (all)
     {
-      fxm = face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, T(-0.5));
-      fxp = face_muscl<2, 0, HANCOCK>(S, qc, sx, sy, sz, h, T(0.5));
-      fym = face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, T(-0.5));
-      fyp = face_muscl<2, 1, HANCOCK>(S, qc, sx, sy, sz, h, T(0.5));
+      fxm = face_muscl<2, 0, HANCOCK, T, FAST>(S, qc, sx, sy, sz, h, T(-0.5));
+      fxp = face_muscl<2, 0, HANCOCK, T, FAST>(S, qc, sx, sy, sz, h, T(0.5));
+      fym = face_muscl<2, 1, HANCOCK, T, FAST>(S, qc, sx, sy, sz, h, T(-0.5));
+      fyp = face_muscl<2, 1, HANCOCK, T, FAST>(S, qc, sx, sy, sz, h, T(0.5));
     }
     else
     {
       // one reconstruction on the lane's axis ax1
-      f1 = face_muscl<2, -1, HANCOCK>(S, qc, sx, sy, sz, h, T(0.5) * static_cast<T>(ib1), ax1);
+      f1 = face_muscl<2, -1, HANCOCK, T, FAST>(S, qc, sx, sy, sz, h, T(0.5) * static_cast<T>(ib1), ax1);
     }
     return;
   }

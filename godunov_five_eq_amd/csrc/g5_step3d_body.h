@@ -270,9 +270,9 @@ struct Pred3
     const PrimT<T> qzm = q3_load(sQ, aslot(P - 1), c);
     const PrimT<T> qzp = q3_load(sQ, aslot(P + 1), c);
     const T        st = static_cast<T>(S.slope_type);
-    sx = slopes_of(q, qxp, qxm, st);
-    sy = slopes_of(q, qyp, qym, st);
-    sz = slopes_of(q, qzp, qzm, st);
+    sx = slopes_of<T, FAST>(q, qxp, qxm, st);
+    sy = slopes_of<T, FAST>(q, qyp, qym, st);
+    sz = slopes_of<T, FAST>(q, qzp, qzm, st);
     axm = qxm.a;
     axp = qxp.a;
     aym = qym.a;
@@ -308,7 +308,7 @@ struct Pred3
   face(SolveDesc const & S, int ib, ThincAxisT<T> const & ta, int ax = AX) const
   {
     if (!THINC)
-      return face_muscl<3, AX, HANCOCK>(S, q, sx, sy, sz, h, T(0.5) * static_cast<T>(ib), ax);
+      return face_muscl<3, AX, HANCOCK, T, FAST>(S, q, sx, sy, sz, h, T(0.5) * static_cast<T>(ib), ax);
     return face_thinc<3, AX, HANCOCK, T, FAST>(q, sx, sy, sz, h, ta, ib, ax);
   }
 };
