@@ -273,7 +273,11 @@ config_from_ini(ConfigMap const & c)
     cfg.gamma[k] = c.getReal(sec, "gamma", 1.4);
     cfg.pinf[k] = eos == "MG_STIFFENED_GAS" ? c.getReal(sec, "pinf", 0.0) : 0.0;
   }
-  cfg.precision = c.getString("hydro", "precision", "FP64") == "MIXED" ? G5_PREC_MIXED : G5_PREC_FP64;
+  // [hydro] precision (this layer's key): FP64 (bit-exact), FP64_FAST (reassociated) or MIXED
+  const std::string prec = c.getString("hydro", "precision", "FP64");
+  if (prec != "FP64" and prec != "FP64_FAST" and prec != "MIXED")
+    throw std::runtime_error("unknown [hydro] precision " + prec + " (FP64, FP64_FAST, MIXED)");
+  cfg.precision = prec == "MIXED" ? G5_PREC_MIXED : (prec == "FP64_FAST" ? G5_PREC_FP64_FAST : G5_PREC_FP64);
   // [volume_fraction_advection] source_term_type (app/test_shock_bubble_air_water_2d.ini:106-108)
   const std::string st = c.getString("volume_fraction_advection", "source_term_type", "ALLAIRE_KOKH");
   if (st == "MILLER_PUCKETT")
