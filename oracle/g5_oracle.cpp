@@ -586,11 +586,24 @@ struct Solver
   std::vector<int32_t> bcoord;  // 3 per block (owned then outside), outside coords may be -1 or nb
   std::vector<int32_t> lookup;  // extended grid (nb+2)^dim -> block index or -1
   std::vector<double>  U, U2, Q, Sx, Sy, Sz, F, Us, Urk;
+  // Two-level static mesh (level 0: the brick's blocks; level 1: coarse blocks refined into 2^dim
+  // blocks of half the cell length, at least one block away from the brick's border).  bcoord
+  // holds a block's coordinates at its own level; refined coarse positions are -1 in lookup.
+  bool                 amr = false;
+  int                  prolong = 0;  // 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD
+  std::vector<int32_t> blev;         // level of every block (outside blocks: 0)
+  std::vector<int32_t> lookup1;      // fine grid (2 nb)^dim -> owned block index or -1
+  std::vector<char>    refined0;     // coarse grid nb^dim: refined
 
-  explicit Solver(g5o_params const & p)
+  explicit Solver(g5o_params const & p, std::vector<int32_t> const & refined = {}, int prolongation = 0)
     : P(p)
     , H(p.dim)
   {
+    amr = not refined.empty();
+    prolong = prolongation;
+    refined0.assign(static_cast<size_t>(nbd_of(p, 0)) * nbd_of(p, 1) * nbd_of(p, 2), 0);
+    for (size_t k = 0; k + 2 < refined.size(); k += 3)
+      refined0[refined[k] + nbd_of(p, 0) * (refined[k + 1] + nbd_of(p, 1) * (p.dim == 3 ? refined[k + 2] : 0))] = 1;
     dim = p.dim;
     b = p.bsize;
     nvar = H.nbvar;
@@ -607,6 +620,11 @@ struct Solver
     build_mesh();
   }
 
+  static int
+  nbd_of(g5o_params const & p, int d)
+  {
+    return d < p.dim ? p.nb[d] : 1;
+  }
   static int
   ipow(int a, int e)
   {
@@ -636,12 +654,28 @@ struct Solver
     for (int z = 0; z < nz; ++z)
       for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x)
-        {
-          bcoord.push_back(x);
-          bcoord.push_back(y);
-          bcoord.push_back(z);
-        }
-    nowned = nx * ny * nz;
+          if (not refined0[x + nx * (y + ny * z)])
+          {
+            bcoord.push_back(x);
+            bcoord.push_back(y);
+            bcoord.push_back(z);
+            blev.push_back(0);
+          }
+    // fine blocks: lexicographic over the fine grid, those whose coarse parent is refined
+    const int fz = dim == 3 ? 2 * nz : 1;
+    lookup1.assign(static_cast<size_t>(2 * nx) * (2 * ny) * fz, -1);
+    for (int z = 0; z < fz; ++z)
+      for (int y = 0; y < 2 * ny; ++y)
+        for (int x = 0; x < 2 * nx; ++x)
+          if (refined0[x / 2 + nx * (y / 2 + ny * (dim == 3 ? z / 2 : 0))])
+          {
+            lookup1[x + 2 * nx * (y + 2 * ny * z)] = static_cast<int32_t>(bcoord.size() / 3);
+            bcoord.push_back(x);
+            bcoord.push_back(y);
+            bcoord.push_back(z);
+            blev.push_back(1);
+          }
+    nowned = static_cast<int>(bcoord.size() / 3);
     const int zlo = dim == 3 ? -1 : 0, zhi = dim == 3 ? nz : 0;
     for (int z = zlo; z <= zhi; ++z)
       for (int y = -1; y <= ny; ++y)
@@ -664,6 +698,7 @@ struct Solver
             bcoord.push_back(x);
             bcoord.push_back(y);
             bcoord.push_back(z);
+            blev.push_back(0);
           }
         }
     ntotal = static_cast<int>(bcoord.size() / 3);
@@ -671,7 +706,8 @@ struct Solver
     const int ez = dim == 3 ? nz + 2 : 1;
     lookup.assign(static_cast<size_t>(nx + 2) * (ny + 2) * ez, -1);
     for (int ib = 0; ib < ntotal; ++ib)
-      lookup[ext_index(bcoord[3 * ib], bcoord[3 * ib + 1], bcoord[3 * ib + 2])] = ib;
+      if (blev[ib] == 0)
+        lookup[ext_index(bcoord[3 * ib], bcoord[3 * ib + 1], bcoord[3 * ib + 2])] = ib;
     U.assign(static_cast<size_t>(ntotal) * nvar * ncell, 0.0);
     U2.assign(static_cast<size_t>(nowned) * nvar * ncell, 0.0);
     Q.assign(static_cast<size_t>(nowned) * nvar * ncellg, 0.0);
@@ -759,6 +795,189 @@ struct Solver
     for (int v = 0; v < nvar; ++v)
       st[v] = u(A, ib, v, cell);
     return st;
+  }
+
+  // ---- two-level mesh ------------------------------------------------------------------------
+  // cell length of a block's level (compute_cell_length(level, bx) * scaling_factor: halved per level)
+  double
+  dx_of(int ib) const
+  {
+    return blev[ib] ? 0.5 * P.dx : P.dx;
+  }
+  static int
+  floor_div(int a, int n)
+  {
+    return a >= 0 ? a / n : -((-a + n - 1) / n);
+  }
+
+  // Conservative state at global cell g of level l (the getNeighLoc lookup of StencilHelper,
+  // ConvertToPrimitivesVariablesFunctor.cpp:876-918, for a two-level mesh):
+  //  - level 0, a coarse leaf (or the outside ring, periodic wrap): the cell itself;
+  //  - level 0 inside a refined block: restriction, get_conservative_vars_restriction (:214-250),
+  //    the average of the 2^dim fine siblings (compute_siblings_average of the absent core:
+  //    summed in lexicographic order, x fastest, then divided by 2^dim) for IE, IU, IV, IW, IAD0,
+  //    IAD1, IA0, with IA1 = 1 - IA0 and ID = IAD0 + IAD1;
+  //  - level 1 outside the fine blocks: prolongation of the coarse parent (:889-910), SIMPLE_COPY
+  //    or EXTRAPOLATE_LINEAR_MINMOD (linear_extrapolate_using_limited_slopes, :553-770).
+  State
+  cons_level(std::vector<double> & A, int l, int const g[3])
+  {
+    if (l == 0)
+    {
+      int c[3] = { 0, 0, 0 }, cell[3] = { 0, 0, 0 };
+      for (int d = 0; d < dim; ++d)
+      {
+        c[d] = floor_div(g[d], b);
+        cell[d] = g[d] - c[d] * b;
+        if (P.periodic[d])
+        {
+          const int n = nbd(d);
+          c[d] = ((c[d] % n) + n) % n;
+        }
+      }
+      const int ib = lookup[ext_index(c[0], c[1], c[2])];
+      if (ib >= 0)
+        return load_cons(A, ib, cidx(cell[0], cell[1], cell[2]));
+      return restriction(A, g);
+    }
+    int c[3] = { 0, 0, 0 }, cell[3] = { 0, 0, 0 };
+    for (int d = 0; d < dim; ++d)
+    {
+      c[d] = floor_div(g[d], b);
+      cell[d] = g[d] - c[d] * b;
+    }
+    const int ib = lookup1[c[0] + 2 * nbd(0) * (c[1] + 2 * nbd(1) * c[2])];
+    if (ib >= 0)
+      return load_cons(A, ib, cidx(cell[0], cell[1], cell[2]));
+    return prolongation(A, g);
+  }
+
+  // coarse cell g covered by fine blocks: average of its 2^dim fine cells
+  State
+  restriction(std::vector<double> & A, int const g[3])
+  {
+    // the averaged variables (IW only in 3-D: in 2-D its id is IV's)
+    const int vars[7] = { H.IE, H.IU, H.IV, H.IAD0, H.IAD1, H.IA0, H.IW };
+    const int nv = dim == 3 ? 7 : 6;
+    State     sum{};
+    bool      first = true;
+    for (int kz = 0; kz < (dim == 3 ? 2 : 1); ++kz)
+      for (int ky = 0; ky < 2; ++ky)
+        for (int kx = 0; kx < 2; ++kx)
+        {
+          const int   f[3] = { 2 * g[0] + kx, 2 * g[1] + ky, dim == 3 ? 2 * g[2] + kz : 0 };
+          const State u1 = cons_level(A, 1, f);
+          for (int t = 0; t < nv; ++t)
+            sum[vars[t]] = first ? u1[vars[t]] : sum[vars[t]] + u1[vars[t]];
+          first = false;
+        }
+    const double vr = dim == 3 ? 8.0 : 4.0;
+    State        u{};
+    for (int t = 0; t < nv; ++t)
+      u[vars[t]] = sum[vars[t]] / vr;
+    u[H.IA1] = 1.0 - u[H.IA0];
+    u[H.ID] = u[H.IAD0] + u[H.IAD1];
+    return u;
+  }
+
+  // fine cell g outside the fine blocks: its coarse parent, copied or extrapolated with minmod
+  // slopes of the conservative variables (slope_type 1; the neighbours of the parent at the
+  // coarse level, restricted where they are refined) at the fine cell's side -1/+1 of the parent
+  State
+  prolongation(std::vector<double> & A, int const g[3])
+  {
+    int gc[3] = { 0, 0, 0 }, ix[3] = { 0, 0, 0 };
+    for (int d = 0; d < dim; ++d)
+    {
+      gc[d] = floor_div(g[d], 2);
+      ix[d] = (g[d] - 2 * gc[d]) == 1 ? 1 : -1;
+    }
+    const State center = cons_level(A, 0, gc);
+    if (prolong == 0)
+      return center;
+    State slope[3];
+    for (int d = 0; d < dim; ++d)
+    {
+      int gl[3] = { gc[0], gc[1], gc[2] }, gr[3] = { gc[0], gc[1], gc[2] };
+      gl[d] -= 1;
+      gr[d] += 1;
+      const State left = cons_level(A, 0, gl), right = cons_level(A, 0, gr);
+      for (int v = 0; v < nvar; ++v)
+        slope[d][v] = slope_unsplit_scalar(center[v], right[v], left[v], 1.0);
+    }
+    // the extrapolated variables (IW only in 3-D: in 2-D its id is IV's)
+    const int vars[7] = { H.IAD0, H.IAD1, H.IA0, H.IE, H.IU, H.IV, H.IW };
+    State     u{};
+    for (int t = 0; t < (dim == 3 ? 7 : 6); ++t)
+    {
+      const int v = vars[t];
+      double val = center[v] + 0.25 * static_cast<double>(ix[0]) * slope[0][v] +
+                   0.25 * static_cast<double>(ix[1]) * slope[1][v];
+      if (dim == 3)
+        val = center[v] + 0.25 * static_cast<double>(ix[0]) * slope[0][v] +
+              0.25 * static_cast<double>(ix[1]) * slope[1][v] + 0.25 * static_cast<double>(ix[2]) * slope[2][v];
+      u[v] = val;
+    }
+    u[H.ID] = u[H.IAD0] + u[H.IAD1];
+    u[H.IA1] = 1.0 - u[H.IA0];
+    return u;
+  }
+
+  // the coarse-level neighbour of owned coarse block ib across face (dir, side) is refined
+  // (conformal status NEIGHBOR_IS_FINER)
+  bool
+  finer_neighbor(int ib, int dir, int side) const
+  {
+    if (not amr or blev[ib] != 0)
+      return false;
+    int c[3] = { bcoord[3 * ib], bcoord[3 * ib + 1], bcoord[3 * ib + 2] };
+    c[dir] += side;
+    for (int d = 0; d < dim; ++d)
+      if (c[d] < 0 or c[d] >= nbd(d))
+        return false;
+    return refined0[c[0] + nbd(0) * (c[1] + nbd(1) * (dim == 3 ? c[2] : 0))] != 0;
+  }
+
+  // get_flux_from_fine_neighbor / get_ustar_flux_from_fine_neighbor
+  // (ReadFluxesAndConservativeUpdateFunctor.h:246-307): the sum of the fluxes (or of u*) of the
+  // 2^(dim-1) fine faces covering the coarse face of cell (i, j, k) of block ib on side `side` of
+  // direction dir (compute_face_siblings_sum of the absent core: summed with the first transverse
+  // index fastest); var < 0: u*
+  double
+  fine_face_sum(int ib, int dir, int side, int i, int j, int k, int var)
+  {
+    const int ci[3] = { i, j, k };
+    int       gc[3];
+    for (int d = 0; d < 3; ++d)
+      gc[d] = bcoord[3 * ib + d] * b + ci[d];
+    const int t1 = (dir + 1) % dim, t2 = dim == 3 ? (dir + 2) % 3 : -1;
+    double    sum = 0.0;
+    bool      first = true;
+    for (int k2 = 0; k2 < (dim == 3 ? 2 : 1); ++k2)
+      for (int k1 = 0; k1 < 2; ++k1)
+      {
+        int f[3] = { 2 * gc[0], 2 * gc[1], dim == 3 ? 2 * gc[2] : 0 };
+        // the fine cells across the face: left of it (side -1) or right of it (side +1)
+        f[dir] = side < 0 ? 2 * gc[dir] - 1 : 2 * gc[dir] + 2;
+        f[t1] += k1;
+        if (t2 >= 0)
+          f[t2] += k2;
+        int fb[3] = { 0, 0, 0 }, fl_[3] = { 0, 0, 0 };
+        for (int d = 0; d < dim; ++d)
+        {
+          fb[d] = floor_div(f[d], b);
+          fl_[d] = f[d] - fb[d] * b;
+        }
+        const int jb = lookup1[fb[0] + 2 * nbd(0) * (fb[1] + 2 * nbd(1) * fb[2])];
+        // the fine cell's right face (use_right_flux) for side -1, its left face for side +1
+        int fi[3] = { fl_[0], fl_[1], fl_[2] };
+        if (side < 0)
+          fi[dir] += 1;
+        const double val = var < 0 ? us(jb, dir, fi[0], fi[1], fi[2]) : fl(jb, var, dir, fi[0], fi[1], fi[2]);
+        sum = first ? val : sum + val;
+        first = false;
+      }
+    return sum;
   }
 
   // -------------------------------------------------------------------------------------------
@@ -877,8 +1096,20 @@ struct Solver
           dir[d] = 1;
         cin[d] = co[d] - dir[d] * b;
       }
-      const int src = (dir[0] == 0 and dir[1] == 0 and dir[2] == 0) ? ib : neighbor(ib, dir);
-      const State uLoc = load_cons(A, src, cidx(cin[0], cin[1], cin[2]));
+      State uLoc;
+      if (not amr or (dir[0] == 0 and dir[1] == 0 and dir[2] == 0))
+      {
+        const int src = (dir[0] == 0 and dir[1] == 0 and dir[2] == 0) ? ib : neighbor(ib, dir);
+        uLoc = load_cons(A, src, cidx(cin[0], cin[1], cin[2]));
+      }
+      else
+      {
+        // fill_ghosts (ConvertToPrimitivesVariablesFunctor.cpp:884-918): same level, finer
+        // (restriction) or coarser (prolongation) neighbour, by the level of the cell's leaf
+        const int g[3] = { bcoord[3 * ib] * b + co[0], bcoord[3 * ib + 1] * b + co[1],
+                           dim == 3 ? bcoord[3 * ib + 2] * b + co[2] : 0 };
+        uLoc = cons_level(A, blev[ib], g);
+      }
       State       qLoc{};
       compute_primitives(uLoc, qLoc, s, eos, dim, H);
       q(ib, H.ID, co[0], co[1], co[2]) = qLoc[H.ID];
@@ -1150,13 +1381,13 @@ struct Solver
   compute_fluxes(double dt, int dir)
   {
     const long   total = static_cast<long>(nowned) * ncellf;
-    const double dtdS = dt / P.dx;
     const bool   thinc = P.thinc_enabled != 0;
     const int    ex = dir == 0 ? b + 1 : b, ey = dir == 1 ? b + 1 : b;
 #pragma omp parallel for schedule(static)
     for (long gi = 0; gi < total; ++gi)
     {
-      const int ib = static_cast<int>(gi / ncellf);
+      const int    ib = static_cast<int>(gi / ncellf);
+      const double dtdS = dt / dx_of(ib); // dt / the block's cell length
       const int c = static_cast<int>(gi % ncellf);
       const int i = c % ex, j = (c / ex) % ey, k = dim == 3 ? c / (ex * ey) : 0;
       int       sh[3] = { 0, 0, 0 };
@@ -1237,15 +1468,42 @@ struct Solver
       double    acc[9];
       for (int v = 0; v < 9; ++v)
         acc[v] = 0.0;
-      for (int t = 0; t < nv; ++t)
-        acc[vars[t]] = acc[vars[t]] + fl(ib, vars[t], dir, i, j, k);
       const double ua0 = u(Uin, ib, H.IA0, c), ua1 = u(Uin, ib, H.IA1, c);
-      acc[H.IA0] -= st0 * us(ib, dir, i, j, k) * ua0;
-      acc[H.IA1] -= st1 * us(ib, dir, i, j, k) * ua1;
-      for (int t = 0; t < nv; ++t)
-        acc[vars[t]] = acc[vars[t]] - fl(ib, vars[t], dir, i + sh[0], j + sh[1], k + sh[2]);
-      acc[H.IA0] += st0 * us(ib, dir, i + sh[0], j + sh[1], k + sh[2]) * ua0;
-      acc[H.IA1] += st1 * us(ib, dir, i + sh[0], j + sh[1], k + sh[2]) * ua1;
+      const int    ci[3] = { i, j, k };
+      // a coarse cell on a face whose neighbour is finer takes the fine faces' sum over the
+      // volume ratio 2^dim (read_fluxes_and_update_{2d,3d} NEIGHBOR_IS_FINER branches,
+      // ReadFluxesAndConservativeUpdateFunctor.cpp:137-199)
+      const double vr = dim == 3 ? 8.0 : 4.0;
+      if (ci[dir] == 0 and finer_neighbor(ib, dir, -1))
+      {
+        for (int t = 0; t < nv; ++t)
+          acc[vars[t]] = acc[vars[t]] + fine_face_sum(ib, dir, -1, i, j, k, vars[t]) / vr;
+        const double ustar_flx = fine_face_sum(ib, dir, -1, i, j, k, -1);
+        acc[H.IA0] -= st0 * ustar_flx * ua0 / vr;
+        acc[H.IA1] -= st1 * ustar_flx * ua1 / vr;
+      }
+      else
+      {
+        for (int t = 0; t < nv; ++t)
+          acc[vars[t]] = acc[vars[t]] + fl(ib, vars[t], dir, i, j, k);
+        acc[H.IA0] -= st0 * us(ib, dir, i, j, k) * ua0;
+        acc[H.IA1] -= st1 * us(ib, dir, i, j, k) * ua1;
+      }
+      if (ci[dir] == b - 1 and finer_neighbor(ib, dir, 1))
+      {
+        for (int t = 0; t < nv; ++t)
+          acc[vars[t]] = acc[vars[t]] - fine_face_sum(ib, dir, 1, i, j, k, vars[t]) / vr;
+        const double ustar_flx = fine_face_sum(ib, dir, 1, i, j, k, -1);
+        acc[H.IA0] += st0 * ustar_flx * ua0 / vr;
+        acc[H.IA1] += st1 * ustar_flx * ua1 / vr;
+      }
+      else
+      {
+        for (int t = 0; t < nv; ++t)
+          acc[vars[t]] = acc[vars[t]] - fl(ib, vars[t], dir, i + sh[0], j + sh[1], k + sh[2]);
+        acc[H.IA0] += st0 * us(ib, dir, i + sh[0], j + sh[1], k + sh[2]) * ua0;
+        acc[H.IA1] += st1 * us(ib, dir, i + sh[0], j + sh[1], k + sh[2]) * ua1;
+      }
       double & vad0 = u(Uout, ib, H.IAD0, c);
       vad0 += acc[H.IAD0];
       if (vad0 < eps)
@@ -1324,12 +1582,13 @@ struct Solver
       const double vx = cs + std::fabs(qLoc[H.IU]);
       const double vy = cs + std::fabs(qLoc[H.IV]);
       double       val;
+      const double dx = dx_of(ib);
       if (dim == 2)
-        val = vx / P.dx + vy / P.dx;
+        val = vx / dx + vy / dx;
       else
       {
         const double vz = cs + std::fabs(qLoc[H.IW]);
-        val = vx / P.dx + vy / P.dx + vz / P.dx;
+        val = vx / dx + vy / dx + vz / dx;
       }
       invDt = std::fmax(invDt, val);
     }
@@ -1384,6 +1643,36 @@ void *
 g5o_create(const g5o_params * p)
 {
   return new Solver(*p);
+}
+
+// a two-level mesh: cfg's brick at level 0 with the n coarse blocks at coords[3k..3k+2] refined into
+// 2^dim blocks of half the cell length; prolongation 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD
+void *
+g5o_create_amr(const g5o_params * p, const int32_t * coords, int32_t n, int32_t prolongation)
+{
+  std::vector<int32_t> r(coords, coords + 3 * static_cast<size_t>(n));
+  return new Solver(*p, r, prolongation);
+}
+
+// level of each owned block (0 coarse, 1 fine); g5o_block_coords gives its coordinates at that level
+void
+g5o_block_levels(void * h, int32_t * levels)
+{
+  Solver * S = static_cast<Solver *>(h);
+  for (int ib = 0; ib < S->nowned; ++ib)
+    levels[ib] = S->blev[ib];
+}
+
+// conservative state at global cell g (3 ints) of level l of a two-level mesh, as the ghost fill
+// reads it (a leaf cell, a restriction or a prolongation)
+void
+g5o_cons_level(void * h, int32_t level, const int32_t * g, double * out)
+{
+  Solver *  S = static_cast<Solver *>(h);
+  const int gg[3] = { g[0], g[1], g[2] };
+  const State u = S->cons_level(S->U, level, gg);
+  for (int v = 0; v < S->nvar; ++v)
+    out[v] = u[v];
 }
 
 void

@@ -62,6 +62,9 @@ def lib(variant=""):
         d, i32, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
         sig = {
             "g5o_create": (vp, [P(OracleParams)]),
+            "g5o_create_amr": (vp, [P(OracleParams), P(i32), i32, i32]),
+            "g5o_block_levels": (None, [vp, P(i32)]),
+            "g5o_cons_level": (None, [vp, i32, P(i32), P(d)]),
             "g5o_destroy": (None, [vp]),
             "g5o_num_owned": (i32, [vp]),
             "g5o_num_outside": (i32, [vp]),
@@ -125,10 +128,17 @@ def params_from(cfg):
 
 
 class Oracle:
-    def __init__(self, cfg, variant=""):
+    def __init__(self, cfg, variant="", refined=None, prolongation=0):
+        """refined: coarse block coordinates (n, 3) of a two-level mesh (each refined into 2^dim
+        blocks of half the cell length); prolongation 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD"""
         self.L = lib(variant)
         self.p = params_from(cfg)
-        self.h = self.L.g5o_create(ctypes.byref(self.p))
+        if refined is None:
+            self.h = self.L.g5o_create(ctypes.byref(self.p))
+        else:
+            r = np.ascontiguousarray(refined, dtype=np.int32).reshape(-1, 3)
+            self.h = self.L.g5o_create_amr(ctypes.byref(self.p), r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                           r.shape[0], prolongation)
         self.nowned = self.L.g5o_num_owned(self.h)
         self.noutside = self.L.g5o_num_outside(self.h)
         self.nvar = 8 if cfg.dim == 2 else 9
@@ -160,6 +170,18 @@ class Oracle:
         U = np.empty((self.nowned + self.noutside) * self.nvar * self.ncell, dtype=np.float64)
         self.L.g5o_get_state_all(self.h, _dp(U))
         return U
+
+    def cons_level(self, level, g):
+        """conservative state at global cell g of a level, as the ghost fill reads it"""
+        gg = (ctypes.c_int32 * 3)(*(list(g) + [0] * (3 - len(g))))
+        out = np.zeros(self.nvar)
+        self.L.g5o_cons_level(self.h, level, gg, _dp(out))
+        return out
+
+    def block_levels(self):
+        lv = np.zeros(self.nowned, dtype=np.int32)
+        self.L.g5o_block_levels(self.h, lv.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        return lv
 
     def block_coords(self):
         c = np.empty(3 * (self.nowned + self.noutside), dtype=np.int32)

@@ -1,0 +1,122 @@
+"""The oracle's two-level static mesh (CPU): the coarse/fine coupling of the reference's V0 step
+(ghost fill by restriction and prolongation, ConvertToPrimitivesVariablesFunctor.cpp:541-918;
+fine-face flux sums at coarse faces, ReadFluxesAndConservativeUpdateFunctor.h:246-307 and the
+NEIGHBOR_IS_FINER branches of .cpp:137-199), pinned by
+  - closed forms: on a field linear in every conservative variable the restriction of a coarse
+    cell is its centre value (the mean of the 2^dim fine cells), minmod prolongation reproduces
+    the fine cells' values and SIMPLE_COPY the coarse value;
+  - invariants: a uniform state stays uniform bit for bit across the levels, and the totals of
+    mass, energy and momentum (cell values times the level's cell volume) are conserved to
+    round-off in a periodic box whose refined region is L-shaped (so coarse cells meet fine ones
+    on faces of both orientations and at a concave corner);
+  - determinism of the layout: coarse leaves first, then fine blocks, each in lexicographic order.
+"""
+import numpy as np
+import pytest
+
+REFINED = [[1, 1, 0], [2, 1, 0], [1, 2, 0]]
+
+
+def periodic_cfg(g5, dim=2, nb=4, **kw):
+    opts = dict(periodic=(1, 1, 1), bc=("PERIODIC",) * 6, gamma=(1.4, 1.4), cfl=0.6)
+    opts.update(kw)
+    return g5.make_config(dim, (nb,) * dim, 1.0 / (16 * nb), **opts)
+
+
+def sample(o, cfg, f):
+    """U[block][var][cell] of pure material 0 with (rho, u, v, p) = f(x, y) at the cell centres"""
+    lv = o.block_levels()
+    bc = o.block_coords()[: o.nowned]
+    U = np.zeros((o.nowned, 8, 256))
+    i = np.arange(16)
+    for k in range(o.nowned):
+        dx = cfg.dx / (2 ** lv[k])
+        X = (bc[k, 0] * 16 + i[None, :] + 0.5) * dx + 0 * i[:, None]
+        Y = (bc[k, 1] * 16 + i[:, None] + 0.5) * dx + 0 * i[None, :]
+        rho, u, v, p = f(X, Y)
+        E = p / 0.4 + 0.5 * rho * (u * u + v * v)
+        for var, val in enumerate([rho, E, rho * u, rho * v, 1.0, rho, 0.0, 0.0]):
+            U[k, var] = np.broadcast_to(val, (16, 16)).ravel()
+    return U.ravel()
+
+
+def totals(o, cfg, U):
+    lv = o.block_levels()
+    V = U.reshape(-1, 8, 256)
+    vol = (cfg.dx / (2.0 ** lv)) ** 2
+    return np.array([(V[:, v].sum(axis=1) * vol).sum() for v in range(8)])
+
+
+def test_layout(g5, orc):
+    o = orc.Oracle(periodic_cfg(g5), refined=REFINED)
+    lv, bc = o.block_levels(), o.block_coords()[: o.nowned]
+    assert o.nowned == 13 + 12 and list(lv[:13]) == [0] * 13 and list(lv[13:]) == [1] * 12
+    coarse = [tuple(c[:2]) for c in bc[:13]]
+    assert coarse == sorted(coarse, key=lambda c: (c[1], c[0])) and (1, 1) not in coarse
+    fine = [tuple(c[:2]) for c in bc[13:]]
+    assert fine == sorted(fine, key=lambda c: (c[1], c[0])) and (2, 2) in fine and (4, 2) in fine
+
+
+@pytest.mark.parametrize("prolongation", [0, 1])
+def test_restriction_and_prolongation_on_a_linear_field(g5, orc, prolongation):
+    cfg = periodic_cfg(g5)
+    o = orc.Oracle(cfg, refined=REFINED, prolongation=prolongation)
+    # every conservative variable linear in x and y (the restriction and the interior prolongation
+    # stencils stay away from the periodic wrap)
+    a, bx, by = np.array([2.0, 5.0, 0.3, -0.2, 1.0, 2.0, 0.0, 0.0]), np.array([0.5, 1.0, 0.25, 0.5, 0, 0.5, 0, 0]), \
+        np.array([-0.25, 0.75, -0.5, 0.125, 0, -0.25, 0, 0])
+    lv, bc = o.block_levels(), o.block_coords()[: o.nowned]
+    U = np.zeros((o.nowned, 8, 256))
+    i = np.arange(16)
+    for k in range(o.nowned):
+        dx = cfg.dx / 2 ** lv[k]
+        X = ((bc[k, 0] * 16 + i[None, :] + 0.5) * dx + 0 * i[:, None]).ravel()
+        Y = ((bc[k, 1] * 16 + i[:, None] + 0.5) * dx + 0 * i[None, :]).ravel()
+        for v in range(8):
+            U[k, v] = a[v] + bx[v] * X + by[v] * Y
+    o.set_state(U.ravel())
+    dx = cfg.dx
+    # restriction: coarse cell (20, 20) lies in refined block (1, 1): the linear field at its centre
+    u = o.cons_level(0, (20, 20))
+    x, y = 20.5 * dx, 20.5 * dx
+    ref = a + bx * x + by * y
+    for v in (1, 2, 3, 4, 5):
+        assert u[v] == pytest.approx(ref[v], rel=1e-14, abs=1e-14), v
+    assert u[6] == 1.0 - u[4] and u[0] == u[5] + u[7]
+    # prolongation: fine cell (30, 40) lies in coarse leaf (0, 1) (coarse cell (15, 20))
+    u = o.cons_level(1, (30, 40))
+    xf, yf = 30.5 * dx / 2, 40.5 * dx / 2
+    xc, yc = 15.5 * dx, 20.5 * dx
+    ref = a + bx * (xf if prolongation else xc) + by * (yf if prolongation else yc)
+    for v in (1, 2, 3, 5):
+        assert u[v] == pytest.approx(ref[v], rel=1e-14, abs=1e-14), v
+    assert u[0] == u[5] + u[7]
+
+
+@pytest.mark.parametrize("prolongation", [0, 1])
+def test_uniform_state_is_preserved(g5, orc, prolongation):
+    cfg = periodic_cfg(g5)
+    o = orc.Oracle(cfg, refined=REFINED, prolongation=prolongation)
+    U0 = sample(o, cfg, lambda x, y: (1.0 + 0 * x, 0.5 + 0 * x, -0.25 + 0 * x, 1.0 + 0 * x))
+    o.set_state(U0)
+    o.run(5)
+    assert np.array_equal(o.get_state(), U0)
+
+
+@pytest.mark.parametrize("prolongation", [0, 1])
+def test_conservation_across_levels(g5, orc, prolongation):
+    cfg = periodic_cfg(g5)
+    o = orc.Oracle(cfg, refined=REFINED, prolongation=prolongation)
+    U0 = sample(o, cfg, lambda x, y: (1.0 + 0.5 * np.exp(-((x - 0.3) ** 2 + (y - 0.4) ** 2) / 0.01), 1.0 + 0 * x,
+                                      0.5 + 0 * x, 1.0 + 0.2 * np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / 0.02)))
+    o.set_state(U0)
+    T0 = totals(o, cfg, U0)
+    _, n = o.run(40)
+    U = o.get_state()
+    T1 = totals(o, cfg, U)
+    assert n == 40 and np.all(np.isfinite(U))
+    # mass (ID), energy and momentum: telescoping flux sums, round-off only
+    for v in (0, 1, 2, 3):
+        assert abs(T1[v] - T0[v]) <= 2e-15 * abs(T0[v]), (v, T0[v], T1[v])
+    # the pulse has crossed into and out of the fine region
+    assert np.abs(U - U0).max() > 1e-3
