@@ -167,6 +167,8 @@ def lib():
             "g5_version": (ctypes.c_char_p, []),
             "g5_last_error": (ctypes.c_char_p, [vp]),
             "g5_create": (ctypes.c_int, [P(G5Config), P(vp)]),
+            "g5_create_two_level": (ctypes.c_int, [P(G5Config), P(i32), i32, i32, P(vp)]),
+            "g5_block_layout": (ctypes.c_int, [vp, P(i32), sz]),
             "g5_destroy": (ctypes.c_int, [vp]),
             "g5_shape": (ctypes.c_int, [vp, P(i32), P(i32), P(i32)]),
             "g5_mem_bytes": (ctypes.c_int, [vp, P(ctypes.c_uint64)]),
@@ -248,6 +250,7 @@ def exported_symbols():
         "g5_pool_properties", "g5_pool_prepare", "g5_host_alloc", "g5_host_free", "g5_pool_set_clock", "g5_pool_step_async", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
         "g5_pool_admit_spec", "g5_serve", "g5_serve_run", "g5_spec_state_size", "g5_pool_failure",
         "g5_set_block_order", "g5_do_time_step_buffers", "g5_compute_dt_buffer", "g5_step_noalloc",
+        "g5_create_two_level", "g5_block_layout",
     ]
 
 
@@ -376,13 +379,25 @@ def rankine_hugoniot(gamma, pinf, rho1, p1, mach=0.0, p2=0.0):
     return tuple(o.value for o in out)
 
 
-class Context:
-    """One solve on the GPU (the GodunovImplemBase surface, scheme/GodunovImplemBase.h:59-198)."""
+PROLONG_SIMPLE_COPY, PROLONG_EXTRAPOLATE_LINEAR_MINMOD = 0, 1
 
-    def __init__(self, cfg):
+
+class Context:
+    """One solve on the GPU (the GodunovImplemBase surface, scheme/GodunovImplemBase.h:59-198).
+
+    refined: (n, 2) coarse block coordinates refined into 2 x 2 blocks of half the cell length (a
+    2-D two-level static mesh, FP64 or FP64_FAST; see g5_create_two_level), with the given
+    prolongation; None: one level."""
+
+    def __init__(self, cfg, refined=None, prolongation=PROLONG_SIMPLE_COPY):
         self.cfg = cfg
         self._h = ctypes.c_void_p()
-        rc = lib().g5_create(ctypes.byref(cfg), ctypes.byref(self._h))
+        if refined is None:
+            rc = lib().g5_create(ctypes.byref(cfg), ctypes.byref(self._h))
+        else:
+            r = np.ascontiguousarray(refined, dtype=np.int32).reshape(-1, 2)
+            rc = lib().g5_create_two_level(ctypes.byref(cfg), r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                           r.shape[0], prolongation, ctypes.byref(self._h))
         if rc != G5_OK:
             raise G5Error(rc, lib().g5_last_error(None).decode())
         nb, nv, nc = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
@@ -432,6 +447,12 @@ class Context:
         """coords: (n_blocks, 3) brick coordinates of the caller's blocks, in the caller's order"""
         c = np.ascontiguousarray(coords, dtype=np.int32).reshape(-1, 3)
         self._check(lib().g5_set_block_order(self._h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), c.shape[0]))
+
+    def block_layout(self):
+        """(n_blocks, 4) int32: x, y, z, level of each block of the state, in order"""
+        c = np.empty((self.nblocks, 4), dtype=np.int32)
+        self._check(lib().g5_block_layout(self._h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), c.size))
+        return c
 
     def device_ptr(self):
         p = ctypes.c_void_p()

@@ -27,7 +27,7 @@ k_inv_dt(double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const
   if (blk >= nblocks)
     return;
   const BlockDesc bd = blocks[blk];
-  const SolveDesc S = solves[bd.solve];
+  const SolveDesc S = at_level(solves[bd.solve], bd.level);
   double const *  U = cur_buf(buf0, buf1, ctl[bd.solve].parity);
   constexpr int   NC = DIM == 2 ? 256 : 4096;
   constexpr int   NV = DIM == 2 ? 8 : 9;

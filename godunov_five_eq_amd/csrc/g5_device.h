@@ -1302,6 +1302,20 @@ fix_volume_fractions(Cons & u)
   u.d = sum_alpha_rho;
 }
 
+// the solve's descriptor as a block of the given level sees it: the fine blocks (level 1) of a
+// two-level solve have half the cell length (compute_cell_length(level, bx), halved per level;
+// both scalings are exact)
+G5D SolveDesc
+at_level(SolveDesc S, int level)
+{
+  if (level != 0)
+  {
+    S.dx *= 0.5;
+    S.inv_dx_p2 *= 2.0;
+  }
+  return S;
+}
+
 // scheme/ComputeDtFiveEqFunctor.cpp:85-134 compute_cfl (one cell's contribution; FAST: with the
 // reassociated conversion)
 template <int DIM, bool FAST = false>

@@ -20,6 +20,14 @@ void launch_step2d_fast(StepArgs const & A, bool thinc, bool hancock, bool ring_
 void launch_step3d_fast(StepArgs const & A, bool thinc, bool hancock, bool zsplit, bool inlet, bool mp,
                         hipStream_t stream);
 
+// the 2-D step over a launch list with blocks of two-level solves (FP64 or reassociated FP64): the
+// first n_first blocks (the fine blocks that border coarse blocks) in one launch, the rest of the
+// list in a second one
+void launch_step2d_two_level(StepArgs const & A, int n_first, bool thinc, bool hancock, int precision, bool mp,
+                             hipStream_t stream);
+void launch_step2d_two_level_fast(StepArgs const & A, int n_first, bool thinc, bool hancock, bool mp,
+                                  hipStream_t stream);
+
 // max over the listed blocks of the CFL rate of the current state into ctl[solve].invdt_bits
 // (the rate of the path of `precision`, G5_PREC_*)
 void launch_inv_dt(int dim, int precision, double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves,

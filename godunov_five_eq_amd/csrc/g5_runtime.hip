@@ -36,7 +36,18 @@ struct SolveRec
   g5_config            cfg{};
   int32_t              nblk = 0;
   int32_t              table_base = 0;
+  int32_t              table_len = 0; // entries of its slot-table segment
   std::vector<int32_t> slots;
+  // two-level static mesh (2-D): (x, y, level) of every block in the caller's block order --
+  // the coarse leaves lexicographic, then the fine blocks lexicographic over the fine grid --
+  // with slots[k] the slot of block k; border[k]: a fine block with a coarse neighbour.  The
+  // slot-table segment holds the level-0 table (nb[0] x nb[1], -1 where refined) then the
+  // level-1 table (2nb[0] x 2nb[1], -1 outside the refined blocks).  Empty: one level, whose
+  // table is slots itself (indexed by the brick's lexicographic block index).
+  std::vector<int32_t> two_level;
+  std::vector<char>    border;
+  std::vector<int32_t> table_host;
+  int32_t              prolong = 0;
 };
 
 // first-fit allocator of contiguous index ranges (slot-table segments)
@@ -112,6 +123,12 @@ struct g5_pool
   bool        list_dirty = true;
   bool        any_inlet = false; // a live solve has an inflow border: the 3-D step kernels resolve it
   bool        any_mp = false;    // a live solve has the MILLER_PUCKETT source term
+  // a live solve has two levels: the launch list starts with the nlist_first fine blocks that
+  // border coarse blocks, and the step runs as two launches over the interface records d_iflux
+  bool        any_two_level = false;
+  int32_t     nlist_first = 0;
+  double *    d_iflux = nullptr; // IFLUX per slot: [4 sides][16 cells][8 values]
+  int64_t     cap_iflux = 0;
   SolveCtl *  h_ctl = nullptr; // pinned mirror of the control records
   int32_t     cap_hctl = 0;
 
@@ -322,10 +339,20 @@ admit_device(g5_pool * p, int32_t sid, double tend, int32_t max_steps)
   p->cap_table = static_cast<int32_t>(ct);
   SolveDesc sd;
   make_solve_desc(r.cfg, r.table_base, sd);
+  std::vector<int32_t> const & tab = r.two_level.empty() ? r.slots : r.table_host;
+  if (!r.two_level.empty())
+  {
+    sd.base1 = r.table_base + r.cfg.nb[0] * r.cfg.nb[1];
+    sd.prolong = r.prolong;
+    // interface records, indexed by slot
+    rc = grow(p, p->d_iflux, p->cap_iflux, p->nslots * kIfluxDoubles, p->stream);
+    if (rc != G5_OK)
+      return rc;
+  }
   SolveCtl c{};
   c.tend = tend;
   c.max_steps = max_steps;
-  G5_HIP(p->err, hipMemcpyAsync(p->d_slots + r.table_base, r.slots.data(), r.nblk * sizeof(int32_t),
+  G5_HIP(p->err, hipMemcpyAsync(p->d_slots + r.table_base, tab.data(), r.table_len * sizeof(int32_t),
                                 hipMemcpyHostToDevice, p->stream));
   G5_HIP(p->err, hipMemcpyAsync(p->d_solves + sid, &sd, sizeof(SolveDesc), hipMemcpyHostToDevice, p->stream));
   G5_HIP(p->err, hipMemcpyAsync(p->d_ctl + sid, &c, sizeof(SolveCtl), hipMemcpyHostToDevice, p->stream));
@@ -333,13 +360,86 @@ admit_device(g5_pool * p, int32_t sid, double tend, int32_t max_steps)
   return G5_OK;
 }
 
+// A two-level static mesh over the brick of cfg: the coarse blocks at refined[2k], refined[2k+1]
+// are each replaced by 2 x 2 blocks of half the cell length.  Fills r.two_level (block order:
+// coarse leaves lexicographic, then the fine blocks lexicographic over the fine grid) and
+// r.border; r.nblk is the block count.
 int
-pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, int32_t * id)
+layout_two_level(g5_config const & cfg, std::vector<int32_t> const & refined, SolveRec & r, std::string & msg)
+{
+  const int nx = cfg.nb[0], ny = cfg.nb[1];
+  if (cfg.dim != 2)
+    return msg = "two-level meshes are 2-D", G5_ERR_UNSUPPORTED;
+  if (cfg.precision == G5_PREC_MIXED)
+    return msg = "two-level meshes exchange FP64 face records: FP64 or FP64_FAST precision", G5_ERR_UNSUPPORTED;
+  if (refined.empty() or refined.size() % 2 != 0)
+    return msg = "no refined blocks", G5_ERR_INVALID;
+  std::vector<char> ref(static_cast<size_t>(nx) * ny, 0);
+  for (size_t k = 0; k < refined.size(); k += 2)
+  {
+    const int x = refined[k], y = refined[k + 1];
+    if (x < 1 or x > nx - 2 or y < 1 or y > ny - 2)
+      return msg = "refined blocks must lie at least one block inside the brick's border", G5_ERR_INVALID;
+    char & f = ref[x + static_cast<size_t>(nx) * y];
+    if (f)
+      return msg = "refined block repeated", G5_ERR_INVALID;
+    f = 1;
+  }
+  r.two_level.clear();
+  r.border.clear();
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x)
+      if (!ref[x + static_cast<size_t>(nx) * y])
+      {
+        r.two_level.insert(r.two_level.end(), { x, y, 0 });
+        r.border.push_back(0);
+      }
+  auto fine = [&](int x, int y) { return ref[(x >> 1) + static_cast<size_t>(nx) * (y >> 1)] != 0; };
+  for (int y = 0; y < 2 * ny; ++y)
+    for (int x = 0; x < 2 * nx; ++x)
+      if (fine(x, y))
+      {
+        r.two_level.insert(r.two_level.end(), { x, y, 1 });
+        r.border.push_back(!fine(x - 1, y) or !fine(x + 1, y) or !fine(x, y - 1) or !fine(x, y + 1));
+      }
+  r.nblk = static_cast<int32_t>(r.border.size());
+  return G5_OK;
+}
+
+// slot-table segment of a two-level solve from its slots
+void
+tables_two_level(SolveRec & r)
+{
+  const int nx = r.cfg.nb[0], ny = r.cfg.nb[1];
+  r.table_host.assign(static_cast<size_t>(5) * nx * ny, -1);
+  for (int32_t k = 0; k < r.nblk; ++k)
+  {
+    const int x = r.two_level[3 * k], y = r.two_level[3 * k + 1];
+    if (r.two_level[3 * k + 2] == 0)
+      r.table_host[x + static_cast<size_t>(nx) * y] = r.slots[k];
+    else
+      r.table_host[static_cast<size_t>(nx) * ny + x + static_cast<size_t>(2 * nx) * y] = r.slots[k];
+  }
+}
+
+// admission of a solve; refined (two coordinates per refined coarse block), when given, makes it
+// a two-level solve with the given prolongation (0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD)
+int
+pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, int32_t * id,
+           std::vector<int32_t> const * refined = nullptr, int prolong = 0)
 {
   std::string msg;
   int         rc = validate(cfg, msg);
   if (rc != G5_OK)
     return fail(p->err, rc, msg);
+  SolveRec tl; // two-level layout
+  if (refined)
+  {
+    if (prolong != 0 and prolong != 1)
+      return fail(p->err, G5_ERR_INVALID, "prolongation must be 0 (SIMPLE_COPY) or 1 (EXTRAPOLATE_LINEAR_MINMOD)");
+    if ((rc = layout_two_level(cfg, *refined, tl, msg)) != G5_OK)
+      return fail(p->err, rc, msg);
+  }
   if (cfg.dim != p->dim or (cfg.thinc_enabled != 0) != (p->thinc != 0) or cfg.precision != p->precision)
     return fail(p->err, G5_ERR_INVALID, "solve does not match the pool (dim, THINC, precision)");
   for (auto const & r : p->solves) // the step graph applies one integrator to the whole launch list
@@ -347,7 +447,7 @@ pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, i
       return fail(p->err, G5_ERR_INVALID, "solve does not match the pool's integrator (one integrator per pool)");
   if (p->stage == nullptr and cfg.integrator == G5_TI_RK2_SSP)
     return fail(p->err, G5_ERR_UNSUPPORTED, "RK2_SSP needs the stage buffer (a single-solve context or g5_pool_create_rk2)");
-  const int64_t nblk = nblocks_of(cfg);
+  const int64_t nblk = refined ? tl.nblk : nblocks_of(cfg);
   if (nblk > p->nslots)
     return fail(p->err, G5_ERR_TOO_LARGE,
                 "the solve has " + std::to_string(nblk) + " blocks, more than the pool's whole budget (" +
@@ -380,7 +480,17 @@ pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, i
     r.slots[k] = p->free_slots.back();
     p->free_slots.pop_back();
   }
-  r.table_base = p->table.take(r.nblk);
+  r.table_len = r.nblk;
+  if (refined)
+  {
+    r.two_level = std::move(tl.two_level);
+    r.border = std::move(tl.border);
+    r.prolong = prolong;
+    tables_two_level(r);
+    r.table_len = static_cast<int32_t>(r.table_host.size());
+    p->any_two_level = true;
+  }
+  r.table_base = p->table.take(r.table_len);
   rc = admit_device(p, sid, tend, max_steps);
   if (rc != G5_OK)
   {
@@ -388,7 +498,7 @@ pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, i
     SolveRec & q = p->solves[sid];
     for (auto it = q.slots.rbegin(); it != q.slots.rend(); ++it)
       p->free_slots.push_back(*it);
-    p->table.give(q.table_base, q.nblk);
+    p->table.give(q.table_base, q.table_len);
     q = SolveRec();
     p->free_ids.push_back(sid);
     return rc;
@@ -407,7 +517,7 @@ pool_release(g5_pool * p, int32_t id)
   SolveRec & r = p->solves[id];
   for (auto it = r.slots.rbegin(); it != r.slots.rend(); ++it)
     p->free_slots.push_back(*it); // the next admission pops them in ascending order
-  p->table.give(r.table_base, r.nblk);
+  p->table.give(r.table_base, r.table_len);
   r = SolveRec();
   p->free_ids.push_back(id);
   SolveCtl c{}; // inactive
@@ -417,11 +527,23 @@ pool_release(g5_pool * p, int32_t id)
   return G5_OK;
 }
 
+// part (two-level solves): 0 the fine blocks with a coarse neighbour, 1 the other blocks, 2 all
 void
-append_blocks(g5_pool const * p, int32_t sid, std::vector<BlockDesc> & out)
+append_blocks(g5_pool const * p, int32_t sid, std::vector<BlockDesc> & out, int part = 2)
 {
   SolveRec const & r = p->solves[sid];
-  const int        nx = r.cfg.nb[0], ny = r.cfg.nb[1], nz = r.cfg.nb[2];
+  if (!r.two_level.empty())
+  {
+    for (int32_t k = 0; k < r.nblk; ++k)
+      if (part == 2 or (part == 0) == (r.border[k] != 0))
+        out.push_back(BlockDesc{ sid, r.slots[k], static_cast<int16_t>(r.two_level[3 * k]),
+                                 static_cast<int16_t>(r.two_level[3 * k + 1]), 0,
+                                 static_cast<int16_t>(r.two_level[3 * k + 2]) });
+    return;
+  }
+  if (part == 0)
+    return;
+  const int nx = r.cfg.nb[0], ny = r.cfg.nb[1], nz = r.cfg.nb[2];
   // 2-D: 4x4-block tiles in x-fastest tile order, so that a block's y neighbours (whose halo rows
   // it reads) run a few workgroups apart (measured 2.6% faster on a 4096-solve batch, although the
   // fetched HBM volume did not drop); 3-D: plain brick order
@@ -444,16 +566,25 @@ rebuild_list(g5_pool * p)
   if (!p->list_dirty)
     return G5_OK;
   std::vector<BlockDesc> list;
-  bool                   inlet = false, mp = false;
+  bool                   inlet = false, mp = false, two = false;
+  for (auto const & r : p->solves)
+    two = two or (r.live and !r.two_level.empty());
+  // with two-level solves: the fine blocks with a coarse neighbour first
+  if (two)
+    for (int32_t s = 0; s < static_cast<int32_t>(p->solves.size()); ++s)
+      if (p->solves[s].live)
+        append_blocks(p, s, list, 0);
+  p->nlist_first = static_cast<int32_t>(list.size());
   for (int32_t s = 0; s < static_cast<int32_t>(p->solves.size()); ++s)
     if (p->solves[s].live)
     {
-      append_blocks(p, s, list);
+      append_blocks(p, s, list, two ? 1 : 2);
       inlet = inlet or p->solves[s].cfg.inlet != G5_INLET_NONE;
       mp = mp or p->solves[s].cfg.source_term == G5_SOURCE_MILLER_PUCKETT;
     }
   p->any_inlet = inlet;
   p->any_mp = mp;
+  p->any_two_level = two;
   int64_t cb = p->cap_blocks;
   int     rc = grow(p, p->d_blocks, cb, static_cast<int64_t>(list.size()), p->stream);
   if (rc != G5_OK)
@@ -486,6 +617,7 @@ step_args(g5_pool * p, BlockDesc const * blocks, int32_t n, double dt_fixed)
   A.wg_per_xcd = (p->dim == 2 and p->launch.wg_per_cu > 0) ? p->launch.wg_per_cu * std::max(1, p->cus / 8) : 0;
   A.remap = p->launch.remap;
   A.dt_fixed = dt_fixed;
+  A.iflux = p->d_iflux;
   return A;
 }
 
@@ -494,7 +626,9 @@ launch_step(g5_pool * p, StepArgs const & A, bool hancock)
 {
   if (A.nblocks <= 0)
     return;
-  if (p->dim == 2)
+  if (p->any_two_level) // every caller passes the whole launch list
+    launch_step2d_two_level(A, p->nlist_first, p->thinc != 0, hancock, p->precision, p->any_mp, p->stream);
+  else if (p->dim == 2)
     launch_step2d(A, p->thinc != 0, hancock, p->precision, p->launch.ring_wave != 0, p->any_mp, p->stream);
   else
     launch_step3d(A, p->thinc != 0, hancock, p->precision, p->launch.zsplit != 0, p->any_inlet, p->any_mp, p->stream);
@@ -607,7 +741,8 @@ autotune(g5_pool * p, bool hancock)
 int
 ensure_tuned(g5_pool * p, bool hancock)
 {
-  if (p->launch_fixed or !p->autotune or p->nlist == 0 or p->tuned_class == size_class(p->nlist))
+  if (p->launch_fixed or !p->autotune or p->nlist == 0 or p->tuned_class == size_class(p->nlist) or
+      p->any_two_level) // the two-level step has one launch form
     return G5_OK;
   const LaunchCfg before = p->launch;
   const int       rc = autotune(p, hancock);
@@ -990,6 +1125,8 @@ pool_init_ic(g5_pool * p, int32_t id, IcDesc const & ic)
 {
   if (id < 0 or id >= static_cast<int32_t>(p->solves.size()) or !p->solves[id].live)
     return fail(p->err, G5_ERR_INVALID, "no such solve");
+  if (!p->solves[id].two_level.empty())
+    return fail(p->err, G5_ERR_UNSUPPORTED, "device initial conditions fill one-level solves (upload the state)");
   IcDesc * d_ic = nullptr;
   G5_HIP(p->err, hipMalloc(&d_ic, sizeof(IcDesc)));
   G5_HIP(p->err, hipMemcpyAsync(d_ic, &ic, sizeof(IcDesc), hipMemcpyHostToDevice, p->stream));
@@ -1136,6 +1273,7 @@ g5_pool_destroy(g5_pool * p)
   hipFree(p->d_start_list);
   hipFree(p->d_start_ids);
   hipFree(p->d_xfer);
+  hipFree(p->d_iflux);
   if (p->h_ctl)
     hipHostFree(p->h_ctl);
   if (p->stream)
@@ -1464,8 +1602,8 @@ g5_pool_set_autotune(g5_pool * p, int32_t enable)
 // =============================================================================================
 // single solve: a pool of one whose slots are the lexicographic block order
 // =============================================================================================
-int
-g5_create(const g5_config * cfg, g5_ctx ** out)
+static int
+create_ctx(const g5_config * cfg, std::vector<int32_t> const * refined, int prolong, g5_ctx ** out)
 {
   if (not cfg or not out)
     return fail(g_create_error, G5_ERR_INVALID, "null argument");
@@ -1478,7 +1616,18 @@ g5_create(const g5_config * cfg, g5_ctx ** out)
   ctx->cfg = *cfg;
   for (int d = cfg->dim; d < 3; ++d)
     ctx->cfg.nb[d] = 1;
-  const int64_t  nblk = nblocks_of(ctx->cfg);
+  int64_t nblk = nblocks_of(ctx->cfg);
+  if (refined)
+  {
+    SolveRec tl;
+    rc = layout_two_level(ctx->cfg, *refined, tl, msg);
+    if (rc != G5_OK)
+    {
+      delete ctx;
+      return fail(g_create_error, rc, msg);
+    }
+    nblk = tl.nblk;
+  }
   const int      bd = (cfg->dim == 2 ? 8 * 256 : 9 * 4096);
   const bool     rk2 = cfg->integrator == G5_TI_RK2_SSP;
   const uint64_t budget = static_cast<uint64_t>(nblk) * bd * sizeof(double) * (rk2 ? 3 : 2);
@@ -1488,7 +1637,7 @@ g5_create(const g5_config * cfg, g5_ctx ** out)
     delete ctx;
     return rc;
   }
-  rc = pool_admit(ctx->pool, ctx->cfg, 1e300, 0x7fffffff, &ctx->id);
+  rc = pool_admit(ctx->pool, ctx->cfg, 1e300, 0x7fffffff, &ctx->id, refined, prolong);
   if (rc != G5_OK)
   {
     g_create_error = ctx->pool->err;
@@ -1519,6 +1668,60 @@ g5_create(const g5_config * cfg, g5_ctx ** out)
 }
 
 int
+g5_create(const g5_config * cfg, g5_ctx ** out)
+{
+  return create_ctx(cfg, nullptr, 0, out);
+}
+
+int
+g5_create_two_level(const g5_config * cfg, const int32_t * refined_coords, int32_t n_refined, int32_t prolongation,
+                    g5_ctx ** out)
+{
+  if (not refined_coords or n_refined < 1)
+    return fail(g_create_error, G5_ERR_INVALID, "no refined blocks");
+  if (prolongation != 0 and prolongation != 1)
+    return fail(g_create_error, G5_ERR_INVALID, "prolongation must be 0 (SIMPLE_COPY) or 1 (EXTRAPOLATE_LINEAR_MINMOD)");
+  const std::vector<int32_t> refined(refined_coords, refined_coords + 2 * static_cast<size_t>(n_refined));
+  return create_ctx(cfg, &refined, prolongation, out);
+}
+
+int
+g5_block_layout(const g5_ctx * ctx, int32_t * coords, size_t n)
+{
+  if (not ctx or not coords)
+    return G5_ERR_INVALID;
+  SolveRec const & r = ctx->pool->solves[ctx->id];
+  if (n != 4 * static_cast<size_t>(r.nblk))
+    return G5_ERR_INVALID;
+  for (int32_t k = 0; k < r.nblk; ++k)
+  {
+    int32_t * c = coords + 4 * static_cast<size_t>(k);
+    if (!r.two_level.empty())
+    {
+      c[0] = r.two_level[3 * k];
+      c[1] = r.two_level[3 * k + 1];
+      c[2] = 0;
+      c[3] = r.two_level[3 * k + 2];
+    }
+    else if (!ctx->block_order.empty())
+    {
+      for (int d = 0; d < 3; ++d)
+        c[d] = ctx->block_order[3 * static_cast<size_t>(k) + d];
+      c[3] = 0;
+    }
+    else
+    {
+      const int nx = ctx->cfg.nb[0], ny = ctx->cfg.nb[1];
+      c[0] = k % nx;
+      c[1] = (k / nx) % ny;
+      c[2] = k / (nx * ny);
+      c[3] = 0;
+    }
+  }
+  return G5_OK;
+}
+
+int
 g5_destroy(g5_ctx * ctx)
 {
   if (not ctx)
@@ -1541,7 +1744,7 @@ g5_shape(const g5_ctx * ctx, int32_t * n_blocks, int32_t * nvar, int32_t * cells
   if (not ctx)
     return G5_ERR_INVALID;
   if (n_blocks)
-    *n_blocks = static_cast<int32_t>(nblocks_of(ctx->cfg));
+    *n_blocks = ctx->pool->solves[ctx->id].nblk;
   if (nvar)
     *nvar = ctx->pool->nvar;
   if (cells_per_block)
@@ -1562,7 +1765,8 @@ g5_mem_bytes(const g5_ctx * ctx, uint64_t * bytes)
   if (ctx->d_ring)
     b += ctx->outside.size() / 3 * p->block_doubles * sizeof(double);
   if (ctx->d_staging)
-    b += static_cast<uint64_t>(nblocks_of(ctx->cfg)) * p->block_doubles * sizeof(double);
+    b += static_cast<uint64_t>(p->solves[ctx->id].nblk) * p->block_doubles * sizeof(double);
+  b += p->cap_iflux * sizeof(double);
   *bytes = b;
   return G5_OK;
 }
@@ -1712,6 +1916,8 @@ g5_set_block_order(g5_ctx * ctx, const int32_t * coords, int32_t n_blocks)
   g5_pool *       p = ctx->pool;
   SolveRec &      r = p->solves[ctx->id];
   const int       nx = ctx->cfg.nb[0], ny = ctx->cfg.nb[1], nz = ctx->cfg.nb[2];
+  if (!r.two_level.empty())
+    return fail(ctx->err, G5_ERR_UNSUPPORTED, "a two-level context keeps its own block order (g5_block_layout)");
   if (n_blocks != r.nblk)
     return fail(ctx->err, G5_ERR_INVALID, "n_blocks must equal the number of owned blocks");
   // caller block k goes to slot k: slots[brick index of block k] = k
@@ -2017,6 +2223,7 @@ g5_step_noalloc(const g5_config * cfg, const double * U_dev, double * U2_dev, do
   A.wg_per_xcd = 0;
   A.remap = 1;
   A.dt_fixed = dt;
+  A.iflux = nullptr; // one level
   const bool hancock = c.integrator == G5_TI_HANCOCK;
   const bool mp = c.source_term == G5_SOURCE_MILLER_PUCKETT;
   if (c.dim == 2)

@@ -18,4 +18,15 @@ launch_step2d(StepArgs const & A, bool thinc, bool hancock, int precision, bool 
        : launch2<double, false, false>(A, thinc, hancock, ring_wave, stream);
 }
 
+void
+launch_step2d_two_level(StepArgs const & A, int n_first, bool thinc, bool hancock, int precision, bool mp,
+                        hipStream_t stream)
+{
+  if (precision == G5_PREC_FP64_FAST)
+    launch_step2d_two_level_fast(A, n_first, thinc, hancock, mp, stream);
+  else
+    mp ? launch2_two_level<true, false>(A, n_first, thinc, hancock, stream)
+       : launch2_two_level<false, false>(A, n_first, thinc, hancock, stream);
+}
+
 } // namespace g5

@@ -253,6 +253,163 @@ predict_faces2(SolveDesc const & S, T const * sQ, int gi, int gj, T dtdx, bool a
   }
 }
 
+// ---- two-level static mesh ----------------------------------------------------------------
+// Level 0 is the brick of nb[0] x nb[1] blocks, some of them refined into 2 x 2 blocks of half
+// the cell length (level 1), at least one block away from the brick's border.  The solve's slot
+// tables: level 0 at S.base (-1 where refined), level 1 at S.base1 over the 2nb[0] x 2nb[1] fine
+// grid (-1 outside the refined blocks).  A cell is named by its global index at its level.
+G5D int
+slot_l0(SolveDesc const & S, int32_t const * __restrict__ slots, int x, int y)
+{
+  return slots[S.base + x + S.nb[0] * y];
+}
+G5D int
+slot_l1(SolveDesc const & S, int32_t const * __restrict__ slots, int x, int y)
+{
+  return slots[S.base1 + x + 2 * S.nb[0] * y];
+}
+G5D Cons
+cell_of(double const * __restrict__ U, int slot, int ci, int cj)
+{
+  return load_cons<2, B2>(U, static_cast<int64_t>(slot) * 8 * 256 + ci + B2 * cj, -1);
+}
+
+// coarse cell (gx, gy) inside a refined block: the average of its four fine cells, summed x
+// fastest then divided by 4, for E, rho u, rho v, alpha_rho 0/1, alpha 0; alpha 1 = 1 - alpha 0
+// and rho = alpha_rho 0 + alpha_rho 1 (get_conservative_vars_restriction,
+// scheme/ConvertToPrimitivesVariablesFunctor.cpp:214-250, with the siblings' average of the core)
+G5D Cons
+restrict_cell(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int gx, int gy)
+{
+  Cons sum{};
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+  {
+    const int  fx = 2 * gx + (k & 1), fy = 2 * gy + (k >> 1);
+    const Cons f = cell_of(U, slot_l1(S, slots, fx >> 4, fy >> 4), fx & 15, fy & 15);
+    sum.e = k == 0 ? f.e : sum.e + f.e;
+    sum.mu = k == 0 ? f.mu : sum.mu + f.mu;
+    sum.mv = k == 0 ? f.mv : sum.mv + f.mv;
+    sum.ar0 = k == 0 ? f.ar0 : sum.ar0 + f.ar0;
+    sum.ar1 = k == 0 ? f.ar1 : sum.ar1 + f.ar1;
+    sum.a0 = k == 0 ? f.a0 : sum.a0 + f.a0;
+  }
+  Cons u;
+  u.e = sum.e / 4.0;
+  u.mu = sum.mu / 4.0;
+  u.mv = sum.mv / 4.0;
+  u.mw = 0.0;
+  u.ar0 = sum.ar0 / 4.0;
+  u.ar1 = sum.ar1 / 4.0;
+  u.a0 = sum.a0 / 4.0;
+  u.a1 = 1.0 - u.a0;
+  u.d = u.ar0 + u.ar1;
+  return u;
+}
+
+// coarse cell (gx, gy) of the parent's 3x3-block neighbourhood (all inside the brick): a coarse
+// leaf's cell, or the restriction where the block is refined
+G5D Cons
+coarse_cell(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int gx, int gy)
+{
+  const int sl = slot_l0(S, slots, gx >> 4, gy >> 4);
+  return sl >= 0 ? cell_of(U, sl, gx & 15, gy & 15) : restrict_cell(S, slots, U, gx, gy);
+}
+
+// fine cell (gx, gy) outside the fine blocks: its coarse parent copied (S.prolong 0, SIMPLE_COPY)
+// or extrapolated to the fine cell's side with minmod slopes of the conservative variables
+// (S.prolong 1, EXTRAPOLATE_LINEAR_MINMOD: linear_extrapolate_using_limited_slopes,
+// scheme/ConvertToPrimitivesVariablesFunctor.cpp:553-770, fill_ghosts :889-910)
+G5D Cons
+prolong_cell(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int gx, int gy)
+{
+  const int  cx = gx >> 1, cy = gy >> 1; // fine cells of the ghost ring are at positive indices
+  const Cons c = coarse_cell(S, slots, U, cx, cy);
+  if (S.prolong == 0)
+    return c;
+  const double hx = (gx & 1) ? 0.25 : -0.25, hy = (gy & 1) ? 0.25 : -0.25;
+  const Cons   xl = coarse_cell(S, slots, U, cx - 1, cy), xr = coarse_cell(S, slots, U, cx + 1, cy);
+  const Cons   yl = coarse_cell(S, slots, U, cx, cy - 1), yr = coarse_cell(S, slots, U, cx, cy + 1);
+  Cons         u;
+  u.ar0 = c.ar0 + hx * slope1<double>(c.ar0, xr.ar0, xl.ar0, 1.0) + hy * slope1<double>(c.ar0, yr.ar0, yl.ar0, 1.0);
+  u.ar1 = c.ar1 + hx * slope1<double>(c.ar1, xr.ar1, xl.ar1, 1.0) + hy * slope1<double>(c.ar1, yr.ar1, yl.ar1, 1.0);
+  u.a0 = c.a0 + hx * slope1<double>(c.a0, xr.a0, xl.a0, 1.0) + hy * slope1<double>(c.a0, yr.a0, yl.a0, 1.0);
+  u.e = c.e + hx * slope1<double>(c.e, xr.e, xl.e, 1.0) + hy * slope1<double>(c.e, yr.e, yl.e, 1.0);
+  u.mu = c.mu + hx * slope1<double>(c.mu, xr.mu, xl.mu, 1.0) + hy * slope1<double>(c.mu, yr.mu, yl.mu, 1.0);
+  u.mv = c.mv + hx * slope1<double>(c.mv, xr.mv, xl.mv, 1.0) + hy * slope1<double>(c.mv, yr.mv, yl.mv, 1.0);
+  u.mw = 0.0;
+  u.d = u.ar0 + u.ar1;
+  u.a1 = 1.0 - u.a0;
+  return u;
+}
+
+// conservative state of ghost cell (gi, gj) of a block of a two-level solve (fill_ghosts,
+// scheme/ConvertToPrimitivesVariablesFunctor.cpp:884-918): a same-level neighbour's cell (border
+// conditions resolved as on one level), a restriction of a finer neighbour, or a prolongation of a
+// coarser one
+G5D Cons
+ghost_two_level(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U,
+                BlockDesc const & bd, int gi, int gj)
+{
+  const int dx = gi < 0 ? -1 : (gi >= B2 ? 1 : 0), dy = gj < 0 ? -1 : (gj >= B2 ? 1 : 0);
+  const int nx = bd.bx + dx, ny = bd.by + dy;
+  const int gx = bd.bx * B2 + gi, gy = bd.by * B2 + gj;
+  if (bd.level == 0)
+  {
+    if (nx >= 0 and nx < S.nb[0] and ny >= 0 and ny < S.nb[1] and slot_l0(S, slots, nx, ny) < 0)
+      return restrict_cell(S, slots, U, gx, gy);
+    int           neg;
+    const int64_t off = resolve_cell<2, B2>(S, slots, bd.bx, bd.by, 0, gi, gj, 0, neg);
+    Cons          u = load_cons<2, B2>(U, off, -1);
+    apply_border<2>(S, u, neg);
+    return u;
+  }
+  const int sl = slot_l1(S, slots, nx, ny); // fine blocks sit inside the fine grid's interior
+  return sl >= 0 ? cell_of(U, sl, gi - dx * B2, gj - dy * B2) : prolong_cell(S, slots, U, gx, gy);
+}
+
+// Face records across a level interface, per coarse slot: [side xmin, xmax, ymin, ymax][cell
+// 0..15][ar0, ar1, a0, a1, mu, mv, us, e].  Entry (side, j) of coarse block C holds, for C's
+// boundary cell j on that side, the sum of the records of the two fine faces that cover the
+// coarse face divided by 4 (the volume ratio): get_flux_from_fine_neighbor /
+// get_ustar_flux_from_fine_neighbor (scheme/ReadFluxesAndConservativeUpdateFunctor.h:256-307) and
+// the NEIGHBOR_IS_FINER branches of read_fluxes_and_update_2d (.cpp:137-199).  The records hold
+// flux * dt / dx_fine; u* enters the update as st * (u*_sum / 4) * alpha, the same bits as the
+// reference's st * u*_sum * alpha / 4 (scalings by 4 are exact).
+G5D int64_t
+iflux_at(int slot, int side, int cell)
+{
+  static_assert(kIfluxDoubles == 4 * 16 * 8, "interface record layout");
+  return ((static_cast<int64_t>(slot) * 4 + side) * 16 + cell) * 8;
+}
+G5D Flux
+iflux_load(double const * __restrict__ R, int64_t o)
+{
+  Flux F;
+  F.ar0 = R[o + 0];
+  F.ar1 = R[o + 1];
+  F.a0 = R[o + 2];
+  F.a1 = R[o + 3];
+  F.mu = R[o + 4];
+  F.mv = R[o + 5];
+  F.mw = 0.0;
+  F.us = R[o + 6];
+  F.e = R[o + 7];
+  return F;
+}
+G5D void
+iflux_store_sum(double * __restrict__ R, int64_t o, Flux const & a, Flux const & b)
+{
+  R[o + 0] = (a.ar0 + b.ar0) / 4.0;
+  R[o + 1] = (a.ar1 + b.ar1) / 4.0;
+  R[o + 2] = (a.a0 + b.a0) / 4.0;
+  R[o + 3] = (a.a1 + b.a1) / 4.0;
+  R[o + 4] = (a.mu + b.mu) / 4.0;
+  R[o + 5] = (a.mv + b.mv) / 4.0;
+  R[o + 6] = (a.us + b.us) / 4.0;
+  R[o + 7] = (a.e + b.e) / 4.0;
+}
+
 } // namespace
 
 // One LDS arena, in doubles.  Phases 1-3 use [primitives 6 x 400 | face states 12 x 272] (T);
@@ -270,10 +427,15 @@ constexpr int ARENA2_DOUBLES =
 // RW: a fifth wavefront (threads 256..319) does the 64 ring-cell face states and the 32 ring
 // Riemann problems in parallel with the owned cells instead of after them on wave 0 -- the form
 // for launch lists too short to fill the GPU, where a block's latency is the step time.
-template <bool THINC, bool HANCOCK, typename T, bool RW = false, bool MP = false, bool FAST = false>
+// AMR: the launch list may hold blocks of two-level solves (FP64 records only): their ghosts come
+// from ghost_two_level, a coarse block takes the faces it shares with fine blocks from A.iflux,
+// and a fine block bordering coarse blocks writes those records there (the fine blocks that do
+// run in a launch of their own before the rest of the list).
+template <bool THINC, bool HANCOCK, typename T, bool RW = false, bool MP = false, bool FAST = false, bool AMR = false>
 G5D void
 step2d_block(StepArgs const & A, int blk)
 {
+  static_assert(!AMR or sizeof(T) == sizeof(double), "two-level solves exchange FP64 face records");
   __shared__ double sArena[ARENA2_DOUBLES<T>];
   __shared__ double sRed[5];
   T * const         sQ = reinterpret_cast<T *>(sArena);
@@ -283,12 +445,29 @@ step2d_block(StepArgs const & A, int blk)
   const int       tr = RW ? t - 256 : t;        // ring lane index when in [0, 64)
   const bool      ring = RW ? !owner : t < 64; // lane with a ring cell
   const BlockDesc bd = A.blocks[blk];
-  const SolveDesc S = A.solves[bd.solve];
+  const SolveDesc S = AMR ? at_level(A.solves[bd.solve], bd.level) : A.solves[bd.solve];
   SolveCtl *      C = A.ctl + bd.solve;
   const int64_t   own = static_cast<int64_t>(bd.slot) * 8 * 256 + t;
   const bool      host_dt = A.dt_fixed > 0.0;
   if (!host_dt and C->active == 0)
     return; // finished solve: its state stays in buffer `parity`
+  // two-level solve: sides (bit s: xmin, xmax, ymin, ymax) whose neighbour is finer (a coarse
+  // block) or coarser (a fine block)
+  const bool two_level = AMR and S.base1 >= 0;
+  int        finer = 0, coarser = 0;
+  if (two_level)
+  {
+#pragma unroll
+    for (int sd = 0; sd < 4; ++sd)
+    {
+      const int nx = bd.bx + (sd == 0 ? -1 : (sd == 1 ? 1 : 0)), ny = bd.by + (sd == 2 ? -1 : (sd == 3 ? 1 : 0));
+      if (bd.level == 0)
+        finer |= (nx >= 0 and nx < S.nb[0] and ny >= 0 and ny < S.nb[1] and slot_l0(S, A.slots, nx, ny) < 0) ? 1 << sd
+                                                                                                             : 0;
+      else
+        coarser |= slot_l1(S, A.slots, nx, ny) < 0 ? 1 << sd : 0;
+    }
+  }
   const int      par = C->parity;
   double const * U = A.U_in ? A.U_in : (par ? A.buf1 : A.buf0);
   double *       U2 = A.U_out ? A.U_out : (par ? A.buf0 : A.buf1);
@@ -309,11 +488,17 @@ step2d_block(StepArgs const & A, int blk)
   }
   if (t < 144)
   {
-    int gi, gj, neg;
+    int  gi, gj, neg;
+    Cons u;
     halo_cell2(t, gi, gj);
-    const int64_t off = resolve_cell<2, B2>(S, A.slots, bd.bx, bd.by, 0, gi, gj, 0, neg);
-    Cons          u = load_cons<2, B2>(U, off, -1);
-    apply_border<2>(S, u, neg);
+    if (two_level)
+      u = ghost_two_level(S, A.slots, U, bd, gi, gj);
+    else
+    {
+      const int64_t off = resolve_cell<2, B2>(S, A.slots, bd.bx, bd.by, 0, gi, gj, 0, neg);
+      u = load_cons<2, B2>(U, off, -1);
+      apply_border<2>(S, u, neg);
+    }
     PrimT<T>      q;
     to_prim<2, FAST>(S, u, q);
     sq_store2(sQ, tidx2(gi, gj), q);
@@ -362,6 +547,20 @@ step2d_block(StepArgs const & A, int blk)
     const FaceT<T> L = ax == 0 ? st_load2(XL, fx2(16, rr)) : st_load2(YL, fy2(rr, 16));
     Fr = riemann<2, FAST>(S, L, fring, dtdx, ax);
   }
+  if constexpr (AMR)
+  {
+    // a coarse block's faces on a finer neighbour take the fine faces' records
+    if (finer != 0)
+    {
+      if (owner and i == 0 and (finer & 1))
+        Fx = iflux_load(A.iflux, iflux_at(bd.slot, 0, j));
+      if (owner and j == 0 and (finer & 4))
+        Fy = iflux_load(A.iflux, iflux_at(bd.slot, 2, i));
+      const int sd = rs == 1 ? 1 : 3;
+      if (ring_face and ((finer >> sd) & 1))
+        Fr = iflux_load(A.iflux, iflux_at(bd.slot, sd, rr));
+    }
+  }
   __syncthreads();
 
   // ---- 4. flux exchange and update ----
@@ -382,6 +581,33 @@ step2d_block(StepArgs const & A, int blk)
       fl_store2(FY, FYE, fy2(rr, 16), Fr);
   }
   __syncthreads();
+  if constexpr (AMR)
+  {
+    // a fine block on a coarse neighbour: per coarse boundary cell, the two fine faces' records
+    // summed and divided by 4 into the coarse block's entry (lane: side t/8, coarse cell t%8 of
+    // the 8 this block covers)
+    const int sd = t >> 3, c = t & 7;
+    if (coarser != 0 and t < 32 and ((coarser >> sd) & 1))
+    {
+      Flux r0, r1;
+      if (sd < 2)
+      {
+        const int fi = sd == 0 ? 0 : 16;
+        r0 = fl_load2(FX, FXE, fx2(fi, 2 * c));
+        r1 = fl_load2(FX, FXE, fx2(fi, 2 * c + 1));
+      }
+      else
+      {
+        const int fj = sd == 2 ? 0 : 16;
+        r0 = fl_load2(FY, FYE, fy2(2 * c, fj));
+        r1 = fl_load2(FY, FYE, fy2(2 * c + 1, fj));
+      }
+      const int nx = bd.bx + (sd == 0 ? -1 : (sd == 1 ? 1 : 0)), ny = bd.by + (sd == 2 ? -1 : (sd == 3 ? 1 : 0));
+      const int cslot = slot_l0(S, A.slots, nx >> 1, ny >> 1);
+      const int cell = 8 * ((sd < 2 ? bd.by : bd.bx) & 1) + c;
+      iflux_store_sum(A.iflux, iflux_at(cslot, sd ^ 1, cell), r0, r1);
+    }
+  }
   double   rate = 0.0;
   uint32_t fail = 0u;
   if (owner)
@@ -424,6 +650,17 @@ k_step2d(StepArgs A)
     step2d_block<THINC, HANCOCK, T, false, MP, FAST>(A, b);
 }
 
+// one workgroup per block, blocks of two-level solves allowed
+template <bool THINC, bool HANCOCK, bool MP, bool FAST>
+__global__ void __launch_bounds__(256)
+k_step2d_two_level(StepArgs A)
+{
+  int b, end, stride;
+  block_range(A, b, end, stride);
+  if (b < end)
+    step2d_block<THINC, HANCOCK, double, false, MP, FAST, true>(A, b);
+}
+
 // one workgroup of five wavefronts per block (ring work on the fifth)
 template <bool THINC, bool HANCOCK, typename T, bool MP, bool FAST>
 __global__ void __launch_bounds__(320)
@@ -463,6 +700,34 @@ launch2k(StepArgs const & A, bool ring_wave, hipStream_t stream)
                        A);
   else
     hipLaunchKernelGGL((k_step2d<THINC, HANCOCK, T, MP, FAST>), dim3(8 * A.per_xcd), block, 0, stream, A);
+}
+
+// a launch list with blocks of two-level solves: the fine blocks that border coarse blocks
+// (the first n_first of the list) write their interface records, then the rest of the list reads them
+template <bool MP, bool FAST>
+static void
+launch2_two_level(StepArgs const & A, int n_first, bool thinc, bool hancock, hipStream_t stream)
+{
+  StepArgs P[2] = { A, A };
+  P[0].nblocks = n_first;
+  P[1].blocks = A.blocks + n_first;
+  P[1].nblocks = A.nblocks - n_first;
+  for (StepArgs & a : P)
+  {
+    if (a.nblocks <= 0)
+      continue;
+    a.per_xcd = (a.nblocks + 7) / 8;
+    a.wg_per_xcd = 0;
+    const dim3 grid(8 * a.per_xcd), block(256);
+    if (thinc and hancock)
+      hipLaunchKernelGGL((k_step2d_two_level<true, true, MP, FAST>), grid, block, 0, stream, a);
+    else if (thinc)
+      hipLaunchKernelGGL((k_step2d_two_level<true, false, MP, FAST>), grid, block, 0, stream, a);
+    else if (hancock)
+      hipLaunchKernelGGL((k_step2d_two_level<false, true, MP, FAST>), grid, block, 0, stream, a);
+    else
+      hipLaunchKernelGGL((k_step2d_two_level<false, false, MP, FAST>), grid, block, 0, stream, a);
+  }
 }
 
 template <typename T, bool MP, bool FAST>

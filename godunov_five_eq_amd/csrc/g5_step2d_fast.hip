@@ -12,4 +12,11 @@ launch_step2d_fast(StepArgs const & A, bool thinc, bool hancock, bool ring_wave,
      : launch2<double, false, true>(A, thinc, hancock, ring_wave, stream);
 }
 
+void
+launch_step2d_two_level_fast(StepArgs const & A, int n_first, bool thinc, bool hancock, bool mp, hipStream_t stream)
+{
+  mp ? launch2_two_level<true, true>(A, n_first, thinc, hancock, stream)
+     : launch2_two_level<false, true>(A, n_first, thinc, hancock, stream);
+}
+
 } // namespace g5

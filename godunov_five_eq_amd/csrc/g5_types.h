@@ -24,6 +24,10 @@ struct SolveDesc
   int32_t inlet;       // G5_INLET_*
   int32_t source_term; // G5_SOURCE_*
   int32_t abort_neg_p; // 1: a negative pressure in a new state fails the solve
+  // two-level static mesh (2-D): slot tables of level 0 at base (nb[0] x nb[1], -1 where refined)
+  // and of level 1 at base1 (2 nb[0] x 2 nb[1], -1 outside the refined blocks); base1 < 0: one level
+  int32_t base1;
+  int32_t prolong; // coarse-to-fine ghost fill: 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD
   double  dx;
   double  cfl;
   double  slope_type;
@@ -66,12 +70,13 @@ struct SolveCtl
   int32_t            pad;
 };
 
-// block of the launch list: solve id, memory slot, brick coordinates
+// block of the launch list: solve id, memory slot, brick coordinates at the block's level, level
+// (0, or 1 for the fine blocks of a two-level mesh)
 struct BlockDesc
 {
   int32_t solve;
   int32_t slot;
-  int16_t bx, by, bz, pad;
+  int16_t bx, by, bz, level;
 };
 
 struct StepArgs
@@ -90,7 +95,12 @@ struct StepArgs
                                 // workgroups per XCD, each looping over its XCD's chunk
   int32_t           remap;      // 1: XCD x works on blocks [x * per_xcd, (x + 1) * per_xcd)
   double            dt_fixed;   // > 0: use this dt for every solve (host-driven step)
+  // two-level meshes: face records of the fine blocks' faces that border coarse blocks, per slot
+  // [side xmin, xmax, ymin, ymax][face 0..15][9 values], written by the fine blocks' launch and
+  // read by their coarse neighbours in the next launch
+  double *          iflux;
 };
+constexpr int kIfluxDoubles = 4 * 16 * 8; // interface records per slot
 
 // launch geometry of the step kernels (chosen by the autotuner or set by the caller); every
 // choice computes the same bits
@@ -118,6 +128,8 @@ make_solve_desc(g5_config const & c, int32_t base, SolveDesc & s)
   s.inlet = c.inlet;
   s.source_term = c.source_term;
   s.abort_neg_p = c.abort_negative_pressure != 0 ? 1 : 0;
+  s.base1 = -1;
+  s.prolong = 0;
   for (int k = 0; k < 9; ++k)
     s.inlet_u[k] = c.inlet_u[k];
   for (int d = 0; d < 3; ++d)

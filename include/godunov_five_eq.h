@@ -159,6 +159,26 @@ const char * g5_last_error(const g5_ctx * ctx);
  * scheme/GodunovImplemV0.cpp:35-61): validates cfg, allocates the device state and scratch. */
 int g5_create(const g5_config * cfg, g5_ctx ** out);
 
+/* A two-level static mesh (2-D, FP64 or FP64_FAST): cfg's brick of nb[0] x nb[1] blocks at level
+ * 0, with the n_refined blocks at refined_coords[2k], refined_coords[2k+1] each replaced by 2 x 2
+ * blocks of half the cell length (level 1).  Refined blocks lie at least one block inside the
+ * brick's border.  The reference's AMR paths on such a mesh: the coarse/fine ghost fill of
+ * ConvertToPrimitivesVariablesFunctor::fill_ghosts (scheme/ConvertToPrimitivesVariablesFunctor.cpp:
+ * 884-918) -- restriction (:214-250) of a finer neighbour, prolongation of a coarser one,
+ * prolongation 0 SIMPLE_COPY or 1 EXTRAPOLATE_LINEAR_MINMOD (:553-770) -- the fine-face flux sums
+ * of read_fluxes_and_update_2d (scheme/ReadFluxesAndConservativeUpdateFunctor.cpp:137-199,
+ * .h:256-307) and the per-level cell length in the fluxes and the CFL rate
+ * (ComputeFluxesAndStoreFunctor, ComputeDtFiveEqFunctor).  The state holds the blocks in the order
+ * of g5_block_layout: the coarse leaves lexicographic, then the fine blocks lexicographic over the
+ * 2nb[0] x 2nb[1] fine grid (the mesh itself, which the reference takes from its AMR manager, is
+ * fixed for the context's life: no regridding). */
+int g5_create_two_level(const g5_config * cfg, const int32_t * refined_coords, int32_t n_refined,
+                        int32_t prolongation, g5_ctx ** out);
+
+/* the blocks of the context's state in order: 4 int32 per block (x, y, z, level), coordinates at
+ * the block's level; n = 4 * n_blocks */
+int g5_block_layout(const g5_ctx * ctx, int32_t * coords, size_t n);
+
 int g5_destroy(g5_ctx * ctx);
 
 /* number of owned blocks, variables per cell and cells per block */

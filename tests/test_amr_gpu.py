@@ -1,0 +1,192 @@
+"""Two-level static mesh on the GPU (g5_create_two_level) against the oracle's two-level mesh.
+
+The coupling under test is the reference's AMR path of the V0 step on a 2-D mesh whose refined
+blocks are fixed: ghosts of a coarse block inside a refined neighbour come from the restriction
+of the four fine cells, ghosts of a fine block outside the fine blocks from the prolongation of
+the coarse parent (SIMPLE_COPY or EXTRAPOLATE_LINEAR_MINMOD)
+(scheme/ConvertToPrimitivesVariablesFunctor.cpp:214-250,553-770,884-918); a coarse cell on a face
+shared with fine blocks is updated with the sum of the fine faces' fluxes over 4
+(scheme/ReadFluxesAndConservativeUpdateFunctor.cpp:137-199, .h:256-307); fluxes and the CFL rate
+use each level's cell length.  The oracle's restatement of it is pinned in
+tests/test_oracle_amr.py (closed forms on linear fields, uniform-state preservation, conservation
+across the levels).
+
+Bars: the FP64 path bit-identical to the oracle (asserted as relative L1 <= 1e-12 per conserved
+variable, the FP64 parity tolerance, and reported exact or not), the same final time; the
+reassociated FP64 path within relative L1 1e-12 (momentum components against the momentum
+vector's L1 norm) and conserving mass, energy and momentum to round-off in a periodic box.
+Refined regions are L-shaped, so coarse cells meet fine ones on faces of both orientations and
+at a concave corner, and the moving features (a shock, an advected material blob) cross the level
+interface during the run.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+RTOL_L1 = 1e-12
+L_SHAPE_8x4 = [[3, 1], [4, 1], [3, 2]]
+L_SHAPE_6x6 = [[2, 2], [3, 2], [2, 3]]
+
+
+def _announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel_l1(a, b, momentum_norm=False):
+    A = a.reshape(-1, 8, 256)
+    B = b.reshape(-1, 8, 256)
+    mom_den = np.abs(B[:, 2]).sum() + np.abs(B[:, 3]).sum()
+    out = []
+    for v in range(8):
+        den = mom_den if (momentum_norm and v in (2, 3)) else np.abs(B[:, v]).sum()
+        out.append(np.abs(A[:, v] - B[:, v]).sum() / den if den > 0 else np.abs(A[:, v] - B[:, v]).sum())
+    return np.array(out)
+
+
+def two_level_state(nbx, layout, U_coarse, U_fine):
+    """The state of a two-level mesh in the context's block order from the one-level states on the
+    coarse grid (nbx x nby blocks) and on the fine grid (2nbx x 2nby blocks of half the cell length)"""
+    Uc = U_coarse.reshape(-1, 8, 256)
+    Uf = U_fine.reshape(-1, 8, 256)
+    return np.stack([Uc[x + nbx * y] if lv == 0 else Uf[x + 2 * nbx * y] for x, y, _, lv in layout]).ravel()
+
+
+def shock_bubble_two_level(g5, refined, prolongation, **kw):
+    cfg, Uc = cases.air_helium_2d(nbx=8, nby=4, **kw)
+    _, Uf = cases.air_helium_2d(nbx=16, nby=8, **kw)
+    ctx = g5.Context(cfg, refined=refined, prolongation=prolongation)
+    return cfg, ctx, two_level_state(8, ctx.block_layout(), Uc, Uf)
+
+
+def oracle_of(orc, cfg, refined, prolongation):
+    r = np.array([[x, y, 0] for x, y in refined], dtype=np.int32)
+    return orc.Oracle(cfg, refined=r, prolongation=prolongation)
+
+
+def test_layout_matches_the_oracle(g5, orc):
+    cfg, ctx, _ = shock_bubble_two_level(g5, L_SHAPE_8x4, 0)
+    with ctx:
+        lay = ctx.block_layout()
+        o = oracle_of(orc, cfg, L_SHAPE_8x4, 0)
+        assert ctx.nblocks == o.nowned == 29 + 12
+        assert np.array_equal(lay[:, 3], o.block_levels()[: o.nowned])
+        assert np.array_equal(lay[:, :2], o.block_coords()[: o.nowned, :2])
+
+
+@pytest.mark.parametrize("prolongation", [0, 1])
+def test_shock_crosses_the_level_interface(g5, orc, prolongation):
+    # air-helium shock-bubble on an 8x4 brick (256x64 coarse cells) with an L-shaped refined region
+    # around the bubble: the Mach 1.22 shock enters the fine blocks after about 100 steps
+    label = f"two-level shock-bubble prolongation={prolongation}"
+    _announce(label)
+    cfg, ctx, U0 = shock_bubble_two_level(g5, L_SHAPE_8x4, prolongation)
+    o = oracle_of(orc, cfg, L_SHAPE_8x4, prolongation)
+    o.set_state(U0)
+    with ctx:
+        ctx.upload(U0)
+        dt_g = ctx.compute_dt()
+        assert dt_g == o.compute_dt(), (dt_g, o.compute_dt())
+        t_g, n_g = ctx.run(150)
+        U_g = ctx.download()
+    t_o, n_o = o.run(150)
+    U_o = o.get_state()
+    assert n_g == n_o == 150 and t_g == t_o, (t_g, t_o)
+    assert np.all(np.isfinite(U_g))
+    # the shock is inside the fine blocks: their state has changed by O(1)
+    fine = np.repeat(o.block_levels()[: o.nowned], 8 * 256) == 1
+    assert np.abs(U_g - U0)[fine].max() > 0.1
+    err = rel_l1(U_g, U_o)
+    exact = np.array_equal(U_g, U_o)
+    _announce(f"{label}: max rel L1 = {err.max():.3e} bit-exact={exact}")
+    assert err.max() <= RTOL_L1, err
+    assert exact
+
+
+def test_host_dt_steps_thinc_walls(g5, orc):
+    # THINC on the two-level mesh with walls on every side, stepped with host dts (do_time_step)
+    label = "two-level THINC host-dt"
+    _announce(label)
+    kw = dict(thinc=True, thinc_beta=1.6, bc=("WALL",) * 4 + ("NONE",) * 2)
+    cfg, ctx, U0 = shock_bubble_two_level(g5, L_SHAPE_8x4, 1, **kw)
+    o = oracle_of(orc, cfg, L_SHAPE_8x4, 1)
+    o.set_state(U0)
+    with ctx:
+        ctx.upload(U0)
+        for _ in range(25):
+            dt = o.compute_dt()
+            o.godunov_unsplit(dt)
+            ctx.do_time_step(dt)
+        U_g = ctx.download()
+    U_o = o.get_state()
+    err = rel_l1(U_g, U_o)
+    exact = np.array_equal(U_g, U_o)
+    _announce(f"{label}: max rel L1 = {err.max():.3e} bit-exact={exact}")
+    assert err.max() <= RTOL_L1 and exact, err
+
+
+def advection_two_level(g5, refined, prolongation, **kw):
+    cfg, Uc, pressure = cases.interface_advection_2d(nb=(6, 6), **kw)
+    _, Uf, _ = cases.interface_advection_2d(nb=(12, 12), **kw)
+    ctx = g5.Context(cfg, refined=refined, prolongation=prolongation)
+    return cfg, ctx, two_level_state(6, ctx.block_layout(), Uc, Uf)
+
+
+def test_rk2_thinc_periodic_advection(g5, orc):
+    # a material blob advected diagonally through an L-shaped refined region, THINC, RK2_SSP,
+    # periodic in x and y, minmod prolongation
+    label = "two-level periodic advection RK2_SSP THINC"
+    _announce(label)
+    cfg, ctx, U0 = advection_two_level(g5, L_SHAPE_6x6, 1, thinc=True, integrator="RK2_SSP")
+    o = oracle_of(orc, cfg, L_SHAPE_6x6, 1)
+    o.set_state(U0)
+    with ctx:
+        ctx.upload(U0)
+        t_g, n_g = ctx.run(30)
+        U_g = ctx.download()
+    t_o, n_o = o.run(30)
+    U_o = o.get_state()
+    assert n_g == n_o == 30 and t_g == t_o
+    err = rel_l1(U_g, U_o)
+    exact = np.array_equal(U_g, U_o)
+    _announce(f"{label}: max rel L1 = {err.max():.3e} bit-exact={exact}")
+    assert err.max() <= RTOL_L1 and exact, err
+
+
+def totals(ctx, cfg, U):
+    lv = ctx.block_layout()[:, 3]
+    V = U.reshape(-1, 8, 256)
+    vol = (cfg.dx / 2.0 ** lv) ** 2
+    return np.array([(V[:, v].sum(axis=1) * vol).sum() for v in range(8)])
+
+
+@pytest.mark.parametrize("prolongation", [0, 1])
+def test_fast_path_two_level(g5, orc, prolongation):
+    # the reassociated FP64 path on both two-level cases: the 1e-12 bar against the oracle, and
+    # conservation of mass, energy and momentum across the levels in the periodic box
+    label = f"two-level FP64_FAST prolongation={prolongation}"
+    _announce(label)
+    cfg, ctx, U0 = shock_bubble_two_level(g5, L_SHAPE_8x4, prolongation, precision=g5.PREC_FP64_FAST)
+    o = oracle_of(orc, cfg, L_SHAPE_8x4, prolongation)
+    o.set_state(U0)
+    with ctx:
+        ctx.upload(U0)
+        t_g, _ = ctx.run(150)
+        U_g = ctx.download()
+    t_o, _ = o.run(150)
+    err = rel_l1(U_g, o.get_state(), momentum_norm=True)
+    _announce(f"{label} shock-bubble: max rel L1 = {err.max():.3e}")
+    assert err.max() <= RTOL_L1 and abs(t_g - t_o) <= RTOL_L1 * t_o, err
+    cfg, ctx, U0 = advection_two_level(g5, L_SHAPE_6x6, prolongation, precision=g5.PREC_FP64_FAST)
+    with ctx:
+        ctx.upload(U0)
+        ctx.run(40)
+        U_g = ctx.download()
+        T0, T1 = totals(ctx, cfg, U0), totals(ctx, cfg, U_g)
+    for v in (0, 1, 2, 3):
+        assert abs(T1[v] - T0[v]) <= 1e-14 * abs(T0[v]), (v, T0[v], T1[v])
