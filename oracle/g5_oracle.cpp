@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 #include <algorithm>
 
@@ -1595,6 +1596,179 @@ struct Solver
     return invDt;
   }
 
+  // ---- refine flags (scheme/ComputeRefineFlags.cpp) ------------------------------------------
+  // Variable v of the cell one step (side -1/+1 along dir) from cell ci of owned block ib, as
+  // getNeighborDataSameLevel (:67-107) reads it: outside a non-periodic border the current cell's
+  // own value; a same-level or coarser neighbour's cell value (a fine cell outside the fine
+  // blocks reads its coarse parent); a finer neighbour's average of the 2^dim fine cells
+  // (compute_siblings_average of the absent core: summed x fastest, divided by 2^dim).
+  double
+  neighbor_value(int ib, int const ci[3], int dir, int side, int v)
+  {
+    int g[3] = { 0, 0, 0 };
+    for (int d = 0; d < dim; ++d)
+      g[d] = bcoord[3 * ib + d] * b + ci[d];
+    g[dir] += side;
+    if (amr and blev[ib] == 1)
+    {
+      int c[3] = { 0, 0, 0 }, f[3] = { 0, 0, 0 };
+      for (int d = 0; d < dim; ++d)
+      {
+        c[d] = floor_div(g[d], b);
+        f[d] = g[d] - c[d] * b;
+      }
+      const int jb = lookup1[c[0] + 2 * nbd(0) * (c[1] + 2 * nbd(1) * c[2])];
+      if (jb >= 0)
+        return u(U, jb, v, cidx(f[0], f[1], f[2]));
+      int gc[3] = { 0, 0, 0 };
+      for (int d = 0; d < dim; ++d)
+        gc[d] = floor_div(g[d], 2);
+      return coarse_value(gc, v);
+    }
+    for (int d = 0; d < dim; ++d)
+    {
+      const int n = nbd(d) * b;
+      if (g[d] < 0 or g[d] >= n)
+      {
+        if (not P.periodic[d])
+          return u(U, ib, v, cidx(ci[0], ci[1], ci[2]));
+        g[d] = ((g[d] % n) + n) % n;
+      }
+    }
+    return coarse_value(g, v);
+  }
+
+  // variable v of level-0 cell g inside the brick: a coarse leaf's value or the average of the
+  // fine cells of a refined block
+  double
+  coarse_value(int const g[3], int v)
+  {
+    int c[3] = { 0, 0, 0 }, f[3] = { 0, 0, 0 };
+    for (int d = 0; d < dim; ++d)
+    {
+      c[d] = floor_div(g[d], b);
+      f[d] = g[d] - c[d] * b;
+    }
+    const int ib = lookup[ext_index(c[0], c[1], c[2])];
+    if (ib >= 0)
+      return u(U, ib, v, cidx(f[0], f[1], f[2]));
+    double sum = 0.0;
+    bool   first = true;
+    for (int kz = 0; kz < (dim == 3 ? 2 : 1); ++kz)
+      for (int ky = 0; ky < 2; ++ky)
+        for (int kx = 0; kx < 2; ++kx)
+        {
+          const int fg[3] = { 2 * g[0] + kx, 2 * g[1] + ky, dim == 3 ? 2 * g[2] + kz : 0 };
+          int       fc[3] = { 0, 0, 0 }, ff[3] = { 0, 0, 0 };
+          for (int d = 0; d < dim; ++d)
+          {
+            fc[d] = floor_div(fg[d], b);
+            ff[d] = fg[d] - fc[d] * b;
+          }
+          const int    jb = lookup1[fc[0] + 2 * nbd(0) * (fc[1] + 2 * nbd(1) * fc[2])];
+          const double val = u(U, jb, v, cidx(ff[0], ff[1], ff[2]));
+          sum = first ? val : sum + val;
+          first = false;
+        }
+    return sum / (dim == 3 ? 8.0 : 4.0);
+  }
+
+  // compute_lohner_split (:117-135): |L - 2C + R| / (|R - C| + |C - L| + eps (|L| + 2|C| + |R|))
+  static double
+  lohner_split(double L, double C, double R, double eps)
+  {
+    return std::fabs(L - 2 * C + R) / (std::fabs(R - C) + std::fabs(C - L) +
+                                       eps * (std::fabs(L) + 2 * std::fabs(C) + std::fabs(R)));
+  }
+  // normalized_gradient (:100-113)
+  static double
+  normalized_gradient(double v1, double v2)
+  {
+    double vmax = std::fmax(std::fabs(v1), std::fabs(v2));
+    if (vmax < 0.001)
+      return 0.0;
+    vmax = std::fabs(v1 - v2) / vmax;
+    return std::fmax(std::fmin(vmax, 1.0), 0.0);
+  }
+
+  // Refine flags of the owned blocks (ComputeRefineFlags::operator(), :253-372) from the
+  // indicator on the partial densities IAD0 and IAD1: per cell, update_flag (:236-249) for each
+  // indicator value (LOHNER_SPLIT: per variable, per direction; LOHNER_UNSPLIT and SIMPLE_GRADIENT:
+  // per variable), then the maximum over the block's cells into flags[ib] (the caller's flags are
+  // max-accumulated, as the reference's atomic_max into the reset flag array).  Flag values in the
+  // order the maximum needs: 0 initial, 1 coarsen, 2 nothing, 3 refine.  indmax[ib] (optional):
+  // the largest indicator value of the block (a NaN value -- 0/0 where a partial density and its
+  // neighbours vanish -- takes part in no comparison and no maximum).
+  // indicator: 0 LOHNER_SPLIT, 1 LOHNER_UNSPLIT, 2 SIMPLE_GRADIENT; base_level: level of level-0
+  void
+  refine_flags(int indicator, int level_min, int level_max, int base_level, double refine_th,
+               double coarsen_th, double eps, int32_t * flags, double * indmax)
+  {
+#pragma omp parallel for schedule(static)
+    for (int ib = 0; ib < nowned; ++ib)
+    {
+      const int level = base_level + (amr ? blev[ib] : 0);
+      int32_t   flag = 0;
+      double    imax = 0.0;
+      auto      update = [&](double ind) {
+        if (level < level_max and ind > refine_th)
+          flag = std::max(flag, 3);
+        else if (level > level_min and ind < coarsen_th)
+          flag = std::max(flag, 1);
+        else
+          flag = std::max(flag, 2);
+        imax = std::fmax(imax, ind);
+      };
+      for (int c = 0; c < ncell; ++c)
+      {
+        const int ci[3] = { c % b, (c / b) % b, dim == 3 ? c / (b * b) : 0 };
+        for (int v : { H.IAD0, H.IAD1 })
+        {
+          const double C = u(U, ib, v, c);
+          double       L[3], R[3];
+          for (int d = 0; d < dim; ++d)
+          {
+            L[d] = neighbor_value(ib, ci, d, -1, v);
+            R[d] = neighbor_value(ib, ci, d, 1, v);
+          }
+          if (indicator == 0)
+          {
+            for (int d = 0; d < dim; ++d)
+              update(lohner_split(L[d], C, R[d], eps));
+          }
+          else if (indicator == 1)
+          {
+            // compute_lohner_unsplit (:140-191)
+            double num = 0.0, den = 0.0;
+            for (int d = 0; d < dim; ++d)
+            {
+              const double s2 = L[d] - 2 * C + R[d];
+              const double t = std::fabs(R[d] - C) + std::fabs(C - L[d]) +
+                               eps * (std::fabs(L[d]) + 2 * std::fabs(C) + std::fabs(R[d]));
+              num = d == 0 ? s2 * s2 : num + s2 * s2;
+              den = d == 0 ? t * t : den + t * t;
+            }
+            update(std::sqrt(num / den));
+          }
+          else
+          {
+            // compute_simple_gradient (:196-231), from the identity of a max reduction
+            double res = -std::numeric_limits<double>::max();
+            for (int d = 0; d < dim; ++d)
+            {
+              res = std::fmax(res, normalized_gradient(C, L[d]));
+              res = std::fmax(res, normalized_gradient(C, R[d]));
+            }
+            update(res);
+          }
+        }
+      }
+      flags[ib] = std::max(flags[ib], flag);
+      if (indmax)
+        indmax[ib] = imax;
+    }
+  }
+
   // SolverGodunovFiveEq.cpp:368-390 compute_dt_local
   double
   compute_dt_local()
@@ -1673,6 +1847,15 @@ g5o_cons_level(void * h, int32_t level, const int32_t * g, double * out)
   const State u = S->cons_level(S->U, level, gg);
   for (int v = 0; v < S->nvar; ++v)
     out[v] = u[v];
+}
+
+// refine flags of the owned blocks (Solver::refine_flags), flags max-accumulated
+void
+g5o_refine_flags(void * h, int32_t indicator, int32_t level_min, int32_t level_max, int32_t base_level,
+                 double refine_th, double coarsen_th, double eps, int32_t * flags, double * indmax)
+{
+  static_cast<Solver *>(h)->refine_flags(indicator, level_min, level_max, base_level, refine_th, coarsen_th, eps,
+                                         flags, indmax);
 }
 
 void

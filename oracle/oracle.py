@@ -65,6 +65,7 @@ def lib(variant=""):
             "g5o_create_amr": (vp, [P(OracleParams), P(i32), i32, i32]),
             "g5o_block_levels": (None, [vp, P(i32)]),
             "g5o_cons_level": (None, [vp, i32, P(i32), P(d)]),
+            "g5o_refine_flags": (None, [vp, i32, i32, i32, i32, d, d, d, P(ctypes.c_int32), P(d)]),
             "g5o_destroy": (None, [vp]),
             "g5o_num_owned": (i32, [vp]),
             "g5o_num_outside": (i32, [vp]),
@@ -177,6 +178,16 @@ class Oracle:
         out = np.zeros(self.nvar)
         self.L.g5o_cons_level(self.h, level, gg, _dp(out))
         return out
+
+    def refine_flags(self, indicator, level_min, level_max, base_level, refine_th, coarsen_th, eps, flags=None):
+        """(flags, indicator max) per owned block; flags max-accumulated into the given array
+        (0 initial, 1 coarsen, 2 nothing, 3 refine); indicator 0 LOHNER_SPLIT, 1 LOHNER_UNSPLIT,
+        2 SIMPLE_GRADIENT"""
+        f = np.zeros(self.nowned, dtype=np.int32) if flags is None else np.ascontiguousarray(flags, dtype=np.int32)
+        imax = np.zeros(self.nowned)
+        self.L.g5o_refine_flags(self.h, indicator, level_min, level_max, base_level, refine_th, coarsen_th, eps,
+                                f.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _dp(imax))
+        return f, imax
 
     def block_levels(self):
         lv = np.zeros(self.nowned, dtype=np.int32)
