@@ -138,6 +138,21 @@ class G5ServeReport(ctypes.Structure):
 
 SERVE_DONE, SERVE_FAILED, SERVE_REJECTED = 0, 1, 2
 
+INDICATOR_LOHNER_SPLIT, INDICATOR_LOHNER_UNSPLIT, INDICATOR_SIMPLE_GRADIENT = 0, 1, 2
+AMR_FLAG_INIT, AMR_COARSEN, AMR_NOTHING, AMR_REFINE = 0, 1, 2, 3
+
+
+class G5RefineParams(ctypes.Structure):
+    _fields_ = [
+        ("indicator", ctypes.c_int32),
+        ("level_min", ctypes.c_int32),
+        ("level_max", ctypes.c_int32),
+        ("base_level", ctypes.c_int32),
+        ("refine_threshold", ctypes.c_double),
+        ("coarsen_threshold", ctypes.c_double),
+        ("epsilon_lohner", ctypes.c_double),
+    ]
+
 
 class G5ServeResult(ctypes.Structure):
     _fields_ = [
@@ -169,6 +184,7 @@ def lib():
             "g5_create": (ctypes.c_int, [P(G5Config), P(vp)]),
             "g5_create_two_level": (ctypes.c_int, [P(G5Config), P(i32), i32, i32, P(vp)]),
             "g5_block_layout": (ctypes.c_int, [vp, P(i32), sz]),
+            "g5_refine_flags": (ctypes.c_int, [vp, P(G5RefineParams), P(i32), P(d), sz]),
             "g5_destroy": (ctypes.c_int, [vp]),
             "g5_shape": (ctypes.c_int, [vp, P(i32), P(i32), P(i32)]),
             "g5_mem_bytes": (ctypes.c_int, [vp, P(ctypes.c_uint64)]),
@@ -250,7 +266,7 @@ def exported_symbols():
         "g5_pool_properties", "g5_pool_prepare", "g5_host_alloc", "g5_host_free", "g5_pool_set_clock", "g5_pool_step_async", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
         "g5_pool_admit_spec", "g5_serve", "g5_serve_run", "g5_spec_state_size", "g5_pool_failure",
         "g5_set_block_order", "g5_do_time_step_buffers", "g5_compute_dt_buffer", "g5_step_noalloc",
-        "g5_create_two_level", "g5_block_layout",
+        "g5_create_two_level", "g5_block_layout", "g5_refine_flags",
     ]
 
 
@@ -447,6 +463,18 @@ class Context:
         """coords: (n_blocks, 3) brick coordinates of the caller's blocks, in the caller's order"""
         c = np.ascontiguousarray(coords, dtype=np.int32).reshape(-1, 3)
         self._check(lib().g5_set_block_order(self._h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), c.shape[0]))
+
+    def refine_flags(self, indicator, level_min, level_max, base_level, refine_threshold=0.01,
+                     coarsen_threshold=0.00125, epsilon_lohner=0.02, flags=None):
+        """(flags, largest indicator value) per block (ComputeRefineFlags::run); flags are
+        max-accumulated into the given array (AMR_FLAG_INIT when None)"""
+        prm = G5RefineParams(indicator, level_min, level_max, base_level, refine_threshold, coarsen_threshold,
+                             epsilon_lohner)
+        f = np.zeros(self.nblocks, dtype=np.int32) if flags is None else np.array(flags, dtype=np.int32)
+        imax = np.empty(self.nblocks)
+        self._check(lib().g5_refine_flags(self._h, ctypes.byref(prm), f.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                          _dptr(imax), f.size))
+        return f, imax
 
     def block_layout(self):
         """(n_blocks, 4) int32: x, y, z, level of each block of the state, in order"""

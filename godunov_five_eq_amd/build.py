@@ -18,7 +18,7 @@ H5LIB = os.path.join(HERE, "libg5h5.so")
 HDF5_PREFIX = os.environ.get("G5_HDF5_PREFIX", "/opt/conda")
 
 HIP_SOURCES = ["g5_step2d.hip", "g5_step2d_fast.hip", "g5_step3d.hip", "g5_step3d_fast.hip", "g5_aux.hip",
-               "g5_runtime.hip"]
+               "g5_refine.hip", "g5_runtime.hip"]
 CXX_SOURCES = ["g5_init.cpp", "g5_specs.cpp", "g5_serve.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

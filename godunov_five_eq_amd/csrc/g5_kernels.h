@@ -28,6 +28,11 @@ void launch_step2d_two_level(StepArgs const & A, int n_first, bool thinc, bool h
 void launch_step2d_two_level_fast(StepArgs const & A, int n_first, bool thinc, bool hancock, bool mp,
                                   hipStream_t stream);
 
+// refine flags and largest indicator values of the listed blocks into R.flags / R.imax by slot
+void launch_refine_flags(int dim, double const * buf0, double const * buf1, BlockDesc const * blocks,
+                         SolveDesc const * solves, SolveCtl const * ctl, int32_t const * slots, RefineArgs const & R,
+                         int nblocks, hipStream_t stream);
+
 // max over the listed blocks of the CFL rate of the current state into ctl[solve].invdt_bits
 // (the rate of the path of `precision`, G5_PREC_*)
 void launch_inv_dt(int dim, int precision, double * buf0, double * buf1, BlockDesc const * blocks, SolveDesc const * solves,

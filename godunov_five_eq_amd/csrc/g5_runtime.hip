@@ -176,6 +176,8 @@ struct g5_ctx
   double *             d_ring = nullptr;
   double *             d_staging = nullptr; // contiguous [block][var][cell] buffer for transfers
   std::vector<int32_t> block_order;         // caller's block order (brick coordinates), empty: lexicographic
+  int32_t *            d_flags = nullptr;   // g5_refine_flags outputs, per slot
+  double *             d_imax = nullptr;
   std::string          err;
 };
 
@@ -1732,6 +1734,8 @@ g5_destroy(g5_ctx * ctx)
     hipFree(ctx->d_outside);
     hipFree(ctx->d_ring);
     hipFree(ctx->d_staging);
+    hipFree(ctx->d_flags);
+    hipFree(ctx->d_imax);
     g5_pool_destroy(ctx->pool);
   }
   delete ctx;
@@ -2070,6 +2074,51 @@ g5_compute_dt_buffer(g5_ctx * ctx, const double * U_dev, double * dt)
     c.failure |= c.failure == 0u ? static_cast<uint32_t>(G5_FAIL_NONFINITE) : 0u;
     return fail(ctx->err, G5_ERR_DIVERGED, failure_message(ctx->id, c));
   }
+  return G5_OK;
+}
+
+int
+g5_refine_flags(g5_ctx * ctx, const g5_refine_params * prm, int32_t * flags, double * indicator_max, size_t n)
+{
+  if (not ctx or not prm or not flags)
+    return ctx ? fail(ctx->err, G5_ERR_INVALID, "null argument") : G5_ERR_INVALID;
+  if (not ctx->has_state)
+    return fail(ctx->err, G5_ERR_STATE, "no state uploaded");
+  g5_pool *     p = ctx->pool;
+  const int32_t nblk = p->solves[ctx->id].nblk;
+  if (n != static_cast<size_t>(nblk))
+    return fail(ctx->err, G5_ERR_INVALID, "n must equal the number of blocks");
+  if (prm->indicator < G5_INDICATOR_LOHNER_SPLIT or prm->indicator > G5_INDICATOR_SIMPLE_GRADIENT)
+    return fail(ctx->err, G5_ERR_INVALID, "unknown refine indicator");
+  int rc = rebuild_list(p);
+  if (rc != G5_OK)
+    return fail(ctx->err, rc, p->err);
+  // a context's slots are its block order: block k lives in slot k
+  if (not ctx->d_flags)
+  {
+    G5_HIP(ctx->err, hipMalloc(&ctx->d_flags, p->nslots * sizeof(int32_t)));
+    G5_HIP(ctx->err, hipMalloc(&ctx->d_imax, p->nslots * sizeof(double)));
+  }
+  G5_HIP(ctx->err, hipMemcpyAsync(ctx->d_flags, flags, nblk * sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
+  G5_HIP(ctx->err, hipMemsetAsync(ctx->d_imax, 0, nblk * sizeof(double), p->stream));
+  RefineArgs R;
+  R.indicator = prm->indicator;
+  R.level_min = prm->level_min;
+  R.level_max = prm->level_max;
+  R.base_level = prm->base_level;
+  R.refine_th = prm->refine_threshold;
+  R.coarsen_th = prm->coarsen_threshold;
+  R.eps = prm->epsilon_lohner;
+  R.flags = ctx->d_flags;
+  R.imax = ctx->d_imax;
+  launch_refine_flags(p->dim, p->buf[0], p->buf[1], p->d_blocks, p->d_solves, p->d_ctl, p->d_slots, R, p->nlist,
+                      p->stream);
+  G5_HIP(ctx->err, hipGetLastError());
+  G5_HIP(ctx->err, hipMemcpyAsync(flags, ctx->d_flags, nblk * sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
+  if (indicator_max)
+    G5_HIP(ctx->err,
+           hipMemcpyAsync(indicator_max, ctx->d_imax, nblk * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  G5_HIP(ctx->err, hipStreamSynchronize(p->stream));
   return G5_OK;
 }
 

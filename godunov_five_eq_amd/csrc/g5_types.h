@@ -102,6 +102,17 @@ struct StepArgs
 };
 constexpr int kIfluxDoubles = 4 * 16 * 8; // interface records per slot
 
+// refine-flag computation (g5_refine_flags): parameters and the per-slot outputs
+struct RefineArgs
+{
+  int32_t   indicator; // G5_INDICATOR_*
+  int32_t   level_min, level_max;
+  int32_t   base_level; // level of the solve's level-0 blocks
+  double    refine_th, coarsen_th, eps;
+  int32_t * flags; // per slot, max-accumulated (G5_AMR_*)
+  double *  imax;  // per slot: largest indicator value (bit patterns max-accumulated; starts at 0)
+};
+
 // launch geometry of the step kernels (chosen by the autotuner or set by the caller); every
 // choice computes the same bits
 struct LaunchCfg

@@ -52,6 +52,24 @@ enum
   G5_FAIL_NEGATIVE_PRESSURE = 2
 };
 
+/* refine indicators ([amr] refine_criterion; scheme/ComputeRefineFlags.cpp:41-62) */
+enum
+{
+  G5_INDICATOR_LOHNER_SPLIT = 0,
+  G5_INDICATOR_LOHNER_UNSPLIT = 1,
+  G5_INDICATOR_SIMPLE_GRADIENT = 2
+};
+
+/* per-block refine flags (the AMR context's flag values, combined by maximum over a block's
+ * cells: any cell that asks for refinement refines the block, all must agree to coarsen it) */
+enum
+{
+  G5_AMR_FLAG_INIT = 0,
+  G5_AMR_COARSEN = 1,
+  G5_AMR_NOTHING = 2,
+  G5_AMR_REFINE = 3
+};
+
 /* border conditions, border_conditions/FillOutside.h:51-55 (BC_HYDRO) */
 enum
 {
@@ -220,6 +238,32 @@ int g5_do_time_step_buffers(g5_ctx * ctx, const double * U_dev, double * U2_dev,
  * order); G5_ERR_DIVERGED for a non-finite (or, with the check, negative-pressure) state.
  * Synchronises the context's stream. */
 int g5_compute_dt_buffer(g5_ctx * ctx, const double * U_dev, double * dt);
+
+/* parameters of g5_refine_flags (RefineIndicatorData as SolverGodunovFiveEq::compute_refine_flags
+ * fills it, SolverGodunovFiveEq.cpp:1036-1057) */
+typedef struct g5_refine_params
+{
+  int32_t indicator;         /* G5_INDICATOR_* ([amr] refine_criterion) */
+  int32_t level_min;         /* [amr] level_min */
+  int32_t level_max;         /* [amr] level_max */
+  int32_t base_level;        /* AMR level of the context's blocks (of its level 0 on a two-level mesh) */
+  double  refine_threshold;  /* [amr] epsilon_refine (the reference's default 0.01) */
+  double  coarsen_threshold; /* [amr] epsilon_coarsen (default 0.00125) */
+  double  epsilon_lohner;    /* [amr] epsilon_lohner (default 0.02) */
+} g5_refine_params;
+
+/* Replaces ComputeRefineFlags::run (scheme/ComputeRefineFlags.cpp:18-62) on the context's current
+ * state: the indicator of the partial densities IAD0 and IAD1 in every cell, each value's vote
+ * (refine above refine_threshold below level_max, coarsen under coarsen_threshold above
+ * level_min, else nothing; update_flag, :236-249) and the maximum over the block, max-accumulated
+ * into flags[k] (G5_AMR_*; the caller resets them, as the reference's caller does).  Neighbour
+ * values follow getNeighborDataSameLevel (:67-96): outside a non-periodic border the cell itself,
+ * a finer neighbour's average, a coarser neighbour's cell.  indicator_max (may be NULL) receives
+ * per block the largest indicator value (NaN values -- 0/0 where a partial density vanishes with
+ * its neighbours -- count in no comparison and no maximum).  n = the number of blocks, in the
+ * context's block order.  Synchronises the context's stream. */
+int g5_refine_flags(g5_ctx * ctx, const g5_refine_params * params, int32_t * flags, double * indicator_max,
+                    size_t n);
 
 /* The step of g5_do_time_step_buffers followed by the dt of its result (g5_compute_dt_buffer on U2),
  * without a context and without allocating: no counterpart in the reference; for callers that
