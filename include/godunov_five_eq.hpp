@@ -411,11 +411,43 @@ public:
     m_ctx.reset(c);
   }
 
+  // a two-level static mesh (2-D): the coarse blocks at refined[2k], refined[2k+1] refined into
+  // 2 x 2 blocks of half the cell length; prolongation 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD
+  // (g5_create_two_level; the state's block order is block_layout())
+  GodunovImplemHip(g5_config const & cfg, std::vector<int32_t> const & refined, int prolongation)
+  {
+    if (cfg.dim != dim)
+      throw std::runtime_error("dimension mismatch");
+    g5_ctx * c = nullptr;
+    check(g5_create_two_level(&cfg, refined.data(), static_cast<int32_t>(refined.size() / 2), prolongation, &c),
+          nullptr);
+    m_ctx.reset(c);
+  }
+
   // static factory, scheme/GodunovImplemV0.h:103-114
   static GodunovImplemHip *
   create(ConfigMap const & config_map)
   {
     return new GodunovImplemHip(config_from_ini(config_map));
+  }
+
+  // (x, y, z, level) of each block of the state, in order
+  std::vector<int32_t>
+  block_layout() const
+  {
+    int32_t n = 0, nv = 0, nc = 0;
+    check(g5_shape(m_ctx.get(), &n, &nv, &nc), m_ctx.get());
+    std::vector<int32_t> c(4 * static_cast<size_t>(n));
+    check(g5_block_layout(m_ctx.get(), c.data(), c.size()), m_ctx.get());
+    return c;
+  }
+
+  // ComputeRefineFlags::run (scheme/ComputeRefineFlags.cpp:18-62) on the current state: flags
+  // (one per block, G5_AMR_*) max-accumulated, as into the AMR context's reset flag array
+  void
+  compute_refine_flags(g5_refine_params const & params, std::vector<int32_t> & flags)
+  {
+    check(g5_refine_flags(m_ctx.get(), &params, flags.data(), nullptr, flags.size()), m_ctx.get());
   }
 
   // scratch is sized at creation for the fixed mesh: nothing to resize

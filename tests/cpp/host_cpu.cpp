@@ -1,8 +1,8 @@
 // The host C++ of the library on the CPU alone, for the sanitizer builds of the test suite: the
 // .ini layer (include/godunov_five_eq.hpp: config_from_ini, initial_state, its refusals), the host
 // initial conditions and Rankine-Hugoniot states (g5_init.cpp), the seeded workload specs
-// (g5_specs.cpp) and the HDF5 checkpoint round trip (g5_hdf5.cpp).  Prints checksums so that a
-// sanitized build can be compared with a plain one.
+// (g5_specs.cpp), the two-level mesh refusals and the HDF5 checkpoint round trip (g5_hdf5.cpp).
+// Prints checksums so that a sanitized build can be compared with a plain one.
 //   host_cpu <golden dir> <scratch dir>
 #include <cstdio>
 #include <string>
@@ -65,6 +65,18 @@ main(int argc, char ** argv)
   bad += refused(base, "material1", "mg_eos_name", "MG_JWL");
   bad += refused(base, "amr", "time_integrator", "RK3");
   bad += refused(base, "amr", "level_max", "5");
+  // a two-level mesh whose refined block touches the brick's border is refused before any device
+  // allocation
+  try
+  {
+    g5_config c2 = g5cpp::config_from_ini(base);
+    g5cpp::GodunovImplemHip<2> impl(c2, { 0, 0 }, 1);
+    bad += 1;
+  }
+  catch (std::exception const & e)
+  {
+    std::printf("refused two-level mesh: %s\n", e.what());
+  }
   // workload specs and their configurations / region states
   double spec_sum = 0.0;
   for (int k = 0; k < 256; ++k)

@@ -91,4 +91,4 @@ def test_host_cpp_asan_ubsan(binaries, tmp_path):
     assert rc == 0, err[-3000:]
     rc0, out0, _ = _run(binaries["hc_plain"], [gold, str(tmp_path / "p")])
     assert rc0 == 0 and out == out0
-    assert out.count("refused ") == 5 and "h5_round_trip=1" in out
+    assert out.count("refused ") == 6 and "h5_round_trip=1" in out
