@@ -3,6 +3,11 @@
 Workloads (--workload):
   air_water_3d  (default) 3-D air-water shock-bubble with THINC at 256^3 cells (4096 blocks of
                 16^3), FP64, HANCOCK, HLLC, CFL 0.5, 50 steps: the plain FP64 THINC configuration.
+                The default arithmetic is the reassociated FP64 path (--precision fp64_fast:
+                relative L1 <= 1e-12 per conserved variable against the reference's operation
+                order); the line also carries the bit-exact path's rate on the same steps and the
+                relative L1 between the two final states (--precision fp64 times the bit-exact
+                path alone).
   air_helium_2d 2-D air-helium shock-bubble, 512x128 cells, FP64, minmod, CFL 0.8, 200 steps (the
                 reference's CPU-runnable case; the state fits in the last-level cache).
   mixed_3d      the air-water THINC case at 512x256x256 cells in mixed precision, 100 steps, with
@@ -95,6 +100,20 @@ def run_context(g5, cfg, U0, steps, warmup):
     return wall, ev_ms, U, n
 
 
+def l1_vs(U, Uref, nvar, ncell, dim):
+    """per-variable relative L1 of U against Uref; the momentum components against the momentum
+    vector's L1 norm (a transverse component can be round-off by itself)"""
+    A = U.reshape(-1, nvar, ncell)
+    B = Uref.reshape(-1, nvar, ncell)
+    mom = list(range(2, 2 + dim))
+    mden = sum(float(np.abs(B[:, m]).sum()) for m in mom)
+    out = []
+    for v in range(nvar):
+        den = mden if v in mom else float(np.abs(B[:, v]).sum())
+        out.append(float(np.abs(A[:, v] - B[:, v]).sum()) / max(den, 1e-300))
+    return out
+
+
 def bench_single(g5, args, name):
     import cases
 
@@ -115,8 +134,19 @@ def bench_single(g5, args, name):
     out["config"] = {"workload": wl, "cells": cells, "grid": grid, "blocks": int(np.prod(cfg.nb[:cfg.dim])),
                      "block": [16] * cfg.dim, "precision": args.precision, "integrator": "HANCOCK", "riemann": "HLLC",
                      "thinc": bool(cfg.thinc_enabled), "cfl": cfg.cfl}
+    suffix = "" if args.precision == "fp64" else "_" + args.precision
     out["roofline"] = roofline(cells, nvar, ev_ms, traffic_from_profile(
-        f"step3d_air_water_thinc_{grid[0]}" if cfg.dim == 3 else "step2d_air_helium_512x128"))
+        (f"step3d_air_water_thinc_{grid[0]}" if cfg.dim == 3 else "step2d_air_helium_512x128") + suffix))
+    if args.precision != "fp64" and not args.no_exact:
+        # the bit-exact path on the same steps: its rate, and the distance of the two final states
+        cfg.precision = PRECISIONS["fp64"]
+        wall_x, ev_x, U_x, n_x = run_context(g5, cfg, U0, args.steps, args.warmup)
+        l1 = l1_vs(U, U_x, nvar, 16 ** cfg.dim, cfg.dim)
+        out["fp64_exact"] = {"value": cells * n_x / wall_x, "ms_per_step": wall_x * 1e3 / n_x,
+                             "kernel_ms_per_step": ev_x}
+        out["config"]["rel_l1_vs_fp64_exact"] = {"max": max(l1), "per_variable": l1,
+                                                 "momentum_norm": "momentum components against the "
+                                                                  "momentum vector's L1 norm"}
     if not args.no_cpu:
         k = args.cpu_steps if cfg.dim == 3 else 200
         cu, dt, th = cpu_rate(cfg, U0, k)
@@ -182,21 +212,26 @@ def bench_batch(g5, args):
     pool.prepare()
     if args.warmup > 0:
         pool.step(args.warmup)
-    pool.status(ids[0])
+    before = [pool.progress(i)[1] for i in ids]  # synchronises
     t0 = time.perf_counter()
     pool.step(steps)
-    done = pool.status(ids[0])  # synchronises
+    pool.progress(ids[0])  # synchronises
     wall = time.perf_counter() - t0
     ev_ms = pool.last_step_ms() / steps
     launch = pool.launch()
-    # every solve advanced by warmup + steps (each has a 100-step budget: the timed loop ends
-    # exactly at the budget with the default warm-up of 0)
-    sample = list(range(0, n, max(1, n // 64)))
+    # a solve whose state leaves the mixture EOS's domain (a cell with c^2 < 0, i.e. a negative
+    # pressure) is stopped there in every precision: count the cell-updates done, per solve
+    after = [pool.progress(i) for i in ids]
+    failed = [k for k, a in enumerate(after) if a[3]]
+    done_updates = sum(256 * s.nb[0] * s.nb[1] * (a[1] - b) for s, a, b in zip(specs, after, before))
+    # every healthy solve advanced by warmup + steps (each has a 100-step budget: the timed loop
+    # ends exactly at the budget with the default warm-up of 0)
+    sample = [k for k in range(0, n, max(1, n // 64)) if k not in failed]
     results = {k: pool.download(ids[k]) for k in sample}
     finite = all(np.isfinite(u).all() for u in results.values())
-    stepped = [pool.status(i)[1] for i in ids]
+    stepped = [after[k][1] for k in range(n) if k not in failed]
     pool.close()
-    assert done is not None and finite
+    assert finite
     l1 = None
     if prec == g5.PREC_MIXED:
         # accuracy at this speed: the sampled solves rerun in FP64 (bit-identical to the oracle)
@@ -216,12 +251,15 @@ def bench_batch(g5, args):
         den[2] = den[3] = mom
         names = ["rho", "E", "rho_u", "rho_v", "alpha0", "alpha_rho0", "alpha1", "alpha_rho1"]
         l1 = dict(zip(names, (num / den).tolist()))
-    out = base_line(args, cells * steps / wall, wall * 1e3 / steps, steps, "mixed-f32-f64" if prec else "f64")
+    out = base_line(args, done_updates / wall, wall * 1e3 / steps, steps, "mixed-f32-f64" if prec else "f64")
     out["config"] = {"workload": "batch_2d_seeded", "seed": SEED, "solves": n, "cells": cells,
                      "precision": "mixed" if prec else "fp64", "block": [16, 16], "blocks": cells // 256,
-                     "steps_per_solve": min(stepped), "admission_s": t_adm, "launch": launch,
+                     "steps_per_solve": min(stepped), "cell_updates": done_updates,
+                     "failed_solves": len(failed), "failed_first": failed[:8],
+                     "admission_s": t_adm, "launch": launch,
                      "hip_graph": True, "rel_l1_vs_fp64_sample": l1, "l1_sample_solves": len(sample)}
-    out["roofline"] = roofline(cells, 8, ev_ms, traffic_from_profile("step2d_batch_mixed" if prec else "step2d_batch_fp64"))
+    out["roofline"] = roofline(done_updates / steps, 8, ev_ms,
+                               traffic_from_profile("step2d_batch_mixed" if prec else "step2d_batch_fp64"))
     if not args.no_cpu:
         # bounded sample: the first solves of the same batch (device-generated initial states)
         cu = dt = 0.0
@@ -248,7 +286,8 @@ def bench_serve(g5, args):
     prec = g5.PREC_FP64 if args.fp64 else g5.PREC_MIXED
     rep, lat = g5.serve(SEED, args.requests, args.rate, frac_3d=0.1, share_3d=0.3, budget_bytes=200 << 30,
                         precision=prec, steps_per_poll=args.poll)
-    assert rep["completed"] == args.requests
+    # a request whose state leaves the mixture EOS's domain (c^2 < 0) is stopped and counted as failed
+    assert rep["completed"] + rep["failed"] == args.requests and rep["rejected"] == 0, rep
     out = base_line(args, rep["cell_updates_per_s"], None, None, "mixed-f32-f64" if prec else "f64")
     out["higher_is_better"] = True
     out["config"] = {"workload": "serve_poisson", "seed": SEED, "requests": args.requests, "rate_hz": args.rate,
@@ -256,6 +295,7 @@ def bench_serve(g5, args):
                      "steps_per_poll": args.poll, "p50_ms": rep["p50_ms"], "p99_ms": rep["p99_ms"],
                      "mean_ms": rep["mean_ms"], "max_ms": rep["max_ms"], "wall_s": rep["wall_s"],
                      "max_inflight": rep["max_inflight"], "cell_updates": rep["cell_updates"],
+                     "completed": rep["completed"], "failed": rep["failed"],
                      "last_arrival_s": rep["last_arrival_s"]}
     out["roofline"] = None
     return out
@@ -276,9 +316,11 @@ def main():
     ap.add_argument("--nb", type=int, default=16, help="blocks per direction of the 3-D cases (16 -> 256)")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-exact", action="store_true",
+                    help="fp64_fast: skip the bit-exact companion run (profiling runs of the fast kernel alone)")
     ap.add_argument("--fp64", action="store_true", help="batch/serve: FP64 instead of mixed precision")
-    ap.add_argument("--precision", default="fp64", choices=sorted(PRECISIONS),
-                    help="air_water_3d / air_helium_2d: bit-exact FP64 or reassociated FP64")
+    ap.add_argument("--precision", default="fp64_fast", choices=sorted(PRECISIONS),
+                    help="air_water_3d / air_helium_2d: reassociated FP64 (default) or bit-exact FP64")
     ap.add_argument("--nsolves", type=int, default=4096)
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--rate", type=float, default=160.0)

@@ -707,6 +707,14 @@ class Pool:
         self._check(lib().g5_pool_status(self._h, sid, ctypes.byref(t), ctypes.byref(n), ctypes.byref(a)))
         return t.value, n.value, bool(a.value)
 
+    def progress(self, sid):
+        """(t, steps, active, G5_FAIL_* bits) of a solve, failed or not (no exception)"""
+        t, n, a = ctypes.c_double(), ctypes.c_int32(), ctypes.c_int32()
+        rc = lib().g5_pool_status(self._h, sid, ctypes.byref(t), ctypes.byref(n), ctypes.byref(a))
+        if rc not in (G5_OK, G5_ERR_DIVERGED):
+            self._check(rc)
+        return t.value, n.value, bool(a.value), self.failure(sid)
+
     def failure(self, sid):
         """G5_FAIL_* bits of the solve (0: healthy)"""
         f = ctypes.c_int32()

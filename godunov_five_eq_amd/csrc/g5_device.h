@@ -1364,15 +1364,18 @@ cell_rate(SolveDesc const & S, Cons const & u, double & p)
     return cell_inv_dt_mixed<DIM>(S, u, p);
 }
 
-// G5_FAIL_* bits of one cell of a new state: a conservative value or a CFL rate that is not finite
-// (the conversion clamps the density and the volume fraction with fmax/fmin, which drop a NaN, so
-// the state itself is tested: the difference s - s of the sum of its values is 0 only when the sum
-// is finite; NaN compares false), and with the solve's check a negative pressure
+// G5_FAIL_* bits of one cell of a new state: a conservative value that is not finite (the
+// conversion clamps the density and the volume fraction with fmax/fmin, which drop a NaN, so the
+// state itself is tested: the difference s - s of the sum of its values is 0 only when the sum is
+// finite; NaN compares false), and with the solve's check a negative pressure.  A cell whose
+// sound speed is not real (c^2 < 0: the mixture EOS outside its domain, which needs p < 0) has a
+// NaN CFL rate; the max reductions drop it, as the reference's do, and the solve goes on unless
+// the negative-pressure check is on (a solve with no finite rate at all fails at its dt)
 G5D uint32_t
-cell_failure(SolveDesc const & S, Cons const & u, double rate, double p)
+cell_failure(SolveDesc const & S, Cons const & u, double, double p)
 {
   const double sum = ((u.d + u.e) + (u.mu + u.mv) + (u.mw + u.a0)) + ((u.ar0 + u.a1) + u.ar1);
-  const bool   ok = sum - sum == 0.0 and rate <= 1.7976931348623157e308;
+  const bool   ok = sum - sum == 0.0;
   uint32_t     f = ok ? 0u : static_cast<uint32_t>(G5_FAIL_NONFINITE);
   if (S.abort_neg_p != 0 and p < 0.0)
     f |= static_cast<uint32_t>(G5_FAIL_NEGATIVE_PRESSURE);

@@ -3,6 +3,15 @@
 
     python -m godunov_five_eq_amd.rocprof_summary stats DIR           # markdown table
     python -m godunov_five_eq_amd.rocprof_summary pmc DIR COUNTER     # json {kernel: mean}
+    python -m godunov_five_eq_amd.rocprof_summary traffic KERNEL CELLS NVAR FETCH WRITE CAL_FETCH \
+        CAL_WRITE [REQ CAL_REQ]                                        # json: HBM bytes per launch
+
+traffic: FETCH_SIZE and WRITE_SIZE (KiB per dispatch, one --pmc run each) of the kernel whose name
+contains KERNEL, each scaled by what the same counter reads for a plain device copy of known size
+(godunov_five_eq_amd.calibrate: 1 GiB read and written per launch), against the algorithmic
+2 * NVAR * 8 B per cell.  With REQ / CAL_REQ (a --pmc run of TCC_EA0_RDREQ_32B, TCC_EA0_RDREQ_64B
+and TCC_EA0_RDREQ_128B over the kernel and over the copy), the read bytes are cross-checked by the
+memory read requests counted by size, scaled the same way by the copy.
 """
 import csv
 import glob
@@ -40,7 +49,53 @@ def counter_means(directory, counter):
     return {k: (n, s / n) for k, (n, s) in acc.items()}
 
 
+def _one(directory, counter, kernel):
+    hits = [(k, v) for k, v in counter_means(directory, counter).items() if kernel in k]
+    if not hits:
+        raise SystemExit(f"no dispatch of a kernel containing {kernel!r} for {counter} in {directory}")
+    return hits[0]
+
+
+def _req_bytes(directory, kernel):
+    sizes = {"TCC_EA0_RDREQ_32B": 32, "TCC_EA0_RDREQ_64B": 64, "TCC_EA0_RDREQ_128B": 128}
+    out = {}
+    for c, b in sizes.items():
+        _, (n, v) = _one(directory, c, kernel)
+        out[c] = v
+    return out, sum(out[c] * b for c, b in sizes.items())
+
+
+def traffic(kernel, cells, nvar, fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, req_dir=None, cal_req_dir=None,
+            copy_kernel="copyBuffer", copy_bytes=1 << 30):
+    name, (nf, fetch) = _one(fetch_dir, "FETCH_SIZE", kernel)
+    _, (nw, write) = _one(write_dir, "WRITE_SIZE", kernel)
+    _, (_, cfetch) = _one(cal_fetch_dir, "FETCH_SIZE", copy_kernel)
+    _, (_, cwrite) = _one(cal_write_dir, "WRITE_SIZE", copy_kernel)
+    fs, ws = copy_bytes / (cfetch * 1024), copy_bytes / (cwrite * 1024)
+    rd, wr = fetch * 1024 * fs, write * 1024 * ws
+    alg = 2 * nvar * 8 * cells
+    out = {"kernel": name, "fetch_kib_per_dispatch": fetch, "write_kib_per_dispatch": write, "dispatches": [nf, nw],
+           "copy_calibration": {"bytes_read": copy_bytes, "bytes_written": copy_bytes, "fetch_kib": cfetch,
+                                "write_kib": cwrite, "fetch_scale": fs, "write_scale": ws},
+           "bytes_per_launch": rd + wr, "algorithmic_bytes_per_launch": alg, "traffic_over_algorithmic": (rd + wr) / alg,
+           "read_bytes_per_cell": rd / cells, "write_bytes_per_cell": wr / cells}
+    if req_dir and cal_req_dir:
+        counts, rb = _req_bytes(req_dir, kernel)
+        ccounts, crb = _req_bytes(cal_req_dir, copy_kernel)
+        rscale = copy_bytes / crb
+        out["read_cross_check"] = {"counters": counts, "copy_counters": ccounts, "request_bytes": rb,
+                                   "copy_request_bytes": crb, "copy_scale": rscale,
+                                   "read_bytes_per_launch": rb * rscale,
+                                   "read_bytes_per_cell": rb * rscale / cells,
+                                   "over_fetch_size_estimate": rb * rscale / rd}
+    return out
+
+
 def main(argv):
+    if argv[0] == "traffic":
+        kernel, cells, nvar = argv[1], int(argv[2]), int(argv[3])
+        print(json.dumps(traffic(kernel, cells, nvar, *argv[4:]), indent=1))
+        return
     if argv[0] == "stats":
         print("| kernel | calls | total ms | average us | % |")
         print("|---|---:|---:|---:|---:|")

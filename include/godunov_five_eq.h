@@ -42,10 +42,12 @@ enum
 };
 
 /* why a solve was stopped (bits of g5_pool_failure).  The step kernels and the dt reduction check
- * every owned cell of each new state: a CFL rate that is not finite (NaN or infinite state, zero
- * density) or an infinite dt sets G5_FAIL_NONFINITE; with g5_config.abort_negative_pressure a
- * negative mixture pressure sets G5_FAIL_NEGATIVE_PRESSURE ([hydro] abort_when_negative_pressure,
- * models/HydroSettings.cpp:28).  The solve then stops stepping; its co-batched solves go on. */
+ * every owned cell of each new state: a value that is not finite, or a solve without any finite
+ * CFL rate (an infinite or undefined dt), sets G5_FAIL_NONFINITE; with
+ * g5_config.abort_negative_pressure a negative mixture pressure sets G5_FAIL_NEGATIVE_PRESSURE
+ * ([hydro] abort_when_negative_pressure, models/HydroSettings.cpp:28).  A cell whose sound speed is
+ * not real (c^2 < 0, which needs p < 0) only drops out of the CFL maximum, as in the reference.
+ * The solve then stops stepping; its co-batched solves go on. */
 enum
 {
   G5_FAIL_NONFINITE = 1,
