@@ -396,6 +396,7 @@ def rankine_hugoniot(gamma, pinf, rho1, p1, mach=0.0, p2=0.0):
 
 
 PROLONG_SIMPLE_COPY, PROLONG_EXTRAPOLATE_LINEAR_MINMOD = 0, 1
+PROLONG_CONSERVATIVE_ORDER_2, PROLONG_CONSERVATIVE_ORDER_4 = 2, 3
 
 
 class Context:

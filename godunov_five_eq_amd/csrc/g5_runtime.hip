@@ -425,7 +425,8 @@ tables_two_level(SolveRec & r)
 }
 
 // admission of a solve; refined (two coordinates per refined coarse block), when given, makes it
-// a two-level solve with the given prolongation (0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD)
+// a two-level solve with the given prolongation (0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD,
+// 2 / 3 CONSERVATIVE_INTERPOLATION_ORDER_2 / _4)
 int
 pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, int32_t * id,
            std::vector<int32_t> const * refined = nullptr, int prolong = 0)
@@ -437,8 +438,8 @@ pool_admit(g5_pool * p, g5_config const & cfg, double tend, int32_t max_steps, i
   SolveRec tl; // two-level layout
   if (refined)
   {
-    if (prolong != 0 and prolong != 1)
-      return fail(p->err, G5_ERR_INVALID, "prolongation must be 0 (SIMPLE_COPY) or 1 (EXTRAPOLATE_LINEAR_MINMOD)");
+    if (prolong < 0 or prolong > 3)
+      return fail(p->err, G5_ERR_INVALID, "unknown prolongation");
     if ((rc = layout_two_level(cfg, *refined, tl, msg)) != G5_OK)
       return fail(p->err, rc, msg);
   }
@@ -1681,8 +1682,9 @@ g5_create_two_level(const g5_config * cfg, const int32_t * refined_coords, int32
 {
   if (not refined_coords or n_refined < 1)
     return fail(g_create_error, G5_ERR_INVALID, "no refined blocks");
-  if (prolongation != 0 and prolongation != 1)
-    return fail(g_create_error, G5_ERR_INVALID, "prolongation must be 0 (SIMPLE_COPY) or 1 (EXTRAPOLATE_LINEAR_MINMOD)");
+  if (prolongation < 0 or prolongation > 3)
+    return fail(g_create_error, G5_ERR_INVALID, "prolongation must be 0 (SIMPLE_COPY), 1 (EXTRAPOLATE_LINEAR_MINMOD), "
+                                                "2 or 3 (CONSERVATIVE_INTERPOLATION_ORDER_2 / _4)");
   const std::vector<int32_t> refined(refined_coords, refined_coords + 2 * static_cast<size_t>(n_refined));
   return create_ctx(cfg, &refined, prolongation, out);
 }

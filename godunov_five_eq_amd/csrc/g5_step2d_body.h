@@ -316,14 +316,82 @@ coarse_cell(SolveDesc const & S, int32_t const * __restrict__ slots, double cons
   return sl >= 0 ? cell_of(U, sl, gx & 15, gy & 15) : restrict_cell(S, slots, U, gx, gy);
 }
 
-// fine cell (gx, gy) outside the fine blocks: its coarse parent copied (S.prolong 0, SIMPLE_COPY)
-// or extrapolated to the fine cell's side with minmod slopes of the conservative variables
+// variable at offset var of the 8-variable layout of level-0 cell (gx, gy): a coarse leaf's value,
+// or the average of the four fine cells where the block is refined (summed x fastest, over 4)
+G5D double
+coarse_var(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int gx, int gy,
+           int var)
+{
+  const int sl = slot_l0(S, slots, gx >> 4, gy >> 4);
+  if (sl >= 0)
+    return U[static_cast<int64_t>(sl) * 8 * 256 + var * 256 + (gx & 15) + B2 * (gy & 15)];
+  double sum = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+  {
+    const int    fx = 2 * gx + (k & 1), fy = 2 * gy + (k >> 1);
+    const int    fs = slot_l1(S, slots, fx >> 4, fy >> 4);
+    const double v = U[static_cast<int64_t>(fs) * 8 * 256 + var * 256 + (fx & 15) + B2 * (fy & 15)];
+    sum = k == 0 ? v : sum + v;
+  }
+  return sum / 4.0;
+}
+
+// CONSERVATIVE_INTERPOLATION_ORDER_2 / _4 (S.prolong 2 / 3; conservative_interpolation_order{2,4}
+// and interpol_order{2,4}, scheme/ConvertToPrimitivesVariablesFunctor.cpp:339-490,751-827): the
+// fine cell's half of its parent (cx, cy) along x and y, by 1-D conservative interpolations over
+// the parent's 3x3 (5x5) coarse neighbourhood, x first, then y.  Weights: the averages over the
+// left half of the polynomial of degree 2 (4) with the given cell averages, (1/8, 1, -1/8) and
+// (-3/128, 22/128, 1, -22/128, 3/128), mirrored for the right half; sums in stencil order.
+G5D double
+conservative_var(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int cx,
+                 int cy, int px, int py, int var)
+{
+  const double w2[3] = { 0.125, 1.0, -0.125 };
+  const double w4[5] = { -3.0 / 128.0, 22.0 / 128.0, 1.0, -22.0 / 128.0, 3.0 / 128.0 };
+  const bool   o2 = S.prolong == 2;
+  const int    m = o2 ? 1 : 2, n = 2 * m + 1;
+  double       valy = 0.0;
+  for (int ky = 0; ky < n; ++ky)
+  {
+    double valx = 0.0;
+    for (int kx = 0; kx < n; ++kx)
+    {
+      const int    ix = px == 0 ? kx : n - 1 - kx;
+      const double x = (o2 ? w2[ix] : w4[ix]) * coarse_var(S, slots, U, cx + kx - m, cy + ky - m, var);
+      valx = kx == 0 ? x : valx + x;
+    }
+    const int    iy = py == 0 ? ky : n - 1 - ky;
+    const double wy = o2 ? w2[iy] : w4[iy];
+    valy = ky == 0 ? wy * valx : valy + wy * valx;
+  }
+  return valy;
+}
+
+// fine cell (gx, gy) outside the fine blocks: its coarse parent copied (S.prolong 0, SIMPLE_COPY),
+// extrapolated to the fine cell's side with minmod slopes of the conservative variables
 // (S.prolong 1, EXTRAPOLATE_LINEAR_MINMOD: linear_extrapolate_using_limited_slopes,
-// scheme/ConvertToPrimitivesVariablesFunctor.cpp:553-770, fill_ghosts :889-910)
+// scheme/ConvertToPrimitivesVariablesFunctor.cpp:553-770, fill_ghosts :889-910), or
+// conservatively interpolated (S.prolong 2, 3)
 G5D Cons
 prolong_cell(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int gx, int gy)
 {
-  const int  cx = gx >> 1, cy = gy >> 1; // fine cells of the ghost ring are at positive indices
+  const int cx = gx >> 1, cy = gy >> 1; // fine cells of the ghost ring are at positive indices
+  if (S.prolong >= 2)
+  {
+    const int px = gx & 1, py = gy & 1;
+    Cons      u;
+    u.ar0 = conservative_var(S, slots, U, cx, cy, px, py, 5);
+    u.ar1 = conservative_var(S, slots, U, cx, cy, px, py, 7);
+    u.a0 = conservative_var(S, slots, U, cx, cy, px, py, 4);
+    u.e = conservative_var(S, slots, U, cx, cy, px, py, 1);
+    u.mu = conservative_var(S, slots, U, cx, cy, px, py, 2);
+    u.mv = conservative_var(S, slots, U, cx, cy, px, py, 3);
+    u.mw = 0.0;
+    u.d = u.ar0 + u.ar1;
+    u.a1 = 1.0 - u.a0;
+    return u;
+  }
   const Cons c = coarse_cell(S, slots, U, cx, cy);
   if (S.prolong == 0)
     return c;

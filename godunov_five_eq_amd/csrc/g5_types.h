@@ -27,7 +27,8 @@ struct SolveDesc
   // two-level static mesh (2-D): slot tables of level 0 at base (nb[0] x nb[1], -1 where refined)
   // and of level 1 at base1 (2 nb[0] x 2 nb[1], -1 outside the refined blocks); base1 < 0: one level
   int32_t base1;
-  int32_t prolong; // coarse-to-fine ghost fill: 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD
+  int32_t prolong; // coarse-to-fine ghost fill: 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD,
+                   // 2 / 3 CONSERVATIVE_INTERPOLATION_ORDER_2 / _4
   double  dx;
   double  cfl;
   double  slope_type;

@@ -185,7 +185,8 @@ int g5_create(const g5_config * cfg, g5_ctx ** out);
  * brick's border.  The reference's AMR paths on such a mesh: the coarse/fine ghost fill of
  * ConvertToPrimitivesVariablesFunctor::fill_ghosts (scheme/ConvertToPrimitivesVariablesFunctor.cpp:
  * 884-918) -- restriction (:214-250) of a finer neighbour, prolongation of a coarser one,
- * prolongation 0 SIMPLE_COPY or 1 EXTRAPOLATE_LINEAR_MINMOD (:553-770) -- the fine-face flux sums
+ * prolongation 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD (:553-770), 2 or 3
+ * CONSERVATIVE_INTERPOLATION_ORDER_2 / _4 (:339-490,751-827) -- the fine-face flux sums
  * of read_fluxes_and_update_2d (scheme/ReadFluxesAndConservativeUpdateFunctor.cpp:137-199,
  * .h:256-307) and the per-level cell length in the fluxes and the CFL rate
  * (ComputeFluxesAndStoreFunctor, ComputeDtFiveEqFunctor).  The state holds the blocks in the order

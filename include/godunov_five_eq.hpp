@@ -412,7 +412,8 @@ public:
   }
 
   // a two-level static mesh (2-D): the coarse blocks at refined[2k], refined[2k+1] refined into
-  // 2 x 2 blocks of half the cell length; prolongation 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD
+  // 2 x 2 blocks of half the cell length; prolongation 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD,
+  // 2 / 3 CONSERVATIVE_INTERPOLATION_ORDER_2 / _4
   // (g5_create_two_level; the state's block order is block_layout())
   GodunovImplemHip(g5_config const & cfg, std::vector<int32_t> const & refined, int prolongation)
   {

@@ -591,7 +591,7 @@ struct Solver
   // blocks of half the cell length, at least one block away from the brick's border).  bcoord
   // holds a block's coordinates at its own level; refined coarse positions are -1 in lookup.
   bool                 amr = false;
-  int                  prolong = 0;  // 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD
+  int                  prolong = 0;  // 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD, 2/3 conservative order 2/4
   std::vector<int32_t> blev;         // level of every block (outside blocks: 0)
   std::vector<int32_t> lookup1;      // fine grid (2 nb)^dim -> owned block index or -1
   std::vector<char>    refined0;     // coarse grid nb^dim: refined
@@ -896,6 +896,8 @@ struct Solver
     const State center = cons_level(A, 0, gc);
     if (prolong == 0)
       return center;
+    if (prolong >= 2)
+      return conservative_interpolation(A, g, gc);
     State slope[3];
     for (int d = 0; d < dim; ++d)
     {
@@ -918,6 +920,62 @@ struct Solver
         val = center[v] + 0.25 * static_cast<double>(ix[0]) * slope[0][v] +
               0.25 * static_cast<double>(ix[1]) * slope[1][v] + 0.25 * static_cast<double>(ix[2]) * slope[2][v];
       u[v] = val;
+    }
+    u[H.ID] = u[H.IAD0] + u[H.IAD1];
+    u[H.IA1] = 1.0 - u[H.IA0];
+    return u;
+  }
+
+  // CONSERVATIVE_INTERPOLATION_ORDER_2 / _4 (prolong 2 / 3; conservative_interpolation_order{2,4}
+  // and interpol_order{2,4}, ConvertToPrimitivesVariablesFunctor.cpp:339-490,751-827): the fine
+  // cell's half of its coarse parent along each direction (position 0 left, 1 right), by the
+  // tensor product of 1-D conservative interpolations over the parent's 3^dim (5^dim) coarse
+  // neighbourhood, x first, then y (then z).  The weights are the averages over the left half of
+  // the polynomial of degree 2 (4) with the given cell averages: (1/8, 1, -1/8) and
+  // (-3/128, 22/128, 1, -22/128, 3/128), mirrored for the right half (the absent core's
+  // coefficient tables restated from that definition); each combination sums in stencil order.
+  State
+  conservative_interpolation(std::vector<double> & A, int const g[3], int const gc[3])
+  {
+    static const double w2[3] = { 0.125, 1.0, -0.125 };
+    static const double w4[5] = { -3.0 / 128.0, 22.0 / 128.0, 1.0, -22.0 / 128.0, 3.0 / 128.0 };
+    const int      m = prolong == 2 ? 1 : 2, n = 2 * m + 1;
+    const double * wl = prolong == 2 ? w2 : w4;
+    int            pos[3] = { 0, 0, 0 };
+    for (int d = 0; d < dim; ++d)
+      pos[d] = g[d] - 2 * gc[d];
+    auto w = [&](int d, int k) { return pos[d] == 0 ? wl[k] : wl[n - 1 - k]; };
+    // the coarse states of the stencil, once
+    std::vector<State> cs(static_cast<size_t>(n) * n * (dim == 3 ? n : 1));
+    for (int kz = 0; kz < (dim == 3 ? n : 1); ++kz)
+      for (int ky = 0; ky < n; ++ky)
+        for (int kx = 0; kx < n; ++kx)
+        {
+          const int c[3] = { gc[0] + kx - m, gc[1] + ky - m, dim == 3 ? gc[2] + kz - m : 0 };
+          cs[kx + n * (ky + n * kz)] = cons_level(A, 0, c);
+        }
+    const int vars[7] = { H.IAD0, H.IAD1, H.IA0, H.IE, H.IU, H.IV, H.IW };
+    State     u{};
+    for (int t = 0; t < (dim == 3 ? 7 : 6); ++t)
+    {
+      const int v = vars[t];
+      double    valz = 0.0;
+      for (int kz = 0; kz < (dim == 3 ? n : 1); ++kz)
+      {
+        double valy = 0.0;
+        for (int ky = 0; ky < n; ++ky)
+        {
+          double valx = 0.0;
+          for (int kx = 0; kx < n; ++kx)
+          {
+            const double x = w(0, kx) * cs[kx + n * (ky + n * kz)][v];
+            valx = kx == 0 ? x : valx + x;
+          }
+          valy = ky == 0 ? w(1, ky) * valx : valy + w(1, ky) * valx;
+        }
+        valz = dim == 3 ? (kz == 0 ? w(2, kz) * valy : valz + w(2, kz) * valy) : valy;
+      }
+      u[v] = valz;
     }
     u[H.ID] = u[H.IAD0] + u[H.IAD1];
     u[H.IA1] = 1.0 - u[H.IA0];
@@ -1820,7 +1878,8 @@ g5o_create(const g5o_params * p)
 }
 
 // a two-level mesh: cfg's brick at level 0 with the n coarse blocks at coords[3k..3k+2] refined into
-// 2^dim blocks of half the cell length; prolongation 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD
+// 2^dim blocks of half the cell length; prolongation 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD,
+// 2 CONSERVATIVE_INTERPOLATION_ORDER_2, 3 CONSERVATIVE_INTERPOLATION_ORDER_4
 void *
 g5o_create_amr(const g5o_params * p, const int32_t * coords, int32_t n, int32_t prolongation)
 {

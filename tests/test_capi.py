@@ -149,7 +149,7 @@ def test_two_level_mesh_refusals(g5):
         (g5.make_config(2, (4, 4), 0.1), [[0, 1]], 0, g5.G5_ERR_INVALID),
         (g5.make_config(2, (4, 4), 0.1), [[1, 3]], 0, g5.G5_ERR_INVALID),
         (g5.make_config(2, (4, 4), 0.1), [[1, 1], [1, 1]], 0, g5.G5_ERR_INVALID),
-        (g5.make_config(2, (4, 4), 0.1), [[1, 1]], 2, g5.G5_ERR_INVALID),
+        (g5.make_config(2, (4, 4), 0.1), [[1, 1]], 4, g5.G5_ERR_INVALID),
         (g5.make_config(2, (4, 4), 0.1), [], 0, g5.G5_ERR_INVALID),
     ]
     for cfg, refined, prolong, code in cases2:

@@ -4,7 +4,9 @@ fine-face flux sums at coarse faces, ReadFluxesAndConservativeUpdateFunctor.h:24
 NEIGHBOR_IS_FINER branches of .cpp:137-199), pinned by
   - closed forms: on a field linear in every conservative variable the restriction of a coarse
     cell is its centre value (the mean of the 2^dim fine cells), minmod prolongation reproduces
-    the fine cells' values and SIMPLE_COPY the coarse value;
+    the fine cells' values and SIMPLE_COPY the coarse value; the conservative interpolations of
+    order 2 and 4 reproduce the fine cell averages of any polynomial of degree <= 2 (<= 4) in each
+    variable, and the four children of a coarse cell average back to it (any field);
   - invariants: a uniform state stays uniform bit for bit across the levels, and the totals of
     mass, energy and momentum (cell values times the level's cell volume) are conserved to
     round-off in a periodic box whose refined region is L-shaped (so coarse cells meet fine ones
@@ -93,7 +95,7 @@ def test_restriction_and_prolongation_on_a_linear_field(g5, orc, prolongation):
     assert u[0] == u[5] + u[7]
 
 
-@pytest.mark.parametrize("prolongation", [0, 1])
+@pytest.mark.parametrize("prolongation", [0, 1, 2, 3])
 def test_uniform_state_is_preserved(g5, orc, prolongation):
     cfg = periodic_cfg(g5)
     o = orc.Oracle(cfg, refined=REFINED, prolongation=prolongation)
@@ -103,7 +105,7 @@ def test_uniform_state_is_preserved(g5, orc, prolongation):
     assert np.array_equal(o.get_state(), U0)
 
 
-@pytest.mark.parametrize("prolongation", [0, 1])
+@pytest.mark.parametrize("prolongation", [0, 1, 2, 3])
 def test_conservation_across_levels(g5, orc, prolongation):
     cfg = periodic_cfg(g5)
     o = orc.Oracle(cfg, refined=REFINED, prolongation=prolongation)
@@ -120,3 +122,58 @@ def test_conservation_across_levels(g5, orc, prolongation):
         assert abs(T1[v] - T0[v]) <= 2e-15 * abs(T0[v]), (v, T0[v], T1[v])
     # the pulse has crossed into and out of the fine region
     assert np.abs(U - U0).max() > 1e-3
+
+
+def _mono_avg(a, lo, hi):
+    return (hi ** (a + 1) - lo ** (a + 1)) / ((a + 1) * (hi - lo))
+
+
+def _poly_avg(poly, x0, x1, y0, y1):
+    return sum(c * _mono_avg(a, x0, x1) * _mono_avg(b, y0, y1) for (a, b), c in poly.items())
+
+
+POLY2 = {(0, 0): 1.0, (1, 0): 0.5, (0, 1): 0.25, (2, 0): 0.125, (1, 1): -0.0625, (0, 2): 0.03125, (2, 1): 0.01}
+POLY4 = {**POLY2, (3, 0): 0.002, (4, 0): -1e-4, (0, 4): 2e-4, (2, 2): -3e-4, (3, 1): 1e-4, (4, 3): 1e-7}
+
+
+@pytest.mark.parametrize("prolongation,poly", [(2, POLY2), (3, POLY4)])
+def test_conservative_interpolation_is_exact_on_polynomials(g5, orc, prolongation, poly):
+    # every cell holds the exact average of a polynomial over the cell (coordinates in coarse
+    # cells); the interpolated fine cell then holds the fine cell's exact average
+    cfg = periodic_cfg(g5)
+    o = orc.Oracle(cfg, refined=REFINED, prolongation=prolongation)
+    lv, bc = o.block_levels(), o.block_coords()[: o.nowned]
+    scale = {1: 3.0, 2: 0.5, 3: -0.25, 4: 0.001, 5: 1.0, 7: 2.0}  # E, rho u, rho v, alpha0, IAD0, IAD1
+    U = np.ones((o.nowned, 8, 256))
+    for k in range(o.nowned):
+        h = 0.5 ** lv[k]
+        for c in range(256):
+            gx, gy = bc[k, 0] * 16 + c % 16, bc[k, 1] * 16 + c // 16
+            a = _poly_avg(poly, gx * h, (gx + 1) * h, gy * h, (gy + 1) * h)
+            for v, f in scale.items():
+                U[k, v, c] = f * a
+    o.set_state(U.ravel())
+    # fine cells outside the fine blocks, in their ghost rings: left of x = 32, below y = 32, and
+    # at the concave corner (coarse block (2, 2) is not refined: fine x, y >= 64 there)
+    # (a stored fine cell would keep ID = 1: the interpolated state has ID = IAD0 + IAD1)
+    for fx, fy in [(31, 40), (30, 41), (40, 31), (41, 30), (31, 31), (64, 70), (70, 64), (64, 64), (65, 65)]:
+        u = o.cons_level(1, (fx, fy))
+        ref = _poly_avg(poly, fx / 2, (fx + 1) / 2, fy / 2, (fy + 1) / 2)
+        for v, f in scale.items():
+            assert u[v] == pytest.approx(f * ref, rel=1e-12, abs=1e-12), (fx, fy, v)
+        assert u[0] == u[5] + u[7] and u[6] == 1.0 - u[4]
+
+
+@pytest.mark.parametrize("prolongation", [2, 3])
+def test_conservative_interpolation_children_average_to_the_parent(g5, orc, prolongation):
+    cfg = periodic_cfg(g5)
+    o = orc.Oracle(cfg, refined=REFINED, prolongation=prolongation)
+    rng = np.random.default_rng(7)
+    o.set_state(rng.uniform(0.5, 2.0, o.nowned * 8 * 256))
+    # coarse leaf cells next to the refined region, whose four children are fine ghost cells
+    for cx, cy in [(15, 20), (20, 15), (15, 15), (23, 40), (40, 23)]:
+        kids = [o.cons_level(1, (2 * cx + i, 2 * cy + j)) for j in (0, 1) for i in (0, 1)]
+        parent = o.cons_level(0, (cx, cy))
+        for v in (1, 2, 3, 4, 5, 7):
+            mean = (kids[0][v] + kids[1][v] + kids[2][v] + kids[3][v]) / 4.0
+            assert mean == pytest.approx(parent[v], rel=4e-16, abs=0.0), (cx, cy, v)
