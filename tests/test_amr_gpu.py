@@ -19,6 +19,7 @@ Refined regions are L-shaped, so coarse cells meet fine ones on faces of both or
 at a concave corner, and the moving features (a shock, an advected material blob) cross the level
 interface during the run.
 """
+import ctypes
 import sys
 
 import numpy as np
@@ -214,3 +215,86 @@ def test_fast_path_conservative_interpolation_smooth(g5, orc, prolongation):
     err = rel_l1(U_g, o.get_state(), momentum_norm=True)
     _announce(f"{label}: max rel L1 = {err.max():.3e}")
     assert err.max() <= RTOL_L1 and abs(t_g - t_o) <= RTOL_L1 * t_o, err
+
+
+def random_two_level(g5, seed):
+    """A seeded two-level configuration: a 2-D brick of 4-7 x 3-6 blocks with 1-5 refined blocks
+    (some adjacent) at least one block inside its border, per-face ZERO_GRADIENT / WALL or periodic
+    directions, THINC or MUSCL, minmod or MC, HANCOCK / RK1 / RK2_SSP, one of the four
+    prolongations, gases or a stiffened gas, sometimes the MILLER_PUCKETT source term; a
+    shock-bubble state sampled on both levels.  Returns (cfg, refined, prolongation, U0 on the
+    coarse and fine grids, steps, label)."""
+    rng = np.random.default_rng(1000 + seed)
+    nbx, nby = int(rng.integers(4, 8)), int(rng.integers(3, 7))
+    inner = [(x, y) for x in range(1, nbx - 1) for y in range(1, nby - 1)]
+    k = int(rng.integers(1, min(5, len(inner)) + 1))
+    refined = [list(inner[i]) for i in sorted(rng.choice(len(inner), size=k, replace=False))]
+    periodic = [int(rng.random() < 0.3), int(rng.random() < 0.3), 0]
+    bc = []
+    for d in range(2):
+        for _ in range(2):
+            bc.append("PERIODIC" if periodic[d] else ("WALL" if rng.random() < 0.5 else "ZERO_GRADIENT"))
+    bc += ["NONE", "NONE"]
+    thinc = bool(rng.random() < 0.5)
+    slope = 2.0 if rng.random() < 0.4 else 1.0
+    integrator = str(rng.choice(["HANCOCK", "HANCOCK", "RK1", "RK2_SSP"]))
+    prolong = int(rng.integers(0, 4))
+    water = bool(rng.random() < 0.3)
+    mp = bool(rng.random() < 0.25) or seed % 5 == 4
+    kw = dict(periodic=periodic, bc=tuple(bc), thinc=thinc, slope_type=slope, integrator=integrator,
+              cfl=float(rng.uniform(0.3, 0.8)), thinc_beta=float(rng.uniform(1.2, 2.0)))
+    if water:
+        kw.update(gamma=(1.4, 4.4), pinf=(0.0, 6e8))
+        rho2, u2, p2, _ = g5.rankine_hugoniot(4.4, 6e8, 1000.0, 1e5, p2=float(rng.uniform(2e8, 1e9)))
+        regs = [g5.region(0.0, (0.0, rho2), p2, u2, float(rng.uniform(-20, 20))), g5.region(0.0, (0.0, 1000.0), 1e5),
+                g5.region(1.0, (1.0, 0.0), 1e5)]
+    else:
+        kw.update(gamma=(1.4, float(rng.uniform(1.1, 1.9))), pinf=(0.0, 0.0))
+        rho2, u2, p2, _ = g5.rankine_hugoniot(1.4, 0.0, 1.0, 1.0, mach=float(rng.uniform(1.1, 3.0)))
+        regs = [g5.region(1.0, (rho2, 0.0), p2, u2, float(rng.uniform(-0.3, 0.3))), g5.region(1.0, (1.0, 0.0), 1.0),
+                g5.region(0.0, (0.0, float(rng.uniform(0.1, 3.0))), 1.0)]
+    dx = 1.0 / (16 * nby)
+    ext = (16 * nbx * dx, 16 * nby * dx)
+    centre = (float(rng.uniform(0.3, 0.7)) * ext[0], float(rng.uniform(0.3, 0.7)) * ext[1])
+    radius = float(rng.uniform(0.15, 0.3)) * ext[1]
+    front = float(rng.uniform(0.05, 0.35)) * ext[0]
+    cfg = g5.make_config(2, (nbx, nby), dx, **kw)
+    cfg_f = g5.make_config(2, (2 * nbx, 2 * nby), dx / 2, **kw)
+    if mp:
+        cfg.source_term = cfg_f.source_term = g5.SOURCE_MILLER_PUCKETT
+    Uc = g5.init_shock_bubble(cfg, regs, front, centre, radius)
+    Uf = g5.init_shock_bubble(cfg_f, regs, front, centre, radius)
+    steps = int(rng.integers(6, 16))
+    label = (f"seed {seed}: nb=({nbx},{nby}) refined={refined} periodic={periodic[:2]} bc={bc[:4]} thinc={thinc} "
+             f"slope={slope} {integrator} prolong={prolong} water={water} mp={mp} steps={steps}")
+    return cfg, refined, prolong, Uc, Uf, steps, label
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_two_level_configurations(g5, orc, seed):
+    # bit-exact against the oracle; a configuration whose state turns non-finite (the order-4
+    # interpolation, which is not monotone, across a shock into water: seeds 1 and 9) stops with
+    # G5_ERR_DIVERGED after the step whose state the oracle also finds non-finite
+    cfg, refined, prolong, Uc, Uf, steps, label = random_two_level(g5, seed)
+    _announce(label)
+    with g5.Context(cfg, refined=refined, prolongation=prolong) as ctx:
+        U0 = two_level_state(cfg.nb[0], ctx.block_layout(), Uc, Uf)
+        ctx.upload(U0)
+        tt, done = ctypes.c_double(0.0), ctypes.c_int32()
+        rc = g5.lib().g5_run(ctx._h, steps, ctypes.byref(tt), 1e300, ctypes.byref(done))
+        t_g, n_g = tt.value, done.value
+        U_g = ctx.download()
+    o = oracle_of(orc, cfg, refined, prolong)
+    o.set_state(U0)
+    if rc == g5.G5_ERR_DIVERGED:
+        for k in range(1, n_g + 1):
+            o.godunov_unsplit(o.compute_dt())
+            assert np.all(np.isfinite(o.get_state())) == (k < n_g), (label, k, n_g)
+        assert prolong == 3 and n_g < steps
+        return
+    assert rc == g5.G5_OK, rc
+    t_o, n_o = o.run(steps)
+    U_o = o.get_state()
+    assert n_g == n_o == steps and t_g == t_o, (t_g, t_o)
+    err = rel_l1(U_g, U_o)
+    assert err.max() <= RTOL_L1 and np.array_equal(U_g, U_o), (label, err)
