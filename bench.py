@@ -14,6 +14,10 @@ Workloads (--workload):
                 the per-field relative L1 error against the FP64 run of the same case.
   batch         4096 independent 2-D solves of the seeded batching workload (g5_batch_spec),
                 mixed precision, one pool, 100 steps each.
+  two_level_2d  the 2-D air-helium shock-bubble on a static two-level mesh: a 2048x512-cell brick
+                (128x32 blocks, dx = 1/512) whose 24x24 blocks around the bubble are refined into
+                96x96 blocks of half the cell length (5824 blocks, 1.49 M cells), reassociated
+                FP64, 100 steps; the one-level brick of the same coarse size timed beside it.
   serve         the native serving loop (g5_serve): 400 requests with Poisson arrivals at 160/s
                 (about 80% of the loop's capacity), 2-D and 10% 3-D requests, HBM budget 200 GB,
                 mixed precision; reports p50/p99 latency.
@@ -41,7 +45,7 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-updates/s on 1 GPU vs plain FP64 at fixed L1 err; p50/p99 solve latency"
 SEED = 20251203
-DEFAULT_STEPS = {"air_water_3d": 50, "air_helium_2d": 200, "mixed_3d": 100, "batch": 100, "serve": 0}
+DEFAULT_STEPS = {"air_water_3d": 50, "air_helium_2d": 200, "mixed_3d": 100, "batch": 100, "serve": 0, "two_level_2d": 100}
 # arithmetic of the single-solve FP64 workloads: the reference's operation order (bit-exact) or the
 # reassociated FP64 form (relative L1 <= 1e-12 per conserved variable against it)
 PRECISIONS = {"fp64": 0, "fp64_fast": 2}
@@ -153,6 +157,44 @@ def bench_single(g5, args, name):
         out["cpu_baseline"] = {"value": cu / dt, "unit": "cell-updates/s", "cores": th, "kind": "port",
                                "sample": f"{k} steps of the same {'x'.join(map(str, grid))} workload, CPU FP64 "
                                          f"oracle ({th} OpenMP threads, {dt:.1f} s)"}
+    return out
+
+
+def bench_two_level(g5, args):
+    import cases
+
+    nbx, nby = 128, 32
+    refined = [[x, y] for y in range(4, 28) for x in range(20, 44)]
+    cfg, Uc = cases.air_helium_2d(nbx=nbx, nby=nby)
+    _, Uf = cases.air_helium_2d(nbx=2 * nbx, nby=2 * nby)
+    cfg.precision = PRECISIONS[args.precision]
+    ctx = g5.Context(cfg, refined=refined, prolongation=g5.PROLONG_EXTRAPOLATE_LINEAR_MINMOD)
+    lay = ctx.block_layout()
+    Uc3, Uf3 = Uc.reshape(-1, 8, 256), Uf.reshape(-1, 8, 256)
+    U0 = np.stack([Uc3[x + nbx * y] if lv == 0 else Uf3[x + 2 * nbx * y] for x, y, _, lv in lay]).ravel()
+    ctx.upload(U0)
+    if args.warmup > 0:
+        ctx.run(args.warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    _, n = ctx.run(args.steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ctx.last_run_ms() / max(n, 1)
+    U = ctx.download()
+    ctx.close()
+    assert n == args.steps and np.all(np.isfinite(U)), "non-finite state"
+    cells = len(lay) * 256
+    nfine = int((lay[:, 3] == 1).sum())
+    # the one-level brick of the coarse size, same arithmetic and steps
+    wall1, ev1, _, n1 = run_context(g5, cfg, Uc, args.steps, args.warmup)
+    out = base_line(args, cells * n / wall, wall * 1e3 / n, n, "f64")
+    out["config"] = {"workload": "shock_bubble_2d_air_helium_two_level", "coarse_grid": [16 * nbx, 16 * nby],
+                     "refined_coarse_blocks": len(refined), "blocks": len(lay), "fine_blocks": nfine, "cells": cells,
+                     "prolongation": "EXTRAPOLATE_LINEAR_MINMOD", "precision": args.precision,
+                     "one_level_same_brick": {"cells": nbx * nby * 256, "value": nbx * nby * 256 * n1 / wall1,
+                                              "ms_per_step": wall1 * 1e3 / n1, "kernel_ms_per_step": ev1}}
+    out["roofline"] = roofline(cells, 8, ev_ms, None)
     return out
 
 
@@ -343,6 +385,8 @@ def main():
         out = bench_mixed3d(g5, args)
     elif args.workload == "batch":
         out = bench_batch(g5, args)
+    elif args.workload == "two_level_2d":
+        out = bench_two_level(g5, args)
     else:
         out = bench_serve(g5, args)
     print(json.dumps(out))
