@@ -211,6 +211,8 @@ def lib():
             "g5_pool_destroy": (ctypes.c_int, [vp]),
             "g5_pool_last_error": (ctypes.c_char_p, [vp]),
             "g5_pool_admit": (ctypes.c_int, [vp, P(G5Config), d, i32, P(i32)]),
+            "g5_pool_admit_two_level": (ctypes.c_int, [vp, P(G5Config), P(i32), i32, i32, d, i32, P(i32)]),
+            "g5_pool_block_layout": (ctypes.c_int, [vp, i32, P(i32), sz, P(i32)]),
             "g5_pool_release": (ctypes.c_int, [vp, i32]),
             "g5_pool_upload": (ctypes.c_int, [vp, i32, P(d), sz]),
             "g5_pool_download": (ctypes.c_int, [vp, i32, P(d), sz]),
@@ -266,7 +268,8 @@ def exported_symbols():
         "g5_pool_properties", "g5_pool_prepare", "g5_host_alloc", "g5_host_free", "g5_pool_set_clock", "g5_pool_step_async", "g5_batch_spec", "g5_serve_specs", "g5_spec_config", "g5_spec_regions",
         "g5_pool_admit_spec", "g5_serve", "g5_serve_run", "g5_spec_state_size", "g5_pool_failure",
         "g5_set_block_order", "g5_do_time_step_buffers", "g5_compute_dt_buffer", "g5_step_noalloc",
-        "g5_create_two_level", "g5_block_layout", "g5_refine_flags",
+        "g5_create_two_level", "g5_block_layout", "g5_refine_flags", "g5_pool_admit_two_level",
+        "g5_pool_block_layout",
     ]
 
 
@@ -655,6 +658,33 @@ class Pool:
         self._limits[sid.value] = (t_end, max_steps)
         return sid.value
 
+    def admit_two_level(self, cfg, refined, prolongation=0, t_end=1e300, max_steps=2 ** 31 - 1):
+        """admit a two-level solve (refined: (n, 2) coarse block coordinates); its state follows
+        block_layout(sid)"""
+        r = np.ascontiguousarray(refined, dtype=np.int32).reshape(-1, 2)
+        sid = ctypes.c_int32()
+        self._check(lib().g5_pool_admit_two_level(self._h, ctypes.byref(cfg),
+                                                  r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), r.shape[0],
+                                                  prolongation, t_end, max_steps, ctypes.byref(sid)))
+        self._cfg[sid.value] = cfg
+        self._limits[sid.value] = (t_end, max_steps)
+        return sid.value
+
+    def block_layout(self, sid):
+        """(n_blocks, 4) int32: x, y, z, level of each block of the solve's state"""
+        n = ctypes.c_int32()
+        self._check(lib().g5_pool_block_layout(self._h, sid, None, 0, ctypes.byref(n)))
+        c = np.empty((n.value, 4), dtype=np.int32)
+        self._check(lib().g5_pool_block_layout(self._h, sid, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), c.size,
+                                               ctypes.byref(n)))
+        return c
+
+    def state_size(self, sid):
+        n = ctypes.c_int32()
+        self._check(lib().g5_pool_block_layout(self._h, sid, None, 0, ctypes.byref(n)))
+        cfg = self._cfg[sid]
+        return n.value * nvar_of(cfg.dim) * cfg.bsize ** cfg.dim
+
     def limits(self, sid):
         """(t_end, max_steps) the solve was admitted with"""
         return self._limits[sid]
@@ -680,7 +710,7 @@ class Pool:
         self._check(lib().g5_pool_upload(self._h, sid, _dptr(U), U.size))
 
     def download(self, sid):
-        U = np.empty(state_size(self._cfg[sid]), dtype=np.float64)
+        U = np.empty(self.state_size(sid), dtype=np.float64)
         self._check(lib().g5_pool_download(self._h, sid, _dptr(U), U.size))
         return U
 

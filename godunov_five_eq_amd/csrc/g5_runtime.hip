@@ -142,6 +142,8 @@ struct g5_pool
   // staging of g5_pool_upload / g5_pool_download for solves whose slots are not one run
   double * d_xfer = nullptr;
   int64_t  cap_xfer = 0;
+  int32_t * d_xslots = nullptr; // slots of a two-level solve in its block order, for those transfers
+  int64_t   cap_xslots = 0;
 
   // scratch of g5_pool_start (block list and ids of the solves being started)
   BlockDesc * d_start_list = nullptr;
@@ -1057,15 +1059,28 @@ pool_transfer(g5_pool * p, int32_t id, double * host, size_t n, bool upload, dou
       return rc;
     st = p->d_xfer;
   }
+  // block k's slot: the solve's slot-table segment, or for a two-level solve (whose segment holds
+  // its level tables) its slot list uploaded for the transfer
+  int32_t const * slot_list = p->d_slots;
+  int32_t         base = r.table_base;
+  if (!r.two_level.empty())
+  {
+    int rc = grow(p, p->d_xslots, p->cap_xslots, static_cast<int64_t>(r.nblk), p->stream);
+    if (rc != G5_OK)
+      return rc;
+    G5_HIP(p->err, hipMemcpyAsync(p->d_xslots, r.slots.data(), r.nblk * sizeof(int32_t), hipMemcpyHostToDevice,
+                                  p->stream));
+    slot_list = p->d_xslots;
+    base = 0;
+  }
   if (upload)
   {
     G5_HIP(p->err, hipMemcpyAsync(st, host, need * sizeof(double), hipMemcpyHostToDevice, p->stream));
-    launch_gather(p->buf[0], p->buf[1], p->d_ctl, id, p->d_slots, r.table_base, r.nblk, p->block_doubles, st, true,
-                  p->stream);
+    launch_gather(p->buf[0], p->buf[1], p->d_ctl, id, slot_list, base, r.nblk, p->block_doubles, st, true, p->stream);
   }
   else
   {
-    launch_gather(p->buf[0], p->buf[1], p->d_ctl, id, p->d_slots, r.table_base, r.nblk, p->block_doubles, st, false,
+    launch_gather(p->buf[0], p->buf[1], p->d_ctl, id, slot_list, base, r.nblk, p->block_doubles, st, false,
                   p->stream);
     G5_HIP(p->err, hipMemcpyAsync(host, st, need * sizeof(double), hipMemcpyDeviceToHost, p->stream));
   }
@@ -1277,6 +1292,7 @@ g5_pool_destroy(g5_pool * p)
   hipFree(p->d_start_ids);
   hipFree(p->d_xfer);
   hipFree(p->d_iflux);
+  hipFree(p->d_xslots);
   if (p->h_ctl)
     hipHostFree(p->h_ctl);
   if (p->stream)
@@ -1297,6 +1313,52 @@ g5_pool_admit(g5_pool * p, const g5_config * cfg, double t_end, int32_t max_step
   if (not p or not cfg or not solve_id)
     return G5_ERR_INVALID;
   return pool_admit(p, *cfg, t_end, max_steps, solve_id);
+}
+
+int
+g5_pool_admit_two_level(g5_pool * p, const g5_config * cfg, const int32_t * refined_coords, int32_t n_refined,
+                        int32_t prolongation, double t_end, int32_t max_steps, int32_t * solve_id)
+{
+  if (not p or not cfg or not solve_id)
+    return G5_ERR_INVALID;
+  if (not refined_coords or n_refined < 1)
+    return fail(p->err, G5_ERR_INVALID, "no refined blocks");
+  const std::vector<int32_t> refined(refined_coords, refined_coords + 2 * static_cast<size_t>(n_refined));
+  return pool_admit(p, *cfg, t_end, max_steps, solve_id, &refined, prolongation);
+}
+
+int
+g5_pool_block_layout(const g5_pool * p, int32_t id, int32_t * coords, size_t n, int32_t * n_blocks)
+{
+  if (not p or id < 0 or id >= static_cast<int32_t>(p->solves.size()) or !p->solves[id].live)
+    return G5_ERR_INVALID;
+  SolveRec const & r = p->solves[id];
+  if (n_blocks)
+    *n_blocks = r.nblk;
+  if (not coords)
+    return G5_OK;
+  if (n != 4 * static_cast<size_t>(r.nblk))
+    return G5_ERR_INVALID;
+  const int nx = r.cfg.nb[0], ny = r.cfg.nb[1];
+  for (int32_t k = 0; k < r.nblk; ++k)
+  {
+    int32_t * c = coords + 4 * static_cast<size_t>(k);
+    if (!r.two_level.empty())
+    {
+      c[0] = r.two_level[3 * k];
+      c[1] = r.two_level[3 * k + 1];
+      c[2] = 0;
+      c[3] = r.two_level[3 * k + 2];
+    }
+    else
+    {
+      c[0] = k % nx;
+      c[1] = (k / nx) % ny;
+      c[2] = k / (nx * ny);
+      c[3] = 0;
+    }
+  }
+  return G5_OK;
 }
 
 int

@@ -382,6 +382,18 @@ const char * g5_pool_last_error(const g5_pool * pool);
  * G5_ERR_UNSUPPORTED for RK2_SSP in a pool without the stage buffer (g5_pool_create_rk2). */
 int g5_pool_admit(g5_pool * pool, const g5_config * cfg, double t_end, int32_t max_steps, int32_t * solve_id);
 
+/* reserve memory for one two-level solve (the mesh of g5_create_two_level: cfg's 2-D brick with the
+ * n_refined blocks at refined_coords[2k], refined_coords[2k+1] refined, the given prolongation),
+ * stepped in the same launch list as the pool's other solves; its state uses the block order of
+ * g5_pool_block_layout.  No reference counterpart beyond g5_create_two_level's: batching is this
+ * library's. */
+int g5_pool_admit_two_level(g5_pool * pool, const g5_config * cfg, const int32_t * refined_coords, int32_t n_refined,
+                            int32_t prolongation, double t_end, int32_t max_steps, int32_t * solve_id);
+
+/* number of blocks of a solve and, when coords is not NULL, their (x, y, z, level), 4 int32 per
+ * block in the order of the solve's state (n = 4 * n_blocks) */
+int g5_pool_block_layout(const g5_pool * pool, int32_t solve_id, int32_t * coords, size_t n, int32_t * n_blocks);
+
 /* free the solve's memory and id */
 int g5_pool_release(g5_pool * pool, int32_t solve_id);
 

@@ -298,3 +298,40 @@ def test_random_two_level_configurations(g5, orc, seed):
     assert n_g == n_o == steps and t_g == t_o, (t_g, t_o)
     err = rel_l1(U_g, U_o)
     assert err.max() <= RTOL_L1 and np.array_equal(U_g, U_o), (label, err)
+
+
+def test_two_level_solves_batched_with_one_level_ones(g5):
+    # one pool, one launch list: a one-level shock-bubble, the two-level shock-bubble and the
+    # two-level periodic advection (admitted after a released solve, so its slots are not one
+    # run); each ends bitwise equal to its run alone in a context of its own
+    cfg_a, Ua = cases.air_helium_2d(nbx=8, nby=4)
+    cfg_b, ctx_b, Ub = shock_bubble_two_level(g5, L_SHAPE_8x4, 1)
+    cfg_c, ctx_c, Uc = advection_two_level(g5, L_SHAPE_6x6, 2)
+    steps = 40
+    alone = {}
+    for name, ctx, U0 in (("b", ctx_b, Ub), ("c", ctx_c, Uc)):
+        with ctx:
+            ctx.upload(U0)
+            ctx.run(steps)
+            alone[name] = ctx.download()
+    with g5.Context(cfg_a) as ctx:
+        ctx.upload(Ua)
+        ctx.run(steps)
+        alone["a"] = ctx.download()
+    _announce("pool: one-level + two two-level solves, fragmented slots")
+    with g5.Pool(2, precision=g5.PREC_FP64, budget_bytes=64 << 20) as pool:
+        filler = pool.admit(g5.make_config(2, (2, 2), 1.0 / 32))
+        sa = pool.admit(cfg_a, max_steps=steps)
+        sb = pool.admit_two_level(cfg_b, L_SHAPE_8x4, 1, max_steps=steps)
+        pool.release(filler)
+        sc = pool.admit_two_level(cfg_c, L_SHAPE_6x6, 2, max_steps=steps)
+        assert len(pool.block_layout(sc)) == 33 + 12 and pool.block_layout(sb)[-1, 3] == 1
+        for sid, U0 in ((sa, Ua), (sb, Ub), (sc, Uc)):
+            pool.upload(sid, U0)
+            assert np.array_equal(pool.download(sid), U0)
+        pool.start([sa, sb, sc])
+        pool.step(steps + 2)
+        for sid, name in ((sa, "a"), (sb, "b"), (sc, "c")):
+            t, n, active = pool.status(sid)
+            assert n == steps and not active
+            assert np.array_equal(pool.download(sid), alone[name]), name
