@@ -177,3 +177,112 @@ def test_conservative_interpolation_children_average_to_the_parent(g5, orc, prol
         for v in (1, 2, 3, 4, 5, 7):
             mean = (kids[0][v] + kids[1][v] + kids[2][v] + kids[3][v]) / 4.0
             assert mean == pytest.approx(parent[v], rel=4e-16, abs=0.0), (cx, cy, v)
+
+
+# ---- 3-D two-level mesh ----------------------------------------------------------------------
+REFINED_3D = [[1, 1, 1], [2, 1, 1], [1, 2, 1], [1, 1, 2]]  # a 3-D corner of four blocks
+
+
+def periodic_cfg3(g5, nb=4, **kw):
+    opts = dict(periodic=(1, 1, 1), bc=("PERIODIC",) * 6, gamma=(1.4, 1.4), cfl=0.6)
+    opts.update(kw)
+    return g5.make_config(3, (nb,) * 3, 1.0 / (16 * nb), **opts)
+
+
+def fill3(o, cfg, f):
+    """U[block][var][cell] of a 3-D two-level mesh: var v = f(v, X, Y, Z) at the cell centres in
+    units of the coarse cell"""
+    lv, bc = o.block_levels(), o.block_coords()[: o.nowned]
+    U = np.zeros((o.nowned, 9, 4096))
+    i = np.arange(16)
+    for k in range(o.nowned):
+        h = 0.5 ** lv[k]
+        X = (bc[k, 0] * 16 + i[None, None, :] + 0.5) * h + 0 * i[:, None, None]
+        Y = (bc[k, 1] * 16 + i[None, :, None] + 0.5) * h + 0 * i[:, None, None]
+        Z = (bc[k, 2] * 16 + i[:, None, None] + 0.5) * h + 0 * i[None, None, :]
+        for v in range(9):
+            U[k, v] = np.broadcast_to(f(v, X, Y, Z), (16, 16, 16)).ravel()
+    return U.ravel()
+
+
+def test_3d_layout_and_uniform_state(g5, orc):
+    cfg = periodic_cfg3(g5)
+    for prolongation in (0, 1, 2, 3):
+        o = orc.Oracle(cfg, refined=REFINED_3D, prolongation=prolongation)
+        lv = o.block_levels()
+        assert o.nowned == 64 - 4 + 32 and list(lv[:60]) == [0] * 60 and list(lv[60:]) == [1] * 32
+        # rho, E, mu, mv, mw, a0, ar0, a1, ar1 of a uniform moving gas
+        vals = [1.0, 3.0, 0.5, -0.25, 0.125, 1.0, 1.0, 0.0, 0.0]
+        U0 = fill3(o, cfg, lambda v, X, Y, Z: vals[v] + 0 * X)
+        o.set_state(U0)
+        o.run(3)
+        assert np.array_equal(o.get_state(), U0), prolongation
+
+
+def test_3d_restriction_and_prolongation_closed_forms(g5, orc):
+    cfg = periodic_cfg3(g5)
+    a = np.array([2.0, 5.0, 0.3, -0.2, 0.1, 1.0, 2.0, 0.0, 0.0])
+    b = np.array([[0.5, 1.0, 0.25, 0.5, 0.125, 0, 0.5, 0, 0], [-0.25, 0.75, -0.5, 0.125, 0.25, 0, -0.25, 0, 0],
+                  [0.125, -0.5, 0.25, -0.125, 0.5, 0, 0.125, 0, 0]]) / 16.0
+    for prolongation in (0, 1, 2, 3):
+        o = orc.Oracle(cfg, refined=REFINED_3D, prolongation=prolongation)
+        o.set_state(fill3(o, cfg, lambda v, X, Y, Z: a[v] + b[0, v] * X + b[1, v] * Y + b[2, v] * Z))
+        # restriction: coarse cell (20, 20, 20) lies in refined block (1, 1, 1)
+        u = o.cons_level(0, (20, 20, 20))
+        ref = a + b[0] * 20.5 + b[1] * 20.5 + b[2] * 20.5
+        for v in (1, 2, 3, 4, 6):
+            assert u[v] == pytest.approx(ref[v], rel=1e-14, abs=1e-14), (prolongation, v)
+        assert u[7] == 1.0 - u[5] and u[0] == u[6] + u[8]
+        # prolongation: fine cell (30, 40, 40) lies in coarse leaf (0, 1, 1), coarse cell (15, 20, 20)
+        u = o.cons_level(1, (30, 40, 40))
+        xf = np.array([30.5, 40.5, 40.5]) / 2
+        xc = np.array([15.5, 20.5, 20.5])
+        pos = xc if prolongation == 0 else xf  # a copy of the parent, or exact on linear fields
+        ref = a + b[0] * pos[0] + b[1] * pos[1] + b[2] * pos[2]
+        for v in (1, 2, 3, 4, 6):
+            assert u[v] == pytest.approx(ref[v], rel=1e-13, abs=1e-13), (prolongation, v)
+
+
+@pytest.mark.parametrize("prolongation", [0, 1, 2, 3])
+def test_3d_conservation_across_levels(g5, orc, prolongation):
+    cfg = periodic_cfg3(g5)
+    o = orc.Oracle(cfg, refined=REFINED_3D, prolongation=prolongation)
+
+    def f(v, X, Y, Z):
+        r2 = (X - 24) ** 2 + (Y - 26) ** 2 + (Z - 28) ** 2
+        rho = 1.0 + 0.5 * np.exp(-r2 / 60.0)
+        p = 1.0 + 0.2 * np.exp(-r2 / 90.0)
+        u, w, s = 1.0, 0.5, -0.25
+        return [rho, p / 0.4 + 0.5 * rho * (u * u + w * w + s * s), rho * u, rho * w, rho * s, 1.0 + 0 * X, rho,
+                0 * X, 0 * X][v]
+
+    U0 = fill3(o, cfg, f)
+    o.set_state(U0)
+    lv = o.block_levels()[: o.nowned]
+    vol = (0.5 ** lv) ** 3
+
+    def totals(U):
+        V = U.reshape(-1, 9, 4096)
+        return np.array([(V[:, v].sum(axis=1) * vol).sum() for v in range(9)])
+
+    T0 = totals(U0)
+    _, n = o.run(12)
+    T1 = totals(o.get_state())
+    assert n == 12
+    for v in (0, 1, 2, 3, 4):
+        assert abs(T1[v] - T0[v]) <= 2e-15 * abs(T0[v]), (v, T0[v], T1[v])
+
+
+@pytest.mark.parametrize("prolongation", [2, 3])
+def test_3d_conservative_interpolation_children_average_to_the_parent(g5, orc, prolongation):
+    cfg = periodic_cfg3(g5)
+    o = orc.Oracle(cfg, refined=REFINED_3D, prolongation=prolongation)
+    rng = np.random.default_rng(11)
+    o.set_state(rng.uniform(0.5, 2.0, o.nowned * 9 * 4096))
+    for c in [(15, 20, 20), (20, 15, 20), (20, 20, 15), (15, 15, 15), (40, 20, 20)]:
+        kids = [o.cons_level(1, (2 * c[0] + i, 2 * c[1] + j, 2 * c[2] + k)) for k in (0, 1) for j in (0, 1)
+                for i in (0, 1)]
+        parent = o.cons_level(0, c)
+        for v in (1, 2, 3, 4, 5, 6, 8):
+            mean = sum(kd[v] for kd in kids) / 8.0
+            assert mean == pytest.approx(parent[v], rel=8e-16, abs=0.0), (c, v)
