@@ -405,8 +405,8 @@ PROLONG_CONSERVATIVE_ORDER_2, PROLONG_CONSERVATIVE_ORDER_4 = 2, 3
 class Context:
     """One solve on the GPU (the GodunovImplemBase surface, scheme/GodunovImplemBase.h:59-198).
 
-    refined: (n, 2) coarse block coordinates refined into 2 x 2 blocks of half the cell length (a
-    2-D two-level static mesh, FP64 or FP64_FAST; see g5_create_two_level), with the given
+    refined: (n, dim) coarse block coordinates refined into 2^dim blocks of half the cell length (a
+    two-level static mesh, FP64 or FP64_FAST; see g5_create_two_level), with the given
     prolongation; None: one level."""
 
     def __init__(self, cfg, refined=None, prolongation=PROLONG_SIMPLE_COPY):
@@ -415,7 +415,7 @@ class Context:
         if refined is None:
             rc = lib().g5_create(ctypes.byref(cfg), ctypes.byref(self._h))
         else:
-            r = np.ascontiguousarray(refined, dtype=np.int32).reshape(-1, 2)
+            r = np.ascontiguousarray(refined, dtype=np.int32).reshape(-1, cfg.dim)
             rc = lib().g5_create_two_level(ctypes.byref(cfg), r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                            r.shape[0], prolongation, ctypes.byref(self._h))
         if rc != G5_OK:
@@ -659,9 +659,9 @@ class Pool:
         return sid.value
 
     def admit_two_level(self, cfg, refined, prolongation=0, t_end=1e300, max_steps=2 ** 31 - 1):
-        """admit a two-level solve (refined: (n, 2) coarse block coordinates); its state follows
+        """admit a two-level solve (refined: (n, dim) coarse block coordinates); its state follows
         block_layout(sid)"""
-        r = np.ascontiguousarray(refined, dtype=np.int32).reshape(-1, 2)
+        r = np.ascontiguousarray(refined, dtype=np.int32).reshape(-1, cfg.dim)
         sid = ctypes.c_int32()
         self._check(lib().g5_pool_admit_two_level(self._h, ctypes.byref(cfg),
                                                   r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), r.shape[0],

@@ -17,8 +17,8 @@ H5LIB = os.path.join(HERE, "libg5h5.so")
 # HDF5 1.10 C library of the image (headers, libhdf5 and h5dump under /opt/conda)
 HDF5_PREFIX = os.environ.get("G5_HDF5_PREFIX", "/opt/conda")
 
-HIP_SOURCES = ["g5_step2d.hip", "g5_step2d_fast.hip", "g5_step3d.hip", "g5_step3d_fast.hip", "g5_aux.hip",
-               "g5_refine.hip", "g5_runtime.hip"]
+HIP_SOURCES = ["g5_step2d.hip", "g5_step2d_fast.hip", "g5_step3d.hip", "g5_step3d_fast.hip", "g5_step3d_amr.hip",
+               "g5_step3d_amr_fast.hip", "g5_aux.hip", "g5_refine.hip", "g5_runtime.hip"]
 CXX_SOURCES = ["g5_init.cpp", "g5_specs.cpp", "g5_serve.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -40,6 +40,8 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"
 # FMA (held to the same relative L1 bar against the bit-exact path as the rest of that path).
 FILE_FLAGS = {"g5_step3d.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
               "g5_step3d_fast.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=fast"],
+              "g5_step3d_amr.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+              "g5_step3d_amr_fast.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=fast"],
               "g5_step2d_fast.hip": ["-ffp-contract=fast"]}
 
 

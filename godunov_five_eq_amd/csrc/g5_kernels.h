@@ -27,6 +27,9 @@ void launch_step2d_two_level(StepArgs const & A, int n_first, bool thinc, bool h
                              hipStream_t stream);
 void launch_step2d_two_level_fast(StepArgs const & A, int n_first, bool thinc, bool hancock, bool mp,
                                   hipStream_t stream);
+void launch_step3d_two_level(StepArgs const & A, int n_first, bool thinc, bool hancock, int precision,
+                             hipStream_t stream);
+void launch_step3d_two_level_fast(StepArgs const & A, int n_first, bool thinc, bool hancock, hipStream_t stream);
 
 // refine flags and largest indicator values of the listed blocks into R.flags / R.imax by slot
 void launch_refine_flags(int dim, double const * buf0, double const * buf1, BlockDesc const * blocks,

@@ -346,10 +346,10 @@ admit_device(g5_pool * p, int32_t sid, double tend, int32_t max_steps)
   std::vector<int32_t> const & tab = r.two_level.empty() ? r.slots : r.table_host;
   if (!r.two_level.empty())
   {
-    sd.base1 = r.table_base + r.cfg.nb[0] * r.cfg.nb[1];
+    sd.base1 = r.table_base + r.cfg.nb[0] * r.cfg.nb[1] * (r.cfg.dim == 3 ? r.cfg.nb[2] : 1);
     sd.prolong = r.prolong;
     // interface records, indexed by slot
-    rc = grow(p, p->d_iflux, p->cap_iflux, p->nslots * kIfluxDoubles, p->stream);
+    rc = grow(p, p->d_iflux, p->cap_iflux, p->nslots * (r.cfg.dim == 3 ? kIflux3Doubles : kIfluxDoubles), p->stream);
     if (rc != G5_OK)
       return rc;
   }
@@ -364,65 +364,76 @@ admit_device(g5_pool * p, int32_t sid, double tend, int32_t max_steps)
   return G5_OK;
 }
 
-// A two-level static mesh over the brick of cfg: the coarse blocks at refined[2k], refined[2k+1]
-// are each replaced by 2 x 2 blocks of half the cell length.  Fills r.two_level (block order:
-// coarse leaves lexicographic, then the fine blocks lexicographic over the fine grid) and
-// r.border; r.nblk is the block count.
+// A two-level static mesh over the brick of cfg: the coarse blocks at refined[dim k .. dim k + dim)
+// are each replaced by 2^dim blocks of half the cell length.  Fills r.two_level (x, y, z, level per
+// block; order: coarse leaves lexicographic, then the fine blocks lexicographic over the fine grid)
+// and r.border; r.nblk is the block count.
 int
 layout_two_level(g5_config const & cfg, std::vector<int32_t> const & refined, SolveRec & r, std::string & msg)
 {
-  const int nx = cfg.nb[0], ny = cfg.nb[1];
-  if (cfg.dim != 2)
-    return msg = "two-level meshes are 2-D", G5_ERR_UNSUPPORTED;
+  const int dim = cfg.dim;
+  const int nx = cfg.nb[0], ny = cfg.nb[1], nz = dim == 3 ? cfg.nb[2] : 1;
   if (cfg.precision == G5_PREC_MIXED)
     return msg = "two-level meshes exchange FP64 face records: FP64 or FP64_FAST precision", G5_ERR_UNSUPPORTED;
-  if (refined.empty() or refined.size() % 2 != 0)
+  if (refined.empty() or refined.size() % dim != 0)
     return msg = "no refined blocks", G5_ERR_INVALID;
-  std::vector<char> ref(static_cast<size_t>(nx) * ny, 0);
-  for (size_t k = 0; k < refined.size(); k += 2)
+  std::vector<char> ref(static_cast<size_t>(nx) * ny * nz, 0);
+  for (size_t k = 0; k < refined.size(); k += dim)
   {
-    const int x = refined[k], y = refined[k + 1];
-    if (x < 1 or x > nx - 2 or y < 1 or y > ny - 2)
-      return msg = "refined blocks must lie at least one block inside the brick's border", G5_ERR_INVALID;
-    char & f = ref[x + static_cast<size_t>(nx) * y];
+    const int c[3] = { refined[k], refined[k + 1], dim == 3 ? refined[k + 2] : 0 };
+    for (int d = 0; d < dim; ++d)
+      if (c[d] < 1 or c[d] > cfg.nb[d] - 2)
+        return msg = "refined blocks must lie at least one block inside the brick's border", G5_ERR_INVALID;
+    char & f = ref[c[0] + static_cast<size_t>(nx) * (c[1] + static_cast<size_t>(ny) * c[2])];
     if (f)
       return msg = "refined block repeated", G5_ERR_INVALID;
     f = 1;
   }
   r.two_level.clear();
   r.border.clear();
-  for (int y = 0; y < ny; ++y)
-    for (int x = 0; x < nx; ++x)
-      if (!ref[x + static_cast<size_t>(nx) * y])
-      {
-        r.two_level.insert(r.two_level.end(), { x, y, 0 });
-        r.border.push_back(0);
-      }
-  auto fine = [&](int x, int y) { return ref[(x >> 1) + static_cast<size_t>(nx) * (y >> 1)] != 0; };
-  for (int y = 0; y < 2 * ny; ++y)
-    for (int x = 0; x < 2 * nx; ++x)
-      if (fine(x, y))
-      {
-        r.two_level.insert(r.two_level.end(), { x, y, 1 });
-        r.border.push_back(!fine(x - 1, y) or !fine(x + 1, y) or !fine(x, y - 1) or !fine(x, y + 1));
-      }
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x)
+        if (!ref[x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)])
+        {
+          r.two_level.insert(r.two_level.end(), { x, y, z, 0 });
+          r.border.push_back(0);
+        }
+  auto fine = [&](int x, int y, int z) {
+    return ref[(x >> 1) + static_cast<size_t>(nx) * ((y >> 1) + static_cast<size_t>(ny) * (z >> 1))] != 0;
+  };
+  const int fz = dim == 3 ? 2 * nz : 1;
+  for (int z = 0; z < fz; ++z)
+    for (int y = 0; y < 2 * ny; ++y)
+      for (int x = 0; x < 2 * nx; ++x)
+        if (fine(x, y, z))
+        {
+          r.two_level.insert(r.two_level.end(), { x, y, z, 1 });
+          bool b = !fine(x - 1, y, z) or !fine(x + 1, y, z) or !fine(x, y - 1, z) or !fine(x, y + 1, z);
+          if (dim == 3)
+            b = b or !fine(x, y, z - 1) or !fine(x, y, z + 1);
+          r.border.push_back(b);
+        }
   r.nblk = static_cast<int32_t>(r.border.size());
   return G5_OK;
 }
 
-// slot-table segment of a two-level solve from its slots
+// slot-table segment of a two-level solve from its slots: level 0 (nb[0] nb[1] nb[2]) then level 1
+// (2nb[0] 2nb[1] 2nb[2], 2-D: 2nb[0] 2nb[1])
 void
 tables_two_level(SolveRec & r)
 {
-  const int nx = r.cfg.nb[0], ny = r.cfg.nb[1];
-  r.table_host.assign(static_cast<size_t>(5) * nx * ny, -1);
+  const int    dim = r.cfg.dim;
+  const int    nx = r.cfg.nb[0], ny = r.cfg.nb[1], nz = dim == 3 ? r.cfg.nb[2] : 1;
+  const size_t n0 = static_cast<size_t>(nx) * ny * nz;
+  r.table_host.assign(n0 * (1 + (dim == 3 ? 8 : 4)), -1);
   for (int32_t k = 0; k < r.nblk; ++k)
   {
-    const int x = r.two_level[3 * k], y = r.two_level[3 * k + 1];
-    if (r.two_level[3 * k + 2] == 0)
-      r.table_host[x + static_cast<size_t>(nx) * y] = r.slots[k];
+    const int x = r.two_level[4 * k], y = r.two_level[4 * k + 1], z = r.two_level[4 * k + 2];
+    if (r.two_level[4 * k + 3] == 0)
+      r.table_host[x + static_cast<size_t>(nx) * (y + static_cast<size_t>(ny) * z)] = r.slots[k];
     else
-      r.table_host[static_cast<size_t>(nx) * ny + x + static_cast<size_t>(2 * nx) * y] = r.slots[k];
+      r.table_host[n0 + x + static_cast<size_t>(2 * nx) * (y + static_cast<size_t>(2 * ny) * z)] = r.slots[k];
   }
 }
 
@@ -541,9 +552,10 @@ append_blocks(g5_pool const * p, int32_t sid, std::vector<BlockDesc> & out, int 
   {
     for (int32_t k = 0; k < r.nblk; ++k)
       if (part == 2 or (part == 0) == (r.border[k] != 0))
-        out.push_back(BlockDesc{ sid, r.slots[k], static_cast<int16_t>(r.two_level[3 * k]),
-                                 static_cast<int16_t>(r.two_level[3 * k + 1]), 0,
-                                 static_cast<int16_t>(r.two_level[3 * k + 2]) });
+        out.push_back(BlockDesc{ sid, r.slots[k], static_cast<int16_t>(r.two_level[4 * k]),
+                                 static_cast<int16_t>(r.two_level[4 * k + 1]),
+                                 static_cast<int16_t>(r.two_level[4 * k + 2]),
+                                 static_cast<int16_t>(r.two_level[4 * k + 3]) });
     return;
   }
   if (part == 0)
@@ -632,7 +644,12 @@ launch_step(g5_pool * p, StepArgs const & A, bool hancock)
   if (A.nblocks <= 0)
     return;
   if (p->any_two_level) // every caller passes the whole launch list
-    launch_step2d_two_level(A, p->nlist_first, p->thinc != 0, hancock, p->precision, p->any_mp, p->stream);
+  {
+    if (p->dim == 2)
+      launch_step2d_two_level(A, p->nlist_first, p->thinc != 0, hancock, p->precision, p->any_mp, p->stream);
+    else
+      launch_step3d_two_level(A, p->nlist_first, p->thinc != 0, hancock, p->precision, p->stream);
+  }
   else if (p->dim == 2)
     launch_step2d(A, p->thinc != 0, hancock, p->precision, p->launch.ring_wave != 0, p->any_mp, p->stream);
   else
@@ -1323,7 +1340,7 @@ g5_pool_admit_two_level(g5_pool * p, const g5_config * cfg, const int32_t * refi
     return G5_ERR_INVALID;
   if (not refined_coords or n_refined < 1)
     return fail(p->err, G5_ERR_INVALID, "no refined blocks");
-  const std::vector<int32_t> refined(refined_coords, refined_coords + 2 * static_cast<size_t>(n_refined));
+  const std::vector<int32_t> refined(refined_coords, refined_coords + cfg->dim * static_cast<size_t>(n_refined));
   return pool_admit(p, *cfg, t_end, max_steps, solve_id, &refined, prolongation);
 }
 
@@ -1345,10 +1362,8 @@ g5_pool_block_layout(const g5_pool * p, int32_t id, int32_t * coords, size_t n, 
     int32_t * c = coords + 4 * static_cast<size_t>(k);
     if (!r.two_level.empty())
     {
-      c[0] = r.two_level[3 * k];
-      c[1] = r.two_level[3 * k + 1];
-      c[2] = 0;
-      c[3] = r.two_level[3 * k + 2];
+      for (int d = 0; d < 4; ++d)
+        c[d] = r.two_level[4 * k + d];
     }
     else
     {
@@ -1747,7 +1762,9 @@ g5_create_two_level(const g5_config * cfg, const int32_t * refined_coords, int32
   if (prolongation < 0 or prolongation > 3)
     return fail(g_create_error, G5_ERR_INVALID, "prolongation must be 0 (SIMPLE_COPY), 1 (EXTRAPOLATE_LINEAR_MINMOD), "
                                                 "2 or 3 (CONSERVATIVE_INTERPOLATION_ORDER_2 / _4)");
-  const std::vector<int32_t> refined(refined_coords, refined_coords + 2 * static_cast<size_t>(n_refined));
+  if (not cfg)
+    return fail(g_create_error, G5_ERR_INVALID, "null argument");
+  const std::vector<int32_t> refined(refined_coords, refined_coords + cfg->dim * static_cast<size_t>(n_refined));
   return create_ctx(cfg, &refined, prolongation, out);
 }
 
@@ -1764,10 +1781,8 @@ g5_block_layout(const g5_ctx * ctx, int32_t * coords, size_t n)
     int32_t * c = coords + 4 * static_cast<size_t>(k);
     if (!r.two_level.empty())
     {
-      c[0] = r.two_level[3 * k];
-      c[1] = r.two_level[3 * k + 1];
-      c[2] = 0;
-      c[3] = r.two_level[3 * k + 2];
+      for (int d = 0; d < 4; ++d)
+        c[d] = r.two_level[4 * k + d];
     }
     else if (!ctx->block_order.empty())
     {
@@ -2154,6 +2169,8 @@ g5_refine_flags(g5_ctx * ctx, const g5_refine_params * prm, int32_t * flags, dou
     return fail(ctx->err, G5_ERR_INVALID, "n must equal the number of blocks");
   if (prm->indicator < G5_INDICATOR_LOHNER_SPLIT or prm->indicator > G5_INDICATOR_SIMPLE_GRADIENT)
     return fail(ctx->err, G5_ERR_INVALID, "unknown refine indicator");
+  if (ctx->cfg.dim == 3 and !p->solves[ctx->id].two_level.empty())
+    return fail(ctx->err, G5_ERR_UNSUPPORTED, "refine flags of a 3-D two-level mesh are not built");
   int rc = rebuild_list(p);
   if (rc != G5_OK)
     return fail(ctx->err, rc, p->err);

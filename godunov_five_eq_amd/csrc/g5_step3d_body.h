@@ -314,10 +314,270 @@ struct Pred3
 };
 } // namespace
 
-template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, bool MP, typename T, bool FAST>
+// ---- two-level static mesh (3-D) -------------------------------------------------------------
+// Level 0 is the brick of nb[0] x nb[1] x nb[2] blocks, some of them refined into 2 x 2 x 2 blocks
+// of half the cell length (level 1), at least one block away from the brick's border.  Slot
+// tables: level 0 at S.base (-1 where refined), level 1 at S.base1 over the 2nb^3 fine grid (-1
+// outside the refined blocks).  A cell is named by its global index at its level.
+namespace
+{
+G5D int
+slot3_l0(SolveDesc const & S, int32_t const * __restrict__ slots, int x, int y, int z)
+{
+  return slots[S.base + x + S.nb[0] * (y + S.nb[1] * z)];
+}
+G5D int
+slot3_l1(SolveDesc const & S, int32_t const * __restrict__ slots, int x, int y, int z)
+{
+  return slots[S.base1 + x + 2 * S.nb[0] * (y + 2 * S.nb[1] * z)];
+}
+G5D Cons
+cell3_of(double const * __restrict__ U, int slot, int ci, int cj, int ck)
+{
+  return load_cons<3, B3>(U, static_cast<int64_t>(slot) * 9 * 4096 + ci + B3 * (cj + B3 * ck), -1);
+}
+G5D double
+var3_of(double const * __restrict__ U, int slot, int var, int ci, int cj, int ck)
+{
+  return U[static_cast<int64_t>(slot) * 9 * 4096 + var * 4096 + ci + B3 * (cj + B3 * ck)];
+}
+
+// coarse cell g inside a refined block: the average of its eight fine cells (summed x fastest, then
+// y, then z; over 8) for E, rho u, rho v, rho w, alpha_rho 0/1, alpha 0; alpha 1 = 1 - alpha 0,
+// rho = alpha_rho 0 + alpha_rho 1 (get_conservative_vars_restriction,
+// scheme/ConvertToPrimitivesVariablesFunctor.cpp:214-250)
+G5D Cons
+restrict3(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int gx, int gy, int gz)
+{
+  Cons sum{};
+  for (int k = 0; k < 8; ++k)
+  {
+    const int  fx = 2 * gx + (k & 1), fy = 2 * gy + ((k >> 1) & 1), fz = 2 * gz + (k >> 2);
+    const Cons f = cell3_of(U, slot3_l1(S, slots, fx >> 4, fy >> 4, fz >> 4), fx & 15, fy & 15, fz & 15);
+    sum.e = k == 0 ? f.e : sum.e + f.e;
+    sum.mu = k == 0 ? f.mu : sum.mu + f.mu;
+    sum.mv = k == 0 ? f.mv : sum.mv + f.mv;
+    sum.mw = k == 0 ? f.mw : sum.mw + f.mw;
+    sum.ar0 = k == 0 ? f.ar0 : sum.ar0 + f.ar0;
+    sum.ar1 = k == 0 ? f.ar1 : sum.ar1 + f.ar1;
+    sum.a0 = k == 0 ? f.a0 : sum.a0 + f.a0;
+  }
+  Cons u;
+  u.e = sum.e / 8.0;
+  u.mu = sum.mu / 8.0;
+  u.mv = sum.mv / 8.0;
+  u.mw = sum.mw / 8.0;
+  u.ar0 = sum.ar0 / 8.0;
+  u.ar1 = sum.ar1 / 8.0;
+  u.a0 = sum.a0 / 8.0;
+  u.a1 = 1.0 - u.a0;
+  u.d = u.ar0 + u.ar1;
+  return u;
+}
+
+// level-0 cell g of the parent's 3x3x3-block neighbourhood: a coarse leaf's cell or a restriction
+G5D Cons
+coarse3(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int gx, int gy, int gz)
+{
+  const int sl = slot3_l0(S, slots, gx >> 4, gy >> 4, gz >> 4);
+  return sl >= 0 ? cell3_of(U, sl, gx & 15, gy & 15, gz & 15) : restrict3(S, slots, U, gx, gy, gz);
+}
+// one variable (offset var of the 9-variable layout) of that cell
+G5D double
+coarse3_var(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int gx, int gy,
+            int gz, int var)
+{
+  const int sl = slot3_l0(S, slots, gx >> 4, gy >> 4, gz >> 4);
+  if (sl >= 0)
+    return var3_of(U, sl, var, gx & 15, gy & 15, gz & 15);
+  double sum = 0.0;
+  for (int k = 0; k < 8; ++k)
+  {
+    const int    fx = 2 * gx + (k & 1), fy = 2 * gy + ((k >> 1) & 1), fz = 2 * gz + (k >> 2);
+    const double v = var3_of(U, slot3_l1(S, slots, fx >> 4, fy >> 4, fz >> 4), var, fx & 15, fy & 15, fz & 15);
+    sum = k == 0 ? v : sum + v;
+  }
+  return sum / 8.0;
+}
+
+// the conservative interpolations of order 2 / 4 (S.prolong 2 / 3): tensor product of the 1-D
+// half-cell weights over the parent's 3^3 (5^3) neighbourhood, x first, then y, then z
+G5D double
+conservative3_var(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int cx,
+                  int cy, int cz, int px, int py, int pz, int var)
+{
+  const double w2[3] = { 0.125, 1.0, -0.125 };
+  const double w4[5] = { -3.0 / 128.0, 22.0 / 128.0, 1.0, -22.0 / 128.0, 3.0 / 128.0 };
+  const bool   o2 = S.prolong == 2;
+  const int    m = o2 ? 1 : 2, n = 2 * m + 1;
+  double       valz = 0.0;
+  for (int kz = 0; kz < n; ++kz)
+  {
+    double valy = 0.0;
+    for (int ky = 0; ky < n; ++ky)
+    {
+      double valx = 0.0;
+      for (int kx = 0; kx < n; ++kx)
+      {
+        const int    ix = px == 0 ? kx : n - 1 - kx;
+        const double x =
+          (o2 ? w2[ix] : w4[ix]) * coarse3_var(S, slots, U, cx + kx - m, cy + ky - m, cz + kz - m, var);
+        valx = kx == 0 ? x : valx + x;
+      }
+      const int    iy = py == 0 ? ky : n - 1 - ky;
+      const double wy = o2 ? w2[iy] : w4[iy];
+      valy = ky == 0 ? wy * valx : valy + wy * valx;
+    }
+    const int    iz = pz == 0 ? kz : n - 1 - kz;
+    const double wz = o2 ? w2[iz] : w4[iz];
+    valz = kz == 0 ? wz * valy : valz + wz * valy;
+  }
+  return valz;
+}
+
+// fine cell g outside the fine blocks: its coarse parent copied (SIMPLE_COPY), extrapolated with
+// minmod slopes (EXTRAPOLATE_LINEAR_MINMOD, linear_extrapolate_using_limited_slopes 3-D,
+// ConvertToPrimitivesVariablesFunctor.cpp:553-770), or conservatively interpolated
+G5D Cons
+prolong3(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int gx, int gy, int gz)
+{
+  const int cx = gx >> 1, cy = gy >> 1, cz = gz >> 1; // the fine ghost cells are at positive indices
+  Cons      u;
+  if (S.prolong >= 2)
+  {
+    const int px = gx & 1, py = gy & 1, pz = gz & 1;
+    u.ar0 = conservative3_var(S, slots, U, cx, cy, cz, px, py, pz, 6);
+    u.ar1 = conservative3_var(S, slots, U, cx, cy, cz, px, py, pz, 8);
+    u.a0 = conservative3_var(S, slots, U, cx, cy, cz, px, py, pz, 5);
+    u.e = conservative3_var(S, slots, U, cx, cy, cz, px, py, pz, 1);
+    u.mu = conservative3_var(S, slots, U, cx, cy, cz, px, py, pz, 2);
+    u.mv = conservative3_var(S, slots, U, cx, cy, cz, px, py, pz, 3);
+    u.mw = conservative3_var(S, slots, U, cx, cy, cz, px, py, pz, 4);
+    u.d = u.ar0 + u.ar1;
+    u.a1 = 1.0 - u.a0;
+    return u;
+  }
+  const Cons c = coarse3(S, slots, U, cx, cy, cz);
+  if (S.prolong == 0)
+    return c;
+  const double hx = (gx & 1) ? 0.25 : -0.25, hy = (gy & 1) ? 0.25 : -0.25, hz = (gz & 1) ? 0.25 : -0.25;
+  const Cons   xl = coarse3(S, slots, U, cx - 1, cy, cz), xr = coarse3(S, slots, U, cx + 1, cy, cz);
+  const Cons   yl = coarse3(S, slots, U, cx, cy - 1, cz), yr = coarse3(S, slots, U, cx, cy + 1, cz);
+  const Cons   zl = coarse3(S, slots, U, cx, cy, cz - 1), zr = coarse3(S, slots, U, cx, cy, cz + 1);
+  auto         ex = [&](double C, double XL, double XR, double YL, double YR, double ZL, double ZR) {
+    return C + hx * slope1<double>(C, XR, XL, 1.0) + hy * slope1<double>(C, YR, YL, 1.0) +
+           hz * slope1<double>(C, ZR, ZL, 1.0);
+  };
+  u.ar0 = ex(c.ar0, xl.ar0, xr.ar0, yl.ar0, yr.ar0, zl.ar0, zr.ar0);
+  u.ar1 = ex(c.ar1, xl.ar1, xr.ar1, yl.ar1, yr.ar1, zl.ar1, zr.ar1);
+  u.a0 = ex(c.a0, xl.a0, xr.a0, yl.a0, yr.a0, zl.a0, zr.a0);
+  u.e = ex(c.e, xl.e, xr.e, yl.e, yr.e, zl.e, zr.e);
+  u.mu = ex(c.mu, xl.mu, xr.mu, yl.mu, yr.mu, zl.mu, zr.mu);
+  u.mv = ex(c.mv, xl.mv, xr.mv, yl.mv, yr.mv, zl.mv, zr.mv);
+  u.mw = ex(c.mw, xl.mw, xr.mw, yl.mw, yr.mw, zl.mw, zr.mw);
+  u.d = u.ar0 + u.ar1;
+  u.a1 = 1.0 - u.a0;
+  return u;
+}
+
+// conservative state of tile cell (gi, gj, gk) of a block of a two-level solve (fill_ghosts,
+// ConvertToPrimitivesVariablesFunctor.cpp:884-918): the block's own cell, a same-level
+// neighbour's (border conditions resolved as on one level), a restriction or a prolongation
+template <bool INLET>
+G5D Cons
+ghost3_two_level(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U,
+                 BlockDesc const & bd, int gi, int gj, int gk)
+{
+  const int dx = gi < 0 ? -1 : (gi >= B3 ? 1 : 0), dy = gj < 0 ? -1 : (gj >= B3 ? 1 : 0),
+            dz = gk < 0 ? -1 : (gk >= B3 ? 1 : 0);
+  const int nx = bd.bx + dx, ny = bd.by + dy, nz = bd.bz + dz;
+  const int gx = bd.bx * B3 + gi, gy = bd.by * B3 + gj, gz = bd.bz * B3 + gk;
+  if (bd.level == 0)
+  {
+    if (nx >= 0 and nx < S.nb[0] and ny >= 0 and ny < S.nb[1] and nz >= 0 and nz < S.nb[2] and
+        slot3_l0(S, slots, nx, ny, nz) < 0)
+      return restrict3(S, slots, U, gx, gy, gz);
+    int           neg;
+    const int64_t off = resolve_cell<3, B3>(S, slots, bd.bx, bd.by, bd.bz, gi, gj, gk, neg);
+    Cons          u = load_cons<3, B3>(U, off, -1);
+    apply_border<3, INLET>(S, u, neg);
+    return u;
+  }
+  const int sl = slot3_l1(S, slots, nx, ny, nz); // fine blocks sit inside the fine grid's interior
+  return sl >= 0 ? cell3_of(U, sl, gi - dx * B3, gj - dy * B3, gk - dz * B3) : prolong3(S, slots, U, gx, gy, gz);
+}
+
+// interface records of a coarse slot: [side][face cell c][fine face sub][9 values] (see
+// kIflux3Doubles); the four fine faces over a coarse face cell are summed in sub order 0..3 and
+// divided by 8 (get_flux_from_fine_neighbor, ReadFluxesAndConservativeUpdateFunctor.h:256-307)
+G5D int64_t
+iflux3_at(int slot, int side, int c, int sub)
+{
+  static_assert(kIflux3Doubles == 6 * 256 * 4 * 9, "interface record layout");
+  return (((static_cast<int64_t>(slot) * 6 + side) * 256 + c) * 4 + sub) * 9;
+}
+G5D void
+iflux3_store(double * __restrict__ R, int64_t o, Flux const & F)
+{
+  R[o + 0] = F.ar0;
+  R[o + 1] = F.ar1;
+  R[o + 2] = F.a0;
+  R[o + 3] = F.a1;
+  R[o + 4] = F.e;
+  R[o + 5] = F.mu;
+  R[o + 6] = F.mv;
+  R[o + 7] = F.mw;
+  R[o + 8] = F.us;
+}
+G5D Flux
+iflux3_sum(double const * __restrict__ R, int slot, int side, int c)
+{
+  const int64_t o = iflux3_at(slot, side, c, 0);
+  double        v[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q)
+    v[q] = ((R[o + q] + R[o + 9 + q]) + R[o + 18 + q]) + R[o + 27 + q];
+  Flux F;
+  F.ar0 = v[0] / 8.0;
+  F.ar1 = v[1] / 8.0;
+  F.a0 = v[2] / 8.0;
+  F.a1 = v[3] / 8.0;
+  F.e = v[4] / 8.0;
+  F.mu = v[5] / 8.0;
+  F.mv = v[6] / 8.0;
+  F.mw = v[7] / 8.0;
+  F.us = v[8] / 8.0;
+  return F;
+}
+// where a fine block's boundary face record goes: the coarse neighbour's slot, side, face cell and
+// sub index, for the fine face at global fine cell (gx, gy, gz) on the fine block's side sd (0..5:
+// xmin, xmax, ymin, ymax, zmin, zmax); transverse axes t1, t2 = (axis+1)%3, (axis+2)%3 as in the
+// reference's face-sibling sum, the face cell index t1 + 16 t2 in the coarse block
+G5D int64_t
+iflux3_dest(SolveDesc const & S, int32_t const * __restrict__ slots, BlockDesc const & bd, int sd, int gx, int gy,
+            int gz)
+{
+  const int ax = sd >> 1;
+  const int n1[3] = { bd.bx + (ax == 0 ? ((sd & 1) ? 1 : -1) : 0), bd.by + (ax == 1 ? ((sd & 1) ? 1 : -1) : 0),
+                      bd.bz + (ax == 2 ? ((sd & 1) ? 1 : -1) : 0) };
+  const int cb[3] = { n1[0] >> 1, n1[1] >> 1, n1[2] >> 1 };
+  const int g[3] = { gx, gy, gz };
+  const int t1 = (ax + 1) % 3, t2 = (ax + 2) % 3;
+  const int c = ((g[t1] >> 1) - 16 * cb[t1]) + 16 * ((g[t2] >> 1) - 16 * cb[t2]);
+  const int sub = (g[t1] & 1) + 2 * (g[t2] & 1);
+  return iflux3_at(slot3_l0(S, slots, cb[0], cb[1], cb[2]), sd ^ 1, c, sub);
+}
+} // namespace
+
+// AMR: the launch list may hold blocks of two-level solves (FP64 records only): their ghosts come
+// from ghost3_two_level, a coarse block takes the faces it shares with fine blocks from A.iflux,
+// and a fine block bordering coarse blocks writes its boundary face records there (the fine blocks
+// that do run in a launch of their own before the rest of the list)
+template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, bool MP, typename T, bool FAST, bool AMR = false>
 G5D void
 step3d_body(StepArgs const & A, int blk)
 {
+  static_assert(!AMR or sizeof(T) == sizeof(double), "two-level solves exchange FP64 face records");
   __shared__ T      sQ[3 * QPL];
   __shared__ T      sXL[7 * NF3];
   __shared__ T      sYL[7 * NF3];
@@ -332,7 +592,7 @@ step3d_body(StepArgs const & A, int blk)
 
   const int       t = threadIdx.x;
   const BlockDesc bd = A.blocks[blk];
-  const SolveDesc S = A.solves[bd.solve];
+  const SolveDesc S = AMR ? at_level(A.solves[bd.solve], bd.level) : A.solves[bd.solve];
   SolveCtl *      C = A.ctl + bd.solve;
   constexpr int   NC = 4096;
   const int64_t   blk_off = static_cast<int64_t>(bd.slot) * 9 * NC;
@@ -363,13 +623,41 @@ step3d_body(StepArgs const & A, int blk)
   ring_cell3(lane, rgi, rgj);
   const bool z_early = !ZSPLIT or wave != 1;
 
+  // two-level solve: sides (bit s: xmin, xmax, ymin, ymax, zmin, zmax) whose neighbour is finer (a
+  // coarse block) or coarser (a fine block)
+  const bool two_level = AMR and S.base1 >= 0;
+  int        finer = 0, coarser = 0;
+  if (two_level)
+  {
+#pragma unroll
+    for (int sd = 0; sd < 6; ++sd)
+    {
+      const int a = sd >> 1, o = (sd & 1) ? 1 : -1;
+      const int nx = bd.bx + (a == 0 ? o : 0), ny = bd.by + (a == 1 ? o : 0), nz = bd.bz + (a == 2 ? o : 0);
+      if (bd.level == 0)
+        finer |= (nx >= 0 and nx < S.nb[0] and ny >= 0 and ny < S.nb[1] and nz >= 0 and nz < S.nb[2] and
+                  slot3_l0(S, A.slots, nx, ny, nz) < 0)
+                   ? 1 << sd
+                   : 0;
+      else
+        coarser |= slot3_l1(S, A.slots, nx, ny, nz) < 0 ? 1 << sd : 0;
+    }
+  }
+
   // neighbour slots once per block (LDS), so that a fetch issues its loads without a dependent
-  // global read of the slot table
-  fill_slot_table<3>(S, A.slots, bd.bx, bd.by, bd.bz, t, sSlot);
+  // global read of the slot table (a two-level solve's ghosts are resolved by level instead)
+  if (!two_level)
+    fill_slot_table<3>(S, A.slots, bd.bx, bd.by, bd.bz, t, sSlot);
   __syncthreads();
   // raw loads; the WALL momentum sign is applied at the conversion, so that the prefetch of a
   // plane is not waited for in the phase that issues it
   auto fetch_raw = [&](int gi, int gj, int gk, int & neg) {
+    if constexpr (AMR)
+      if (two_level)
+      {
+        neg = -1; // border conditions resolved inside
+        return ghost3_two_level<INLET>(S, A.slots, U, bd, gi, gj, gk);
+      }
     const int64_t off = resolve_cell_tab<3, B3, INLET>(S, sSlot, bd.bx, bd.by, bd.bz, gi, gj, gk, neg);
     return load_cons<3, B3>(U, off, -1);
   };
@@ -405,7 +693,7 @@ step3d_body(StepArgs const & A, int blk)
   const bool    inlet_block = INLET and S.inlet != G5_INLET_NONE and bd.bx == 0; // halo cells may be inflow cells
   int           x_neg0 = -1;
   int64_t       x_off0 = 0;
-  if (has_x)
+  if (has_x and !two_level)
     x_off0 = resolve_cell_tab<3, B3, INLET>(S, sSlot, bd.bx, bd.by, bd.bz, xgi, xgj, 0, x_neg0);
 
   // prologue: planes -2 and -1 into LDS, plane 0 prefetched
@@ -443,6 +731,19 @@ step3d_body(StepArgs const & A, int blk)
   };
   auto z_solve = [&](int P, FaceT<T> const & fzm) {
     zF = riemann<3, FAST>(S, zL, fzm, dtdx, 2);
+    if constexpr (AMR)
+    {
+      // the block's bottom (face 0) and top (face 16) z faces across a level interface
+      const int sd = P == 0 ? 4 : 5;
+      if (P == 0 or P == 16)
+      {
+        if ((finer >> sd) & 1)
+          zF = iflux3_sum(A.iflux, bd.slot, sd, i + 16 * j);
+        else if ((coarser >> sd) & 1)
+          iflux3_store(A.iflux, iflux3_dest(S, A.slots, bd, sd, bd.bx * B3 + i, bd.by * B3 + j,
+                                            bd.bz * B3 + (P == 0 ? 0 : 15)), zF);
+      }
+    }
     if (!LATE)
       z_complete(P);
   };
@@ -469,8 +770,13 @@ step3d_body(StepArgs const & A, int blk)
       neg_i = -1;
       if (has_x)
       {
-        pf_x = load_cons<3, B3>(U, x_off0 + 256 * (P + 2), -1);
-        neg_x = x_neg0;
+        if (two_level)
+          pf_x = fetch_raw(xgi, xgj, P + 2, neg_x);
+        else
+        {
+          pf_x = load_cons<3, B3>(U, x_off0 + 256 * (P + 2), -1);
+          neg_x = x_neg0;
+        }
       }
     }
     else if (P + 2 <= 17)
@@ -516,8 +822,19 @@ step3d_body(StepArgs const & A, int blk)
     // (e) Riemann problems on the x and y faces of plane P
     if (inner_plane)
     {
-      fl_store3(sFX, sFXE, fx3(i, j), riemann<3, FAST>(S, st_load3(sXL, fx3(i, j)), fxm, dtdx, 0));
-      fl_store3(sFY, sFYE, fy3(i, j), riemann<3, FAST>(S, st_load3(sYL, fy3(i, j)), fym, dtdx, 1));
+      FluxT<T> fx = riemann<3, FAST>(S, st_load3(sXL, fx3(i, j)), fxm, dtdx, 0);
+      FluxT<T> fy = riemann<3, FAST>(S, st_load3(sYL, fy3(i, j)), fym, dtdx, 1);
+      if constexpr (AMR)
+      {
+        // a coarse block's faces on a finer neighbour take the fine faces' records (x sides: face
+        // cell y + 16 z; y sides: z + 16 x)
+        if (i == 0 and (finer & 1))
+          fx = iflux3_sum(A.iflux, bd.slot, 0, j + 16 * P);
+        if (j == 0 and (finer & 4))
+          fy = iflux3_sum(A.iflux, bd.slot, 2, P + 16 * i);
+      }
+      fl_store3(sFX, sFXE, fx3(i, j), fx);
+      fl_store3(sFY, sFYE, fy3(i, j), fy);
       if (wave == 0 and (rs & 1))
       {
         // the last x face (rs 1) or y face (rs 3) of row/column rr: one solve on the run-time axis
@@ -525,7 +842,11 @@ step3d_body(StepArgs const & A, int blk)
         const int k = ax == 0 ? fx3(16, rr) : fy3(rr, 16);
         const FaceT<T> L = st_load3(ax == 0 ? sXL : sYL, k);
         const FaceT<T> R = st_load3<16>(ax == 0 ? sXR : sYR, rr);
-        fl_store3(ax == 0 ? sFX : sFY, ax == 0 ? sFXE : sFYE, k, riemann<3, FAST>(S, L, R, dtdx, ax));
+        FluxT<T>       fr = riemann<3, FAST>(S, L, R, dtdx, ax);
+        if constexpr (AMR)
+          if ((finer >> rs) & 1)
+            fr = iflux3_sum(A.iflux, bd.slot, rs, ax == 0 ? rr + 16 * P : P + 16 * rr);
+        fl_store3(ax == 0 ? sFX : sFY, ax == 0 ? sFXE : sFYE, k, fr);
       }
     }
     if (!z_early)
@@ -540,6 +861,18 @@ step3d_body(StepArgs const & A, int blk)
 
     // (g) completion of plane P-1 (z update, normalisation, store, CFL), then (i) the x and y
     // update of plane P
+    if constexpr (AMR)
+    {
+      // a fine block on a coarse neighbour: its boundary x/y face records of plane P, one per lane
+      // of wave 0 (side t/16, row or column t%16)
+      const int sd = t >> 4, r = t & 15;
+      if (inner_plane and coarser != 0 and t < 64 and ((coarser >> sd) & 1))
+      {
+        const Flux F = sd < 2 ? fl_load3(sFX, sFXE, fx3(sd == 0 ? 0 : 16, r)) : fl_load3(sFY, sFYE, fy3(r, sd == 2 ? 0 : 16));
+        const int  gi = sd == 0 ? 0 : (sd == 1 ? 15 : r), gj = sd < 2 ? r : (sd == 2 ? 0 : 15);
+        iflux3_store(A.iflux, iflux3_dest(S, A.slots, bd, sd, bd.bx * B3 + gi, bd.by * B3 + gj, bd.bz * B3 + P), F);
+      }
+    }
     if (LATE and P >= 0)
       z_complete(P);
     if (inner_plane)
@@ -581,14 +914,51 @@ step3d_body(StepArgs const & A, int blk)
 // One workgroup per block.  (A persistent form, workgroups looping over several blocks, made
 // an illegal memory access on the GPU with no cause found in the code; the runtime never asks
 // for it in 3-D: the launch must have wg_per_xcd == 0.)
-template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, bool MP, typename T, bool FAST = false>
+template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, bool MP, typename T, bool FAST = false, bool AMR = false>
 G5D void
 step3d_blocks(StepArgs const & A)
 {
   int b, end, stride;
   block_range(A, b, end, stride);
   if (b < end)
-    step3d_body<THINC, HANCOCK, ZSPLIT, INLET, MP, T, FAST>(A, b);
+    step3d_body<THINC, HANCOCK, ZSPLIT, INLET, MP, T, FAST, AMR>(A, b);
+}
+
+// a launch list with blocks of two-level solves: the first n_first blocks (the fine blocks that
+// border coarse blocks) write their interface records, then the rest of the list reads them.
+// The two-level instances compile the inflow-border and Miller-Puckett code (each block applies
+// its own solve's choices) and the split z solve.
+template <bool THINC, bool HANCOCK, bool FAST>
+__global__ void __launch_bounds__(256)
+k_step3d_two_level(StepArgs A)
+{
+  step3d_blocks<THINC, HANCOCK, true, true, true, double, FAST, true>(A);
+}
+
+template <bool FAST>
+static void
+launch3_two_level(StepArgs const & A, int n_first, bool thinc, bool hancock, hipStream_t stream)
+{
+  StepArgs P[2] = { A, A };
+  P[0].nblocks = n_first;
+  P[1].blocks = A.blocks + n_first;
+  P[1].nblocks = A.nblocks - n_first;
+  for (StepArgs & a : P)
+  {
+    if (a.nblocks <= 0)
+      continue;
+    a.per_xcd = (a.nblocks + 7) / 8;
+    a.wg_per_xcd = 0;
+    const dim3 grid(8 * a.per_xcd), block(256);
+    if (thinc and hancock)
+      hipLaunchKernelGGL((k_step3d_two_level<true, true, FAST>), grid, block, 0, stream, a);
+    else if (thinc)
+      hipLaunchKernelGGL((k_step3d_two_level<true, false, FAST>), grid, block, 0, stream, a);
+    else if (hancock)
+      hipLaunchKernelGGL((k_step3d_two_level<false, true, FAST>), grid, block, 0, stream, a);
+    else
+      hipLaunchKernelGGL((k_step3d_two_level<false, false, FAST>), grid, block, 0, stream, a);
+  }
 }
 
 } // namespace g5

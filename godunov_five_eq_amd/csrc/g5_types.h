@@ -101,7 +101,10 @@ struct StepArgs
   // read by their coarse neighbours in the next launch
   double *          iflux;
 };
-constexpr int kIfluxDoubles = 4 * 16 * 8; // interface records per slot
+constexpr int kIfluxDoubles = 4 * 16 * 8; // interface records per slot (2-D)
+// 3-D: per coarse slot, [side xmin, xmax, ymin, ymax, zmin, zmax][face cell 0..255][fine face 0..3]
+// [ar0, ar1, a0, a1, e, mu, mv, mw, us]
+constexpr int kIflux3Doubles = 6 * 256 * 4 * 9;
 
 // refine-flag computation (g5_refine_flags): parameters and the per-slot outputs
 struct RefineArgs

@@ -179,20 +179,20 @@ const char * g5_last_error(const g5_ctx * ctx);
  * scheme/GodunovImplemV0.cpp:35-61): validates cfg, allocates the device state and scratch. */
 int g5_create(const g5_config * cfg, g5_ctx ** out);
 
-/* A two-level static mesh (2-D, FP64 or FP64_FAST): cfg's brick of nb[0] x nb[1] blocks at level
- * 0, with the n_refined blocks at refined_coords[2k], refined_coords[2k+1] each replaced by 2 x 2
- * blocks of half the cell length (level 1).  Refined blocks lie at least one block inside the
- * brick's border.  The reference's AMR paths on such a mesh: the coarse/fine ghost fill of
+/* A two-level static mesh (2-D or 3-D, FP64 or FP64_FAST): cfg's brick at level 0, with the
+ * n_refined blocks at refined_coords[dim k .. dim k + dim) each replaced by 2^dim blocks of half the
+ * cell length (level 1).  Refined blocks lie at least one block inside the brick's border.  The
+ * reference's AMR paths on such a mesh: the coarse/fine ghost fill of
  * ConvertToPrimitivesVariablesFunctor::fill_ghosts (scheme/ConvertToPrimitivesVariablesFunctor.cpp:
  * 884-918) -- restriction (:214-250) of a finer neighbour, prolongation of a coarser one,
  * prolongation 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD (:553-770), 2 or 3
- * CONSERVATIVE_INTERPOLATION_ORDER_2 / _4 (:339-490,751-827) -- the fine-face flux sums
- * of read_fluxes_and_update_2d (scheme/ReadFluxesAndConservativeUpdateFunctor.cpp:137-199,
+ * CONSERVATIVE_INTERPOLATION_ORDER_2 / _4 (:339-490,751-827) -- the fine-face flux sums of
+ * read_fluxes_and_update_{2d,3d} (scheme/ReadFluxesAndConservativeUpdateFunctor.cpp:137-199,
  * .h:256-307) and the per-level cell length in the fluxes and the CFL rate
  * (ComputeFluxesAndStoreFunctor, ComputeDtFiveEqFunctor).  The state holds the blocks in the order
  * of g5_block_layout: the coarse leaves lexicographic, then the fine blocks lexicographic over the
- * 2nb[0] x 2nb[1] fine grid (the mesh itself, which the reference takes from its AMR manager, is
- * fixed for the context's life: no regridding). */
+ * fine grid of 2nb blocks per direction (the mesh itself, which the reference takes from its AMR
+ * manager, is fixed for the context's life: no regridding). */
 int g5_create_two_level(const g5_config * cfg, const int32_t * refined_coords, int32_t n_refined,
                         int32_t prolongation, g5_ctx ** out);
 
@@ -382,8 +382,8 @@ const char * g5_pool_last_error(const g5_pool * pool);
  * G5_ERR_UNSUPPORTED for RK2_SSP in a pool without the stage buffer (g5_pool_create_rk2). */
 int g5_pool_admit(g5_pool * pool, const g5_config * cfg, double t_end, int32_t max_steps, int32_t * solve_id);
 
-/* reserve memory for one two-level solve (the mesh of g5_create_two_level: cfg's 2-D brick with the
- * n_refined blocks at refined_coords[2k], refined_coords[2k+1] refined, the given prolongation),
+/* reserve memory for one two-level solve (the mesh of g5_create_two_level: cfg's brick with the
+ * n_refined blocks at refined_coords[dim k .. dim k + dim) refined, the given prolongation),
  * stepped in the same launch list as the pool's other solves; its state uses the block order of
  * g5_pool_block_layout.  No reference counterpart beyond g5_create_two_level's: batching is this
  * library's. */

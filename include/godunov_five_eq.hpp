@@ -411,8 +411,8 @@ public:
     m_ctx.reset(c);
   }
 
-  // a two-level static mesh (2-D): the coarse blocks at refined[2k], refined[2k+1] refined into
-  // 2 x 2 blocks of half the cell length; prolongation 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD,
+  // a two-level static mesh: the coarse blocks at refined[dim k .. dim k + dim) refined into 2^dim
+  // blocks of half the cell length; prolongation 0 SIMPLE_COPY, 1 EXTRAPOLATE_LINEAR_MINMOD,
   // 2 / 3 CONSERVATIVE_INTERPOLATION_ORDER_2 / _4
   // (g5_create_two_level; the state's block order is block_layout())
   GodunovImplemHip(g5_config const & cfg, std::vector<int32_t> const & refined, int prolongation)
@@ -420,7 +420,7 @@ public:
     if (cfg.dim != dim)
       throw std::runtime_error("dimension mismatch");
     g5_ctx * c = nullptr;
-    check(g5_create_two_level(&cfg, refined.data(), static_cast<int32_t>(refined.size() / 2), prolongation, &c),
+    check(g5_create_two_level(&cfg, refined.data(), static_cast<int32_t>(refined.size() / dim), prolongation, &c),
           nullptr);
     m_ctx.reset(c);
   }
