@@ -139,12 +139,12 @@ def test_checkpoint_header_round_trip(g5, tmp_path):
 
 def test_two_level_mesh_refusals(g5):
     """g5_create_two_level refuses, before allocating anything, what the two-level kernels do not
-    cover: 3-D, the mixed precision (its face records are star states, not summable fluxes), refined
-    blocks on the brick's border (their fine ghosts would need border conditions at level 1),
-    repeated blocks and unknown prolongations."""
+    cover: the mixed precision (its face records are star states, not summable fluxes), refined
+    blocks on the brick's border in 2-D or 3-D (their fine ghosts would need border conditions at
+    level 1), repeated blocks and unknown prolongations."""
     L = g5.lib()
     cases2 = [
-        (g5.make_config(3, (4, 4, 4), 0.1), [[1, 1]], 0, g5.G5_ERR_UNSUPPORTED),
+        (g5.make_config(3, (4, 4, 4), 0.1), [[1, 1, 0]], 0, g5.G5_ERR_INVALID),
         (g5.make_config(2, (4, 4), 0.1, precision=g5.PREC_MIXED), [[1, 1]], 0, g5.G5_ERR_UNSUPPORTED),
         (g5.make_config(2, (4, 4), 0.1), [[0, 1]], 0, g5.G5_ERR_INVALID),
         (g5.make_config(2, (4, 4), 0.1), [[1, 3]], 0, g5.G5_ERR_INVALID),
@@ -153,7 +153,7 @@ def test_two_level_mesh_refusals(g5):
         (g5.make_config(2, (4, 4), 0.1), [], 0, g5.G5_ERR_INVALID),
     ]
     for cfg, refined, prolong, code in cases2:
-        r = np.ascontiguousarray(np.array(refined, dtype=np.int32).reshape(-1, 2))
+        r = np.ascontiguousarray(np.array(refined, dtype=np.int32).reshape(-1, cfg.dim))
         h = ctypes.c_void_p()
         rc = L.g5_create_two_level(ctypes.byref(cfg), r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                    r.shape[0], prolong, ctypes.byref(h))
