@@ -26,6 +26,12 @@
 //    the sum rule, pure-phase limits and equilibrium tests only.  Parity unpinned.
 //  - tanh: a deterministic expm1-based tanh (Cody-Waite reduction + Taylor polynomial) used
 //    identically by the HIP path; pinned against libm tanh to a few ulp.
+//  - initial conditions (region states, shock-bubble, triple point): restated from
+//    common.h:68-144 and init/InitShockBubble.cpp / init/InitTriplePoint.cpp; the bubble's cell
+//    volume fraction uses the reference's tangent-plane construction, whose two helpers
+//    (get_tangent_to_sphere, vof::compute_volume_fraction_of_rect_below_plane) are absent: restated
+//    from their names' definitions and pinned by an exact clipping and the sphere-volume bound;
+//    exact formulas of the absent helpers: parity unpinned.
 //
 // Layout: U[block][var][cell], cells x-fastest, owned blocks in the caller's order followed by
 // the outside ring.  Variable ids follow models/FiveEq.h:35-50.
@@ -1861,6 +1867,225 @@ struct Solver
   }
 };
 
+// ---------------------------------------------------------------------------------------------
+// Initial conditions: region states (common.h:68-144 get_region_init_state), the shock-bubble
+// (init/InitShockBubble.cpp:50-225) and the triple point (init/InitTriplePoint.cpp:46-133).
+// The bubble's volume fraction follows the reference's construction: the plane tangent to the
+// sphere at the point nearest the cell centre (get_tangent_to_sphere) and the fraction of the cell
+// below it (vof::compute_volume_fraction_of_rect_below_plane).  Both are in the absent core and
+// are restated here: outward unit normal n = (x_c - c)/|x_c - c|, plane n.x = n.c + R, and the
+// exact volume of a box below a plane by inclusion-exclusion over its corners.
+// A region is given as {alpha0, alpha_rho0, alpha_rho1, p, u, v, w}.
+// ---------------------------------------------------------------------------------------------
+State
+region_state(int dim, Eos const & e, double const * r)
+{
+  Ids          H(dim);
+  const double alpha0 = r[0], ar0 = r[1], ar1 = r[2], p = r[3];
+  State        q{}, U{};
+  q[H.IU] = r[4];
+  q[H.IV] = r[5];
+  if (dim == 3)
+    q[H.IW] = r[6];
+  const double rho = ar0 + ar1;
+  const double eint_specific = e.mixture_specific_eint(rho, p, alpha0, 1 - alpha0, ar0, ar1);
+  const double ekin_specific = dim == 2 ? 0.5 * (q[H.IU] * q[H.IU] + q[H.IV] * q[H.IV])
+                                        : 0.5 * (q[H.IU] * q[H.IU] + q[H.IV] * q[H.IV] + q[H.IW] * q[H.IW]);
+  U[H.ID] = rho;
+  U[H.IA0] = alpha0;
+  U[H.IA1] = 1.0 - alpha0;
+  U[H.IAD0] = ar0;
+  U[H.IAD1] = ar1;
+  U[H.IE] = (eint_specific + ekin_specific) * rho;
+  U[H.IU] = q[H.IU] * rho;
+  U[H.IV] = q[H.IV] * rho;
+  if (dim == 3)
+    U[H.IW] = q[H.IW] * rho;
+  return U;
+}
+
+// fraction of the box [lo, lo + dx]^dim where n.x <= alpha: with m_i = |n_i| dx (axes with
+// n_i < 0 mirrored) and beta the plane's offset from the box's lowest corner, the volume of
+// {xi in [0,1]^k : m.xi <= beta} is sum_v (-1)^|v| max(0, beta - m.v)^k / (k! prod m) over the
+// corners v of the k axes with m_i > 0 (axes along which the plane does not vary drop out)
+double
+box_fraction_below_plane(int dim, double const * n, double alpha, double const * lo, double dx)
+{
+  double m[3], beta = alpha, mmax = 0.0;
+  for (int d = 0; d < dim; ++d)
+  {
+    m[d] = n[d] * dx;
+    beta -= n[d] * lo[d];
+    if (m[d] < 0.0)
+    {
+      beta -= m[d];
+      m[d] = -m[d];
+    }
+    mmax = std::max(mmax, m[d]);
+  }
+  if (mmax == 0.0)
+    return beta >= 0.0 ? 1.0 : 0.0;
+  double mk[3], msum = 0.0;
+  int    k = 0;
+  for (int d = 0; d < dim; ++d)
+    if (m[d] > 1e-10 * mmax)
+    {
+      mk[k++] = m[d];
+      msum += m[d];
+    }
+  if (beta <= 0.0)
+    return 0.0;
+  if (beta >= msum)
+    return 1.0;
+  double sum = 0.0;
+  for (int v = 0; v < (1 << k); ++v)
+  {
+    double s = beta;
+    int    bits = 0;
+    for (int d = 0; d < k; ++d)
+      if ((v >> d) & 1)
+      {
+        s -= mk[d];
+        ++bits;
+      }
+    if (s > 0.0)
+      sum += ((bits & 1) ? -1.0 : 1.0) * std::pow(s, k);
+  }
+  double den = 1.0;
+  for (int d = 0; d < k; ++d)
+    den *= mk[d] * (d + 1);
+  return std::min(1.0, std::max(0.0, sum / den));
+}
+
+// volume fraction of the cell [lo, lo + dx]^dim inside the sphere (centre c, radius R) by the
+// tangent plane at the point of the sphere nearest the cell centre
+double
+tangent_sphere_fraction(int dim, double const * lo, double dx, double const * c, double R)
+{
+  double xc[3], n[3], d2 = 0.0;
+  for (int d = 0; d < dim; ++d)
+  {
+    xc[d] = lo[d] + 0.5 * dx;
+    n[d] = xc[d] - c[d];
+    d2 += n[d] * n[d];
+  }
+  if (d2 == 0.0)
+    return 1.0;
+  const double dist = std::sqrt(d2);
+  double       alpha = R;
+  for (int d = 0; d < dim; ++d)
+  {
+    n[d] /= dist;
+    alpha += n[d] * c[d];
+  }
+  return box_fraction_below_plane(dim, n, alpha, lo, dx);
+}
+
+// InitShockBubbleDataFunctor::operator() (one bubble): regions 0 (x_c <= x_front), 1 (beyond) and
+// the bubble 2, owned blocks in brick order
+void
+init_shock_bubble(g5o_params const & P, double const * regions, double x_front, double const * center, double radius,
+                  double * Uown)
+{
+  const int dim = P.dim, b = P.bsize;
+  Ids       H(dim);
+  Eos       e;
+  e.init(P.gamma, P.pinf);
+  State S[3];
+  for (int r = 0; r < 3; ++r)
+    S[r] = region_state(dim, e, regions + 7 * r);
+  // pure-state specific internal energies (InitShockBubble.cpp:183-207)
+  auto eint_of = [&](State const & s) {
+    double ekin = s[H.IU] * s[H.IU] + s[H.IV] * s[H.IV];
+    if (dim == 3)
+      ekin += s[H.IW] * s[H.IW];
+    ekin /= (2.0 * s[H.ID]);
+    return s[H.IE] - ekin;
+  };
+  const double eint_pre = eint_of(S[1]), eint_bub = eint_of(S[2]);
+  const int    ncell = dim == 2 ? b * b : b * b * b;
+  const long   nblk = static_cast<long>(P.nb[0]) * P.nb[1] * (dim == 3 ? P.nb[2] : 1);
+  for (long blk = 0; blk < nblk; ++blk)
+  {
+    const int co[3] = { static_cast<int>(blk % P.nb[0]), static_cast<int>((blk / P.nb[0]) % P.nb[1]),
+                        static_cast<int>(blk / (static_cast<long>(P.nb[0]) * P.nb[1])) };
+    for (int cell = 0; cell < ncell; ++cell)
+    {
+      const int cc[3] = { cell % b, (cell / b) % b, dim == 3 ? cell / (b * b) : 0 };
+      double    lo[3];
+      for (int d = 0; d < 3; ++d)
+        lo[d] = P.xmin[d] + static_cast<double>(co[d] * b + cc[d]) * P.dx;
+      const double xc = lo[0] + 0.5 * P.dx;
+      State        u = xc <= x_front ? S[0] : S[1];
+      u[H.ID] = u[H.IAD0] + u[H.IAD1];
+      u[H.IA1] = 1.0 - u[H.IA0];
+      const double frac = tangent_sphere_fraction(dim, lo, P.dx, center, radius);
+      if (frac > 0.0)
+      {
+        const double phi0 = 1.0 - frac;
+        u[H.IAD0] = phi0 * S[1][H.IAD0] + (1.0 - phi0) * S[2][H.IAD0];
+        u[H.IAD1] = phi0 * S[1][H.IAD1] + (1.0 - phi0) * S[2][H.IAD1];
+        u[H.ID] = u[H.IAD0] + u[H.IAD1];
+        u[H.IA0] = phi0 * S[1][H.IA0] + (1.0 - phi0) * S[2][H.IA0];
+        u[H.IA1] = 1.0 - u[H.IA0];
+        u[H.IU] = phi0 * S[1][H.IU] + (1.0 - phi0) * S[2][H.IU];
+        u[H.IV] = phi0 * S[1][H.IV] + (1.0 - phi0) * S[2][H.IV];
+        if (dim == 3)
+          u[H.IW] = phi0 * S[1][H.IW] + (1.0 - phi0) * S[2][H.IW];
+        const double rho_mixed = u[H.IAD0] + u[H.IAD1];
+        double       ekin_mixed = u[H.IU] * u[H.IU] + u[H.IV] * u[H.IV];
+        if (dim == 3)
+          ekin_mixed += u[H.IW] * u[H.IW];
+        ekin_mixed /= (2.0 * rho_mixed);
+        const double eint_mixed = phi0 * eint_pre + (1.0 - phi0) * eint_bub;
+        u[H.IE] = eint_mixed + ekin_mixed;
+      }
+      for (int v = 0; v < H.nbvar; ++v)
+        Uown[(blk * H.nbvar + v) * ncell + cell] = u[v];
+    }
+  }
+}
+
+// InitTriplePointDataFunctor::operator(): region 0 for x_c <= xd, else region 1 where y_c <= yd
+// (3-D: sqrt(y_c^2 + z_c^2) <= yd) and region 2 elsewhere
+void
+init_triple_point(g5o_params const & P, double const * regions, double xd, double yd, double * Uown)
+{
+  const int dim = P.dim, b = P.bsize;
+  Ids       H(dim);
+  Eos       e;
+  e.init(P.gamma, P.pinf);
+  State S[3];
+  for (int r = 0; r < 3; ++r)
+    S[r] = region_state(dim, e, regions + 7 * r);
+  const int  ncell = dim == 2 ? b * b : b * b * b;
+  const long nblk = static_cast<long>(P.nb[0]) * P.nb[1] * (dim == 3 ? P.nb[2] : 1);
+  for (long blk = 0; blk < nblk; ++blk)
+  {
+    const int co[3] = { static_cast<int>(blk % P.nb[0]), static_cast<int>((blk / P.nb[0]) % P.nb[1]),
+                        static_cast<int>(blk / (static_cast<long>(P.nb[0]) * P.nb[1])) };
+    for (int cell = 0; cell < ncell; ++cell)
+    {
+      const int cc[3] = { cell % b, (cell / b) % b, dim == 3 ? cell / (b * b) : 0 };
+      double    xyz[3];
+      for (int d = 0; d < 3; ++d)
+        xyz[d] = P.xmin[d] + static_cast<double>(co[d] * b + cc[d]) * P.dx + 0.5 * P.dx;
+      int r = 0;
+      if (!(xyz[0] <= xd))
+      {
+        bool in1 = xyz[1] <= yd;
+        if (dim == 3)
+          in1 = std::sqrt(xyz[1] * xyz[1] + xyz[2] * xyz[2]) <= yd;
+        r = in1 ? 1 : 2;
+      }
+      State u = S[r];
+      u[H.ID] = u[H.IAD0] + u[H.IAD1];
+      for (int v = 0; v < H.nbvar; ++v)
+        Uown[(blk * H.nbvar + v) * ncell + cell] = u[v];
+    }
+  }
+}
+
 } // namespace g5o
 
 // =============================================================================================
@@ -2136,6 +2361,41 @@ void
 g5o_youngs(int dim, const double * a, double * n)
 {
   youngs_normal(dim, a, n);
+}
+
+// ---- initial conditions (regions: 7 values each, {alpha0, alpha_rho0, alpha_rho1, p, u, v, w}) ----
+void
+g5o_region_state(const g5o_params * p, const double * region, double * U)
+{
+  Eos e = make_eos(p->gamma, p->pinf);
+  const State u = region_state(p->dim, e, region);
+  for (int v = 0; v < Ids(p->dim).nbvar; ++v)
+    U[v] = u[v];
+}
+
+double
+g5o_box_fraction_below_plane(int dim, const double * n, double alpha, const double * lo, double dx)
+{
+  return box_fraction_below_plane(dim, n, alpha, lo, dx);
+}
+
+double
+g5o_tangent_sphere_fraction(int dim, const double * lo, double dx, const double * c, double R)
+{
+  return tangent_sphere_fraction(dim, lo, dx, c, R);
+}
+
+void
+g5o_init_shock_bubble(const g5o_params * p, const double * regions, double x_front, const double * center,
+                      double radius, double * Uown)
+{
+  init_shock_bubble(*p, regions, x_front, center, radius, Uown);
+}
+
+void
+g5o_init_triple_point(const g5o_params * p, const double * regions, double xd, double yd, double * Uown)
+{
+  init_triple_point(*p, regions, xd, yd, Uown);
 }
 
 } // extern "C"

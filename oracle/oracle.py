@@ -90,6 +90,11 @@ def lib(variant=""):
             "g5o_tanh": (d, [d]),
             "g5o_thinc_alpha": (d, [d, d, d, d, d, i32]),
             "g5o_youngs": (None, [i32, P(d), P(d)]),
+            "g5o_region_state": (None, [P(OracleParams), P(d), P(d)]),
+            "g5o_box_fraction_below_plane": (d, [i32, P(d), d, P(d), d]),
+            "g5o_tangent_sphere_fraction": (d, [i32, P(d), d, P(d), d]),
+            "g5o_init_shock_bubble": (None, [P(OracleParams), P(d), d, P(d), d, P(d)]),
+            "g5o_init_triple_point": (None, [P(OracleParams), P(d), d, d, P(d)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -288,3 +293,58 @@ def source_factors(cfg, u):
     out = np.zeros(2)
     lib().g5o_source_factors(ctypes.byref(p), _dp(uu), _dp(out))
     return out[0], out[1]
+
+
+# ---- initial conditions (common.h:68-144, init/InitShockBubble.cpp:50-225,
+# init/InitTriplePoint.cpp:46-133); a region is (alpha0, (alpha_rho0, alpha_rho1), p, u, v, w) ----
+def _regions(regions):
+    out = []
+    for r in regions:
+        alpha0, (ar0, ar1), p = r[0], r[1], r[2]
+        vel = list(r[3:]) + [0.0] * (3 - len(r[3:]))
+        out += [alpha0, ar0, ar1, p] + vel[:3]
+    return np.array(out, dtype=np.float64)
+
+
+def region_state(cfg, region):
+    """Conservative state (nvar values) of a region (get_region_init_state)."""
+    u = np.zeros(8 if cfg.dim == 2 else 9)
+    r = _regions([region])
+    lib().g5o_region_state(ctypes.byref(params_from(cfg)), _dp(r), _dp(u))
+    return u
+
+
+def box_fraction_below_plane(n, alpha, lo, dx):
+    """Fraction of the box [lo, lo + dx]^d where n.x <= alpha (exact)."""
+    nn = np.ascontiguousarray(n, dtype=np.float64)
+    ll = np.ascontiguousarray(lo, dtype=np.float64)
+    return lib().g5o_box_fraction_below_plane(nn.size, _dp(nn), alpha, _dp(ll), dx)
+
+
+def tangent_sphere_fraction(lo, dx, center, radius):
+    """Volume fraction of the cell inside the sphere by the tangent plane nearest its centre."""
+    ll = np.ascontiguousarray(lo, dtype=np.float64)
+    cc = np.ascontiguousarray(center, dtype=np.float64)
+    return lib().g5o_tangent_sphere_fraction(ll.size, _dp(ll), dx, _dp(cc), radius)
+
+
+def _state_size(cfg):
+    nb = cfg.nb[0] * cfg.nb[1] * (cfg.nb[2] if cfg.dim == 3 else 1)
+    return nb * (8 if cfg.dim == 2 else 9) * cfg.bsize ** cfg.dim
+
+
+def init_shock_bubble(cfg, regions, x_front, center, radius):
+    """Shock-bubble initial state [block][var][cell] (owned blocks in brick order)."""
+    U = np.empty(_state_size(cfg))
+    r = _regions(regions)
+    c = np.array(list(center) + [0.0] * (3 - len(center)), dtype=np.float64)
+    lib().g5o_init_shock_bubble(ctypes.byref(params_from(cfg)), _dp(r), x_front, _dp(c), radius, _dp(U))
+    return U
+
+
+def init_triple_point(cfg, regions, xd, yd):
+    """Triple-point initial state [block][var][cell] (owned blocks in brick order)."""
+    U = np.empty(_state_size(cfg))
+    r = _regions(regions)
+    lib().g5o_init_triple_point(ctypes.byref(params_from(cfg)), _dp(r), xd, yd, _dp(U))
+    return U

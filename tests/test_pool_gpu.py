@@ -215,6 +215,38 @@ def test_device_triple_point_and_box(g5):
         assert area == pytest.approx(0.5 * 0.4, rel=1e-12)
 
 
+def test_device_initial_conditions_match_oracle(g5, orc):
+    # the device generators against the oracle's restatement of InitShockBubble / InitTriplePoint:
+    # bit for bit away from the bubble surface and for the triple point, cut-cell fractions within
+    # the sub-cell quadrature's distance from the tangent-plane value
+    import math
+
+    import test_oracle_init as T
+
+    say("device IC vs oracle: shock-bubble 2-D/3-D, triple point 2-D/3-D")
+    cfg = g5.make_config(2, (16, 8), 1.0 / 128, gamma=(1.4, 1.648))
+    regs = T.regions_air_helium(g5)
+    cfg3, _ = cases.air_water_3d(nb=2)
+    regs3 = T.regions_air_water(g5)
+    for c, rg, args, a_pre, a_bub, tol, exact in (
+            (cfg, regs, (0.6, (1.0, 0.5), 0.25), 1.0, 0.0, 0.02, math.pi * (0.25 * 128) ** 2),
+            (cfg3, regs3, (0.25, (0.5, 0.5, 0.5), 0.2), 0.0, 1.0, 0.04, 4.0 / 3.0 * math.pi * (0.2 * 32) ** 3)):
+        with g5.Pool(c.dim, thinc=bool(c.thinc_enabled), budget_bytes=1 << 28) as pool:
+            sid = pool.admit(c)
+            pool.init_shock_bubble(sid, T.g5_regions(g5, rg), args[0], args[1], args[2], shape=0)
+            dev = pool.download(sid)
+        T.compare_bubble(dev, orc.init_shock_bubble(c, rg, *args), c, a_pre, a_bub, tol, exact)
+    for dim, nb in ((2, (7, 3)), (3, (4, 2, 2))):
+        c = g5.make_config(dim, nb, 1.0 / 16 if dim == 3 else 1.0 / 48, gamma=(1.5, 4.4), pinf=(0.0, 6e8))
+        rg = [(1.0, (1.0, 0.0), 1.0, 0.5), (0.0, (0.0, 1000.0), 1e5), (1.0, (0.125, 0.0), 0.1)]
+        yd = 1.5 if dim == 3 else 0.75
+        with g5.Pool(dim, budget_bytes=1 << 27) as pool:
+            sid = pool.admit(c)
+            pool.init_triple_point(sid, T.g5_regions(g5, rg), 1.0, yd)
+            dev = pool.download(sid)
+        assert np.array_equal(dev, orc.init_triple_point(c, rg, 1.0, yd)), dim
+
+
 def test_device_triple_point_3d_matches_host(g5):
     say("device IC triple point 3d vs host")
     cfg = g5.make_config(3, (4, 2, 2), 1.0 / 16, gamma=(1.5, 1.4))
