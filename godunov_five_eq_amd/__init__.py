@@ -74,6 +74,7 @@ class G5Launch(ctypes.Structure):
         ("xcd_remap", ctypes.c_int32),
         ("zsplit", ctypes.c_int32),
         ("ring_wave", ctypes.c_int32),
+        ("col_len", ctypes.c_int32),
         ("autotuned", ctypes.c_int32),
         ("tuned_ms", ctypes.c_double),
     ]
@@ -769,19 +770,19 @@ class Pool:
         self._check(lib().g5_pool_last_step_ms(self._h, ctypes.byref(ms)))
         return ms.value
 
-    def set_launch(self, wg_per_cu=None, xcd_remap=1, zsplit=1, ring_wave=0):
+    def set_launch(self, wg_per_cu=None, xcd_remap=1, zsplit=1, ring_wave=0, col_len=1):
         """Fix the step kernels' launch geometry; wg_per_cu=None returns to autotuning."""
         if wg_per_cu is None:
             self._check(lib().g5_pool_set_launch(self._h, None))
             return
-        L = G5Launch(wg_per_cu, xcd_remap, zsplit, ring_wave, 0, 0.0)
+        L = G5Launch(wg_per_cu, xcd_remap, zsplit, ring_wave, col_len, 0, 0.0)
         self._check(lib().g5_pool_set_launch(self._h, ctypes.byref(L)))
 
     def launch(self):
         L = G5Launch()
         self._check(lib().g5_pool_get_launch(self._h, ctypes.byref(L)))
         return dict(wg_per_cu=L.wg_per_cu, xcd_remap=L.xcd_remap, zsplit=L.zsplit, ring_wave=L.ring_wave,
-                    autotuned=bool(L.autotuned), tuned_ms=L.tuned_ms)
+                    col_len=L.col_len, autotuned=bool(L.autotuned), tuned_ms=L.tuned_ms)
 
     def set_autotune(self, enable):
         self._check(lib().g5_pool_set_autotune(self._h, 1 if enable else 0))

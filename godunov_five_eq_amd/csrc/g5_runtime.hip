@@ -120,6 +120,11 @@ struct g5_pool
   BlockDesc * d_blocks = nullptr;
   int64_t     cap_blocks = 0;
   int32_t     nlist = 0;
+  // 3-D one-level lists: the step kernels' list of z-columns of kMaxColumn blocks (padded with
+  // solve = -1), ncols of them
+  BlockDesc * d_cols = nullptr;
+  int64_t     cap_cols = 0;
+  int32_t     ncols = 0;
   bool        list_dirty = true;
   bool        any_inlet = false; // a live solve has an inflow border: the 3-D step kernels resolve it
   bool        any_mp = false;    // a live solve has the MILLER_PUCKETT source term
@@ -153,7 +158,7 @@ struct g5_pool
 
   // launch geometry of the step kernels: fixed by the caller, or chosen by the autotuner for the
   // current launch-list size class
-  LaunchCfg launch{ 0, 1, 1, 0 };
+  LaunchCfg launch{ 0, 1, 1, 0, kMaxColumn };
   bool      launch_fixed = false;
   bool      autotune = true;
   int32_t   tuned_class = -1; // size class the current launch was tuned (or looked up) for
@@ -273,6 +278,7 @@ pool_create(int dim, int thinc, int precision, uint64_t budget, bool rk2_stage, 
 {
   g5_pool * p = new g5_pool();
   p->dim = dim;
+  p->launch.col_len = dim == 3 ? kMaxColumn : 1;
   p->thinc = thinc;
   p->precision = precision;
   p->nvar = dim == 2 ? 8 : 9;
@@ -612,6 +618,36 @@ rebuild_list(g5_pool * p)
                                   p->stream));
   G5_HIP(p->err, hipStreamSynchronize(p->stream));
   p->nlist = static_cast<int32_t>(list.size());
+  // 3-D one-level lists: columns of kMaxColumn consecutive z blocks per solve, in the order
+  // (column z range, y, x), padded with solve = -1 where nb[2] is not a multiple
+  std::vector<BlockDesc> cols;
+  if (p->dim == 3 and !two)
+    for (int32_t s = 0; s < static_cast<int32_t>(p->solves.size()); ++s)
+    {
+      SolveRec const & r = p->solves[s];
+      if (!r.live)
+        continue;
+      const int nx = r.cfg.nb[0], ny = r.cfg.nb[1], nz = r.cfg.nb[2];
+      for (int z0 = 0; z0 < nz; z0 += kMaxColumn)
+        for (int y = 0; y < ny; ++y)
+          for (int x = 0; x < nx; ++x)
+            for (int z = z0; z < z0 + kMaxColumn; ++z)
+              cols.push_back(z < nz ? BlockDesc{ s, r.slots[x + nx * (y + ny * z)], static_cast<int16_t>(x),
+                                                 static_cast<int16_t>(y), static_cast<int16_t>(z), 0 }
+                                    : BlockDesc{ -1, -1, 0, 0, 0, 0 });
+    }
+  if (!cols.empty())
+  {
+    int64_t cc = p->cap_cols;
+    rc = grow(p, p->d_cols, cc, static_cast<int64_t>(cols.size()), p->stream);
+    if (rc != G5_OK)
+      return rc;
+    p->cap_cols = cc;
+    G5_HIP(p->err, hipMemcpyAsync(p->d_cols, cols.data(), cols.size() * sizeof(BlockDesc), hipMemcpyHostToDevice,
+                                  p->stream));
+    G5_HIP(p->err, hipStreamSynchronize(p->stream));
+  }
+  p->ncols = static_cast<int32_t>(cols.size() / kMaxColumn);
   p->list_dirty = false;
   p->graph_valid = false;
   return G5_OK;
@@ -635,6 +671,17 @@ step_args(g5_pool * p, BlockDesc const * blocks, int32_t n, double dt_fixed)
   A.remap = p->launch.remap;
   A.dt_fixed = dt_fixed;
   A.iflux = p->d_iflux;
+  A.col_stride = 1;
+  A.col_len = 1;
+  if (p->dim == 3 and !p->any_two_level and blocks == p->d_blocks and p->ncols > 0)
+  {
+    // the 3-D step marches z-columns: items of col_len blocks of the column list
+    A.blocks = p->d_cols;
+    A.col_stride = kMaxColumn;
+    A.col_len = p->launch.col_len;
+    A.nblocks = p->ncols * (kMaxColumn / p->launch.col_len);
+    A.per_xcd = (A.nblocks + 7) / 8;
+  }
   return A;
 }
 
@@ -701,10 +748,13 @@ tune_candidates(g5_pool const * p)
   std::vector<LaunchCfg> c;
   for (int z = (p->dim == 3 ? 0 : 1); z <= 1; ++z)
   {
-    c.push_back({ 0, 1, z, 0 });
-    c.push_back({ 0, 0, z, 0 });
+    c.push_back({ 0, 1, z, 0, 1 });
+    c.push_back({ 0, 0, z, 0, 1 });
     if (p->dim == 2) // a fifth wavefront for the ring work: shorter per-block latency, fewer registers per lane
-      c.push_back({ 0, 1, z, 1 });
+      c.push_back({ 0, 1, z, 1, 1 });
+    else // z-columns of 2 and 4 blocks per workgroup (fewer column ends and prologues per plane)
+      for (int cl = 2; cl <= kMaxColumn; cl *= 2)
+        c.push_back({ 0, 1, z, 0, cl });
   }
   return c;
 }
@@ -1305,6 +1355,7 @@ g5_pool_destroy(g5_pool * p)
   hipFree(p->d_ctl);
   hipFree(p->d_slots);
   hipFree(p->d_blocks);
+  hipFree(p->d_cols);
   hipFree(p->d_start_list);
   hipFree(p->d_start_ids);
   hipFree(p->d_xfer);
@@ -1577,6 +1628,7 @@ to_launch(LaunchCfg const & c, g5_pool const * p, g5_launch * out)
   out->xcd_remap = c.remap;
   out->zsplit = c.zsplit;
   out->ring_wave = c.ring_wave;
+  out->col_len = c.col_len;
   out->autotuned = (!p->launch_fixed and p->autotune and p->tuned_class >= 0) ? 1 : 0;
   out->tuned_ms = p->tuned_ms;
 }
@@ -1596,7 +1648,11 @@ set_launch(g5_pool * p, g5_launch const * l)
     return fail(p->err, G5_ERR_UNSUPPORTED, "the 3-D step has no persistent form (wg_per_cu must be 0)");
   if (l->ring_wave != 0 and (p->dim != 2 or l->wg_per_cu != 0))
     return fail(p->err, G5_ERR_UNSUPPORTED, "ring_wave is a 2-D form with one workgroup per block");
-  p->launch = LaunchCfg{ l->wg_per_cu, l->xcd_remap, l->zsplit, l->ring_wave };
+  if (l->col_len != 1 and l->col_len != 2 and l->col_len != kMaxColumn)
+    return fail(p->err, G5_ERR_INVALID, "launch: col_len in {1, 2, 4}");
+  if (l->col_len != 1 and p->dim != 3)
+    return fail(p->err, G5_ERR_UNSUPPORTED, "col_len > 1 is a form of the 3-D step");
+  p->launch = LaunchCfg{ l->wg_per_cu, l->xcd_remap, l->zsplit, l->ring_wave, p->dim == 3 ? l->col_len : 1 };
   p->launch_fixed = true;
   p->graph_valid = false;
   return G5_OK;
@@ -2354,6 +2410,8 @@ g5_step_noalloc(const g5_config * cfg, const double * U_dev, double * U2_dev, do
   A.remap = 1;
   A.dt_fixed = dt;
   A.iflux = nullptr; // one level
+  A.col_stride = 1;  // one block per workgroup
+  A.col_len = 1;
   const bool hancock = c.integrator == G5_TI_HANCOCK;
   const bool mp = c.source_term == G5_SOURCE_MILLER_PUCKETT;
   if (c.dim == 2)

@@ -575,7 +575,7 @@ iflux3_dest(SolveDesc const & S, int32_t const * __restrict__ slots, BlockDesc c
 // that do run in a launch of their own before the rest of the list)
 template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, bool MP, typename T, bool FAST, bool AMR = false>
 G5D void
-step3d_body(StepArgs const & A, int blk)
+step3d_body(StepArgs const & A, int item)
 {
   static_assert(!AMR or sizeof(T) == sizeof(double), "two-level solves exchange FP64 face records");
   __shared__ T      sQ[3 * QPL];
@@ -588,14 +588,26 @@ step3d_body(StepArgs const & A, int blk)
   __shared__ double sFXE[NF3];
   __shared__ double sFYE[NF3];
   __shared__ double sRed[4];
-  __shared__ int    sSlot[27];
+  __shared__ int    sSlot[kMaxColumn * 27];
 
+  // the work item: L <= col_len consecutive z blocks (x, y, z0 + k) of one solve, marched through as
+  // one column of planes 0 .. 16 L - 1 (a block's own planes are 16 k .. 16 k + 15); two-level
+  // solves run one block per item
   const int       t = threadIdx.x;
-  const BlockDesc bd = A.blocks[blk];
+  const int       per = A.col_stride / A.col_len;
+  const int       first = (item / per) * A.col_stride + (item % per) * A.col_len;
+  int             L = 1;
+  while (L < A.col_len and A.blocks[first + L].solve >= 0)
+    ++L;
+  const BlockDesc bd = A.blocks[first];
+  if (bd.solve < 0)
+    return; // padding of a short column
   const SolveDesc S = AMR ? at_level(A.solves[bd.solve], bd.level) : A.solves[bd.solve];
   SolveCtl *      C = A.ctl + bd.solve;
   constexpr int   NC = 4096;
-  const int64_t   blk_off = static_cast<int64_t>(bd.slot) * 9 * NC;
+  // offset of block k of the column in the state buffers
+  auto            col_off = [&](int k) { return static_cast<int64_t>(A.blocks[first + k].slot) * 9 * NC; };
+  const int       zend = 16 * L; // first plane above the column
   const bool      host_dt = A.dt_fixed > 0.0;
   if (!host_dt and C->active == 0)
     return; // finished solve: its state stays in buffer `parity`
@@ -646,11 +658,13 @@ step3d_body(StepArgs const & A, int blk)
 
   // neighbour slots once per block (LDS), so that a fetch issues its loads without a dependent
   // global read of the slot table (a two-level solve's ghosts are resolved by level instead)
-  if (!two_level)
-    fill_slot_table<3>(S, A.slots, bd.bx, bd.by, bd.bz, t, sSlot);
+  if (!two_level and t < 27 * L)
+    fill_slot_table<3>(S, A.slots, bd.bx, bd.by, bd.bz + t / 27, t % 27, sSlot + 27 * (t / 27));
   __syncthreads();
   // raw loads; the WALL momentum sign is applied at the conversion, so that the prefetch of a
   // plane is not waited for in the phase that issues it
+  // the block of the column that plane g (in [-2, 16 L + 1]) is resolved from, and g relative to it
+  auto blk_of = [&](int g) { return g < 0 ? 0 : (g >= zend ? L - 1 : g >> 4); };
   auto fetch_raw = [&](int gi, int gj, int gk, int & neg) {
     if constexpr (AMR)
       if (two_level)
@@ -658,7 +672,9 @@ step3d_body(StepArgs const & A, int blk)
         neg = -1; // border conditions resolved inside
         return ghost3_two_level<INLET>(S, A.slots, U, bd, gi, gj, gk);
       }
-    const int64_t off = resolve_cell_tab<3, B3, INLET>(S, sSlot, bd.bx, bd.by, bd.bz, gi, gj, gk, neg);
+    const int     kb = blk_of(gk);
+    const int64_t off = resolve_cell_tab<3, B3, INLET>(S, sSlot + 27 * kb, bd.bx, bd.by, bd.bz + kb, gi, gj,
+                                                       gk - 16 * kb, neg);
     return load_cons<3, B3>(U, off, -1);
   };
   auto fetch = [&](int gi, int gj, int gk) {
@@ -687,9 +703,10 @@ step3d_body(StepArgs const & A, int blk)
   double   inv_max = 0.0;
   uint32_t fail = 0u; // G5_FAIL_* bits of the lane's new cells
 
-  // column offsets at plane 0 of the own column and of the extra conversion item: within the
-  // block's z range (dz = 0) no border transform acts on z, so plane k is at offset + 256 k
-  const int64_t own_off0 = blk_off + i + 16 * j;
+  // column offsets at plane 0 of the current block of the own column and of the extra conversion
+  // item: within a block's z range (dz = 0) no border transform acts on z, so its plane k is at
+  // offset + 256 k (the extra item's offset is resolved again when the march enters a block)
+  const int     own_c = i + 16 * j;
   const bool    inlet_block = INLET and S.inlet != G5_INLET_NONE and bd.bx == 0; // halo cells may be inflow cells
   int           x_neg0 = -1;
   int64_t       x_off0 = 0;
@@ -721,7 +738,7 @@ step3d_body(StepArgs const & A, int blk)
     {
       apply_face_pair<3>(u2prev, zFprev, zF, uaprev0, uaprev1, dtdx, 2, stprev0, stprev1);
       fix_volume_fractions<FAST>(u2prev);
-      store_cons<3, B3>(U2, blk_off + i + 16 * j + 256 * (P - 1), u2prev);
+      store_cons<3, B3>(U2, col_off((P - 1) >> 4) + i + 16 * j + 256 * ((P - 1) & 15), u2prev);
       double       pc;
       const double rate = cell_rate<3, T, FAST>(S, u2prev, pc);
       inv_max = fmax(inv_max, rate);
@@ -735,22 +752,22 @@ step3d_body(StepArgs const & A, int blk)
     {
       // the block's bottom (face 0) and top (face 16) z faces across a level interface
       const int sd = P == 0 ? 4 : 5;
-      if (P == 0 or P == 16)
+      if (P == 0 or P == zend)
       {
         if ((finer >> sd) & 1)
           zF = iflux3_sum(A.iflux, bd.slot, sd, i + 16 * j);
         else if ((coarser >> sd) & 1)
           iflux3_store(A.iflux, iflux3_dest(S, A.slots, bd, sd, bd.bx * B3 + i, bd.by * B3 + j,
-                                            bd.bz * B3 + (P == 0 ? 0 : 15)), zF);
+                                            bd.bz * B3 + (P == 0 ? 0 : zend - 1)), zF);
       }
     }
     if (!LATE)
       z_complete(P);
   };
 
-  for (int P = -1; P <= 16; ++P)
+  for (int P = -1; P <= zend; ++P)
   {
-    const bool inner_plane = P >= 0 and P <= 15;
+    const bool inner_plane = P >= 0 and P < zend;
     // (a) plane P+1 from the prefetched raw state
     negate_momentum(pf_i, neg_i); // the own column is never left of x-min: no inflow cell
     put(P + 1, i, j, pf_i);
@@ -763,10 +780,12 @@ step3d_body(StepArgs const & A, int blk)
     __syncthreads(); // (b)
 
     // (c) prefetch plane P+2, predictor of plane P
-    if (P + 2 <= 15 and not inlet_block)
+    if (P + 2 < zend and not inlet_block)
     {
-      // planes of the block's own z range: each lane's column offset is linear in the plane
-      pf_i = load_cons<3, B3>(U, own_off0 + 256 * (P + 2), -1);
+      // planes of the column's own z range: within a block each lane's column offset is linear in
+      // the plane
+      const int kb = (P + 2) >> 4, pl = (P + 2) & 15;
+      pf_i = load_cons<3, B3>(U, col_off(kb) + own_c + 256 * pl, -1);
       neg_i = -1;
       if (has_x)
       {
@@ -774,12 +793,14 @@ step3d_body(StepArgs const & A, int blk)
           pf_x = fetch_raw(xgi, xgj, P + 2, neg_x);
         else
         {
-          pf_x = load_cons<3, B3>(U, x_off0 + 256 * (P + 2), -1);
+          if (pl == 0 and kb > 0) // the march enters block kb
+            x_off0 = resolve_cell_tab<3, B3, INLET>(S, sSlot + 27 * kb, bd.bx, bd.by, bd.bz + kb, xgi, xgj, 0, x_neg0);
+          pf_x = load_cons<3, B3>(U, x_off0 + 256 * pl, -1);
           neg_x = x_neg0;
         }
       }
     }
-    else if (P + 2 <= 17)
+    else if (P + 2 <= zend + 1)
     {
       pf_i = fetch_raw(i, j, P + 2, neg_i);
       if (has_x)

@@ -100,11 +100,18 @@ struct StepArgs
   // [side xmin, xmax, ymin, ymax][face 0..15][9 values], written by the fine blocks' launch and
   // read by their coarse neighbours in the next launch
   double *          iflux;
+  // 3-D z-columns: the list holds columns of col_stride consecutive z blocks of one solve (padded
+  // with solve = -1 entries), and work item w (nblocks of them) marches col_len of them: column
+  // w / (col_stride / col_len), part w % (col_stride / col_len); a plain list has both 1
+  int32_t           col_stride;
+  int32_t           col_len;
 };
 constexpr int kIfluxDoubles = 4 * 16 * 8; // interface records per slot (2-D)
 // 3-D: per coarse slot, [side xmin, xmax, ymin, ymax, zmin, zmax][face cell 0..255][fine face 0..3]
 // [ar0, ar1, a0, a1, e, mu, mv, mw, us]
 constexpr int kIflux3Doubles = 6 * 256 * 4 * 9;
+// 3-D z-columns: at most this many blocks per column of the launch list (StepArgs.col_stride)
+constexpr int kMaxColumn = 4;
 
 // refine-flag computation (g5_refine_flags): parameters and the per-slot outputs
 struct RefineArgs
@@ -125,6 +132,7 @@ struct LaunchCfg
   int32_t remap;     // XCD-contiguous block ranges
   int32_t zsplit;    // 3-D: z Riemann solve split around the x/y exchange
   int32_t ring_wave; // 2-D, one workgroup per block: a fifth wavefront for the ring work
+  int32_t col_len;   // 3-D: blocks marched in z by one workgroup (1, 2 or 4)
 };
 
 // filling the host-side SolveDesc from a g5_config

@@ -260,30 +260,34 @@ def test_device_triple_point_3d_matches_host(g5):
 
 def test_launch_geometries_bit_identical(g5):
     # every launch geometry the autotuner may pick computes the same bits; the 2-D persistent
-    # geometries loop over several blocks per workgroup here (1920 blocks, 256..1024 workgroups)
-    say("launch geometries: 2-D FP64/mixed, 3-D FP64/mixed")
-    for dim, prec in ((2, g5.PREC_FP64), (2, g5.PREC_MIXED), (3, g5.PREC_FP64), (3, g5.PREC_MIXED)):
+    # geometries loop over several blocks per workgroup here (1920 blocks, 256..1024 workgroups);
+    # the 3-D z-columns of 2 and 4 blocks march through 3 blocks in z (a column padded by one)
+    say("launch geometries: 2-D FP64/mixed, 3-D FP64/mixed/reassociated")
+    for dim, prec in ((2, g5.PREC_FP64), (2, g5.PREC_MIXED), (3, g5.PREC_FP64), (3, g5.PREC_MIXED),
+                      (3, g5.PREC_FP64_FAST)):
         if dim == 2:
             cfg, U0 = cases.air_helium_2d(nbx=48, nby=40, thinc=True, thinc_beta=1.6, precision=prec)
-            geoms = [(0, 1, 1, 0), (0, 0, 1, 0), (1, 1, 1, 0), (2, 1, 1, 0), (4, 0, 1, 0), (0, 1, 1, 1), (0, 0, 1, 1)]
+            geoms = [(0, 1, 1, 0, 1), (0, 0, 1, 0, 1), (1, 1, 1, 0, 1), (2, 1, 1, 0, 1), (4, 0, 1, 0, 1),
+                     (0, 1, 1, 1, 1), (0, 0, 1, 1, 1)]
         else:
             cfg, U0 = cases.air_water_3d(nb=3, precision=prec)
-            geoms = [(0, 1, 1, 0), (0, 0, 1, 0), (0, 1, 0, 0), (0, 0, 0, 0)]
+            geoms = [(0, 1, 1, 0, 1), (0, 0, 1, 0, 1), (0, 1, 0, 0, 1), (0, 0, 0, 0, 1), (0, 1, 1, 0, 2),
+                     (0, 1, 1, 0, 4), (0, 0, 0, 0, 4), (0, 1, 0, 0, 2)]
         ref = None
-        for wg, remap, zs, rw in geoms:
-            say(f"dim={dim} prec={prec} launch wg_per_cu={wg} remap={remap} zsplit={zs} ring_wave={rw}")
+        for wg, remap, zs, rw, cl in geoms:
+            say(f"dim={dim} prec={prec} launch wg_per_cu={wg} remap={remap} zsplit={zs} ring_wave={rw} col_len={cl}")
             with g5.Pool(dim, thinc=True, precision=prec, budget_bytes=1 << 28) as pool:
-                pool.set_launch(wg, remap, zs, rw)
+                pool.set_launch(wg, remap, zs, rw, cl)
                 sid = pool.admit(cfg, max_steps=6)
                 pool.upload(sid, U0)
                 pool.start([sid])
                 pool.step(6)
                 U = pool.download(sid)
-                assert pool.launch()["wg_per_cu"] == wg
+                assert pool.launch()["wg_per_cu"] == wg and pool.launch()["col_len"] == cl
             if ref is None:
                 ref = U
             else:
-                assert np.array_equal(U, ref), (dim, prec, wg, remap, zs, rw)
+                assert np.array_equal(U, ref), (dim, prec, wg, remap, zs, rw, cl)
         # the autotuned pool gives the same bits as well
         with g5.Pool(dim, thinc=True, precision=prec, budget_bytes=1 << 28) as pool:
             sid = pool.admit(cfg, max_steps=6)
@@ -294,15 +298,56 @@ def test_launch_geometries_bit_identical(g5):
             say(f"dim={dim} prec={prec} autotuned launch {pool.launch()}")
 
 
+def test_z_columns_in_a_pool_of_uneven_solves(g5):
+    # three 3-D solves of 3, 5 and 2 blocks in z in one pool, marched in z-columns of 4 blocks
+    # (padded columns, columns of different solves side by side in the list): each ends with the
+    # bits of its run alone with one block per workgroup
+    say("z-columns: pool of 3-D solves with 3, 5 and 2 blocks in z")
+    shapes = [(3, 2, 3), (2, 2, 5), (2, 3, 2)]
+    alone = []
+    runs = []
+    for nb in shapes:
+        cfg, U0 = cases.air_water_3d(nb=max(nb))
+        cfg = g5.make_config(3, nb, cfg.dx, bc=("ZERO_GRADIENT",) * 6, cfl=0.5, gamma=(1.4, 4.4), pinf=(0.0, 6e8),
+                             thinc=True, thinc_beta=1.6, thinc_epsilon=1e-3, precision=g5.PREC_FP64_FAST)
+        rho2, u2, p2, _ = g5.rankine_hugoniot(4.4, 6e8, 1000.0, 1e5, p2=1e9)
+        regs = [g5.region(0.0, (0.0, rho2), p2, u2), g5.region(0.0, (0.0, 1000.0), 1e5), g5.region(1.0, (1.0, 0.0), 1e5)]
+        centre = tuple(0.5 * 16 * n * cfg.dx for n in nb)
+        U0 = g5.init_shock_bubble(cfg, regs, 0.25 * 16 * nb[0] * cfg.dx, centre, 0.3 * 16 * min(nb) * cfg.dx)
+        runs.append((cfg, U0))
+        with g5.Pool(3, thinc=True, precision=g5.PREC_FP64_FAST, budget_bytes=1 << 28) as pool:
+            pool.set_launch(0, 1, 1, 0, 1)
+            sid = pool.admit(cfg, max_steps=5)
+            pool.upload(sid, U0)
+            pool.start([sid])
+            pool.step(5)
+            alone.append(pool.download(sid))
+    for cl in (2, 4):
+        with g5.Pool(3, thinc=True, precision=g5.PREC_FP64_FAST, budget_bytes=1 << 29) as pool:
+            pool.set_launch(0, 1, 1, 0, cl)
+            sids = []
+            for cfg, U0 in runs:
+                sids.append(pool.admit(cfg, max_steps=5))
+                pool.upload(sids[-1], U0)
+            pool.start(sids)
+            pool.step(5)
+            for k, sid in enumerate(sids):
+                assert np.array_equal(pool.download(sid), alone[k]), (cl, k)
+
+
 def test_3d_persistent_launch_is_rejected(g5):
     with g5.Pool(3, thinc=True, budget_bytes=1 << 26) as pool:
         with pytest.raises(g5.G5Error):
             pool.set_launch(1, 1, 1)
         with pytest.raises(g5.G5Error):  # the ring wavefront is a 2-D form
             pool.set_launch(0, 1, 1, 1)
+        with pytest.raises(g5.G5Error):  # z-columns of 1, 2 or 4 blocks
+            pool.set_launch(0, 1, 1, 0, 3)
     with g5.Pool(2, thinc=True, budget_bytes=1 << 26) as pool:
         with pytest.raises(g5.G5Error):  # ... with one workgroup per block
             pool.set_launch(2, 1, 1, 1)
+        with pytest.raises(g5.G5Error):  # z-columns are a 3-D form
+            pool.set_launch(0, 1, 1, 0, 2)
 
 
 def test_checkpoint_restart_is_bit_identical(g5, tmp_path):
