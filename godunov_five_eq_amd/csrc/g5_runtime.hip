@@ -752,7 +752,7 @@ tune_candidates(g5_pool const * p)
     c.push_back({ 0, 0, z, 0, 1 });
     if (p->dim == 2) // a fifth wavefront for the ring work: shorter per-block latency, fewer registers per lane
       c.push_back({ 0, 1, z, 1, 1 });
-    else // z-columns of 2 and 4 blocks per workgroup (fewer column ends and prologues per plane)
+    else // z-columns of 2, 4 and 8 blocks per workgroup (fewer column ends and prologues per plane)
       for (int cl = 2; cl <= kMaxColumn; cl *= 2)
         c.push_back({ 0, 1, z, 0, cl });
   }
@@ -1648,8 +1648,8 @@ set_launch(g5_pool * p, g5_launch const * l)
     return fail(p->err, G5_ERR_UNSUPPORTED, "the 3-D step has no persistent form (wg_per_cu must be 0)");
   if (l->ring_wave != 0 and (p->dim != 2 or l->wg_per_cu != 0))
     return fail(p->err, G5_ERR_UNSUPPORTED, "ring_wave is a 2-D form with one workgroup per block");
-  if (l->col_len != 1 and l->col_len != 2 and l->col_len != kMaxColumn)
-    return fail(p->err, G5_ERR_INVALID, "launch: col_len in {1, 2, 4}");
+  if (l->col_len < 1 or l->col_len > kMaxColumn or (l->col_len & (l->col_len - 1)) != 0)
+    return fail(p->err, G5_ERR_INVALID, "launch: col_len in {1, 2, 4, 8}");
   if (l->col_len != 1 and p->dim != 3)
     return fail(p->err, G5_ERR_UNSUPPORTED, "col_len > 1 is a form of the 3-D step");
   p->launch = LaunchCfg{ l->wg_per_cu, l->xcd_remap, l->zsplit, l->ring_wave, p->dim == 3 ? l->col_len : 1 };

@@ -111,7 +111,7 @@ constexpr int kIfluxDoubles = 4 * 16 * 8; // interface records per slot (2-D)
 // [ar0, ar1, a0, a1, e, mu, mv, mw, us]
 constexpr int kIflux3Doubles = 6 * 256 * 4 * 9;
 // 3-D z-columns: at most this many blocks per column of the launch list (StepArgs.col_stride)
-constexpr int kMaxColumn = 4;
+constexpr int kMaxColumn = 8;
 
 // refine-flag computation (g5_refine_flags): parameters and the per-slot outputs
 struct RefineArgs
@@ -132,7 +132,7 @@ struct LaunchCfg
   int32_t remap;     // XCD-contiguous block ranges
   int32_t zsplit;    // 3-D: z Riemann solve split around the x/y exchange
   int32_t ring_wave; // 2-D, one workgroup per block: a fifth wavefront for the ring work
-  int32_t col_len;   // 3-D: blocks marched in z by one workgroup (1, 2 or 4)
+  int32_t col_len;   // 3-D: blocks marched in z by one workgroup (1, 2, 4 or 8)
 };
 
 // filling the host-side SolveDesc from a g5_config

@@ -453,14 +453,14 @@ int g5_pool_last_step_ms(g5_pool * pool, double * ms);
  * next CFL rate, so the extra launches change nothing) and the fastest is kept for the process,
  * per dimension, THINC flag, precision, integrator and size class.  G5_AUTOTUNE=0 in the
  * environment turns tuning off (default geometry: one workgroup per block, XCD ranges, z split;
- * in 3-D one workgroup per column of 4 blocks). */
+ * in 3-D one workgroup per column of 8 blocks). */
 typedef struct g5_launch
 {
   int32_t wg_per_cu; /* 0: one workgroup per block; 1..16: persistent workgroups per CU (2-D only) */
   int32_t xcd_remap; /* 1: each XCD takes a contiguous range of the block list */
   int32_t zsplit;    /* 3-D: z Riemann solve split around the x/y flux exchange (1) or not (0) */
   int32_t ring_wave; /* 2-D, wg_per_cu 0: a fifth wavefront per workgroup for the ring cells (1) */
-  int32_t col_len;   /* 3-D: blocks one workgroup marches through in z, 1, 2 or 4 (one-level lists) */
+  int32_t col_len;   /* 3-D: blocks one workgroup marches through in z, 1, 2, 4 or 8 (one-level lists) */
   int32_t autotuned; /* out: 1 when the geometry came from the autotuner */
   double  tuned_ms;  /* out: best step-kernel time measured by the autotuner (ms) */
 } g5_launch;

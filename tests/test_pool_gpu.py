@@ -261,7 +261,7 @@ def test_device_triple_point_3d_matches_host(g5):
 def test_launch_geometries_bit_identical(g5):
     # every launch geometry the autotuner may pick computes the same bits; the 2-D persistent
     # geometries loop over several blocks per workgroup here (1920 blocks, 256..1024 workgroups);
-    # the 3-D z-columns of 2 and 4 blocks march through 3 blocks in z (a column padded by one)
+    # the 3-D z-columns of 2, 4 and 8 blocks march through 3 blocks in z (padded columns)
     say("launch geometries: 2-D FP64/mixed, 3-D FP64/mixed/reassociated")
     for dim, prec in ((2, g5.PREC_FP64), (2, g5.PREC_MIXED), (3, g5.PREC_FP64), (3, g5.PREC_MIXED),
                       (3, g5.PREC_FP64_FAST)):
@@ -272,7 +272,7 @@ def test_launch_geometries_bit_identical(g5):
         else:
             cfg, U0 = cases.air_water_3d(nb=3, precision=prec)
             geoms = [(0, 1, 1, 0, 1), (0, 0, 1, 0, 1), (0, 1, 0, 0, 1), (0, 0, 0, 0, 1), (0, 1, 1, 0, 2),
-                     (0, 1, 1, 0, 4), (0, 0, 0, 0, 4), (0, 1, 0, 0, 2)]
+                     (0, 1, 1, 0, 4), (0, 0, 0, 0, 4), (0, 1, 0, 0, 2), (0, 1, 1, 0, 8)]
         ref = None
         for wg, remap, zs, rw, cl in geoms:
             say(f"dim={dim} prec={prec} launch wg_per_cu={wg} remap={remap} zsplit={zs} ring_wave={rw} col_len={cl}")
@@ -299,7 +299,7 @@ def test_launch_geometries_bit_identical(g5):
 
 
 def test_z_columns_in_a_pool_of_uneven_solves(g5):
-    # three 3-D solves of 3, 5 and 2 blocks in z in one pool, marched in z-columns of 4 blocks
+    # three 3-D solves of 3, 5 and 2 blocks in z in one pool, marched in z-columns of 2, 4, 8 blocks
     # (padded columns, columns of different solves side by side in the list): each ends with the
     # bits of its run alone with one block per workgroup
     say("z-columns: pool of 3-D solves with 3, 5 and 2 blocks in z")
@@ -322,7 +322,7 @@ def test_z_columns_in_a_pool_of_uneven_solves(g5):
             pool.start([sid])
             pool.step(5)
             alone.append(pool.download(sid))
-    for cl in (2, 4):
+    for cl in (2, 4, 8):
         with g5.Pool(3, thinc=True, precision=g5.PREC_FP64_FAST, budget_bytes=1 << 29) as pool:
             pool.set_launch(0, 1, 1, 0, cl)
             sids = []
@@ -341,7 +341,7 @@ def test_3d_persistent_launch_is_rejected(g5):
             pool.set_launch(1, 1, 1)
         with pytest.raises(g5.G5Error):  # the ring wavefront is a 2-D form
             pool.set_launch(0, 1, 1, 1)
-        with pytest.raises(g5.G5Error):  # z-columns of 1, 2 or 4 blocks
+        with pytest.raises(g5.G5Error):  # z-columns of 1, 2, 4 or 8 blocks
             pool.set_launch(0, 1, 1, 0, 3)
     with g5.Pool(2, thinc=True, budget_bytes=1 << 26) as pool:
         with pytest.raises(g5.G5Error):  # ... with one workgroup per block
