@@ -17,7 +17,8 @@ H5LIB = os.path.join(HERE, "libg5h5.so")
 # HDF5 1.10 C library of the image (headers, libhdf5 and h5dump under /opt/conda)
 HDF5_PREFIX = os.environ.get("G5_HDF5_PREFIX", "/opt/conda")
 
-HIP_SOURCES = ["g5_step2d.hip", "g5_step2d_fast.hip", "g5_step3d.hip", "g5_step3d_fast.hip", "g5_step3d_amr.hip",
+HIP_SOURCES = ["g5_step2d.hip", "g5_step2d_fast.hip", "g5_step2d_mixed.hip", "g5_step3d.hip", "g5_step3d_fast.hip",
+               "g5_step3d_mixed.hip", "g5_step3d_amr.hip",
                "g5_step3d_amr_fast.hip", "g5_aux.hip", "g5_refine.hip", "g5_runtime.hip"]
 CXX_SOURCES = ["g5_init.cpp", "g5_specs.cpp", "g5_serve.cpp"]
 
@@ -42,7 +43,11 @@ FILE_FLAGS = {"g5_step3d.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
               "g5_step3d_fast.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=fast"],
               "g5_step3d_amr.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
               "g5_step3d_amr_fast.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=fast"],
-              "g5_step2d_fast.hip": ["-ffp-contract=fast"]}
+              "g5_step2d_fast.hip": ["-ffp-contract=fast"],
+              # the mixed-precision instances may contract too (held to their accuracy bar against the
+              # FP64 oracle); the 3-D ones keep the 3-D file's scheduler
+              "g5_step2d_mixed.hip": ["-ffp-contract=fast"],
+              "g5_step3d_mixed.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=fast"]}
 
 
 def _run(cmd, verbose):

@@ -13,12 +13,16 @@ namespace g5
 // precision: G5_PREC_* (FP64 bit-exact, mixed, reassociated FP64)
 void launch_step2d(StepArgs const & A, bool thinc, bool hancock, int precision, bool ring_wave, bool mp,
                    hipStream_t stream);
-void launch_step3d(StepArgs const & A, bool thinc, bool hancock, int precision, bool zsplit, bool inlet, bool mp,
+void launch_step3d(StepArgs const & A, bool thinc, bool hancock, int precision, int zsplit, bool inlet, bool mp,
                    hipStream_t stream);
 // the reassociated FP64 instances (separate translation units)
 void launch_step2d_fast(StepArgs const & A, bool thinc, bool hancock, bool ring_wave, bool mp, hipStream_t stream);
 void launch_step3d_fast(StepArgs const & A, bool thinc, bool hancock, bool zsplit, bool inlet, bool mp,
                         hipStream_t stream);
+// the mixed-precision instances (separate translation units, FMA contraction allowed)
+void launch_step2d_mixed(StepArgs const & A, bool thinc, bool hancock, bool ring_wave, bool mp, hipStream_t stream);
+void launch_step3d_mixed(StepArgs const & A, bool thinc, bool hancock, bool zsplit, bool inlet, bool mp,
+                         hipStream_t stream);
 
 // the 2-D step over a launch list with blocks of two-level solves (FP64 or reassociated FP64): the
 // first n_first blocks (the fine blocks that border coarse blocks) in one launch, the rest of the

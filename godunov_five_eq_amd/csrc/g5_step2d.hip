@@ -1,5 +1,6 @@
-// g5_step2d.hip -- the FP64 (bit-exact) and mixed-precision instances of the fused 2-D step (the
-// body is in g5_step2d_body.h; the reassociated FP64 instances are in g5_step2d_fast.hip).
+// g5_step2d.hip -- the FP64 (bit-exact) instances of the fused 2-D step (the body is in
+// g5_step2d_body.h; the reassociated FP64 instances are in g5_step2d_fast.hip, the mixed-precision
+// ones in g5_step2d_mixed.hip).
 #include "g5_step2d_body.h"
 
 namespace g5
@@ -9,8 +10,7 @@ void
 launch_step2d(StepArgs const & A, bool thinc, bool hancock, int precision, bool ring_wave, bool mp, hipStream_t stream)
 {
   if (precision == G5_PREC_MIXED)
-    mp ? launch2<float, true, false>(A, thinc, hancock, ring_wave, stream)
-       : launch2<float, false, false>(A, thinc, hancock, ring_wave, stream);
+    launch_step2d_mixed(A, thinc, hancock, ring_wave, mp, stream);
   else if (precision == G5_PREC_FP64_FAST)
     launch_step2d_fast(A, thinc, hancock, ring_wave, mp, stream);
   else
