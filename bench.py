@@ -338,7 +338,11 @@ def bench_serve(g5, args):
                      "mean_ms": rep["mean_ms"], "max_ms": rep["max_ms"], "wall_s": rep["wall_s"],
                      "max_inflight": rep["max_inflight"], "cell_updates": rep["cell_updates"],
                      "completed": rep["completed"], "failed": rep["failed"],
-                     "last_arrival_s": rep["last_arrival_s"]}
+                     "last_arrival_s": rep["last_arrival_s"],
+                     # where the loop's time goes: HIP-event time of each pool's steps, host time
+                     # per loop phase
+                     "step_s_2d": rep["step_s_2d"], "step_s_3d": rep["step_s_3d"], "admit_s": rep["admit_s"],
+                     "start_s": rep["start_s"], "wait_s": rep["wait_s"], "collect_s": rep["collect_s"]}
     out["roofline"] = None
     return out
 

@@ -648,6 +648,17 @@ rebuild_list(g5_pool * p)
     G5_HIP(p->err, hipStreamSynchronize(p->stream));
   }
   p->ncols = static_cast<int32_t>(cols.size() / kMaxColumn);
+  // without tuning: the longest z-columns that still leave two work items per CU (a short list
+  // marched in long columns would leave CUs idle)
+  if (p->dim == 3 and !p->launch_fixed and !p->autotune)
+  {
+    int cl = kMaxColumn;
+    while (cl > 1 and p->nlist / cl < 2 * p->cus)
+      cl /= 2;
+    if (cl != p->launch.col_len)
+      p->graph_valid = false;
+    p->launch.col_len = cl;
+  }
   p->list_dirty = false;
   p->graph_valid = false;
   return G5_OK;
@@ -673,9 +684,10 @@ step_args(g5_pool * p, BlockDesc const * blocks, int32_t n, double dt_fixed)
   A.iflux = p->d_iflux;
   A.col_stride = 1;
   A.col_len = 1;
-  if (p->dim == 3 and !p->any_two_level and blocks == p->d_blocks and p->ncols > 0)
+  if (p->dim == 3 and !p->any_two_level and blocks == p->d_blocks and p->ncols > 0 and p->launch.col_len > 1)
   {
-    // the 3-D step marches z-columns: items of col_len blocks of the column list
+    // the 3-D step marches z-columns: items of col_len blocks of the column list (with one block
+    // per item the plain list serves, without the columns' padding entries)
     A.blocks = p->d_cols;
     A.col_stride = kMaxColumn;
     A.col_len = p->launch.col_len;

@@ -453,7 +453,7 @@ int g5_pool_last_step_ms(g5_pool * pool, double * ms);
  * next CFL rate, so the extra launches change nothing) and the fastest is kept for the process,
  * per dimension, THINC flag, precision, integrator and size class.  G5_AUTOTUNE=0 in the
  * environment turns tuning off (default geometry: one workgroup per block, XCD ranges, z split;
- * in 3-D one workgroup per column of 8 blocks). */
+ * in 3-D z-columns of 8, 4, 2 or 1 blocks, the longest that leave two per CU). */
 typedef struct g5_launch
 {
   int32_t wg_per_cu; /* 0: one workgroup per block; 1..16: persistent workgroups per CU (2-D only) */
