@@ -658,8 +658,10 @@ step3d_body(StepArgs const & A, int item)
 
   // neighbour slots once per block (LDS), so that a fetch issues its loads without a dependent
   // global read of the slot table (a two-level solve's ghosts are resolved by level instead)
-  if (!two_level and t < 27 * L)
-    fill_slot_table<3>(S, A.slots, bd.bx, bd.by, bd.bz + t / 27, t % 27, sSlot + 27 * (t / 27));
+  // 27 entries per block of the column: more than one per thread beyond 9 blocks
+  if (!two_level)
+    for (int e = t; e < 27 * L; e += 256)
+      fill_slot_table<3>(S, A.slots, bd.bx, bd.by, bd.bz + e / 27, e % 27, sSlot + 27 * (e / 27));
   __syncthreads();
   // raw loads; the WALL momentum sign is applied at the conversion, so that the prefetch of a
   // plane is not waited for in the phase that issues it

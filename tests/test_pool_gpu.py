@@ -335,6 +335,29 @@ def test_z_columns_in_a_pool_of_uneven_solves(g5):
                 assert np.array_equal(pool.download(sid), alone[k]), (cl, k)
 
 
+def test_z_columns_on_every_step_path(g5, orc):
+    # z-columns of 8 blocks against one block per workgroup on the paths the step kernels branch on:
+    # RK2_SSP (stage buffer, two launches per step), a jet inflow border (the INLET instances, with
+    # the per-plane border resolution of inflow blocks), the MILLER_PUCKETT source term, walls on
+    # every face and a periodic z direction (column ends wrap); bit-exact FP64 against the oracle
+    say("z-columns: RK2_SSP, jet inlet, MILLER_PUCKETT, periodic z")
+    runs = [("rk2", cases.air_water_3d(nb=2, integrator="RK2_SSP"), True),
+            ("jet", cases.jet(3, nb=(3, 2, 5)), False),
+            ("mp", cases.air_water_3d(nb=3, source_term="MILLER_PUCKETT"), False),
+            ("walls-periodic-z", cases.air_helium_3d(nb=3, periodic=(0, 0, 1),
+                                                     bc=("WALL",) * 4 + ("PERIODIC", "PERIODIC")), False)]
+    for label, (cfg, U0), rk2 in runs:
+        ref = oracle_run(orc, cfg, U0, 5)
+        for cl in (1, 2, 8):
+            with g5.Pool(3, thinc=bool(cfg.thinc_enabled), budget_bytes=1 << 28, rk2=rk2) as pool:
+                pool.set_launch(0, 1, 1, 0, cl)
+                sid = pool.admit(cfg, max_steps=5)
+                pool.upload(sid, U0)
+                pool.start([sid])
+                pool.step(5)
+                assert np.array_equal(pool.download(sid), ref), (label, cl)
+
+
 def test_3d_persistent_launch_is_rejected(g5):
     with g5.Pool(3, thinc=True, budget_bytes=1 << 26) as pool:
         with pytest.raises(g5.G5Error):
