@@ -28,25 +28,27 @@ var_at(double const * __restrict__ U, int slot, int var, int ci, int cj, int ck)
   return U[(static_cast<int64_t>(slot) * NV + var) * NC + ci + RB * (cj + RB * ck)];
 }
 
-// variable var of level-0 cell (gx, gy) inside the brick of a two-level solve: a coarse leaf's
-// value, or the average of the four fine cells of a refined block (summed x fastest, over 4)
+// variable var of level-0 cell g inside the brick of a two-level solve: a coarse leaf's value, or
+// the average of the 2^DIM fine cells of a refined block (summed x fastest, then y, then z, over
+// 2^DIM)
+template <int DIM>
 G5D double
-coarse_value2(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int gx, int gy,
-              int var)
+coarse_value(SolveDesc const & S, int32_t const * __restrict__ slots, double const * __restrict__ U, int gx, int gy,
+             int gz, int var)
 {
-  const int sl = slots[S.base + (gx >> 4) + S.nb[0] * (gy >> 4)];
+  const int sl = slots[S.base + (gx >> 4) + S.nb[0] * ((gy >> 4) + S.nb[1] * (gz >> 4))];
   if (sl >= 0)
-    return var_at<2>(U, sl, var, gx & 15, gy & 15, 0);
+    return var_at<DIM>(U, sl, var, gx & 15, gy & 15, gz & 15);
   double sum = 0.0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < (1 << DIM); ++k)
   {
-    const int    fx = 2 * gx + (k & 1), fy = 2 * gy + (k >> 1);
-    const int    fs = slots[S.base1 + (fx >> 4) + 2 * S.nb[0] * (fy >> 4)];
-    const double v = var_at<2>(U, fs, var, fx & 15, fy & 15, 0);
+    const int    fx = 2 * gx + (k & 1), fy = 2 * gy + ((k >> 1) & 1), fz = DIM == 3 ? 2 * gz + (k >> 2) : 0;
+    const int    fs = slots[S.base1 + (fx >> 4) + 2 * S.nb[0] * ((fy >> 4) + 2 * S.nb[1] * (fz >> 4))];
+    const double v = var_at<DIM>(U, fs, var, fx & 15, fy & 15, fz & 15);
     sum = k == 0 ? v : sum + v;
   }
-  return sum / 4.0;
+  return sum / (DIM == 3 ? 8.0 : 4.0);
 }
 
 // getNeighborDataSameLevel (scheme/ComputeRefineFlags.cpp:67-96): variable var of the cell one
@@ -60,13 +62,13 @@ neighbour_value(SolveDesc const & S, int32_t const * __restrict__ slots, double 
 {
   int g[3] = { bd.bx * RB + c[0], bd.by * RB + c[1], DIM == 3 ? bd.bz * RB + c[2] : 0 };
   g[dir] += side;
-  if (S.base1 >= 0) // two-level solve (2-D); fine blocks lie inside the brick, away from its border
+  if (S.base1 >= 0) // two-level solve; fine blocks lie at least one block inside the brick's border
   {
     if (bd.level == 1)
     {
-      const int fs = slots[S.base1 + (g[0] >> 4) + 2 * S.nb[0] * (g[1] >> 4)];
-      return fs >= 0 ? var_at<2>(U, fs, var, g[0] & 15, g[1] & 15, 0)
-                     : coarse_value2(S, slots, U, g[0] >> 1, g[1] >> 1, var);
+      const int fs = slots[S.base1 + (g[0] >> 4) + 2 * S.nb[0] * ((g[1] >> 4) + 2 * S.nb[1] * (g[2] >> 4))];
+      return fs >= 0 ? var_at<DIM>(U, fs, var, g[0] & 15, g[1] & 15, g[2] & 15)
+                     : coarse_value<DIM>(S, slots, U, g[0] >> 1, g[1] >> 1, g[2] >> 1, var);
     }
     const int n = S.nb[dir] * RB;
     if (g[dir] < 0 or g[dir] >= n)
@@ -75,7 +77,7 @@ neighbour_value(SolveDesc const & S, int32_t const * __restrict__ slots, double 
         return self;
       g[dir] = g[dir] < 0 ? g[dir] + n : g[dir] - n;
     }
-    return coarse_value2(S, slots, U, g[0], g[1], var);
+    return coarse_value<DIM>(S, slots, U, g[0], g[1], g[2], var);
   }
   const int n = S.nb[dir] * RB;
   if (g[dir] < 0 or g[dir] >= n)

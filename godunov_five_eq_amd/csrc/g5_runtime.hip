@@ -2237,8 +2237,6 @@ g5_refine_flags(g5_ctx * ctx, const g5_refine_params * prm, int32_t * flags, dou
     return fail(ctx->err, G5_ERR_INVALID, "n must equal the number of blocks");
   if (prm->indicator < G5_INDICATOR_LOHNER_SPLIT or prm->indicator > G5_INDICATOR_SIMPLE_GRADIENT)
     return fail(ctx->err, G5_ERR_INVALID, "unknown refine indicator");
-  if (ctx->cfg.dim == 3 and !p->solves[ctx->id].two_level.empty())
-    return fail(ctx->err, G5_ERR_UNSUPPORTED, "refine flags of a 3-D two-level mesh are not built");
   int rc = rebuild_list(p);
   if (rc != G5_OK)
     return fail(ctx->err, rc, p->err);

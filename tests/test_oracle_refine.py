@@ -15,7 +15,8 @@ definitions fix by hand:
     field giving the transposed flags;
   - on a two-level mesh: a coarse cell next to a refined block reads the average of the four fine
     cells (0 second difference on a linear field) and a fine cell next to a coarse leaf reads its
-    coarse parent (a known non-zero value on a linear field).
+    coarse parent (a known non-zero value on a linear field); in 3-D the same with the eight fine
+    cells and a field along each of the three axes in turn.
 The flag values and their order are the ones the block maximum needs (the absent core defines
 AMRContextBase's constants); they are this library's output values.
 """
@@ -170,3 +171,44 @@ def test_two_level_neighbours(g5, orc):
     # borders along the field's direction: block (0, 1) has the x border, block (2, 1) does not
     k21 = [k for k in range(o.nowned) if lv[k] == 0 and tuple(bc[k, :2]) == (2, 1)][0]
     assert imax[k21] == 0.0
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_two_level_neighbours_3d(g5, orc, axis):
+    # 4x4x4 brick, block (1, 1, 1) refined; IAD0 linear along one axis at the cell centres, in
+    # units of the coarse cell (coarse cell g: g + 0.5, fine cell f: f / 2 + 0.25); IAD1 = 1
+    cfg = g5.make_config(3, (4, 4, 4), 1.0 / 64)
+    o = orc.Oracle(cfg, refined=np.array([[1, 1, 1]], dtype=np.int32))
+    lv, bc = o.block_levels(), o.block_coords()[: o.nowned]
+    U = np.ones((o.nowned, 9, 16, 16, 16))
+    i = np.arange(16)
+    shape = [1, 1, 1]
+    shape[2 - axis] = 16  # cell arrays are [z][y][x]
+    for k in range(o.nowned):
+        g = (bc[k, axis] * 16 + i).reshape(shape)
+        U[k, 6] = np.broadcast_to(g + 0.5 if lv[k] == 0 else g / 2.0 + 0.25, (16, 16, 16))
+    o.set_state(U.ravel())
+    f, imax = o.refine_flags(LOHNER_SPLIT, 0, 5, 2, 0.1, 0.0125, EPS)
+
+    def lohner(L, C, R):
+        return abs(L - 2 * C + R) / (abs(R - C) + abs(C - L) + EPS * (abs(L) + 2 * abs(C) + abs(R)))
+    # coarse leaves: every second difference is 0 -- inside a block, across blocks, and next to the
+    # refined block, whose eight fine cells average to the coarse centre value -- except in the
+    # brick's first and last blocks along the field's axis, whose border cells read themselves
+    for k in range(o.nowned):
+        if lv[k] == 0:
+            assert (imax[k] == 0.0) == (bc[k, axis] in (1, 2)), (axis, bc[k])
+    # fine block (2, 2, 2): its low faces read coarse parents.  Along the axis, f = 32 reads the
+    # parent of f = 31 (coarse 15: 15.5), C = 16.25, R = 16.75.  Across it, at f_axis = 32, the
+    # parent of the cell one step down on another axis has 16.5 along the axis and C = R = 16.25;
+    # that value is the block's largest (larger f_axis have larger |C| at the same |L - C| = 0.25).
+    along, across = lohner(15.5, 16.25, 16.75), lohner(16.5, 16.25, 16.25)
+    assert across > along > 0.0
+    kf = [k for k in range(o.nowned) if lv[k] == 1 and tuple(bc[k]) == (2, 2, 2)][0]
+    assert imax[kf] == across
+    # fine blocks (3, 3, 3) touch coarse leaves only on their high faces: f = 63 along the axis
+    # reads coarse 32 (32.5) with C = 31.75 and L = 31.25; across it, f_axis = 48 reads a parent
+    # with 24.5 and C = R = 24.25 (the larger of the two)
+    hi_along, hi_across = lohner(31.25, 31.75, 32.5), lohner(24.25, 24.25, 24.5)
+    kf3 = [k for k in range(o.nowned) if lv[k] == 1 and tuple(bc[k]) == (3, 3, 3)][0]
+    assert imax[kf3] == max(hi_along, hi_across)

@@ -6,9 +6,9 @@ Bar: integer flags equal, and the per-block largest indicator value bit-identica
 evaluates the reference's expressions in the same order, FP64 with IEEE division and square root,
 no contraction).  Cases: the 2-D air-helium shock-bubble at 512x128 after 50 steps (walls in y,
 zero-gradient x: the outside-is-self rule at both kinds of border), the 3-D air-water THINC
-shock-bubble at 64^3, a periodic 2-D box (wrap), the two-level shock-bubble after the shock has
-entered the fine blocks (restriction and coarse-parent neighbours), and max-accumulation into
-flags the caller set before.
+shock-bubble at 64^3, a periodic 2-D box (wrap), the 2-D and 3-D two-level shock-bubbles after the
+shock has entered the fine blocks (restriction and coarse-parent neighbours), and max-accumulation
+into flags the caller set before.
 """
 import sys
 
@@ -96,3 +96,23 @@ def test_refine_flags_two_level(g5, orc):
         o = orc.Oracle(cfg, refined=np.array([[x, y, 0] for x, y in refined], dtype=np.int32), prolongation=1)
         o.set_state(U)
         compare(ctx, o, "two-level shock-bubble +150 steps", base_level=2, level_min=2, level_max=3)
+
+
+def test_refine_flags_two_level_3d(g5, orc):
+    # the 3-D air-helium shock-bubble on a 4^3 brick with a corner of four refined blocks, after
+    # the shock has entered them: coarse cells next to fine blocks read averages of eight fine
+    # cells, fine cells next to coarse leaves read their parents on faces of all three orientations
+    refined = [[1, 1, 1], [2, 1, 1], [1, 2, 1], [1, 1, 2]]
+    cfg, Uc = cases.air_helium_3d(nb=4)
+    _, Uf = cases.air_helium_3d(nb=8)
+    with g5.Context(cfg, refined=refined, prolongation=1) as ctx:
+        lay = ctx.block_layout()
+        Uc3, Uf3 = Uc.reshape(-1, 9, 4096), Uf.reshape(-1, 9, 4096)
+        U0 = np.stack([Uc3[x + 4 * (y + 4 * z)] if lv == 0 else Uf3[x + 8 * (y + 8 * z)]
+                       for x, y, z, lv in lay]).ravel()
+        ctx.upload(U0)
+        ctx.run(45)
+        U = ctx.download()
+        o = orc.Oracle(cfg, refined=np.array(refined, dtype=np.int32), prolongation=1)
+        o.set_state(U)
+        compare(ctx, o, "3-D two-level air-helium +45 steps", base_level=2, level_min=2, level_max=3)
