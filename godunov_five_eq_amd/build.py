@@ -45,9 +45,13 @@ FILE_FLAGS = {"g5_step3d.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
               "g5_step3d_amr_fast.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=fast"],
               "g5_step2d_fast.hip": ["-ffp-contract=fast"],
               # the mixed-precision instances may contract too (held to their accuracy bar against the
-              # FP64 oracle); the 3-D ones keep the 3-D file's scheduler
-              "g5_step2d_mixed.hip": ["-ffp-contract=fast"],
-              "g5_step3d_mixed.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=fast"]}
+              # FP64 oracle); the 3-D ones keep the 3-D file's scheduler.  Their FP32 arithmetic flushes
+              # denormals (FP64 keeps IEEE denormals): an FP32 quotient is then the hardware reciprocal
+              # and a product, and a square root the hardware one, with no range-scaling steps
+              # (6-7% faster steps, measured on the 2-D batch and the 3-D mixed case)
+              "g5_step2d_mixed.hip": ["-ffp-contract=fast", "-fgpu-flush-denormals-to-zero"],
+              "g5_step3d_mixed.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=fast",
+                                      "-fgpu-flush-denormals-to-zero"]}
 
 
 def _run(cmd, verbose):

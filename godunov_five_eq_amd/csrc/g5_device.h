@@ -225,12 +225,13 @@ cons_to_prim(SolveDesc const & S, Cons const & U, Prim & q)
 // Mixed-precision conversion: the pressure is recovered in FP64 from the total energy
 // (p xi = rho e_int - Pi cancels Pi ~ 1e9 against p xi ~ 1e4 when pinf ~ 6e8), then every
 // primitive is rounded to FP32.  Same sanitising as cons_to_prim; returns the FP32 sound speed.
+// The FP64 reciprocal of the density is the refined hardware one (within about an ulp).
 template <int DIM>
 G5D float
 cons_to_prim_mixed(SolveDesc const & S, Cons const & U, PrimT<float> & q)
 {
   const double d = fmax(U.d, S.smallr);
-  const double inv_d = 1.0 / d;
+  const double inv_d = rcp64(d);
   const double ux = U.mu * inv_d;
   const double uy = U.mv * inv_d;
   double       m2 = U.mu * ux + U.mv * uy;
@@ -1289,7 +1290,8 @@ source_factors(SolveDesc const & S, Cons const & U, double & st0, double & st1)
   st1 = 1.0 - kbar / k1;
 }
 
-// scheme/FixVolumeFractionsNormalization.cpp:58-70 (FAST: the reciprocal by rcp64)
+// scheme/FixVolumeFractionsNormalization.cpp:58-70 (FAST -- the reassociated FP64 and the mixed
+// paths: the reciprocal by rcp64)
 template <bool FAST = false>
 G5D void
 fix_volume_fractions(Cons & u)
