@@ -61,12 +61,6 @@ def traffic_from_profile(name):
     return None
 
 
-def rel_l1(a, b, nvar, ncell):
-    A = a.reshape(-1, nvar, ncell)
-    B = b.reshape(-1, nvar, ncell)
-    return [float(np.abs(A[:, v] - B[:, v]).sum() / max(np.abs(B[:, v]).sum(), 1e-300)) for v in range(nvar)]
-
-
 def cpu_rate(cfg, U0, nsteps):
     """Cell-updates/s of the CPU FP64 oracle on one configuration (bounded sample)."""
     import oracle
@@ -213,13 +207,14 @@ def bench_mixed3d(g5, args):
     wall, ev, Um, n = run_context(g5, cfgm, U0, args.steps, args.warmup)
     assert n == n64 == args.steps and np.all(np.isfinite(Um)) and np.all(np.isfinite(U64))
     # both runs include the warm-up steps: the states compared are after warmup + steps steps
-    l1 = rel_l1(Um, U64, 9, 4096)
+    l1 = l1_vs(Um, U64, 9, 4096, 3)
     names = ["rho", "E", "rho_u", "rho_v", "rho_w", "alpha0", "alpha_rho0", "alpha1", "alpha_rho1"]
     out = base_line(args, cells * n / wall, wall * 1e3 / n, n, "mixed-f32-f64")
     out["config"] = {"workload": "shock_bubble_3d_air_water_thinc_mixed", "cells": cells,
                      "grid": [16 * nbx, 16 * args.nb, 16 * args.nb], "blocks": nbx * args.nb ** 2,
                      "precision": "mixed", "l1_vs_fp64_after_steps": args.steps + args.warmup,
                      "rel_l1_vs_fp64": dict(zip(names, l1)),
+                     "momentum_norm": "momentum components against the momentum vector's L1 norm",
                      "fp64_value": cells * n64 / wall64, "speedup_vs_fp64": wall64 / wall}
     out["roofline"] = roofline(cells, 9, ev, traffic_from_profile(f"step3d_mixed_air_water_thinc_{16 * nbx}"))
     if not args.no_cpu:

@@ -6,6 +6,10 @@
     python -m godunov_five_eq_amd.rocprof_summary traffic KERNEL CELLS NVAR FETCH WRITE CAL_FETCH \
         CAL_WRITE [REQ CAL_REQ]                                        # json: HBM bytes per launch
 
+--last N (stats, pmc, traffic): the last N dispatches of each kernel only -- the steps that run
+after the launch-geometry autotuner's trial launches of the same kernel, which use other
+geometries (stats then reads the kernel trace of the same --kernel-trace run).
+
 traffic: FETCH_SIZE and WRITE_SIZE (KiB per dispatch, one --pmc run each) of the kernel whose name
 contains KERNEL, each scaled by what the same counter reads for a plain device copy of known size
 (godunov_five_eq_amd.calibrate: 1 GiB read and written per launch), against the algorithmic
@@ -29,24 +33,40 @@ def _rows(directory, suffix):
 
 
 def kernel_stats(directory):
-    """[(name, calls, total_ns, avg_ns, percent)] sorted by total time."""
+    """[(name, calls, total_ns, avg_ns, percent)] sorted by total time; with --last N, from the
+    kernel trace: the last N dispatches of each kernel only"""
     out = []
+    if LAST:
+        acc = {}
+        for r in sorted(_rows(directory, "kernel_trace.csv"), key=lambda r: int(r["Dispatch_Id"])):
+            acc.setdefault(r["Kernel_Name"], []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        tot = {k: sum(v[-LAST:]) for k, v in acc.items()}
+        whole = sum(tot.values())
+        for k, v in acc.items():
+            v = v[-LAST:]
+            out.append((k, len(v), float(tot[k]), tot[k] / len(v), 100.0 * tot[k] / whole))
+        return sorted(out, key=lambda x: -x[2])
     for r in _rows(directory, "kernel_stats.csv"):
         out.append((r["Name"], int(r["Calls"]), float(r["TotalDurationNs"]), float(r["AverageNs"]),
                     float(r["Percentage"])))
     return sorted(out, key=lambda x: -x[2])
 
 
+LAST = None  # --last N: only the last N dispatches of each kernel
+
+
 def counter_means(directory, counter):
-    """{kernel name: (dispatches, mean counter value per dispatch)}"""
+    """{kernel name: (dispatches, mean counter value per dispatch)}, in dispatch order per kernel"""
     acc = {}
-    for r in _rows(directory, "counter_collection.csv"):
+    for r in sorted(_rows(directory, "counter_collection.csv"), key=lambda r: int(r.get("Dispatch_Id") or 0)):
         if r.get("Counter_Name") != counter:
             continue
-        name = r["Kernel_Name"]
-        n, s = acc.get(name, (0, 0.0))
-        acc[name] = (n + 1, s + float(r["Counter_Value"]))
-    return {k: (n, s / n) for k, (n, s) in acc.items()}
+        acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    out = {}
+    for k, v in acc.items():
+        v = v[-LAST:] if LAST else v
+        out[k] = (len(v), sum(v) / len(v))
+    return out
 
 
 def _one(directory, counter, kernel):
@@ -92,6 +112,11 @@ def traffic(kernel, cells, nvar, fetch_dir, write_dir, cal_fetch_dir, cal_write_
 
 
 def main(argv):
+    global LAST
+    if "--last" in argv:
+        k = argv.index("--last")
+        LAST = int(argv[k + 1])
+        argv = argv[:k] + argv[k + 2:]
     if argv[0] == "traffic":
         kernel, cells, nvar = argv[1], int(argv[2]), int(argv[3])
         print(json.dumps(traffic(kernel, cells, nvar, *argv[4:]), indent=1))
