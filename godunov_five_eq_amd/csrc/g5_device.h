@@ -146,6 +146,30 @@ rsqrt64(double x)
   return 2.0 * fma(h, r, h);
 }
 
+// the reciprocal and reciprocal square root of the reassociated forms in the arithmetic of the
+// path: FP64 refined as above; FP32 (mixed path) the hardware estimates themselves (about 1 ulp;
+// the mixed translation units flush FP32 denormals, so 1/x is the bare v_rcp_f32)
+G5D double
+rcp_of(double x)
+{
+  return rcp64(x);
+}
+G5D float
+rcp_of(float x)
+{
+  return 1.0f / x;
+}
+G5D double
+rsqrt_of(double x)
+{
+  return rsqrt64(x);
+}
+G5D float
+rsqrt_of(float x)
+{
+  return __builtin_amdgcn_rsqf(x);
+}
+
 template <typename T>
 G5D T
 eos_xi(SolveDesc const & S, T a0, T a1)
@@ -343,16 +367,17 @@ slope1(T q, T qp, T qm, T slope_type)
   return dq;
 }
 
-// Reassociated FP64 path: minmod (slope_type 1) as sign(dc) min(|dl|, |dr|) when dl dr > 0, without
-// the bound by |dc| = |dl + dr|/2 >= min(|dl|, |dr|) that holds in exact arithmetic
-G5D double
-slope1_fast(double q, double qp, double qm, double slope_type)
+// Reassociated forms (FP64 and mixed): minmod (slope_type 1) as sign(dc) min(|dl|, |dr|) when
+// dl dr > 0, without the bound by |dc| = |dl + dr|/2 >= min(|dl|, |dr|) that holds in exact arithmetic
+template <typename T>
+G5D T
+slope1_fast(T q, T qp, T qm, T slope_type)
 {
-  if (slope_type == 1.0)
+  if (slope_type == T(1.0))
   {
-    const double dl = q - qm, dr = qp - q;
-    const double m = fmin(fabs(dl), fabs(dr));
-    return dl * dr <= 0.0 ? 0.0 : ((qp - qm) >= 0.0 ? m : -m);
+    const T dl = q - qm, dr = qp - q;
+    const T m = fmin(fabs(dl), fabs(dr));
+    return dl * dr <= T(0.0) ? T(0.0) : ((qp - qm) >= T(0.0) ? m : -m);
   }
   return slope1(q, qp, qm, slope_type);
 }
@@ -394,8 +419,8 @@ struct HancockT // 0.5 * (sr0_0, sr0_1, sp0, su0, sv0, sw0)
 };
 using Hancock = HancockT<double>;
 
-// FAST (reassociated FP64 path): the pressure-gradient terms multiply by one reciprocal of r, and
-// r c^2 is the mixture's rho c^2 = ((xi+1) p + Pi)/xi itself (no square root).
+// FAST (the reassociated forms, FP64 and mixed): the pressure-gradient terms multiply by one
+// reciprocal of r, and r c^2 is the mixture's rho c^2 = ((xi+1) p + Pi)/xi itself (no square root).
 template <int DIM, typename T, bool FAST = false>
 G5D HancockT<T>
 hancock_terms(SolveDesc const & S, PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy,
@@ -407,14 +432,14 @@ hancock_terms(SolveDesc const & S, PrimT<T> const & q, SlopeT<T> const & sx, Slo
   const T     uu = q.u, vv = q.v, ww = q.w;
   if constexpr (FAST)
   {
-    const double x = eos_xi(S, q.a, 1.0 - q.a);
-    const double rc2 = ((x + 1.0) * q.p + eos_pi(S, q.a, 1.0 - q.a)) * rcp64(x);
-    const double ri = rcp64(r);
-    double       sr0 = (-uu * (sx.r0 + sx.r1) - sx.u * r) * dtdx + (-vv * (sy.r0 + sy.r1) - sy.v * r) * dtdy;
-    double       su0 = (-uu * sx.u - sx.p * ri) * dtdx + (-vv * sy.u) * dtdy;
-    double       sv0 = (-uu * sx.v) * dtdx + (-vv * sy.v - sy.p * ri) * dtdy;
-    double       sp0 = (-uu * sx.p - sx.u * rc2) * dtdx + (-vv * sy.p - sy.v * rc2) * dtdy;
-    double       sw0 = 0.0;
+    const T x = eos_xi(S, q.a, T(1.0) - q.a);
+    const T rc2 = ((x + T(1.0)) * q.p + eos_pi(S, q.a, T(1.0) - q.a)) * rcp_of(x);
+    const T ri = rcp_of(r);
+    T       sr0 = (-uu * (sx.r0 + sx.r1) - sx.u * r) * dtdx + (-vv * (sy.r0 + sy.r1) - sy.v * r) * dtdy;
+    T       su0 = (-uu * sx.u - sx.p * ri) * dtdx + (-vv * sy.u) * dtdy;
+    T       sv0 = (-uu * sx.v) * dtdx + (-vv * sy.v - sy.p * ri) * dtdy;
+    T       sp0 = (-uu * sx.p - sx.u * rc2) * dtdx + (-vv * sy.p - sy.v * rc2) * dtdy;
+    T       sw0 = T(0.0);
     if (DIM == 3)
     {
       sr0 += (-ww * (sz.r0 + sz.r1) - sz.w * r) * dtdz;
@@ -423,12 +448,12 @@ hancock_terms(SolveDesc const & S, PrimT<T> const & q, SlopeT<T> const & sx, Slo
       sw0 = (-uu * sx.w) * dtdx + (-vv * sy.w) * dtdy + (-ww * sz.w - sz.p * ri) * dtdz;
       sp0 += (-ww * sz.p - sz.w * rc2) * dtdz;
     }
-    h.r0 = 0.5 * (sr0 * q.a);
-    h.r1 = 0.5 * (sr0 * (1.0 - q.a));
-    h.p = 0.5 * sp0;
-    h.u = 0.5 * su0;
-    h.v = 0.5 * sv0;
-    h.w = 0.5 * sw0;
+    h.r0 = T(0.5) * (sr0 * q.a);
+    h.r1 = T(0.5) * (sr0 * (T(1.0) - q.a));
+    h.p = T(0.5) * sp0;
+    h.u = T(0.5) * su0;
+    h.v = T(0.5) * sv0;
+    h.w = T(0.5) * sw0;
     return h;
   }
   const T c = eos_sound_speed(S, r, q.p, q.a, T(1.0) - q.a);
@@ -471,7 +496,7 @@ hancock_terms(SolveDesc const & S, PrimT<T> const & q, SlopeT<T> const & sx, Slo
 // q + off_x*sx + off_y*sy (+ off_z*sz), one offset +-0.5 on axis AX, the others 0.0 (kept so that
 // the operation sequence is the reference's).  AX = -1: the axis is the run-time value ax (one
 // code path for lanes that reconstruct faces on different axes; the same operations per lane).
-// FAST (reassociated FP64 path): q + o * s of the face axis alone.
+// FAST (the reassociated forms, FP64 and mixed): q + o * s of the face axis alone.
 template <int DIM, int AX, typename T, bool FAST = false>
 G5D T
 muscl_extrap(T qv, T sxv, T syv, T szv, T o, int ax = AX)
@@ -599,7 +624,8 @@ tanh_of(float x)
 // ---------------------------------------------------------------------------------------------
 // Youngs (1982) normal from the 3^d alpha stencil (same weights and order as the host oracle).
 // ---------------------------------------------------------------------------------------------
-// FAST (reassociated FP64 path): the components multiply by one reciprocal square root
+// FAST (the reassociated forms, FP64 and mixed): the components multiply by one reciprocal square
+// root.
 template <typename T, bool FAST = false>
 G5D void
 youngs2(T const (&a)[9], T & nx, T & ny)
@@ -609,8 +635,8 @@ youngs2(T const (&a)[9], T & nx, T & ny)
   const T gy = ((a[6] + T(2.0) * a[7]) + a[8]) - ((a[0] + T(2.0) * a[1]) + a[2]);
   if constexpr (FAST)
   {
-    const double n2 = gx * gx + gy * gy;
-    const double rn = n2 > 0.0 ? rsqrt64(n2) : 0.0;
+    const T n2 = gx * gx + gy * gy;
+    const T rn = n2 > T(0.0) ? rsqrt_of(n2) : T(0.0);
     nx = gx * rn;
     ny = gy * rn;
     return;
@@ -659,8 +685,8 @@ youngs3(T const (&a)[27], T & nx, T & ny, T & nz)
   const T gz = youngs_plane<2>(a, 1) - youngs_plane<2>(a, -1);
   if constexpr (FAST)
   {
-    const double n2 = gx * gx + gy * gy + gz * gz;
-    const double rn = n2 > 0.0 ? rsqrt64(n2) : 0.0;
+    const T n2 = gx * gx + gy * gy + gz * gz;
+    const T rn = n2 > T(0.0) ? rsqrt_of(n2) : T(0.0);
     nx = gx * rn;
     ny = gy * rn;
     nz = gz * rn;
@@ -691,7 +717,8 @@ struct ThincAxisT // per-cell, per-axis THINC quantities shared by the two faces
 };
 using ThincAxis = ThincAxisT<double>;
 
-// FAST (reassociated FP64 path): (phi - A)/B as a product by rcp64(B), the FAST tanh
+// FAST (the reassociated forms, FP64 and mixed): (phi - A)/B as a product by the reciprocal of B,
+// the FAST tanh
 template <typename T, bool FAST = false>
 G5D ThincAxisT<T>
 thinc_axis(SolveDesc const & S, T qa, T am, T ac, T ap, T nd)
@@ -708,7 +735,7 @@ thinc_axis(SolveDesc const & S, T qa, T am, T ac, T ap, T nd)
     t.C = tanh_of<FAST>(T(0.5) * beta);
     const T phi = ac;
     if constexpr (FAST)
-      t.D = tanh_of<FAST>(T(0.5) * beta * (phi - t.A) * rcp64(t.B));
+      t.D = tanh_of<FAST>(T(0.5) * beta * (phi - t.A) * rcp_of(t.B));
     else
       t.D = tanh_of(T(0.5) * beta * (phi - t.A) / t.B);
   }
@@ -716,8 +743,8 @@ thinc_axis(SolveDesc const & S, T qa, T am, T ac, T ap, T nd)
 }
 
 // reconstruct_state_{2d,3d} of the THINC functor: alpha-related part and MUSCL part for p, u, v, w.
-// FAST (reassociated FP64 path): the quotients by C, 1 + b D, alpha and 1 - alpha as products by
-// reciprocals (those of C, alpha and 1 - alpha are shared by the faces of the cell).
+// FAST (the reassociated forms, FP64 and mixed): the quotients by C, 1 + b D, alpha and 1 - alpha as
+// products by reciprocals (those of C, alpha and 1 - alpha are shared by the faces of the cell).
 template <int DIM, int AX, bool HANCOCK, typename T, bool FAST = false>
 G5D FaceT<T>
 face_thinc(PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy, SlopeT<T> const & sz,
@@ -730,9 +757,9 @@ face_thinc(PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy, Slope
     const T bb = static_cast<T>(ib);
     if constexpr (FAST)
     {
-      f.a = th.A + bb * th.B * ((th.C + bb * th.D * rcp64(th.C)) * rcp64(1.0 + bb * th.D));
-      f.r0 = q.r0 + q.r0 * rcp64(q.a) * (f.a - q.a);
-      f.r1 = q.r1 + q.r1 * rcp64(1.0 - q.a) * (q.a - f.a);
+      f.a = th.A + bb * th.B * ((th.C + bb * th.D * rcp_of(th.C)) * rcp_of(T(1.0) + bb * th.D));
+      f.r0 = q.r0 + q.r0 * rcp_of(q.a) * (f.a - q.a);
+      f.r1 = q.r1 + q.r1 * rcp_of(T(1.0) - q.a) * (q.a - f.a);
     }
     else
     {

@@ -739,7 +739,7 @@ step3d_body(StepArgs const & A, int item)
     if (P >= 1)
     {
       apply_face_pair<3>(u2prev, zFprev, zF, uaprev0, uaprev1, dtdx, 2, stprev0, stprev1);
-      fix_volume_fractions<FAST or sizeof(T) == sizeof(float)>(u2prev);
+      fix_volume_fractions<FAST>(u2prev);
       store_cons<3, B3>(U2, col_off((P - 1) >> 4) + i + 16 * j + 256 * ((P - 1) & 15), u2prev);
       double       pc;
       const double rate = cell_rate<3, T, FAST>(S, u2prev, pc);

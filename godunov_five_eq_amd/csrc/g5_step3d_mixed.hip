@@ -2,7 +2,9 @@
 // g5_step3d_body.h): FP32 primitives, face states and wave speeds, FP64 state, pressure recovery,
 // energy flux and dt.  Their own translation unit so that the compiler may contract a*b+c into FMA
 // here: the mixed path is held to its accuracy bar against the FP64 oracle, not to the reference's
-// operation order, which the bit-exact FP64 instances of g5_step3d.hip keep.
+// operation order, which the bit-exact FP64 instances of g5_step3d.hip keep.  They take the
+// reassociated forms of the device functions (FAST: products by reciprocals, the face axis's term
+// alone in the MUSCL extrapolation, minmod without the |dc| bound).
 #include "g5_step3d_body.h"
 
 namespace g5
@@ -13,7 +15,7 @@ template <bool THINC, bool HANCOCK, bool ZSPLIT, bool INLET, bool MP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_step3d_mixed(StepArgs A)
 {
-  step3d_blocks<THINC, HANCOCK, ZSPLIT, INLET, MP, float>(A);
+  step3d_blocks<THINC, HANCOCK, ZSPLIT, INLET, MP, float, true>(A);
 }
 
 template <bool ZSPLIT, bool INLET, bool MP>
