@@ -38,19 +38,24 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"
 # hides more latency (measured 2.6% faster on the 256^3 FP64 step, neutral on the mixed one; the
 # 2-D step, at several wavefronts per SIMD, is 10-21% slower with it, so it keeps the default)
 # The reassociated FP64 instances (G5_PREC_FP64_FAST) also let the compiler contract a*b+c into
-# FMA (held to the same relative L1 bar against the bit-exact path as the rest of that path).
+# FMA (held to the same relative L1 bar against the bit-exact path as the rest of that path), within
+# a source expression only (-ffp-contract=on): contraction across statements (=fast) depends on the
+# code around each inlined copy of a device function, so the z solve of the split and the in-place
+# z-solve kernels could round differently and a request's bits would depend on the launch geometry
+# (seen with the mixed kernels); per expression, every copy rounds alike (and the reassociated 3-D
+# step was 1.4% faster this way, the mixed one 3.7% slower than with =fast).
 FILE_FLAGS = {"g5_step3d.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
-              "g5_step3d_fast.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=fast"],
+              "g5_step3d_fast.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=on"],
               "g5_step3d_amr.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
-              "g5_step3d_amr_fast.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=fast"],
-              "g5_step2d_fast.hip": ["-ffp-contract=fast"],
-              # the mixed-precision instances may contract too (held to their accuracy bar against the
-              # FP64 oracle); the 3-D ones keep the 3-D file's scheduler.  Their FP32 arithmetic flushes
+              "g5_step3d_amr_fast.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=on"],
+              "g5_step2d_fast.hip": ["-ffp-contract=on"],
+              # the mixed-precision instances may contract too, per expression (held to their accuracy
+              # bar against the FP64 oracle); the 3-D ones keep the 3-D file's scheduler.  Their FP32 arithmetic flushes
               # denormals (FP64 keeps IEEE denormals): an FP32 quotient is then the hardware reciprocal
               # and a product, and a square root the hardware one, with no range-scaling steps
               # (6-7% faster steps, measured on the 2-D batch and the 3-D mixed case)
-              "g5_step2d_mixed.hip": ["-ffp-contract=fast", "-fgpu-flush-denormals-to-zero"],
-              "g5_step3d_mixed.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=fast",
+              "g5_step2d_mixed.hip": ["-ffp-contract=on", "-fgpu-flush-denormals-to-zero"],
+              "g5_step3d_mixed.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp", "-ffp-contract=on",
                                       "-fgpu-flush-denormals-to-zero"]}
 
 

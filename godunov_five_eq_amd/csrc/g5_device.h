@@ -367,19 +367,34 @@ slope1(T q, T qp, T qm, T slope_type)
   return dq;
 }
 
-// Reassociated forms (FP64 and mixed): minmod (slope_type 1) as sign(dc) min(|dl|, |dr|) when
-// dl dr > 0, without the bound by |dc| = |dl + dr|/2 >= min(|dl|, |dr|) that holds in exact arithmetic
+// the median of a, b and 0: minmod(a, b) -- 0 when the signs differ, else the one nearer 0
+G5D double
+med0(double a, double b)
+{
+  return fmax(fmin(a, b), fmin(fmax(a, b), 0.0));
+}
+G5D float
+med0(float a, float b)
+{
+  return __builtin_amdgcn_fmed3f(a, b, 0.0f);
+}
+
+// Reassociated forms (FP64 and mixed): the limited slope as medians with 0.  minmod (slope_type 1):
+// the median of dl, dr and 0, which is sign(dc) min(|dl|, |dr|) when dl dr > 0 and 0 otherwise,
+// without the bound by |dc| = |dl + dr|/2 >= min(|dl|, |dr|) that holds in exact arithmetic (and
+// without the product dl dr, so a product that underflows no longer zeroes the slope).  MC
+// (slope_type 2): the median of that minmod of 2 dl and 2 dr, dc and 0 -- the reference's
+// sign(dc) min(dlim, |dc|).
 template <typename T>
 G5D T
 slope1_fast(T q, T qp, T qm, T slope_type)
 {
+  const T dl = q - qm, dr = qp - q;
   if (slope_type == T(1.0))
-  {
-    const T dl = q - qm, dr = qp - q;
-    const T m = fmin(fabs(dl), fabs(dr));
-    return dl * dr <= T(0.0) ? T(0.0) : ((qp - qm) >= T(0.0) ? m : -m);
-  }
-  return slope1(q, qp, qm, slope_type);
+    return med0(dl, dr);
+  if (slope_type == T(2.0))
+    return med0(med0(T(2.0) * dl, T(2.0) * dr), T(0.5) * (qp - qm));
+  return T(0.0);
 }
 
 template <typename T, bool FAST = false>

@@ -19,7 +19,7 @@ void launch_step3d(StepArgs const & A, bool thinc, bool hancock, int precision, 
 void launch_step2d_fast(StepArgs const & A, bool thinc, bool hancock, bool ring_wave, bool mp, hipStream_t stream);
 void launch_step3d_fast(StepArgs const & A, bool thinc, bool hancock, bool zsplit, bool inlet, bool mp,
                         hipStream_t stream);
-// the mixed-precision instances (separate translation units, FMA contraction allowed)
+// the mixed-precision instances (separate translation units, FMA contraction within expressions)
 void launch_step2d_mixed(StepArgs const & A, bool thinc, bool hancock, bool ring_wave, bool mp, hipStream_t stream);
 void launch_step3d_mixed(StepArgs const & A, bool thinc, bool hancock, bool zsplit, bool inlet, bool mp,
                          hipStream_t stream);

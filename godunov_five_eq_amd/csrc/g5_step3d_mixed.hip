@@ -1,10 +1,10 @@
 // g5_step3d_mixed.hip -- the mixed-precision instances of the fused 3-D step (body in
 // g5_step3d_body.h): FP32 primitives, face states and wave speeds, FP64 state, pressure recovery,
 // energy flux and dt.  Their own translation unit so that the compiler may contract a*b+c into FMA
-// here: the mixed path is held to its accuracy bar against the FP64 oracle, not to the reference's
-// operation order, which the bit-exact FP64 instances of g5_step3d.hip keep.  They take the
-// reassociated forms of the device functions (FAST: products by reciprocals, the face axis's term
-// alone in the MUSCL extrapolation, minmod without the |dc| bound).
+// (within an expression) here: the mixed path is held to its accuracy bar against the FP64
+// oracle, not to the reference's operation order, which the bit-exact FP64 instances of
+// g5_step3d.hip keep.  They take the reassociated forms of the device functions (FAST: products by
+// reciprocals, the face axis's term alone in the MUSCL extrapolation, minmod as a median).
 #include "g5_step3d_body.h"
 
 namespace g5

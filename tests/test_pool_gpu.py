@@ -298,6 +298,36 @@ def test_launch_geometries_bit_identical(g5):
             say(f"dim={dim} prec={prec} autotuned launch {pool.launch()}")
 
 
+def test_served_3d_request_bits_do_not_depend_on_the_launch(g5):
+    # a 3-D request of the seeded serving set (MUSCL, downsized to 4^3 blocks) in mixed precision
+    # and in the reassociated FP64 form: the split and the in-place z solve and every z-column
+    # length give the same bits.  The mixed kernels once rounded the two z-solve placements
+    # differently for this request, when the compiler contracted a*b+c across statements (each
+    # inlined copy of the z solve then fused according to the code around it); contraction within
+    # an expression rounds every copy alike.
+    spec = g5.serve_specs(20251203, 32, 3000.0, 0.25)[11]
+    assert spec.dim == 3
+    spec.nb[0] = spec.nb[1] = spec.nb[2] = 4
+    spec.steps = 9
+    for prec in (g5.PREC_MIXED, g5.PREC_FP64_FAST):
+        ref = None
+        for zs in (0, 1):
+            for cl in (1, 2, 4, 8):
+                say(f"served 3-D request, prec={prec} zsplit={zs} col_len={cl}")
+                with g5.Pool(3, precision=prec, budget_bytes=1 << 28) as pool:
+                    pool.set_launch(0, 1, zs, 0, cl)
+                    sid = pool.admit_spec(spec)
+                    pool.start([sid])
+                    pool.step(spec.steps + 3)
+                    t, n, _ = pool.status(sid)
+                    U = pool.download(sid)
+                assert n == spec.steps and np.all(np.isfinite(U))
+                if ref is None:
+                    ref = U
+                else:
+                    assert np.array_equal(U, ref), (prec, zs, cl)
+
+
 def test_z_columns_in_a_pool_of_uneven_solves(g5):
     # three 3-D solves of 3, 5 and 2 blocks in z in one pool, marched in z-columns of 2, 4, 8 blocks
     # (padded columns, columns of different solves side by side in the list): each ends with the
