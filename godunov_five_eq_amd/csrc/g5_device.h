@@ -527,6 +527,16 @@ muscl_extrap(T qv, T sxv, T syv, T szv, T o, int ax = AX)
   return qv + ox * sxv + oy * syv + oz * szv;
 }
 
+// Reassociated forms: (q + h) + o s of the face axis -- the Hancock-advanced cell value q + h is
+// the same expression for every face of the cell, so the compiler computes it once per cell
+template <bool HANCOCK, int AX, typename T>
+G5D T
+hancock_extrap(T qv, T hv, T sxv, T syv, T szv, T o, int ax)
+{
+  const int a = AX >= 0 ? AX : ax;
+  return (HANCOCK ? qv + hv : qv) + o * (a == 0 ? sxv : (a == 1 ? syv : szv));
+}
+
 // reconstruct_state_{2d,3d} face part (scheme/ComputeFluxesAndStoreFunctor.cpp:164-211)
 template <int DIM, int AX, bool HANCOCK, typename T, bool FAST = false>
 G5D FaceT<T>
@@ -534,6 +544,19 @@ face_muscl(SolveDesc const & S, PrimT<T> const & q, SlopeT<T> const & sx, SlopeT
            SlopeT<T> const & sz, HancockT<T> const & h, T o, int ax = AX)
 {
   FaceT<T> f;
+  if constexpr (FAST)
+  {
+    f.r0 = hancock_extrap<HANCOCK, AX>(q.r0, h.r0, sx.r0, sy.r0, sz.r0, o, ax);
+    f.r1 = hancock_extrap<HANCOCK, AX>(q.r1, h.r1, sx.r1, sy.r1, sz.r1, o, ax);
+    f.p = hancock_extrap<HANCOCK, AX>(q.p, h.p, sx.p, sy.p, sz.p, o, ax);
+    f.u = hancock_extrap<HANCOCK, AX>(q.u, h.u, sx.u, sy.u, sz.u, o, ax);
+    f.v = hancock_extrap<HANCOCK, AX>(q.v, h.v, sx.v, sy.v, sz.v, o, ax);
+    f.w = DIM == 3 ? hancock_extrap<HANCOCK, AX>(q.w, h.w, sx.w, sy.w, sz.w, o, ax) : T(0.0);
+    f.r0 = fmax(Par<T>::smallr(S), f.r0);
+    f.r1 = fmax(Par<T>::smallr(S), f.r1);
+    f.a = q.a;
+    return f;
+  }
   f.r0 = muscl_extrap<DIM, AX, T, FAST>(q.r0, sx.r0, sy.r0, sz.r0, o, ax);
   f.r1 = muscl_extrap<DIM, AX, T, FAST>(q.r1, sx.r1, sy.r1, sz.r1, o, ax);
   f.p = muscl_extrap<DIM, AX, T, FAST>(q.p, sx.p, sy.p, sz.p, o, ax);
@@ -767,21 +790,34 @@ face_thinc(PrimT<T> const & q, SlopeT<T> const & sx, SlopeT<T> const & sy, Slope
 {
   FaceT<T> f;
   const T  o = T(0.5) * static_cast<T>(ib); // HALF_F * offsets[AX]
-  if (th.req)
+  if constexpr (FAST)
   {
-    const T bb = static_cast<T>(ib);
-    if constexpr (FAST)
+    // (q + h) shared by the faces of the cell, as in face_muscl
+    if (th.req)
     {
+      const T bb = static_cast<T>(ib);
       f.a = th.A + bb * th.B * ((th.C + bb * th.D * rcp_of(th.C)) * rcp_of(T(1.0) + bb * th.D));
-      f.r0 = q.r0 + q.r0 * rcp_of(q.a) * (f.a - q.a);
-      f.r1 = q.r1 + q.r1 * rcp_of(T(1.0) - q.a) * (q.a - f.a);
+      f.r0 = (HANCOCK ? q.r0 + h.r0 : q.r0) + q.r0 * rcp_of(q.a) * (f.a - q.a);
+      f.r1 = (HANCOCK ? q.r1 + h.r1 : q.r1) + q.r1 * rcp_of(T(1.0) - q.a) * (q.a - f.a);
     }
     else
     {
-      f.a = th.A + bb * th.B * (th.C + bb * th.D / th.C) / (T(1.0) + bb * th.D);
-      f.r0 = q.r0 + q.r0 / q.a * (f.a - q.a);
-      f.r1 = q.r1 + q.r1 / (T(1.0) - q.a) * (q.a - f.a);
+      f.a = q.a;
+      f.r0 = hancock_extrap<HANCOCK, AX>(q.r0, h.r0, sx.r0, sy.r0, sz.r0, o, ax);
+      f.r1 = hancock_extrap<HANCOCK, AX>(q.r1, h.r1, sx.r1, sy.r1, sz.r1, o, ax);
     }
+    f.p = hancock_extrap<HANCOCK, AX>(q.p, h.p, sx.p, sy.p, sz.p, o, ax);
+    f.u = hancock_extrap<HANCOCK, AX>(q.u, h.u, sx.u, sy.u, sz.u, o, ax);
+    f.v = hancock_extrap<HANCOCK, AX>(q.v, h.v, sx.v, sy.v, sz.v, o, ax);
+    f.w = DIM == 3 ? hancock_extrap<HANCOCK, AX>(q.w, h.w, sx.w, sy.w, sz.w, o, ax) : T(0.0);
+    return f;
+  }
+  if (th.req)
+  {
+    const T bb = static_cast<T>(ib);
+    f.a = th.A + bb * th.B * (th.C + bb * th.D / th.C) / (T(1.0) + bb * th.D);
+    f.r0 = q.r0 + q.r0 / q.a * (f.a - q.a);
+    f.r1 = q.r1 + q.r1 / (T(1.0) - q.a) * (q.a - f.a);
   }
   else
   {
@@ -956,7 +992,8 @@ hllc(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
 //   - no e(p) -> p(e) round trip on each side (the identity in exact arithmetic): rho e = p xi + Pi;
 //   - c^2 = ((xi+1) p + Pi)/(xi rho), one reciprocal, and one square root for max(cl, cr);
 //   - u* and p* multiplied by one reciprocal of rcl + rcr, and the star factor and star energy of
-//     the sampled side by one reciprocal of S - u*.
+//     the sampled side by one reciprocal of S - u*;
+//   - the kinetic energy as rho/2 |u|^2, and the fluxes times dt/dx through u* dt/dx.
 template <int DIM>
 G5D Flux
 hllc_fast(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
@@ -973,10 +1010,10 @@ hllc_fast(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int 
   const double pl = fmax(L.p, rl * smallp);
   const double ul = ulN;
   const double xl = eos_xi(S, L.a, 1.0 - L.a), pil = eos_pi(S, L.a, 1.0 - L.a);
-  double       ecinl = 0.5 * rl * ul * ul;
-  ecinl += 0.5 * rl * ulT1 * ulT1;
+  double       kl = ul * ul + ulT1 * ulT1; // |u|^2, one product by rl/2 for the kinetic energy
   if (DIM == 3)
-    ecinl += 0.5 * rl * ulT2 * ulT2;
+    kl += ulT2 * ulT2;
+  const double ecinl = 0.5 * rl * kl;
   const double ql = ((xl + 1.0) * pl + pil) * rcp64(xl * rl);
   const double etotl = (pl * xl + pil) + ecinl;
 
@@ -984,10 +1021,10 @@ hllc_fast(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int 
   const double pr = fmax(R.p, rr * smallp);
   const double ur = urN;
   const double xr = eos_xi(S, R.a, 1.0 - R.a), pir = eos_pi(S, R.a, 1.0 - R.a);
-  double       ecinr = 0.5 * rr * ur * ur;
-  ecinr += 0.5 * rr * urT1 * urT1;
+  double       kr = ur * ur + urT1 * urT1; // |u|^2, one product by rr/2 for the kinetic energy
   if (DIM == 3)
-    ecinr += 0.5 * rr * urT2 * urT2;
+    kr += urT2 * urT2;
+  const double ecinr = 0.5 * rr * kr;
   const double qr = ((xr + 1.0) * pr + pir) * rcp64(xr * rr);
   const double etotr = (pr * xr + pir) + ecinr;
 
@@ -1043,28 +1080,27 @@ hllc_fast(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int 
     etoto = etotr;
   }
 
+  // the fluxes times dt/dx, with u* dt/dx formed once
   const double fd = (ro_0 + ro_1) * uo;
-  const double fn = ro * uo * uo + ptoto;
-  const double fe = (etoto + ptoto) * uo;
   const bool   up = fd > 0.0;
-  const double ft1 = fd * (up ? ulT1 : urT1);
-  const double ft2 = DIM == 3 ? fd * (up ? ulT2 : urT2) : 0.0;
+  const double uod = uo * dtdx;
+  const double fdd = fd * dtdx;
+  const double fn = (ro * uo) * uod + ptoto * dtdx;
+  const double ft1 = fdd * (up ? ulT1 : urT1);
+  const double ft2 = DIM == 3 ? fdd * (up ? ulT2 : urT2) : 0.0;
   const double phi0 = up ? L.a : R.a;
   const double phi1 = up ? 1.0 - L.a : 1.0 - R.a;
 
   Flux F;
-  F.ar0 = (ro_0 * uo) * dtdx;
-  F.ar1 = (ro_1 * uo) * dtdx;
-  F.a0 = (uo * phi0) * dtdx;
-  F.a1 = (uo * phi1) * dtdx;
-  F.e = fe * dtdx;
-  const double fu = AX == 0 ? fn : (AX == 1 ? ft1 : ft2);
-  const double fv = AX == 1 ? fn : ft1;
-  const double fw = AX == 2 ? fn : ft2;
-  F.mu = fu * dtdx;
-  F.mv = fv * dtdx;
-  F.mw = DIM == 3 ? fw * dtdx : 0.0;
-  F.us = uo * dtdx;
+  F.ar0 = ro_0 * uod;
+  F.ar1 = ro_1 * uod;
+  F.a0 = phi0 * uod;
+  F.a1 = phi1 * uod;
+  F.e = (etoto + ptoto) * uod;
+  F.mu = AX == 0 ? fn : (AX == 1 ? ft1 : ft2);
+  F.mv = AX == 1 ? fn : ft1;
+  F.mw = DIM == 3 ? (AX == 2 ? fn : ft2) : 0.0;
+  F.us = uod;
   return F;
 }
 
