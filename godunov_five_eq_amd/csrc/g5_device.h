@@ -1238,13 +1238,41 @@ riemann(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, dou
 // (.h:319-393): accumulate left minus right flux with the Allaire-Kokh alpha source term (time-n
 // alpha of the cell), then add to the running U2 with the 1e-12 clamps.
 // ---------------------------------------------------------------------------------------------
-// st0, st1: the source term factors (1 for ALLAIRE_KOKH, source_factors for MILLER_PUCKETT)
-template <int DIM>
+// st0, st1: the source term factors (1 for ALLAIRE_KOKH, source_factors for MILLER_PUCKETT).
+// FAST (the reassociated forms): the differences FL - FR with the source term on the difference of
+// u* dt/dx, instead of the reference's accumulation from 0 face by face.
+template <int DIM, bool FAST = false>
 G5D void
 apply_flux_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double ua1, double st0 = 1.0,
                 double st1 = 1.0)
 {
   const double eps = 1e-12;
+  if constexpr (FAST)
+  {
+    const double dus = FL.us - FR.us;
+    u2.ar0 += FL.ar0 - FR.ar0;
+    if (u2.ar0 < eps)
+      u2.ar0 = 0.0;
+    u2.ar1 += FL.ar1 - FR.ar1;
+    if (u2.ar1 < eps)
+      u2.ar1 = 0.0;
+    u2.a0 += (FL.a0 - FR.a0) - st0 * ua0 * dus;
+    if (u2.a0 < eps)
+      u2.a0 = 0.0;
+    if (u2.a0 > 1.0 - eps)
+      u2.a0 = 1.0;
+    u2.a1 += (FL.a1 - FR.a1) - st1 * ua1 * dus;
+    if (u2.a1 < eps)
+      u2.a1 = 0.0;
+    if (u2.a1 > 1.0 - eps)
+      u2.a1 = 1.0;
+    u2.e += FL.e - FR.e;
+    u2.mu += FL.mu - FR.mu;
+    u2.mv += FL.mv - FR.mv;
+    if (DIM == 3)
+      u2.mw += FL.mw - FR.mw;
+    return;
+  }
   double       acc_r0 = 0.0 + FL.ar0;
   double       acc_r1 = 0.0 + FL.ar1;
   double       acc_a0 = 0.0 + FL.a0;
@@ -1290,49 +1318,47 @@ apply_flux_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double 
 }
 
 // Mixed path: the FP64 face flux (scaled by dt/dx, rotated back to x, y, z momentum) from the
-// FP32 star state of an AX face; the same operations as the flux part of hllc.
+// FP32 star state of an AX face; the operations of the flux part of hllc_fast.
 template <int DIM>
 G5D Flux
 assemble_flux(FluxT<float> const & s, double dtdx, int AX)
 {
   const double ro0 = s.ro0, ro1 = s.ro1, ro = s.ro, uo = s.uo, pt = s.pt;
-  const double fd = (ro0 + ro1) * uo;
-  const double fn = ro * uo * uo + pt;
-  const double fe = (s.etot + pt) * uo;
-  const double ft1 = fd * static_cast<double>(s.t1);
-  const double ft2 = DIM == 3 ? fd * static_cast<double>(s.t2) : 0.0;
+  const double uod = uo * dtdx; // the fluxes times dt/dx through u* dt/dx
+  const double fdd = (ro0 + ro1) * uod;
+  const double fn = (ro * uo) * uod + pt * dtdx;
+  const double ft1 = fdd * static_cast<double>(s.t1);
+  const double ft2 = DIM == 3 ? fdd * static_cast<double>(s.t2) : 0.0;
   const double phi0 = s.phi0;
   const double phi1 = 1.0 - phi0;
   Flux         F;
-  F.ar0 = (ro0 * uo) * dtdx;
-  F.ar1 = (ro1 * uo) * dtdx;
-  F.a0 = (uo * phi0) * dtdx;
-  F.a1 = (uo * phi1) * dtdx;
-  F.e = fe * dtdx;
-  const double fu = AX == 0 ? fn : (AX == 1 ? ft1 : ft2);
-  const double fv = AX == 1 ? fn : ft1;
-  const double fw = AX == 2 ? fn : ft2;
-  F.mu = fu * dtdx;
-  F.mv = fv * dtdx;
-  F.mw = DIM == 3 ? fw * dtdx : 0.0;
-  F.us = uo * dtdx;
+  F.ar0 = ro0 * uod;
+  F.ar1 = ro1 * uod;
+  F.a0 = phi0 * uod;
+  F.a1 = phi1 * uod;
+  F.e = (s.etot + pt) * uod;
+  F.mu = AX == 0 ? fn : (AX == 1 ? ft1 : ft2);
+  F.mv = AX == 1 ? fn : ft1;
+  F.mw = DIM == 3 ? (AX == 2 ? fn : ft2) : 0.0;
+  F.us = uod;
   return F;
 }
 
 // the update with the face records of either path (dtdx and AX are used by the mixed path)
-template <int DIM>
+template <int DIM, bool FAST = false>
 G5D void
 apply_face_pair(Cons & u2, Flux const & FL, Flux const & FR, double ua0, double ua1, double, int, double st0 = 1.0,
                 double st1 = 1.0)
 {
-  apply_flux_pair<DIM>(u2, FL, FR, ua0, ua1, st0, st1);
+  apply_flux_pair<DIM, FAST>(u2, FL, FR, ua0, ua1, st0, st1);
 }
-template <int DIM>
+template <int DIM, bool FAST = false>
 G5D void
 apply_face_pair(Cons & u2, FluxT<float> const & FL, FluxT<float> const & FR, double ua0, double ua1, double dtdx,
                 int AX, double st0 = 1.0, double st1 = 1.0)
 {
-  apply_flux_pair<DIM>(u2, assemble_flux<DIM>(FL, dtdx, AX), assemble_flux<DIM>(FR, dtdx, AX), ua0, ua1, st0, st1);
+  apply_flux_pair<DIM, FAST>(u2, assemble_flux<DIM>(FL, dtdx, AX), assemble_flux<DIM>(FR, dtdx, AX), ua0, ua1, st0,
+                             st1);
 }
 
 // scheme/ReadFluxesAndConservativeUpdateFunctor.h:412-443 get_source_term_factor for

@@ -685,8 +685,8 @@ step2d_block(StepArgs const & A, int blk)
     if constexpr (MP) // the list holds a MILLER_PUCKETT solve; this block's solve decides
       if (S.source_term == G5_SOURCE_MILLER_PUCKETT)
         source_factors<2>(S, uown, st0, st1);
-    apply_face_pair<2>(u2, Fx, fl_load2(FX, FXE, fx2(i + 1, j)), uown.a0, uown.a1, dtdx, 0, st0, st1);
-    apply_face_pair<2>(u2, Fy, fl_load2(FY, FYE, fy2(i, j + 1)), uown.a0, uown.a1, dtdx, 1, st0, st1);
+    apply_face_pair<2, FAST>(u2, Fx, fl_load2(FX, FXE, fx2(i + 1, j)), uown.a0, uown.a1, dtdx, 0, st0, st1);
+    apply_face_pair<2, FAST>(u2, Fy, fl_load2(FY, FYE, fy2(i, j + 1)), uown.a0, uown.a1, dtdx, 1, st0, st1);
     fix_volume_fractions<FAST>(u2);
     store_cons<2, B2>(U2, own, u2);
     double pc;

@@ -738,7 +738,7 @@ step3d_body(StepArgs const & A, int item)
   auto           z_complete = [&](int P) {
     if (P >= 1)
     {
-      apply_face_pair<3>(u2prev, zFprev, zF, uaprev0, uaprev1, dtdx, 2, stprev0, stprev1);
+      apply_face_pair<3, FAST>(u2prev, zFprev, zF, uaprev0, uaprev1, dtdx, 2, stprev0, stprev1);
       fix_volume_fractions<FAST>(u2prev);
       store_cons<3, B3>(U2, col_off((P - 1) >> 4) + i + 16 * j + 256 * ((P - 1) & 15), u2prev);
       double       pc;
@@ -905,9 +905,9 @@ step3d_body(StepArgs const & A, int item)
       if constexpr (MP) // the list holds a MILLER_PUCKETT solve; this block's solve decides
         if (S.source_term == G5_SOURCE_MILLER_PUCKETT)
           source_factors<3>(S, uraw_cur, stc0, stc1);
-      apply_face_pair<3>(u2prev, fl_load3(sFX, sFXE, fx3(i, j)), fl_load3(sFX, sFXE, fx3(i + 1, j)), uraw_cur.a0,
+      apply_face_pair<3, FAST>(u2prev, fl_load3(sFX, sFXE, fx3(i, j)), fl_load3(sFX, sFXE, fx3(i + 1, j)), uraw_cur.a0,
                          uraw_cur.a1, dtdx, 0, stc0, stc1);
-      apply_face_pair<3>(u2prev, fl_load3(sFY, sFYE, fy3(i, j)), fl_load3(sFY, sFYE, fy3(i, j + 1)), uraw_cur.a0,
+      apply_face_pair<3, FAST>(u2prev, fl_load3(sFY, sFYE, fy3(i, j)), fl_load3(sFY, sFYE, fy3(i, j + 1)), uraw_cur.a0,
                          uraw_cur.a1, dtdx, 1, stc0, stc1);
       uaprev0 = uraw_cur.a0;
       uaprev1 = uraw_cur.a1;
