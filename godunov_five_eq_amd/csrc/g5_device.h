@@ -387,14 +387,15 @@ med0(float a, float b)
 // sign(dc) min(dlim, |dc|).
 template <typename T>
 G5D T
-slope1_fast(T q, T qp, T qm, T slope_type)
+minmod_fast(T q, T qp, T qm)
 {
-  const T dl = q - qm, dr = qp - q;
-  if (slope_type == T(1.0))
-    return med0(dl, dr);
-  if (slope_type == T(2.0))
-    return med0(med0(T(2.0) * dl, T(2.0) * dr), T(0.5) * (qp - qm));
-  return T(0.0);
+  return med0(q - qm, qp - q);
+}
+template <typename T>
+G5D T
+mc_fast(T q, T qp, T qm)
+{
+  return med0(med0(T(2.0) * (q - qm), T(2.0) * (qp - q)), T(0.5) * (qp - qm));
 }
 
 template <typename T, bool FAST = false>
@@ -404,12 +405,27 @@ slopes_of(PrimT<T> const & c, PrimT<T> const & p, PrimT<T> const & m, T st)
   SlopeT<T> s;
   if constexpr (FAST)
   {
-    s.r0 = slope1_fast(c.r0, p.r0, m.r0, st);
-    s.r1 = slope1_fast(c.r1, p.r1, m.r1, st);
-    s.p = slope1_fast(c.p, p.p, m.p, st);
-    s.u = slope1_fast(c.u, p.u, m.u, st);
-    s.v = slope1_fast(c.v, p.v, m.v, st);
-    s.w = slope1_fast(c.w, p.w, m.w, st);
+    // one test of the limiter per cell and axis, not per variable
+    if (st == T(1.0))
+    {
+      s.r0 = minmod_fast(c.r0, p.r0, m.r0);
+      s.r1 = minmod_fast(c.r1, p.r1, m.r1);
+      s.p = minmod_fast(c.p, p.p, m.p);
+      s.u = minmod_fast(c.u, p.u, m.u);
+      s.v = minmod_fast(c.v, p.v, m.v);
+      s.w = minmod_fast(c.w, p.w, m.w);
+    }
+    else if (st == T(2.0))
+    {
+      s.r0 = mc_fast(c.r0, p.r0, m.r0);
+      s.r1 = mc_fast(c.r1, p.r1, m.r1);
+      s.p = mc_fast(c.p, p.p, m.p);
+      s.u = mc_fast(c.u, p.u, m.u);
+      s.v = mc_fast(c.v, p.v, m.v);
+      s.w = mc_fast(c.w, p.w, m.w);
+    }
+    else
+      s = SlopeT<T>{ T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
     return s;
   }
   s.r0 = slope1(c.r0, p.r0, m.r0, st);
@@ -1038,47 +1054,21 @@ hllc_fast(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int 
   const double ustar0 = (rcr * ur + rcl * ul + (pl - pr)) * inv_rc;
   const double ptotstar = (rcr * pl + rcl * pr + rcl * rcr * (ul - ur)) * inv_rc;
 
-  double ro, ro_0, ro_1, uo, ptoto, etoto;
-  if (SL > 0.0)
-  {
-    ro_0 = L.r0;
-    ro_1 = L.r1;
-    ro = rl;
-    uo = ul;
-    ptoto = pl;
-    etoto = etotl;
-  }
-  else if (ustar0 > 0.0)
-  {
-    const double inv = rcp64(SL - ustar0);
-    const double factor = (SL - ul) * inv;
-    ro_0 = L.r0 * factor;
-    ro_1 = L.r1 * factor;
-    ro = rl * factor;
-    uo = ustar0;
-    ptoto = ptotstar;
-    etoto = ((SL - ul) * etotl - pl * ul + ptotstar * ustar0) * inv;
-  }
-  else if (SR > 0.0)
-  {
-    const double inv = rcp64(SR - ustar0);
-    const double factor = (SR - ur) * inv;
-    ro_0 = R.r0 * factor;
-    ro_1 = R.r1 * factor;
-    ro = rr * factor;
-    uo = ustar0;
-    ptoto = ptotstar;
-    etoto = ((SR - ur) * etotr - pr * ur + ptotstar * ustar0) * inv;
-  }
-  else
-  {
-    ro_0 = R.r0;
-    ro_1 = R.r1;
-    ro = rr;
-    uo = ur;
-    ptoto = pr;
-    etoto = etotr;
-  }
+  // sampling at x/t = 0 without branches: the fan side K (left when SL > 0 or u* > 0), then its
+  // star state or, for a supersonic side, its own state -- the same operations on the selected
+  // operands as the four branches of hllc
+  const bool   left = SL > 0.0 or ustar0 > 0.0;
+  const bool   star = left ? !(SL > 0.0) : SR > 0.0;
+  const double SK = left ? SL : SR, uK = left ? ul : ur, rK = left ? rl : rr, pK = left ? pl : pr;
+  const double r0K = left ? L.r0 : R.r0, r1K = left ? L.r1 : R.r1, eK = left ? etotl : etotr;
+  const double inv = rcp64(SK - ustar0);
+  const double factor = (SK - uK) * inv;
+  const double ro_0 = star ? r0K * factor : r0K;
+  const double ro_1 = star ? r1K * factor : r1K;
+  const double ro = star ? rK * factor : rK;
+  const double uo = star ? ustar0 : uK;
+  const double ptoto = star ? ptotstar : pK;
+  const double etoto = star ? ((SK - uK) * eK - pK * uK + ptotstar * ustar0) * inv : eK;
 
   // the fluxes times dt/dx, with u* dt/dx formed once
   const double fd = (ro_0 + ro_1) * uo;
@@ -1132,17 +1122,15 @@ hllc_mixed(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, 
            0.5 * static_cast<double>(r) * k;
   };
 
-  const float  rl = fmaxf(L.r0 + L.r1, smallr);
-  const float  pl = fmaxf(L.p, rl * smallp);
-  const float  ul = ulN;
-  const float  cl = eos_sound_speed(S, rl, pl, L.a, 1.0f - L.a);
-  const double etotl = etot_of(rl, pl, L.a, ul, ulT1, ulT2);
+  const float rl = fmaxf(L.r0 + L.r1, smallr);
+  const float pl = fmaxf(L.p, rl * smallp);
+  const float ul = ulN;
+  const float cl = eos_sound_speed(S, rl, pl, L.a, 1.0f - L.a);
 
-  const float  rr = fmaxf(R.r0 + R.r1, smallr);
-  const float  pr = fmaxf(R.p, rr * smallp);
-  const float  ur = urN;
-  const float  cr = eos_sound_speed(S, rr, pr, R.a, 1.0f - R.a);
-  const double etotr = etot_of(rr, pr, R.a, ur, urT1, urT2);
+  const float rr = fmaxf(R.r0 + R.r1, smallr);
+  const float pr = fmaxf(R.p, rr * smallp);
+  const float ur = urN;
+  const float cr = eos_sound_speed(S, rr, pr, R.a, 1.0f - R.a);
 
   const float cfastl = fmaxf(cl, smallc);
   const float cfastr = fmaxf(cr, smallc);
@@ -1161,46 +1149,73 @@ hllc_mixed(SolveDesc const & S, FaceT<float> const & L, FaceT<float> const & R, 
 
   float  ro, ro_0, ro_1, uo, ptoto;
   double etoto;
-  if (SL > 0.0f)
+  if constexpr (DIM == 3)
   {
-    ro_0 = L.r0;
-    ro_1 = L.r1;
-    ro = rl;
-    uo = ul;
-    ptoto = pl;
-    etoto = etotl;
-  }
-  else if (ustar0 > 0.0f)
-  {
+    // sampling at x/t = 0 without branches: the fan side K (left when SL > 0 or u* > 0), then its
+    // star state or, for a supersonic side, its own state; the total energy of side K alone
     // etot* = f etot + (p* u* - p u)/(S - u*), f = (S - u)/(S - u*) (exactly 1 when u* = u)
-    const float inv = 1.0f / (SL - ustar0);
-    const float factor = ustar0 == ul ? 1.0f : (SL - ul) * inv;
-    ro_0 = L.r0 * factor;
-    ro_1 = L.r1 * factor;
-    ro = rl * factor;
-    uo = ustar0;
-    ptoto = ptotstar;
-    etoto = static_cast<double>(factor) * etotl + static_cast<double>((ptotstar * ustar0 - pl * ul) * inv);
-  }
-  else if (SR > 0.0f)
-  {
-    const float inv = 1.0f / (SR - ustar0);
-    const float factor = ustar0 == ur ? 1.0f : (SR - ur) * inv;
-    ro_0 = R.r0 * factor;
-    ro_1 = R.r1 * factor;
-    ro = rr * factor;
-    uo = ustar0;
-    ptoto = ptotstar;
-    etoto = static_cast<double>(factor) * etotr + static_cast<double>((ptotstar * ustar0 - pr * ur) * inv);
+    const bool   left = SL > 0.0f or ustar0 > 0.0f;
+    const bool   star = left ? !(SL > 0.0f) : SR > 0.0f;
+    const float  SK = left ? SL : SR, uK = left ? ul : ur, rK = left ? rl : rr, pK = left ? pl : pr;
+    const float  r0K = left ? L.r0 : R.r0, r1K = left ? L.r1 : R.r1, aK = left ? L.a : R.a;
+    const double eK = etot_of(rK, pK, aK, uK, left ? ulT1 : urT1, left ? ulT2 : urT2);
+    const float  inv = 1.0f / (SK - ustar0);
+    const float  factor = ustar0 == uK ? 1.0f : (SK - uK) * inv;
+    ro_0 = star ? r0K * factor : r0K;
+    ro_1 = star ? r1K * factor : r1K;
+    ro = star ? rK * factor : rK;
+    uo = star ? ustar0 : uK;
+    ptoto = star ? ptotstar : pK;
+    etoto =
+      star ? static_cast<double>(factor) * eK + static_cast<double>((ptotstar * ustar0 - pK * uK) * inv) : eK;
   }
   else
   {
-    ro_0 = R.r0;
-    ro_1 = R.r1;
-    ro = rr;
-    uo = ur;
-    ptoto = pr;
-    etoto = etotr;
+    // 2-D: the four branches (the selects of the branch-free form cost the 2-D step registers
+    // and a wavefront per SIMD)
+    const double etotl = etot_of(rl, pl, L.a, ul, ulT1, ulT2);
+    const double etotr = etot_of(rr, pr, R.a, ur, urT1, urT2);
+    if (SL > 0.0f)
+    {
+      ro_0 = L.r0;
+      ro_1 = L.r1;
+      ro = rl;
+      uo = ul;
+      ptoto = pl;
+      etoto = etotl;
+    }
+    else if (ustar0 > 0.0f)
+    {
+      // etot* = f etot + (p* u* - p u)/(S - u*), f = (S - u)/(S - u*) (exactly 1 when u* = u)
+      const float inv = 1.0f / (SL - ustar0);
+      const float factor = ustar0 == ul ? 1.0f : (SL - ul) * inv;
+      ro_0 = L.r0 * factor;
+      ro_1 = L.r1 * factor;
+      ro = rl * factor;
+      uo = ustar0;
+      ptoto = ptotstar;
+      etoto = static_cast<double>(factor) * etotl + static_cast<double>((ptotstar * ustar0 - pl * ul) * inv);
+    }
+    else if (SR > 0.0f)
+    {
+      const float inv = 1.0f / (SR - ustar0);
+      const float factor = ustar0 == ur ? 1.0f : (SR - ur) * inv;
+      ro_0 = R.r0 * factor;
+      ro_1 = R.r1 * factor;
+      ro = rr * factor;
+      uo = ustar0;
+      ptoto = ptotstar;
+      etoto = static_cast<double>(factor) * etotr + static_cast<double>((ptotstar * ustar0 - pr * ur) * inv);
+    }
+    else
+    {
+      ro_0 = R.r0;
+      ro_1 = R.r1;
+      ro = rr;
+      uo = ur;
+      ptoto = pr;
+      etoto = etotr;
+    }
   }
 
   const bool   up = (ro_0 + ro_1) * uo > 0.0f;
