@@ -348,23 +348,19 @@ to_prim(SolveDesc const & S, Cons const & U, PrimT<float> & q)
   return cons_to_prim_mixed<DIM>(S, U, q);
 }
 
-// scheme/ComputeLimitedSlopesFunctor.cpp:114-137 slope_unsplit_scalar
+// scheme/ComputeLimitedSlopesFunctor.cpp:114-137 slope_unsplit_scalar, for slope_type 1 or 2 (the
+// caller tests it; other values give 0)
 template <typename T>
 G5D T
-slope1(T q, T qp, T qm, T slope_type)
+slope1_limited(T q, T qp, T qm, T slope_type)
 {
-  T dq = T(0.0);
-  if (slope_type == T(1.0) or slope_type == T(2.0))
-  {
-    const T dlft = slope_type * (q - qm);
-    const T drgt = slope_type * (qp - q);
-    const T dcen = T(0.5) * (qp - qm);
-    const T dsgn = (dcen >= T(0.0)) ? T(1.0) : T(-1.0);
-    const T slop = fmin(fabs(dlft), fabs(drgt));
-    const T dlim = (dlft * drgt) <= T(0.0) ? T(0.0) : slop;
-    dq = dsgn * fmin(dlim, fabs(dcen));
-  }
-  return dq;
+  const T dlft = slope_type * (q - qm);
+  const T drgt = slope_type * (qp - q);
+  const T dcen = T(0.5) * (qp - qm);
+  const T dsgn = (dcen >= T(0.0)) ? T(1.0) : T(-1.0);
+  const T slop = fmin(fabs(dlft), fabs(drgt));
+  const T dlim = (dlft * drgt) <= T(0.0) ? T(0.0) : slop;
+  return dsgn * fmin(dlim, fabs(dcen));
 }
 
 // the median of a, b and 0: minmod(a, b) -- 0 when the signs differ, else the one nearer 0
@@ -428,12 +424,19 @@ slopes_of(PrimT<T> const & c, PrimT<T> const & p, PrimT<T> const & m, T st)
       s = SlopeT<T>{ T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
     return s;
   }
-  s.r0 = slope1(c.r0, p.r0, m.r0, st);
-  s.r1 = slope1(c.r1, p.r1, m.r1, st);
-  s.p = slope1(c.p, p.p, m.p, st);
-  s.u = slope1(c.u, p.u, m.u, st);
-  s.v = slope1(c.v, p.v, m.v, st);
-  s.w = slope1(c.w, p.w, m.w, st);
+  // the reference's slope of each variable, with its test of the limiter made once for the six
+  // (the same arithmetic per variable)
+  if (st == T(1.0) or st == T(2.0))
+  {
+    s.r0 = slope1_limited(c.r0, p.r0, m.r0, st);
+    s.r1 = slope1_limited(c.r1, p.r1, m.r1, st);
+    s.p = slope1_limited(c.p, p.p, m.p, st);
+    s.u = slope1_limited(c.u, p.u, m.u, st);
+    s.v = slope1_limited(c.v, p.v, m.v, st);
+    s.w = slope1_limited(c.w, p.w, m.w, st);
+  }
+  else
+    s = SlopeT<T>{ T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
   return s;
 }
 
@@ -1449,6 +1452,13 @@ cell_inv_dt(SolveDesc const & S, Cons const & u, double & p)
   const double vx = c + fabs(q.u);
   const double vy = c + fabs(q.v);
   const double vz = DIM == 3 ? c + fabs(q.w) : 0.0;
+  if constexpr (FAST)
+  {
+    // products by 1/dx (exact for dx = 2^k; otherwise within an ulp of the quotients), with no
+    // branch in the step's loop: the reciprocal does not depend on the cell
+    const double r = S.inv_dx_p2 > 0.0 ? S.inv_dx_p2 : 1.0 / S.dx;
+    return DIM == 2 ? vx * r + vy * r : vx * r + vy * r + vz * r;
+  }
   if (S.inv_dx_p2 > 0.0) // dx = 2^k: the quotients by dx are the exact products by 2^-k
   {
     const double r = S.inv_dx_p2;

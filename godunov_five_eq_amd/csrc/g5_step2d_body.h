@@ -399,12 +399,12 @@ prolong_cell(SolveDesc const & S, int32_t const * __restrict__ slots, double con
   const Cons   xl = coarse_cell(S, slots, U, cx - 1, cy), xr = coarse_cell(S, slots, U, cx + 1, cy);
   const Cons   yl = coarse_cell(S, slots, U, cx, cy - 1), yr = coarse_cell(S, slots, U, cx, cy + 1);
   Cons         u;
-  u.ar0 = c.ar0 + hx * slope1<double>(c.ar0, xr.ar0, xl.ar0, 1.0) + hy * slope1<double>(c.ar0, yr.ar0, yl.ar0, 1.0);
-  u.ar1 = c.ar1 + hx * slope1<double>(c.ar1, xr.ar1, xl.ar1, 1.0) + hy * slope1<double>(c.ar1, yr.ar1, yl.ar1, 1.0);
-  u.a0 = c.a0 + hx * slope1<double>(c.a0, xr.a0, xl.a0, 1.0) + hy * slope1<double>(c.a0, yr.a0, yl.a0, 1.0);
-  u.e = c.e + hx * slope1<double>(c.e, xr.e, xl.e, 1.0) + hy * slope1<double>(c.e, yr.e, yl.e, 1.0);
-  u.mu = c.mu + hx * slope1<double>(c.mu, xr.mu, xl.mu, 1.0) + hy * slope1<double>(c.mu, yr.mu, yl.mu, 1.0);
-  u.mv = c.mv + hx * slope1<double>(c.mv, xr.mv, xl.mv, 1.0) + hy * slope1<double>(c.mv, yr.mv, yl.mv, 1.0);
+  u.ar0 = c.ar0 + hx * slope1_limited<double>(c.ar0, xr.ar0, xl.ar0, 1.0) + hy * slope1_limited<double>(c.ar0, yr.ar0, yl.ar0, 1.0);
+  u.ar1 = c.ar1 + hx * slope1_limited<double>(c.ar1, xr.ar1, xl.ar1, 1.0) + hy * slope1_limited<double>(c.ar1, yr.ar1, yl.ar1, 1.0);
+  u.a0 = c.a0 + hx * slope1_limited<double>(c.a0, xr.a0, xl.a0, 1.0) + hy * slope1_limited<double>(c.a0, yr.a0, yl.a0, 1.0);
+  u.e = c.e + hx * slope1_limited<double>(c.e, xr.e, xl.e, 1.0) + hy * slope1_limited<double>(c.e, yr.e, yl.e, 1.0);
+  u.mu = c.mu + hx * slope1_limited<double>(c.mu, xr.mu, xl.mu, 1.0) + hy * slope1_limited<double>(c.mu, yr.mu, yl.mu, 1.0);
+  u.mv = c.mv + hx * slope1_limited<double>(c.mv, xr.mv, xl.mv, 1.0) + hy * slope1_limited<double>(c.mv, yr.mv, yl.mv, 1.0);
   u.mw = 0.0;
   u.d = u.ar0 + u.ar1;
   u.a1 = 1.0 - u.a0;

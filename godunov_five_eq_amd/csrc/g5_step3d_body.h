@@ -465,8 +465,8 @@ prolong3(SolveDesc const & S, int32_t const * __restrict__ slots, double const *
   const Cons   yl = coarse3(S, slots, U, cx, cy - 1, cz), yr = coarse3(S, slots, U, cx, cy + 1, cz);
   const Cons   zl = coarse3(S, slots, U, cx, cy, cz - 1), zr = coarse3(S, slots, U, cx, cy, cz + 1);
   auto         ex = [&](double C, double XL, double XR, double YL, double YR, double ZL, double ZR) {
-    return C + hx * slope1<double>(C, XR, XL, 1.0) + hy * slope1<double>(C, YR, YL, 1.0) +
-           hz * slope1<double>(C, ZR, ZL, 1.0);
+    return C + hx * slope1_limited<double>(C, XR, XL, 1.0) + hy * slope1_limited<double>(C, YR, YL, 1.0) +
+           hz * slope1_limited<double>(C, ZR, ZL, 1.0);
   };
   u.ar0 = ex(c.ar0, xl.ar0, xr.ar0, yl.ar0, yr.ar0, zl.ar0, zr.ar0);
   u.ar1 = ex(c.ar1, xl.ar1, xr.ar1, yl.ar1, yr.ar1, zl.ar1, zr.ar1);
