@@ -394,50 +394,55 @@ mc_fast(T q, T qp, T qm)
   return med0(med0(T(2.0) * (q - qm), T(2.0) * (qp - q)), T(0.5) * (qp - qm));
 }
 
-template <typename T, bool FAST = false>
+// the limited slopes of the six reconstructed variables along one axis, for a known limiter
+template <typename T>
 G5D SlopeT<T>
-slopes_of(PrimT<T> const & c, PrimT<T> const & p, PrimT<T> const & m, T st)
+slopes_minmod_fast(PrimT<T> const & c, PrimT<T> const & p, PrimT<T> const & m)
 {
-  SlopeT<T> s;
-  if constexpr (FAST)
+  return SlopeT<T>{ minmod_fast(c.r0, p.r0, m.r0), minmod_fast(c.r1, p.r1, m.r1), minmod_fast(c.p, p.p, m.p),
+                    minmod_fast(c.u, p.u, m.u),    minmod_fast(c.v, p.v, m.v),    minmod_fast(c.w, p.w, m.w) };
+}
+template <typename T>
+G5D SlopeT<T>
+slopes_mc_fast(PrimT<T> const & c, PrimT<T> const & p, PrimT<T> const & m)
+{
+  return SlopeT<T>{ mc_fast(c.r0, p.r0, m.r0), mc_fast(c.r1, p.r1, m.r1), mc_fast(c.p, p.p, m.p),
+                    mc_fast(c.u, p.u, m.u),    mc_fast(c.v, p.v, m.v),    mc_fast(c.w, p.w, m.w) };
+}
+template <typename T>
+G5D SlopeT<T>
+slopes_limited(PrimT<T> const & c, PrimT<T> const & p, PrimT<T> const & m, T st)
+{
+  return SlopeT<T>{ slope1_limited(c.r0, p.r0, m.r0, st), slope1_limited(c.r1, p.r1, m.r1, st),
+                    slope1_limited(c.p, p.p, m.p, st),    slope1_limited(c.u, p.u, m.u, st),
+                    slope1_limited(c.v, p.v, m.v, st),    slope1_limited(c.w, p.w, m.w, st) };
+}
+
+// the limited slopes of a cell along NAX axes (neighbours p[a] and m[a] on axis a), with one test
+// of the limiter for all of them: the reference's slope_unsplit_scalar per variable (slope_type 1
+// or 2, otherwise 0), or with FAST the median forms of minmod_fast / mc_fast
+template <typename T, bool FAST, int NAX>
+G5D void
+slopes_axes(PrimT<T> const & c, PrimT<T> const (&p)[NAX], PrimT<T> const (&m)[NAX], T st, SlopeT<T> (&s)[NAX])
+{
+  if (st == T(1.0))
   {
-    // one test of the limiter per cell and axis, not per variable
-    if (st == T(1.0))
-    {
-      s.r0 = minmod_fast(c.r0, p.r0, m.r0);
-      s.r1 = minmod_fast(c.r1, p.r1, m.r1);
-      s.p = minmod_fast(c.p, p.p, m.p);
-      s.u = minmod_fast(c.u, p.u, m.u);
-      s.v = minmod_fast(c.v, p.v, m.v);
-      s.w = minmod_fast(c.w, p.w, m.w);
-    }
-    else if (st == T(2.0))
-    {
-      s.r0 = mc_fast(c.r0, p.r0, m.r0);
-      s.r1 = mc_fast(c.r1, p.r1, m.r1);
-      s.p = mc_fast(c.p, p.p, m.p);
-      s.u = mc_fast(c.u, p.u, m.u);
-      s.v = mc_fast(c.v, p.v, m.v);
-      s.w = mc_fast(c.w, p.w, m.w);
-    }
-    else
-      s = SlopeT<T>{ T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
-    return s;
+#pragma unroll
+    for (int a = 0; a < NAX; ++a)
+      s[a] = FAST ? slopes_minmod_fast(c, p[a], m[a]) : slopes_limited(c, p[a], m[a], T(1.0));
   }
-  // the reference's slope of each variable, with its test of the limiter made once for the six
-  // (the same arithmetic per variable)
-  if (st == T(1.0) or st == T(2.0))
+  else if (st == T(2.0))
   {
-    s.r0 = slope1_limited(c.r0, p.r0, m.r0, st);
-    s.r1 = slope1_limited(c.r1, p.r1, m.r1, st);
-    s.p = slope1_limited(c.p, p.p, m.p, st);
-    s.u = slope1_limited(c.u, p.u, m.u, st);
-    s.v = slope1_limited(c.v, p.v, m.v, st);
-    s.w = slope1_limited(c.w, p.w, m.w, st);
+#pragma unroll
+    for (int a = 0; a < NAX; ++a)
+      s[a] = FAST ? slopes_mc_fast(c, p[a], m[a]) : slopes_limited(c, p[a], m[a], T(2.0));
   }
   else
-    s = SlopeT<T>{ T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
-  return s;
+  {
+#pragma unroll
+    for (int a = 0; a < NAX; ++a)
+      s[a] = SlopeT<T>{ T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
+  }
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -201,8 +201,14 @@ predict_faces2(SolveDesc const & S, T const * sQ, int gi, int gj, T dtdx, bool a
   const PrimT<T>  qym = sq_load2(sQ, tidx2(gi, gj - 1));
   const PrimT<T>  qyp = sq_load2(sQ, tidx2(gi, gj + 1));
   const T         st = static_cast<T>(S.slope_type);
-  const SlopeT<T> sx = slopes_of<T, FAST>(qc, qxp, qxm, st);
-  const SlopeT<T> sy = slopes_of<T, FAST>(qc, qyp, qym, st);
+  // the limiter tested per axis here: tested once for both axes, the mixed 2-D step needs 83
+  // registers instead of 79, and one wavefront per SIMD less
+  const PrimT<T>  qpx[1] = { qxp }, qmx[1] = { qxm }, qpy[1] = { qyp }, qmy[1] = { qym };
+  SlopeT<T>       slx[1], sly[1];
+  slopes_axes<T, FAST, 1>(qc, qpx, qmx, st, slx);
+  slopes_axes<T, FAST, 1>(qc, qpy, qmy, st, sly);
+  const SlopeT<T> sx = slx[0];
+  const SlopeT<T> sy = sly[0];
   const SlopeT<T> sz = { T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
   HancockT<T>     h = { T(0.0), T(0.0), T(0.0), T(0.0), T(0.0), T(0.0) };
   if (HANCOCK)

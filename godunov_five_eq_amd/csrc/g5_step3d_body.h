@@ -270,9 +270,14 @@ struct Pred3
     const PrimT<T> qzm = q3_load(sQ, aslot(P - 1), c);
     const PrimT<T> qzp = q3_load(sQ, aslot(P + 1), c);
     const T        st = static_cast<T>(S.slope_type);
-    sx = slopes_of<T, FAST>(q, qxp, qxm, st);
-    sy = slopes_of<T, FAST>(q, qyp, qym, st);
-    sz = slopes_of<T, FAST>(q, qzp, qzm, st);
+    {
+      const PrimT<T> qp[3] = { qxp, qyp, qzp }, qm[3] = { qxm, qym, qzm };
+      SlopeT<T>      sl[3];
+      slopes_axes<T, FAST, 3>(q, qp, qm, st, sl);
+      sx = sl[0];
+      sy = sl[1];
+      sz = sl[2];
+    }
     axm = qxm.a;
     axp = qxp.a;
     aym = qym.a;
