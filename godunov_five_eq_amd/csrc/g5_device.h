@@ -106,19 +106,20 @@ struct Par<float>
 
 // ---------------------------------------------------------------------------------------------
 // FP64 arithmetic of the reassociated path (G5_PREC_FP64_FAST).  The hardware reciprocal and
-// reciprocal square root estimates (v_rcp_f64, v_rsq_f64) are refined by Newton-Raphson /
-// Goldschmidt steps, each of which squares the relative error, to within about one ulp -- instead
-// of the IEEE division and square root sequences (scale, estimate, refinement, fused correction
-// and fixup), whose dependent chains are about twice as long.  Arguments are finite and > 0.
+// reciprocal square root estimates (v_rcp_f64, v_rsq_f64; relative error below 2^-24) are refined
+// by a third-order step (the reciprocal) or Goldschmidt / Newton steps (the roots) to within about
+// one ulp -- instead of the IEEE division and square root sequences (scale, estimate, refinement,
+// fused correction and fixup), whose dependent chains are about twice as long.  Arguments are
+// finite and > 0.
 // ---------------------------------------------------------------------------------------------
 G5D double
 rcp64(double b)
 {
-  double r = __builtin_amdgcn_rcp(b);
-  double e = fma(-b, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-b, r, 1.0);
-  return fma(r, e, r);
+  // one third-order step r (1 + e + e^2), e = 1 - b r: the estimate's relative error (below
+  // 2^-24) cubed, then the rounding of the last fused operations
+  const double r = __builtin_amdgcn_rcp(b);
+  const double e = fma(-b, r, 1.0);
+  return fma(r, fma(e, e, e), r);
 }
 
 // sqrt(x) for x >= 0: g = x y, h = y/2 from y ~ 1/sqrt(x), one Goldschmidt step, then one
