@@ -940,48 +940,23 @@ hllc(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int AX)
   const double ustar0 = (rcr * ur + rcl * ul + (ptotl - ptotr)) / (rcr + rcl);
   const double ptotstar = (rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur)) / (rcr + rcl);
 
-  const double factorL = (SL - ul) / (SL - ustar0);
-  const double etotstarl = ((SL - ul) * etotl - ptotl * ul + ptotstar * ustar0) / (SL - ustar0);
-  const double factorR = (SR - ur) / (SR - ustar0);
-  const double etotstarr = ((SR - ur) * etotr - ptotr * ur + ptotstar * ustar0) / (SR - ustar0);
-
-  double ro, ro_0, ro_1, uo, ptoto, etoto;
-  if (SL > 0.0)
-  {
-    ro_0 = L.r0;
-    ro_1 = L.r1;
-    ro = rl;
-    uo = ul;
-    ptoto = ptotl;
-    etoto = etotl;
-  }
-  else if (ustar0 > 0.0)
-  {
-    ro_0 = L.r0 * factorL;
-    ro_1 = L.r1 * factorL;
-    ro = rl * factorL;
-    uo = ustar0;
-    ptoto = ptotstar;
-    etoto = etotstarl;
-  }
-  else if (SR > 0.0)
-  {
-    ro_0 = R.r0 * factorR;
-    ro_1 = R.r1 * factorR;
-    ro = rr * factorR;
-    uo = ustar0;
-    ptoto = ptotstar;
-    etoto = etotstarr;
-  }
-  else
-  {
-    ro_0 = R.r0;
-    ro_1 = R.r1;
-    ro = rr;
-    uo = ur;
-    ptoto = ptotr;
-    etoto = etotr;
-  }
+  // Sampling at x/t = 0.  The reference computes the star factor and energy of both sides and then
+  // picks one of four states; here the side K of the fan (left when SL > 0 or u* > 0) is selected
+  // first and only its star quantities are computed -- the same operations on the same operands,
+  // so the same bits -- and the star state or, for a supersonic side, the side's own state taken.
+  const bool   left = SL > 0.0 or ustar0 > 0.0;
+  const bool   star = left ? !(SL > 0.0) : SR > 0.0;
+  const double SK = left ? SL : SR, uK = left ? ul : ur, rK = left ? rl : rr;
+  const double pK = left ? ptotl : ptotr, eK = left ? etotl : etotr;
+  const double r0K = left ? L.r0 : R.r0, r1K = left ? L.r1 : R.r1;
+  const double factorK = (SK - uK) / (SK - ustar0);
+  const double etotstarK = ((SK - uK) * eK - pK * uK + ptotstar * ustar0) / (SK - ustar0);
+  const double ro_0 = star ? r0K * factorK : r0K;
+  const double ro_1 = star ? r1K * factorK : r1K;
+  const double ro = star ? rK * factorK : rK;
+  const double uo = star ? ustar0 : uK;
+  const double ptoto = star ? ptotstar : pK;
+  const double etoto = star ? etotstarK : eK;
 
   const double fr0 = ro_0 * uo;
   const double fr1 = ro_1 * uo;
