@@ -1038,21 +1038,70 @@ hllc_fast(SolveDesc const & S, Face const & L, Face const & R, double dtdx, int 
   const double ustar0 = (rcr * ur + rcl * ul + (pl - pr)) * inv_rc;
   const double ptotstar = (rcr * pl + rcl * pr + rcl * rcr * (ul - ur)) * inv_rc;
 
-  // sampling at x/t = 0 without branches: the fan side K (left when SL > 0 or u* > 0), then its
-  // star state or, for a supersonic side, its own state -- the same operations on the selected
-  // operands as the four branches of hllc
-  const bool   left = SL > 0.0 or ustar0 > 0.0;
-  const bool   star = left ? !(SL > 0.0) : SR > 0.0;
-  const double SK = left ? SL : SR, uK = left ? ul : ur, rK = left ? rl : rr, pK = left ? pl : pr;
-  const double r0K = left ? L.r0 : R.r0, r1K = left ? L.r1 : R.r1, eK = left ? etotl : etotr;
-  const double inv = rcp64(SK - ustar0);
-  const double factor = (SK - uK) * inv;
-  const double ro_0 = star ? r0K * factor : r0K;
-  const double ro_1 = star ? r1K * factor : r1K;
-  const double ro = star ? rK * factor : rK;
-  const double uo = star ? ustar0 : uK;
-  const double ptoto = star ? ptotstar : pK;
-  const double etoto = star ? ((SK - uK) * eK - pK * uK + ptotstar * ustar0) * inv : eK;
+  double ro, ro_0, ro_1, uo, ptoto, etoto;
+  if constexpr (DIM == 3)
+  {
+    // sampling at x/t = 0 without branches: the fan side K (left when SL > 0 or u* > 0), then its
+    // star state or, for a supersonic side, its own state -- the same operations on the selected
+    // operands as the four branches of hllc
+    const bool   left = SL > 0.0 or ustar0 > 0.0;
+    const bool   star = left ? !(SL > 0.0) : SR > 0.0;
+    const double SK = left ? SL : SR, uK = left ? ul : ur, rK = left ? rl : rr, pK = left ? pl : pr;
+    const double r0K = left ? L.r0 : R.r0, r1K = left ? L.r1 : R.r1, eK = left ? etotl : etotr;
+    const double inv = rcp64(SK - ustar0);
+    const double factor = (SK - uK) * inv;
+    ro_0 = star ? r0K * factor : r0K;
+    ro_1 = star ? r1K * factor : r1K;
+    ro = star ? rK * factor : rK;
+    uo = star ? ustar0 : uK;
+    ptoto = star ? ptotstar : pK;
+    etoto = star ? ((SK - uK) * eK - pK * uK + ptotstar * ustar0) * inv : eK;
+  }
+  else
+  {
+    // 2-D: the four branches (the selects of the branch-free form raise the 2-D step from 130 to
+    // 185 registers: two wavefronts per SIMD instead of three)
+    if (SL > 0.0)
+    {
+      ro_0 = L.r0;
+      ro_1 = L.r1;
+      ro = rl;
+      uo = ul;
+      ptoto = pl;
+      etoto = etotl;
+    }
+    else if (ustar0 > 0.0)
+    {
+      const double inv = rcp64(SL - ustar0);
+      const double factor = (SL - ul) * inv;
+      ro_0 = L.r0 * factor;
+      ro_1 = L.r1 * factor;
+      ro = rl * factor;
+      uo = ustar0;
+      ptoto = ptotstar;
+      etoto = ((SL - ul) * etotl - pl * ul + ptotstar * ustar0) * inv;
+    }
+    else if (SR > 0.0)
+    {
+      const double inv = rcp64(SR - ustar0);
+      const double factor = (SR - ur) * inv;
+      ro_0 = R.r0 * factor;
+      ro_1 = R.r1 * factor;
+      ro = rr * factor;
+      uo = ustar0;
+      ptoto = ptotstar;
+      etoto = ((SR - ur) * etotr - pr * ur + ptotstar * ustar0) * inv;
+    }
+    else
+    {
+      ro_0 = R.r0;
+      ro_1 = R.r1;
+      ro = rr;
+      uo = ur;
+      ptoto = pr;
+      etoto = etotr;
+    }
+  }
 
   // the fluxes times dt/dx, with u* dt/dx formed once
   const double fd = (ro_0 + ro_1) * uo;
